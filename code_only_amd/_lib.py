@@ -1,0 +1,102 @@
+"""ctypes binding of libsg.so (the C ABI in include/sg.h).
+
+The library is the product path: there is no CPU fallback.  If libsg.so is missing this module
+raises on first use instead of degrading silently.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsg.so")
+
+SG_OK = 0
+SG_ERR_ARG = 1
+SG_ERR_HIP = 2
+SG_ERR_NOT_CONVERGED = 3
+SG_ERR_NONFINITE = 4
+
+SG_STOP_REFERENCE = 0
+SG_STOP_TRUE_RESIDUAL = 1
+
+SG_PREC_FP64 = 1
+SG_PREC_MIXED = 2
+
+SG_HIST = 64
+
+
+class SgParams(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("maxit", ctypes.c_int), ("stop_mode", ctypes.c_int),
+                ("precision", ctypes.c_int), ("ncycle", ctypes.c_int), ("npre", ctypes.c_int),
+                ("npost", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+class SgStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("rel_linf", ctypes.c_double),
+                ("rel_l2", ctypes.c_double), ("resmaxg", ctypes.c_double), ("ms", ctypes.c_double),
+                ("hist", ctypes.c_double * SG_HIST)]
+
+    def as_dict(self):
+        return dict(iterations=self.iterations, status=self.status, rel_linf=self.rel_linf,
+                    rel_l2=self.rel_l2, resmaxg=self.resmaxg, ms=self.ms,
+                    hist=[self.hist[i] for i in range(min(self.iterations + 1, SG_HIST))])
+
+
+class SgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libsg error {code}: {msg}")
+        self.code = code
+
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every symbol include/sg.h declares.
+SIGNATURES = {
+    "sg_last_error": (ctypes.c_char_p, []),
+    "sg_version": (_I, []),
+    "sg_default_params": (None, [ctypes.POINTER(SgParams)]),
+    "sg_uniform_create": (_I, [ctypes.POINTER(_VP), _I, _D, _I]),
+    "sg_uniform_destroy": (_I, [_VP]),
+    "sg_uniform_cycle_dev": (_I, [_VP, _VP, _VP, ctypes.POINTER(SgParams), ctypes.POINTER(SgStats)]),
+    "sg_uniform_cycle_host": (_I, [_VP, _DP, _DP, ctypes.POINTER(SgParams), ctypes.POINTER(SgStats)]),
+    "sg_uniform_gradient_dev": (_I, [_VP, _VP, _VP]),
+    "sg_uniform_relax_dev": (_I, [_VP, _I, _VP, _VP, _I, _D, _DP]),
+    "sg_uniform_resid_dev": (_I, [_VP, _I, _VP, _VP, _VP, _I, _D]),
+    "sg_uniform_restrict_dev": (_I, [_VP, _I, _VP, _VP, _I]),
+    "sg_uniform_interp_dev": (_I, [_VP, _I, _VP, _VP, _I, _I]),
+    "sg_uniform_time_smoother": (_I, [_VP, _I, _I, _DP]),
+}
+
+_LIB = None
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m code_only_amd.build` (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, allow=()):
+    if rc != SG_OK and rc not in allow:
+        msg = load().sg_last_error()
+        raise SgError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def default_params(**kw):
+    p = SgParams()
+    load().sg_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p

@@ -1,0 +1,566 @@
+// Uniform periodic-grid multigrid on one MI355X: host orchestration of mg_cycle / mg_lin
+// (mg.F90:42-218) over device-resident level hierarchies, and the uniform-grid part of the C ABI
+// declared in sg.h.
+//
+// Two inner solvers share the same kernels:
+//  * SG_PREC_FP64 ("plain"): every level array FP64, one launch per reference loop nest,
+//    launched eagerly.  Bitwise equal to the CPU FP64 restatement (same operand order, no FMA
+//    contraction).
+//  * SG_PREC_MIXED: the inner mg_lin runs on FP32 level arrays; the outer residual
+//    r = f - L phi and the update phi += e stay FP64.  One outer iteration (inner solve, update,
+//    next outer residual and its norms) is captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "sg_common.h"
+#include "uniform_kernels.h"
+
+namespace sg {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *last_error() { return g_err; }
+
+constexpr int kMaxLev = 16;
+constexpr int kBlock = 256;
+constexpr int kMaxN = 1024;
+
+template <typename T> struct Levels {
+    T *U[kMaxLev] = {};
+    T *RES[kMaxLev] = {};
+    T *RHO[kMaxLev] = {};  // RHO[l] also serves as RHS[l]: see mg_lin below
+};
+
+static inline unsigned nblocks(long work) { return (unsigned)((work + kBlock - 1) / kBlock); }
+static long n3(int n) { return (long)n * n * n; }
+
+} // namespace sg
+
+using namespace sg;
+
+struct sg_uniform {
+    int n = 0, nlev = 0, precisions = 0;
+    double h = 0.0;
+    int nl[kMaxLev] = {};
+    double hl[kMaxLev] = {};
+    Levels<double> d;
+    Levels<float> f;
+    double *psi_dev = nullptr, *src_dev = nullptr;   // staging for the host-pointer entry point
+    double *pmax = nullptr, *psum = nullptr;          // norm partials
+    int npart = 0;
+    double *dnorm = nullptr;   // [0] max|r| [1] sum r^2 [2] resmaxg*h^2 [3] max|f| [4] sum f^2
+    double *hnorm = nullptr;   // pinned host mirror of dnorm
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // graph of one mixed-precision outer iteration, keyed on the arguments it was captured with
+    hipGraphExec_t gexec = nullptr;
+    double *g_psi = nullptr;
+    const double *g_src = nullptr;
+    int g_ncycle = -1, g_npre = -1, g_npost = -1;
+};
+
+namespace sg {
+
+template <typename T> static Levels<T> &levels(sg_uniform *c);
+template <> Levels<double> &levels<double>(sg_uniform *c) { return c->d; }
+template <> Levels<float> &levels<float>(sg_uniform *c) { return c->f; }
+
+template <typename T> static int launch_fill(hipStream_t s, T *p, long len)
+{
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks(len)), dim3(kBlock), 0, s, p, len, T(0));
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+template <typename T>
+static int launch_relax(hipStream_t s, T *u, const T *rhs, int n, double h, double *resmax)
+{
+    const long work = (long)((n + 1) / 2) * n * n;
+    const T h2 = (T)(h * h);
+    for (int ipass = 1; ipass <= 2; ++ipass) {
+        hipLaunchKernelGGL(k_relax_color<T>, dim3(nblocks(work)), dim3(kBlock), 0, s, u, rhs, n, h2, ipass,
+                           resmax);
+        SG_CHECK_LAUNCH();
+    }
+    return SG_OK;
+}
+
+// norms != nullptr: also reduce max|res| and sum res^2 into norms[0..1] (deterministic order).
+template <typename Tin, typename Tout>
+static int launch_resid(sg_uniform *c, Tout *res, const Tin *u, const Tin *rhs, int n, double h, double *norms)
+{
+    const long work = n3(n);
+    const Tin h2i = (Tin)(1.0 / (h * h));
+    const unsigned nb = nblocks(work);
+    if (norms && (int)nb > c->npart) {
+        set_error("norm partial buffer too small (%u > %d)", nb, c->npart);
+        return SG_ERR_ARG;
+    }
+    hipLaunchKernelGGL((k_resid<Tin, Tout>), dim3(nb), dim3(kBlock), 0, c->stream, res, u, rhs, n, h2i,
+                       norms ? c->pmax : nullptr, norms ? c->psum : nullptr);
+    SG_CHECK_LAUNCH();
+    if (norms) {
+        hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, (int)nb, norms);
+        SG_CHECK_LAUNCH();
+    }
+    return SG_OK;
+}
+
+// max|a| and sum a^2 of a dense FP64 array (the source norms of the relative residual).
+__global__ void __launch_bounds__(256) k_norm_partial(const double *__restrict__ a, long len, double *pmax,
+                                                      double *psum)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double mx = 0.0, sm = 0.0;
+    if (c < len) { mx = fabs(a[c]); sm = a[c] * a[c]; }
+    block_max_sum(mx, sm);
+    if (threadIdx.x == 0) { pmax[blockIdx.x] = mx; psum[blockIdx.x] = sm; }
+}
+
+static int launch_norm(sg_uniform *c, const double *a, long len, double *out)
+{
+    const unsigned nb = nblocks(len);
+    SG_REQUIRE((int)nb <= c->npart, "norm partial buffer too small");
+    hipLaunchKernelGGL(k_norm_partial, dim3(nb), dim3(kBlock), 0, c->stream, a, len, c->pmax, c->psum);
+    SG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, (int)nb, out);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+template <typename T> static int launch_restrict(hipStream_t s, T *uc, const T *uf, int nc)
+{
+    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(n3(nc))), dim3(kBlock), 0, s, uc, uf, nc);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+// mg_interp (mg.F90:608-633) on a periodic grid: the ghost fills become index wrap, so only the
+// prolongation itself remains.  Tricubic needs a coarse array of >= 4 cells with ghosts
+// (interpolation.F90:87-96); narrower ones fall back to trilinear.
+template <typename T> static int launch_interp(hipStream_t s, T *uf, const T *uc, int nf, bool cubic, bool add)
+{
+    const int nc = nf / 2;
+    const unsigned nb = nblocks(n3(nf));
+    if (cubic && nc + 2 >= 4 && !add) {
+        hipLaunchKernelGGL(k_interp_cubic<T>, dim3(nb), dim3(kBlock), 0, s, uf, uc, nf);
+    } else if (add) {
+        hipLaunchKernelGGL((k_interp_linear<T, true>), dim3(nb), dim3(kBlock), 0, s, uf, uc, nf);
+    } else {
+        hipLaunchKernelGGL((k_interp_linear<T, false>), dim3(nb), dim3(kBlock), 0, s, uf, uc, nf);
+    }
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+// mg.F90:106-218 with Ncycle V(Npre,Npost) cycles per FMG level.  The reference copies RHO into a
+// separate RHS array at each FMG level (mg.F90:176) and later overwrites RHS of coarser levels
+// with restricted residuals (mg.F90:188).  RHO of a level is never read again after that copy
+// within one mg_lin call (the next call restricts RHO afresh from level 0), so RHS and RHO share
+// storage here; every kernel reads the same values as in the reference.
+// resmax: receives max|resh2| of the last smoothing call (the one mg_cycle's test reads).
+template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax)
+{
+    Levels<T> &A = levels<T>(c);
+    hipStream_t s = c->stream;
+    const int lmax = c->nlev - 1;
+    for (int l = 1; l <= lmax; ++l) SG_TRY(launch_restrict<T>(s, A.RHO[l], A.RHO[l - 1], c->nl[l]));
+    SG_TRY(launch_fill<T>(s, A.U[lmax], n3(c->nl[lmax])));
+    for (int it = 0; it < 2; ++it) SG_TRY(launch_relax<T>(s, A.U[lmax], A.RHO[lmax], c->nl[lmax], c->hl[lmax], nullptr));
+    for (int ml = lmax - 1; ml >= 0; --ml) {
+        SG_TRY(launch_interp<T>(s, A.U[ml], A.U[ml + 1], c->nl[ml], true, false));
+        for (int jc = 0; jc < ncycle; ++jc) {
+            for (int v = ml; v <= lmax - 1; ++v) {
+                for (int p = 0; p < npre; ++p)
+                    SG_TRY(launch_relax<T>(s, A.U[v], A.RHO[v], c->nl[v], c->hl[v], nullptr));
+                SG_TRY((launch_resid<T, T>(c, A.RES[v], A.U[v], A.RHO[v], c->nl[v], c->hl[v], nullptr)));
+                SG_TRY(launch_restrict<T>(s, A.RHO[v + 1], A.RES[v], c->nl[v + 1]));
+                SG_TRY(launch_fill<T>(s, A.U[v + 1], n3(c->nl[v + 1])));
+            }
+            for (int it = 0; it < 2; ++it)
+                SG_TRY(launch_relax<T>(s, A.U[lmax], A.RHO[lmax], c->nl[lmax], c->hl[lmax], nullptr));
+            for (int v = lmax - 1; v >= ml; --v) {
+                SG_TRY(launch_interp<T>(s, A.U[v], A.U[v + 1], c->nl[v], false, true));
+                for (int p = 0; p < npost; ++p) {
+                    const bool last = (ml == 0 && jc == ncycle - 1 && v == 0 && p == npost - 1);
+                    SG_TRY(launch_relax<T>(s, A.U[v], A.RHO[v], c->nl[v], c->hl[v], last ? resmax : nullptr));
+                }
+            }
+        }
+    }
+    return SG_OK;
+}
+
+template <typename Tu> static int launch_update(sg_uniform *c, double *psi, const Tu *u)
+{
+    hipLaunchKernelGGL(k_update<Tu>, dim3(nblocks(n3(c->n))), dim3(kBlock), 0, c->stream, psi, u, n3(c->n));
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+static int read_norms(sg_uniform *c)
+{
+    SG_HIP(hipMemcpyAsync(c->hnorm, c->dnorm, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+// One outer iteration after the first residual: inner solve on RHO[0] (= f - L phi), update
+// phi += U[0], next residual with its norms into dnorm[0..1].  Resmaxg lands in dnorm[2].
+static int outer_iteration(sg_uniform *c, double *psi, const double *src, const sg_params *p)
+{
+    SG_TRY(launch_fill<double>(c->stream, c->dnorm + 2, 1));
+    if (p->precision == SG_PREC_MIXED) {
+        SG_TRY(mg_lin<float>(c, p->ncycle, p->npre, p->npost, c->dnorm + 2));
+        SG_TRY(launch_update<float>(c, psi, c->f.U[0]));
+        SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+    } else {
+        SG_TRY(mg_lin<double>(c, p->ncycle, p->npre, p->npost, c->dnorm + 2));
+        SG_TRY(launch_update<double>(c, psi, c->d.U[0]));
+        SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+    }
+    return SG_OK;
+}
+
+static void drop_graph(sg_uniform *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_psi = nullptr;
+    c->g_src = nullptr;
+}
+
+static int run_outer_iteration(sg_uniform *c, double *psi, const double *src, const sg_params *p)
+{
+    if (!(p->use_graph && p->precision == SG_PREC_MIXED)) return outer_iteration(c, psi, src, p);
+    const bool same = c->gexec && c->g_psi == psi && c->g_src == src && c->g_ncycle == p->ncycle &&
+                      c->g_npre == p->npre && c->g_npost == p->npost;
+    if (!same) {
+        drop_graph(c);
+        hipGraph_t graph = nullptr;
+        SG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = outer_iteration(c, psi, src, p);
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc != SG_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        SG_HIP(e);
+        const hipError_t e2 = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        SG_HIP(e2);
+        c->g_psi = psi;
+        c->g_src = src;
+        c->g_ncycle = p->ncycle;
+        c->g_npre = p->npre;
+        c->g_npost = p->npost;
+    }
+    SG_HIP(hipGraphLaunch(c->gexec, c->stream));
+    return SG_OK;
+}
+
+// mg.F90:42-94: FP64 defect correction psi += L^-1 (src - L psi) with the chosen inner solver.
+static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg_params *p, sg_stats *st)
+{
+    SG_REQUIRE(c && psi && src, "null context or array");
+    SG_REQUIRE(p && st, "null params/stats");
+    SG_REQUIRE(p->precision == SG_PREC_FP64 || p->precision == SG_PREC_MIXED, "bad precision %d", p->precision);
+    SG_REQUIRE(c->precisions & p->precision, "context was not created with precision %d", p->precision);
+    SG_REQUIRE(p->maxit >= 0 && p->maxit <= 1000, "bad maxit %d", p->maxit);
+    SG_REQUIRE(p->ncycle >= 1 && p->ncycle <= 64 && p->npre >= 0 && p->npre <= 64 && p->npost >= 0 && p->npost <= 64,
+               "bad cycle shape");
+    SG_REQUIRE(p->stop_mode == SG_STOP_REFERENCE || p->stop_mode == SG_STOP_TRUE_RESIDUAL, "bad stop_mode");
+    SG_REQUIRE(p->tol >= 0.0, "bad tol");
+    std::memset(st, 0, sizeof(*st));
+    const bool mixed = p->precision == SG_PREC_MIXED;
+    SG_HIP(hipEventRecord(c->ev0, c->stream));
+    SG_TRY(launch_norm(c, src, n3(c->n), c->dnorm + 3));
+    if (mixed) SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+    else SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+    SG_TRY(read_norms(c));
+    const double fmax = c->hnorm[3], fl2 = std::sqrt(c->hnorm[4]);
+    const double rhomax = c->hnorm[0];
+    int it = 0;
+    int status = SG_ERR_NOT_CONVERGED;
+    for (;;) {
+        const double rmax = c->hnorm[0], rl2 = std::sqrt(c->hnorm[1]);
+        st->rel_linf = fmax > 0 ? rmax / fmax : 0.0;
+        st->rel_l2 = fl2 > 0 ? rl2 / fl2 : 0.0;
+        if (it < SG_HIST) st->hist[it] = st->rel_linf;
+        if (!std::isfinite(rmax) || !std::isfinite(rl2)) { status = SG_ERR_NONFINITE; break; }
+        if (p->stop_mode == SG_STOP_TRUE_RESIDUAL && st->rel_linf < p->tol) { status = SG_OK; break; }
+        if (it > 0 && p->stop_mode == SG_STOP_REFERENCE) {
+            // mg.F90:91 Resmaxg / rhomax with Resmaxg = max|resh2| / h^2 (mg_poisson.F90:77)
+            const double resmaxg = c->hnorm[2] / (c->h * c->h);
+            st->resmaxg = resmaxg;
+            if ((rhomax > 0 ? resmaxg / rhomax : 0.0) < p->tol) { status = SG_OK; break; }
+        }
+        if (it >= p->maxit) break;
+        SG_TRY(run_outer_iteration(c, psi, src, p));
+        SG_TRY(read_norms(c));
+        ++it;
+        st->resmaxg = c->hnorm[2] / (c->h * c->h);
+    }
+    SG_HIP(hipEventRecord(c->ev1, c->stream));
+    SG_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    st->ms = ms;
+    st->iterations = it;
+    st->status = status;
+    return status;
+}
+
+template <typename T> static int alloc_levels(sg_uniform *c, Levels<T> &A)
+{
+    for (int l = 0; l < c->nlev; ++l) {
+        const size_t bytes = sizeof(T) * (size_t)n3(c->nl[l]);
+        SG_HIP(hipMalloc(&A.U[l], bytes));
+        SG_HIP(hipMalloc(&A.RES[l], bytes));
+        SG_HIP(hipMalloc(&A.RHO[l], bytes));
+        SG_HIP(hipMemsetAsync(A.U[l], 0, bytes, c->stream));
+        SG_HIP(hipMemsetAsync(A.RES[l], 0, bytes, c->stream));
+        SG_HIP(hipMemsetAsync(A.RHO[l], 0, bytes, c->stream));
+    }
+    return SG_OK;
+}
+
+template <typename T> static void free_levels(Levels<T> &A)
+{
+    for (int l = 0; l < kMaxLev; ++l) {
+        if (A.U[l]) (void)hipFree(A.U[l]);
+        if (A.RES[l]) (void)hipFree(A.RES[l]);
+        if (A.RHO[l]) (void)hipFree(A.RHO[l]);
+        A.U[l] = A.RES[l] = A.RHO[l] = nullptr;
+    }
+}
+
+static void destroy(sg_uniform *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    free_levels(c->d);
+    free_levels(c->f);
+    if (c->psi_dev) (void)hipFree(c->psi_dev);
+    if (c->src_dev) (void)hipFree(c->src_dev);
+    if (c->pmax) (void)hipFree(c->pmax);
+    if (c->psum) (void)hipFree(c->psum);
+    if (c->dnorm) (void)hipFree(c->dnorm);
+    if (c->hnorm) (void)hipHostFree(c->hnorm);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int create(sg_uniform **out, int n, double h, int precisions)
+{
+    SG_REQUIRE(out, "null output pointer");
+    *out = nullptr;
+    SG_REQUIRE(n >= 2 && n <= kMaxN && (n & (n - 1)) == 0, "n must be a power of two in [2, %d], got %d", kMaxN, n);
+    SG_REQUIRE(h > 0.0 && std::isfinite(h), "bad h");
+    SG_REQUIRE(precisions > 0 && (precisions & ~(SG_PREC_FP64 | SG_PREC_MIXED)) == 0, "bad precisions mask");
+    sg_uniform *c = new sg_uniform();
+    c->n = n;
+    c->h = h;
+    c->precisions = precisions;
+    // mg_data.F90:57-75: levels n/2^l down to 1 cell; h doubles per level (mg_data.F90:554-559)
+    int lmax = 0;
+    while ((1 << lmax) < n) ++lmax;
+    c->nlev = lmax + 1;
+    for (int l = 0; l <= lmax; ++l) {
+        c->nl[l] = n >> l;
+        c->hl[l] = h * (double)(1 << l);
+    }
+    int rc = SG_OK;
+    auto fail = [&](int r) { destroy(c); return r; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreateWithFlags failed");
+        return fail(SG_ERR_HIP);
+    }
+    if ((rc = [&]() -> int {
+            SG_HIP(hipEventCreate(&c->ev0));
+            SG_HIP(hipEventCreate(&c->ev1));
+            c->npart = (int)nblocks(n3(n));
+            SG_HIP(hipMalloc(&c->pmax, sizeof(double) * c->npart));
+            SG_HIP(hipMalloc(&c->psum, sizeof(double) * c->npart));
+            SG_HIP(hipMalloc(&c->dnorm, sizeof(double) * 8));
+            SG_HIP(hipMemsetAsync(c->dnorm, 0, sizeof(double) * 8, c->stream));
+            SG_HIP(hipHostMalloc(&c->hnorm, sizeof(double) * 8, hipHostMallocDefault));
+            if (precisions & SG_PREC_FP64) SG_TRY(alloc_levels<double>(c, c->d));
+            if (precisions & SG_PREC_MIXED) SG_TRY(alloc_levels<float>(c, c->f));
+            SG_HIP(hipStreamSynchronize(c->stream));
+            return SG_OK;
+        }()) != SG_OK)
+        return fail(rc);
+    *out = c;
+    return SG_OK;
+}
+
+} // namespace sg
+
+// ----------------------------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------------------------
+extern "C" {
+
+const char *sg_last_error(void) { return sg::last_error(); }
+
+int sg_version(void) { return 1; }
+
+void sg_default_params(sg_params *p)
+{
+    if (!p) return;
+    p->tol = 1e-2;             // mg.F90:50 errormax
+    p->maxit = 10;             // mg.F90:52 nmg
+    p->stop_mode = SG_STOP_REFERENCE;
+    p->precision = SG_PREC_FP64;
+    p->ncycle = 2;             // mg.F90:123-125
+    p->npre = 2;
+    p->npost = 2;
+    p->use_graph = 1;
+}
+
+int sg_uniform_create(sg_uniform **ctx, int n, double h, int precisions) { return sg::create(ctx, n, h, precisions); }
+
+int sg_uniform_destroy(sg_uniform *ctx)
+{
+    sg::destroy(ctx);
+    return SG_OK;
+}
+
+int sg_uniform_cycle_dev(sg_uniform *ctx, double *psi, const double *src, const sg_params *p, sg_stats *st)
+{
+    return sg::uniform_cycle(ctx, psi, src, p, st);
+}
+
+int sg_uniform_cycle_host(sg_uniform *c, double *psi_gh, const double *src_gh, const sg_params *p, sg_stats *st)
+{
+    SG_REQUIRE(c && psi_gh && src_gh, "null argument");
+    const int n = c->n, ng = n + 2;
+    const size_t dense = (size_t)n * n * n;
+    if (!c->psi_dev) {
+        SG_HIP(hipMalloc(&c->psi_dev, sizeof(double) * dense));
+        SG_HIP(hipMalloc(&c->src_dev, sizeof(double) * dense));
+    }
+    std::vector<double> a(dense), b(dense);
+    auto gi = [&](int i, int j, int k) { return ((size_t)(k + 1) * ng + (j + 1)) * ng + (i + 1); };
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                a[((size_t)k * n + j) * n + i] = psi_gh[gi(i, j, k)];
+                b[((size_t)k * n + j) * n + i] = src_gh[gi(i, j, k)];
+            }
+    SG_HIP(hipMemcpyAsync(c->psi_dev, a.data(), sizeof(double) * dense, hipMemcpyHostToDevice, c->stream));
+    SG_HIP(hipMemcpyAsync(c->src_dev, b.data(), sizeof(double) * dense, hipMemcpyHostToDevice, c->stream));
+    const int rc = sg::uniform_cycle(c, c->psi_dev, c->src_dev, p, st);
+    if (rc != SG_OK && rc != SG_ERR_NOT_CONVERGED && rc != SG_ERR_NONFINITE) return rc;
+    SG_HIP(hipMemcpyAsync(a.data(), c->psi_dev, sizeof(double) * dense, hipMemcpyDeviceToHost, c->stream));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    // write back with periodic ghosts (fmg_boundary_phys_periodic.F90:80-102 order i, j, k)
+    auto wrap = [&](int q) { return q < 0 ? q + n : (q >= n ? q - n : q); };
+    for (int k = -1; k <= n; ++k)
+        for (int j = -1; j <= n; ++j)
+            for (int i = -1; i <= n; ++i)
+                psi_gh[gi(i, j, k)] = a[((size_t)wrap(k) * n + wrap(j)) * n + wrap(i)];
+    return rc;
+}
+
+int sg_uniform_gradient_dev(sg_uniform *c, const double *psi, double *g)
+{
+    SG_REQUIRE(c && psi && g, "null argument");
+    hipLaunchKernelGGL(sg::k_gradient, dim3(sg::nblocks(sg::n3(c->n))), dim3(sg::kBlock), 0, c->stream, g, psi,
+                       c->n, 1.0 / c->h);
+    SG_CHECK_LAUNCH();
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_uniform_relax_dev(sg_uniform *c, int prec, void *u, const void *rhs, int n, double h, double *resmaxg)
+{
+    SG_REQUIRE(c && u && rhs, "null argument");
+    SG_REQUIRE(n >= 1 && n <= c->n, "bad n");
+    SG_TRY(sg::launch_fill<double>(c->stream, c->dnorm + 2, 1));
+    if (prec == SG_PREC_FP64)
+        SG_TRY(sg::launch_relax<double>(c->stream, (double *)u, (const double *)rhs, n, h, c->dnorm + 2));
+    else if (prec == SG_PREC_MIXED)
+        SG_TRY(sg::launch_relax<float>(c->stream, (float *)u, (const float *)rhs, n, h, c->dnorm + 2));
+    else
+        SG_REQUIRE(false, "bad precision");
+    SG_TRY(sg::read_norms(c));
+    if (resmaxg) *resmaxg = c->hnorm[2] / (h * h);
+    return SG_OK;
+}
+
+int sg_uniform_resid_dev(sg_uniform *c, int prec, void *res, const void *u, const void *rhs, int n, double h)
+{
+    SG_REQUIRE(c && res && u && rhs, "null argument");
+    SG_REQUIRE(n >= 1 && n <= c->n, "bad n");
+    if (prec == SG_PREC_FP64)
+        SG_TRY((sg::launch_resid<double, double>(c, (double *)res, (const double *)u, (const double *)rhs, n, h, nullptr)));
+    else if (prec == SG_PREC_MIXED)
+        SG_TRY((sg::launch_resid<float, float>(c, (float *)res, (const float *)u, (const float *)rhs, n, h, nullptr)));
+    else
+        SG_REQUIRE(false, "bad precision");
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_uniform_restrict_dev(sg_uniform *c, int prec, void *uc, const void *uf, int nc)
+{
+    SG_REQUIRE(c && uc && uf, "null argument");
+    SG_REQUIRE(nc >= 1 && 2 * nc <= c->n, "bad nc");
+    if (prec == SG_PREC_FP64) SG_TRY(sg::launch_restrict<double>(c->stream, (double *)uc, (const double *)uf, nc));
+    else if (prec == SG_PREC_MIXED) SG_TRY(sg::launch_restrict<float>(c->stream, (float *)uc, (const float *)uf, nc));
+    else SG_REQUIRE(false, "bad precision");
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_uniform_time_smoother(sg_uniform *c, int prec, int reps, double *ms_per_sweep)
+{
+    SG_REQUIRE(c && ms_per_sweep && reps > 0 && reps <= 100000, "bad argument");
+    SG_REQUIRE(c->precisions & prec, "context lacks precision %d", prec);
+    SG_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (prec == SG_PREC_FP64)
+            SG_TRY(sg::launch_relax<double>(c->stream, c->d.U[0], c->d.RHO[0], c->n, c->h, nullptr));
+        else
+            SG_TRY(sg::launch_relax<float>(c->stream, c->f.U[0], c->f.RHO[0], c->n, c->h, nullptr));
+    }
+    SG_HIP(hipEventRecord(c->ev1, c->stream));
+    SG_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *ms_per_sweep = (double)ms / reps;
+    return SG_OK;
+}
+
+int sg_uniform_interp_dev(sg_uniform *c, int prec, void *uf, const void *uc, int nf, int cubic)
+{
+    SG_REQUIRE(c && uf && uc, "null argument");
+    SG_REQUIRE(nf >= 2 && nf <= c->n && nf % 2 == 0, "bad nf");
+    if (prec == SG_PREC_FP64)
+        SG_TRY(sg::launch_interp<double>(c->stream, (double *)uf, (const double *)uc, nf, cubic != 0, false));
+    else if (prec == SG_PREC_MIXED)
+        SG_TRY(sg::launch_interp<float>(c->stream, (float *)uf, (const float *)uc, nf, cubic != 0, false));
+    else SG_REQUIRE(false, "bad precision");
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+} // extern "C"

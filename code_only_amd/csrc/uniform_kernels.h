@@ -1,0 +1,249 @@
+// Uniform periodic-grid stencil kernels (gfx950).  Dense n^3 arrays, index (k*n + j)*n + i, with
+// the periodic ghost layer of the reference (fmg_boundary_phys_periodic.F90:71-104) replaced by
+// index wrap.  Arithmetic follows the reference's operand order; the library is compiled with
+// -ffp-contract=off so the FP64 instantiations round exactly like the CPU restatement.
+#pragma once
+#include "sg_common.h"
+
+namespace sg {
+
+__device__ __forceinline__ int wrap_p(int i, int n) { return (i + 1 == n) ? 0 : i + 1; }
+__device__ __forceinline__ int wrap_m(int i, int n) { return (i == 0) ? n - 1 : i - 1; }
+// Coarse ghost index -1..n mapped to its periodic image.
+__device__ __forceinline__ int wrap_g(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// floor(i/2) (config.h:249 IJKC with origin 0).
+__device__ __forceinline__ int ijkc(int i) { return (i >= 0) ? (i >> 1) : -((1 - i) >> 1); }
+
+template <typename T> struct Consts;
+template <> struct Consts<double> {
+    static __device__ __forceinline__ double sixth() { return 1.0 / 6.0; }
+};
+template <> struct Consts<float> {
+    static __device__ __forceinline__ float sixth() { return 1.0f / 6.0f; }
+};
+
+// mg_poisson.F90:55-74, one colour (ipass 1 = cells with i+j+k odd, 2 = even).  One thread per
+// colour cell.  resmax (nullable) receives max |resh2| by atomic max.
+template <typename T>
+__global__ void __launch_bounds__(256) k_relax_color(T *__restrict__ u, const T *__restrict__ rhs,
+                                                     int n, T h2, int ipass, double *resmax)
+{
+    const int nh = (n + 1) >> 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)nh * n * n;
+    double amax = 0.0;
+    if (t < total) {
+        const int ih = (int)(t % nh);
+        const long r = t / nh;
+        const int j = (int)(r % n), k = (int)(r / n);
+        const int i = 2 * ih + ((j + k + ipass) & 1);
+        if (i < n) {
+            const int c = (k * n + j) * n + i;
+            const int row = (k * n + j) * n;
+            T s = u[row + wrap_p(i, n)] + u[row + wrap_m(i, n)];
+            s = s + u[(k * n + wrap_p(j, n)) * n + i];
+            s = s + u[(k * n + wrap_m(j, n)) * n + i];
+            s = s + u[(wrap_p(k, n) * n + j) * n + i];
+            s = s + u[(wrap_m(k, n) * n + j) * n + i];
+            const T uc = u[c];
+            s = s - T(6) * uc;
+            s = s - h2 * rhs[c];
+            u[c] = uc + Consts<T>::sixth() * s;
+            amax = fabs((double)s);
+        }
+    }
+    if (resmax) {
+        amax = wave_max(amax);
+        if ((threadIdx.x % kWave) == 0) atomic_max_nonneg(resmax, amax);
+    }
+}
+
+// mg_poisson.F90:17-27: res = -(h2i * (sum6 - 6u)) + rhs, computed in Tin, stored as Tout.
+// pmax/psum (nullable): per-block partial max|res| and sum res^2 (deterministic two-stage norm).
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin *__restrict__ u,
+                                               const Tin *__restrict__ rhs, int n, Tin h2i,
+                                               double *pmax, double *psum)
+{
+    const long n3 = (long)n * n * n;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double mx = 0.0, sm = 0.0;
+    if (c < n3) {
+        const int i = (int)(c % n);
+        const int j = (int)((c / n) % n);
+        const int k = (int)(c / ((long)n * n));
+        const int row = (k * n + j) * n;
+        Tin s = u[row + wrap_p(i, n)] + u[row + wrap_m(i, n)];
+        s = s + u[(k * n + wrap_p(j, n)) * n + i];
+        s = s + u[(k * n + wrap_m(j, n)) * n + i];
+        s = s + u[(wrap_p(k, n) * n + j) * n + i];
+        s = s + u[(wrap_m(k, n) * n + j) * n + i];
+        s = s - Tin(6) * u[c];
+        const Tin r = -(h2i * s) + rhs[c];
+        res[c] = (Tout)r;
+        mx = fabs((double)r);
+        sm = (double)r * (double)r;
+    }
+    if (pmax) {
+        block_max_sum(mx, sm);
+        if (threadIdx.x == 0) { pmax[blockIdx.x] = mx; psum[blockIdx.x] = sm; }
+    }
+}
+
+// Second stage of the norm: one block reduces np partials in a fixed order -> out[0]=max, out[1]=sum.
+__global__ void __launch_bounds__(1024) k_norm_finalize(const double *pmax, const double *psum, int np,
+                                                        double *out)
+{
+    double mx = 0.0, sm = 0.0;
+    for (int q = threadIdx.x; q < np; q += blockDim.x) { mx = fmax(mx, pmax[q]); sm += psum[q]; }
+    block_max_sum(mx, sm);
+    if (threadIdx.x == 0) { out[0] = mx; out[1] = sm; }
+}
+
+// restriction.F90:132-147, one thread per coarse cell.
+template <typename T>
+__global__ void __launch_bounds__(256) k_restrict(T *__restrict__ uc, const T *__restrict__ uf, int nc)
+{
+    const long n3 = (long)nc * nc * nc;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n3) return;
+    const int ic = (int)(c % nc), jc = (int)((c / nc) % nc), kc = (int)(c / ((long)nc * nc));
+    const int nf = 2 * nc;
+    const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
+    auto F = [&](int a, int b, int d) { return uf[((k + d) * nf + (j + b)) * nf + (i + a)]; };
+    T s = F(0, 0, 0) + F(1, 0, 0);
+    s = s + F(0, 1, 0);
+    s = s + F(0, 0, 1);
+    s = s + F(1, 1, 0);
+    s = s + F(1, 0, 1);
+    s = s + F(0, 1, 1);
+    s = s + F(1, 1, 1);
+    uc[c] = s * T(0.125);
+}
+
+// interpolation.F90:226-245 trilinear prolongation of a periodic coarse grid; ADD selects
+// uf = uf + P uc (mg_addint, mg.F90:727-731) instead of uf = P uc.
+template <typename T, bool ADD>
+__global__ void __launch_bounds__(256) k_interp_linear(T *__restrict__ uf, const T *__restrict__ uc, int nf)
+{
+    const long n3 = (long)nf * nf * nf;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n3) return;
+    const int nc = nf >> 1;
+    const int i = (int)(c % nf), jf = (int)((c / nf) % nf), kf = (int)(c / ((long)nf * nf));
+    const int ic = i >> 1, jc = jf >> 1, kc = kf >> 1;
+    const int ic1 = wrap_g(ic + 2 * (i & 1) - 1, nc);
+    const int jc1 = wrap_g(jc + 2 * (jf & 1) - 1, nc);
+    const int kc1 = wrap_g(kc + 2 * (kf & 1) - 1, nc);
+    auto C = [&](int a, int b, int d) { return uc[(d * nc + b) * nc + a]; };
+    T b = C(ic1, jc, kc) + C(ic, jc1, kc);
+    b = b + C(ic, jc, kc1);
+    T e = C(ic, jc1, kc1) + C(ic1, jc, kc1);
+    e = e + C(ic1, jc1, kc);
+    T s = T(27) * C(ic, jc, kc) + T(9) * b;
+    s = s + T(3) * e;
+    s = s + C(ic1, jc1, kc1);
+    const T v = s / T(64);
+    if (ADD) uf[c] = uf[c] + v;
+    else uf[c] = v;
+}
+
+// One axis of the tricubic stencil (interpolation.F90:106-127): coarse indices shifted to stay
+// inside the ghosted coarse range lo..hi, and the four Lagrange weights, in FP64 as there.
+__device__ __forceinline__ void cubic_stencil(int f, int lo, int hi, int idx[4], double a[4])
+{
+    const int cp = ijkc(f);
+    const int cn = cp + 2 * (f & 1) - 1;
+    const int c1 = cp < cn ? cp : cn;
+    const int c2 = cp < cn ? cn : cp;
+    int c0 = 2 * c1 - c2, c3 = 2 * c2 - c1;
+    int nshift = 0;
+    if (c3 > hi) nshift = c3 - hi;
+    if (c0 < lo) nshift = c0 - lo;
+    c0 -= nshift;
+    const int d1 = c1 - nshift, d2 = c2 - nshift;
+    c3 -= nshift;
+    const double xc = cp + 0.25 * (double)(2 * (f & 1) - 1);
+    a[0] = (xc - d1) * (xc - d2) * (xc - c3) / (double)((c0 - d1) * (c0 - d2) * (c0 - c3));
+    a[1] = (xc - c0) * (xc - d2) * (xc - c3) / (double)((d1 - c0) * (d1 - d2) * (d1 - c3));
+    a[2] = (xc - c0) * (xc - d1) * (xc - c3) / (double)((d2 - c0) * (d2 - d1) * (d2 - c3));
+    a[3] = (xc - c0) * (xc - d1) * (xc - d2) / (double)((c3 - c0) * (c3 - d1) * (c3 - d2));
+    idx[0] = c0; idx[1] = d1; idx[2] = d2; idx[3] = c3;
+}
+
+// interpolation.F90:98-217 tricubic prolongation of a periodic coarse grid (nc >= 2): for each
+// fine cell the x pass (bufx), y pass (bufxy) and z pass are evaluated in the same operand order
+// as the reference's three separable sweeps, so the result is bitwise that of the sweeps.
+template <typename T>
+__global__ void __launch_bounds__(256) k_interp_cubic(T *__restrict__ uf, const T *__restrict__ uc, int nf)
+{
+    const long n3 = (long)nf * nf * nf;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n3) return;
+    const int nc = nf >> 1;
+    const int i = (int)(c % nf), jf = (int)((c / nf) % nf), kf = (int)(c / ((long)nf * nf));
+    int ix[4], iy[4], iz[4];
+    double ax[4], ay[4], az[4];
+    cubic_stencil(i, -1, nc, ix, ax);
+    cubic_stencil(jf, -1, nc, iy, ay);
+    cubic_stencil(kf, -1, nc, iz, az);
+    T wx[4], wy[4], wz[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        wx[q] = (T)ax[q]; wy[q] = (T)ay[q]; wz[q] = (T)az[q];
+        ix[q] = wrap_g(ix[q], nc); iy[q] = wrap_g(iy[q], nc); iz[q] = wrap_g(iz[q], nc);
+    }
+    T bz[4];
+#pragma unroll
+    for (int kq = 0; kq < 4; ++kq) {
+        T by[4];
+#pragma unroll
+        for (int jq = 0; jq < 4; ++jq) {
+            const T *row = uc + ((long)iz[kq] * nc + iy[jq]) * nc;
+            T s = wx[0] * row[ix[0]] + wx[1] * row[ix[1]];
+            s = s + wx[2] * row[ix[2]];
+            s = s + wx[3] * row[ix[3]];
+            by[jq] = s;
+        }
+        T s = wy[0] * by[0] + wy[1] * by[1];
+        s = s + wy[2] * by[2];
+        s = s + wy[3] * by[3];
+        bz[kq] = s;
+    }
+    T s = wz[0] * bz[0] + wz[1] * bz[1];
+    s = s + wz[2] * bz[2];
+    s = s + wz[3] * bz[3];
+    uf[c] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(T *__restrict__ p, long len, T v)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < len) p[c] = v;
+}
+
+// mg.F90:87 psi = psi + u, with u in the inner solve's precision.
+template <typename Tu>
+__global__ void __launch_bounds__(256) k_update(double *__restrict__ psi, const Tu *__restrict__ u, long len)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < len) psi[c] = psi[c] + (double)u[c];
+}
+
+// fmg_poisson.F90:29-31: g_d = -(u(+1) - u(-1)) * hi * 0.5, periodic.
+__global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g, const double *__restrict__ u, int n,
+                                                  double hi)
+{
+    const long n3 = (long)n * n * n;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n3) return;
+    const int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((long)n * n));
+    const int row = (k * n + j) * n;
+    g[c] = -(u[row + wrap_p(i, n)] - u[row + wrap_m(i, n)]) * hi * 0.5;
+    g[n3 + c] = -(u[(k * n + wrap_p(j, n)) * n + i] - u[(k * n + wrap_m(j, n)) * n + i]) * hi * 0.5;
+    g[2 * n3 + c] = -(u[(wrap_p(k, n) * n + j) * n + i] - u[(wrap_m(k, n) * n + j) * n + i]) * hi * 0.5;
+}
+
+} // namespace sg

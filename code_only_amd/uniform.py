@@ -1,0 +1,99 @@
+"""Host-side mirror of the reference's uniform-grid multigrid interface (module ``mg``,
+mg.F90:14: mg_init, mg_cycle, mg_finalize) over libsg's C ABI, for torch device tensors.
+
+Arrays are dense n^3 float64 CUDA(HIP) tensors, C order [k][j][i], x fastest.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import (SG_ERR_NONFINITE, SG_ERR_NOT_CONVERGED, SG_PREC_FP64, SG_PREC_MIXED, SG_STOP_REFERENCE,
+                   SG_STOP_TRUE_RESIDUAL)
+
+__all__ = ["UniformSolver", "SG_PREC_FP64", "SG_PREC_MIXED", "SG_STOP_REFERENCE", "SG_STOP_TRUE_RESIDUAL"]
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class UniformSolver:
+    """One periodic n^3 level hierarchy on the current GPU (mg_init / mg_data_init,
+    mg.F90:525-528, mg_data.F90:45-90).  ``precisions`` selects which level arrays to allocate."""
+
+    def __init__(self, n, h=None, precisions=SG_PREC_FP64 | SG_PREC_MIXED):
+        self.n = int(n)
+        self.h = float(h) if h is not None else 1.0 / n
+        self._lib = _lib.load()
+        ctx = ctypes.c_void_p()
+        _lib.check(self._lib.sg_uniform_create(ctypes.byref(ctx), self.n, self.h, int(precisions)))
+        self._ctx = ctx
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.sg_uniform_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, t, dtype=torch.float64, shape=None):
+        if not (t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+            raise ValueError(f"expected a contiguous {dtype} GPU tensor, got {t.dtype} on {t.device}")
+        if shape is not None and tuple(t.shape) != tuple(shape):
+            raise ValueError(f"expected shape {shape}, got {tuple(t.shape)}")
+
+    def cycle(self, psi, src, tol=1e-10, maxit=50, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_MIXED,
+              ncycle=2, npre=2, npost=2, use_graph=True, raise_on_fail=True):
+        """mg_cycle(ju, jrho) (mg.F90:42-94): psi updated in place; returns stats dict."""
+        n = self.n
+        self._check(psi, shape=(n, n, n))
+        self._check(src, shape=(n, n, n))
+        p = _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=precision, ncycle=ncycle,
+                                npre=npre, npost=npost, use_graph=int(bool(use_graph)))
+        st = _lib.SgStats()
+        torch.cuda.synchronize()
+        rc = self._lib.sg_uniform_cycle_dev(self._ctx, _ptr(psi), _ptr(src), ctypes.byref(p), ctypes.byref(st))
+        allow = () if raise_on_fail else (SG_ERR_NOT_CONVERGED, SG_ERR_NONFINITE)
+        _lib.check(rc, allow=allow)
+        return st.as_dict()
+
+    def gradient(self, psi):
+        """g = -grad psi, centred (fmg_poisson.F90:29-31); returns (3, n, n, n)."""
+        n = self.n
+        self._check(psi, shape=(n, n, n))
+        g = torch.empty((3, n, n, n), dtype=torch.float64, device=psi.device)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_gradient_dev(self._ctx, _ptr(psi), _ptr(g)))
+        return g
+
+    # single operators (parity tests)
+    def relax(self, u, rhs, h, prec=SG_PREC_FP64):
+        r = ctypes.c_double()
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_relax_dev(self._ctx, prec, _ptr(u), _ptr(rhs), u.shape[0], h, ctypes.byref(r)))
+        return r.value
+
+    def resid(self, u, rhs, h, prec=SG_PREC_FP64):
+        res = torch.empty_like(u)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_resid_dev(self._ctx, prec, _ptr(res), _ptr(u), _ptr(rhs), u.shape[0], h))
+        return res
+
+    def restrict(self, uf, prec=SG_PREC_FP64):
+        nc = uf.shape[0] // 2
+        uc = torch.empty((nc, nc, nc), dtype=uf.dtype, device=uf.device)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_restrict_dev(self._ctx, prec, _ptr(uc), _ptr(uf), nc))
+        return uc
+
+    def interp(self, uc, cubic, prec=SG_PREC_FP64):
+        nf = uc.shape[0] * 2
+        uf = torch.empty((nf, nf, nf), dtype=uc.dtype, device=uc.device)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_interp_dev(self._ctx, prec, _ptr(uf), _ptr(uc), nf, int(bool(cubic))))
+        return uf

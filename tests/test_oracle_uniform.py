@@ -1,0 +1,189 @@
+"""Pins of the CPU oracle's uniform-grid multigrid (oracle/mg_uniform.c) against facts fixed by the
+mathematics, independent of the oracle's own code: the discrete Fourier symbol of the 7-point
+operator, exact eigenfunctions, np.pad wrap ghosts, block means, tensor-product interpolation
+weights and polynomial reproduction, and an FFT solve of the whole problem.  The reference ships no
+golden vectors for this path and cannot be built here (it needs an MPI Fortran module), so these
+are the pins."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import fft_poisson as fp
+from code_only_amd import problems
+
+
+def rand(n, seed):
+    return np.random.default_rng(seed).standard_normal((n, n, n))
+
+
+@pytest.mark.parametrize("n", [4, 8, 16])
+def test_resid_is_minus_laplacian(n):
+    h = 1.0 / n
+    u = rand(n, 1)
+    f = rand(n, 2)
+    res = oracle.resid(u, f, h)
+    ref = f - fp.apply_laplacian(u, h)
+    assert np.max(np.abs(res - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_eigenfunction():
+    n, h = 16, 1.0 / 16
+    u = fp.eigenfunction(n, 1, 2, 3)
+    lam = fp.eigenvalue(n, h, 1, 2, 3)
+    res = oracle.resid(u, np.zeros_like(u), h)   # = -L u
+    assert np.allclose(-res, lam * u, rtol=0, atol=1e-10 * abs(lam))
+
+
+def test_relax_colours_and_local_solve():
+    """mg_poisson.F90:55-74: pass 1 updates cells with i+j+k odd, pass 2 the even ones, and after a
+    pass the updated cells satisfy their discrete equation exactly (Gauss-Seidel)."""
+    n, h = 8, 1.0 / 8
+    u0 = rand(n, 3)
+    f = rand(n, 4)
+    u = u0.copy()
+    oracle.relax(u, f, h)
+    k, j, i = np.indices((n, n, n))
+    odd = (i + j + k) % 2 == 1
+    # even cells were updated last, with the odd cells' new values: their residual is ~0
+    r = oracle.resid(u, f, h)
+    assert np.max(np.abs(r[~odd])) < 1e-10 * np.max(np.abs(f)) * 1e3
+    # odd cells were updated first: recompute their local solve from u with even cells reset
+    u_half = u0.copy()
+    u_half[odd] = u[odd]
+    r_half = oracle.resid(u_half, f, h)
+    assert np.max(np.abs(r_half[odd])) < 1e-9
+    assert np.max(np.abs(r[odd])) > 1e-3  # black pass disturbed them: colours really alternate
+
+
+def test_relax_resmaxg_definition():
+    """Resmaxg = max|resh2| / h^2 with resh2 = h^2 * (local residual before the update)."""
+    n, h = 4, 0.25
+    u = np.zeros((n, n, n))
+    f = rand(n, 5)
+    uu = u.copy()
+    rm = oracle.relax(uu, f, h)
+    # on the first (odd) pass from u = 0, resh2 = -h^2 f exactly; the second pass sees new values
+    k, j, i = np.indices((n, n, n))
+    odd = (i + j + k) % 2 == 1
+    assert rm >= np.max(np.abs(f[odd])) * (1 - 1e-15)
+
+
+@pytest.mark.parametrize("nc", [1, 2, 4, 8])
+def test_restrict_linear_is_block_mean(nc):
+    uf = rand(2 * nc, 6)
+    uc = oracle.restrict(uf)
+    ref = uf.reshape(nc, 2, nc, 2, nc, 2).mean(axis=(1, 3, 5))
+    assert np.max(np.abs(uc - ref)) < 1e-15 * 8
+
+
+def test_rstrct_cubic_weights():
+    w = oracle.rstrct_cubic_weights()
+    assert w == [-1 / 16, 9 / 16, 9 / 16, -1 / 16]
+
+
+def test_trilinear_weights_by_delta():
+    """(27, 9, 3, 1)/64 = tensor product of (3/4, 1/4); periodic wrap of the coarse neighbour."""
+    nc = 4
+    uc = np.zeros((nc, nc, nc))
+    uc[1, 2, 3] = 1.0
+    uf = oracle.interp(uc, cubic=False)
+    w1 = np.zeros(2 * nc)
+    # 1D: coarse cell c -> fine children 2c, 2c+1 weight 3/4; outer neighbours 2c-1, 2c+2 weight 1/4
+    def w1d(c):
+        w = np.zeros(2 * nc)
+        w[2 * c] = w[2 * c + 1] = 0.75
+        w[(2 * c - 1) % (2 * nc)] = 0.25
+        w[(2 * c + 2) % (2 * nc)] = 0.25
+        return w
+    ref = w1d(1)[:, None, None] * w1d(2)[None, :, None] * w1d(3)[None, None, :]
+    assert np.array_equal(uf, ref)
+    del w1
+
+
+def lagrange_weights(nodes, x):
+    w = []
+    for a in range(4):
+        num = np.prod([x - nodes[b] for b in range(4) if b != a])
+        den = np.prod([nodes[a] - nodes[b] for b in range(4) if b != a])
+        w.append(num / den)
+    return w
+
+
+@pytest.mark.parametrize("nc", [2, 3, 8])
+def test_cubic_stencil_polynomial_reproduction(nc):
+    """interpolation.F90:106-127: for every fine index (edges included) the shifted 4-point
+    stencil reproduces cubics at the fine-cell centre x = floor(f/2) -/+ 1/4, and stays inside
+    the ghosted coarse range -1..nc."""
+    lo, hi = -1, nc
+    for f in range(2 * nc):
+        idx, a = oracle.cubic_stencil(f, lo, hi)
+        assert lo <= min(idx) and max(idx) <= hi and idx == list(range(idx[0], idx[0] + 4))
+        xc = f // 2 + (0.25 if f % 2 else -0.25)
+        assert np.allclose(a, lagrange_weights(idx, xc), rtol=0, atol=1e-15)
+        for deg in range(4):
+            assert abs(sum(w * (q ** deg) for w, q in zip(a, idx)) - xc ** deg) < 1e-12
+        if lo < idx[0] and idx[-1] < hi:   # unshifted interior stencil: (-5,35,105,-7)/128 or mirror
+            assert sorted(np.array(a) * 128) == sorted([-5.0, 35.0, 105.0, -7.0])
+
+
+def test_tricubic_reproduces_cubic_polynomials_inside():
+    """Fine cells whose stencils stay inside 0..nc-1 reproduce a tricubic polynomial exactly."""
+    nc = 8
+    c = np.arange(nc, dtype=float)
+    px = lambda x: 0.3 * x ** 3 - x ** 2 + 2 * x - 1
+    py = lambda y: -0.2 * y ** 3 + 0.5 * y + 3
+    pz = lambda z: 0.1 * z ** 3 + z ** 2
+    uc = pz(c)[:, None, None] * py(c)[None, :, None] * px(c)[None, None, :]
+    uf = oracle.interp(uc, cubic=True)
+    xf = np.array([f // 2 + (0.25 if f % 2 else -0.25) for f in range(2 * nc)])
+    ref = pz(xf)[:, None, None] * py(xf)[None, :, None] * px(xf)[None, None, :]
+    inside = np.zeros(2 * nc, bool)
+    for f in range(2 * nc):
+        idx, _ = oracle.cubic_stencil(f, -1, nc)
+        inside[f] = idx[0] >= 0 and idx[-1] <= nc - 1
+    m = inside[:, None, None] & inside[None, :, None] & inside[None, None, :]
+    assert m.sum() > 0
+    assert np.max(np.abs(uf[m] - ref[m])) < 1e-10 * np.max(np.abs(ref))
+
+
+def test_tricubic_falls_back_to_trilinear_on_1cell_coarse():
+    """interpolation.F90:87-96: coarse 1^3 (+ghosts = 3 < 4) uses trilinear."""
+    uc = np.array([[[2.5]]])
+    assert np.array_equal(oracle.interp(uc, cubic=True), oracle.interp(uc, cubic=False))
+    assert np.all(oracle.interp(uc, cubic=True) == 2.5)
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_mg_cycle_matches_fft_solution(n):
+    f = problems.clumpy_rhs(n, seed=n)
+    h = 1.0 / n
+    psi, st = oracle.mg_cycle(f, h, tol=1e-13, maxit=30)
+    assert st["status"] == 0
+    ref = fp.solve(f, h)
+    d = (psi - psi.mean()) - ref
+    assert np.linalg.norm(d) / np.linalg.norm(ref) < 1e-12
+    g = oracle.gradient(psi, h)
+    gref = fp.gradient(ref, h)
+    assert np.linalg.norm(g - gref) / np.linalg.norm(gref) < 1e-12
+
+
+def test_mg_lin_contracts_residual():
+    """A correct multigrid cycle reduces the residual by orders of magnitude per outer iteration."""
+    f = problems.clumpy_rhs(32, seed=7)
+    _, st = oracle.mg_cycle(f, 1 / 32, tol=1e-12, maxit=20)
+    hist = st["hist"]
+    assert all(hist[i + 1] < 1e-2 * hist[i] for i in range(len(hist) - 2))
+
+
+def test_reference_stop_mode():
+    """mg.F90:83-92: errormax 1e-2, nmg 10, Resmaxg/rhomax test: one FMG pass suffices here."""
+    f = problems.clumpy_rhs(16, seed=1)
+    _, st = oracle.mg_cycle(f, 1 / 16, tol=1e-2, maxit=10, mode=oracle.STOP_REFERENCE)
+    assert st["status"] == 0 and 1 <= st["iterations"] <= 10
+    assert st["rel_linf"] < 1e-2
+
+
+def test_clumpy_rhs_is_zero_mean():
+    f = problems.clumpy_rhs(32, seed=0)
+    assert abs(f.mean()) < 1e-14
+    assert f.shape == (32, 32, 32)

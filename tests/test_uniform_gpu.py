@@ -1,0 +1,187 @@
+"""GPU parity of the uniform periodic multigrid (mg.F90 path) against the CPU oracle.
+
+FP64 kernels and the plain FP64 solve must be bit-for-bit equal to the oracle (same operand
+order, no FMA contraction on either side).  FP32 kernels are compared within FP32 rounding, and
+the mixed-precision solve is checked by its FP64 residual (recomputed here with numpy) and by its
+distance to the FFT solution."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import fft_poisson as fp
+from code_only_amd import problems
+from code_only_amd.uniform import (SG_PREC_FP64, SG_PREC_MIXED, SG_STOP_REFERENCE, SG_STOP_TRUE_RESIDUAL,
+                                   UniformSolver)
+
+pytestmark = pytest.mark.gpu
+
+
+def rand(n, seed):
+    return np.random.default_rng(seed).standard_normal((n, n, n))
+
+
+def T(a, dev, dtype=torch.float64):
+    return torch.tensor(a, dtype=dtype, device=dev).contiguous()
+
+
+@pytest.fixture(scope="module")
+def solver64(gpu):
+    s = UniformSolver(64, precisions=SG_PREC_FP64 | SG_PREC_MIXED)
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 16, 64])
+def test_relax_bitwise(solver64, gpu, n):
+    h = 1.0 / n
+    u = rand(n, 10 + n)
+    f = rand(n, 20 + n)
+    ug = T(u, gpu)
+    rm = solver64.relax(ug, T(f, gpu), h)
+    rm_ref = oracle.relax(u, f, h)
+    assert np.array_equal(ug.cpu().numpy(), u)
+    assert rm == rm_ref
+
+
+@pytest.mark.parametrize("n", [2, 8, 64])
+def test_resid_bitwise(solver64, gpu, n):
+    h = 1.0 / n
+    u, f = rand(n, 1), rand(n, 2)
+    res = solver64.resid(T(u, gpu), T(f, gpu), h).cpu().numpy()
+    assert np.array_equal(res, oracle.resid(u, f, h))
+
+
+@pytest.mark.parametrize("nc", [1, 4, 32])
+def test_restrict_bitwise(solver64, gpu, nc):
+    uf = rand(2 * nc, 3)
+    assert np.array_equal(solver64.restrict(T(uf, gpu)).cpu().numpy(), oracle.restrict(uf))
+
+
+@pytest.mark.parametrize("nc", [1, 2, 3, 16, 32])
+@pytest.mark.parametrize("cubic", [False, True])
+def test_interp_bitwise(solver64, gpu, nc, cubic):
+    if nc == 3:
+        pytest.skip("odd coarse sizes do not occur on power-of-two hierarchies")
+    uc = rand(nc, 4)
+    got = solver64.interp(T(uc, gpu), cubic).cpu().numpy()
+    assert np.array_equal(got, oracle.interp(uc, cubic))
+
+
+@pytest.mark.parametrize("cubic", [False, True])
+def test_fp32_kernels_within_rounding(solver64, gpu, cubic):
+    n, h = 32, 1.0 / 32
+    u, f = rand(n, 5), rand(n, 6)
+    r32 = solver64.resid(T(u, gpu, torch.float32), T(f, gpu, torch.float32), h, prec=SG_PREC_MIXED).double().cpu().numpy()
+    r64 = oracle.resid(u.astype(np.float32).astype(np.float64), f.astype(np.float32).astype(np.float64), h)
+    assert np.max(np.abs(r32 - r64)) <= 2e-6 * np.max(np.abs(r64))
+    ug = T(u, gpu, torch.float32)
+    solver64.relax(ug, T(f, gpu, torch.float32), h, prec=SG_PREC_MIXED)
+    u64 = u.astype(np.float32).astype(np.float64)
+    oracle.relax(u64, f.astype(np.float32).astype(np.float64), h)
+    assert np.max(np.abs(ug.double().cpu().numpy() - u64)) <= 2e-6 * np.max(np.abs(u64))
+    uc = rand(16, 7)
+    i32 = solver64.interp(T(uc, gpu, torch.float32), cubic, prec=SG_PREC_MIXED).double().cpu().numpy()
+    i64 = oracle.interp(uc.astype(np.float32).astype(np.float64), cubic)
+    assert np.max(np.abs(i32 - i64)) <= 1e-6 * np.max(np.abs(i64))
+
+
+@pytest.mark.parametrize("n", [16, 64])
+def test_plain_fp64_cycle_bitwise(gpu, n):
+    f = problems.clumpy_rhs(n, seed=3)
+    h = 1.0 / n
+    s = UniformSolver(n, precisions=SG_PREC_FP64)
+    psi = torch.zeros((n, n, n), dtype=torch.float64, device=gpu)
+    st = s.cycle(psi, T(f, gpu), tol=1e-10, maxit=30, precision=SG_PREC_FP64)
+    ref, st_ref = oracle.mg_cycle(f, h, tol=1e-10, maxit=30)
+    assert st["iterations"] == st_ref["iterations"]
+    assert np.array_equal(psi.cpu().numpy(), ref)
+    assert st["rel_linf"] == st_ref["rel_linf"] and st["rel_linf"] < 1e-10
+    g = s.gradient(psi).cpu().numpy()
+    assert np.array_equal(g, oracle.gradient(ref, h))
+    s.close()
+
+
+def test_plain_fp64_reference_stop_mode(gpu):
+    n = 32
+    f = problems.clumpy_rhs(n, seed=4)
+    s = UniformSolver(n, precisions=SG_PREC_FP64)
+    psi = torch.zeros((n, n, n), dtype=torch.float64, device=gpu)
+    st = s.cycle(psi, T(f, gpu), tol=1e-2, maxit=10, stop_mode=SG_STOP_REFERENCE, precision=SG_PREC_FP64)
+    ref, st_ref = oracle.mg_cycle(f, 1.0 / n, tol=1e-2, maxit=10, mode=oracle.STOP_REFERENCE)
+    assert st["iterations"] == st_ref["iterations"]
+    assert st["resmaxg"] == st_ref["resmaxg"]
+    assert np.array_equal(psi.cpu().numpy(), ref)
+    s.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_mixed_cycle_converges_and_matches_oracle(gpu, use_graph):
+    n = 64
+    h = 1.0 / n
+    f = problems.clumpy_rhs(n, seed=0)
+    s = UniformSolver(n, precisions=SG_PREC_MIXED)
+    psi = torch.zeros((n, n, n), dtype=torch.float64, device=gpu)
+    st = s.cycle(psi, T(f, gpu), tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=use_graph)
+    p = psi.cpu().numpy()
+    r = f - fp.apply_laplacian(p, h)
+    assert np.max(np.abs(r)) / np.max(np.abs(f)) < 1e-10
+    assert abs(np.max(np.abs(r)) / np.max(np.abs(f)) - st["rel_linf"]) < 1e-12
+    ref = fp.solve(f, h)
+    err = np.linalg.norm((p - p.mean()) - ref) / np.linalg.norm(ref)
+    assert err < 1e-9
+    # replaying the captured graph a second time gives the same answer
+    psi2 = torch.zeros_like(psi)
+    st2 = s.cycle(psi2, T(f, gpu), tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=use_graph)
+    assert st2["iterations"] == st["iterations"]
+    assert torch.equal(psi, psi2)
+    s.close()
+
+
+def test_host_pointer_entry_matches_device_entry(gpu):
+    """sg_uniform_cycle_host takes the reference's ghosted u(-1:n,...) layout."""
+    import ctypes
+    from code_only_amd import _lib
+    n = 16
+    h = 1.0 / n
+    f = problems.clumpy_rhs(n, seed=9)
+    s = UniformSolver(n, precisions=SG_PREC_FP64)
+    fg = np.pad(f, 1, mode="wrap")
+    pg = np.zeros_like(fg)
+    p = _lib.default_params(tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64)
+    st = _lib.SgStats()
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = s._lib.sg_uniform_cycle_host(s._ctx, pg.ctypes.data_as(dp), fg.ctypes.data_as(dp), ctypes.byref(p),
+                                      ctypes.byref(st))
+    _lib.check(rc)
+    ref, _ = oracle.mg_cycle(f, h, tol=1e-10, maxit=30)
+    assert np.array_equal(pg, np.pad(ref, 1, mode="wrap"))
+    s.close()
+
+
+def test_full_size_256_mixed_vs_plain(gpu):
+    """256^3 (the benchmark workload): both solvers reach 1e-10 by an independent FP64 residual
+    and agree with each other; the FFT solution pins both."""
+    n = 256
+    h = 1.0 / n
+    f = problems.clumpy_rhs(n, seed=0)
+    ft = T(f, gpu)
+    s = UniformSolver(n)
+    out = {}
+    for prec in (SG_PREC_FP64, SG_PREC_MIXED):
+        psi = torch.zeros_like(ft)
+        st = s.cycle(psi, ft, tol=1e-10, maxit=40, precision=prec)
+        # independent residual with torch.roll in FP64
+        lap = -6.0 * psi
+        for ax in range(3):
+            lap = lap + torch.roll(psi, 1, ax) + torch.roll(psi, -1, ax)
+        r = ft - lap / (h * h)
+        rel = (r.abs().max() / ft.abs().max()).item()
+        assert rel < 1e-10, (prec, rel, st)
+        out[prec] = psi - psi.mean()
+    d = (out[SG_PREC_FP64] - out[SG_PREC_MIXED]).norm() / out[SG_PREC_FP64].norm()
+    assert d.item() < 1e-9
+    ref = torch.tensor(fp.solve(f, h), device=gpu)
+    e = (out[SG_PREC_FP64] - ref).norm() / ref.norm()
+    assert e.item() < 1e-9
+    s.close()
