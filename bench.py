@@ -104,6 +104,20 @@ def main():
     achieved = sweep_bytes / (ms_sweep.value * 1e-3) / 1e9
     log(f"[bench] smoother sweep {ms_sweep.value * 1e3:.1f} us -> {achieved:.0f} GB/s algorithmic")
 
+    tiles = {}
+    dims = [(32, 16, 16, 512), (32, 16, 16, 1024), (32, 8, 8, 256), (16, 8, 8, 256), (64, 8, 8, 512), (32, 8, 16, 512)]
+    lev, m = 0, n
+    while m > 16:
+        cfg = ctypes.c_int()
+        ms = (ctypes.c_float * 8)()
+        _lib.check(solver._lib.sg_uniform_tile_info(solver._ctx, lev, ctypes.byref(cfg), ms, 8))
+        tiles[str(m)] = {"tile": "x".join(map(str, dims[cfg.value][:3])), "threads": dims[cfg.value][3],
+                         "us_per_sweep": {"x".join(map(str, d[:3])) + "/" + str(d[3]): round(ms[q] * 1e3, 1)
+                                          for q, d in enumerate(dims) if ms[q] > 0}}
+        lev += 1
+        m //= 2
+    log(f"[bench] autotuned tiles {json.dumps(tiles)}")
+
     p64 = psi64 - psi64.mean()
     pmx = psimx - psimx.mean()
     err_mx_vs_64 = ((pmx - p64).norm() / p64.norm()).item()
@@ -151,6 +165,7 @@ def main():
                      "kernel": "fine-level FP32 smoother sweep (both colours)",
                      "ms_per_launch": ms_sweep.value},
         "cpu_baseline": cpu,
+        "autotune": tiles,
     }
     print(json.dumps(out), flush=True)
     solver.close()

@@ -27,7 +27,8 @@ SG_HIST = 64
 class SgParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxit", ctypes.c_int), ("stop_mode", ctypes.c_int),
                 ("precision", ctypes.c_int), ("ncycle", ctypes.c_int), ("npre", ctypes.c_int),
-                ("npost", ctypes.c_int), ("use_graph", ctypes.c_int)]
+                ("npost", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("fused", ctypes.c_int)]
 
 
 class SgStats(ctypes.Structure):
@@ -67,6 +68,7 @@ SIGNATURES = {
     "sg_uniform_restrict_dev": (_I, [_VP, _I, _VP, _VP, _I]),
     "sg_uniform_interp_dev": (_I, [_VP, _I, _VP, _VP, _I, _I]),
     "sg_uniform_time_smoother": (_I, [_VP, _I, _I, _DP]),
+    "sg_uniform_tile_info": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), _I]),
 }
 
 _LIB = None

@@ -1,0 +1,492 @@
+// Fused FP32 kernels of the mixed-precision inner solve (gfx950).
+//
+//  * k_sweep_tile: one full red-black sweep (mg_poisson.F90:55-74, both colours) of a TX*TY*TZ
+//    tile held in LDS with an H-cell halo, optionally
+//      - starting from u = 0 (the zero fill of mg.F90:189 folded into the load),
+//      - adding the trilinear coarse-grid correction on load (mg_addint, mg.F90:727-731), the
+//        coarse tile staged in LDS first,
+//      - finishing with the residual (mg_poisson.F90:17-27) restricted to the coarse grid
+//        (restriction.F90:132-147), which needs H = 3.
+//    Reads u from one buffer and writes the tile core to another: a workgroup's halo is then
+//    never a value another workgroup has already updated.  The redundant halo updates repeat the
+//    owner's arithmetic exactly, so the result equals the two-launch colour sweep bit for bit.
+//    Tile shape and workgroup size are template parameters; the host picks one per level size by
+//    timing (all shapes give identical values).
+//  * k_interp_cubic_tile: the FMG tricubic prolongation (interpolation.F90:98-217) as the
+//    reference's three separable passes, on an LDS-staged coarse tile.
+//  * k_coarse: every level of <= 16^3 cells in one workgroup's LDS: the FMG part of mg_lin
+//    (mg.F90:142-215) from the coarsest level up to the first fine level, or one V-cycle from that
+//    level down and back (the bottom of each fine-level V-cycle).
+//
+// Every cell value is produced by the same helpers (relax_value, resid_value, restrict_value,
+// trilinear_value, cubic_stencil) as the one-kernel-per-loop path, in the same order, so the fused
+// solve reproduces the unfused FP32 solve exactly.
+#pragma once
+#include "uniform_kernels.h"
+
+namespace sg {
+
+constexpr int kCoarseMaxN = 16;
+constexpr int kCoarseThreads = 1024;
+
+__device__ __forceinline__ int wrapn(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Trilinear prolongation value at fine cell (i, j, k) of a periodic coarse grid of nc cells;
+// C(a, b, d) returns coarse cell (a, b, d).
+template <typename T, typename Acc>
+__device__ __forceinline__ T trilinear_point(const Acc &C, int i, int j, int k, int nc)
+{
+    const int ic = i >> 1, jc = j >> 1, kc = k >> 1;
+    const int ic1 = wrap_g(ic + 2 * (i & 1) - 1, nc);
+    const int jc1 = wrap_g(jc + 2 * (j & 1) - 1, nc);
+    const int kc1 = wrap_g(kc + 2 * (k & 1) - 1, nc);
+    return trilinear_value<T>(C(ic, jc, kc), C(ic1, jc, kc), C(ic, jc1, kc), C(ic, jc, kc1), C(ic, jc1, kc1),
+                              C(ic1, jc, kc1), C(ic1, jc1, kc), C(ic1, jc1, kc1));
+}
+
+// 4-point cubic combination in the reference's order: (a0 v0 + a1 v1) + a2 v2 + a3 v3.
+template <typename T> __device__ __forceinline__ T cubic4(const T w[4], T v0, T v1, T v2, T v3)
+{
+    T s = w[0] * v0 + w[1] * v1;
+    s = s + w[2] * v2;
+    s = s + w[3] * v3;
+    return s;
+}
+
+// Tricubic value at fine cell (i, j, k) (interpolation.F90:98-217), nc >= 2.
+template <typename T, typename Acc>
+__device__ __forceinline__ T tricubic_point(const Acc &C, int i, int j, int k, int nc)
+{
+    int ix[4], iy[4], iz[4];
+    double ax[4], ay[4], az[4];
+    cubic_stencil(i, -1, nc, ix, ax);
+    cubic_stencil(j, -1, nc, iy, ay);
+    cubic_stencil(k, -1, nc, iz, az);
+    T wx[4], wy[4], wz[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        wx[q] = (T)ax[q]; wy[q] = (T)ay[q]; wz[q] = (T)az[q];
+        ix[q] = wrap_g(ix[q], nc); iy[q] = wrap_g(iy[q], nc); iz[q] = wrap_g(iz[q], nc);
+    }
+    T bz[4];
+#pragma unroll
+    for (int kq = 0; kq < 4; ++kq) {
+        T by[4];
+#pragma unroll
+        for (int jq = 0; jq < 4; ++jq)
+            by[jq] = cubic4<T>(wx, C(ix[0], iy[jq], iz[kq]), C(ix[1], iy[jq], iz[kq]), C(ix[2], iy[jq], iz[kq]),
+                               C(ix[3], iy[jq], iz[kq]));
+        bz[kq] = cubic4<T>(wy, by[0], by[1], by[2], by[3]);
+    }
+    return cubic4<T>(wz, bz[0], bz[1], bz[2], bz[3]);
+}
+
+// Tile index with XCD-aware placement: consecutive workgroups are dealt round-robin to the 8
+// XCDs, so workgroup b is given tile (b % 8) * per + b / 8 -- each XCD works through a contiguous
+// range of tiles (z-slabs), and the halos its tiles share stay in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int ntiles)
+{
+    const int per = (ntiles + 7) / 8;
+    const int b = (int)blockIdx.x;
+    return (b % 8) * per + b / 8;
+}
+
+template <int TX_, int TY_, int TZ_, int NT_> struct Tile {
+    static constexpr int TX = TX_, TY = TY_, TZ = TZ_, NT = NT_;
+};
+
+// One colour of the sweep over the LDS region shrunk by margin M: cells with
+// (gx + gy + gz) % 2 == colour, where colour 1 is the reference's first pass (ipass = 1).  The
+// source values are fetched for all of a thread's cells before any update.
+template <class TL, int LX, int LY, int LZ, int M, int H>
+__device__ __forceinline__ void tile_colour(float *su, const float *__restrict__ f, int n, int x0, int y0, int z0,
+                                            int colour, float h2, double &amax)
+{
+    constexpr int RX = (LX - 2 * M) / 2, RY = LY - 2 * M, RZ = LZ - 2 * M;
+    constexpr int CNT = RX * RY * RZ;
+    constexpr int IT = (CNT + TL::NT - 1) / TL::NT;
+    static_assert((LX - 2 * M) % 2 == 0, "region x extent must be even");
+    static_assert(IT <= 32, "too many cells per thread");
+    float fv[IT];
+    int cv[IT];
+    unsigned core = 0u;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int q = threadIdx.x + it * TL::NT;
+        cv[it] = -1;
+        fv[it] = 0.0f;
+        if (CNT % TL::NT == 0 || q < CNT) {
+            const int xh = q % RX;
+            const int y = M + (q / RX) % RY;
+            const int z = M + q / (RX * RY);
+            const int d = (colour + x0 + M + y0 + y + z0 + z) & 1;
+            const int x = M + 2 * xh + d;
+            cv[it] = (z * LY + y) * LX + x;
+            if (x >= H && x < H + TL::TX && y >= H && y < H + TL::TY && z >= H && z < H + TL::TZ) core |= 1u << it;
+            const int gi = wrapn(x0 + x, n), gj = wrapn(y0 + y, n), gk = wrapn(z0 + z, n);
+            fv[it] = f[(gk * n + gj) * n + gi];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int c = cv[it];
+        if (c >= 0) {
+            float s;
+            su[c] = relax_value<float>(su[c], su[c + 1], su[c - 1], su[c + LX], su[c - LX], su[c + LX * LY],
+                                       su[c - LX * LY], fv[it], h2, &s);
+            if (core & (1u << it)) amax = fmax(amax, fabs((double)s));
+        }
+    }
+}
+
+template <class TL, int H, bool ZERO, bool PROLONG, bool RESID>
+__global__ void __launch_bounds__(TL::NT)
+k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const float *__restrict__ f, int n, float h2,
+             float h2i, const float *__restrict__ uc, float *__restrict__ rc, double *resmax)
+{
+    constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, NT = TL::NT;
+    constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = TZ + 2 * H;
+    constexpr int CX = LX / 2 + 2, CY = LY / 2 + 2, CZ = LZ / 2 + 2;  // staged coarse region
+    static_assert(!RESID || H >= 3, "residual needs a 3-cell halo");
+    static_assert(H % 2 == 0 || !PROLONG, "prolongation staging assumes an even tile origin");
+    __shared__ float su[LX * LY * LZ];
+    __shared__ float sc[PROLONG ? CX * CY * CZ : 1];
+    const int ntx = n / TX, nty = n / TY, ntz = n / TZ;
+    const int ntiles = ntx * nty * ntz;
+    const int t = xcd_tile(ntiles);
+    if (t >= ntiles) return;  // uniform per workgroup, before any barrier
+    const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+    const int x0 = tx * TX - H, y0 = ty * TY - H, z0 = tz * TZ - H;
+    const int nc = n >> 1;
+
+    if (PROLONG) {
+        // coarse cells x0/2 - 1 .. x0/2 + LX/2 (x0 even), periodic images
+        constexpr int CNT = CX * CY * CZ, IT = (CNT + NT - 1) / NT;
+        const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1, cz0 = z0 / 2 - 1;
+        float v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) {
+                const int x = q % CX, y = (q / CX) % CY, z = q / (CX * CY);
+                v[it] = uc[(wrapn(cz0 + z, nc) * nc + wrapn(cy0 + y, nc)) * nc + wrapn(cx0 + x, nc)];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) sc[q] = v[it];
+        }
+        __syncthreads();
+    }
+    {
+        // load u (+ prolongated coarse correction) on tile + halo
+        constexpr int CNT = LX * LY * LZ, IT = (CNT + NT - 1) / NT;
+        float v[IT];
+        if (!ZERO) {
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int q = threadIdx.x + it * NT;
+                if (CNT % NT == 0 || q < CNT) {
+                    const int x = q % LX, y = (q / LX) % LY, z = q / (LX * LY);
+                    v[it] = uin[(wrapn(z0 + z, n) * n + wrapn(y0 + y, n)) * n + wrapn(x0 + x, n)];
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) {
+                float val = ZERO ? 0.0f : v[it];
+                if (PROLONG) {
+                    const int x = q % LX, y = (q / LX) % LY, z = q / (LX * LY);
+                    // local coarse index of the parent: floor(x/2) + 1; neighbour toward the cell
+                    const int ic = (x >> 1) + 1, jc = (y >> 1) + 1, kc = (z >> 1) + 1;
+                    const int ic1 = ic + 2 * (x & 1) - 1, jc1 = jc + 2 * (y & 1) - 1, kc1 = kc + 2 * (z & 1) - 1;
+                    auto C = [&](int a, int b, int d) { return sc[(d * CY + b) * CX + a]; };
+                    val = val + trilinear_value<float>(C(ic, jc, kc), C(ic1, jc, kc), C(ic, jc1, kc), C(ic, jc, kc1),
+                                                       C(ic, jc1, kc1), C(ic1, jc, kc1), C(ic1, jc1, kc),
+                                                       C(ic1, jc1, kc1));
+                }
+                su[q] = val;
+            }
+        }
+    }
+    __syncthreads();
+    double amax = 0.0;
+    tile_colour<TL, LX, LY, LZ, 1, H>(su, f, n, x0, y0, z0, 1, h2, amax);
+    __syncthreads();
+    tile_colour<TL, LX, LY, LZ, 2, H>(su, f, n, x0, y0, z0, 0, h2, amax);
+    __syncthreads();
+    {
+        constexpr int CNT = TX * TY * TZ, IT = (CNT + NT - 1) / NT;
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) {
+                const int x = q % TX, y = (q / TX) % TY, z = q / (TX * TY);
+                const int gi = x0 + H + x, gj = y0 + H + y, gk = z0 + H + z;
+                uout[(gk * n + gj) * n + gi] = su[((z + H) * LY + (y + H)) * LX + (x + H)];
+            }
+        }
+    }
+    if (RESID) {
+        // residual of the 8 fine cells of each coarse cell of the tile, restricted
+        constexpr int QX = TX / 2, QY = TY / 2, QZ = TZ / 2;
+        constexpr int CNT = QX * QY * QZ, IT = (CNT + NT - 1) / NT;
+        float fv[IT][8];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) {
+                const int cx = q % QX, cy = (q / QX) % QY, cz = q / (QX * QY);
+#pragma unroll
+                for (int o = 0; o < 8; ++o) {
+                    const int a = o & 1, b = (o >> 1) & 1, d = (o >> 2) & 1;
+                    const int gi = x0 + H + 2 * cx + a, gj = y0 + H + 2 * cy + b, gk = z0 + H + 2 * cz + d;
+                    fv[it][o] = f[(gk * n + gj) * n + gi];
+                }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) {
+                const int cx = q % QX, cy = (q / QX) % QY, cz = q / (QX * QY);
+                float r[8];
+#pragma unroll
+                for (int o = 0; o < 8; ++o) {
+                    const int a = o & 1, b = (o >> 1) & 1, d = (o >> 2) & 1;
+                    const int c = ((H + 2 * cz + d) * LY + (H + 2 * cy + b)) * LX + (H + 2 * cx + a);
+                    r[o] = resid_value<float>(su[c], su[c + 1], su[c - 1], su[c + LX], su[c - LX], su[c + LX * LY],
+                                              su[c - LX * LY], fv[it][o], h2i);
+                }
+                // restriction.F90:140-143 order: 000 100 010 001 110 101 011 111
+                const int gci = tx * QX + cx, gcj = ty * QY + cy, gck = tz * QZ + cz;
+                rc[(gck * nc + gcj) * nc + gci] =
+                    restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
+            }
+        }
+    }
+    if (resmax) block_atomic_max(resmax, amax);
+}
+
+// FMG tricubic prolongation uf = P3 uc (interpolation.F90:98-217) on an LDS tile: the coarse cells
+// any stencil of the fine tile can touch are staged, then the x, y and z passes run on LDS exactly
+// as the reference's bufx / bufxy sweeps.  Fine tile FX*FY*FZ, nf >= FX, nf % FY == nf % FZ == 0.
+template <int FX, int FY, int FZ, int NT>
+__global__ void __launch_bounds__(NT) k_interp_cubic_tile(float *__restrict__ uf, const float *__restrict__ uc, int nf)
+{
+    constexpr int CX = FX / 2 + 4, CY = FY / 2 + 4, CZ = FZ / 2 + 4;
+    __shared__ float sc[CX * CY * CZ];
+    __shared__ float bx[CZ * CY * FX];
+    __shared__ float bxy[CZ * FY * FX];
+    const int nc = nf >> 1;
+    const int ntx = nf / FX, nty = nf / FY, ntz = nf / FZ;
+    const int ntiles = ntx * nty * ntz;
+    const int t = xcd_tile(ntiles);
+    if (t >= ntiles) return;
+    const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+    const int fx0 = tx * FX, fy0 = ty * FY, fz0 = tz * FZ;
+    const int bxo = fx0 / 2 - 2, byo = fy0 / 2 - 2, bzo = fz0 / 2 - 2;  // ghosted coarse index of local 0
+    {
+        constexpr int CNT = CX * CY * CZ, IT = (CNT + NT - 1) / NT;
+        float v[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) {
+                const int x = q % CX, y = (q / CX) % CY, z = q / (CX * CY);
+                v[it] = uc[(wrapn(bzo + z, nc) * nc + wrapn(byo + y, nc)) * nc + wrapn(bxo + x, nc)];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) sc[q] = v[it];
+        }
+    }
+    // per-axis stencil tables (interpolation.F90:106-127), computed once per tile
+    __shared__ int tix[FX][4], tiy[FY][4], tiz[FZ][4];
+    __shared__ float twx[FX][4], twy[FY][4], twz[FZ][4];
+    if (threadIdx.x < FX + FY + FZ) {
+        const int q = threadIdx.x;
+        int idx[4];
+        double a[4];
+        if (q < FX) {
+            cubic_stencil(fx0 + q, -1, nc, idx, a);
+            for (int e = 0; e < 4; ++e) { tix[q][e] = idx[e] - bxo; twx[q][e] = (float)a[e]; }
+        } else if (q < FX + FY) {
+            cubic_stencil(fy0 + q - FX, -1, nc, idx, a);
+            for (int e = 0; e < 4; ++e) { tiy[q - FX][e] = idx[e] - byo; twy[q - FX][e] = (float)a[e]; }
+        } else {
+            cubic_stencil(fz0 + q - FX - FY, -1, nc, idx, a);
+            for (int e = 0; e < 4; ++e) { tiz[q - FX - FY][e] = idx[e] - bzo; twz[q - FX - FY][e] = (float)a[e]; }
+        }
+    }
+    __syncthreads();
+    // x pass: bx(fx, jc, kc) over the staged coarse y, z range
+    for (int q = threadIdx.x; q < CZ * CY * FX; q += NT) {
+        const int x = q % FX, r = q / FX;
+        const float *row = sc + r * CX;
+        bx[q] = cubic4<float>(twx[x], row[tix[x][0]], row[tix[x][1]], row[tix[x][2]], row[tix[x][3]]);
+    }
+    __syncthreads();
+    // y pass: bxy(fx, fy, kc)
+    for (int q = threadIdx.x; q < CZ * FY * FX; q += NT) {
+        const int x = q % FX, y = (q / FX) % FY, z = q / (FX * FY);
+        const float *col = bx + z * CY * FX + x;
+        bxy[q] = cubic4<float>(twy[y], col[tiy[y][0] * FX], col[tiy[y][1] * FX], col[tiy[y][2] * FX],
+                               col[tiy[y][3] * FX]);
+    }
+    __syncthreads();
+    // z pass: uf(fx, fy, fz)
+    for (int q = threadIdx.x; q < FZ * FY * FX; q += NT) {
+        const int x = q % FX, y = (q / FX) % FY, z = q / (FX * FY);
+        const float *p = bxy + y * FX + x;
+        constexpr int PL = FY * FX;
+        uf[((fz0 + z) * nf + (fy0 + y)) * nf + (fx0 + x)] =
+            cubic4<float>(twz[z], p[tiz[z][0] * PL], p[tiz[z][1] * PL], p[tiz[z][2] * PL], p[tiz[z][3] * PL]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Single-workgroup solver for the levels of <= 16^3 cells, all resident in LDS.
+// ------------------------------------------------------------------------------------------------
+struct CoarseLevels {
+    int nlev;               // levels held: 0 .. nlev-1 (local numbering), n_0 <= 16
+    int n[8], lg[8];        // cells per side and its log2
+    float h2[8], h2i[8];
+    int off[8];             // offset of level l inside each LDS array
+};
+
+constexpr int kCoarseCells = 16 * 16 * 16 + 8 * 8 * 8 + 4 * 4 * 4 + 2 * 2 * 2 + 1;
+
+__device__ __forceinline__ void c_relax(float *U, const float *R, int n, int lg, float h2, double *resmax)
+{
+    double amax = 0.0;
+    const int lgh = lg > 0 ? lg - 1 : 0;  // log2 of cells per colour row (n >= 2), 1 for n == 1
+    const int m = n - 1;
+    for (int ipass = 1; ipass <= 2; ++ipass) {
+        for (int t = threadIdx.x; t < (n * n * n + 1) / 2; t += kCoarseThreads) {
+            const int ih = t & ((1 << lgh) - 1), j = (t >> lgh) & m, k = t >> (lgh + lg);
+            const int i = (n == 1) ? 0 : 2 * ih + ((j + k + ipass) & 1);
+            if (n == 1 && ipass == 1) continue;   // single cell has colour "even" (ipass 2)
+            const int c = (k * n + j) * n + i, row = (k * n + j) * n;
+            float s;
+            U[c] = relax_value<float>(U[c], U[row + ((i + 1) & m)], U[row + ((i - 1) & m)],
+                                      U[(k * n + ((j + 1) & m)) * n + i], U[(k * n + ((j - 1) & m)) * n + i],
+                                      U[(((k + 1) & m) * n + j) * n + i], U[(((k - 1) & m) * n + j) * n + i], R[c],
+                                      h2, &s);
+            amax = fmax(amax, fabs((double)s));
+        }
+        __syncthreads();
+    }
+    if (resmax) block_atomic_max(resmax, amax);
+}
+
+// residual of level (n, U, R) restricted into Rc: each coarse cell evaluates its 8 fine residuals
+// (mg_poisson.F90:17-27 then restriction.F90:132-147, same values as the two separate passes).
+__device__ __forceinline__ void c_resid_restrict(const float *U, const float *R, int n, int lg, float h2i, float *Rc)
+{
+    const int nc = n >> 1, lgc = lg - 1, m = n - 1;
+    for (int q = threadIdx.x; q < nc * nc * nc; q += kCoarseThreads) {
+        const int cx = q & (nc - 1), cy = (q >> lgc) & (nc - 1), cz = q >> (2 * lgc);
+        float r[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            const int i = 2 * cx + (o & 1), j = 2 * cy + ((o >> 1) & 1), k = 2 * cz + ((o >> 2) & 1);
+            const int c = (k * n + j) * n + i, row = (k * n + j) * n;
+            r[o] = resid_value<float>(U[c], U[row + ((i + 1) & m)], U[row + ((i - 1) & m)],
+                                      U[(k * n + ((j + 1) & m)) * n + i], U[(k * n + ((j - 1) & m)) * n + i],
+                                      U[(((k + 1) & m) * n + j) * n + i], U[(((k - 1) & m) * n + j) * n + i], R[c],
+                                      h2i);
+        }
+        Rc[q] = restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void c_restrict(float *Rc, const float *Rf, int nc, int lgc)
+{
+    const int nf = 2 * nc;
+    for (int q = threadIdx.x; q < nc * nc * nc; q += kCoarseThreads) {
+        const int i = 2 * (q & (nc - 1)), j = 2 * ((q >> lgc) & (nc - 1)), k = 2 * (q >> (2 * lgc));
+        auto F = [&](int a, int b, int d) { return Rf[((k + d) * nf + (j + b)) * nf + (i + a)]; };
+        Rc[q] = restrict_value<float>(F(0, 0, 0), F(1, 0, 0), F(0, 1, 0), F(0, 0, 1), F(1, 1, 0), F(1, 0, 1),
+                                      F(0, 1, 1), F(1, 1, 1));
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void c_fill0(float *U, int n)
+{
+    for (int c = threadIdx.x; c < n * n * n; c += kCoarseThreads) U[c] = 0.0f;
+    __syncthreads();
+}
+
+// mg_interp (mg.F90:608-633): cubic (FMG) unless the coarse grid plus ghosts is narrower than 4.
+__device__ __forceinline__ void c_interp(float *Uf, const float *Uc, int nf, int lgf, bool cubic, bool add)
+{
+    const int nc = nf >> 1;
+    auto C = [&](int a, int b, int d) { return Uc[(d * nc + b) * nc + a]; };
+    for (int c = threadIdx.x; c < nf * nf * nf; c += kCoarseThreads) {
+        const int i = c & (nf - 1), j = (c >> lgf) & (nf - 1), k = c >> (2 * lgf);
+        float v;
+        if (cubic && nc + 2 >= 4) v = tricubic_point<float>(C, i, j, k, nc);
+        else v = trilinear_point<float>(C, i, j, k, nc);
+        Uf[c] = add ? Uf[c] + v : v;
+    }
+    __syncthreads();
+}
+
+// mode 0: FMG over all held levels from RHO of local level 0 (mg.F90:142-215 restricted to them);
+// mode 1: one V-cycle from local level 0 with zero initial guess (mg.F90:179-210).
+// resmax receives max|resh2| of the final smoothing call when non-null (only meaningful when the
+// held level 0 is the finest level of the problem).
+__global__ void __launch_bounds__(kCoarseThreads)
+k_coarse(const float *__restrict__ rhs_in, float *__restrict__ u_out, CoarseLevels L, int mode, int ncycle,
+         int npre, int npost, double *resmax)
+{
+    __shared__ float sU[kCoarseCells], sR[kCoarseCells];
+    const int lmax = L.nlev - 1;
+    const int n0 = L.n[0];
+    for (int c = threadIdx.x; c < n0 * n0 * n0; c += kCoarseThreads) sR[c] = rhs_in[c];
+    __syncthreads();
+    auto U = [&](int l) { return sU + L.off[l]; };
+    auto R = [&](int l) { return sR + L.off[l]; };
+    auto vcycle = [&](int ml, bool last_cycle) {
+        for (int v = ml; v <= lmax - 1; ++v) {
+            for (int p = 0; p < npre; ++p) c_relax(U(v), R(v), L.n[v], L.lg[v], L.h2[v], nullptr);
+            c_resid_restrict(U(v), R(v), L.n[v], L.lg[v], L.h2i[v], R(v + 1));
+            c_fill0(U(v + 1), L.n[v + 1]);
+        }
+        for (int it = 0; it < 2; ++it) c_relax(U(lmax), R(lmax), L.n[lmax], L.lg[lmax], L.h2[lmax], nullptr);
+        for (int v = lmax - 1; v >= ml; --v) {
+            c_interp(U(v), U(v + 1), L.n[v], L.lg[v], false, true);
+            for (int p = 0; p < npost; ++p) {
+                const bool last = last_cycle && v == 0 && p == npost - 1;
+                c_relax(U(v), R(v), L.n[v], L.lg[v], L.h2[v], last ? resmax : nullptr);
+            }
+        }
+    };
+    if (mode == 0) {
+        for (int l = 1; l <= lmax; ++l) c_restrict(R(l), R(l - 1), L.n[l], L.lg[l]);
+        c_fill0(U(lmax), L.n[lmax]);
+        for (int it = 0; it < 2; ++it) c_relax(U(lmax), R(lmax), L.n[lmax], L.lg[lmax], L.h2[lmax], nullptr);
+        for (int ml = lmax - 1; ml >= 0; --ml) {
+            c_interp(U(ml), U(ml + 1), L.n[ml], L.lg[ml], true, false);
+            for (int jc = 0; jc < ncycle; ++jc) vcycle(ml, ml == 0 && jc == ncycle - 1);
+        }
+    } else {
+        c_fill0(U(0), n0);
+        if (lmax == 0) {
+            for (int it = 0; it < 2; ++it) c_relax(U(0), R(0), n0, L.lg[0], L.h2[0], nullptr);
+        } else {
+            vcycle(0, false);
+        }
+    }
+    for (int c = threadIdx.x; c < n0 * n0 * n0; c += kCoarseThreads) u_out[c] = sU[c];
+}
+
+} // namespace sg

@@ -81,4 +81,21 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v)
               static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// Block-wide max of per-thread non-negative values, folded into *addr with one atomic per block.
+// Every thread of the block must call it (it contains a barrier).
+__device__ __forceinline__ void block_atomic_max(double *addr, double v)
+{
+    __shared__ double smax[16];
+    v = wave_max(v);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+    if (lane == 0) smax[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = smax[0];
+        for (int w = 1; w < nw; ++w) a = fmax(a, smax[w]);
+        atomic_max_nonneg(addr, a);
+    }
+}
+
 } // namespace sg

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <vector>
 
+#include "fast_kernels.h"
 #include "sg_common.h"
 #include "uniform_kernels.h"
 
@@ -37,6 +38,7 @@ const char *last_error() { return g_err; }
 constexpr int kMaxLev = 16;
 constexpr int kBlock = 256;
 constexpr int kMaxN = 1024;
+constexpr int kNormBlocks = 2048;  // grid of the norm-reducing kernels (fixed: deterministic sums)
 
 template <typename T> struct Levels {
     T *U[kMaxLev] = {};
@@ -58,6 +60,9 @@ struct sg_uniform {
     double hl[kMaxLev] = {};
     Levels<double> d;
     Levels<float> f;
+    float *U1[kMaxLev] = {};   // second FP32 U buffer per level: the fused sweeps ping-pong
+    int tile_cfg[kMaxLev] = {}; // autotuned sweep tile shape per fine level
+    float tune_ms[kMaxLev][8] = {};
     double *psi_dev = nullptr, *src_dev = nullptr;   // staging for the host-pointer entry point
     double *pmax = nullptr, *psum = nullptr;          // norm partials
     int npart = 0;
@@ -69,7 +74,7 @@ struct sg_uniform {
     hipGraphExec_t gexec = nullptr;
     double *g_psi = nullptr;
     const double *g_src = nullptr;
-    int g_ncycle = -1, g_npre = -1, g_npost = -1;
+    int g_ncycle = -1, g_npre = -1, g_npost = -1, g_fused = -1, g_stop = -1;
 };
 
 namespace sg {
@@ -104,7 +109,7 @@ static int launch_resid(sg_uniform *c, Tout *res, const Tin *u, const Tin *rhs, 
 {
     const long work = n3(n);
     const Tin h2i = (Tin)(1.0 / (h * h));
-    const unsigned nb = nblocks(work);
+    const unsigned nb = std::min<unsigned>(nblocks(work), (unsigned)kNormBlocks);
     if (norms && (int)nb > c->npart) {
         set_error("norm partial buffer too small (%u > %d)", nb, c->npart);
         return SG_ERR_ARG;
@@ -123,16 +128,18 @@ static int launch_resid(sg_uniform *c, Tout *res, const Tin *u, const Tin *rhs, 
 __global__ void __launch_bounds__(256) k_norm_partial(const double *__restrict__ a, long len, double *pmax,
                                                       double *psum)
 {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double mx = 0.0, sm = 0.0;
-    if (c < len) { mx = fabs(a[c]); sm = a[c] * a[c]; }
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < len; c += (long)gridDim.x * blockDim.x) {
+        mx = fmax(mx, fabs(a[c]));
+        sm += a[c] * a[c];
+    }
     block_max_sum(mx, sm);
     if (threadIdx.x == 0) { pmax[blockIdx.x] = mx; psum[blockIdx.x] = sm; }
 }
 
 static int launch_norm(sg_uniform *c, const double *a, long len, double *out)
 {
-    const unsigned nb = nblocks(len);
+    const unsigned nb = std::min<unsigned>(nblocks(len), (unsigned)kNormBlocks);
     SG_REQUIRE((int)nb <= c->npart, "norm partial buffer too small");
     hipLaunchKernelGGL(k_norm_partial, dim3(nb), dim3(kBlock), 0, c->stream, a, len, c->pmax, c->psum);
     SG_CHECK_LAUNCH();
@@ -204,6 +211,204 @@ template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int
     return SG_OK;
 }
 
+// Tile shapes the autotuner chooses from (all give identical values).
+using Tile0 = Tile<32, 16, 16, 512>;
+using Tile1 = Tile<32, 16, 16, 1024>;
+using Tile2 = Tile<32, 8, 8, 256>;
+using Tile3 = Tile<16, 8, 8, 256>;
+using Tile4 = Tile<64, 8, 8, 512>;
+using Tile5 = Tile<32, 8, 16, 512>;
+constexpr int kNumTiles = 6;
+static const int kTileDims[kNumTiles][4] = {
+    {32, 16, 16, 512}, {32, 16, 16, 1024}, {32, 8, 8, 256}, {16, 8, 8, 256}, {64, 8, 8, 512}, {32, 8, 16, 512}};
+
+static bool tile_fits(int cfg, int n)
+{
+    const int *d = kTileDims[cfg];
+    return n >= d[0] && n >= d[1] && n >= d[2] && n % d[0] == 0 && n % d[1] == 0 && n % d[2] == 0;
+}
+
+template <class TL>
+static void sweep_variant(hipStream_t s, const float *uin, float *uout, const float *f, int n, float h2, float h2i,
+                          bool zero, bool prolong, bool resid, const float *uc, float *rc, double *resmax)
+{
+    const int ntiles = (n / TL::TX) * (n / TL::TY) * (n / TL::TZ);
+    const dim3 grid(8 * ((ntiles + 7) / 8)), block(TL::NT);
+    if (resid && zero)
+        hipLaunchKernelGGL((k_sweep_tile<TL, 3, true, false, true>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else if (resid)
+        hipLaunchKernelGGL((k_sweep_tile<TL, 3, false, false, true>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else if (prolong)
+        hipLaunchKernelGGL((k_sweep_tile<TL, 2, false, true, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else if (zero)
+        hipLaunchKernelGGL((k_sweep_tile<TL, 2, true, false, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else
+        hipLaunchKernelGGL((k_sweep_tile<TL, 2, false, false, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+}
+
+// One fused red-black sweep of a fine level (n >= 32) with tile shape cfg: uin -> uout.
+static int launch_sweep_cfg(hipStream_t s, int cfg, const float *uin, float *uout, const float *f, int n, double h,
+                            bool zero, bool prolong, bool resid, const float *uc, float *rc, double *resmax)
+{
+    SG_REQUIRE(cfg >= 0 && cfg < kNumTiles && tile_fits(cfg, n), "tile shape %d does not fit n = %d", cfg, n);
+    const float h2 = (float)(h * h), h2i = (float)(1.0 / (h * h));
+    switch (cfg) {
+    case 0: sweep_variant<Tile0>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 1: sweep_variant<Tile1>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 2: sweep_variant<Tile2>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 3: sweep_variant<Tile3>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 4: sweep_variant<Tile4>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    default: sweep_variant<Tile5>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    }
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+static int level_of(const sg_uniform *c, int n)
+{
+    for (int l = 0; l < c->nlev; ++l)
+        if (c->nl[l] == n) return l;
+    return -1;
+}
+
+static int launch_sweep(sg_uniform *c, const float *uin, float *uout, const float *f, int n, double h, bool zero,
+                        bool prolong, bool resid, const float *uc, float *rc, double *resmax)
+{
+    const int l = level_of(c, n);
+    SG_REQUIRE(l >= 0, "no level of size %d", n);
+    return launch_sweep_cfg(c->stream, c->tile_cfg[l], uin, uout, f, n, h, zero, prolong, resid, uc, rc, resmax);
+}
+
+constexpr int kCubicFX = 32, kCubicFY = 8, kCubicFZ = 8, kCubicNT = 256;
+
+// FMG tricubic prolongation of a fine level n >= 32 through LDS tiles.
+static int launch_interp_cubic_tile(hipStream_t s, float *uf, const float *uc, int nf)
+{
+    SG_REQUIRE(nf >= kCubicFX && nf % kCubicFX == 0 && nf % kCubicFY == 0 && nf % kCubicFZ == 0,
+               "tiled tricubic needs nf >= 32, got %d", nf);
+    const int ntiles = (nf / kCubicFX) * (nf / kCubicFY) * (nf / kCubicFZ);
+    hipLaunchKernelGGL((k_interp_cubic_tile<kCubicFX, kCubicFY, kCubicFZ, kCubicNT>), dim3(8 * ((ntiles + 7) / 8)),
+                       dim3(kCubicNT), 0, s, uf, uc, nf);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+// Autotuner: for every fine level, time each fitting tile shape on the level's own arrays and
+// keep the fastest.  Runs once per context; values do not depend on the choice.
+static int autotune_tiles(sg_uniform *c)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    SG_HIP(hipEventCreate(&a));
+    SG_HIP(hipEventCreate(&b));
+    int rc = SG_OK;
+    for (int l = 0; l < c->nlev && rc == SG_OK; ++l) {
+        const int n = c->nl[l];
+        c->tile_cfg[l] = -1;
+        if (n <= kCoarseMaxN) continue;
+        float best = 1e30f;
+        for (int cfg = 0; cfg < kNumTiles && rc == SG_OK; ++cfg) {
+            if (!tile_fits(cfg, n)) continue;
+            const int reps = n >= 256 ? 4 : 8;
+            for (int r = 0; r < reps + 1 && rc == SG_OK; ++r) {
+                if (r == 1) {
+                    if (hipEventRecord(a, c->stream) != hipSuccess) rc = SG_ERR_HIP;
+                }
+                if (rc == SG_OK)
+                    rc = launch_sweep_cfg(c->stream, cfg, (r & 1) ? c->U1[l] : c->f.U[l], (r & 1) ? c->f.U[l] : c->U1[l],
+                                          c->f.RHO[l], n, c->hl[l], false, false, false, nullptr, nullptr, nullptr);
+            }
+            if (rc != SG_OK) break;
+            float ms = 0.f;
+            if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                rc = SG_ERR_HIP;
+                break;
+            }
+            c->tune_ms[l][cfg] = ms / reps;
+            if (ms < best) {
+                best = ms;
+                c->tile_cfg[l] = cfg;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc == SG_ERR_HIP) set_error("autotune: HIP call failed");
+    return rc;
+}
+
+static int launch_coarse(sg_uniform *c, const CoarseLevels &L, const float *rhs, float *u, int mode, int ncycle,
+                         int npre, int npost, double *resmax)
+{
+    hipLaunchKernelGGL(k_coarse, dim3(1), dim3(kCoarseThreads), 0, c->stream, rhs, u, L, mode, ncycle, npre, npost,
+                       resmax);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+// mg_lin (mg.F90:106-218) in FP32 with fused kernels: the levels of <= 16^3 cells run inside one
+// single-workgroup kernel (k_coarse); each fine-level visit of a V-cycle is npre + npost fused
+// sweeps, the last pre-sweep carrying the residual and its restriction, the first post-sweep the
+// prolongated correction.  Same values as mg_lin<float>, bit for bit.  *u0 receives the buffer
+// that holds the level-0 correction.
+static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax, const float **u0)
+{
+    SG_REQUIRE(npre >= 1 && npost >= 1, "fused path needs npre, npost >= 1");
+    Levels<float> &A = c->f;
+    hipStream_t s = c->stream;
+    const int lmax = c->nlev - 1;
+    int lc = 0;
+    while (c->nl[lc] > kCoarseMaxN) ++lc;
+    CoarseLevels L{};
+    L.nlev = lmax - lc + 1;
+    SG_REQUIRE(L.nlev <= 8, "too many coarse levels");
+    int off = 0;
+    for (int q = 0; q < L.nlev; ++q) {
+        const int l = lc + q;
+        L.n[q] = c->nl[l];
+        L.lg[q] = 0;
+        while ((1 << L.lg[q]) < c->nl[l]) ++L.lg[q];
+        L.h2[q] = (float)(c->hl[l] * c->hl[l]);
+        L.h2i[q] = (float)(1.0 / (c->hl[l] * c->hl[l]));
+        L.off[q] = off;
+        off += c->nl[l] * c->nl[l] * c->nl[l];
+    }
+    SG_REQUIRE(off <= kCoarseCells, "coarse levels exceed LDS capacity");
+    int cur[kMaxLev] = {};
+    auto buf = [&](int l, int b) -> float * { return b ? c->U1[l] : A.U[l]; };
+    for (int l = 1; l <= lc; ++l) SG_TRY(launch_restrict<float>(s, A.RHO[l], A.RHO[l - 1], c->nl[l]));
+    SG_TRY(launch_coarse(c, L, A.RHO[lc], A.U[lc], 0, ncycle, npre, npost, lc == 0 ? resmax : nullptr));
+    cur[lc] = 0;
+    for (int ml = lc - 1; ml >= 0; --ml) {
+        SG_TRY(launch_interp_cubic_tile(s, A.U[ml], buf(ml + 1, cur[ml + 1]), c->nl[ml]));
+        cur[ml] = 0;
+        for (int jc = 0; jc < ncycle; ++jc) {
+            for (int v = ml; v <= lc - 1; ++v) {
+                for (int p = 0; p < npre; ++p) {
+                    const bool zero = v > ml && p == 0;
+                    const bool resid = p == npre - 1;
+                    SG_TRY(launch_sweep(c, buf(v, cur[v]), buf(v, 1 - cur[v]), A.RHO[v], c->nl[v], c->hl[v], zero,
+                                        false, resid, nullptr, resid ? A.RHO[v + 1] : nullptr, nullptr));
+                    cur[v] ^= 1;
+                }
+            }
+            SG_TRY(launch_coarse(c, L, A.RHO[lc], A.U[lc], 1, ncycle, npre, npost, nullptr));
+            cur[lc] = 0;
+            for (int v = lc - 1; v >= ml; --v) {
+                for (int p = 0; p < npost; ++p) {
+                    const bool last = (ml == 0 && jc == ncycle - 1 && v == 0 && p == npost - 1);
+                    SG_TRY(launch_sweep(c, buf(v, cur[v]), buf(v, 1 - cur[v]), A.RHO[v], c->nl[v], c->hl[v], false,
+                                        p == 0, false, p == 0 ? buf(v + 1, cur[v + 1]) : nullptr, nullptr,
+                                        last ? resmax : nullptr));
+                    cur[v] ^= 1;
+                }
+            }
+        }
+    }
+    *u0 = buf(0, cur[0]);
+    return SG_OK;
+}
+
 template <typename Tu> static int launch_update(sg_uniform *c, double *psi, const Tu *u)
 {
     hipLaunchKernelGGL(k_update<Tu>, dim3(nblocks(n3(c->n))), dim3(kBlock), 0, c->stream, psi, u, n3(c->n));
@@ -222,13 +427,17 @@ static int read_norms(sg_uniform *c)
 // phi += U[0], next residual with its norms into dnorm[0..1].  Resmaxg lands in dnorm[2].
 static int outer_iteration(sg_uniform *c, double *psi, const double *src, const sg_params *p)
 {
+    // Resmaxg is only read by the reference stopping test; the true-residual test skips it.
+    double *resmax = p->stop_mode == SG_STOP_REFERENCE ? c->dnorm + 2 : nullptr;
     SG_TRY(launch_fill<double>(c->stream, c->dnorm + 2, 1));
     if (p->precision == SG_PREC_MIXED) {
-        SG_TRY(mg_lin<float>(c, p->ncycle, p->npre, p->npost, c->dnorm + 2));
-        SG_TRY(launch_update<float>(c, psi, c->f.U[0]));
+        const float *u0 = c->f.U[0];
+        if (p->fused) SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost, resmax, &u0));
+        else SG_TRY(mg_lin<float>(c, p->ncycle, p->npre, p->npost, resmax));
+        SG_TRY(launch_update<float>(c, psi, u0));
         SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
     } else {
-        SG_TRY(mg_lin<double>(c, p->ncycle, p->npre, p->npost, c->dnorm + 2));
+        SG_TRY(mg_lin<double>(c, p->ncycle, p->npre, p->npost, resmax));
         SG_TRY(launch_update<double>(c, psi, c->d.U[0]));
         SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
     }
@@ -247,7 +456,8 @@ static int run_outer_iteration(sg_uniform *c, double *psi, const double *src, co
 {
     if (!(p->use_graph && p->precision == SG_PREC_MIXED)) return outer_iteration(c, psi, src, p);
     const bool same = c->gexec && c->g_psi == psi && c->g_src == src && c->g_ncycle == p->ncycle &&
-                      c->g_npre == p->npre && c->g_npost == p->npost;
+                      c->g_npre == p->npre && c->g_npost == p->npost && c->g_fused == p->fused &&
+                      c->g_stop == p->stop_mode;
     if (!same) {
         drop_graph(c);
         hipGraph_t graph = nullptr;
@@ -267,6 +477,8 @@ static int run_outer_iteration(sg_uniform *c, double *psi, const double *src, co
         c->g_ncycle = p->ncycle;
         c->g_npre = p->npre;
         c->g_npost = p->npost;
+        c->g_fused = p->fused;
+        c->g_stop = p->stop_mode;
     }
     SG_HIP(hipGraphLaunch(c->gexec, c->stream));
     return SG_OK;
@@ -284,6 +496,8 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
                "bad cycle shape");
     SG_REQUIRE(p->stop_mode == SG_STOP_REFERENCE || p->stop_mode == SG_STOP_TRUE_RESIDUAL, "bad stop_mode");
     SG_REQUIRE(p->tol >= 0.0, "bad tol");
+    SG_REQUIRE(!p->fused || p->precision != SG_PREC_MIXED || (p->npre >= 1 && p->npost >= 1),
+               "fused mixed path needs npre, npost >= 1");
     std::memset(st, 0, sizeof(*st));
     const bool mixed = p->precision == SG_PREC_MIXED;
     SG_HIP(hipEventRecord(c->ev0, c->stream));
@@ -355,6 +569,8 @@ static void destroy(sg_uniform *c)
     drop_graph(c);
     free_levels(c->d);
     free_levels(c->f);
+    for (int l = 0; l < kMaxLev; ++l)
+        if (c->U1[l]) (void)hipFree(c->U1[l]);
     if (c->psi_dev) (void)hipFree(c->psi_dev);
     if (c->src_dev) (void)hipFree(c->src_dev);
     if (c->pmax) (void)hipFree(c->pmax);
@@ -402,7 +618,15 @@ static int create(sg_uniform **out, int n, double h, int precisions)
             SG_HIP(hipMemsetAsync(c->dnorm, 0, sizeof(double) * 8, c->stream));
             SG_HIP(hipHostMalloc(&c->hnorm, sizeof(double) * 8, hipHostMallocDefault));
             if (precisions & SG_PREC_FP64) SG_TRY(alloc_levels<double>(c, c->d));
-            if (precisions & SG_PREC_MIXED) SG_TRY(alloc_levels<float>(c, c->f));
+            if (precisions & SG_PREC_MIXED) {
+                SG_TRY(alloc_levels<float>(c, c->f));
+                for (int l = 0; l < c->nlev; ++l) {
+                    const size_t bytes = sizeof(float) * (size_t)n3(c->nl[l]);
+                    SG_HIP(hipMalloc(&c->U1[l], bytes));
+                    SG_HIP(hipMemsetAsync(c->U1[l], 0, bytes, c->stream));
+                }
+                SG_TRY(autotune_tiles(c));
+            }
             SG_HIP(hipStreamSynchronize(c->stream));
             return SG_OK;
         }()) != SG_OK)
@@ -433,6 +657,7 @@ void sg_default_params(sg_params *p)
     p->npre = 2;
     p->npost = 2;
     p->use_graph = 1;
+    p->fused = 1;
 }
 
 int sg_uniform_create(sg_uniform **ctx, int n, double h, int precisions) { return sg::create(ctx, n, h, precisions); }
@@ -535,11 +760,16 @@ int sg_uniform_time_smoother(sg_uniform *c, int prec, int reps, double *ms_per_s
 {
     SG_REQUIRE(c && ms_per_sweep && reps > 0 && reps <= 100000, "bad argument");
     SG_REQUIRE(c->precisions & prec, "context lacks precision %d", prec);
+    const bool fused = prec == SG_PREC_MIXED && c->n > sg::kCoarseMaxN;
     SG_HIP(hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < reps; ++r) {
         if (prec == SG_PREC_FP64)
             SG_TRY(sg::launch_relax<double>(c->stream, c->d.U[0], c->d.RHO[0], c->n, c->h, nullptr));
-        else
+        else if (fused) {
+            // the mixed path's fine-level sweep: U[0] -> U1[0] and back on alternate repetitions
+            float *a = (r & 1) ? c->U1[0] : c->f.U[0], *b = (r & 1) ? c->f.U[0] : c->U1[0];
+            SG_TRY(sg::launch_sweep(c, a, b, c->f.RHO[0], c->n, c->h, false, false, false, nullptr, nullptr, nullptr));
+        } else
             SG_TRY(sg::launch_relax<float>(c->stream, c->f.U[0], c->f.RHO[0], c->n, c->h, nullptr));
     }
     SG_HIP(hipEventRecord(c->ev1, c->stream));
@@ -547,6 +777,14 @@ int sg_uniform_time_smoother(sg_uniform *c, int prec, int reps, double *ms_per_s
     float ms = 0.f;
     SG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *ms_per_sweep = (double)ms / reps;
+    return SG_OK;
+}
+
+int sg_uniform_tile_info(sg_uniform *c, int level, int *cfg, float *ms, int nms)
+{
+    SG_REQUIRE(c && cfg && level >= 0 && level < c->nlev, "bad argument");
+    *cfg = c->tile_cfg[level];
+    for (int q = 0; ms && q < nms && q < sg::kNumTiles; ++q) ms[q] = c->tune_ms[level][q];
     return SG_OK;
 }
 

@@ -23,6 +23,66 @@ template <> struct Consts<float> {
     static __device__ __forceinline__ float sixth() { return 1.0f / 6.0f; }
 };
 
+// ---- per-cell arithmetic shared by every kernel (plain, fused, single-workgroup) -------------
+// Operand order is the reference's: x+, x-, y+, y-, z+, z- neighbours, then -6u, then the source.
+
+// mg_poisson.F90:60-66: returns the updated value, *resh2 = the local h^2-weighted residual.
+template <typename T>
+__device__ __forceinline__ T relax_value(T uc, T xp, T xm, T yp, T ym, T zp, T zm, T f, T h2, T *resh2)
+{
+    T s = xp + xm;
+    s = s + yp;
+    s = s + ym;
+    s = s + zp;
+    s = s + zm;
+    s = s - T(6) * uc;
+    s = s - h2 * f;
+    *resh2 = s;
+    return uc + Consts<T>::sixth() * s;
+}
+
+// mg_poisson.F90:20-24: res = -h2i * (sum6 - 6u) + rhs.
+template <typename T>
+__device__ __forceinline__ T resid_value(T uc, T xp, T xm, T yp, T ym, T zp, T zm, T f, T h2i)
+{
+    T s = xp + xm;
+    s = s + yp;
+    s = s + ym;
+    s = s + zp;
+    s = s + zm;
+    s = s - T(6) * uc;
+    return -(h2i * s) + f;
+}
+
+// restriction.F90:139-143: (000 + 100 + 010 + 001 + 110 + 101 + 011 + 111) * 1/8.
+template <typename T>
+__device__ __forceinline__ T restrict_value(T a000, T a100, T a010, T a001, T a110, T a101, T a011, T a111)
+{
+    T s = a000 + a100;
+    s = s + a010;
+    s = s + a001;
+    s = s + a110;
+    s = s + a101;
+    s = s + a011;
+    s = s + a111;
+    return s * T(0.125);
+}
+
+// interpolation.F90:236-241: (27 c + 9 (x + y + z) + 3 (yz + xz + xy) + xyz) / 64, where x, y, z
+// are the face neighbours toward the fine cell, etc.
+template <typename T>
+__device__ __forceinline__ T trilinear_value(T c, T x, T y, T z, T yz, T xz, T xy, T xyz)
+{
+    T b = x + y;
+    b = b + z;
+    T e = yz + xz;
+    e = e + xy;
+    T s = T(27) * c + T(9) * b;
+    s = s + T(3) * e;
+    s = s + xyz;
+    return s / T(64);
+}
+
 // mg_poisson.F90:55-74, one colour (ipass 1 = cells with i+j+k odd, 2 = even).  One thread per
 // colour cell.  resmax (nullable) receives max |resh2| by atomic max.
 template <typename T>
@@ -41,49 +101,39 @@ __global__ void __launch_bounds__(256) k_relax_color(T *__restrict__ u, const T 
         if (i < n) {
             const int c = (k * n + j) * n + i;
             const int row = (k * n + j) * n;
-            T s = u[row + wrap_p(i, n)] + u[row + wrap_m(i, n)];
-            s = s + u[(k * n + wrap_p(j, n)) * n + i];
-            s = s + u[(k * n + wrap_m(j, n)) * n + i];
-            s = s + u[(wrap_p(k, n) * n + j) * n + i];
-            s = s + u[(wrap_m(k, n) * n + j) * n + i];
-            const T uc = u[c];
-            s = s - T(6) * uc;
-            s = s - h2 * rhs[c];
-            u[c] = uc + Consts<T>::sixth() * s;
+            T s;
+            u[c] = relax_value<T>(u[c], u[row + wrap_p(i, n)], u[row + wrap_m(i, n)],
+                                  u[(k * n + wrap_p(j, n)) * n + i], u[(k * n + wrap_m(j, n)) * n + i],
+                                  u[(wrap_p(k, n) * n + j) * n + i], u[(wrap_m(k, n) * n + j) * n + i], rhs[c],
+                                  h2, &s);
             amax = fabs((double)s);
         }
     }
-    if (resmax) {
-        amax = wave_max(amax);
-        if ((threadIdx.x % kWave) == 0) atomic_max_nonneg(resmax, amax);
-    }
+    if (resmax) block_atomic_max(resmax, amax);
 }
 
 // mg_poisson.F90:17-27: res = -(h2i * (sum6 - 6u)) + rhs, computed in Tin, stored as Tout.
-// pmax/psum (nullable): per-block partial max|res| and sum res^2 (deterministic two-stage norm).
+// Grid-stride over the cells; pmax/psum (nullable): per-block partial max|res| and sum res^2 for a
+// deterministic two-stage norm (fixed grid, fixed per-thread order).
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin *__restrict__ u,
                                                const Tin *__restrict__ rhs, int n, Tin h2i,
                                                double *pmax, double *psum)
 {
     const long n3 = (long)n * n * n;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double mx = 0.0, sm = 0.0;
-    if (c < n3) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += (long)gridDim.x * blockDim.x) {
         const int i = (int)(c % n);
         const int j = (int)((c / n) % n);
         const int k = (int)(c / ((long)n * n));
         const int row = (k * n + j) * n;
-        Tin s = u[row + wrap_p(i, n)] + u[row + wrap_m(i, n)];
-        s = s + u[(k * n + wrap_p(j, n)) * n + i];
-        s = s + u[(k * n + wrap_m(j, n)) * n + i];
-        s = s + u[(wrap_p(k, n) * n + j) * n + i];
-        s = s + u[(wrap_m(k, n) * n + j) * n + i];
-        s = s - Tin(6) * u[c];
-        const Tin r = -(h2i * s) + rhs[c];
+        const Tin r = resid_value<Tin>(u[c], u[row + wrap_p(i, n)], u[row + wrap_m(i, n)],
+                                       u[(k * n + wrap_p(j, n)) * n + i], u[(k * n + wrap_m(j, n)) * n + i],
+                                       u[(wrap_p(k, n) * n + j) * n + i], u[(wrap_m(k, n) * n + j) * n + i],
+                                       rhs[c], h2i);
         res[c] = (Tout)r;
-        mx = fabs((double)r);
-        sm = (double)r * (double)r;
+        mx = fmax(mx, fabs((double)r));
+        sm += (double)r * (double)r;
     }
     if (pmax) {
         block_max_sum(mx, sm);
@@ -112,14 +162,8 @@ __global__ void __launch_bounds__(256) k_restrict(T *__restrict__ uc, const T *_
     const int nf = 2 * nc;
     const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
     auto F = [&](int a, int b, int d) { return uf[((k + d) * nf + (j + b)) * nf + (i + a)]; };
-    T s = F(0, 0, 0) + F(1, 0, 0);
-    s = s + F(0, 1, 0);
-    s = s + F(0, 0, 1);
-    s = s + F(1, 1, 0);
-    s = s + F(1, 0, 1);
-    s = s + F(0, 1, 1);
-    s = s + F(1, 1, 1);
-    uc[c] = s * T(0.125);
+    uc[c] = restrict_value<T>(F(0, 0, 0), F(1, 0, 0), F(0, 1, 0), F(0, 0, 1), F(1, 1, 0), F(1, 0, 1), F(0, 1, 1),
+                              F(1, 1, 1));
 }
 
 // interpolation.F90:226-245 trilinear prolongation of a periodic coarse grid; ADD selects
@@ -137,14 +181,8 @@ __global__ void __launch_bounds__(256) k_interp_linear(T *__restrict__ uf, const
     const int jc1 = wrap_g(jc + 2 * (jf & 1) - 1, nc);
     const int kc1 = wrap_g(kc + 2 * (kf & 1) - 1, nc);
     auto C = [&](int a, int b, int d) { return uc[(d * nc + b) * nc + a]; };
-    T b = C(ic1, jc, kc) + C(ic, jc1, kc);
-    b = b + C(ic, jc, kc1);
-    T e = C(ic, jc1, kc1) + C(ic1, jc, kc1);
-    e = e + C(ic1, jc1, kc);
-    T s = T(27) * C(ic, jc, kc) + T(9) * b;
-    s = s + T(3) * e;
-    s = s + C(ic1, jc1, kc1);
-    const T v = s / T(64);
+    const T v = trilinear_value<T>(C(ic, jc, kc), C(ic1, jc, kc), C(ic, jc1, kc), C(ic, jc, kc1), C(ic, jc1, kc1),
+                                   C(ic1, jc, kc1), C(ic1, jc1, kc), C(ic1, jc1, kc1));
     if (ADD) uf[c] = uf[c] + v;
     else uf[c] = v;
 }

@@ -48,13 +48,13 @@ class UniformSolver:
             raise ValueError(f"expected shape {shape}, got {tuple(t.shape)}")
 
     def cycle(self, psi, src, tol=1e-10, maxit=50, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_MIXED,
-              ncycle=2, npre=2, npost=2, use_graph=True, raise_on_fail=True):
+              ncycle=2, npre=2, npost=2, use_graph=True, fused=True, raise_on_fail=True):
         """mg_cycle(ju, jrho) (mg.F90:42-94): psi updated in place; returns stats dict."""
         n = self.n
         self._check(psi, shape=(n, n, n))
         self._check(src, shape=(n, n, n))
         p = _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=precision, ncycle=ncycle,
-                                npre=npre, npost=npost, use_graph=int(bool(use_graph)))
+                                npre=npre, npost=npost, use_graph=int(bool(use_graph)), fused=int(bool(fused)))
         st = _lib.SgStats()
         torch.cuda.synchronize()
         rc = self._lib.sg_uniform_cycle_dev(self._ctx, _ptr(psi), _ptr(src), ctypes.byref(p), ctypes.byref(st))

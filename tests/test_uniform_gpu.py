@@ -185,3 +185,40 @@ def test_full_size_256_mixed_vs_plain(gpu):
     e = (out[SG_PREC_FP64] - ref).norm() / ref.norm()
     assert e.item() < 1e-9
     s.close()
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_fused_mixed_equals_unfused_bitwise(gpu, n, use_graph):
+    """The fused FP32 kernels (LDS tiles, single-workgroup coarse solve) reproduce the
+    one-kernel-per-loop FP32 inner solve bit for bit, including Resmaxg."""
+    f = problems.clumpy_rhs(n, seed=11)
+    s = UniformSolver(n, precisions=SG_PREC_MIXED)
+    ft = T(f, gpu)
+    out = []
+    for fused in (False, True):
+        psi = torch.zeros_like(ft)
+        st = s.cycle(psi, ft, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, fused=fused, use_graph=use_graph)
+        out.append((psi, st))
+    (p0, s0), (p1, s1) = out
+    assert s0["iterations"] == s1["iterations"]
+    assert s0["resmaxg"] == s1["resmaxg"]
+    assert torch.equal(p0, p1)
+    s.close()
+
+
+@pytest.mark.parametrize("npre,npost,ncycle", [(1, 1, 1), (3, 2, 2), (2, 3, 1)])
+def test_fused_other_cycle_shapes_bitwise(gpu, npre, npost, ncycle):
+    n = 64
+    f = problems.clumpy_rhs(n, seed=12)
+    s = UniformSolver(n, precisions=SG_PREC_MIXED)
+    ft = T(f, gpu)
+    res = []
+    for fused in (False, True):
+        psi = torch.zeros_like(ft)
+        st = s.cycle(psi, ft, tol=1e-10, maxit=40, precision=SG_PREC_MIXED, fused=fused, npre=npre, npost=npost,
+                     ncycle=ncycle, stop_mode=SG_STOP_REFERENCE, raise_on_fail=False)
+        res.append((psi, st))
+    assert torch.equal(res[0][0], res[1][0])
+    assert res[0][1]["resmaxg"] == res[1][1]["resmaxg"]
+    s.close()
