@@ -105,15 +105,20 @@ def main():
     log(f"[bench] smoother sweep {ms_sweep.value * 1e3:.1f} us -> {achieved:.0f} GB/s algorithmic")
 
     tiles = {}
-    dims = [(32, 16, 16, 512), (32, 16, 16, 1024), (32, 8, 8, 256), (16, 8, 8, 256), (64, 8, 8, 512), (32, 8, 16, 512)]
+    dims = [(32, 16, 16, 1024), (16, 8, 8, 1024), (16, 8, 8, 256), (64, 8, 8, 512), (8, 8, 8, 512)]
     lev, m = 0, n
     while m > 16:
-        cfg = ctypes.c_int()
-        ms = (ctypes.c_float * 8)()
-        _lib.check(solver._lib.sg_uniform_tile_info(solver._ctx, lev, ctypes.byref(cfg), ms, 8))
+        cfg, ns, fs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_float * 20)()
+        _lib.check(solver._lib.sg_uniform_tile_info(solver._ctx, lev, ctypes.byref(cfg), ctypes.byref(ns),
+                                                    ctypes.byref(fs), ms, 20))
+        cand = {}
+        for q in range(4 * len(dims)):
+            d, k, st = dims[q // 4], 1 + (q // 2) % 2, q % 2
+            if ms[q] > 0:
+                cand["x".join(map(str, d[:3])) + f"/{d[3]}/{k}sw" + ("/fLDS" if st else "")] = round(ms[q] * 1e3, 1)
         tiles[str(m)] = {"tile": "x".join(map(str, dims[cfg.value][:3])), "threads": dims[cfg.value][3],
-                         "us_per_sweep": {"x".join(map(str, d[:3])) + "/" + str(d[3]): round(ms[q] * 1e3, 1)
-                                          for q, d in enumerate(dims) if ms[q] > 0}}
+                         "sweeps_per_launch": ns.value, "source_in_lds": bool(fs.value), "us_per_2_sweeps": cand}
         lev += 1
         m //= 2
     log(f"[bench] autotuned tiles {json.dumps(tiles)}")

@@ -68,7 +68,8 @@ SIGNATURES = {
     "sg_uniform_restrict_dev": (_I, [_VP, _I, _VP, _VP, _I]),
     "sg_uniform_interp_dev": (_I, [_VP, _I, _VP, _VP, _I, _I]),
     "sg_uniform_time_smoother": (_I, [_VP, _I, _I, _DP]),
-    "sg_uniform_tile_info": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), _I]),
+    "sg_uniform_tile_info": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                  ctypes.POINTER(ctypes.c_float), _I]),
 }
 
 _LIB = None

@@ -98,9 +98,9 @@ template <int TX_, int TY_, int TZ_, int NT_> struct Tile {
 // One colour of the sweep over the LDS region shrunk by margin M: cells with
 // (gx + gy + gz) % 2 == colour, where colour 1 is the reference's first pass (ipass = 1).  The
 // source values are fetched for all of a thread's cells before any update.
-template <class TL, int LX, int LY, int LZ, int M, int H>
-__device__ __forceinline__ void tile_colour(float *su, const float *__restrict__ f, int n, int x0, int y0, int z0,
-                                            int colour, float h2, double &amax)
+template <class TL, int LX, int LY, int LZ, int M, int H, bool FS>
+__device__ __forceinline__ void tile_colour(float *su, const float *sf, const float *__restrict__ f, int n, int x0,
+                                            int y0, int z0, int colour, float h2, double &amax)
 {
     constexpr int RX = (LX - 2 * M) / 2, RY = LY - 2 * M, RZ = LZ - 2 * M;
     constexpr int CNT = RX * RY * RZ;
@@ -123,8 +123,12 @@ __device__ __forceinline__ void tile_colour(float *su, const float *__restrict__
             const int x = M + 2 * xh + d;
             cv[it] = (z * LY + y) * LX + x;
             if (x >= H && x < H + TL::TX && y >= H && y < H + TL::TY && z >= H && z < H + TL::TZ) core |= 1u << it;
-            const int gi = wrapn(x0 + x, n), gj = wrapn(y0 + y, n), gk = wrapn(z0 + z, n);
-            fv[it] = f[(gk * n + gj) * n + gi];
+            if (FS) {
+                fv[it] = sf[cv[it]];
+            } else {
+                const int gi = wrapn(x0 + x, n), gj = wrapn(y0 + y, n), gk = wrapn(z0 + z, n);
+                fv[it] = f[(gk * n + gj) * n + gi];
+            }
         }
     }
 #pragma unroll
@@ -139,17 +143,55 @@ __device__ __forceinline__ void tile_colour(float *su, const float *__restrict__
     }
 }
 
-template <class TL, int H, bool ZERO, bool PROLONG, bool RESID>
+// Sweeps S .. NS-1 of a multi-sweep tile: sweep s updates the first colour on margin 2s+1 and the
+// second on margin 2s+2.  Only the last sweep feeds the max|resh2| (one mg_relax call's Resmaxg).
+template <class TL, int LX, int LY, int LZ, int H, int S, int NS, bool FS> struct TileSweeps {
+    static __device__ __forceinline__ void run(float *su, const float *sf, const float *__restrict__ f, int n, int x0,
+                                               int y0, int z0, float h2, double &amax)
+    {
+        double dummy = 0.0;
+        double &acc = (S == NS - 1) ? amax : dummy;
+        tile_colour<TL, LX, LY, LZ, 2 * S + 1, H, FS>(su, sf, f, n, x0, y0, z0, 1, h2, acc);
+        __syncthreads();
+        tile_colour<TL, LX, LY, LZ, 2 * S + 2, H, FS>(su, sf, f, n, x0, y0, z0, 0, h2, acc);
+        __syncthreads();
+        TileSweeps<TL, LX, LY, LZ, H, S + 1, NS, FS>::run(su, sf, f, n, x0, y0, z0, h2, amax);
+    }
+};
+template <class TL, int LX, int LY, int LZ, int H, int NS, bool FS> struct TileSweeps<TL, LX, LY, LZ, H, NS, NS, FS> {
+    static __device__ __forceinline__ void run(float *, const float *, const float *__restrict__, int, int, int, int,
+                                               float, double &)
+    {
+    }
+};
+
+// LDS bytes of the largest variant of a (tile, sweeps, staging) choice: the residual variant has
+// the widest halo, the prolongation variant adds the staged coarse tile.
+template <class TL, int NS, bool FS> constexpr int sweep_lds_bytes()
+{
+    constexpr int H1 = 2 * NS + 1, H2 = 2 * NS;
+    constexpr int r1 = (TL::TX + 2 * H1) * (TL::TY + 2 * H1) * (TL::TZ + 2 * H1) * (FS ? 2 : 1);
+    constexpr int r2 = (TL::TX + 2 * H2) * (TL::TY + 2 * H2) * (TL::TZ + 2 * H2) * (FS ? 2 : 1) +
+                       ((TL::TX + 2 * H2) / 2 + 2) * ((TL::TY + 2 * H2) / 2 + 2) * ((TL::TZ + 2 * H2) / 2 + 2);
+    return 4 * (r1 > r2 ? r1 : r2) + 1024;
+}
+template <class TL, int NS, bool FS> constexpr bool sweep_fits_lds() { return sweep_lds_bytes<TL, NS, FS>() <= 163840; }
+
+// NS full sweeps (temporal blocking: halo 2 per sweep, +1 for the residual).  FS stages the source
+// on the same LDS region as u, loaded together with u, so the colour passes touch LDS only.
+template <class TL, int NS, bool ZERO, bool PROLONG, bool RESID, bool FS>
 __global__ void __launch_bounds__(TL::NT)
 k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const float *__restrict__ f, int n, float h2,
              float h2i, const float *__restrict__ uc, float *__restrict__ rc, double *resmax)
 {
+    constexpr int H = 2 * NS + (RESID ? 1 : 0);
     constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, NT = TL::NT;
     constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = TZ + 2 * H;
     constexpr int CX = LX / 2 + 2, CY = LY / 2 + 2, CZ = LZ / 2 + 2;  // staged coarse region
     static_assert(!RESID || H >= 3, "residual needs a 3-cell halo");
     static_assert(H % 2 == 0 || !PROLONG, "prolongation staging assumes an even tile origin");
     __shared__ float su[LX * LY * LZ];
+    __shared__ float sf[FS ? LX * LY * LZ : 1];
     __shared__ float sc[PROLONG ? CX * CY * CZ : 1];
     const int ntx = n / TX, nty = n / TY, ntz = n / TZ;
     const int ntiles = ntx * nty * ntz;
@@ -159,65 +201,73 @@ k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const floa
     const int x0 = tx * TX - H, y0 = ty * TY - H, z0 = tz * TZ - H;
     const int nc = n >> 1;
 
-    if (PROLONG) {
-        // coarse cells x0/2 - 1 .. x0/2 + LX/2 (x0 even), periodic images
-        constexpr int CNT = CX * CY * CZ, IT = (CNT + NT - 1) / NT;
-        const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1, cz0 = z0 / 2 - 1;
-        float v[IT];
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int q = threadIdx.x + it * NT;
-            if (CNT % NT == 0 || q < CNT) {
-                const int x = q % CX, y = (q / CX) % CY, z = q / (CX * CY);
-                v[it] = uc[(wrapn(cz0 + z, nc) * nc + wrapn(cy0 + y, nc)) * nc + wrapn(cx0 + x, nc)];
-            }
-        }
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int q = threadIdx.x + it * NT;
-            if (CNT % NT == 0 || q < CNT) sc[q] = v[it];
-        }
-        __syncthreads();
-    }
     {
-        // load u (+ prolongated coarse correction) on tile + halo
+        // one batch of global loads: u and (FS) the source on tile + halo, (PROLONG) the coarse
+        // cells x0/2 - 1 .. x0/2 + LX/2 (x0 even) that the trilinear correction reads
         constexpr int CNT = LX * LY * LZ, IT = (CNT + NT - 1) / NT;
+        constexpr int CNTC = CX * CY * CZ, ITC = PROLONG ? (CNTC + NT - 1) / NT : 1;
         float v[IT];
-        if (!ZERO) {
+        float fvl[FS ? IT : 1];
+        float cvl[ITC];
+        if (!ZERO || FS) {
 #pragma unroll
             for (int it = 0; it < IT; ++it) {
                 const int q = threadIdx.x + it * NT;
                 if (CNT % NT == 0 || q < CNT) {
                     const int x = q % LX, y = (q / LX) % LY, z = q / (LX * LY);
-                    v[it] = uin[(wrapn(z0 + z, n) * n + wrapn(y0 + y, n)) * n + wrapn(x0 + x, n)];
+                    const int g = (wrapn(z0 + z, n) * n + wrapn(y0 + y, n)) * n + wrapn(x0 + x, n);
+                    if (!ZERO) v[it] = uin[g];
+                    if (FS) fvl[FS ? it : 0] = f[g];
                 }
+            }
+        }
+        if (PROLONG) {
+            const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 1, cz0 = z0 / 2 - 1;
+#pragma unroll
+            for (int it = 0; it < ITC; ++it) {
+                const int q = threadIdx.x + it * NT;
+                if (CNTC % NT == 0 || q < CNTC) {
+                    const int x = q % CX, y = (q / CX) % CY, z = q / (CX * CY);
+                    cvl[it] = uc[(wrapn(cz0 + z, nc) * nc + wrapn(cy0 + y, nc)) * nc + wrapn(cx0 + x, nc)];
+                }
+            }
+#pragma unroll
+            for (int it = 0; it < ITC; ++it) {
+                const int q = threadIdx.x + it * NT;
+                if (CNTC % NT == 0 || q < CNTC) sc[q] = cvl[it];
             }
         }
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const int q = threadIdx.x + it * NT;
             if (CNT % NT == 0 || q < CNT) {
-                float val = ZERO ? 0.0f : v[it];
-                if (PROLONG) {
-                    const int x = q % LX, y = (q / LX) % LY, z = q / (LX * LY);
-                    // local coarse index of the parent: floor(x/2) + 1; neighbour toward the cell
-                    const int ic = (x >> 1) + 1, jc = (y >> 1) + 1, kc = (z >> 1) + 1;
-                    const int ic1 = ic + 2 * (x & 1) - 1, jc1 = jc + 2 * (y & 1) - 1, kc1 = kc + 2 * (z & 1) - 1;
-                    auto C = [&](int a, int b, int d) { return sc[(d * CY + b) * CX + a]; };
-                    val = val + trilinear_value<float>(C(ic, jc, kc), C(ic1, jc, kc), C(ic, jc1, kc), C(ic, jc, kc1),
-                                                       C(ic, jc1, kc1), C(ic1, jc, kc1), C(ic1, jc1, kc),
-                                                       C(ic1, jc1, kc1));
-                }
-                su[q] = val;
+                su[q] = ZERO ? 0.0f : v[it];
+                if (FS) sf[q] = fvl[FS ? it : 0];
             }
         }
     }
     __syncthreads();
+    if (PROLONG) {
+        // u += trilinear(coarse) on tile + halo (mg_addint, mg.F90:727-731)
+        constexpr int CNT = LX * LY * LZ, IT = (CNT + NT - 1) / NT;
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int q = threadIdx.x + it * NT;
+            if (CNT % NT == 0 || q < CNT) {
+                const int x = q % LX, y = (q / LX) % LY, z = q / (LX * LY);
+                // local coarse index of the parent: floor(x/2) + 1; neighbour toward the cell
+                const int ic = (x >> 1) + 1, jc = (y >> 1) + 1, kc = (z >> 1) + 1;
+                const int ic1 = ic + 2 * (x & 1) - 1, jc1 = jc + 2 * (y & 1) - 1, kc1 = kc + 2 * (z & 1) - 1;
+                auto C = [&](int a, int b, int d) { return sc[(d * CY + b) * CX + a]; };
+                su[q] = su[q] + trilinear_value<float>(C(ic, jc, kc), C(ic1, jc, kc), C(ic, jc1, kc), C(ic, jc, kc1),
+                                                       C(ic, jc1, kc1), C(ic1, jc, kc1), C(ic1, jc1, kc),
+                                                       C(ic1, jc1, kc1));
+            }
+        }
+        __syncthreads();
+    }
     double amax = 0.0;
-    tile_colour<TL, LX, LY, LZ, 1, H>(su, f, n, x0, y0, z0, 1, h2, amax);
-    __syncthreads();
-    tile_colour<TL, LX, LY, LZ, 2, H>(su, f, n, x0, y0, z0, 0, h2, amax);
-    __syncthreads();
+    TileSweeps<TL, LX, LY, LZ, H, 0, NS, FS>::run(su, sf, f, n, x0, y0, z0, h2, amax);
     {
         constexpr int CNT = TX * TY * TZ, IT = (CNT + NT - 1) / NT;
 #pragma unroll
@@ -243,8 +293,12 @@ k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const floa
 #pragma unroll
                 for (int o = 0; o < 8; ++o) {
                     const int a = o & 1, b = (o >> 1) & 1, d = (o >> 2) & 1;
-                    const int gi = x0 + H + 2 * cx + a, gj = y0 + H + 2 * cy + b, gk = z0 + H + 2 * cz + d;
-                    fv[it][o] = f[(gk * n + gj) * n + gi];
+                    if (FS) {
+                        fv[it][o] = sf[((H + 2 * cz + d) * LY + (H + 2 * cy + b)) * LX + (H + 2 * cx + a)];
+                    } else {
+                        const int gi = x0 + H + 2 * cx + a, gj = y0 + H + 2 * cy + b, gk = z0 + H + 2 * cz + d;
+                        fv[it][o] = f[(gk * n + gj) * n + gi];
+                    }
                 }
             }
         }

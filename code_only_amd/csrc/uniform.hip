@@ -48,6 +48,12 @@ template <typename T> struct Levels {
 
 static inline unsigned nblocks(long work) { return (unsigned)((work + kBlock - 1) / kBlock); }
 static long n3(int n) { return (long)n * n * n; }
+static int ilog2(int n)
+{
+    int lg = 0;
+    while ((1 << lg) < n) ++lg;
+    return lg;
+}
 
 } // namespace sg
 
@@ -62,19 +68,26 @@ struct sg_uniform {
     Levels<float> f;
     float *U1[kMaxLev] = {};   // second FP32 U buffer per level: the fused sweeps ping-pong
     int tile_cfg[kMaxLev] = {}; // autotuned sweep tile shape per fine level
-    float tune_ms[kMaxLev][8] = {};
+    int tile_ns[kMaxLev] = {};  // autotuned sweeps per launch per fine level
+    int tile_fs[kMaxLev] = {};  // autotuned: source staged in LDS
+    float tune_ms[kMaxLev][20] = {};
     double *psi_dev = nullptr, *src_dev = nullptr;   // staging for the host-pointer entry point
+    double *psi_alt = nullptr;                         // second phi buffer of the fused outer step
     double *pmax = nullptr, *psum = nullptr;          // norm partials
     int npart = 0;
     double *dnorm = nullptr;   // [0] max|r| [1] sum r^2 [2] resmaxg*h^2 [3] max|f| [4] sum f^2
     double *hnorm = nullptr;   // pinned host mirror of dnorm
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // graph of one mixed-precision outer iteration, keyed on the arguments it was captured with
-    hipGraphExec_t gexec = nullptr;
-    double *g_psi = nullptr;
-    const double *g_src = nullptr;
-    int g_ncycle = -1, g_npre = -1, g_npost = -1, g_fused = -1, g_stop = -1;
+    // graphs of one mixed-precision outer iteration (one per phi buffer parity), keyed on the
+    // arguments they were captured with
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        const double *pin = nullptr;
+        double *pout = nullptr;
+        const double *src = nullptr;
+        int ncycle = -1, npre = -1, npost = -1, fused = -1, stop = -1;
+    } graph[2];
 };
 
 namespace sg {
@@ -96,7 +109,7 @@ static int launch_relax(hipStream_t s, T *u, const T *rhs, int n, double h, doub
     const long work = (long)((n + 1) / 2) * n * n;
     const T h2 = (T)(h * h);
     for (int ipass = 1; ipass <= 2; ++ipass) {
-        hipLaunchKernelGGL(k_relax_color<T>, dim3(nblocks(work)), dim3(kBlock), 0, s, u, rhs, n, h2, ipass,
+        hipLaunchKernelGGL(k_relax_color<T>, dim3(nblocks(work)), dim3(kBlock), 0, s, u, rhs, ilog2(n), h2, ipass,
                            resmax);
         SG_CHECK_LAUNCH();
     }
@@ -114,7 +127,7 @@ static int launch_resid(sg_uniform *c, Tout *res, const Tin *u, const Tin *rhs, 
         set_error("norm partial buffer too small (%u > %d)", nb, c->npart);
         return SG_ERR_ARG;
     }
-    hipLaunchKernelGGL((k_resid<Tin, Tout>), dim3(nb), dim3(kBlock), 0, c->stream, res, u, rhs, n, h2i,
+    hipLaunchKernelGGL((k_resid<Tin, Tout>), dim3(nb), dim3(kBlock), 0, c->stream, res, u, rhs, ilog2(n), h2i,
                        norms ? c->pmax : nullptr, norms ? c->psum : nullptr);
     SG_CHECK_LAUNCH();
     if (norms) {
@@ -150,7 +163,7 @@ static int launch_norm(sg_uniform *c, const double *a, long len, double *out)
 
 template <typename T> static int launch_restrict(hipStream_t s, T *uc, const T *uf, int nc)
 {
-    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(n3(nc))), dim3(kBlock), 0, s, uc, uf, nc);
+    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(n3(nc))), dim3(kBlock), 0, s, uc, uf, ilog2(nc));
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
@@ -163,11 +176,11 @@ template <typename T> static int launch_interp(hipStream_t s, T *uf, const T *uc
     const int nc = nf / 2;
     const unsigned nb = nblocks(n3(nf));
     if (cubic && nc + 2 >= 4 && !add) {
-        hipLaunchKernelGGL(k_interp_cubic<T>, dim3(nb), dim3(kBlock), 0, s, uf, uc, nf);
+        hipLaunchKernelGGL(k_interp_cubic<T>, dim3(nb), dim3(kBlock), 0, s, uf, uc, ilog2(nf));
     } else if (add) {
-        hipLaunchKernelGGL((k_interp_linear<T, true>), dim3(nb), dim3(kBlock), 0, s, uf, uc, nf);
+        hipLaunchKernelGGL((k_interp_linear<T, true>), dim3(nb), dim3(kBlock), 0, s, uf, uc, ilog2(nf));
     } else {
-        hipLaunchKernelGGL((k_interp_linear<T, false>), dim3(nb), dim3(kBlock), 0, s, uf, uc, nf);
+        hipLaunchKernelGGL((k_interp_linear<T, false>), dim3(nb), dim3(kBlock), 0, s, uf, uc, ilog2(nf));
     }
     SG_CHECK_LAUNCH();
     return SG_OK;
@@ -211,16 +224,17 @@ template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int
     return SG_OK;
 }
 
-// Tile shapes the autotuner chooses from (all give identical values).
-using Tile0 = Tile<32, 16, 16, 512>;
-using Tile1 = Tile<32, 16, 16, 1024>;
-using Tile2 = Tile<32, 8, 8, 256>;
-using Tile3 = Tile<16, 8, 8, 256>;
-using Tile4 = Tile<64, 8, 8, 512>;
-using Tile5 = Tile<32, 8, 16, 512>;
-constexpr int kNumTiles = 6;
+// Tile shapes the autotuner chooses from (all give identical values), each with 1 or 2 sweeps
+// per launch.
+using Tile0 = Tile<32, 16, 16, 1024>;
+using Tile1 = Tile<16, 8, 8, 1024>;
+using Tile2 = Tile<16, 8, 8, 256>;
+using Tile3 = Tile<64, 8, 8, 512>;
+using Tile4 = Tile<8, 8, 8, 512>;
+constexpr int kNumTiles = 5;
+constexpr int kMaxSweepsPerLaunch = 2;
 static const int kTileDims[kNumTiles][4] = {
-    {32, 16, 16, 512}, {32, 16, 16, 1024}, {32, 8, 8, 256}, {16, 8, 8, 256}, {64, 8, 8, 512}, {32, 8, 16, 512}};
+    {32, 16, 16, 1024}, {16, 8, 8, 1024}, {16, 8, 8, 256}, {64, 8, 8, 512}, {8, 8, 8, 512}};
 
 static bool tile_fits(int cfg, int n)
 {
@@ -228,37 +242,76 @@ static bool tile_fits(int cfg, int n)
     return n >= d[0] && n >= d[1] && n >= d[2] && n % d[0] == 0 && n % d[1] == 0 && n % d[2] == 0;
 }
 
-template <class TL>
+template <class TL, int NS, bool FS>
 static void sweep_variant(hipStream_t s, const float *uin, float *uout, const float *f, int n, float h2, float h2i,
                           bool zero, bool prolong, bool resid, const float *uc, float *rc, double *resmax)
 {
     const int ntiles = (n / TL::TX) * (n / TL::TY) * (n / TL::TZ);
     const dim3 grid(8 * ((ntiles + 7) / 8)), block(TL::NT);
     if (resid && zero)
-        hipLaunchKernelGGL((k_sweep_tile<TL, 3, true, false, true>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+        hipLaunchKernelGGL((k_sweep_tile<TL, NS, true, false, true, FS>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
     else if (resid)
-        hipLaunchKernelGGL((k_sweep_tile<TL, 3, false, false, true>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+        hipLaunchKernelGGL((k_sweep_tile<TL, NS, false, false, true, FS>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
     else if (prolong)
-        hipLaunchKernelGGL((k_sweep_tile<TL, 2, false, true, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+        hipLaunchKernelGGL((k_sweep_tile<TL, NS, false, true, false, FS>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
     else if (zero)
-        hipLaunchKernelGGL((k_sweep_tile<TL, 2, true, false, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+        hipLaunchKernelGGL((k_sweep_tile<TL, NS, true, false, false, FS>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
     else
-        hipLaunchKernelGGL((k_sweep_tile<TL, 2, false, false, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+        hipLaunchKernelGGL((k_sweep_tile<TL, NS, false, false, false, FS>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
 }
 
-// One fused red-black sweep of a fine level (n >= 32) with tile shape cfg: uin -> uout.
-static int launch_sweep_cfg(hipStream_t s, int cfg, const float *uin, float *uout, const float *f, int n, double h,
-                            bool zero, bool prolong, bool resid, const float *uc, float *rc, double *resmax)
+template <class TL>
+static void sweep_ns(hipStream_t s, int ns, bool fs, const float *uin, float *uout, const float *f, int n, float h2,
+                     float h2i, bool zero, bool prolong, bool resid, const float *uc, float *rc, double *resmax)
+{
+    if (ns == 2 && fs) {
+        if constexpr (sweep_fits_lds<TL, 2, true>())
+            sweep_variant<TL, 2, true>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax);
+    } else if (ns == 2) {
+        if constexpr (sweep_fits_lds<TL, 2, false>())
+            sweep_variant<TL, 2, false>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax);
+    } else if (fs) {
+        if constexpr (sweep_fits_lds<TL, 1, true>())
+            sweep_variant<TL, 1, true>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax);
+    } else {
+        sweep_variant<TL, 1, false>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax);
+    }
+}
+
+template <class TL> static bool fits_lds(int ns, bool fs)
+{
+    if (ns == 2) return fs ? sweep_fits_lds<TL, 2, true>() : sweep_fits_lds<TL, 2, false>();
+    return fs ? sweep_fits_lds<TL, 1, true>() : sweep_fits_lds<TL, 1, false>();
+}
+
+// whether (tile shape, sweeps per launch, staging) is instantiated (fits the 160 KB of LDS)
+static bool cand_ok(int cfg, int ns, bool fs)
+{
+    switch (cfg) {
+    case 0: return fits_lds<Tile0>(ns, fs);
+    case 1: return fits_lds<Tile1>(ns, fs);
+    case 2: return fits_lds<Tile2>(ns, fs);
+    case 3: return fits_lds<Tile3>(ns, fs);
+    default: return fits_lds<Tile4>(ns, fs);
+    }
+}
+
+// ns (1 or 2) fused red-black sweeps of a fine level (n >= 32) with tile shape cfg, the source
+// staged in LDS when fs: uin -> uout.
+static int launch_sweep_cfg(hipStream_t s, int cfg, int ns, bool fs, const float *uin, float *uout, const float *f,
+                            int n, double h, bool zero, bool prolong, bool resid, const float *uc, float *rc,
+                            double *resmax)
 {
     SG_REQUIRE(cfg >= 0 && cfg < kNumTiles && tile_fits(cfg, n), "tile shape %d does not fit n = %d", cfg, n);
+    SG_REQUIRE(ns >= 1 && ns <= kMaxSweepsPerLaunch, "bad sweeps per launch %d", ns);
+    SG_REQUIRE(cand_ok(cfg, ns, fs), "tile shape %d with %d sweeps (staged %d) exceeds LDS", cfg, ns, (int)fs);
     const float h2 = (float)(h * h), h2i = (float)(1.0 / (h * h));
     switch (cfg) {
-    case 0: sweep_variant<Tile0>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
-    case 1: sweep_variant<Tile1>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
-    case 2: sweep_variant<Tile2>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
-    case 3: sweep_variant<Tile3>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
-    case 4: sweep_variant<Tile4>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
-    default: sweep_variant<Tile5>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 0: sweep_ns<Tile0>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 1: sweep_ns<Tile1>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 2: sweep_ns<Tile2>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case 3: sweep_ns<Tile3>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    default: sweep_ns<Tile4>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     }
     SG_CHECK_LAUNCH();
     return SG_OK;
@@ -271,12 +324,12 @@ static int level_of(const sg_uniform *c, int n)
     return -1;
 }
 
-static int launch_sweep(sg_uniform *c, const float *uin, float *uout, const float *f, int n, double h, bool zero,
-                        bool prolong, bool resid, const float *uc, float *rc, double *resmax)
+// One launch of ns sweeps on level l with its autotuned tile shape.
+static int launch_sweep(sg_uniform *c, int l, int ns, const float *uin, float *uout, bool zero, bool prolong,
+                        bool resid, const float *uc, float *rc, double *resmax)
 {
-    const int l = level_of(c, n);
-    SG_REQUIRE(l >= 0, "no level of size %d", n);
-    return launch_sweep_cfg(c->stream, c->tile_cfg[l], uin, uout, f, n, h, zero, prolong, resid, uc, rc, resmax);
+    return launch_sweep_cfg(c->stream, c->tile_cfg[l], ns, c->tile_fs[l] != 0, uin, uout, c->f.RHO[l], c->nl[l],
+                            c->hl[l], zero, prolong, resid, uc, rc, resmax);
 }
 
 constexpr int kCubicFX = 32, kCubicFY = 8, kCubicFZ = 8, kCubicNT = 256;
@@ -293,8 +346,9 @@ static int launch_interp_cubic_tile(hipStream_t s, float *uf, const float *uc, i
     return SG_OK;
 }
 
-// Autotuner: for every fine level, time each fitting tile shape on the level's own arrays and
-// keep the fastest.  Runs once per context; values do not depend on the choice.
+// Autotuner: for every fine level, time each fitting (tile shape, sweeps per launch) on the
+// level's own arrays as ms per two sweeps, and keep the fastest.  Runs once per context; values do
+// not depend on the choice.
 static int autotune_tiles(sg_uniform *c)
 {
     hipEvent_t a = nullptr, b = nullptr;
@@ -304,30 +358,40 @@ static int autotune_tiles(sg_uniform *c)
     for (int l = 0; l < c->nlev && rc == SG_OK; ++l) {
         const int n = c->nl[l];
         c->tile_cfg[l] = -1;
+        c->tile_ns[l] = 1;
         if (n <= kCoarseMaxN) continue;
         float best = 1e30f;
-        for (int cfg = 0; cfg < kNumTiles && rc == SG_OK; ++cfg) {
-            if (!tile_fits(cfg, n)) continue;
-            const int reps = n >= 256 ? 4 : 8;
-            for (int r = 0; r < reps + 1 && rc == SG_OK; ++r) {
-                if (r == 1) {
-                    if (hipEventRecord(a, c->stream) != hipSuccess) rc = SG_ERR_HIP;
+        for (int cand = 0; cand < kNumTiles * kMaxSweepsPerLaunch * 2 && rc == SG_OK; ++cand) {
+            const int cfg = cand / (2 * kMaxSweepsPerLaunch), ns = 1 + (cand / 2) % kMaxSweepsPerLaunch;
+            const bool fs = cand % 2;
+            if (!tile_fits(cfg, n) || !cand_ok(cfg, ns, fs)) continue;
+            {
+                const int launches = 2 / ns;             // two sweeps
+                const int reps = n >= 256 ? 3 : 6;       // timed repetitions of two sweeps
+                int flip = 0;
+                for (int r = 0; r < (reps + 1) * launches && rc == SG_OK; ++r) {
+                    if (r == launches && hipEventRecord(a, c->stream) != hipSuccess) rc = SG_ERR_HIP;
+                    if (rc == SG_OK)
+                        rc = launch_sweep_cfg(c->stream, cfg, ns, fs, flip ? c->U1[l] : c->f.U[l],
+                                              flip ? c->f.U[l] : c->U1[l], c->f.RHO[l], n, c->hl[l], false, false,
+                                              false, nullptr, nullptr, nullptr);
+                    flip ^= 1;
                 }
-                if (rc == SG_OK)
-                    rc = launch_sweep_cfg(c->stream, cfg, (r & 1) ? c->U1[l] : c->f.U[l], (r & 1) ? c->f.U[l] : c->U1[l],
-                                          c->f.RHO[l], n, c->hl[l], false, false, false, nullptr, nullptr, nullptr);
-            }
-            if (rc != SG_OK) break;
-            float ms = 0.f;
-            if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-                hipEventElapsedTime(&ms, a, b) != hipSuccess) {
-                rc = SG_ERR_HIP;
-                break;
-            }
-            c->tune_ms[l][cfg] = ms / reps;
-            if (ms < best) {
-                best = ms;
-                c->tile_cfg[l] = cfg;
+                if (rc != SG_OK) break;
+                float ms = 0.f;
+                if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                    hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                    rc = SG_ERR_HIP;
+                    break;
+                }
+                ms /= reps;
+                c->tune_ms[l][cand] = ms;
+                if (ms < best) {
+                    best = ms;
+                    c->tile_cfg[l] = cfg;
+                    c->tile_ns[l] = ns;
+                    c->tile_fs[l] = fs;
+                }
             }
         }
     }
@@ -384,23 +448,28 @@ static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *r
         cur[ml] = 0;
         for (int jc = 0; jc < ncycle; ++jc) {
             for (int v = ml; v <= lc - 1; ++v) {
-                for (int p = 0; p < npre; ++p) {
-                    const bool zero = v > ml && p == 0;
-                    const bool resid = p == npre - 1;
-                    SG_TRY(launch_sweep(c, buf(v, cur[v]), buf(v, 1 - cur[v]), A.RHO[v], c->nl[v], c->hl[v], zero,
-                                        false, resid, nullptr, resid ? A.RHO[v + 1] : nullptr, nullptr));
+                // npre sweeps in launches of up to tile_ns[v]; the first may start from zero, the
+                // last carries the residual restriction
+                for (int r = npre, first = 1; r > 0; first = 0) {
+                    const int k = std::min(r, c->tile_ns[v]);
+                    const bool last = r == k;
+                    SG_TRY(launch_sweep(c, v, k, buf(v, cur[v]), buf(v, 1 - cur[v]), first && v > ml, false, last,
+                                        nullptr, last ? A.RHO[v + 1] : nullptr, nullptr));
                     cur[v] ^= 1;
+                    r -= k;
                 }
             }
             SG_TRY(launch_coarse(c, L, A.RHO[lc], A.U[lc], 1, ncycle, npre, npost, nullptr));
             cur[lc] = 0;
             for (int v = lc - 1; v >= ml; --v) {
-                for (int p = 0; p < npost; ++p) {
-                    const bool last = (ml == 0 && jc == ncycle - 1 && v == 0 && p == npost - 1);
-                    SG_TRY(launch_sweep(c, buf(v, cur[v]), buf(v, 1 - cur[v]), A.RHO[v], c->nl[v], c->hl[v], false,
-                                        p == 0, false, p == 0 ? buf(v + 1, cur[v + 1]) : nullptr, nullptr,
-                                        last ? resmax : nullptr));
+                // npost sweeps; the first launch adds the prolongated correction
+                for (int r = npost, first = 1; r > 0; first = 0) {
+                    const int k = std::min(r, c->tile_ns[v]);
+                    const bool lastcall = (ml == 0 && jc == ncycle - 1 && v == 0 && r == k);
+                    SG_TRY(launch_sweep(c, v, k, buf(v, cur[v]), buf(v, 1 - cur[v]), false, first, false,
+                                        first ? buf(v + 1, cur[v + 1]) : nullptr, nullptr, lastcall ? resmax : nullptr));
                     cur[v] ^= 1;
+                    r -= k;
                 }
             }
         }
@@ -423,9 +492,38 @@ static int read_norms(sg_uniform *c)
     return SG_OK;
 }
 
+static int launch_update_resid(sg_uniform *c, const double *pin, const float *e, double *pout, const double *src)
+{
+    const int n = c->n, lg = ilog2(n);
+    const long cells = n3(n);
+    int zc = (int)std::max<long>(4, std::min<long>(32, cells / (256L * kNormBlocks)));
+    zc = std::min(zc, n);
+    const dim3 grid(n / 64, n / 4, n / zc);
+    const int nb = (int)(grid.x * grid.y * grid.z);
+    SG_REQUIRE(nb <= c->npart, "norm partial buffer too small");
+    const double h2i = 1.0 / (c->h * c->h);
+    if (e)
+        hipLaunchKernelGGL(k_update_resid<true>, grid, dim3(256), 0, c->stream, pin, e, pout, src, c->f.RHO[0], lg, h2i,
+                           zc, c->pmax, c->psum);
+    else
+        hipLaunchKernelGGL(k_update_resid<false>, grid, dim3(256), 0, c->stream, pin, e, pout, src, c->f.RHO[0], lg,
+                           h2i, zc, c->pmax, c->psum);
+    SG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, nb, c->dnorm);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+// whether the mixed solve uses the fused update+residual step with a second phi buffer
+static bool pingpong(const sg_uniform *c, const sg_params *p)
+{
+    return p->precision == SG_PREC_MIXED && p->fused && c->n >= 64 && c->psi_alt;
+}
+
 // One outer iteration after the first residual: inner solve on RHO[0] (= f - L phi), update
-// phi += U[0], next residual with its norms into dnorm[0..1].  Resmaxg lands in dnorm[2].
-static int outer_iteration(sg_uniform *c, double *psi, const double *src, const sg_params *p)
+// phi += U[0] (from pin into pout; in place when pin == pout), next residual with its norms into
+// dnorm[0..1].  Resmaxg lands in dnorm[2].
+static int outer_iteration(sg_uniform *c, const double *pin, double *pout, const double *src, const sg_params *p)
 {
     // Resmaxg is only read by the reference stopping test; the true-residual test skips it.
     double *resmax = p->stop_mode == SG_STOP_REFERENCE ? c->dnorm + 2 : nullptr;
@@ -434,53 +532,60 @@ static int outer_iteration(sg_uniform *c, double *psi, const double *src, const 
         const float *u0 = c->f.U[0];
         if (p->fused) SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost, resmax, &u0));
         else SG_TRY(mg_lin<float>(c, p->ncycle, p->npre, p->npost, resmax));
-        SG_TRY(launch_update<float>(c, psi, u0));
-        SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+        if (pin != pout) {
+            SG_TRY(launch_update_resid(c, pin, u0, pout, src));
+        } else {
+            SG_TRY(launch_update<float>(c, pout, u0));
+            SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], pout, src, c->n, c->h, c->dnorm)));
+        }
     } else {
         SG_TRY(mg_lin<double>(c, p->ncycle, p->npre, p->npost, resmax));
-        SG_TRY(launch_update<double>(c, psi, c->d.U[0]));
-        SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+        SG_TRY(launch_update<double>(c, pout, c->d.U[0]));
+        SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], pout, src, c->n, c->h, c->dnorm)));
     }
     return SG_OK;
 }
 
-static void drop_graph(sg_uniform *c)
+static void drop_graphs(sg_uniform *c)
 {
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-    c->g_psi = nullptr;
-    c->g_src = nullptr;
+    for (auto &g : c->graph) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = sg_uniform::Graph();
+    }
 }
 
-static int run_outer_iteration(sg_uniform *c, double *psi, const double *src, const sg_params *p)
+static int run_outer_iteration(sg_uniform *c, const double *pin, double *pout, const double *src,
+                               const sg_params *p, int parity)
 {
-    if (!(p->use_graph && p->precision == SG_PREC_MIXED)) return outer_iteration(c, psi, src, p);
-    const bool same = c->gexec && c->g_psi == psi && c->g_src == src && c->g_ncycle == p->ncycle &&
-                      c->g_npre == p->npre && c->g_npost == p->npost && c->g_fused == p->fused &&
-                      c->g_stop == p->stop_mode;
+    if (!(p->use_graph && p->precision == SG_PREC_MIXED)) return outer_iteration(c, pin, pout, src, p);
+    sg_uniform::Graph &g = c->graph[parity & 1];
+    const bool same = g.exec && g.pin == pin && g.pout == pout && g.src == src && g.ncycle == p->ncycle &&
+                      g.npre == p->npre && g.npost == p->npost && g.fused == p->fused && g.stop == p->stop_mode;
     if (!same) {
-        drop_graph(c);
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = sg_uniform::Graph();
         hipGraph_t graph = nullptr;
         SG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = outer_iteration(c, psi, src, p);
+        const int rc = outer_iteration(c, pin, pout, src, p);
         const hipError_t e = hipStreamEndCapture(c->stream, &graph);
         if (rc != SG_OK) {
             if (graph) (void)hipGraphDestroy(graph);
             return rc;
         }
         SG_HIP(e);
-        const hipError_t e2 = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        const hipError_t e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         SG_HIP(e2);
-        c->g_psi = psi;
-        c->g_src = src;
-        c->g_ncycle = p->ncycle;
-        c->g_npre = p->npre;
-        c->g_npost = p->npost;
-        c->g_fused = p->fused;
-        c->g_stop = p->stop_mode;
+        g.pin = pin;
+        g.pout = pout;
+        g.src = src;
+        g.ncycle = p->ncycle;
+        g.npre = p->npre;
+        g.npost = p->npost;
+        g.fused = p->fused;
+        g.stop = p->stop_mode;
     }
-    SG_HIP(hipGraphLaunch(c->gexec, c->stream));
+    SG_HIP(hipGraphLaunch(g.exec, c->stream));
     return SG_OK;
 }
 
@@ -502,8 +607,11 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
     const bool mixed = p->precision == SG_PREC_MIXED;
     SG_HIP(hipEventRecord(c->ev0, c->stream));
     SG_TRY(launch_norm(c, src, n3(c->n), c->dnorm + 3));
-    if (mixed) SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+    const bool pp = pingpong(c, p);
+    if (pp) SG_TRY(launch_update_resid(c, psi, nullptr, nullptr, src));
+    else if (mixed) SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
     else SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+    double *cur = psi;  // buffer holding the current phi (psi or psi_alt)
     SG_TRY(read_norms(c));
     const double fmax = c->hnorm[3], fl2 = std::sqrt(c->hnorm[4]);
     const double rhomax = c->hnorm[0];
@@ -523,11 +631,14 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
             if ((rhomax > 0 ? resmaxg / rhomax : 0.0) < p->tol) { status = SG_OK; break; }
         }
         if (it >= p->maxit) break;
-        SG_TRY(run_outer_iteration(c, psi, src, p));
+        double *next = pp ? (cur == psi ? c->psi_alt : psi) : cur;
+        SG_TRY(run_outer_iteration(c, cur, next, src, p, cur == psi ? 0 : 1));
+        cur = next;
         SG_TRY(read_norms(c));
         ++it;
         st->resmaxg = c->hnorm[2] / (c->h * c->h);
     }
+    if (cur != psi) SG_HIP(hipMemcpyAsync(psi, cur, sizeof(double) * n3(c->n), hipMemcpyDeviceToDevice, c->stream));
     SG_HIP(hipEventRecord(c->ev1, c->stream));
     SG_HIP(hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -566,12 +677,13 @@ static void destroy(sg_uniform *c)
 {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    drop_graph(c);
+    drop_graphs(c);
     free_levels(c->d);
     free_levels(c->f);
     for (int l = 0; l < kMaxLev; ++l)
         if (c->U1[l]) (void)hipFree(c->U1[l]);
     if (c->psi_dev) (void)hipFree(c->psi_dev);
+    if (c->psi_alt) (void)hipFree(c->psi_alt);
     if (c->src_dev) (void)hipFree(c->src_dev);
     if (c->pmax) (void)hipFree(c->pmax);
     if (c->psum) (void)hipFree(c->psum);
@@ -626,6 +738,7 @@ static int create(sg_uniform **out, int n, double h, int precisions)
                     SG_HIP(hipMemsetAsync(c->U1[l], 0, bytes, c->stream));
                 }
                 SG_TRY(autotune_tiles(c));
+                SG_HIP(hipMalloc(&c->psi_alt, sizeof(double) * (size_t)n3(n)));
             }
             SG_HIP(hipStreamSynchronize(c->stream));
             return SG_OK;
@@ -709,7 +822,7 @@ int sg_uniform_gradient_dev(sg_uniform *c, const double *psi, double *g)
 {
     SG_REQUIRE(c && psi && g, "null argument");
     hipLaunchKernelGGL(sg::k_gradient, dim3(sg::nblocks(sg::n3(c->n))), dim3(sg::kBlock), 0, c->stream, g, psi,
-                       c->n, 1.0 / c->h);
+                       sg::ilog2(c->n), 1.0 / c->h);
     SG_CHECK_LAUNCH();
     SG_HIP(hipStreamSynchronize(c->stream));
     return SG_OK;
@@ -718,7 +831,7 @@ int sg_uniform_gradient_dev(sg_uniform *c, const double *psi, double *g)
 int sg_uniform_relax_dev(sg_uniform *c, int prec, void *u, const void *rhs, int n, double h, double *resmaxg)
 {
     SG_REQUIRE(c && u && rhs, "null argument");
-    SG_REQUIRE(n >= 1 && n <= c->n, "bad n");
+    SG_REQUIRE(n >= 1 && n <= c->n && (n & (n - 1)) == 0, "bad n");
     SG_TRY(sg::launch_fill<double>(c->stream, c->dnorm + 2, 1));
     if (prec == SG_PREC_FP64)
         SG_TRY(sg::launch_relax<double>(c->stream, (double *)u, (const double *)rhs, n, h, c->dnorm + 2));
@@ -734,7 +847,7 @@ int sg_uniform_relax_dev(sg_uniform *c, int prec, void *u, const void *rhs, int 
 int sg_uniform_resid_dev(sg_uniform *c, int prec, void *res, const void *u, const void *rhs, int n, double h)
 {
     SG_REQUIRE(c && res && u && rhs, "null argument");
-    SG_REQUIRE(n >= 1 && n <= c->n, "bad n");
+    SG_REQUIRE(n >= 1 && n <= c->n && (n & (n - 1)) == 0, "bad n");
     if (prec == SG_PREC_FP64)
         SG_TRY((sg::launch_resid<double, double>(c, (double *)res, (const double *)u, (const double *)rhs, n, h, nullptr)));
     else if (prec == SG_PREC_MIXED)
@@ -748,7 +861,7 @@ int sg_uniform_resid_dev(sg_uniform *c, int prec, void *res, const void *u, cons
 int sg_uniform_restrict_dev(sg_uniform *c, int prec, void *uc, const void *uf, int nc)
 {
     SG_REQUIRE(c && uc && uf, "null argument");
-    SG_REQUIRE(nc >= 1 && 2 * nc <= c->n, "bad nc");
+    SG_REQUIRE(nc >= 1 && 2 * nc <= c->n && (nc & (nc - 1)) == 0, "bad nc");
     if (prec == SG_PREC_FP64) SG_TRY(sg::launch_restrict<double>(c->stream, (double *)uc, (const double *)uf, nc));
     else if (prec == SG_PREC_MIXED) SG_TRY(sg::launch_restrict<float>(c->stream, (float *)uc, (const float *)uf, nc));
     else SG_REQUIRE(false, "bad precision");
@@ -768,7 +881,8 @@ int sg_uniform_time_smoother(sg_uniform *c, int prec, int reps, double *ms_per_s
         else if (fused) {
             // the mixed path's fine-level sweep: U[0] -> U1[0] and back on alternate repetitions
             float *a = (r & 1) ? c->U1[0] : c->f.U[0], *b = (r & 1) ? c->f.U[0] : c->U1[0];
-            SG_TRY(sg::launch_sweep(c, a, b, c->f.RHO[0], c->n, c->h, false, false, false, nullptr, nullptr, nullptr));
+            SG_TRY(sg::launch_sweep_cfg(c->stream, c->tile_cfg[0], 1, c->tile_fs[0] != 0, a, b, c->f.RHO[0], c->n,
+                                        c->h, false, false, false, nullptr, nullptr, nullptr));
         } else
             SG_TRY(sg::launch_relax<float>(c->stream, c->f.U[0], c->f.RHO[0], c->n, c->h, nullptr));
     }
@@ -780,18 +894,20 @@ int sg_uniform_time_smoother(sg_uniform *c, int prec, int reps, double *ms_per_s
     return SG_OK;
 }
 
-int sg_uniform_tile_info(sg_uniform *c, int level, int *cfg, float *ms, int nms)
+int sg_uniform_tile_info(sg_uniform *c, int level, int *cfg, int *ns, int *fs, float *ms, int nms)
 {
-    SG_REQUIRE(c && cfg && level >= 0 && level < c->nlev, "bad argument");
+    SG_REQUIRE(c && cfg && ns && fs && level >= 0 && level < c->nlev, "bad argument");
     *cfg = c->tile_cfg[level];
-    for (int q = 0; ms && q < nms && q < sg::kNumTiles; ++q) ms[q] = c->tune_ms[level][q];
+    *ns = c->tile_ns[level];
+    *fs = c->tile_fs[level];
+    for (int q = 0; ms && q < nms && q < sg::kNumTiles * sg::kMaxSweepsPerLaunch * 2; ++q) ms[q] = c->tune_ms[level][q];
     return SG_OK;
 }
 
 int sg_uniform_interp_dev(sg_uniform *c, int prec, void *uf, const void *uc, int nf, int cubic)
 {
     SG_REQUIRE(c && uf && uc, "null argument");
-    SG_REQUIRE(nf >= 2 && nf <= c->n && nf % 2 == 0, "bad nf");
+    SG_REQUIRE(nf >= 2 && nf <= c->n && (nf & (nf - 1)) == 0, "bad nf");
     if (prec == SG_PREC_FP64)
         SG_TRY(sg::launch_interp<double>(c->stream, (double *)uf, (const double *)uc, nf, cubic != 0, false));
     else if (prec == SG_PREC_MIXED)

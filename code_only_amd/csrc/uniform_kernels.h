@@ -83,31 +83,33 @@ __device__ __forceinline__ T trilinear_value(T c, T x, T y, T z, T yz, T xz, T x
     return s / T(64);
 }
 
+// Dense periodic n^3 grids with n = 2^lg: cell c = (k << 2lg) | (j << lg) | i, neighbours by mask.
+
 // mg_poisson.F90:55-74, one colour (ipass 1 = cells with i+j+k odd, 2 = even).  One thread per
-// colour cell.  resmax (nullable) receives max |resh2| by atomic max.
+// colour cell (n = 1: the single cell belongs to the second pass).  resmax (nullable) receives
+// max |resh2| by one atomic max per block.
 template <typename T>
 __global__ void __launch_bounds__(256) k_relax_color(T *__restrict__ u, const T *__restrict__ rhs,
-                                                     int n, T h2, int ipass, double *resmax)
+                                                     int lg, T h2, int ipass, double *resmax)
 {
-    const int nh = (n + 1) >> 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long total = (long)nh * n * n;
+    const int n = 1 << lg, m = n - 1;
+    const int lgh = lg > 0 ? lg - 1 : 0;
+    const int total = lg > 0 ? (1 << (3 * lg - 1)) : 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double amax = 0.0;
-    if (t < total) {
-        const int ih = (int)(t % nh);
-        const long r = t / nh;
-        const int j = (int)(r % n), k = (int)(r / n);
-        const int i = 2 * ih + ((j + k + ipass) & 1);
-        if (i < n) {
-            const int c = (k * n + j) * n + i;
-            const int row = (k * n + j) * n;
-            T s;
-            u[c] = relax_value<T>(u[c], u[row + wrap_p(i, n)], u[row + wrap_m(i, n)],
-                                  u[(k * n + wrap_p(j, n)) * n + i], u[(k * n + wrap_m(j, n)) * n + i],
-                                  u[(wrap_p(k, n) * n + j) * n + i], u[(wrap_m(k, n) * n + j) * n + i], rhs[c],
-                                  h2, &s);
-            amax = fabs((double)s);
-        }
+    if (t < total && (lg > 0 || ipass == 2)) {
+        const int ih = t & ((1 << lgh) - 1);
+        const int j = (t >> lgh) & m, k = t >> (lgh + lg);
+        const int i = lg > 0 ? 2 * ih + ((j + k + ipass) & 1) : 0;
+        const int row = (k << (2 * lg)) | (j << lg);
+        const int c = row | i;
+        T s;
+        u[c] = relax_value<T>(u[c], u[row | ((i + 1) & m)], u[row | ((i - 1) & m)],
+                              u[(k << (2 * lg)) | (((j + 1) & m) << lg) | i],
+                              u[(k << (2 * lg)) | (((j - 1) & m) << lg) | i],
+                              u[(((k + 1) & m) << (2 * lg)) | (j << lg) | i],
+                              u[(((k - 1) & m) << (2 * lg)) | (j << lg) | i], rhs[c], h2, &s);
+        amax = fabs((double)s);
     }
     if (resmax) block_atomic_max(resmax, amax);
 }
@@ -117,20 +119,19 @@ __global__ void __launch_bounds__(256) k_relax_color(T *__restrict__ u, const T 
 // deterministic two-stage norm (fixed grid, fixed per-thread order).
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin *__restrict__ u,
-                                               const Tin *__restrict__ rhs, int n, Tin h2i,
+                                               const Tin *__restrict__ rhs, int lg, Tin h2i,
                                                double *pmax, double *psum)
 {
-    const long n3 = (long)n * n * n;
+    const int n = 1 << lg, m = n - 1, n3 = 1 << (3 * lg);
     double mx = 0.0, sm = 0.0;
-    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(c % n);
-        const int j = (int)((c / n) % n);
-        const int k = (int)(c / ((long)n * n));
-        const int row = (k * n + j) * n;
-        const Tin r = resid_value<Tin>(u[c], u[row + wrap_p(i, n)], u[row + wrap_m(i, n)],
-                                       u[(k * n + wrap_p(j, n)) * n + i], u[(k * n + wrap_m(j, n)) * n + i],
-                                       u[(wrap_p(k, n) * n + j) * n + i], u[(wrap_m(k, n) * n + j) * n + i],
-                                       rhs[c], h2i);
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n3; c += gridDim.x * blockDim.x) {
+        const int i = c & m, j = (c >> lg) & m, k = c >> (2 * lg);
+        const int row = c - i;
+        const Tin r = resid_value<Tin>(u[c], u[row | ((i + 1) & m)], u[row | ((i - 1) & m)],
+                                       u[(k << (2 * lg)) | (((j + 1) & m) << lg) | i],
+                                       u[(k << (2 * lg)) | (((j - 1) & m) << lg) | i],
+                                       u[(((k + 1) & m) << (2 * lg)) | (j << lg) | i],
+                                       u[(((k - 1) & m) << (2 * lg)) | (j << lg) | i], rhs[c], h2i);
         res[c] = (Tout)r;
         mx = fmax(mx, fabs((double)r));
         sm += (double)r * (double)r;
@@ -138,6 +139,47 @@ __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin
     if (pmax) {
         block_max_sum(mx, sm);
         if (threadIdx.x == 0) { pmax[blockIdx.x] = mx; psum[blockIdx.x] = sm; }
+    }
+}
+
+// Outer FP64 step of the mixed solve, fused: pout = pin + e (mg.F90:87) and
+// r = f - L pout (mg_poisson.F90:17-27) stored as FP32 for the next inner solve, with the norm
+// partials of r.  Neighbour values are formed as pin + e on the fly, exactly the values pout
+// receives, so the result equals the separate update and residual kernels bit for bit; pin and
+// pout must differ.  UPD = false: r = f - L pin only.  One thread per (i, j) column of a 64 x 4
+// patch marching through zc planes, the z neighbours kept in registers.  n = 2^lg >= 64.
+template <bool UPD>
+__global__ void __launch_bounds__(256) k_update_resid(const double *__restrict__ pin, const float *__restrict__ e,
+                                                      double *__restrict__ pout, const double *__restrict__ f,
+                                                      float *__restrict__ r32, int lg, double h2i, int zc,
+                                                      double *pmax, double *psum)
+{
+    const int n = 1 << lg, m = n - 1;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z0 = blockIdx.z * zc;
+    auto at = [&](int ii, int jj, int kk) { return (kk << (2 * lg)) | (jj << lg) | ii; };
+    auto P = [&](int q) -> double { return UPD ? pin[q] + (double)e[q] : pin[q]; };
+    const int ip = (i + 1) & m, im = (i - 1) & m, jp = (j + 1) & m, jm = (j - 1) & m;
+    double zm = P(at(i, j, (z0 - 1) & m)), zcen = P(at(i, j, z0));
+    double mx = 0.0, sm = 0.0;
+    for (int k = z0; k < z0 + zc; ++k) {
+        const double zp = P(at(i, j, (k + 1) & m));
+        const int c = at(i, j, k);
+        const double rv = resid_value<double>(zcen, P(at(ip, j, k)), P(at(im, j, k)), P(at(i, jp, k)),
+                                              P(at(i, jm, k)), zp, zm, f[c], h2i);
+        r32[c] = (float)rv;
+        if (UPD) pout[c] = zcen;
+        mx = fmax(mx, fabs(rv));
+        sm += rv * rv;
+        zm = zcen;
+        zcen = zp;
+    }
+    block_max_sum(mx, sm);
+    if (threadIdx.x == 0) {
+        const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        pmax[b] = mx;
+        psum[b] = sm;
     }
 }
 
@@ -151,36 +193,34 @@ __global__ void __launch_bounds__(1024) k_norm_finalize(const double *pmax, cons
     if (threadIdx.x == 0) { out[0] = mx; out[1] = sm; }
 }
 
-// restriction.F90:132-147, one thread per coarse cell.
+// restriction.F90:132-147, one thread per coarse cell (nc = 2^lgc).
 template <typename T>
-__global__ void __launch_bounds__(256) k_restrict(T *__restrict__ uc, const T *__restrict__ uf, int nc)
+__global__ void __launch_bounds__(256) k_restrict(T *__restrict__ uc, const T *__restrict__ uf, int lgc)
 {
-    const long n3 = (long)nc * nc * nc;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n3) return;
-    const int ic = (int)(c % nc), jc = (int)((c / nc) % nc), kc = (int)(c / ((long)nc * nc));
-    const int nf = 2 * nc;
+    const int nc = 1 << lgc, c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (1 << (3 * lgc))) return;
+    const int ic = c & (nc - 1), jc = (c >> lgc) & (nc - 1), kc = c >> (2 * lgc);
+    const int lgf = lgc + 1;
     const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
-    auto F = [&](int a, int b, int d) { return uf[((k + d) * nf + (j + b)) * nf + (i + a)]; };
+    auto F = [&](int a, int b, int d) { return uf[((k + d) << (2 * lgf)) | ((j + b) << lgf) | (i + a)]; };
     uc[c] = restrict_value<T>(F(0, 0, 0), F(1, 0, 0), F(0, 1, 0), F(0, 0, 1), F(1, 1, 0), F(1, 0, 1), F(0, 1, 1),
                               F(1, 1, 1));
 }
 
-// interpolation.F90:226-245 trilinear prolongation of a periodic coarse grid; ADD selects
-// uf = uf + P uc (mg_addint, mg.F90:727-731) instead of uf = P uc.
+// interpolation.F90:226-245 trilinear prolongation of a periodic coarse grid (nf = 2^lgf); ADD
+// selects uf = uf + P uc (mg_addint, mg.F90:727-731) instead of uf = P uc.
 template <typename T, bool ADD>
-__global__ void __launch_bounds__(256) k_interp_linear(T *__restrict__ uf, const T *__restrict__ uc, int nf)
+__global__ void __launch_bounds__(256) k_interp_linear(T *__restrict__ uf, const T *__restrict__ uc, int lgf)
 {
-    const long n3 = (long)nf * nf * nf;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n3) return;
-    const int nc = nf >> 1;
-    const int i = (int)(c % nf), jf = (int)((c / nf) % nf), kf = (int)(c / ((long)nf * nf));
+    const int nf = 1 << lgf, c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (1 << (3 * lgf))) return;
+    const int lgc = lgf - 1, nc = nf >> 1;
+    const int i = c & (nf - 1), jf = (c >> lgf) & (nf - 1), kf = c >> (2 * lgf);
     const int ic = i >> 1, jc = jf >> 1, kc = kf >> 1;
-    const int ic1 = wrap_g(ic + 2 * (i & 1) - 1, nc);
-    const int jc1 = wrap_g(jc + 2 * (jf & 1) - 1, nc);
-    const int kc1 = wrap_g(kc + 2 * (kf & 1) - 1, nc);
-    auto C = [&](int a, int b, int d) { return uc[(d * nc + b) * nc + a]; };
+    const int ic1 = (ic + 2 * (i & 1) - 1) & (nc - 1);
+    const int jc1 = (jc + 2 * (jf & 1) - 1) & (nc - 1);
+    const int kc1 = (kc + 2 * (kf & 1) - 1) & (nc - 1);
+    auto C = [&](int a, int b, int d) { return uc[(d << (2 * lgc)) | (b << lgc) | a]; };
     const T v = trilinear_value<T>(C(ic, jc, kc), C(ic1, jc, kc), C(ic, jc1, kc), C(ic, jc, kc1), C(ic, jc1, kc1),
                                    C(ic1, jc, kc1), C(ic1, jc1, kc), C(ic1, jc1, kc1));
     if (ADD) uf[c] = uf[c] + v;
@@ -214,13 +254,12 @@ __device__ __forceinline__ void cubic_stencil(int f, int lo, int hi, int idx[4],
 // fine cell the x pass (bufx), y pass (bufxy) and z pass are evaluated in the same operand order
 // as the reference's three separable sweeps, so the result is bitwise that of the sweeps.
 template <typename T>
-__global__ void __launch_bounds__(256) k_interp_cubic(T *__restrict__ uf, const T *__restrict__ uc, int nf)
+__global__ void __launch_bounds__(256) k_interp_cubic(T *__restrict__ uf, const T *__restrict__ uc, int lgf)
 {
-    const long n3 = (long)nf * nf * nf;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n3) return;
+    const int nf = 1 << lgf, c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (1 << (3 * lgf))) return;
     const int nc = nf >> 1;
-    const int i = (int)(c % nf), jf = (int)((c / nf) % nf), kf = (int)(c / ((long)nf * nf));
+    const int i = c & (nf - 1), jf = (c >> lgf) & (nf - 1), kf = c >> (2 * lgf);
     int ix[4], iy[4], iz[4];
     double ax[4], ay[4], az[4];
     cubic_stencil(i, -1, nc, ix, ax);
@@ -271,13 +310,12 @@ __global__ void __launch_bounds__(256) k_update(double *__restrict__ psi, const 
 }
 
 // fmg_poisson.F90:29-31: g_d = -(u(+1) - u(-1)) * hi * 0.5, periodic.
-__global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g, const double *__restrict__ u, int n,
+__global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g, const double *__restrict__ u, int lg,
                                                   double hi)
 {
-    const long n3 = (long)n * n * n;
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = 1 << lg, n3 = 1 << (3 * lg), c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n3) return;
-    const int i = (int)(c % n), j = (int)((c / n) % n), k = (int)(c / ((long)n * n));
+    const int i = c & (n - 1), j = (c >> lg) & (n - 1), k = c >> (2 * lg);
     const int row = (k * n + j) * n;
     g[c] = -(u[row + wrap_p(i, n)] - u[row + wrap_m(i, n)]) * hi * 0.5;
     g[n3 + c] = -(u[(k * n + wrap_p(j, n)) * n + i] - u[(k * n + wrap_m(j, n)) * n + i]) * hi * 0.5;
