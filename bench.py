@@ -32,7 +32,24 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--smoother-reps", type=int, default=50)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (reference reading for HBM counters)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """5 device-to-device copies of a float32 tensor of `mib` MiB: a kernel whose HBM traffic is
+    known (mib MiB read + mib MiB written per launch), to scale the FETCH_SIZE / WRITE_SIZE
+    counters of the solver's kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "launches": 5}), flush=True)
 
 
 def log(msg):
@@ -51,6 +68,9 @@ def cpu_model():
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build's solve runs on a single MI355X "
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
@@ -123,6 +143,14 @@ def main():
         m //= 2
     log(f"[bench] autotuned tiles {json.dumps(tiles)}")
 
+    # HBM bytes per launch of the roofline kernel, from the committed rocprofv3 counter summary
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "uniform256_hbm_traffic.md")
+    if n == 256 and tiles.get("256", {}).get("tile") == "32x16x16" and os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = int(line.split(":")[1])
+
     p64 = psi64 - psi64.mean()
     pmx = psimx - psimx.mean()
     err_mx_vs_64 = ((pmx - p64).norm() / p64.norm()).item()
@@ -166,8 +194,9 @@ def main():
         "phi_err_vs_fp64_gpu": err_mx_vs_64,
         "phi_err_vs_cpu_oracle": err_vs_oracle,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "fine-level FP32 smoother sweep (both colours)",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "fine-level FP32 smoother sweep (both colours), 32x16x16 LDS tile",
+                     "algorithmic_bytes_per_launch": sweep_bytes,
                      "ms_per_launch": ms_sweep.value},
         "cpu_baseline": cpu,
         "autotune": tiles,

@@ -37,8 +37,40 @@ def build_hip(verbose=False):
     return LIB
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
+FLANG = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+FORTRAN_MODULES = ["sg_capi.F90", "mg_gpu.F90"]
+FORTRAN_TESTS = [os.path.join(ROOT, "tests", "fortran", "mg_drop_in_test.F90")]
+
+
+def build_fortran(verbose=False):
+    """Compile the Fortran drop-in modules and the Fortran test driver, linked against libsg.so."""
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    objs = []
+    for src in FORTRAN_MODULES:
+        obj = os.path.join(FORTRAN_BUILD, src.replace(".F90", ".o"))
+        cmd = [FLANG, "-O2", "-fPIC", "-c", os.path.join(FORTRAN_DIR, src), "-module-dir", FORTRAN_BUILD,
+               "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    exes = []
+    for src in FORTRAN_TESTS:
+        exe = os.path.join(FORTRAN_BUILD, os.path.basename(src).replace(".F90", ""))
+        cmd = [FLANG, "-O2", src, "-I", FORTRAN_BUILD] + objs + [
+            "-L", HERE, "-lsg", "-Wl,-rpath," + HERE + ",-rpath,$ORIGIN/../..", "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        exes.append(exe)
+    return exes
+
+
 def build(verbose=False):
     build_hip(verbose)
+    build_fortran(verbose)
     return LIB
 
 
