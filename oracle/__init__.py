@@ -16,11 +16,19 @@ _LIB = None
 STOP_REFERENCE = 0
 STOP_TRUE_RESIDUAL = 1
 HIST = 64
+BC_PERIODIC = 0
+BC_DIRICHLET = 1
 
 
 class OrcParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxit", ctypes.c_int), ("mode", ctypes.c_int),
                 ("ncycle", ctypes.c_int), ("npre", ctypes.c_int), ("npost", ctypes.c_int)]
+
+
+class OrcAmrStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int),
+                ("rel_linf", ctypes.c_double), ("rel_l2", ctypes.c_double),
+                ("ratio", ctypes.c_double), ("hist", ctypes.c_double * HIST)]
 
 
 class OrcStats(ctypes.Structure):
@@ -55,6 +63,26 @@ def lib():
         L.orc_gradient_dense.argtypes = [dp, dp, ci, cd]
         L.orc_cubic_stencil.argtypes = [ci, ci, ci, ctypes.POINTER(ci), dp]
         L.orc_rstrct_cubic_weights.argtypes = [dp]
+        L.orc_set_bc.argtypes = [ci]
+        ip = ctypes.POINTER(ci)
+        L.orc_amr_create.argtypes = [ci, ci, ip, ip, ip, ip, ip, ip, cd, dp, ci]
+        L.orc_amr_create.restype = ctypes.c_void_p
+        L.orc_amr_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_amr_set_zero_average.argtypes = [ctypes.c_void_p, ci]
+        L.orc_amr_block_len.argtypes = [ctypes.c_void_p, ci]
+        L.orc_amr_block_len.restype = ctypes.c_size_t
+        L.orc_amr_get_fill0_level.argtypes = [ctypes.c_void_p]
+        L.orc_amr_get_fill0_level.restype = ci
+        L.orc_amr_set_fill0_level.argtypes = [ctypes.c_void_p, ci]
+        L.orc_amr_resh2maxg.argtypes = [ctypes.c_void_p]
+        L.orc_amr_resh2maxg.restype = cd
+        L.orc_amr_set_array.argtypes = [ctypes.c_void_p, ci, ci, dp]
+        L.orc_amr_get_array.argtypes = [ctypes.c_void_p, ci, ci, dp]
+        L.orc_amr_op.argtypes = [ctypes.c_void_p, ci, ci, ci, ci, ci, ci]
+        L.orc_amr_op.restype = cd
+        L.orc_amr_solve.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.POINTER(OrcParams),
+                                    ctypes.POINTER(OrcAmrStats)]
+        L.orc_amr_solve.restype = ci
         _LIB = L
     return _LIB
 
@@ -104,14 +132,20 @@ def rstrct_cubic_weights():
     return list(a)
 
 
-def mg_cycle(src, h, psi=None, tol=1e-10, maxit=50, mode=STOP_TRUE_RESIDUAL, ncycle=2, npre=2, npost=2):
-    """Uniform periodic mg_cycle (mg.F90:42-94) on dense n^3 arrays.  Returns (psi, stats dict)."""
+def mg_cycle(src, h, psi=None, tol=1e-10, maxit=50, mode=STOP_TRUE_RESIDUAL, ncycle=2, npre=2, npost=2,
+             bc=BC_PERIODIC):
+    """Uniform mg_cycle (mg.F90:42-94) on dense n^3 arrays, periodic or zero-Dirichlet ghosts.
+    Returns (psi, stats dict)."""
     n = src.shape[0]
     src = np.ascontiguousarray(src, np.float64)
     psi = np.zeros_like(src) if psi is None else np.ascontiguousarray(psi, np.float64).copy()
     p = OrcParams(tol, maxit, mode, ncycle, npre, npost)
     st = OrcStats()
-    lib().orc_mg_cycle_dense(_p(psi), _p(src), n, h, ctypes.byref(p), ctypes.byref(st))
+    lib().orc_set_bc(bc)
+    try:
+        lib().orc_mg_cycle_dense(_p(psi), _p(src), n, h, ctypes.byref(p), ctypes.byref(st))
+    finally:
+        lib().orc_set_bc(BC_PERIODIC)
     stats = dict(iterations=st.iterations, status=st.status, rel_linf=st.rel_linf, rel_l2=st.rel_l2,
                  resmaxg=st.resmaxg, hist=[st.hist[i] for i in range(min(st.iterations + 1, HIST))])
     return psi, stats
@@ -122,3 +156,78 @@ def gradient(u, h):
     g = np.zeros((3, n, n, n))
     lib().orc_gradient_dense(_p(g), _p(np.ascontiguousarray(u)), n, h)
     return g
+
+
+# ---- AMR (fmg.F90 / vmg.F90) ---------------------------------------------------------------
+U, RES, RHS, RHO, SRC, PSI = range(6)
+OP_C2P, OP_GHOSTFIX, OP_FILL0, OP_EXTRAP, OP_RELAX, OP_RESID, OP_RSTRCT, OP_INTERP, OP_ADDINT, \
+    OP_VMG_FAS, OP_FMG_LIN, OP_SPHERE_BC, OP_REMOVE_MEAN = range(1, 14)
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Amr:
+    """The AMR restatement (amr.c) over a ``code_only_amd.amr_hierarchy.Hierarchy``."""
+
+    def __init__(self, h, zero_average=None):
+        self.h = h
+        self._keep = [np.ascontiguousarray(a, np.int32) for a in (h.level, h.ijk, h.parent, h.child, h.nbr)]
+        dims = np.asarray(h.base_dims, np.int32)
+        x0 = np.asarray(h.x0, np.float64)
+        self._keep += [dims, x0]
+        self.ptr = lib().orc_amr_create(h.nblk, h.nlev, *[_ip(a) for a in self._keep[:5]], _ip(dims),
+                                        float(h.h0), _p(x0), int(h.bc))
+        if not self.ptr:
+            raise ValueError("orc_amr_create rejected the hierarchy")
+        if zero_average is not None:
+            lib().orc_amr_set_zero_average(self.ptr, int(zero_average))
+
+    def close(self):
+        if self.ptr:
+            lib().orc_amr_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def n(self, L):
+        return 8 >> L
+
+    def get(self, L, code):
+        n = self.n(L)
+        a = np.empty((self.h.nblk, n + 2, n + 2, n + 2))
+        lib().orc_amr_get_array(self.ptr, L, code, _p(a))
+        return a
+
+    def set(self, L, code, a):
+        n = self.n(L)
+        a = np.ascontiguousarray(a, np.float64).reshape(self.h.nblk, n + 2, n + 2, n + 2)
+        lib().orc_amr_set_array(self.ptr, L, code, _p(a))
+
+    @property
+    def fill0_level(self):
+        return lib().orc_amr_get_fill0_level(self.ptr)
+
+    @fill0_level.setter
+    def fill0_level(self, v):
+        lib().orc_amr_set_fill0_level(self.ptr, int(v))
+
+    def op(self, op, L=0, a=0, b=0, c=0, flag=0):
+        return lib().orc_amr_op(self.ptr, op, L, a, b, c, flag)
+
+    def solve(self, src, psi=None, tol=1e-10, maxit=30, mode=STOP_TRUE_RESIDUAL, want_g=True):
+        """fmg_cycle on block arrays [nblk, 8, 8, 8].  Returns (psi, g or None, stats dict)."""
+        src = np.ascontiguousarray(src, np.float64)
+        psi = np.zeros_like(src) if psi is None else np.ascontiguousarray(psi, np.float64).copy()
+        g = np.zeros((self.h.nblk, 3) + src.shape[1:]) if want_g else None
+        p = OrcParams(tol, maxit, mode, 2, 2, 2)
+        st = OrcAmrStats()
+        lib().orc_amr_solve(self.ptr, _p(psi), _p(src), _p(g) if want_g else None, ctypes.byref(p),
+                            ctypes.byref(st))
+        stats = dict(iterations=st.iterations, status=st.status, rel_linf=st.rel_linf,
+                     rel_l2=st.rel_l2, ratio=st.ratio,
+                     hist=[st.hist[i] for i in range(min(st.iterations + 1, HIST))])
+        return psi, g, stats

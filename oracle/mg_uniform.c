@@ -36,8 +36,49 @@
 
 size_t orc_ghosted_len(int n) { return (size_t)(n + 2) * (size_t)(n + 2) * (size_t)(n + 2); }
 
-/* fmg_boundary_phys_periodic.F90:71-104 -- ghost wrap i, then j, then k (edges/corners wrap too). */
+/* Boundary mode of the uniform grid (the base grid of the MG tier):
+ *   ORC_BC_PERIODIC  -- fmg_boundary_phys_periodic.F90: mg_boundary_u wraps the ghosts; the base
+ *                       blocks have no physical boundary, so fill0 / extrap do nothing;
+ *   ORC_BC_DIRICHLET -- fmg_boundary_phys_sphere.F90:71-75: mg_boundary_u does nothing; every face
+ *                       touches the physical boundary, so mg_boundary_fill0 zeroes the ghosts
+ *                       (mg.F90:735-750) and mg_boundary_extrap mirrors them (mg.F90:754-771). */
+static int g_bc = ORC_BC_PERIODIC;
+void orc_set_bc(int bc) { g_bc = bc; }
+
+static void wrap_u(double *u, int n);
+
+/* mg_boundary_u of the selected boundary module. */
 void orc_boundary_u(double *u, int n)
+{
+    if (g_bc == ORC_BC_PERIODIC) wrap_u(u, n);
+}
+
+/* mg.F90:735-750 with every face touching the physical boundary (Dirichlet mode only). */
+void orc_boundary_fill0(double *u, int n)
+{
+    if (g_bc != ORC_BC_DIRICHLET) return;
+    for (int k = -1; k <= n; ++k)
+        for (int j = -1; j <= n; ++j) { u[GI(n, -1, j, k)] = 0.0; u[GI(n, n, j, k)] = 0.0; }
+    for (int k = -1; k <= n; ++k)
+        for (int i = -1; i <= n; ++i) { u[GI(n, i, -1, k)] = 0.0; u[GI(n, i, n, k)] = 0.0; }
+    for (int j = -1; j <= n; ++j)
+        for (int i = -1; i <= n; ++i) { u[GI(n, i, j, -1)] = 0.0; u[GI(n, i, j, n)] = 0.0; }
+}
+
+/* mg.F90:754-771: u(ghost) = -u(first interior), x faces, then y, then z (Dirichlet mode only). */
+void orc_boundary_extrap(double *u, int n)
+{
+    if (g_bc != ORC_BC_DIRICHLET) return;
+    for (int k = -1; k <= n; ++k)
+        for (int j = -1; j <= n; ++j) { u[GI(n, -1, j, k)] = -u[GI(n, 0, j, k)]; u[GI(n, n, j, k)] = -u[GI(n, n - 1, j, k)]; }
+    for (int k = -1; k <= n; ++k)
+        for (int i = -1; i <= n; ++i) { u[GI(n, i, -1, k)] = -u[GI(n, i, 0, k)]; u[GI(n, i, n, k)] = -u[GI(n, i, n - 1, k)]; }
+    for (int j = -1; j <= n; ++j)
+        for (int i = -1; i <= n; ++i) { u[GI(n, i, j, -1)] = -u[GI(n, i, j, 0)]; u[GI(n, i, j, n)] = -u[GI(n, i, j, n - 1)]; }
+}
+
+/* fmg_boundary_phys_periodic.F90:71-104 -- ghost wrap i, then j, then k (edges/corners wrap too). */
+static void wrap_u(double *u, int n)
 {
     for (int k = -1; k <= n; ++k)
         for (int j = -1; j <= n; ++j) {
@@ -294,16 +335,17 @@ static void orc_mg_free(orc_mg *m)
 
 static void fill0(double *u, int n) { memset(u, 0, sizeof(double) * orc_ghosted_len(n)); }
 
-/* mg.F90:608-633 for a periodic uniform grid: coarse ghosts wrapped, then cubic or linear
- * prolongation into the fine interior, then fine and coarse ghosts wrapped again.  (The
- * extrapolation / zero fills at mg.F90:620,629-630 act on ghosts only and are overwritten by the
- * periodic wrap that follows each of them.) */
+/* mg.F90:608-633: coarse ghosts extrapolated and refreshed, cubic or linear prolongation into the
+ * fine interior, then fine and coarse ghosts zeroed (Dirichlet) and refreshed (periodic). */
 void orc_mg_interp(double *uf, double *uc, int nf, int cubic)
 {
     int nc = nf / 2;
+    orc_boundary_extrap(uc, nc);
     orc_boundary_u(uc, nc);
     if (cubic) orc_interp_tricubic(uf, uc, nf);
     else orc_interp_trilinear(uf, uc, nf);
+    orc_boundary_fill0(uf, nf);
+    orc_boundary_fill0(uc, nc);
     orc_boundary_u(uf, nf);
     orc_boundary_u(uc, nc);
 }
@@ -374,6 +416,7 @@ int orc_mg_cycle(double *psi, const double *src, int n, double h, const orc_para
     double *PSI = psi;
     double *SRC = (double *)malloc(sizeof(double) * len);
     memcpy(SRC, src, sizeof(double) * len);
+    orc_boundary_fill0(PSI, n);   /* mg.F90:67-68 */
     orc_boundary_u(PSI, n);
     double rhomax = 0.0, srcmax, srcl2;
     interior_norms(SRC, n, &srcmax, &srcl2);

@@ -10,6 +10,8 @@
 #define ORC_STOP_REFERENCE 0     /* mg.F90:91  Resmaxg / rhomax < tol            */
 #define ORC_STOP_TRUE_RESIDUAL 1 /* max|src - L psi| / max|src| < tol, after update */
 #define ORC_HIST 64
+#define ORC_BC_PERIODIC 0
+#define ORC_BC_DIRICHLET 1
 
 typedef struct {
     double tol;
@@ -28,7 +30,10 @@ typedef struct {
 } orc_stats;
 
 size_t orc_ghosted_len(int n);
+void orc_set_bc(int bc);
 void orc_boundary_u(double *u, int n);
+void orc_boundary_fill0(double *u, int n);
+void orc_boundary_extrap(double *u, int n);
 double orc_relax(double *u, const double *rhs, int n, double h);
 void orc_resid(double *res, double *u, const double *rhs, int n, double h);
 void orc_rstrct_linear(double *uc, const double *uf, int nc);
@@ -49,5 +54,45 @@ void orc_rstrct_dense(double *uc, const double *uf, int nc, int cubic);
 void orc_interp_dense(double *uf, const double *uc, int nf, int cubic);
 int orc_mg_cycle_dense(double *psi, const double *src, int n, double h, const orc_params *p, orc_stats *st);
 void orc_gradient_dense(double *g, const double *u, int n, double h);
+
+/* ---- AMR (fmg.F90 / vmg.F90) ---- */
+#define ORC_AMR_PERIODIC 0
+#define ORC_AMR_SPHERE 1
+#define ORC_AMR_NCODE 6          /* U, RES, RHS, RHO, SRC, PSI (fmg_data.F90:90) */
+#define ORC_AMR_MAXLEV 16
+#define ORC_AMR_NBR_NONE 0       /* physical boundary face */
+#define ORC_AMR_NBR_SAME 1       /* NeighborSameLevel */
+#define ORC_AMR_NBR_PARENT 2     /* NeighborParentLevel */
+
+enum {
+    ORC_AMR_OP_C2P = 1, ORC_AMR_OP_GHOSTFIX, ORC_AMR_OP_FILL0, ORC_AMR_OP_EXTRAP, ORC_AMR_OP_RELAX,
+    ORC_AMR_OP_RESID, ORC_AMR_OP_RSTRCT, ORC_AMR_OP_INTERP, ORC_AMR_OP_ADDINT, ORC_AMR_OP_VMG_FAS,
+    ORC_AMR_OP_FMG_LIN, ORC_AMR_OP_SPHERE_BC, ORC_AMR_OP_REMOVE_MEAN
+};
+
+typedef struct {
+    int iterations;
+    int status;
+    double rel_linf;     /* max |src - L psi| / max |src| over leaf cells */
+    double rel_l2;       /* volume-weighted L2 ratio over leaf cells */
+    double ratio;        /* Resh2maxg / h2absmax(src) of the last iteration (fmg.F90:183) */
+    double hist[ORC_HIST];
+} orc_amr_stats;
+
+typedef struct orc_amr orc_amr;
+orc_amr *orc_amr_create(int nblk, int nlev, const int *level, const int *ijk, const int *parent,
+                        const int *child, const int *nbr, const int base_dims[3], double h0,
+                        const double x0[3], int bc);
+void orc_amr_destroy(orc_amr *s);
+void orc_amr_set_zero_average(orc_amr *s, int on);
+size_t orc_amr_block_len(const orc_amr *s, int L);
+int orc_amr_get_fill0_level(const orc_amr *s);
+void orc_amr_set_fill0_level(orc_amr *s, int v);
+double orc_amr_resh2maxg(const orc_amr *s);
+void orc_amr_set_array(orc_amr *s, int L, int code, const double *src);
+void orc_amr_get_array(const orc_amr *s, int L, int code, double *dst);
+double orc_amr_op(orc_amr *s, int op, int L, int a, int b, int c, int flag);
+void orc_amr_sphere_bc(orc_amr *s, int ju, int jrho);
+int orc_amr_solve(orc_amr *s, double *psi, const double *src, double *g, const orc_params *p, orc_amr_stats *st);
 
 #endif
