@@ -23,6 +23,13 @@ SG_PREC_MIXED = 2
 
 SG_HIST = 64
 
+SG_BC_PERIODIC = 0
+SG_BC_SPHERE = 1
+
+(SG_AMR_OP_C2P, SG_AMR_OP_GHOSTFIX, SG_AMR_OP_FILL0, SG_AMR_OP_EXTRAP, SG_AMR_OP_RELAX, SG_AMR_OP_RESID,
+ SG_AMR_OP_RSTRCT, SG_AMR_OP_INTERP, SG_AMR_OP_ADDINT, SG_AMR_OP_VMG_FAS, SG_AMR_OP_FMG_LIN,
+ SG_AMR_OP_SPHERE_BC, SG_AMR_OP_REMOVE_MEAN) = range(1, 14)
+
 
 class SgParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxit", ctypes.c_int), ("stop_mode", ctypes.c_int),
@@ -52,6 +59,7 @@ _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
 _DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
 
 # name -> (restype, argtypes); every symbol include/sg.h declares.
 SIGNATURES = {
@@ -70,6 +78,16 @@ SIGNATURES = {
     "sg_uniform_time_smoother": (_I, [_VP, _I, _I, _DP]),
     "sg_uniform_tile_info": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                   ctypes.POINTER(ctypes.c_float), _I]),
+    "sg_amr_create": (_I, [ctypes.POINTER(_VP), _I, _I, _IP, _IP, _IP, _IP, _IP, _IP, _D, _DP, _I]),
+    "sg_amr_destroy": (_I, [_VP]),
+    "sg_amr_set_zero_average": (_I, [_VP, _I]),
+    "sg_amr_solve_dev": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SgParams), ctypes.POINTER(SgStats)]),
+    "sg_amr_solve_host": (_I, [_VP, _DP, _DP, _DP, ctypes.POINTER(SgParams), ctypes.POINTER(SgStats)]),
+    "sg_amr_set_array": (_I, [_VP, _I, _I, _DP]),
+    "sg_amr_get_array": (_I, [_VP, _I, _I, _DP]),
+    "sg_amr_get_fill0_level": (_I, [_VP, _IP]),
+    "sg_amr_set_fill0_level": (_I, [_VP, _I]),
+    "sg_amr_op": (_I, [_VP, _I, _I, _I, _I, _I, _I, _DP]),
 }
 
 _LIB = None

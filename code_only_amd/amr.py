@@ -1,0 +1,112 @@
+"""Host-side mirror of the reference's AMR self-gravity interface (module ``fmg``: fmg_poisson,
+fmg.F90:63-74, with fmg_data's per-block arrays) over libsg's C ABI.
+
+Block arrays are float64, dense [nblk, 8, 8, 8] in (k, j, i) order (x fastest); the hierarchy
+comes from ``code_only_amd.amr_hierarchy``.  ``solve`` takes torch GPU tensors; ``solve_host``
+numpy arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (SG_BC_PERIODIC, SG_BC_SPHERE, SG_PREC_FP64, SG_STOP_REFERENCE, SG_STOP_TRUE_RESIDUAL)
+
+__all__ = ["AmrSolver", "SG_BC_PERIODIC", "SG_BC_SPHERE", "SG_STOP_REFERENCE", "SG_STOP_TRUE_RESIDUAL"]
+
+U, RES, RHS, RHO, SRC, PSI = range(6)
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class AmrSolver:
+    """One hierarchy with all of its FMG-level block arrays on the current GPU (fmg_data_init,
+    fmg_alloc_LoL and fmg_make_linklist, done once)."""
+
+    def __init__(self, hier, zero_average=None):
+        self.h = hier
+        self._lib = _lib.load()
+        self._keep = [np.ascontiguousarray(a, np.int32) for a in (hier.level, hier.ijk, hier.parent, hier.child,
+                                                                  hier.nbr, np.asarray(hier.base_dims))]
+        self._x0 = np.ascontiguousarray(hier.x0, np.float64)
+        ctx = ctypes.c_void_p()
+        _lib.check(self._lib.sg_amr_create(ctypes.byref(ctx), hier.nblk, hier.nlev, *[_ip(a) for a in self._keep],
+                                           float(hier.h0), _dp(self._x0), int(hier.bc)))
+        self._ctx = ctx
+        if zero_average is not None:
+            _lib.check(self._lib.sg_amr_set_zero_average(self._ctx, int(bool(zero_average))))
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.sg_amr_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _params(tol, maxit, stop_mode):
+        return _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=SG_PREC_FP64)
+
+    def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL):
+        """fmg_poisson on torch float64 GPU tensors [nblk, 8, 8, 8]; psi in place, g [nblk, 3, 8, 8, 8]."""
+        import torch
+        for t in (psi, src) + ((g,) if g is not None else ()):
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError("expected contiguous float64 GPU tensors")
+        if tuple(psi.shape) != (self.h.nblk, 8, 8, 8) or psi.shape != src.shape:
+            raise ValueError("psi and src must be [nblk, 8, 8, 8]")
+        p = self._params(tol, maxit, stop_mode)
+        st = _lib.SgStats()
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_solve_dev(self._ctx, ctypes.c_void_p(psi.data_ptr()),
+                                              ctypes.c_void_p(src.data_ptr()),
+                                              ctypes.c_void_p(g.data_ptr()) if g is not None else None,
+                                              ctypes.byref(p), ctypes.byref(st)))
+        return st.as_dict()
+
+    def solve_host(self, src, psi=None, want_g=True, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL):
+        src = np.ascontiguousarray(src, np.float64)
+        psi = np.zeros_like(src) if psi is None else np.ascontiguousarray(psi, np.float64).copy()
+        g = np.zeros((self.h.nblk, 3, 8, 8, 8)) if want_g else None
+        p = self._params(tol, maxit, stop_mode)
+        st = _lib.SgStats()
+        _lib.check(self._lib.sg_amr_solve_host(self._ctx, _dp(psi), _dp(src), _dp(g) if want_g else None,
+                                               ctypes.byref(p), ctypes.byref(st)))
+        return psi, g, st.as_dict()
+
+    # ---- operator-level access (parity tests) ----
+    def get(self, L, code):
+        n = 8 >> L
+        a = np.empty((self.h.nblk, n + 2, n + 2, n + 2))
+        _lib.check(self._lib.sg_amr_get_array(self._ctx, L, code, _dp(a)))
+        return a
+
+    def set(self, L, code, a):
+        n = 8 >> L
+        a = np.ascontiguousarray(a, np.float64).reshape(self.h.nblk, n + 2, n + 2, n + 2)
+        _lib.check(self._lib.sg_amr_set_array(self._ctx, L, code, _dp(a)))
+
+    @property
+    def fill0_level(self):
+        v = ctypes.c_int()
+        _lib.check(self._lib.sg_amr_get_fill0_level(self._ctx, ctypes.byref(v)))
+        return v.value
+
+    @fill0_level.setter
+    def fill0_level(self, v):
+        _lib.check(self._lib.sg_amr_set_fill0_level(self._ctx, int(v)))
+
+    def op(self, op, L=0, a=0, b=0, c=0, flag=0):
+        r = ctypes.c_double()
+        _lib.check(self._lib.sg_amr_op(self._ctx, op, L, a, b, c, flag, ctypes.byref(r)))
+        return r.value

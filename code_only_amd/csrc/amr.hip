@@ -1,0 +1,930 @@
+// Block-structured AMR self-gravity solve on one MI355X: host orchestration of fmg_cycle /
+// fmg_lin (fmg.F90:134-377) with the FAS coarse solve vmg_fas (vmg.F90:116-243) and the base-grid
+// mg_cycle (mg.F90:42-94), over device-resident block arrays, and the AMR part of the C ABI.
+//
+// The sequence of operations is the reference's, operation for operation; each operation is one
+// kernel over all blocks it applies to (amr_kernels.h), so FP64 results equal a sequential
+// execution of the reference's loops in the same operand order.  Block tables (levels, parents,
+// children, neighbours and the lists derived from them) are built once per hierarchy at create
+// time; the reference rebuilds them on every call (fmg.F90:155-160).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "amr_kernels.h"
+#include "base_mg.h"
+#include "sg_common.h"
+
+namespace sg {
+const char *last_error();
+}
+
+using namespace sg;
+using namespace sg::amr;
+
+namespace {
+
+constexpr int kNfmg = 3;          // FMG levels with 8^3 blocks: 8^3, 4^3, 2^3 (fmg_data.F90:115)
+constexpr int kMaxLev = 16;       // AMR levels
+constexpr int kThreads = 256;
+
+inline int nof(int L) { return 8 >> L; }
+inline long lenof(int L) { long w = nof(L) + 2; return w * w * w; }
+inline int pow2ge(int m) { int p = 1; while (p < m) p <<= 1; return p; }
+
+enum Kinds { K_SAME = 1, K_PARENT = 2, K_BOTH = 3 };
+
+} // namespace
+
+struct sg_amr {
+    int nblk = 0, nlev = 0, bc = 0, zero_average = 0;
+    int base_dims[3] = {0, 0, 0};
+    double h0 = 0.0, x0[3] = {0, 0, 0};
+    std::vector<int> level, ijk, parent, child, nbr, nkind, octant, covered, skip, lev_start;
+    std::vector<int> active, phys, cf, cf_slot, ovslot, cov;
+    std::vector<int> phys_off, cf_off, cov_off;   // per AMR level, size nlev + 1
+    int nslots = 0, nbase = 0;
+    // device tables
+    int *d_level = nullptr, *d_ijk = nullptr, *d_parent = nullptr, *d_octant = nullptr, *d_nbr = nullptr;
+    int *d_nkind = nullptr, *d_covered = nullptr, *d_skip = nullptr, *d_active = nullptr, *d_phys = nullptr;
+    int *d_cf = nullptr, *d_cf_slot = nullptr, *d_ovslot = nullptr, *d_cov = nullptr;
+    // device state
+    double *A[kNfmg][NCODE] = {};
+    double *ov[kNfmg] = {};
+    double hl[kNfmg][kMaxLev] = {}, hil[kNfmg][kMaxLev] = {};
+    double *d_h[kNfmg] = {}, *d_hi[kNfmg] = {};
+    double *d_scal = nullptr;   // [0] Resh2maxg [1] scratch max [2..4] norms [5] mean [8..23] multipole
+    double *h_scal = nullptr;   // pinned mirror
+    double *d_part = nullptr;   // tree-sum scratch
+    int P_all = 1, P_base = 1;
+    double *d_dense = nullptr;  // staging for host entry points
+    Cubic64 st{};
+    // base grid MG
+    BaseMG bm{};
+    double *bm_mem = nullptr;
+    int *d_iters = nullptr;
+    int base_mg_ok = 0;
+    int fill0_level = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+template <typename K, typename... Args> int go(sg_amr &s, K kern, long work, Args... args)
+{
+    if (work <= 0) return SG_OK;
+    unsigned grid = (unsigned)((work + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, s.stream, args...);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+template <typename K, typename... Args> int go1(sg_amr &s, K kern, Args... args)
+{
+    hipLaunchKernelGGL(kern, dim3(1), dim3(1024), 0, s.stream, args...);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+__global__ void k_set(double *p, double v) { *p = v; }
+
+int set_scalar(sg_amr &s, int idx, double v)
+{
+    hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, s.stream, s.d_scal + idx, v);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+#define SG_NDISPATCH(L, FN, ...)                        \
+    switch (L) {                                        \
+    case 0: return FN<8>(__VA_ARGS__);                  \
+    case 1: return FN<4>(__VA_ARGS__);                  \
+    default: return FN<2>(__VA_ARGS__);                 \
+    }
+
+// ---- fmg_converge ----
+template <int N> int c2p_lev_n(sg_amr &s, int L, int code, int lc, int cubic)
+{
+    int b0 = s.lev_start[lc + 1], nb = s.lev_start[lc + 2] - b0;
+    long work = (long)nb * (N / 2) * (N / 2) * (N / 2);
+    if (cubic) return go(s, k_c2p<N, true>, work, s.A[L][code], b0, nb, s.d_parent, s.d_octant, s.st);
+    return go(s, k_c2p<N, false>, work, s.A[L][code], b0, nb, s.d_parent, s.d_octant, s.st);
+}
+int c2p_lev(sg_amr &s, int L, int code, int lc, int cubic) { SG_NDISPATCH(L, c2p_lev_n, s, L, code, lc, cubic); }
+
+int c2p(sg_amr &s, int L, int code, int cubic)
+{
+    for (int lc = s.nlev - 2; lc >= 0; --lc) SG_TRY(c2p_lev(s, L, code, lc, cubic));
+    return SG_OK;
+}
+
+// ---- fmg_ghostcell ----
+template <int N> int ghost_pass_n(sg_amr &s, int l, int L, int d, int code, int cubic, int kinds)
+{
+    int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+    if (l == 0) kinds &= K_SAME;   // fmg_ghfix_parentlev returns at the base level
+    if (!kinds) return SG_OK;
+    long W = N + 2;
+    return go(s, k_ghost<N>, (long)nb * 2 * W * W, s.A[L][code], b0, nb, d, cubic, kinds, s.d_nbr, s.d_nkind,
+              s.d_octant);
+}
+int ghost_pass(sg_amr &s, int l, int L, int d, int code, int cubic, int kinds)
+{
+    SG_NDISPATCH(L, ghost_pass_n, s, l, L, d, code, cubic, kinds);
+}
+
+int ghostcell_fix(sg_amr &s, int L, int code, int cubic)
+{
+    for (int l = 0; l < s.nlev; ++l)
+        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, l, L, d, code, cubic, K_BOTH));
+    return SG_OK;
+}
+
+// ---- fmg_boundary ----
+template <int N> int fill0_faces_n(sg_amr &s, int L, int code, int l0, int l1)
+{
+    int e0 = s.phys_off[l0], ne = s.phys_off[l1] - e0;
+    long W = N + 2;
+    return go(s, k_fill0_faces<N>, (long)ne * W * W, s.A[L][code], s.d_phys + e0, ne);
+}
+int fill0_faces(sg_amr &s, int L, int code, int l0, int l1) { SG_NDISPATCH(L, fill0_faces_n, s, L, code, l0, l1); }
+
+template <int N> int extrap_faces_n(sg_amr &s, int L, int code, int l0, int l1)
+{
+    int e0 = s.phys_off[l0], ne = s.phys_off[l1] - e0;
+    long W = N + 2;
+    for (int d = 0; d < 3; ++d) SG_TRY(go(s, k_extrap_faces<N>, (long)ne * W * W, s.A[L][code], s.d_phys + e0, ne, d));
+    return SG_OK;
+}
+int extrap_faces(sg_amr &s, int L, int code, int l0, int l1) { SG_NDISPATCH(L, extrap_faces_n, s, L, code, l0, l1); }
+
+// ---- fmg_poisson: flux refresh + reflux, relax, resid ----
+template <int N> int reflux_n(sg_amr &s, int L, int code)
+{
+    int ncf = (int)s.cf.size();
+    return go(s, k_reflux<N>, (long)ncf * (N / 2) * (N / 2), s.A[L][code], s.ov[L], s.d_cf, s.d_cf_slot, ncf,
+              s.d_hi[L], s.d_level, s.d_octant);
+}
+int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, reflux_n, s, L, code); }
+
+// fmg_poisson_flux (fmg_poisson.F90:53-124): the face fluxes themselves are formed by their
+// consumers; what remains here is the ghost refresh and the reflux overrides.
+int poisson_flux(sg_amr &s, int L, int ju, int fill0flag)
+{
+    if (s.fill0_level != L) {
+        SG_TRY(c2p(s, L, ju, 0));
+        SG_TRY(ghostcell_fix(s, L, ju, 0));
+        if (fill0flag) SG_TRY(fill0_faces(s, L, ju, 0, s.nlev));
+    }
+    s.fill0_level = -1;
+    return reflux(s, L, ju);
+}
+
+template <int N> int relax_pass_n(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int nb, int ipass,
+                                  const int *ovslot, double *resmax)
+{
+    long work = (long)nb * N * N * ((N + 1) / 2);
+    return go(s, k_relax<N>, work, s.A[L][ju], (const double *)s.A[L][jrhs], list, b0, nb, ipass,
+              (const double *)s.d_h[L], (const double *)s.d_hi[L], (const int *)s.d_level, ovslot,
+              (const double *)s.ov[L], resmax);
+}
+int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int nb, int ipass, const int *ovslot,
+               double *resmax)
+{
+    SG_NDISPATCH(L, relax_pass_n, s, L, ju, jrhs, list, b0, nb, ipass, ovslot, resmax);
+}
+
+// fmg_poisson_relax (fmg_poisson.F90:130-213); Resh2maxg lands in d_scal[0]
+int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int fill0flag)
+{
+    SG_TRY(set_scalar(s, 0, 0.0));
+    for (int ipass = 1; ipass <= 2; ++ipass) {
+        SG_TRY(poisson_flux(s, L, ju, fill0flag));
+        SG_TRY(relax_pass(s, L, ju, jrhs, s.d_active, 0, (int)s.active.size(), ipass, s.d_ovslot, s.d_scal + 0));
+    }
+    return SG_OK;
+}
+
+template <int N> int resid_n(sg_amr &s, int L, int jres, int ju, int jrhs)
+{
+    int na = (int)s.active.size();
+    return go(s, k_resid<N>, (long)na * N * N * N, s.A[L][jres], (const double *)s.A[L][ju],
+              (const double *)s.A[L][jrhs], (const int *)s.d_active, na, (const double *)s.d_hi[L],
+              (const int *)s.d_level, (const int *)s.d_ovslot, (const double *)s.ov[L]);
+}
+
+int poisson_resid(sg_amr &s, int L, int jres, int ju, int jrhs, int fill0flag)
+{
+    SG_TRY(poisson_flux(s, L, ju, fill0flag));
+    SG_NDISPATCH(L, resid_n, s, L, jres, ju, jrhs);
+}
+
+// ---- fmg.F90 helpers ----
+int fmg_rstrct(sg_amr &s, int Lc, int iuc, int iuf)
+{
+    int na = (int)s.active.size();
+    if (Lc == 1) return go(s, k_rstrct<4>, (long)na * 64, s.A[1][iuc], (const double *)s.A[0][iuf], (const int *)s.d_active, na);
+    return go(s, k_rstrct<2>, (long)na * 8, s.A[2][iuc], (const double *)s.A[1][iuf], (const int *)s.d_active, na);
+}
+
+int interp_blocks(sg_amr &s, int Lf, int iuf, int iuc, int cubic)
+{
+    int na = (int)s.active.size();
+    const int *al = s.d_active;
+    if (Lf == 0) {
+        if (cubic) return go(s, k_interp<8, true>, (long)na * 512, s.A[0][iuf], (const double *)s.A[1][iuc], al, na);
+        return go(s, k_interp<8, false>, (long)na * 512, s.A[0][iuf], (const double *)s.A[1][iuc], al, na);
+    }
+    if (cubic) return go(s, k_interp<4, true>, (long)na * 64, s.A[1][iuf], (const double *)s.A[2][iuc], al, na);
+    return go(s, k_interp<4, false>, (long)na * 64, s.A[1][iuf], (const double *)s.A[2][iuc], al, na);
+}
+
+// fmg_interp (fmg.F90:786-828)
+int fmg_interp(sg_amr &s, int Lf, int iuf, int iuc, int cubic)
+{
+    const int Lc = Lf + 1;
+    SG_TRY(c2p(s, Lc, iuc, cubic));
+    SG_TRY(ghostcell_fix(s, Lc, iuc, cubic));
+    SG_TRY(extrap_faces(s, Lc, iuc, 0, s.nlev));
+    SG_TRY(interp_blocks(s, Lf, iuf, iuc, cubic));
+    SG_TRY(fill0_faces(s, Lc, iuc, 0, s.nlev));
+    return fill0_faces(s, Lf, iuf, 0, s.nlev);
+}
+
+int fmg_fill0(sg_amr &s, int L, int code)
+{
+    int na = (int)s.active.size();
+    return go(s, k_blocks_fill0, (long)na * lenof(L), s.A[L][code], (const int *)s.d_active, 0, na, (int)lenof(L));
+}
+
+int fmg_copy(sg_amr &s, int L, int out, int in, int noskip)
+{
+    int na = noskip ? s.nblk : (int)s.active.size();
+    return go(s, k_blocks_copy, (long)na * lenof(L), s.A[L][out], (const double *)s.A[L][in],
+              noskip ? (const int *)nullptr : (const int *)s.d_active, 0, na, (int)lenof(L));
+}
+
+int fmg_add(sg_amr &s, int L, int a, int da)
+{
+    int na = (int)s.active.size();
+    return go(s, k_blocks_add, (long)na * lenof(L), s.A[L][a], (const double *)s.A[L][da], (const int *)s.d_active, 0,
+              na, (int)lenof(L));
+}
+
+int fmg_addint(sg_amr &s, int Lf, int juf, int juc, int jres)
+{
+    SG_TRY(fmg_interp(s, Lf, jres, juc, 0));
+    return fmg_add(s, Lf, juf, jres);
+}
+
+// ---- vmg.F90 at the coarsest FMG level (2^3 per block) ----
+constexpr int LV = kNfmg - 1;
+constexpr long LENV = 64;
+
+int lev_copy(sg_amr &s, int l, int out, int in)
+{
+    int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+    return go(s, k_blocks_copy, (long)nb * LENV, s.A[LV][out], (const double *)s.A[LV][in], (const int *)nullptr, b0, nb,
+              (int)LENV);
+}
+
+int lev_add(sg_amr &s, int l, int a, int da, bool covered_only)
+{
+    if (covered_only) {
+        int c0 = s.cov_off[l], nc = s.cov_off[l + 1] - c0;
+        return go(s, k_blocks_add, (long)nc * LENV, s.A[LV][a], (const double *)s.A[LV][da], (const int *)(s.d_cov + c0), 0,
+                  nc, (int)LENV);
+    }
+    int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+    return go(s, k_blocks_add, (long)nb * LENV, s.A[LV][a], (const double *)s.A[LV][da], (const int *)nullptr, b0, nb,
+              (int)LENV);
+}
+
+int vmg_ghostcell(sg_amr &s, int l, int code)
+{
+    for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, l, LV, d, code, 0, K_SAME));
+    return fill0_faces(s, LV, code, l, l + 1);
+}
+
+int vmg_gridboundary(sg_amr &s, int l, int code)
+{
+    for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, l, LV, d, code, 0, K_PARENT));
+    return fill0_faces(s, LV, code, l, l + 1);
+}
+
+int vmg_relax(sg_amr &s, int l, int ju, int jrhs)
+{
+    int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+    for (int ipass = 1; ipass <= 2; ++ipass) {
+        SG_TRY(vmg_ghostcell(s, l, ju));
+        SG_TRY(relax_pass(s, LV, ju, jrhs, nullptr, b0, nb, ipass, nullptr, s.d_scal + 1));
+    }
+    return SG_OK;
+}
+
+int vmg_tau(sg_amr &s, int lc, int jtau, int ju)
+{
+    const int lf = lc + 1;
+    if (lf > s.nlev - 1) return SG_OK;
+    SG_TRY(vmg_ghostcell(s, lf, ju));
+    int b0 = s.lev_start[lf], nb = s.lev_start[lf + 1] - b0;
+    SG_TRY(go(s, k_tau_fine<2>, (long)nb * LENV, s.A[LV][jtau], (const double *)s.A[LV][ju], b0, nb, s.hil[LV][lf]));
+    SG_TRY(c2p_lev(s, LV, jtau, lc, 0));
+    SG_TRY(vmg_ghostcell(s, lc, ju));
+    int c0 = s.cov_off[lc], nc = s.cov_off[lc + 1] - c0;
+    SG_TRY(go(s, k_tau_add<2>, (long)nc * 8, s.A[LV][jtau], (const double *)s.A[LV][ju], (const int *)(s.d_cov + c0), nc,
+              s.hil[LV][lc]));
+    int e0 = s.cf_off[lf], ne = s.cf_off[lf + 1] - e0;
+    return go(s, k_tau_crop<2>, (long)ne * 16, s.A[LV][jtau], (const int *)(s.d_cf + e0), ne, (const int *)s.d_parent);
+}
+
+int vmg_interp(sg_amr &s, int l, int juf, int juc)
+{
+    SG_TRY(vmg_ghostcell(s, l - 1, juc));
+    SG_TRY(extrap_faces(s, LV, juc, l - 1, l));
+    int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+    SG_TRY(go(s, k_p2c<2>, (long)nb * 8, s.A[LV][juf], (const double *)s.A[LV][juc], b0, nb, (const int *)s.d_parent,
+              (const int *)s.d_octant));
+    return fill0_faces(s, LV, juc, l - 1, l);
+}
+
+int vmg_slvsml(sg_amr &s, int ju, int jrhs)
+{
+    if (!s.base_mg_ok) {
+        for (int it = 0; it < 10; ++it) SG_TRY(vmg_relax(s, 0, ju, jrhs));
+        return SG_OK;
+    }
+    const int NG = s.bm.n[0];
+    SG_TRY(go(s, k_base_gather<2>, (long)s.nbase * 8, s.bm.psi, s.bm.src, (const double *)s.A[LV][ju],
+              (const double *)s.A[LV][jrhs], s.nbase, (const int *)s.d_ijk, NG));
+    SG_TRY(go1(s, k_base_mg, s.bm));
+    return go(s, k_base_scatter<2>, (long)s.nbase * 8, s.A[LV][ju], (const double *)s.bm.psi, s.nbase,
+              (const int *)s.d_ijk, NG);
+}
+
+// vmg_fas (vmg.F90:116-243): Ncycle = 1, Npre = Npost = 2
+int vmg_fas(sg_amr &s, int jrho)
+{
+    if (jrho != C_RHO)
+        for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy(s, l, C_RHO, jrho));
+    SG_TRY(c2p(s, LV, C_U, 0));
+    SG_TRY(c2p(s, LV, C_RHO, 0));
+    SG_TRY(fill0_faces(s, LV, C_U, 0, s.nlev));
+    for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy(s, l, C_RHS, C_RHO));
+    for (int v = s.nlev - 1; v >= 1; --v) {
+        SG_TRY(vmg_gridboundary(s, v, C_U));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax(s, v, C_U, C_RHS));
+        // vmg_rstrct(v-1, U, cubic): per direction parent-level then same-level faces (one
+        // launch: the two write disjoint faces and read nothing the other writes)
+        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, v, LV, d, C_U, 1, K_BOTH));
+        SG_TRY(c2p_lev(s, LV, C_U, v - 1, 1));
+        SG_TRY(lev_copy(s, v - 1, C_PSI, C_U));            // IRUF is the PSI slot (vmg.F90:115)
+        SG_TRY(c2p_lev(s, LV, C_RHS, v - 1, 0));
+        SG_TRY(vmg_tau(s, v - 1, C_RES, C_U));
+        SG_TRY(lev_add(s, v - 1, C_RHS, C_RES, true));     // vmg_add_onov
+    }
+    if (s.zero_average)
+        SG_TRY(go1(s, k_zero_average<2>, s.A[LV][C_RHS], s.nbase, s.P_base, s.d_part));
+    SG_TRY(vmg_slvsml(s, C_U, C_RHS));
+    for (int v = 1; v < s.nlev; ++v) {
+        int b0 = s.lev_start[v - 1], nb = s.lev_start[v] - b0;
+        SG_TRY(go(s, k_blocks_sub, (long)nb * LENV, s.A[LV][C_RES], (const double *)s.A[LV][C_U],
+                  (const double *)s.A[LV][C_PSI], b0, nb, (int)LENV));
+        SG_TRY(vmg_interp(s, v, C_RES, C_RES));
+        SG_TRY(lev_add(s, v, C_U, C_RES, false));
+        SG_TRY(vmg_gridboundary(s, v, C_U));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax(s, v, C_U, C_RHS));
+    }
+    return SG_OK;
+}
+
+// fmg_lin (fmg.F90:270-377)
+int fmg_lin(sg_amr &s)
+{
+    const int Lmax = kNfmg - 1;
+    for (int L = 1; L <= Lmax; ++L) SG_TRY(fmg_rstrct(s, L, C_RHO, C_RHO));
+    SG_TRY(fmg_fill0(s, Lmax, C_U));
+    SG_TRY(vmg_fas(s, C_RHO));
+    for (int Lf = Lmax - 1; Lf >= 0; --Lf) {
+        SG_TRY(fmg_interp(s, Lf, C_U, C_U, 1));
+        SG_TRY(fmg_copy(s, Lf, C_RHS, C_RHO, 0));
+        for (int jc = 0; jc < 2; ++jc) {
+            for (int v = Lf; v <= Lmax - 1; ++v) {
+                for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax(s, v, C_U, C_RHS, 1));
+                SG_TRY(poisson_resid(s, v, C_RES, C_U, C_RHS, 1));
+                SG_TRY(fmg_rstrct(s, v + 1, C_RHS, C_RES));
+                SG_TRY(fmg_fill0(s, v + 1, C_U));
+                s.fill0_level = v + 1;
+            }
+            SG_TRY(vmg_fas(s, C_RHS));
+            for (int v = Lmax - 1; v >= Lf; --v) {
+                SG_TRY(fmg_addint(s, v, C_U, C_U, C_RES));
+                for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax(s, v, C_U, C_RHS, 1));
+            }
+        }
+    }
+    return c2p(s, 0, C_U, 0);
+}
+
+Geo geo(sg_amr &s) { return Geo{s.d_ijk, s.d_level, s.d_h[0], s.x0[0], s.x0[1], s.x0[2]}; }
+
+// fmg_boundary_physical_grid of the sphere module (fmg_boundary_phys_sphere.F90:80-354)
+int sphere_bc(sg_amr &s, int ju, int jrho)
+{
+    const int n = 8;
+    const int NX = s.base_dims[0] * n, NY = s.base_dims[1] * n, NZ = s.base_dims[2] * n;
+    double xmin = s.x0[0] + 0.5 * s.h0, ymin = s.x0[1] + 0.5 * s.h0, zmin = s.x0[2] + 0.5 * s.h0;
+    double xmax = s.x0[0] + ((double)(NX - 1) + 0.5) * s.h0;
+    double ymax = s.x0[1] + ((double)(NY - 1) + 0.5) * s.h0;
+    double zmax = s.x0[2] + ((double)(NZ - 1) + 0.5) * s.h0;
+    double m = xmax;
+    if (ymax < m) m = ymax;
+    if (zmax < m) m = zmax;
+    if (-xmin < m) m = -xmin;
+    if (-ymin < m) m = -ymin;
+    if (-zmin < m) m = -zmin;
+    const double rad2 = m * m;
+    Geo G = geo(s);
+    SG_TRY(go(s, k_sphere_mask<8>, (long)s.nblk * lenof(0), s.A[0][jrho], s.nblk, G, rad2));
+    const double dv = (s.h0 * s.h0) * s.h0;
+    SG_TRY(go(s, k_sphere_moments<8>, (long)s.nbase, (const double *)s.A[0][jrho], s.nbase, s.P_base, G, dv, s.d_part));
+    SG_TRY(go1(s, k_sphere_coef, s.d_part, s.nbase, s.P_base, s.d_scal + 8));
+    int ne = (int)s.phys.size();
+    return go(s, k_sphere_ghost<8>, (long)ne * 100, s.A[0][ju], (const int *)s.d_phys, ne, G,
+              (const double *)(s.d_scal + 8));
+}
+
+// fmg_make_physvar mean removal over leaf cells (fmg_data_if.F90:53-78)
+int remove_leaf_mean(sg_amr &s, int code)
+{
+    double *pm = s.d_part, *pv = s.d_part + s.P_all;
+    SG_TRY(go(s, k_leaf_mass<8>, (long)s.nblk, (const double *)s.A[0][code], s.nblk, (const int *)s.d_covered,
+              (const double *)s.d_h[0], (const int *)s.d_level, pm, pv));
+    SG_TRY(go1(s, k_tree_mass, pm, pv, s.nblk, s.P_all, s.d_scal + 5));
+    return go(s, k_sub_scalar, (long)s.nblk * lenof(0), s.A[0][code], (long)s.nblk * lenof(0),
+              (const double *)(s.d_scal + 5));
+}
+
+// norms of an FMG-level-0 array over the leaf blocks -> d_scal[2..4]
+int norms(sg_amr &s, int code)
+{
+    SG_TRY(set_scalar(s, 2, 0.0));
+    SG_TRY(set_scalar(s, 3, 0.0));
+    SG_TRY(set_scalar(s, 4, 0.0));
+    int na = (int)s.active.size();
+    return go(s, k_norms<8>, (long)na * 512, (const double *)s.A[0][code], (const int *)s.d_active, na,
+              (const double *)s.d_h[0], (const int *)s.d_level, s.d_scal + 2);
+}
+
+int read_scalars(sg_amr &s)
+{
+    SG_HIP(hipMemcpyAsync(s.h_scal, s.d_scal, 32 * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return SG_OK;
+}
+
+__global__ void k_dense_to_blocks(double *A, const double *dense, int nblk)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nblk * 512) return;
+    int b = (int)(t / 512), c = (int)(t % 512);
+    A[(long)b * 1000 + Blk<8>::gi(c % 8, (c / 8) % 8, c / 64)] = dense[t];
+}
+
+__global__ void k_blocks_to_dense(double *dense, const double *A, int nblk)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nblk * 512) return;
+    int b = (int)(t / 512), c = (int)(t % 512);
+    dense[t] = A[(long)b * 1000 + Blk<8>::gi(c % 8, (c / 8) % 8, c / 64)];
+}
+
+int zero_state(sg_amr &s)
+{
+    for (int L = 0; L < kNfmg; ++L)
+        for (int c = 0; c < NCODE; ++c)
+            SG_HIP(hipMemsetAsync(s.A[L][c], 0, sizeof(double) * s.nblk * lenof(L), s.stream));
+    s.fill0_level = -1;
+    return SG_OK;
+}
+
+// fmg_cycle (fmg.F90:134-233).  psi, src, g: device arrays, dense [nblk][8^3] / [nblk][3][8^3].
+int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params &p, sg_stats &st)
+{
+    std::memset(&st, 0, sizeof(st));
+    SG_TRY(zero_state(s));
+    SG_HIP(hipEventRecord(s.ev0, s.stream));
+    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.A[0][C_U], (const double *)psi, s.nblk));
+    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.A[0][C_RHO], src, s.nblk));
+    if (s.zero_average) SG_TRY(remove_leaf_mean(s, C_RHO));
+    SG_TRY(fmg_copy(s, 0, C_PSI, C_U, 0));
+    SG_TRY(fmg_copy(s, 0, C_SRC, C_RHO, 0));
+    SG_TRY(c2p(s, 0, C_SRC, 0));
+    SG_TRY(c2p(s, 0, C_PSI, 0));
+    if (s.bc == 1) SG_TRY(sphere_bc(s, C_PSI, C_SRC));
+    SG_TRY(norms(s, C_SRC));
+    SG_TRY(read_scalars(s));
+    const double resh2max = s.h_scal[2], srcmax = s.h_scal[3], srcsq = s.h_scal[4];
+    int nmax = p.maxit;
+    if (std::fabs(resh2max) <= 2.2250738585072014e-308) nmax = 0;
+    double prev = HUGE_VAL;
+    int it = 0, status = 1;
+    auto measure = [&](int slot) -> int {
+        SG_TRY(norms(s, C_RHO));
+        SG_TRY(read_scalars(s));
+        st.rel_linf = srcmax > 0 ? s.h_scal[3] / srcmax : 0.0;
+        st.rel_l2 = srcsq > 0 ? std::sqrt(s.h_scal[4] / srcsq) : 0.0;
+        if (slot >= 0 && slot < 64) st.hist[slot] = st.rel_linf;
+        return SG_OK;
+    };
+    for (;;) {
+        SG_TRY(poisson_resid(s, 0, C_RHO, C_PSI, C_SRC, 0));
+        SG_TRY(measure(it));
+        if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = 0; break; }
+        if (it >= nmax) break;
+        SG_TRY(fmg_lin(s));
+        SG_TRY(fmg_add(s, 0, C_PSI, C_U));
+        ++it;
+        SG_TRY(read_scalars(s));
+        double ratio = s.h_scal[0] / resh2max;
+        st.resmaxg = ratio;
+        if (p.stop_mode == SG_STOP_REFERENCE) {
+            if (ratio < p.tol) { status = 0; break; }
+            if (ratio > prev * 0.5) break;
+            prev = ratio;
+        }
+    }
+    if (nmax == 0) status = 0;
+    if (p.stop_mode == SG_STOP_REFERENCE) {
+        SG_TRY(poisson_resid(s, 0, C_RHO, C_PSI, C_SRC, 0));
+        SG_TRY(measure(-1));
+    }
+    st.iterations = it;
+    st.status = status;
+    SG_TRY(fmg_copy(s, 0, C_U, C_PSI, 1));
+    SG_TRY(c2p(s, 0, C_U, 0));
+    SG_TRY(go(s, k_blocks_to_dense, (long)s.nblk * 512, psi, (const double *)s.A[0][C_U], s.nblk));
+    if (g) {
+        SG_TRY(poisson_flux(s, 0, C_U, 0));
+        SG_TRY(go(s, k_psi2g<8>, (long)s.nblk * 512, g, (const double *)s.A[0][C_U], s.nblk, (const double *)s.d_hi[0],
+                  (const int *)s.d_level, (const int *)s.d_skip, (const int *)s.d_ovslot, (const double *)s.ov[0]));
+    }
+    SG_HIP(hipEventRecord(s.ev1, s.stream));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    st.ms = ms;
+    return SG_OK;
+}
+
+template <typename T> int upload(sg_amr &s, T **dptr, const std::vector<T> &v)
+{
+    size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+    SG_HIP(hipMalloc((void **)dptr, bytes));
+    if (!v.empty()) SG_HIP(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SG_OK;
+}
+
+void release(sg_amr *s)
+{
+    if (!s) return;
+    int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
+                 s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters};
+    for (int *q : ip) if (q) (void)hipFree(q);
+    for (int L = 0; L < kNfmg; ++L) {
+        for (int c = 0; c < NCODE; ++c) if (s->A[L][c]) (void)hipFree(s->A[L][c]);
+        if (s->ov[L]) (void)hipFree(s->ov[L]);
+        if (s->d_h[L]) (void)hipFree(s->d_h[L]);
+        if (s->d_hi[L]) (void)hipFree(s->d_hi[L]);
+    }
+    if (s->d_scal) (void)hipFree(s->d_scal);
+    if (s->h_scal) (void)hipHostFree(s->h_scal);
+    if (s->d_part) (void)hipFree(s->d_part);
+    if (s->d_dense) (void)hipFree(s->d_dense);
+    if (s->bm_mem) (void)hipFree(s->bm_mem);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, const int *parent, const int *child,
+           const int *nbr, const int base_dims[3], double h0, const double x0[3], int bc)
+{
+    SG_REQUIRE(nblk > 0 && nlev > 0 && nlev <= kMaxLev, "sg_amr_create: bad sizes nblk=%d nlev=%d", nblk, nlev);
+    SG_REQUIRE(bc == SG_BC_PERIODIC || bc == SG_BC_SPHERE, "sg_amr_create: unknown boundary %d", bc);
+    SG_REQUIRE(h0 > 0, "sg_amr_create: h0 must be positive");
+    s->nblk = nblk;
+    s->nlev = nlev;
+    s->bc = bc;
+    s->zero_average = bc == SG_BC_PERIODIC;
+    s->h0 = h0;
+    for (int a = 0; a < 3; ++a) { s->base_dims[a] = base_dims[a]; s->x0[a] = x0[a]; }
+    s->level.assign(level, level + nblk);
+    s->ijk.assign(ijk, ijk + 3 * nblk);
+    s->parent.assign(parent, parent + nblk);
+    s->child.assign(child, child + 8 * nblk);
+    s->nbr.assign(nbr, nbr + 6 * nblk);
+    s->nkind.assign(6 * nblk, NBR_NONE);
+    s->octant.assign(nblk, -1);
+    s->covered.assign(nblk, 0);
+    s->skip.assign(nblk, 0);
+    s->lev_start.assign(nlev + 2, 0);
+    for (int b = 0; b < nblk; ++b) {
+        SG_REQUIRE(level[b] >= 0 && level[b] < nlev, "sg_amr_create: block %d has level %d", b, level[b]);
+        SG_REQUIRE(b == 0 || level[b] >= level[b - 1], "sg_amr_create: blocks must be ordered by level");
+        SG_REQUIRE((level[b] == 0) == (parent[b] < 0), "sg_amr_create: block %d parent/level mismatch", b);
+        s->lev_start[level[b] + 1] = b + 1;
+        s->covered[b] = child[8 * b] >= 0;
+        if (parent[b] >= 0) {
+            SG_REQUIRE(parent[b] < nblk && level[parent[b]] == level[b] - 1, "sg_amr_create: bad parent of %d", b);
+            s->octant[b] = (ijk[3 * b] & 1) + 2 * (ijk[3 * b + 1] & 1) + 4 * (ijk[3 * b + 2] & 1);
+            SG_REQUIRE(child[8 * parent[b] + s->octant[b]] == b, "sg_amr_create: child table disagrees for %d", b);
+        }
+        for (int o = 0; o < 8; ++o)
+            SG_REQUIRE((child[8 * b + o] >= 0) == s->covered[b] && child[8 * b + o] < nblk,
+                       "sg_amr_create: block %d must have all or no children", b);
+        for (int f = 0; f < 6; ++f) {
+            int m = nbr[6 * b + f];
+            SG_REQUIRE(m < nblk, "sg_amr_create: neighbour out of range");
+            if (m < 0) {
+                SG_REQUIRE(bc == SG_BC_SPHERE, "sg_amr_create: periodic hierarchy with an open face (block %d)", b);
+                s->nkind[6 * b + f] = NBR_NONE;
+            } else if (level[m] == level[b]) {
+                s->nkind[6 * b + f] = NBR_SAME;
+            } else {
+                SG_REQUIRE(level[m] == level[b] - 1, "sg_amr_create: block %d face %d is not properly nested", b, f);
+                s->nkind[6 * b + f] = NBR_PARENT;
+            }
+        }
+    }
+    for (int l = 1; l <= nlev + 1; ++l) s->lev_start[l] = std::max(s->lev_start[l], s->lev_start[l - 1]);
+    for (int l = 0; l < nlev; ++l)
+        SG_REQUIRE(s->lev_start[l + 1] > s->lev_start[l], "sg_amr_create: level %d has no blocks", l);
+    for (int b = 0; b < nblk; ++b) {
+        s->skip[b] = level[b] != nlev - 1 && s->covered[b];
+        SG_REQUIRE(!(level[b] == nlev - 1 && s->covered[b]), "sg_amr_create: finest-level block %d has children", b);
+        if (!s->skip[b]) s->active.push_back(b);
+    }
+    s->nbase = s->lev_start[1];
+    SG_REQUIRE(s->nbase == base_dims[0] * base_dims[1] * base_dims[2], "sg_amr_create: base level must hold %d blocks",
+               base_dims[0] * base_dims[1] * base_dims[2]);
+    // per-level lists
+    s->phys_off.assign(nlev + 1, 0);
+    s->cf_off.assign(nlev + 1, 0);
+    s->cov_off.assign(nlev + 1, 0);
+    s->ovslot.assign(6 * nblk, -1);
+    for (int l = 0; l < nlev; ++l) {
+        s->phys_off[l] = (int)s->phys.size();
+        s->cf_off[l] = (int)s->cf.size();
+        s->cov_off[l] = (int)s->cov.size();
+        for (int b = s->lev_start[l]; b < s->lev_start[l + 1]; ++b) {
+            if (s->covered[b]) s->cov.push_back(b);
+            for (int f = 0; f < 6; ++f) {
+                if (s->nkind[6 * b + f] == NBR_NONE) s->phys.push_back(6 * b + f);
+                if (s->nkind[6 * b + f] == NBR_PARENT) {
+                    int m = nbr[6 * b + f], fc = 6 * m + (f ^ 1);
+                    SG_REQUIRE(!s->covered[b], "sg_amr_create: block %d meets a coarser level but has children", b);
+                    SG_REQUIRE(!s->covered[m], "sg_amr_create: coarse neighbour %d of %d has children", m, b);
+                    if (s->ovslot[fc] < 0) s->ovslot[fc] = s->nslots++;
+                    s->cf.push_back(6 * b + f);
+                    s->cf_slot.push_back(s->ovslot[fc]);
+                }
+            }
+        }
+    }
+    s->phys_off[nlev] = (int)s->phys.size();
+    s->cf_off[nlev] = (int)s->cf.size();
+    s->cov_off[nlev] = (int)s->cov.size();
+    // device tables
+    SG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    SG_HIP(hipEventCreate(&s->ev0));
+    SG_HIP(hipEventCreate(&s->ev1));
+    SG_TRY(upload(*s, &s->d_level, s->level));
+    SG_TRY(upload(*s, &s->d_ijk, s->ijk));
+    SG_TRY(upload(*s, &s->d_parent, s->parent));
+    SG_TRY(upload(*s, &s->d_octant, s->octant));
+    SG_TRY(upload(*s, &s->d_nbr, s->nbr));
+    SG_TRY(upload(*s, &s->d_nkind, s->nkind));
+    SG_TRY(upload(*s, &s->d_covered, s->covered));
+    SG_TRY(upload(*s, &s->d_skip, s->skip));
+    SG_TRY(upload(*s, &s->d_active, s->active));
+    SG_TRY(upload(*s, &s->d_phys, s->phys));
+    SG_TRY(upload(*s, &s->d_cf, s->cf));
+    SG_TRY(upload(*s, &s->d_cf_slot, s->cf_slot));
+    SG_TRY(upload(*s, &s->d_ovslot, s->ovslot));
+    SG_TRY(upload(*s, &s->d_cov, s->cov));
+    for (int L = 0; L < kNfmg; ++L) {
+        for (int c = 0; c < NCODE; ++c) SG_HIP(hipMalloc((void **)&s->A[L][c], sizeof(double) * nblk * lenof(L)));
+        SG_HIP(hipMalloc((void **)&s->ov[L], sizeof(double) * std::max(1, s->nslots) * nof(L) * nof(L)));
+        for (int l = 0; l < nlev; ++l) {
+            s->hl[L][l] = std::ldexp(h0, L - l);          // fmg_data.F90:288 CellWidth * 2^fmglev
+            s->hil[L][l] = 1.0 / s->hl[L][l];
+        }
+        std::vector<double> hv(s->hl[L], s->hl[L] + kMaxLev), hiv(s->hil[L], s->hil[L] + kMaxLev);
+        SG_TRY(upload(*s, &s->d_h[L], hv));
+        SG_TRY(upload(*s, &s->d_hi[L], hiv));
+    }
+    SG_HIP(hipMalloc((void **)&s->d_scal, 32 * sizeof(double)));
+    SG_HIP(hipMemset(s->d_scal, 0, 32 * sizeof(double)));
+    SG_HIP(hipHostMalloc((void **)&s->h_scal, 32 * sizeof(double), hipHostMallocDefault));
+    s->P_all = pow2ge(nblk);
+    s->P_base = pow2ge(s->nbase);
+    SG_HIP(hipMalloc((void **)&s->d_part, sizeof(double) * std::max<long>(2L * s->P_all, 16L * s->P_base)));
+    SG_HIP(hipMalloc((void **)&s->d_iters, sizeof(int)));
+    // cubic restriction weights St (restriction.F90:38-63)
+    {
+        const double ic = 1.5;
+        const int ii[4] = {0, 1, 2, 3};
+        double s1[4];
+        s1[0] = (ic - ii[1]) * (ic - ii[2]) * (ic - ii[3]) / (double)((ii[0] - ii[1]) * (ii[0] - ii[2]) * (ii[0] - ii[3]));
+        s1[1] = (ic - ii[0]) * (ic - ii[2]) * (ic - ii[3]) / (double)((ii[1] - ii[0]) * (ii[1] - ii[2]) * (ii[1] - ii[3]));
+        s1[2] = (ic - ii[0]) * (ic - ii[1]) * (ic - ii[3]) / (double)((ii[2] - ii[0]) * (ii[2] - ii[1]) * (ii[2] - ii[3]));
+        s1[3] = (ic - ii[0]) * (ic - ii[1]) * (ic - ii[2]) / (double)((ii[3] - ii[0]) * (ii[3] - ii[1]) * (ii[3] - ii[2]));
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                for (int c = 0; c < 4; ++c) s->st.w[(c * 4 + b) * 4 + a] = (s1[a] * s1[b]) * s1[c];
+    }
+    // base-grid MG (vmg_slvsml): power-of-two cube of base blocks, else 10 base relaxations
+    const int bx = base_dims[0];
+    s->base_mg_ok = bx == base_dims[1] && bx == base_dims[2] && (bx & (bx - 1)) == 0;
+    if (s->base_mg_ok) {
+        const int NG = bx * 2;
+        int nl = 0;
+        while ((1 << nl) < NG) ++nl;
+        nl += 1;
+        SG_REQUIRE(nl <= kBaseMaxLev, "sg_amr_create: base grid %d too large", NG);
+        long total = 2L * (NG + 2) * (NG + 2) * (NG + 2);
+        for (int l = 0; l < nl; ++l) { long w = (NG >> l) + 2; total += 4 * w * w * w; }
+        SG_HIP(hipMalloc((void **)&s->bm_mem, sizeof(double) * total));
+        SG_HIP(hipMemset(s->bm_mem, 0, sizeof(double) * total));
+        double *q = s->bm_mem;
+        BaseMG &M = s->bm;
+        M.nlev = nl;
+        M.dirichlet = bc == SG_BC_SPHERE;
+        M.tol = (double)1.e-2f;   // mg.F90:50: the single-precision literal 1.e-2 widened to double
+        M.maxit = 10;             // nmg (mg.F90:52)
+        M.iters = s->d_iters;
+        long w0 = NG + 2;
+        M.psi = q; q += w0 * w0 * w0;
+        M.src = q; q += w0 * w0 * w0;
+        for (int l = 0; l < nl; ++l) {
+            M.n[l] = NG >> l;
+            M.h[l] = s->hl[LV][0] * (double)(1 << l);   // mg_data.F90:554-559
+            long w = M.n[l] + 2, len = w * w * w;
+            M.U[l] = q; q += len;
+            M.RHS[l] = q; q += len;
+            M.RES[l] = q; q += len;
+            M.RHO[l] = q; q += len;
+        }
+    }
+    SG_HIP(hipDeviceSynchronize());
+    return SG_OK;
+}
+
+int op(sg_amr &s, int opc, int L, int a, int b, int c, int flag, double *ret)
+{
+    SG_REQUIRE(L >= 0 && L < kNfmg, "sg_amr_op: FMG level %d out of range", L);
+    SG_REQUIRE(a >= 0 && a < NCODE && b >= 0 && b < NCODE && c >= 0 && c < NCODE, "sg_amr_op: bad array code");
+    double r = 0.0;
+    switch (opc) {
+    case SG_AMR_OP_C2P: SG_TRY(c2p(s, L, a, flag)); break;
+    case SG_AMR_OP_GHOSTFIX: SG_TRY(ghostcell_fix(s, L, a, flag)); break;
+    case SG_AMR_OP_FILL0: SG_TRY(fill0_faces(s, L, a, 0, s.nlev)); break;
+    case SG_AMR_OP_EXTRAP: SG_TRY(extrap_faces(s, L, a, 0, s.nlev)); break;
+    case SG_AMR_OP_RELAX:
+        SG_TRY(poisson_relax(s, L, a, b, flag));
+        SG_TRY(read_scalars(s));
+        r = s.h_scal[0];
+        break;
+    case SG_AMR_OP_RESID: SG_TRY(poisson_resid(s, L, a, b, c, flag)); break;
+    case SG_AMR_OP_RSTRCT:
+        SG_REQUIRE(L >= 1, "sg_amr_op: restriction needs FMG level >= 1");
+        SG_TRY(fmg_rstrct(s, L, a, b));
+        break;
+    case SG_AMR_OP_INTERP:
+        SG_REQUIRE(L <= 1, "sg_amr_op: interpolation needs FMG level <= 1");
+        SG_TRY(fmg_interp(s, L, a, b, flag));
+        break;
+    case SG_AMR_OP_ADDINT:
+        SG_REQUIRE(L <= 1, "sg_amr_op: interpolation needs FMG level <= 1");
+        SG_TRY(fmg_addint(s, L, a, b, c));
+        break;
+    case SG_AMR_OP_VMG_FAS: SG_TRY(vmg_fas(s, a)); break;
+    case SG_AMR_OP_FMG_LIN:
+        SG_TRY(fmg_lin(s));
+        SG_TRY(read_scalars(s));
+        r = s.h_scal[0];
+        break;
+    case SG_AMR_OP_SPHERE_BC:
+        SG_REQUIRE(s.bc == SG_BC_SPHERE, "sg_amr_op: not a sphere hierarchy");
+        SG_TRY(sphere_bc(s, a, b));
+        break;
+    case SG_AMR_OP_REMOVE_MEAN: SG_TRY(remove_leaf_mean(s, a)); break;
+    default: SG_REQUIRE(false, "sg_amr_op: unknown operation %d", opc);
+    }
+    SG_HIP(hipStreamSynchronize(s.stream));
+    if (ret) *ret = r;
+    return SG_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int sg_amr_create(sg_amr **ctx, int nblk, int nlev, const int *level, const int *ijk, const int *parent,
+                  const int *child, const int *nbr, const int *base_dims, double h0, const double *x0, int bc)
+{
+    SG_REQUIRE(ctx && level && ijk && parent && child && nbr && base_dims && x0, "sg_amr_create: null argument");
+    *ctx = nullptr;
+    sg_amr *s = new sg_amr();
+    int rc = create(s, nblk, nlev, level, ijk, parent, child, nbr, base_dims, h0, x0, bc);
+    if (rc != SG_OK) {
+        release(s);
+        return rc;
+    }
+    *ctx = s;
+    return SG_OK;
+}
+
+int sg_amr_destroy(sg_amr *ctx)
+{
+    release(ctx);
+    return SG_OK;
+}
+
+int sg_amr_set_zero_average(sg_amr *ctx, int on)
+{
+    SG_REQUIRE(ctx, "sg_amr_set_zero_average: null context");
+    ctx->zero_average = on ? 1 : 0;
+    return SG_OK;
+}
+
+int sg_amr_solve_dev(sg_amr *ctx, double *psi, const double *src, double *g, const sg_params *p, sg_stats *st)
+{
+    SG_REQUIRE(ctx && psi && src && p && st, "sg_amr_solve_dev: null argument");
+    SG_REQUIRE(p->precision == SG_PREC_FP64, "sg_amr_solve_dev: only SG_PREC_FP64 is provided for AMR hierarchies");
+    return solve(*ctx, psi, src, g, *p, *st);
+}
+
+int sg_amr_solve_host(sg_amr *ctx, double *psi, const double *src, double *g, const sg_params *p, sg_stats *st)
+{
+    SG_REQUIRE(ctx && psi && src && p && st, "sg_amr_solve_host: null argument");
+    sg_amr &s = *ctx;
+    const size_t n = (size_t)s.nblk * 512;
+    if (!s.d_dense) SG_HIP(hipMalloc((void **)&s.d_dense, sizeof(double) * 5 * n));
+    double *dpsi = s.d_dense, *dsrc = dpsi + n, *dg = dsrc + n;
+    SG_HIP(hipMemcpyAsync(dpsi, psi, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
+    SG_HIP(hipMemcpyAsync(dsrc, src, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
+    SG_TRY(sg_amr_solve_dev(ctx, dpsi, dsrc, g ? dg : nullptr, p, st));
+    SG_HIP(hipMemcpyAsync(psi, dpsi, sizeof(double) * n, hipMemcpyDeviceToHost, s.stream));
+    if (g) SG_HIP(hipMemcpyAsync(g, dg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s.stream));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return SG_OK;
+}
+
+int sg_amr_set_array(sg_amr *ctx, int fmglev, int code, const double *host)
+{
+    SG_REQUIRE(ctx && host, "sg_amr_set_array: null argument");
+    SG_REQUIRE(fmglev >= 0 && fmglev < kNfmg && code >= 0 && code < NCODE, "sg_amr_set_array: bad level/code");
+    SG_HIP(hipMemcpyAsync(ctx->A[fmglev][code], host, sizeof(double) * ctx->nblk * lenof(fmglev), hipMemcpyHostToDevice,
+                          ctx->stream));
+    SG_HIP(hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+
+int sg_amr_get_array(sg_amr *ctx, int fmglev, int code, double *host)
+{
+    SG_REQUIRE(ctx && host, "sg_amr_get_array: null argument");
+    SG_REQUIRE(fmglev >= 0 && fmglev < kNfmg && code >= 0 && code < NCODE, "sg_amr_get_array: bad level/code");
+    SG_HIP(hipMemcpyAsync(host, ctx->A[fmglev][code], sizeof(double) * ctx->nblk * lenof(fmglev), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    SG_HIP(hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+
+int sg_amr_get_fill0_level(sg_amr *ctx, int *v)
+{
+    SG_REQUIRE(ctx && v, "sg_amr_get_fill0_level: null argument");
+    *v = ctx->fill0_level;
+    return SG_OK;
+}
+
+int sg_amr_set_fill0_level(sg_amr *ctx, int v)
+{
+    SG_REQUIRE(ctx, "sg_amr_set_fill0_level: null context");
+    ctx->fill0_level = v;
+    return SG_OK;
+}
+
+int sg_amr_op(sg_amr *ctx, int opc, int fmglev, int a, int b, int c, int flag, double *ret)
+{
+    SG_REQUIRE(ctx, "sg_amr_op: null context");
+    return op(*ctx, opc, fmglev, a, b, c, flag, ret);
+}
+
+} // extern "C"

@@ -1,0 +1,741 @@
+// Device kernels of the block-structured AMR solve (fmg.F90 / vmg.F90 / fmg_poisson.F90 /
+// fmg_ghostcell.F90 / fmg_converge.F90 / fmg_reflux.F90 / fmg_boundary*.F90), FP64.
+//
+// Every block keeps, per FMG level, one array per code of (N+2)^3 doubles (N = 8, 4, 2 cells per
+// side plus one ghost layer, i fastest): element (i, j, k), i, j, k in -1..N, sits at
+// ((k+1)(N+2) + j+1)(N+2) + i+1.  All arrays of one code and FMG level are contiguous, block b at
+// offset b (N+2)^3; blocks are numbered level by level, coarsest first.
+//
+// Each kernel performs one loop nest of the reference over all blocks it applies to, one thread
+// per output element, with the reference's operand order (the library is built without
+// multiply-add contraction), so results are bitwise those of a sequential execution.  Face fluxes
+// are not stored: every consumer recomputes F(i) = (u(i+1) - u(i)) / h from the block's own values,
+// except on coarse faces that meet a finer level, whose fluxes are the 2x2 averages of the finer
+// block's face fluxes (fmg_reflux.F90), kept in a small per-face "override" array.
+#pragma once
+#include "sg_common.h"
+
+namespace sg {
+namespace amr {
+
+enum { C_U = 0, C_RES = 1, C_RHS = 2, C_RHO = 3, C_SRC = 4, C_PSI = 5, NCODE = 6 };
+enum { NBR_NONE = 0, NBR_SAME = 1, NBR_PARENT = 2 };
+
+template <int N> struct Blk {
+    static constexpr int W = N + 2;
+    static constexpr int LEN = W * W * W;
+    static constexpr int H = N / 2;
+    __device__ static constexpr int gi(int i, int j, int k) { return ((k + 1) * W + (j + 1)) * W + (i + 1); }
+    // element at coordinate a along d and (t1, t2) along the two other axes in x, y, z order
+    __device__ static int rot(int d, int a, int t1, int t2)
+    {
+        return d == 0 ? gi(a, t1, t2) : (d == 1 ? gi(t1, a, t2) : gi(t1, t2, a));
+    }
+};
+
+__device__ __forceinline__ int ijkc(int i) { return i >= 0 ? i / 2 : -((1 - i) / 2); }   // floor(i/2)
+__device__ __forceinline__ int near_(int f, int c) { return c + 2 * (((f % 2) + 2) % 2) - 1; }
+
+// interpolation.F90:106-127: cubic Lagrange stencil of fine index f over coarse bounds lo..hi.
+__device__ __forceinline__ void cubic_stencil(int f, int lo, int hi, int idx[4], double a[4])
+{
+    int cp = ijkc(f), cn = near_(f, cp);
+    int c1 = cp < cn ? cp : cn, c2 = cp < cn ? cn : cp;
+    int c0 = 2 * c1 - c2, c3 = 2 * c2 - c1;
+    int ns = 0;
+    if (c3 > hi) ns = c3 - hi;
+    if (c0 < lo) ns = c0 - lo;
+    c0 -= ns; c1 -= ns; c2 -= ns; c3 -= ns;
+    double xc = cp + 0.25 * (double)(2 * (((f % 2) + 2) % 2) - 1);
+    a[0] = (xc - c1) * (xc - c2) * (xc - c3) / (double)((c0 - c1) * (c0 - c2) * (c0 - c3));
+    a[1] = (xc - c0) * (xc - c2) * (xc - c3) / (double)((c1 - c0) * (c1 - c2) * (c1 - c3));
+    a[2] = (xc - c0) * (xc - c1) * (xc - c3) / (double)((c2 - c0) * (c2 - c1) * (c2 - c3));
+    a[3] = (xc - c0) * (xc - c1) * (xc - c2) / (double)((c3 - c0) * (c3 - c1) * (c3 - c2));
+    idx[0] = c0; idx[1] = c1; idx[2] = c2; idx[3] = c3;
+}
+
+__device__ __forceinline__ double comb4(const double a[4], double v0, double v1, double v2, double v3)
+{
+    double s = a[0] * v0 + a[1] * v1;
+    s = s + a[2] * v2;
+    s = s + a[3] * v3;
+    return s;
+}
+
+struct Cubic64 { double w[64]; };   // St(a, b, c) of restriction.F90:38-63 at w[(c*4 + b)*4 + a]
+
+// restriction.F90:127-151
+template <int N> __device__ __forceinline__ double rstr_lin(const double *uf, int i, int j, int k)
+{
+    using B = Blk<N>;
+    double s = uf[B::gi(i, j, k)] + uf[B::gi(i + 1, j, k)];
+    s = s + uf[B::gi(i, j + 1, k)];
+    s = s + uf[B::gi(i, j, k + 1)];
+    s = s + uf[B::gi(i + 1, j + 1, k)];
+    s = s + uf[B::gi(i + 1, j, k + 1)];
+    s = s + uf[B::gi(i, j + 1, k + 1)];
+    s = s + uf[B::gi(i + 1, j + 1, k + 1)];
+    return s * (1.0 / 8.0);
+}
+
+// restriction.F90:71-119 (stencil i1-1 .. i1+2 per axis)
+template <int N> __device__ __forceinline__ double rstr_cub(const double *uf, int i1, int j1, int k1, const Cubic64 &st)
+{
+    using B = Blk<N>;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                double t = uf[B::gi(i1 - 1 + a, j1 - 1 + b, k1 - 1 + c)] * st.w[(c * 4 + b) * 4 + a];
+                s = (a == 0 && b == 0 && c == 0) ? t : s + t;
+            }
+    return s;
+}
+
+// interpolation.F90:222-247 on a coarse array addressed through cget(ic, jc, kc)
+template <typename G> __device__ __forceinline__ double trilinear_at(int i, int j, int k, G cget)
+{
+    int ic = ijkc(i), jc = ijkc(j), kc = ijkc(k);
+    int ic1 = near_(i, ic), jc1 = near_(j, jc), kc1 = near_(k, kc);
+    double b = cget(ic1, jc, kc) + cget(ic, jc1, kc);
+    b = b + cget(ic, jc, kc1);
+    double e = cget(ic, jc1, kc1) + cget(ic1, jc, kc1);
+    e = e + cget(ic1, jc1, kc);
+    double s = 27.0 * cget(ic, jc, kc) + 9.0 * b;
+    s = s + 3.0 * e;
+    s = s + cget(ic1, jc1, kc1);
+    return s / 64.0;
+}
+
+// interpolation.F90:40-218 evaluated for one fine cell: x pass (bufx), y pass (bufxy), z pass,
+// each intermediate formed exactly as the three-pass loop forms it.  Coarse bounds -1..nc.
+template <typename G> __device__ __forceinline__ double tricubic_at(int i, int j, int k, int nc, G cget)
+{
+    int ix[4], jx[4], kx[4];
+    double ax[4], ay[4], az[4];
+    cubic_stencil(i, -1, nc, ix, ax);
+    cubic_stencil(j, -1, nc, jx, ay);
+    cubic_stencil(k, -1, nc, kx, az);
+    double bxy[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double bx[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            bx[b] = comb4(ax, cget(ix[0], jx[b], kx[c]), cget(ix[1], jx[b], kx[c]), cget(ix[2], jx[b], kx[c]),
+                          cget(ix[3], jx[b], kx[c]));
+        bxy[c] = comb4(ay, bx[0], bx[1], bx[2], bx[3]);
+    }
+    return comb4(az, bxy[0], bxy[1], bxy[2], bxy[3]);
+}
+
+// ---- fmg_converge_c2p_lev: each child of [b0, b0+nb) restricted into its parent's octant ----
+template <int N, bool CUBIC>
+__global__ void k_c2p(double *A, int b0, int nb, const int *parent, const int *octant, Cubic64 st)
+{
+    using B = Blk<N>;
+    constexpr int H = N / 2, HC = H * H * H;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * HC) return;
+    int b = b0 + (int)(t / HC), c = (int)(t % HC);
+    int ic = c % H, jc = (c / H) % H, kc = c / (H * H);
+    const double *uf = A + (long)b * B::LEN;
+    double *uc = A + (long)parent[b] * B::LEN;
+    int o = octant[b];
+    int oi = (o & 1) * H, oj = ((o >> 1) & 1) * H, ok = ((o >> 2) & 1) * H;
+    uc[B::gi(oi + ic, oj + jc, ok + kc)] = CUBIC ? rstr_cub<N>(uf, 2 * ic, 2 * jc, 2 * kc, st)
+                                                 : rstr_lin<N>(uf, 2 * ic, 2 * jc, 2 * kc);
+}
+
+// ---- one direction of fmg_ghostcell_fix for the blocks [b0, b0+nb) of one AMR level:
+//      same-level faces copied (fmg_ghfix_samelev), parent-level faces interpolated
+//      (fmg_ghfix_parentlev: bilinear or separable cubic tangentially, quadratic normally) ----
+template <int N>
+__global__ void k_ghost(double *A, int b0, int nb, int d, int cubic, int kinds, const int *nbr, const int *nkind,
+                        const int *octant)
+{
+    using B = Blk<N>;
+    constexpr int W = B::W, H = N / 2;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * 2 * W * W) return;
+    int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1, lr = (int)((t / (W * W)) & 1);
+    int b = b0 + (int)(t / (2 * W * W));
+    int f = 2 * d + lr;
+    int kind = nkind[b * 6 + f];
+    if (kind == NBR_NONE || (kind == NBR_SAME && !(kinds & 1)) || (kind == NBR_PARENT && !(kinds & 2))) return;
+    double *ud = A + (long)b * B::LEN;
+    const double *us = A + (long)nbr[b * 6 + f] * B::LEN;
+    const int ag = lr ? N : -1;
+    if (kind == NBR_SAME) {
+        ud[B::rot(d, ag, t1, t2)] = us[B::rot(d, lr ? 0 : N - 1, t1, t2)];
+        return;
+    }
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int o = octant[b];
+    const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
+    const int an = lr ? 0 : N - 1;
+    auto buf = [&](int c1, int c2) { return us[B::rot(d, an, c1 + e1, c2 + e2)]; };
+    double uH;
+    if (cubic && H + 2 >= 4) {
+        int i1[4], i2[4];
+        double a1[4], a2[4];
+        cubic_stencil(t1, -1, H, i1, a1);
+        cubic_stencil(t2, -1, H, i2, a2);
+        double bt[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            bt[q] = comb4(a1, buf(i1[0], i2[q]), buf(i1[1], i2[q]), buf(i1[2], i2[q]), buf(i1[3], i2[q]));
+        uH = comb4(a2, bt[0], bt[1], bt[2], bt[3]);
+    } else {
+        int c1 = ijkc(t1), c2 = ijkc(t2);
+        int c1n = near_(t1, c1), c2n = near_(t2, c2);
+        double S = 9.0 * buf(c1, c2) + buf(c1n, c2n);
+        S = S + 3.0 * buf(c1, c2n);
+        S = S + 3.0 * buf(c1n, c2);
+        uH = S / 16.0;
+    }
+    const int inner = lr ? -1 : 1;
+    double u0 = ud[B::rot(d, ag + inner, t1, t2)];
+    double uN = ud[B::rot(d, ag + 2 * inner, t1, t2)];
+    double q = 10.0 * u0 + 8.0 * uH;
+    q = q - 3.0 * uN;
+    ud[B::rot(d, ag, t1, t2)] = q / 15.0;
+}
+
+// ---- fmg_boundary_fill0 on a list of physical faces (entries b*6+f) ----
+template <int N> __global__ void k_fill0_faces(double *A, const int *faces, int nf)
+{
+    using B = Blk<N>;
+    constexpr int W = B::W;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nf * W * W) return;
+    int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1;
+    int e = faces[t / (W * W)];
+    int b = e / 6, f = e % 6, d = f / 2;
+    A[(long)b * B::LEN + B::rot(d, (f & 1) ? N : -1, t1, t2)] = 0.0;
+}
+
+// ---- fmg_boundary_extrap, the faces of direction d only ----
+template <int N> __global__ void k_extrap_faces(double *A, const int *faces, int nf, int d)
+{
+    using B = Blk<N>;
+    constexpr int W = B::W;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nf * W * W) return;
+    int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1;
+    int e = faces[t / (W * W)];
+    int b = e / 6, f = e % 6;
+    if (f / 2 != d) return;
+    double *u = A + (long)b * B::LEN;
+    u[B::rot(d, (f & 1) ? N : -1, t1, t2)] = -u[B::rot(d, (f & 1) ? N - 1 : 0, t1, t2)];
+}
+
+// ---- fmg_reflux_do: fine face fluxes averaged 2x2 into the coarse neighbour's override ----
+template <int N>
+__global__ void k_reflux(const double *A, double *ov, const int *cf, const int *cf_slot, int ncf,
+                         const double *hi_lev, const int *level, const int *octant)
+{
+    using B = Blk<N>;
+    constexpr int H = N / 2;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ncf * H * H) return;
+    int c1 = (int)(t % H), c2 = (int)((t / H) % H), e = (int)(t / (H * H));
+    int b = cf[e] / 6, f = cf[e] % 6, d = f / 2, lr = f & 1;
+    const double hi = hi_lev[level[b]];
+    const double *u = A + (long)b * B::LEN;
+    const int af = lr ? N - 1 : -1;
+    auto ff = [&](int a1, int a2) { return (u[B::rot(d, af + 1, a1, a2)] - u[B::rot(d, af, a1, a2)]) * hi; };
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int o = octant[b];
+    const int off1 = ((o >> ta) & 1) * H, off2 = ((o >> tb) & 1) * H;
+    int f1 = 2 * c1, f2 = 2 * c2;
+    double v = ff(f1, f2) + ff(f1, f2 + 1);
+    v = v + ff(f1 + 1, f2);
+    v = v + ff(f1 + 1, f2 + 1);
+    ov[(long)cf_slot[e] * N * N + (off1 + c1) + N * (off2 + c2)] = v * 0.25;
+}
+
+// Face-flux view of one block: fluxes from the block's values, or the override on reflux faces.
+template <int N> struct Flux {
+    const double *u;
+    const int *slot;     // ovslot + 6 b, or nullptr
+    const double *ov;
+    double hi;
+    __device__ double ov_or(int f, int t1, int t2, double computed) const
+    {
+        if (slot) {
+            int s = slot[f];
+            if (s >= 0) return ov[(long)s * N * N + t1 + N * t2];
+        }
+        return computed;
+    }
+    // sum of flux differences at (i, j, k) in the reference's operand order
+    __device__ double div(int i, int j, int k) const
+    {
+        using B = Blk<N>;
+        const double uc = u[B::gi(i, j, k)];
+        double fxp = (i == N - 1) ? ov_or(1, j, k, (u[B::gi(i + 1, j, k)] - uc) * hi) : (u[B::gi(i + 1, j, k)] - uc) * hi;
+        double fxm = (i == 0) ? ov_or(0, j, k, (uc - u[B::gi(i - 1, j, k)]) * hi) : (uc - u[B::gi(i - 1, j, k)]) * hi;
+        double fyp = (j == N - 1) ? ov_or(3, i, k, (u[B::gi(i, j + 1, k)] - uc) * hi) : (u[B::gi(i, j + 1, k)] - uc) * hi;
+        double fym = (j == 0) ? ov_or(2, i, k, (uc - u[B::gi(i, j - 1, k)]) * hi) : (uc - u[B::gi(i, j - 1, k)]) * hi;
+        double fzp = (k == N - 1) ? ov_or(5, i, j, (u[B::gi(i, j, k + 1)] - uc) * hi) : (u[B::gi(i, j, k + 1)] - uc) * hi;
+        double fzm = (k == 0) ? ov_or(4, i, j, (uc - u[B::gi(i, j, k - 1)]) * hi) : (uc - u[B::gi(i, j, k - 1)]) * hi;
+        double v = fxp - fxm;
+        v = v + fyp;
+        v = v - fym;
+        v = v + fzp;
+        v = v - fzm;
+        return v;
+    }
+};
+
+__device__ __forceinline__ int pick_block(const int *list, int b0, int i) { return list ? list[i] : b0 + i; }
+
+// ---- fmg_poisson_relax / vmg_poisson_relax: one colour pass over the listed blocks ----
+template <int N>
+__global__ void k_relax(double *Au, const double *Arhs, const int *list, int b0, int nb, int ipass,
+                        const double *h_lev, const double *hi_lev, const int *level, const int *ovslot,
+                        const double *ov, double *resmax)
+{
+    using B = Blk<N>;
+    constexpr int HN = (N + 1) / 2, PER = N * N * HN;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    if (t < (long)nb * PER) {
+        int ih = (int)(t % HN), j = (int)((t / HN) % N), k = (int)((t / (HN * N)) % N);
+        int b = pick_block(list, b0, (int)(t / PER));
+        int i = 2 * ih + ((j + k + ipass) & 1);
+        if (i < N) {
+            const int l = level[b];
+            const double h = h_lev[l];
+            double *u = Au + (long)b * B::LEN;
+            Flux<N> F{u, ovslot ? ovslot + 6 * b : nullptr, ov, hi_lev[l]};
+            double r = F.div(i, j, k) - Arhs[(long)b * B::LEN + B::gi(i, j, k)] * h;
+            double resh2 = r * h;
+            u[B::gi(i, j, k)] = u[B::gi(i, j, k)] + (1.0 / 6.0) * resh2;
+            a = fabs(resh2);
+        }
+    }
+    block_atomic_max(resmax, a);
+}
+
+// ---- fmg_poisson_resid: res = rhs - div F / h on the listed blocks ----
+template <int N>
+__global__ void k_resid(double *Ares, const double *Au, const double *Arhs, const int *list, int nb,
+                        const double *hi_lev, const int *level, const int *ovslot, const double *ov)
+{
+    using B = Blk<N>;
+    constexpr int PER = N * N * N;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * PER) return;
+    int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
+    int b = list[t / PER];
+    const int l = level[b];
+    Flux<N> F{Au + (long)b * B::LEN, ovslot + 6 * b, ov, hi_lev[l]};
+    const long q = (long)b * B::LEN + B::gi(i, j, k);
+    Ares[q] = Arhs[q] - F.div(i, j, k) * hi_lev[l];
+}
+
+// ---- fmg_rstrct (linear): coarse level with NC cells from the fine level with 2 NC ----
+template <int NC> __global__ void k_rstrct(double *Ac, const double *Af, const int *list, int nb)
+{
+    constexpr int PER = NC * NC * NC;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * PER) return;
+    int i = (int)(t % NC), j = (int)((t / NC) % NC), k = (int)((t / (NC * NC)) % NC);
+    int b = list[t / PER];
+    Ac[(long)b * Blk<NC>::LEN + Blk<NC>::gi(i, j, k)] = rstr_lin<2 * NC>(Af + (long)b * Blk<2 * NC>::LEN, 2 * i, 2 * j, 2 * k);
+}
+
+// ---- fmg_interp_cubic / fmg_interp_linear: fine interior of each listed block from its own
+//      coarser FMG level (ghosts included) ----
+template <int NF, bool CUBIC> __global__ void k_interp(double *Af, const double *Ac, const int *list, int nb)
+{
+    constexpr int NC = NF / 2, PER = NF * NF * NF;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * PER) return;
+    int i = (int)(t % NF), j = (int)((t / NF) % NF), k = (int)((t / (NF * NF)) % NF);
+    int b = list[t / PER];
+    const double *uc = Ac + (long)b * Blk<NC>::LEN;
+    auto cget = [&](int a, int bb, int c) { return uc[Blk<NC>::gi(a, bb, c)]; };
+    double v = (CUBIC && NC + 2 >= 4) ? tricubic_at(i, j, k, NC, cget) : trilinear_at(i, j, k, cget);
+    Af[(long)b * Blk<NF>::LEN + Blk<NF>::gi(i, j, k)] = v;
+}
+
+// ---- whole-array block operations (ghosts included) ----
+__global__ void k_blocks_fill0(double *A, const int *list, int b0, int nb, int len)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * len) return;
+    A[(long)pick_block(list, b0, (int)(t / len)) * len + t % len] = 0.0;
+}
+
+__global__ void k_blocks_copy(double *O, const double *I, const int *list, int b0, int nb, int len)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * len) return;
+    long q = (long)pick_block(list, b0, (int)(t / len)) * len + t % len;
+    O[q] = I[q];
+}
+
+__global__ void k_blocks_add(double *A, const double *D, const int *list, int b0, int nb, int len)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * len) return;
+    long q = (long)pick_block(list, b0, (int)(t / len)) * len + t % len;
+    A[q] = A[q] + D[q];
+}
+
+__global__ void k_blocks_sub(double *O, const double *A, const double *C, int b0, int nb, int len)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * len) return;
+    long q = (long)(b0 + t / len) * len + t % len;
+    O[q] = A[q] - C[q];
+}
+
+// ---- vmg_poisson_tau, fine part: tau = 0, then tau = -(div F)/h inside, all blocks of a level ----
+template <int N> __global__ void k_tau_fine(double *Atau, const double *Au, int b0, int nb, double hi)
+{
+    using B = Blk<N>;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * B::LEN) return;
+    int b = b0 + (int)(t / B::LEN), q = (int)(t % B::LEN);
+    int i = q % B::W - 1, j = (q / B::W) % B::W - 1, k = q / (B::W * B::W) - 1;
+    double v = 0.0;
+    if (i >= 0 && i < N && j >= 0 && j < N && k >= 0 && k < N) {
+        Flux<N> F{Au + (long)b * B::LEN, nullptr, nullptr, hi};
+        v = -(F.div(i, j, k) * hi);
+    }
+    Atau[(long)b * B::LEN + q] = v;
+}
+
+// ---- vmg_poisson_tau, coarse part: tau += (div F)/h on the listed (covered) blocks ----
+template <int N> __global__ void k_tau_add(double *Atau, const double *Au, const int *list, int nb, double hi)
+{
+    using B = Blk<N>;
+    constexpr int PER = N * N * N;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * PER) return;
+    int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
+    int b = list[t / PER];
+    Flux<N> F{Au + (long)b * B::LEN, nullptr, nullptr, hi};
+    long q = (long)b * B::LEN + B::gi(i, j, k);
+    Atau[q] = Atau[q] + F.div(i, j, k) * hi;
+}
+
+// ---- vmg_tau_crop: a child face meeting the parent level zeroes the parent's boundary plane ----
+template <int N> __global__ void k_tau_crop(double *Atau, const int *cf, int ncf, const int *parent)
+{
+    using B = Blk<N>;
+    constexpr int W = B::W;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ncf * W * W) return;
+    int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1, e = cf[t / (W * W)];
+    int b = e / 6, f = e % 6;
+    Atau[(long)parent[b] * B::LEN + B::rot(f / 2, (f & 1) ? N - 1 : 0, t1, t2)] = 0.0;
+}
+
+// ---- vmg_interp_p2c_lev (trilinear): child interiors of [b0, b0+nb) from the parent's octant ----
+template <int N> __global__ void k_p2c(double *Af, const double *Ac, int b0, int nb, const int *parent, const int *octant)
+{
+    using B = Blk<N>;
+    constexpr int H = N / 2, PER = N * N * N;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * PER) return;
+    int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
+    int b = b0 + (int)(t / PER);
+    const double *uc = Ac + (long)parent[b] * B::LEN;
+    int o = octant[b];
+    int oi = (o & 1) * H, oj = ((o >> 1) & 1) * H, ok = ((o >> 2) & 1) * H;
+    auto cget = [&](int a, int bb, int c) { return uc[B::gi(oi + a, oj + bb, ok + c)]; };
+    Af[(long)b * B::LEN + B::gi(i, j, k)] = trilinear_at(i, j, k, cget);
+}
+
+// ---- fmg_psi2g: g = -(F(i) + F(i-1))/2 on leaves, centred difference on covered blocks ----
+template <int N>
+__global__ void k_psi2g(double *g, const double *Au, int nblk, const double *hi_lev, const int *level,
+                        const int *skip, const int *ovslot, const double *ov)
+{
+    using B = Blk<N>;
+    constexpr int PER = N * N * N;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nblk * PER) return;
+    int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
+    int b = (int)(t / PER);
+    const double hi = hi_lev[level[b]];
+    const double *u = Au + (long)b * B::LEN;
+    double *gx = g + (long)b * 3 * PER, *gy = gx + PER, *gz = gy + PER;
+    long q = (long)(k * N + j) * N + i;
+    if (skip[b]) {
+        gx[q] = -(u[B::gi(i + 1, j, k)] - u[B::gi(i - 1, j, k)]) * hi * 0.5;
+        gy[q] = -(u[B::gi(i, j + 1, k)] - u[B::gi(i, j - 1, k)]) * hi * 0.5;
+        gz[q] = -(u[B::gi(i, j, k + 1)] - u[B::gi(i, j, k - 1)]) * hi * 0.5;
+        return;
+    }
+    Flux<N> F{u, ovslot + 6 * b, ov, hi};
+    const double uc = u[B::gi(i, j, k)];
+    double fxp = (u[B::gi(i + 1, j, k)] - uc) * hi, fxm = (uc - u[B::gi(i - 1, j, k)]) * hi;
+    double fyp = (u[B::gi(i, j + 1, k)] - uc) * hi, fym = (uc - u[B::gi(i, j - 1, k)]) * hi;
+    double fzp = (u[B::gi(i, j, k + 1)] - uc) * hi, fzm = (uc - u[B::gi(i, j, k - 1)]) * hi;
+    if (i == N - 1) fxp = F.ov_or(1, j, k, fxp);
+    if (i == 0) fxm = F.ov_or(0, j, k, fxm);
+    if (j == N - 1) fyp = F.ov_or(3, i, k, fyp);
+    if (j == 0) fym = F.ov_or(2, i, k, fym);
+    if (k == N - 1) fzp = F.ov_or(5, i, j, fzp);
+    if (k == 0) fzm = F.ov_or(4, i, j, fzm);
+    gx[q] = -(fxp + fxm) * 0.5;
+    gy[q] = -(fyp + fym) * 0.5;
+    gz[q] = -(fzp + fzm) * 0.5;
+}
+
+// ---- norms over the listed blocks' interiors: max |a| h^2 (fmg_get_h2absmax) -> out[0],
+//      max |a| -> out[1], sum a^2 dv -> out[2] ----
+template <int N>
+__global__ void k_norms(const double *A, const int *list, int nb, const double *h_lev, const int *level, double *out)
+{
+    using B = Blk<N>;
+    constexpr int PER = N * N * N;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double m2 = 0.0, m = 0.0, s = 0.0;
+    if (t < (long)nb * PER) {
+        int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
+        int b = list[t / PER];
+        double h = h_lev[level[b]];
+        double v = A[(long)b * B::LEN + B::gi(i, j, k)];
+        m = fabs(v);
+        m2 = m * (h * h);
+        s = v * v * ((h * h) * h);
+    }
+    double mm = m2;
+    block_max_sum(mm, s);
+    if (threadIdx.x == 0) {
+        atomic_max_nonneg(out + 0, mm);
+        atomicAdd(out + 2, s);
+    }
+    block_atomic_max(out + 1, m);
+}
+
+// ---- single-workgroup helpers (blockDim.x threads, one workgroup) ----
+// Pairwise halving over a[0..P) after zero-padding a[m..P): the fixed summation order of block
+// partials shared with the CPU restatement.
+__device__ inline void wg_tree_sum(double *a, int m, int P)
+{
+    for (int q = m + (int)threadIdx.x; q < P; q += blockDim.x) a[q] = 0.0;
+    __syncthreads();
+    for (int st = P / 2; st >= 1; st >>= 1) {
+        for (int q = threadIdx.x; q < st; q += blockDim.x) a[q] = a[q] + a[q + st];
+        __syncthreads();
+    }
+}
+
+// vmg_zeroAverage (vmg.F90:890-915): average of the base blocks' interiors removed from them.
+template <int N> __global__ void k_zero_average(double *A, int nbase, int P, double *scratch)
+{
+    using B = Blk<N>;
+    for (int b = threadIdx.x; b < nbase; b += blockDim.x) {
+        const double *a = A + (long)b * B::LEN;
+        double t = 0.0;
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < N; ++j)
+                for (int i = 0; i < N; ++i) t = t + a[B::gi(i, j, k)];
+        scratch[b] = t;
+    }
+    __syncthreads();
+    wg_tree_sum(scratch, nbase, P);
+    const double ave = scratch[0] / (double)(N * N * N * nbase);
+    for (int q = threadIdx.x; q < nbase * N * N * N; q += blockDim.x) {
+        int b = q / (N * N * N), c = q % (N * N * N);
+        long e = (long)b * B::LEN + B::gi(c % N, (c / N) % N, c / (N * N));
+        A[e] = A[e] - ave;
+    }
+}
+
+// fmg_make_physvar mean (fmg_data_if.F90:53-78): per-block partials of mass and volume over leaves
+template <int N>
+__global__ void k_leaf_mass(const double *A, int nblk, const int *covered, const double *h_lev, const int *level,
+                            double *pm, double *pv)
+{
+    using B = Blk<N>;
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    if (covered[b]) { pm[b] = 0.0; pv[b] = 0.0; return; }
+    double h = h_lev[level[b]];
+    double dv = (h * h) * h;
+    const double *a = A + (long)b * B::LEN;
+    double t = 0.0;
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) t = t + a[B::gi(i, j, k)];
+    pm[b] = t * dv;
+    pv[b] = (double)(N * N * N) * dv;
+}
+
+__global__ void k_tree_mass(double *pm, double *pv, int nblk, int P, double *ave)
+{
+    wg_tree_sum(pm, nblk, P);
+    wg_tree_sum(pv, nblk, P);
+    if (threadIdx.x == 0) *ave = pm[0] / pv[0];
+}
+
+__global__ void k_sub_scalar(double *A, long len, const double *s)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= len) return;
+    A[t] = A[t] - *s;
+}
+
+// ---- fmg_boundary_phys_sphere ----
+struct Geo {
+    const int *ijk, *level;
+    const double *h0_lev;   // cell width per AMR level at FMG level 0
+    double x0, y0, z0;
+    __device__ double x(int b, int a, int i) const
+    {
+        double o = a == 0 ? x0 : (a == 1 ? y0 : z0);
+        return o + ((double)(ijk[3 * b + a] * 8 + i) + 0.5) * h0_lev[level[b]];
+    }
+};
+
+template <int N> __global__ void k_sphere_mask(double *A, int nblk, Geo G, double rad2)
+{
+    using B = Blk<N>;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nblk * B::LEN) return;
+    int b = (int)(t / B::LEN), q = (int)(t % B::LEN);
+    int i = q % B::W - 1, j = (q / B::W) % B::W - 1, k = q / (B::W * B::W) - 1;
+    double x = G.x(b, 0, i), y = G.x(b, 1, j), z = G.x(b, 2, k);
+    if (x * x + y * y + z * z >= rad2) A[t] = 0.0;
+}
+
+// multipole moments (l <= 3, real form) of one base block, cells in (k, j, i) order
+template <int N> __global__ void k_sphere_moments(const double *A, int nbase, int P, Geo G, double dv, double *part)
+{
+    using B = Blk<N>;
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbase) return;
+    const double *rho = A + (long)b * B::LEN;
+    double a[16];
+    for (int q = 0; q < 16; ++q) a[q] = 0.0;
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+                double x = G.x(b, 0, i), y = G.x(b, 1, j), z = G.x(b, 2, k);
+                double r2 = x * x + y * y + z * z;
+                double r = sqrt(r2);
+                double r3 = r2 * r;
+                double rc = sqrt(x * x + y * y);
+                double ri = 1.0 / r, rci = 1.0 / rc;
+                double cost = z * ri, sint = rc * ri, cosp = x * rci, sinp = y * rci;
+                double c2 = cost * cost, s2 = sint * sint, s3 = s2 * sint;
+                double cp2 = cosp * cosp, sp2 = sinp * sinp;
+                double rdv = rho[B::gi(i, j, k)] * dv;
+                a[0] = a[0] + rdv;
+                a[1] = a[1] + rdv * r * cost;
+                a[2] = a[2] + rdv * r * sint * cosp;
+                a[3] = a[3] + rdv * r * sint * sinp;
+                a[4] = a[4] + rdv * r2 * (3.0 * c2 - 1.0);
+                a[5] = a[5] + rdv * r2 * sint * cost * cosp;
+                a[6] = a[6] + rdv * r2 * sint * cost * sinp;
+                a[7] = a[7] + rdv * r2 * s2 * (cp2 - sp2);
+                a[8] = a[8] + rdv * r2 * s2 * 2.0 * sinp * cosp;
+                a[9] = a[9] + rdv * r3 * cost * (5.0 * c2 - 3.0);
+                a[10] = a[10] + rdv * r3 * sint * (5.0 * c2 - 1.0) * cosp;
+                a[11] = a[11] + rdv * r3 * sint * (5.0 * c2 - 1.0) * sinp;
+                a[12] = a[12] + rdv * r3 * cost * s2 * (cp2 - sp2);
+                a[13] = a[13] + rdv * r3 * cost * s2 * 2.0 * sinp * cosp;
+                a[14] = a[14] + rdv * r3 * s3 * cosp * (cp2 - 3.0 * sp2);
+                a[15] = a[15] + rdv * r3 * s3 * sinp * (3.0 * cp2 - sp2);
+            }
+    for (int q = 0; q < 16; ++q) part[(long)q * P + b] = a[q];
+}
+
+__global__ void k_sphere_coef(double *part, int nbase, int P, double *coef)
+{
+    for (int q = 0; q < 16; ++q) wg_tree_sum(part + (long)q * P, nbase, P);
+    if (threadIdx.x != 0) return;
+    const double pi = 3.14159265358979323846;
+    double A[16];
+    for (int q = 0; q < 16; ++q) A[q] = part[(long)q * P];
+    coef[0] = -A[0] / (4.0 * pi);
+    coef[1] = -A[1] / (4.0 * pi);
+    coef[2] = -A[2] / (8.0 * pi);
+    coef[3] = -A[3] / (8.0 * pi);
+    coef[4] = -A[4] / (16.0 * pi);
+    coef[5] = -A[5] / (8.0 * pi) * 3.0;
+    coef[6] = -A[6] / (8.0 * pi) * 3.0;
+    coef[7] = -A[7] / (32.0 * pi) * 3.0;
+    coef[8] = -A[8] / (32.0 * pi) * 3.0;
+    coef[9] = -A[9] / (16.0 * pi);
+    coef[10] = -A[10] / (64.0 * pi) * 3.0;
+    coef[11] = -A[11] / (64.0 * pi) * 3.0;
+    coef[12] = -A[12] / (32.0 * pi) * 15.0;
+    coef[13] = -A[13] / (32.0 * pi) * 15.0;
+    coef[14] = -A[14] / (64.0 * pi) * 5.0;
+    coef[15] = -A[15] / (64.0 * pi) * 5.0;
+}
+
+template <int N> __global__ void k_sphere_ghost(double *A, const int *faces, int nf, Geo G, const double *coef)
+{
+    using B = Blk<N>;
+    constexpr int W = B::W;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nf * W * W) return;
+    int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1, e = faces[t / (W * W)];
+    int b = e / 6, f = e % 6, d = f / 2, ag = (f & 1) ? N : -1;
+    int ii = d == 0 ? ag : t1, jj = d == 1 ? ag : (d == 0 ? t1 : t2), kk = d == 2 ? ag : t2;
+    double x = G.x(b, 0, ii), y = G.x(b, 1, jj), z = G.x(b, 2, kk);
+    const double *c = coef;
+    double r2i = 1.0 / (x * x + y * y + z * z);
+    double ri = sqrt(r2i);
+    double r3i = r2i * ri, r4i = r2i * r2i;
+    double rc = sqrt(x * x + y * y);
+    double rci = 1.0 / rc;
+    double cost = z * ri, sint = rc * ri, cosp = x * rci, sinp = y * rci;
+    double c2 = cost * cost, s2 = sint * sint, s3 = s2 * sint;
+    double cp2 = cosp * cosp, sp2 = sinp * sinp;
+    double ub = ri * c[0];
+    ub = ub + r2i * cost * c[1];
+    ub = ub + r2i * sint * (cosp * c[2] + sinp * c[3]) * 2.0;
+    ub = ub + r3i * (3.0 * c2 - 1.0) * c[4];
+    ub = ub + r3i * sint * cost * (cosp * c[5] + sinp * c[6]) * 2.0;
+    ub = ub + r3i * s2 * ((cp2 - sp2) * c[7] + 2.0 * sinp * cosp * c[8]) * 2.0;
+    ub = ub + r4i * cost * (5.0 * c2 - 3.0) * c[9];
+    ub = ub + r4i * sint * (5.0 * c2 - 1.0) * (cosp * c[10] + sinp * c[11]) * 2.0;
+    ub = ub + r4i * cost * s2 * ((cp2 - sp2) * c[12] + 2.0 * sinp * cosp * c[13]) * 2.0;
+    ub = ub + r4i * s3 * (cosp * (cp2 - 3.0 * sp2) * c[14] + sinp * (3.0 * cp2 - sp2) * c[15]) * 2.0;
+    A[(long)b * B::LEN + B::rot(d, ag, t1, t2)] = ub;
+}
+
+// ---- base grid: gather of the base blocks' interiors (mg_dataPrepare) and scatter back ----
+template <int N> __global__ void k_base_gather(double *psi, double *src, const double *Au, const double *Arhs,
+                                              int nbase, const int *ijk, int NG)
+{
+    using B = Blk<N>;
+    constexpr int PER = N * N * N;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nbase * PER) return;
+    int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N), b = (int)(t / PER);
+    long W = NG + 2;
+    long q = ((long)(ijk[3 * b + 2] * N + k + 1) * W + (ijk[3 * b + 1] * N + j + 1)) * W + (ijk[3 * b] * N + i + 1);
+    psi[q] = Au[(long)b * B::LEN + B::gi(i, j, k)];
+    src[q] = Arhs[(long)b * B::LEN + B::gi(i, j, k)];
+}
+
+template <int N> __global__ void k_base_scatter(double *Au, const double *psi, int nbase, const int *ijk, int NG)
+{
+    using B = Blk<N>;
+    constexpr int PER = N * N * N;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nbase * PER) return;
+    int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N), b = (int)(t / PER);
+    long W = NG + 2;
+    long q = ((long)(ijk[3 * b + 2] * N + k + 1) * W + (ijk[3 * b + 1] * N + j + 1)) * W + (ijk[3 * b] * N + i + 1);
+    Au[(long)b * B::LEN + B::gi(i, j, k)] = psi[q];
+}
+
+} // namespace amr
+} // namespace sg

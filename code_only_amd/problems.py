@@ -36,3 +36,37 @@ def clumpy_rhs(n, seed=0, nclump=8, sigma_cells=4.0):
     """Right-hand side 4 pi G (rho - <rho>) with G = 1/(4 pi): zero mean, solvable periodic."""
     rho = clumpy_density(n, seed=seed, nclump=nclump, sigma_cells=sigma_cells)
     return rho - rho.mean()
+
+
+def amr_source(hier, seed=0, nclump=4, sigma=0.06):
+    """Clumpy source on every block of an AMR hierarchy ([nblk, 8, 8, 8]): seeded Gaussian clumps
+    (centres uniform in the middle half of the domain, LogNormal(0, 1) amplitudes, width ``sigma``
+    in domain units) on a unit background.  For periodic hierarchies the solver removes the
+    leaf-volume mean (ZERO_AVERAGE_DENSITY)."""
+    from .amr_hierarchy import to_blocks
+    rng = np.random.default_rng(seed)
+    ext = np.asarray(hier.base_dims) * 8 * hier.h0
+    lo = np.asarray(hier.x0)
+    centres = lo + ext * rng.uniform(0.25, 0.75, size=(nclump, 3))
+    amps = rng.lognormal(0.0, 1.0, size=nclump)
+
+    def f(x, y, z):
+        out = np.ones(np.broadcast(x, y, z).shape)
+        for c in range(nclump):
+            r2 = (x - centres[c, 0]) ** 2 + (y - centres[c, 1]) ** 2 + (z - centres[c, 2]) ** 2
+            out = out + amps[c] * np.exp(-r2 / (2 * sigma * sigma))
+        return out
+    return to_blocks(hier, f)
+
+
+def sis_source(hier, seed=0, perturb=0.05):
+    """Singular isothermal sphere for a domain centred on the origin: rho = 1 / (r^2 + r_c^2) with
+    r_c one cell of the finest level (a one-cell core), times seeded lognormal perturbations of
+    ``perturb`` standard deviation, one independent factor per cell of every block.  With
+    G = 1/(4 pi) the source 4 pi G rho equals rho."""
+    from .amr_hierarchy import to_blocks
+    rc = hier.h(hier.nlev - 1)
+    rng = np.random.default_rng(seed)
+    rho = to_blocks(hier, lambda x, y, z: 1.0 / (x * x + y * y + z * z + rc * rc))
+    s = np.sqrt(np.log1p(perturb * perturb))
+    return rho * rng.lognormal(-0.5 * s * s, s, size=rho.shape)

@@ -54,3 +54,42 @@ def test_create_rejects_bad_sizes_without_touching_the_gpu():
         rc = lib.sg_uniform_create(ctypes.byref(ctx), n, 1.0, _lib.SG_PREC_FP64)
         assert rc == _lib.SG_ERR_ARG
         assert b"power of two" in lib.sg_last_error()
+
+
+def test_amr_create_rejects_improper_tables_without_touching_the_gpu():
+    import numpy as np
+    from code_only_amd import _lib
+    from code_only_amd import amr_hierarchy as ah
+    lib = _lib.load()
+    h = ah.nested(base=4, levels=2, bc=ah.SPHERE)
+    ip = lambda a: np.ascontiguousarray(a, np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    x0 = np.asarray(h.x0, np.float64)
+    keep = [np.ascontiguousarray(a, np.int32) for a in (h.level, h.ijk, h.parent, h.child, h.nbr, h.base_dims)]
+
+    def create(level=None, nbr=None, bc=_lib.SG_BC_SPHERE):
+        arrs = list(keep)
+        if level is not None:
+            arrs[0] = np.ascontiguousarray(level, np.int32)
+        if nbr is not None:
+            arrs[4] = np.ascontiguousarray(nbr, np.int32)
+        ctx = ctypes.c_void_p()
+        return lib.sg_amr_create(ctypes.byref(ctx), h.nblk, h.nlev, *[ip(a) for a in arrs], h.h0,
+                                 x0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), bc)
+
+    lvl = h.level.copy()
+    lvl[[0, -1]] = lvl[[-1, 0]]         # not ordered by level
+    assert create(level=lvl) == _lib.SG_ERR_ARG
+    assert b"ordered by level" in lib.sg_last_error() or b"parent" in lib.sg_last_error()
+    h3 = ah.nested(base=4, levels=3, bc=ah.SPHERE)
+    b3 = int(np.nonzero(h3.level == 2)[0][0])
+    nb3 = h3.nbr.copy()
+    nb3[b3, 0] = 0                      # level 2 next to level 0
+    keep3 = [np.ascontiguousarray(a, np.int32) for a in (h3.level, h3.ijk, h3.parent, h3.child, nb3, h3.base_dims)]
+    ctx = ctypes.c_void_p()
+    rc = lib.sg_amr_create(ctypes.byref(ctx), h3.nblk, h3.nlev, *[ip(a) for a in keep3], h3.h0,
+                           np.asarray(h3.x0, np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           _lib.SG_BC_SPHERE)
+    assert rc == _lib.SG_ERR_ARG and b"properly nested" in lib.sg_last_error()
+    # a periodic hierarchy may not have open faces
+    assert create(bc=_lib.SG_BC_PERIODIC) == _lib.SG_ERR_ARG
+    assert b"open face" in lib.sg_last_error()
