@@ -32,6 +32,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--smoother-reps", type=int, default=50)
+    ap.add_argument("--no-amr", action="store_true", help="skip the AMR hierarchy solve")
+    ap.add_argument("--amr-steps", type=int, default=3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (reference reading for HBM counters)")
     return ap.parse_args()
@@ -64,6 +66,89 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def bench_amr(a, dev):
+    """Sphere hierarchy: 16^3 base blocks (128^3 cells), four nested levels each refining the
+    central half-width of its parent (5 x 4096 blocks of 8^3, 9 437 184 leaf cells), singular
+    isothermal sphere with a one-cell core and 5 % lognormal perturbations, multipole boundary."""
+    import numpy as np
+    import torch
+    from code_only_amd import amr_hierarchy as ah
+    from code_only_amd import problems
+    from code_only_amd.amr import AmrSolver
+
+    hier = ah.nested(base=16, levels=5, bc=ah.SPHERE)
+    src = problems.sis_source(hier, seed=1)
+    solver = AmrSolver(hier)
+    s_t = torch.tensor(src, device=dev)
+    g_t = torch.zeros((hier.nblk, 3, 8, 8, 8), dtype=torch.float64, device=dev)
+    p_t = torch.zeros_like(s_t)
+    log(f"[bench] AMR sphere hierarchy: {hier.nblk} blocks, {hier.cells()} leaf cells, plain FP64")
+    st = None
+    for _ in range(1):
+        p_t.zero_()
+        st = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30)
+    times = []
+    for _ in range(max(1, a.amr_steps)):
+        p_t.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms64 = float(np.median(times))
+    log(f"[bench] AMR fp64: {ms64:.2f} ms/solve, {st['iterations']} it, rel {st['rel_linf']:.3e}")
+    p64 = p_t.clone()
+    g64 = g_t.clone()
+    # mixed precision: FP32 inner fmg_lin inside FP64 defect correction, one outer iteration replayed
+    # as a hipGraph
+    from code_only_amd.amr import SG_PREC_MIXED
+    stm = None
+    for _ in range(1):
+        p_t.zero_()
+        stm = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30, precision=SG_PREC_MIXED)
+    times = []
+    for _ in range(max(1, a.amr_steps)):
+        p_t.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stm = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30, precision=SG_PREC_MIXED)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    msmx = float(np.median(times))
+    err_p = ((p_t - p64).norm() / p64.norm()).item()
+    err_g = ((g_t - g64).norm() / g64.norm()).item()
+    log(f"[bench] AMR mixed: {msmx:.2f} ms/solve, {stm['iterations']} it, rel {stm['rel_linf']:.3e}, "
+        f"psi err vs fp64 {err_p:.2e}")
+    out = {"workload": "sphere AMR: 128^3 base (16^3 blocks of 8^3), 4 nested levels of the central "
+                       "half-width, singular isothermal sphere (1-cell core, 5% lognormal), multipole BC",
+           "blocks": hier.nblk, "leaf_cells": hier.cells(), "fp64_gpu_ms": round(ms64, 3),
+           "fp64_iterations": st["iterations"], "fp64_rel_linf": st["rel_linf"], "fp64_rel_l2": st["rel_l2"],
+           "fp64_history": st["hist"],
+           "mixed_gpu_ms": round(msmx, 3), "mixed_iterations": stm["iterations"], "mixed_rel_linf": stm["rel_linf"],
+           "mixed_rel_l2": stm["rel_l2"], "mixed_history": stm["hist"], "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
+           "mixed_vs_fp64_gpu": {"psi_rel_l2": err_p, "g_rel_l2": err_g}}
+    if not a.no_cpu:
+        import oracle
+        log("[bench] AMR CPU oracle (FP64, 1 core) on the same hierarchy")
+        o = oracle.Amr(hier)
+        t0 = time.perf_counter()
+        po, go_, so = o.solve(src, tol=a.tol, maxit=30)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        pg, gg = p64.cpu().numpy(), g64.cpu().numpy()
+        pm, gm = p_t.cpu().numpy(), g_t.cpu().numpy()
+        out["cpu_oracle_ms"] = round(cpu_ms, 1)
+        out["cpu_oracle_iterations"] = so["iterations"]
+        out["fp64_gpu_vs_oracle"] = {"psi_rel_l2": float(np.linalg.norm(pg - po) / np.linalg.norm(po)),
+                                     "g_rel_l2": float(np.linalg.norm(gg - go_) / np.linalg.norm(go_)),
+                                     "bitwise": bool(np.array_equal(pg, po) and np.array_equal(gg, go_))}
+        out["mixed_gpu_vs_oracle"] = {"psi_rel_l2": float(np.linalg.norm(pm - po) / np.linalg.norm(po)),
+                                      "g_rel_l2": float(np.linalg.norm(gm - go_) / np.linalg.norm(go_))}
+        log(f"[bench] AMR cpu oracle {cpu_ms:.0f} ms; parity {out['fp64_gpu_vs_oracle']}")
+        o.close()
+    solver.close()
+    return out
 
 
 def main():
@@ -201,6 +286,8 @@ def main():
         "cpu_baseline": cpu,
         "autotune": tiles,
     }
+    if not a.no_amr:
+        out["amr"] = bench_amr(a, dev)
     print(json.dumps(out), flush=True)
     solver.close()
 

@@ -78,6 +78,9 @@ SIGNATURES = {
     "sg_uniform_time_smoother": (_I, [_VP, _I, _I, _DP]),
     "sg_uniform_tile_info": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                   ctypes.POINTER(ctypes.c_float), _I]),
+    "sg_uniform_adjoint_dev": (_I, [_VP, _VP, _VP, ctypes.POINTER(SgParams), ctypes.POINTER(SgStats)]),
+    "sg_uniform_gradient_sample_dev": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "sg_uniform_gradient_sample_adjoint_dev": (_I, [_VP, _VP, _VP, _I, _VP]),
     "sg_amr_create": (_I, [ctypes.POINTER(_VP), _I, _I, _IP, _IP, _IP, _IP, _IP, _IP, _D, _DP, _I]),
     "sg_amr_destroy": (_I, [_VP]),
     "sg_amr_set_zero_average": (_I, [_VP, _I]),

@@ -10,9 +10,11 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (SG_BC_PERIODIC, SG_BC_SPHERE, SG_PREC_FP64, SG_STOP_REFERENCE, SG_STOP_TRUE_RESIDUAL)
+from ._lib import (SG_BC_PERIODIC, SG_BC_SPHERE, SG_PREC_FP64, SG_PREC_MIXED, SG_STOP_REFERENCE,
+                   SG_STOP_TRUE_RESIDUAL)
 
-__all__ = ["AmrSolver", "SG_BC_PERIODIC", "SG_BC_SPHERE", "SG_STOP_REFERENCE", "SG_STOP_TRUE_RESIDUAL"]
+__all__ = ["AmrSolver", "SG_BC_PERIODIC", "SG_BC_SPHERE", "SG_PREC_FP64", "SG_PREC_MIXED", "SG_STOP_REFERENCE",
+           "SG_STOP_TRUE_RESIDUAL"]
 
 U, RES, RHS, RHO, SRC, PSI = range(6)
 
@@ -54,10 +56,12 @@ class AmrSolver:
             pass
 
     @staticmethod
-    def _params(tol, maxit, stop_mode):
-        return _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=SG_PREC_FP64)
+    def _params(tol, maxit, stop_mode, precision=SG_PREC_FP64, use_graph=True):
+        return _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=precision,
+                                   use_graph=int(bool(use_graph)))
 
-    def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL):
+    def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64,
+              use_graph=True):
         """fmg_poisson on torch float64 GPU tensors [nblk, 8, 8, 8]; psi in place, g [nblk, 3, 8, 8, 8]."""
         import torch
         for t in (psi, src) + ((g,) if g is not None else ()):
@@ -65,7 +69,7 @@ class AmrSolver:
                 raise ValueError("expected contiguous float64 GPU tensors")
         if tuple(psi.shape) != (self.h.nblk, 8, 8, 8) or psi.shape != src.shape:
             raise ValueError("psi and src must be [nblk, 8, 8, 8]")
-        p = self._params(tol, maxit, stop_mode)
+        p = self._params(tol, maxit, stop_mode, precision, use_graph)
         st = _lib.SgStats()
         torch.cuda.synchronize()
         _lib.check(self._lib.sg_amr_solve_dev(self._ctx, ctypes.c_void_p(psi.data_ptr()),
@@ -74,11 +78,12 @@ class AmrSolver:
                                               ctypes.byref(p), ctypes.byref(st)))
         return st.as_dict()
 
-    def solve_host(self, src, psi=None, want_g=True, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL):
+    def solve_host(self, src, psi=None, want_g=True, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL,
+                   precision=SG_PREC_FP64, use_graph=True):
         src = np.ascontiguousarray(src, np.float64)
         psi = np.zeros_like(src) if psi is None else np.ascontiguousarray(psi, np.float64).copy()
         g = np.zeros((self.h.nblk, 3, 8, 8, 8)) if want_g else None
-        p = self._params(tol, maxit, stop_mode)
+        p = self._params(tol, maxit, stop_mode, precision, use_graph)
         st = _lib.SgStats()
         _lib.check(self._lib.sg_amr_solve_host(self._ctx, _dp(psi), _dp(src), _dp(g) if want_g else None,
                                                ctypes.byref(p), ctypes.byref(st)))

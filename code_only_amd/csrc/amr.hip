@@ -12,9 +12,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "amr_kernels.h"
+#include "base_direct.h"
 #include "base_mg.h"
 #include "sg_common.h"
 
@@ -37,6 +39,12 @@ inline int pow2ge(int m) { int p = 1; while (p < m) p <<= 1; return p; }
 
 enum Kinds { K_SAME = 1, K_PARENT = 2, K_BOTH = 3 };
 
+// One set of FMG-level block arrays (codes U..PSI) and reflux overrides.
+template <typename T> struct Arrays {
+    T *A[kNfmg][NCODE] = {};
+    T *ov[kNfmg] = {};
+};
+
 } // namespace
 
 struct sg_amr {
@@ -51,14 +59,23 @@ struct sg_amr {
     int *d_level = nullptr, *d_ijk = nullptr, *d_parent = nullptr, *d_octant = nullptr, *d_nbr = nullptr;
     int *d_nkind = nullptr, *d_covered = nullptr, *d_skip = nullptr, *d_active = nullptr, *d_phys = nullptr;
     int *d_cf = nullptr, *d_cf_slot = nullptr, *d_ovslot = nullptr, *d_cov = nullptr;
-    // device state
-    double *A[kNfmg][NCODE] = {};
-    double *ov[kNfmg] = {};
+    // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
+    // mixed solve's inner fmg_lin), allocated on first mixed use
+    Arrays<double> d;
+    Arrays<float> f;
+    int have_f = 0;
+    BaseDirect bd{};
+    float *bd_mem = nullptr;
+    size_t bd_lds_bytes = 0;
+    int bd_ok = 0;
+    hipGraphExec_t graph = nullptr;   // mixed solve: one outer iteration, captured once
     double hl[kNfmg][kMaxLev] = {}, hil[kNfmg][kMaxLev] = {};
     double *d_h[kNfmg] = {}, *d_hi[kNfmg] = {};
     double *d_scal = nullptr;   // [0] Resh2maxg [1] scratch max [2..4] norms [5] mean [8..23] multipole
     double *h_scal = nullptr;   // pinned mirror
     double *d_part = nullptr;   // tree-sum scratch
+    double *d_npart = nullptr;  // norm partials [3][kNormGroups]
+    double *d_slots = nullptr;  // smoother max slots [kMaxSlots]
     int P_all = 1, P_base = 1;
     double *d_dense = nullptr;  // staging for host entry points
     Cubic64 st{};
@@ -67,12 +84,17 @@ struct sg_amr {
     double *bm_mem = nullptr;
     int *d_iters = nullptr;
     int base_mg_ok = 0;
+    long bm_lds = 0;            // LDS doubles of the base-grid kernel
     int fill0_level = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 namespace {
+
+template <typename T> inline Arrays<T> &ar(sg_amr &s);
+template <> inline Arrays<double> &ar<double>(sg_amr &s) { return s.d; }
+template <> inline Arrays<float> &ar<float>(sg_amr &s) { return s.f; }
 
 template <typename K, typename... Args> int go(sg_amr &s, K kern, long work, Args... args)
 {
@@ -101,333 +123,396 @@ int set_scalar(sg_amr &s, int idx, double v)
 
 #define SG_NDISPATCH(L, FN, ...)                        \
     switch (L) {                                        \
-    case 0: return FN<8>(__VA_ARGS__);                  \
-    case 1: return FN<4>(__VA_ARGS__);                  \
-    default: return FN<2>(__VA_ARGS__);                 \
+    case 0: return FN<8, T>(__VA_ARGS__);               \
+    case 1: return FN<4, T>(__VA_ARGS__);               \
+    default: return FN<2, T>(__VA_ARGS__);              \
     }
 
+// Every routine below runs on the FP64 arrays (T = double: the plain path, bitwise the reference's
+// sequence) or on the FP32 arrays of the mixed path's inner solve (T = float).
+template <typename T> constexpr bool kInner = std::is_same<T, float>::value;
+
 // ---- fmg_converge ----
-template <int N> int c2p_lev_n(sg_amr &s, int L, int code, int lc, int cubic)
+template <int N, typename T> int c2p_lev_n(sg_amr &s, int L, int code, int lc, int cubic)
 {
     int b0 = s.lev_start[lc + 1], nb = s.lev_start[lc + 2] - b0;
     long work = (long)nb * (N / 2) * (N / 2) * (N / 2);
-    if (cubic) return go(s, k_c2p<N, true>, work, s.A[L][code], b0, nb, s.d_parent, s.d_octant, s.st);
-    return go(s, k_c2p<N, false>, work, s.A[L][code], b0, nb, s.d_parent, s.d_octant, s.st);
+    T *A = ar<T>(s).A[L][code];
+    if (cubic) return go(s, k_c2p<N, true, T>, work, A, b0, nb, s.d_parent, s.d_octant, s.st);
+    return go(s, k_c2p<N, false, T>, work, A, b0, nb, s.d_parent, s.d_octant, s.st);
 }
-int c2p_lev(sg_amr &s, int L, int code, int lc, int cubic) { SG_NDISPATCH(L, c2p_lev_n, s, L, code, lc, cubic); }
-
-int c2p(sg_amr &s, int L, int code, int cubic)
+template <typename T> int c2p_lev(sg_amr &s, int L, int code, int lc, int cubic)
 {
-    for (int lc = s.nlev - 2; lc >= 0; --lc) SG_TRY(c2p_lev(s, L, code, lc, cubic));
+    SG_NDISPATCH(L, c2p_lev_n, s, L, code, lc, cubic);
+}
+
+template <typename T> int c2p(sg_amr &s, int L, int code, int cubic)
+{
+    for (int lc = s.nlev - 2; lc >= 0; --lc) SG_TRY(c2p_lev<T>(s, L, code, lc, cubic));
     return SG_OK;
 }
 
 // ---- fmg_ghostcell ----
-template <int N> int ghost_pass_n(sg_amr &s, int l, int L, int d, int code, int cubic, int kinds)
+template <int N, typename T> int ghost_pass_n(sg_amr &s, int l, int L, int d, int code, int cubic, int kinds)
 {
     int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
     if (l == 0) kinds &= K_SAME;   // fmg_ghfix_parentlev returns at the base level
     if (!kinds) return SG_OK;
     long W = N + 2;
-    return go(s, k_ghost<N>, (long)nb * 2 * W * W, s.A[L][code], b0, nb, d, cubic, kinds, s.d_nbr, s.d_nkind,
+    return go(s, k_ghost<N, T>, (long)nb * 2 * W * W, ar<T>(s).A[L][code], b0, nb, d, cubic, kinds, s.d_nbr, s.d_nkind,
               s.d_octant);
 }
-int ghost_pass(sg_amr &s, int l, int L, int d, int code, int cubic, int kinds)
+template <typename T> int ghost_pass(sg_amr &s, int l, int L, int d, int code, int cubic, int kinds)
 {
     SG_NDISPATCH(L, ghost_pass_n, s, l, L, d, code, cubic, kinds);
 }
 
-int ghostcell_fix(sg_amr &s, int L, int code, int cubic)
+template <typename T> int ghostcell_fix(sg_amr &s, int L, int code, int cubic)
 {
     for (int l = 0; l < s.nlev; ++l)
-        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, l, L, d, code, cubic, K_BOTH));
+        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, L, d, code, cubic, K_BOTH));
     return SG_OK;
 }
 
 // ---- fmg_boundary ----
-template <int N> int fill0_faces_n(sg_amr &s, int L, int code, int l0, int l1)
+template <int N, typename T> int fill0_faces_n(sg_amr &s, int L, int code, int l0, int l1)
 {
     int e0 = s.phys_off[l0], ne = s.phys_off[l1] - e0;
     long W = N + 2;
-    return go(s, k_fill0_faces<N>, (long)ne * W * W, s.A[L][code], s.d_phys + e0, ne);
+    return go(s, k_fill0_faces<N, T>, (long)ne * W * W, ar<T>(s).A[L][code], s.d_phys + e0, ne);
 }
-int fill0_faces(sg_amr &s, int L, int code, int l0, int l1) { SG_NDISPATCH(L, fill0_faces_n, s, L, code, l0, l1); }
+template <typename T> int fill0_faces(sg_amr &s, int L, int code, int l0, int l1)
+{
+    SG_NDISPATCH(L, fill0_faces_n, s, L, code, l0, l1);
+}
 
-template <int N> int extrap_faces_n(sg_amr &s, int L, int code, int l0, int l1)
+template <int N, typename T> int extrap_faces_n(sg_amr &s, int L, int code, int l0, int l1)
 {
     int e0 = s.phys_off[l0], ne = s.phys_off[l1] - e0;
     long W = N + 2;
-    for (int d = 0; d < 3; ++d) SG_TRY(go(s, k_extrap_faces<N>, (long)ne * W * W, s.A[L][code], s.d_phys + e0, ne, d));
+    for (int d = 0; d < 3; ++d)
+        SG_TRY(go(s, k_extrap_faces<N, T>, (long)ne * W * W, ar<T>(s).A[L][code], s.d_phys + e0, ne, d));
     return SG_OK;
 }
-int extrap_faces(sg_amr &s, int L, int code, int l0, int l1) { SG_NDISPATCH(L, extrap_faces_n, s, L, code, l0, l1); }
+template <typename T> int extrap_faces(sg_amr &s, int L, int code, int l0, int l1)
+{
+    SG_NDISPATCH(L, extrap_faces_n, s, L, code, l0, l1);
+}
 
 // ---- fmg_poisson: flux refresh + reflux, relax, resid ----
-template <int N> int reflux_n(sg_amr &s, int L, int code)
+template <int N, typename T> int reflux_n(sg_amr &s, int L, int code)
 {
     int ncf = (int)s.cf.size();
-    return go(s, k_reflux<N>, (long)ncf * (N / 2) * (N / 2), s.A[L][code], s.ov[L], s.d_cf, s.d_cf_slot, ncf,
-              s.d_hi[L], s.d_level, s.d_octant);
+    return go(s, k_reflux<N, T>, (long)ncf * (N / 2) * (N / 2), (const T *)ar<T>(s).A[L][code], ar<T>(s).ov[L],
+              (const int *)s.d_cf, (const int *)s.d_cf_slot, ncf, (const double *)s.d_hi[L], (const int *)s.d_level,
+              (const int *)s.d_octant);
 }
-int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, reflux_n, s, L, code); }
+template <typename T> int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, reflux_n, s, L, code); }
 
 // fmg_poisson_flux (fmg_poisson.F90:53-124): the face fluxes themselves are formed by their
 // consumers; what remains here is the ghost refresh and the reflux overrides.
-int poisson_flux(sg_amr &s, int L, int ju, int fill0flag)
+template <typename T> int poisson_flux(sg_amr &s, int L, int ju, int fill0flag)
 {
     if (s.fill0_level != L) {
-        SG_TRY(c2p(s, L, ju, 0));
-        SG_TRY(ghostcell_fix(s, L, ju, 0));
-        if (fill0flag) SG_TRY(fill0_faces(s, L, ju, 0, s.nlev));
+        SG_TRY(c2p<T>(s, L, ju, 0));
+        SG_TRY(ghostcell_fix<T>(s, L, ju, 0));
+        if (fill0flag) SG_TRY(fill0_faces<T>(s, L, ju, 0, s.nlev));
     }
     s.fill0_level = -1;
-    return reflux(s, L, ju);
+    return reflux<T>(s, L, ju);
 }
 
-template <int N> int relax_pass_n(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int nb, int ipass,
-                                  const int *ovslot, double *resmax)
+template <int N, typename T>
+int relax_pass_n(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int nb, int ipass, const int *ovslot,
+                 double *resmax, bool halo_free)
 {
     long work = (long)nb * N * N * ((N + 1) / 2);
-    return go(s, k_relax<N>, work, s.A[L][ju], (const double *)s.A[L][jrhs], list, b0, nb, ipass,
+    return go(s, k_relax<N, T>, work, ar<T>(s).A[L][ju], (const T *)ar<T>(s).A[L][jrhs], list, b0, nb, ipass,
               (const double *)s.d_h[L], (const double *)s.d_hi[L], (const int *)s.d_level, ovslot,
-              (const double *)s.ov[L], resmax);
+              (const T *)ar<T>(s).ov[L], resmax, halo_free ? (const int *)s.d_nbr : (const int *)nullptr,
+              halo_free ? (const int *)s.d_nkind : (const int *)nullptr);
 }
+template <typename T>
 int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int nb, int ipass, const int *ovslot,
-               double *resmax)
+               double *resmax, bool halo_free = false)
 {
-    SG_NDISPATCH(L, relax_pass_n, s, L, ju, jrhs, list, b0, nb, ipass, ovslot, resmax);
+    SG_NDISPATCH(L, relax_pass_n, s, L, ju, jrhs, list, b0, nb, ipass, ovslot, resmax, halo_free);
 }
 
-// fmg_poisson_relax (fmg_poisson.F90:130-213); Resh2maxg lands in d_scal[0]
-int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int fill0flag)
+// fmg_poisson_relax (fmg_poisson.F90:130-213).  With want_max, Resh2maxg lands in d_scal[0]; the
+// reference overwrites Resh2maxg in every call but reads only the last one of fmg_lin.
+template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int fill0flag, bool want_max)
 {
-    SG_TRY(set_scalar(s, 0, 0.0));
+    double *slots = want_max ? s.d_slots : nullptr;
+    if (want_max) {
+        hipLaunchKernelGGL(k_zero_slots, dim3(1), dim3(64), 0, s.stream, s.d_slots);
+        SG_CHECK_LAUNCH();
+    }
     for (int ipass = 1; ipass <= 2; ++ipass) {
-        SG_TRY(poisson_flux(s, L, ju, fill0flag));
-        SG_TRY(relax_pass(s, L, ju, jrhs, s.d_active, 0, (int)s.active.size(), ipass, s.d_ovslot, s.d_scal + 0));
+        SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
+        if (L == 0) {
+            hipLaunchKernelGGL((k_blk8<true, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, (T *)nullptr,
+                               ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_active, ipass,
+                               (const double *)s.d_h[0], (const double *)s.d_hi[0], (const int *)s.d_level,
+                               (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], slots);
+            SG_CHECK_LAUNCH();
+        } else {
+            SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_active, 0, (int)s.active.size(), ipass, s.d_ovslot, slots));
+        }
+    }
+    if (want_max) {
+        hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(64), 0, s.stream, (const double *)s.d_slots, s.d_scal + 0);
+        SG_CHECK_LAUNCH();
     }
     return SG_OK;
 }
 
-template <int N> int resid_n(sg_amr &s, int L, int jres, int ju, int jrhs)
+template <int N, typename T> int resid_n(sg_amr &s, int L, int jres, int ju, int jrhs)
 {
     int na = (int)s.active.size();
-    return go(s, k_resid<N>, (long)na * N * N * N, s.A[L][jres], (const double *)s.A[L][ju],
-              (const double *)s.A[L][jrhs], (const int *)s.d_active, na, (const double *)s.d_hi[L],
-              (const int *)s.d_level, (const int *)s.d_ovslot, (const double *)s.ov[L]);
+    return go(s, k_resid<N, T>, (long)na * N * N * N, ar<T>(s).A[L][jres], (const T *)ar<T>(s).A[L][ju],
+              (const T *)ar<T>(s).A[L][jrhs], (const int *)s.d_active, na, (const double *)s.d_hi[L],
+              (const int *)s.d_level, (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[L]);
 }
 
-int poisson_resid(sg_amr &s, int L, int jres, int ju, int jrhs, int fill0flag)
+template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int jrhs, int fill0flag)
 {
-    SG_TRY(poisson_flux(s, L, ju, fill0flag));
+    SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
+    if (L == 0) {
+        hipLaunchKernelGGL((k_blk8<false, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, ar<T>(s).A[0][jres],
+                           ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_active, 0,
+                           (const double *)s.d_h[0], (const double *)s.d_hi[0], (const int *)s.d_level,
+                           (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], (double *)nullptr);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
+    }
     SG_NDISPATCH(L, resid_n, s, L, jres, ju, jrhs);
 }
 
 // ---- fmg.F90 helpers ----
-int fmg_rstrct(sg_amr &s, int Lc, int iuc, int iuf)
+template <typename T> int fmg_rstrct(sg_amr &s, int Lc, int iuc, int iuf)
 {
     int na = (int)s.active.size();
-    if (Lc == 1) return go(s, k_rstrct<4>, (long)na * 64, s.A[1][iuc], (const double *)s.A[0][iuf], (const int *)s.d_active, na);
-    return go(s, k_rstrct<2>, (long)na * 8, s.A[2][iuc], (const double *)s.A[1][iuf], (const int *)s.d_active, na);
+    auto &R = ar<T>(s);
+    if (Lc == 1)
+        return go(s, k_rstrct<4, T>, (long)na * 64, R.A[1][iuc], (const T *)R.A[0][iuf], (const int *)s.d_active, na);
+    return go(s, k_rstrct<2, T>, (long)na * 8, R.A[2][iuc], (const T *)R.A[1][iuf], (const int *)s.d_active, na);
 }
 
-int interp_blocks(sg_amr &s, int Lf, int iuf, int iuc, int cubic)
+template <typename T> int interp_blocks(sg_amr &s, int Lf, int iuf, int iuc, int cubic)
 {
     int na = (int)s.active.size();
     const int *al = s.d_active;
+    auto &R = ar<T>(s);
     if (Lf == 0) {
-        if (cubic) return go(s, k_interp<8, true>, (long)na * 512, s.A[0][iuf], (const double *)s.A[1][iuc], al, na);
-        return go(s, k_interp<8, false>, (long)na * 512, s.A[0][iuf], (const double *)s.A[1][iuc], al, na);
+        if (cubic) return go(s, k_interp<8, true, T>, (long)na * 512, R.A[0][iuf], (const T *)R.A[1][iuc], al, na);
+        return go(s, k_interp<8, false, T>, (long)na * 512, R.A[0][iuf], (const T *)R.A[1][iuc], al, na);
     }
-    if (cubic) return go(s, k_interp<4, true>, (long)na * 64, s.A[1][iuf], (const double *)s.A[2][iuc], al, na);
-    return go(s, k_interp<4, false>, (long)na * 64, s.A[1][iuf], (const double *)s.A[2][iuc], al, na);
+    if (cubic) return go(s, k_interp<4, true, T>, (long)na * 64, R.A[1][iuf], (const T *)R.A[2][iuc], al, na);
+    return go(s, k_interp<4, false, T>, (long)na * 64, R.A[1][iuf], (const T *)R.A[2][iuc], al, na);
 }
 
 // fmg_interp (fmg.F90:786-828)
-int fmg_interp(sg_amr &s, int Lf, int iuf, int iuc, int cubic)
+template <typename T> int fmg_interp(sg_amr &s, int Lf, int iuf, int iuc, int cubic)
 {
     const int Lc = Lf + 1;
-    SG_TRY(c2p(s, Lc, iuc, cubic));
-    SG_TRY(ghostcell_fix(s, Lc, iuc, cubic));
-    SG_TRY(extrap_faces(s, Lc, iuc, 0, s.nlev));
-    SG_TRY(interp_blocks(s, Lf, iuf, iuc, cubic));
-    SG_TRY(fill0_faces(s, Lc, iuc, 0, s.nlev));
-    return fill0_faces(s, Lf, iuf, 0, s.nlev);
+    SG_TRY(c2p<T>(s, Lc, iuc, cubic));
+    SG_TRY(ghostcell_fix<T>(s, Lc, iuc, cubic));
+    SG_TRY(extrap_faces<T>(s, Lc, iuc, 0, s.nlev));
+    SG_TRY(interp_blocks<T>(s, Lf, iuf, iuc, cubic));
+    SG_TRY(fill0_faces<T>(s, Lc, iuc, 0, s.nlev));
+    return fill0_faces<T>(s, Lf, iuf, 0, s.nlev);
 }
 
-int fmg_fill0(sg_amr &s, int L, int code)
+template <typename T> int fmg_fill0(sg_amr &s, int L, int code)
 {
     int na = (int)s.active.size();
-    return go(s, k_blocks_fill0, (long)na * lenof(L), s.A[L][code], (const int *)s.d_active, 0, na, (int)lenof(L));
+    return go(s, k_blocks_fill0<T>, (long)na * lenof(L), ar<T>(s).A[L][code], (const int *)s.d_active, 0, na,
+              (int)lenof(L));
 }
 
-int fmg_copy(sg_amr &s, int L, int out, int in, int noskip)
+template <typename T> int fmg_copy(sg_amr &s, int L, int out, int in, int noskip)
 {
     int na = noskip ? s.nblk : (int)s.active.size();
-    return go(s, k_blocks_copy, (long)na * lenof(L), s.A[L][out], (const double *)s.A[L][in],
+    return go(s, k_blocks_copy<T>, (long)na * lenof(L), ar<T>(s).A[L][out], (const T *)ar<T>(s).A[L][in],
               noskip ? (const int *)nullptr : (const int *)s.d_active, 0, na, (int)lenof(L));
 }
 
-int fmg_add(sg_amr &s, int L, int a, int da)
+template <typename T> int fmg_add(sg_amr &s, int L, int a, int da)
 {
     int na = (int)s.active.size();
-    return go(s, k_blocks_add, (long)na * lenof(L), s.A[L][a], (const double *)s.A[L][da], (const int *)s.d_active, 0,
-              na, (int)lenof(L));
+    return go(s, k_blocks_add<T>, (long)na * lenof(L), ar<T>(s).A[L][a], (const T *)ar<T>(s).A[L][da],
+              (const int *)s.d_active, 0, na, (int)lenof(L));
 }
 
-int fmg_addint(sg_amr &s, int Lf, int juf, int juc, int jres)
+template <typename T> int fmg_addint(sg_amr &s, int Lf, int juf, int juc, int jres)
 {
-    SG_TRY(fmg_interp(s, Lf, jres, juc, 0));
-    return fmg_add(s, Lf, juf, jres);
+    SG_TRY(fmg_interp<T>(s, Lf, jres, juc, 0));
+    return fmg_add<T>(s, Lf, juf, jres);
 }
 
 // ---- vmg.F90 at the coarsest FMG level (2^3 per block) ----
 constexpr int LV = kNfmg - 1;
 constexpr long LENV = 64;
 
-int lev_copy(sg_amr &s, int l, int out, int in)
+template <typename T> int lev_copy(sg_amr &s, int l, int out, int in)
 {
     int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
-    return go(s, k_blocks_copy, (long)nb * LENV, s.A[LV][out], (const double *)s.A[LV][in], (const int *)nullptr, b0, nb,
-              (int)LENV);
+    return go(s, k_blocks_copy<T>, (long)nb * LENV, ar<T>(s).A[LV][out], (const T *)ar<T>(s).A[LV][in],
+              (const int *)nullptr, b0, nb, (int)LENV);
 }
 
-int lev_add(sg_amr &s, int l, int a, int da, bool covered_only)
+template <typename T> int lev_add(sg_amr &s, int l, int a, int da, bool covered_only)
 {
+    auto &R = ar<T>(s);
     if (covered_only) {
         int c0 = s.cov_off[l], nc = s.cov_off[l + 1] - c0;
-        return go(s, k_blocks_add, (long)nc * LENV, s.A[LV][a], (const double *)s.A[LV][da], (const int *)(s.d_cov + c0), 0,
-                  nc, (int)LENV);
+        return go(s, k_blocks_add<T>, (long)nc * LENV, R.A[LV][a], (const T *)R.A[LV][da], (const int *)(s.d_cov + c0),
+                  0, nc, (int)LENV);
     }
     int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
-    return go(s, k_blocks_add, (long)nb * LENV, s.A[LV][a], (const double *)s.A[LV][da], (const int *)nullptr, b0, nb,
+    return go(s, k_blocks_add<T>, (long)nb * LENV, R.A[LV][a], (const T *)R.A[LV][da], (const int *)nullptr, b0, nb,
               (int)LENV);
 }
 
-int vmg_ghostcell(sg_amr &s, int l, int code)
+template <typename T> int vmg_ghostcell(sg_amr &s, int l, int code)
 {
-    for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, l, LV, d, code, 0, K_SAME));
-    return fill0_faces(s, LV, code, l, l + 1);
+    for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, LV, d, code, 0, K_SAME));
+    return fill0_faces<T>(s, LV, code, l, l + 1);
 }
 
-int vmg_gridboundary(sg_amr &s, int l, int code)
+template <typename T> int vmg_gridboundary(sg_amr &s, int l, int code)
 {
-    for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, l, LV, d, code, 0, K_PARENT));
-    return fill0_faces(s, LV, code, l, l + 1);
+    for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, LV, d, code, 0, K_PARENT));
+    return fill0_faces<T>(s, LV, code, l, l + 1);
 }
 
-int vmg_relax(sg_amr &s, int l, int ju, int jrhs)
+// vmg_poisson_relax (vmg_poisson.F90:88-145).  The inner (FP32) solve reads same-level neighbours
+// directly instead of refreshing the ghost layer before each colour (the values the smoother sees
+// are the same; the ghost arrays are left as the last refresh made them).
+template <typename T> int vmg_relax(sg_amr &s, int l, int ju, int jrhs)
 {
     int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
     for (int ipass = 1; ipass <= 2; ++ipass) {
-        SG_TRY(vmg_ghostcell(s, l, ju));
-        SG_TRY(relax_pass(s, LV, ju, jrhs, nullptr, b0, nb, ipass, nullptr, s.d_scal + 1));
+        if (!kInner<T>) SG_TRY(vmg_ghostcell<T>(s, l, ju));
+        SG_TRY(relax_pass<T>(s, LV, ju, jrhs, nullptr, b0, nb, ipass, nullptr, nullptr, kInner<T>));
     }
     return SG_OK;
 }
 
-int vmg_tau(sg_amr &s, int lc, int jtau, int ju)
+template <typename T> int vmg_tau(sg_amr &s, int lc, int jtau, int ju)
 {
     const int lf = lc + 1;
     if (lf > s.nlev - 1) return SG_OK;
-    SG_TRY(vmg_ghostcell(s, lf, ju));
+    auto &R = ar<T>(s);
+    SG_TRY(vmg_ghostcell<T>(s, lf, ju));
     int b0 = s.lev_start[lf], nb = s.lev_start[lf + 1] - b0;
-    SG_TRY(go(s, k_tau_fine<2>, (long)nb * LENV, s.A[LV][jtau], (const double *)s.A[LV][ju], b0, nb, s.hil[LV][lf]));
-    SG_TRY(c2p_lev(s, LV, jtau, lc, 0));
-    SG_TRY(vmg_ghostcell(s, lc, ju));
+    SG_TRY(go(s, k_tau_fine<2, T>, (long)nb * LENV, R.A[LV][jtau], (const T *)R.A[LV][ju], b0, nb, s.hil[LV][lf]));
+    SG_TRY(c2p_lev<T>(s, LV, jtau, lc, 0));
+    SG_TRY(vmg_ghostcell<T>(s, lc, ju));
     int c0 = s.cov_off[lc], nc = s.cov_off[lc + 1] - c0;
-    SG_TRY(go(s, k_tau_add<2>, (long)nc * 8, s.A[LV][jtau], (const double *)s.A[LV][ju], (const int *)(s.d_cov + c0), nc,
+    SG_TRY(go(s, k_tau_add<2, T>, (long)nc * 8, R.A[LV][jtau], (const T *)R.A[LV][ju], (const int *)(s.d_cov + c0), nc,
               s.hil[LV][lc]));
     int e0 = s.cf_off[lf], ne = s.cf_off[lf + 1] - e0;
-    return go(s, k_tau_crop<2>, (long)ne * 16, s.A[LV][jtau], (const int *)(s.d_cf + e0), ne, (const int *)s.d_parent);
+    return go(s, k_tau_crop<2, T>, (long)ne * 16, R.A[LV][jtau], (const int *)(s.d_cf + e0), ne, (const int *)s.d_parent);
 }
 
-int vmg_interp(sg_amr &s, int l, int juf, int juc)
+template <typename T> int vmg_interp(sg_amr &s, int l, int juf, int juc)
 {
-    SG_TRY(vmg_ghostcell(s, l - 1, juc));
-    SG_TRY(extrap_faces(s, LV, juc, l - 1, l));
+    SG_TRY(vmg_ghostcell<T>(s, l - 1, juc));
+    SG_TRY(extrap_faces<T>(s, LV, juc, l - 1, l));
     int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
-    SG_TRY(go(s, k_p2c<2>, (long)nb * 8, s.A[LV][juf], (const double *)s.A[LV][juc], b0, nb, (const int *)s.d_parent,
-              (const int *)s.d_octant));
-    return fill0_faces(s, LV, juc, l - 1, l);
+    SG_TRY(go(s, k_p2c<2, T>, (long)nb * 8, ar<T>(s).A[LV][juf], (const T *)ar<T>(s).A[LV][juc], b0, nb,
+              (const int *)s.d_parent, (const int *)s.d_octant));
+    return fill0_faces<T>(s, LV, juc, l - 1, l);
 }
 
-int vmg_slvsml(sg_amr &s, int ju, int jrhs)
+// vmg_slvsml (vmg.F90:387-432).  FP64: the reference's mg_cycle on the gathered base grid.  Inner
+// (FP32) solve: the same base problem solved exactly by separable transforms in one workgroup.
+template <typename T> int vmg_slvsml(sg_amr &s, int ju, int jrhs)
 {
-    if (!s.base_mg_ok) {
-        for (int it = 0; it < 10; ++it) SG_TRY(vmg_relax(s, 0, ju, jrhs));
+    if (kInner<T> && s.bd_ok) {
+        hipLaunchKernelGGL(k_base_direct, dim3(1), dim3(kDirectThreads), s.bd_lds_bytes, s.stream, s.f.A[LV][ju],
+                           (const float *)s.f.A[LV][jrhs], s.nbase, (const int *)s.d_ijk, s.bd);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
+    }
+    if (kInner<T> || !s.base_mg_ok) {
+        for (int it = 0; it < 10; ++it) SG_TRY(vmg_relax<T>(s, 0, ju, jrhs));
         return SG_OK;
     }
     const int NG = s.bm.n[0];
-    SG_TRY(go(s, k_base_gather<2>, (long)s.nbase * 8, s.bm.psi, s.bm.src, (const double *)s.A[LV][ju],
-              (const double *)s.A[LV][jrhs], s.nbase, (const int *)s.d_ijk, NG));
-    SG_TRY(go1(s, k_base_mg, s.bm));
-    return go(s, k_base_scatter<2>, (long)s.nbase * 8, s.A[LV][ju], (const double *)s.bm.psi, s.nbase,
+    SG_TRY(go(s, k_base_gather<2, double>, (long)s.nbase * 8, s.bm.psi, s.bm.src, (const double *)s.d.A[LV][ju],
+              (const double *)s.d.A[LV][jrhs], s.nbase, (const int *)s.d_ijk, NG));
+    hipLaunchKernelGGL(k_base_mg, dim3(1), dim3(kBaseThreads), sizeof(double) * s.bm_lds, s.stream, s.bm);
+    SG_CHECK_LAUNCH();
+    return go(s, k_base_scatter<2, double>, (long)s.nbase * 8, s.d.A[LV][ju], (const double *)s.bm.psi, s.nbase,
               (const int *)s.d_ijk, NG);
 }
 
 // vmg_fas (vmg.F90:116-243): Ncycle = 1, Npre = Npost = 2
-int vmg_fas(sg_amr &s, int jrho)
+template <typename T> int vmg_fas(sg_amr &s, int jrho)
 {
+    auto &R = ar<T>(s);
     if (jrho != C_RHO)
-        for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy(s, l, C_RHO, jrho));
-    SG_TRY(c2p(s, LV, C_U, 0));
-    SG_TRY(c2p(s, LV, C_RHO, 0));
-    SG_TRY(fill0_faces(s, LV, C_U, 0, s.nlev));
-    for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy(s, l, C_RHS, C_RHO));
+        for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy<T>(s, l, C_RHO, jrho));
+    SG_TRY(c2p<T>(s, LV, C_U, 0));
+    SG_TRY(c2p<T>(s, LV, C_RHO, 0));
+    SG_TRY(fill0_faces<T>(s, LV, C_U, 0, s.nlev));
+    for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy<T>(s, l, C_RHS, C_RHO));
     for (int v = s.nlev - 1; v >= 1; --v) {
-        SG_TRY(vmg_gridboundary(s, v, C_U));
-        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax(s, v, C_U, C_RHS));
+        SG_TRY(vmg_gridboundary<T>(s, v, C_U));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS));
         // vmg_rstrct(v-1, U, cubic): per direction parent-level then same-level faces (one
         // launch: the two write disjoint faces and read nothing the other writes)
-        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass(s, v, LV, d, C_U, 1, K_BOTH));
-        SG_TRY(c2p_lev(s, LV, C_U, v - 1, 1));
-        SG_TRY(lev_copy(s, v - 1, C_PSI, C_U));            // IRUF is the PSI slot (vmg.F90:115)
-        SG_TRY(c2p_lev(s, LV, C_RHS, v - 1, 0));
-        SG_TRY(vmg_tau(s, v - 1, C_RES, C_U));
-        SG_TRY(lev_add(s, v - 1, C_RHS, C_RES, true));     // vmg_add_onov
+        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, v, LV, d, C_U, 1, K_BOTH));
+        SG_TRY(c2p_lev<T>(s, LV, C_U, v - 1, 1));
+        SG_TRY(lev_copy<T>(s, v - 1, C_PSI, C_U));            // IRUF is the PSI slot (vmg.F90:115)
+        SG_TRY(c2p_lev<T>(s, LV, C_RHS, v - 1, 0));
+        SG_TRY(vmg_tau<T>(s, v - 1, C_RES, C_U));
+        SG_TRY(lev_add<T>(s, v - 1, C_RHS, C_RES, true));     // vmg_add_onov
     }
-    if (s.zero_average)
-        SG_TRY(go1(s, k_zero_average<2>, s.A[LV][C_RHS], s.nbase, s.P_base, s.d_part));
-    SG_TRY(vmg_slvsml(s, C_U, C_RHS));
+    if (s.zero_average) SG_TRY(go1(s, k_zero_average<2, T>, R.A[LV][C_RHS], s.nbase, s.P_base, s.d_part));
+    SG_TRY(vmg_slvsml<T>(s, C_U, C_RHS));
     for (int v = 1; v < s.nlev; ++v) {
         int b0 = s.lev_start[v - 1], nb = s.lev_start[v] - b0;
-        SG_TRY(go(s, k_blocks_sub, (long)nb * LENV, s.A[LV][C_RES], (const double *)s.A[LV][C_U],
-                  (const double *)s.A[LV][C_PSI], b0, nb, (int)LENV));
-        SG_TRY(vmg_interp(s, v, C_RES, C_RES));
-        SG_TRY(lev_add(s, v, C_U, C_RES, false));
-        SG_TRY(vmg_gridboundary(s, v, C_U));
-        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax(s, v, C_U, C_RHS));
+        SG_TRY(go(s, k_blocks_sub<T>, (long)nb * LENV, R.A[LV][C_RES], (const T *)R.A[LV][C_U], (const T *)R.A[LV][C_PSI],
+                  b0, nb, (int)LENV));
+        SG_TRY(vmg_interp<T>(s, v, C_RES, C_RES));
+        SG_TRY(lev_add<T>(s, v, C_U, C_RES, false));
+        SG_TRY(vmg_gridboundary<T>(s, v, C_U));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS));
     }
     return SG_OK;
 }
 
 // fmg_lin (fmg.F90:270-377)
-int fmg_lin(sg_amr &s)
+template <typename T> int fmg_lin(sg_amr &s)
 {
     const int Lmax = kNfmg - 1;
-    for (int L = 1; L <= Lmax; ++L) SG_TRY(fmg_rstrct(s, L, C_RHO, C_RHO));
-    SG_TRY(fmg_fill0(s, Lmax, C_U));
-    SG_TRY(vmg_fas(s, C_RHO));
+    for (int L = 1; L <= Lmax; ++L) SG_TRY(fmg_rstrct<T>(s, L, C_RHO, C_RHO));
+    SG_TRY(fmg_fill0<T>(s, Lmax, C_U));
+    SG_TRY(vmg_fas<T>(s, C_RHO));
     for (int Lf = Lmax - 1; Lf >= 0; --Lf) {
-        SG_TRY(fmg_interp(s, Lf, C_U, C_U, 1));
-        SG_TRY(fmg_copy(s, Lf, C_RHS, C_RHO, 0));
+        SG_TRY(fmg_interp<T>(s, Lf, C_U, C_U, 1));
+        SG_TRY(fmg_copy<T>(s, Lf, C_RHS, C_RHO, 0));
         for (int jc = 0; jc < 2; ++jc) {
             for (int v = Lf; v <= Lmax - 1; ++v) {
-                for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax(s, v, C_U, C_RHS, 1));
-                SG_TRY(poisson_resid(s, v, C_RES, C_U, C_RHS, 1));
-                SG_TRY(fmg_rstrct(s, v + 1, C_RHS, C_RES));
-                SG_TRY(fmg_fill0(s, v + 1, C_U));
+                for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax<T>(s, v, C_U, C_RHS, 1, false));
+                SG_TRY(poisson_resid<T>(s, v, C_RES, C_U, C_RHS, 1));
+                SG_TRY(fmg_rstrct<T>(s, v + 1, C_RHS, C_RES));
+                SG_TRY(fmg_fill0<T>(s, v + 1, C_U));
                 s.fill0_level = v + 1;
             }
-            SG_TRY(vmg_fas(s, C_RHS));
+            SG_TRY(vmg_fas<T>(s, C_RHS));
             for (int v = Lmax - 1; v >= Lf; --v) {
-                SG_TRY(fmg_addint(s, v, C_U, C_U, C_RES));
-                for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax(s, v, C_U, C_RHS, 1));
+                SG_TRY(fmg_addint<T>(s, v, C_U, C_U, C_RES));
+                for (int p = 0; p < 2; ++p)
+                    SG_TRY(poisson_relax<T>(s, v, C_U, C_RHS, 1, Lf == 0 && jc == 1 && v == 0 && p == 1));
             }
         }
     }
-    return c2p(s, 0, C_U, 0);
+    return c2p<T>(s, 0, C_U, 0);
 }
 
 Geo geo(sg_amr &s) { return Geo{s.d_ijk, s.d_level, s.d_h[0], s.x0[0], s.x0[1], s.x0[2]}; }
@@ -449,12 +534,12 @@ int sphere_bc(sg_amr &s, int ju, int jrho)
     if (-zmin < m) m = -zmin;
     const double rad2 = m * m;
     Geo G = geo(s);
-    SG_TRY(go(s, k_sphere_mask<8>, (long)s.nblk * lenof(0), s.A[0][jrho], s.nblk, G, rad2));
+    SG_TRY(go(s, k_sphere_mask<8>, (long)s.nblk * lenof(0), s.d.A[0][jrho], s.nblk, G, rad2));
     const double dv = (s.h0 * s.h0) * s.h0;
-    SG_TRY(go(s, k_sphere_moments<8>, (long)s.nbase, (const double *)s.A[0][jrho], s.nbase, s.P_base, G, dv, s.d_part));
+    SG_TRY(go(s, k_sphere_moments<8>, (long)s.nbase, (const double *)s.d.A[0][jrho], s.nbase, s.P_base, G, dv, s.d_part));
     SG_TRY(go1(s, k_sphere_coef, s.d_part, s.nbase, s.P_base, s.d_scal + 8));
     int ne = (int)s.phys.size();
-    return go(s, k_sphere_ghost<8>, (long)ne * 100, s.A[0][ju], (const int *)s.d_phys, ne, G,
+    return go(s, k_sphere_ghost<8>, (long)ne * 100, s.d.A[0][ju], (const int *)s.d_phys, ne, G,
               (const double *)(s.d_scal + 8));
 }
 
@@ -462,22 +547,25 @@ int sphere_bc(sg_amr &s, int ju, int jrho)
 int remove_leaf_mean(sg_amr &s, int code)
 {
     double *pm = s.d_part, *pv = s.d_part + s.P_all;
-    SG_TRY(go(s, k_leaf_mass<8>, (long)s.nblk, (const double *)s.A[0][code], s.nblk, (const int *)s.d_covered,
+    SG_TRY(go(s, k_leaf_mass<8>, (long)s.nblk, (const double *)s.d.A[0][code], s.nblk, (const int *)s.d_covered,
               (const double *)s.d_h[0], (const int *)s.d_level, pm, pv));
     SG_TRY(go1(s, k_tree_mass, pm, pv, s.nblk, s.P_all, s.d_scal + 5));
-    return go(s, k_sub_scalar, (long)s.nblk * lenof(0), s.A[0][code], (long)s.nblk * lenof(0),
+    return go(s, k_sub_scalar, (long)s.nblk * lenof(0), s.d.A[0][code], (long)s.nblk * lenof(0),
               (const double *)(s.d_scal + 5));
 }
 
 // norms of an FMG-level-0 array over the leaf blocks -> d_scal[2..4]
+constexpr int kNormGroups = 1024;
 int norms(sg_amr &s, int code)
 {
-    SG_TRY(set_scalar(s, 2, 0.0));
-    SG_TRY(set_scalar(s, 3, 0.0));
-    SG_TRY(set_scalar(s, 4, 0.0));
     int na = (int)s.active.size();
-    return go(s, k_norms<8>, (long)na * 512, (const double *)s.A[0][code], (const int *)s.d_active, na,
-              (const double *)s.d_h[0], (const int *)s.d_level, s.d_scal + 2);
+    hipLaunchKernelGGL(k_norms_part<8>, dim3(kNormGroups), dim3(kThreads), 0, s.stream, (const double *)s.d.A[0][code],
+                       (const int *)s.d_active, na, (const double *)s.d_h[0], (const int *)s.d_level, s.d_npart);
+    SG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(1024), 0, s.stream, (const double *)s.d_npart, kNormGroups,
+                       s.d_scal + 2);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
 }
 
 int read_scalars(sg_amr &s)
@@ -507,7 +595,7 @@ int zero_state(sg_amr &s)
 {
     for (int L = 0; L < kNfmg; ++L)
         for (int c = 0; c < NCODE; ++c)
-            SG_HIP(hipMemsetAsync(s.A[L][c], 0, sizeof(double) * s.nblk * lenof(L), s.stream));
+            SG_HIP(hipMemsetAsync(s.d.A[L][c], 0, sizeof(double) * s.nblk * lenof(L), s.stream));
     s.fill0_level = -1;
     return SG_OK;
 }
@@ -518,13 +606,13 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     std::memset(&st, 0, sizeof(st));
     SG_TRY(zero_state(s));
     SG_HIP(hipEventRecord(s.ev0, s.stream));
-    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.A[0][C_U], (const double *)psi, s.nblk));
-    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.A[0][C_RHO], src, s.nblk));
+    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.d.A[0][C_U], (const double *)psi, s.nblk));
+    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.d.A[0][C_RHO], src, s.nblk));
     if (s.zero_average) SG_TRY(remove_leaf_mean(s, C_RHO));
-    SG_TRY(fmg_copy(s, 0, C_PSI, C_U, 0));
-    SG_TRY(fmg_copy(s, 0, C_SRC, C_RHO, 0));
-    SG_TRY(c2p(s, 0, C_SRC, 0));
-    SG_TRY(c2p(s, 0, C_PSI, 0));
+    SG_TRY(fmg_copy<double>(s, 0, C_PSI, C_U, 0));
+    SG_TRY(fmg_copy<double>(s, 0, C_SRC, C_RHO, 0));
+    SG_TRY(c2p<double>(s, 0, C_SRC, 0));
+    SG_TRY(c2p<double>(s, 0, C_PSI, 0));
     if (s.bc == 1) SG_TRY(sphere_bc(s, C_PSI, C_SRC));
     SG_TRY(norms(s, C_SRC));
     SG_TRY(read_scalars(s));
@@ -542,12 +630,12 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
         return SG_OK;
     };
     for (;;) {
-        SG_TRY(poisson_resid(s, 0, C_RHO, C_PSI, C_SRC, 0));
+        SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
         SG_TRY(measure(it));
         if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = 0; break; }
         if (it >= nmax) break;
-        SG_TRY(fmg_lin(s));
-        SG_TRY(fmg_add(s, 0, C_PSI, C_U));
+        SG_TRY(fmg_lin<double>(s));
+        SG_TRY(fmg_add<double>(s, 0, C_PSI, C_U));
         ++it;
         SG_TRY(read_scalars(s));
         double ratio = s.h_scal[0] / resh2max;
@@ -560,18 +648,18 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     }
     if (nmax == 0) status = 0;
     if (p.stop_mode == SG_STOP_REFERENCE) {
-        SG_TRY(poisson_resid(s, 0, C_RHO, C_PSI, C_SRC, 0));
+        SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
         SG_TRY(measure(-1));
     }
     st.iterations = it;
     st.status = status;
-    SG_TRY(fmg_copy(s, 0, C_U, C_PSI, 1));
-    SG_TRY(c2p(s, 0, C_U, 0));
-    SG_TRY(go(s, k_blocks_to_dense, (long)s.nblk * 512, psi, (const double *)s.A[0][C_U], s.nblk));
+    SG_TRY(fmg_copy<double>(s, 0, C_U, C_PSI, 1));
+    SG_TRY(c2p<double>(s, 0, C_U, 0));
+    SG_TRY(go(s, k_blocks_to_dense, (long)s.nblk * 512, psi, (const double *)s.d.A[0][C_U], s.nblk));
     if (g) {
-        SG_TRY(poisson_flux(s, 0, C_U, 0));
-        SG_TRY(go(s, k_psi2g<8>, (long)s.nblk * 512, g, (const double *)s.A[0][C_U], s.nblk, (const double *)s.d_hi[0],
-                  (const int *)s.d_level, (const int *)s.d_skip, (const int *)s.d_ovslot, (const double *)s.ov[0]));
+        SG_TRY(poisson_flux<double>(s, 0, C_U, 0));
+        SG_TRY(go(s, k_psi2g<8>, (long)s.nblk * 512, g, (const double *)s.d.A[0][C_U], s.nblk, (const double *)s.d_hi[0],
+                  (const int *)s.d_level, (const int *)s.d_skip, (const int *)s.d_ovslot, (const double *)s.d.ov[0]));
     }
     SG_HIP(hipEventRecord(s.ev1, s.stream));
     SG_HIP(hipStreamSynchronize(s.stream));
@@ -579,6 +667,139 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
     st.ms = ms;
     return SG_OK;
+}
+
+// ---- mixed precision: FP64 defect correction around the FP32 inner fmg_lin ----
+int ensure_mixed(sg_amr &s)
+{
+    if (s.have_f) return SG_OK;
+    for (int L = 0; L < kNfmg; ++L) {
+        for (int c = 0; c < NCODE; ++c) SG_HIP(hipMalloc((void **)&s.f.A[L][c], sizeof(float) * s.nblk * lenof(L)));
+        SG_HIP(hipMalloc((void **)&s.f.ov[L], sizeof(float) * std::max(1, s.nslots) * nof(L) * nof(L)));
+    }
+    const int bx = s.base_dims[0], n = 2 * bx;
+    s.bd_ok = bx == s.base_dims[1] && bx == s.base_dims[2] && n <= kDirectMaxN;
+    if (s.bd_ok) {
+        const double h = s.hl[LV][0], pi = 3.14159265358979323846;
+        std::vector<float> host((size_t)n * n + n);
+        for (int m = 0; m < n; ++m) {
+            for (int k = 0; k < n; ++k) {
+                double q = s.bc == SG_BC_SPHERE
+                               ? std::sqrt(2.0 / (n + 1)) * std::sin(pi * (m + 1) * (k + 1) / (n + 1))
+                               : (std::cos(2 * pi * m * k / n) + std::sin(2 * pi * m * k / n)) / std::sqrt((double)n);
+                host[(size_t)m * n + k] = (float)q;
+            }
+            double th = s.bc == SG_BC_SPHERE ? pi * (m + 1) / (n + 1) : 2 * pi * m / n;
+            host[(size_t)n * n + m] = (float)((2.0 * std::cos(th) - 2.0) / (h * h));
+        }
+        if (s.bc != SG_BC_SPHERE) host[(size_t)n * n] = 0.f;   // the periodic zero mode
+        SG_HIP(hipMalloc((void **)&s.bd_mem, sizeof(float) * host.size()));
+        SG_HIP(hipMemcpy(s.bd_mem, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice));
+        s.bd.n = n;
+        s.bd.Q = s.bd_mem;
+        s.bd.lam = s.bd_mem + (size_t)n * n;
+        s.bd_lds_bytes = base_direct_lds_bytes(n);
+        SG_HIP(hipFuncSetAttribute((const void *)k_base_direct, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)s.bd_lds_bytes));
+    }
+    s.have_f = 1;
+    return SG_OK;
+}
+
+// One outer iteration: FP32 copy of the FP64 residual, inner fmg_lin in FP32, FP64 update
+// psi += e, next FP64 composite residual (the reference operator, staged ghosts and reflux) and its
+// norms.  Only launches: it is captured into a hipGraph once and replayed.
+int mixed_iteration(sg_amr &s)
+{
+    const int nb = s.nblk;
+    SG_TRY(go(s, k_cast_blocks, (long)nb * lenof(0), s.f.A[0][C_RHO], (const double *)s.d.A[0][C_RHO], (const int *)nullptr,
+              nb, (int)lenof(0)));
+    SG_TRY(fmg_lin<float>(s));
+    const int na = (int)s.active.size();
+    SG_TRY(go(s, k_add_cast_blocks, (long)na * lenof(0), s.d.A[0][C_PSI], (const float *)s.f.A[0][C_U],
+              (const int *)s.d_active, na, (int)lenof(0)));
+    SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
+    return norms(s, C_RHO);
+}
+
+int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_params &p, sg_stats &st)
+{
+    std::memset(&st, 0, sizeof(st));
+    SG_TRY(ensure_mixed(s));
+    SG_TRY(zero_state(s));
+    for (int L = 0; L < kNfmg; ++L)
+        for (int c = 0; c < NCODE; ++c)
+            SG_HIP(hipMemsetAsync(s.f.A[L][c], 0, sizeof(float) * s.nblk * lenof(L), s.stream));
+    SG_HIP(hipEventRecord(s.ev0, s.stream));
+    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.d.A[0][C_U], (const double *)psi, s.nblk));
+    SG_TRY(go(s, k_dense_to_blocks, (long)s.nblk * 512, s.d.A[0][C_RHO], src, s.nblk));
+    if (s.zero_average) SG_TRY(remove_leaf_mean(s, C_RHO));
+    SG_TRY(fmg_copy<double>(s, 0, C_PSI, C_U, 0));
+    SG_TRY(fmg_copy<double>(s, 0, C_SRC, C_RHO, 0));
+    SG_TRY(c2p<double>(s, 0, C_SRC, 0));
+    SG_TRY(c2p<double>(s, 0, C_PSI, 0));
+    if (s.bc == 1) SG_TRY(sphere_bc(s, C_PSI, C_SRC));
+    SG_TRY(norms(s, C_SRC));
+    SG_TRY(read_scalars(s));
+    const double resh2max = s.h_scal[2], srcmax = s.h_scal[3], srcsq = s.h_scal[4];
+    int nmax = p.maxit;
+    if (std::fabs(resh2max) <= 2.2250738585072014e-308) nmax = 0;
+    SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
+    SG_TRY(norms(s, C_RHO));
+    SG_TRY(read_scalars(s));
+    double prev = HUGE_VAL;
+    int it = 0, status = 1;
+    for (;;) {
+        st.rel_linf = srcmax > 0 ? s.h_scal[3] / srcmax : 0.0;
+        st.rel_l2 = srcsq > 0 ? std::sqrt(s.h_scal[4] / srcsq) : 0.0;
+        if (it < 64) st.hist[it] = st.rel_linf;
+        if (!std::isfinite(st.rel_linf)) { status = SG_ERR_NONFINITE; break; }
+        if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = 0; break; }
+        if (it >= nmax) break;
+        if (p.use_graph) {
+            if (!s.graph) {
+                hipGraph_t gr = nullptr;
+                SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+                int rc = mixed_iteration(s);
+                hipError_t e = hipStreamEndCapture(s.stream, &gr);
+                if (rc != SG_OK) return rc;
+                SG_HIP(e);
+                SG_HIP(hipGraphInstantiate(&s.graph, gr, nullptr, nullptr, 0));
+                SG_HIP(hipGraphDestroy(gr));
+            }
+            SG_HIP(hipGraphLaunch(s.graph, s.stream));
+        } else {
+            SG_TRY(mixed_iteration(s));
+        }
+        ++it;
+        SG_TRY(read_scalars(s));
+        double ratio = s.h_scal[0] / resh2max;
+        st.resmaxg = ratio;
+        if (p.stop_mode == SG_STOP_REFERENCE) {
+            st.rel_linf = srcmax > 0 ? s.h_scal[3] / srcmax : 0.0;
+            st.rel_l2 = srcsq > 0 ? std::sqrt(s.h_scal[4] / srcsq) : 0.0;
+            if (ratio < p.tol) { status = 0; break; }
+            if (ratio > prev * 0.5) break;
+            prev = ratio;
+        }
+    }
+    if (nmax == 0) status = 0;
+    st.iterations = it;
+    st.status = status;
+    SG_TRY(fmg_copy<double>(s, 0, C_U, C_PSI, 1));
+    SG_TRY(c2p<double>(s, 0, C_U, 0));
+    SG_TRY(go(s, k_blocks_to_dense, (long)s.nblk * 512, psi, (const double *)s.d.A[0][C_U], s.nblk));
+    if (g) {
+        SG_TRY(poisson_flux<double>(s, 0, C_U, 0));
+        SG_TRY(go(s, k_psi2g<8>, (long)s.nblk * 512, g, (const double *)s.d.A[0][C_U], s.nblk, (const double *)s.d_hi[0],
+                  (const int *)s.d_level, (const int *)s.d_skip, (const int *)s.d_ovslot, (const double *)s.d.ov[0]));
+    }
+    SG_HIP(hipEventRecord(s.ev1, s.stream));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    st.ms = ms;
+    return status == SG_ERR_NONFINITE ? SG_ERR_NONFINITE : SG_OK;
 }
 
 template <typename T> int upload(sg_amr &s, T **dptr, const std::vector<T> &v)
@@ -595,15 +816,21 @@ void release(sg_amr *s)
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
                  s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters};
     for (int *q : ip) if (q) (void)hipFree(q);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    if (s->bd_mem) (void)hipFree(s->bd_mem);
     for (int L = 0; L < kNfmg; ++L) {
-        for (int c = 0; c < NCODE; ++c) if (s->A[L][c]) (void)hipFree(s->A[L][c]);
-        if (s->ov[L]) (void)hipFree(s->ov[L]);
+        for (int c = 0; c < NCODE; ++c) if (s->d.A[L][c]) (void)hipFree(s->d.A[L][c]);
+        if (s->d.ov[L]) (void)hipFree(s->d.ov[L]);
+        for (int c = 0; c < NCODE; ++c) if (s->f.A[L][c]) (void)hipFree(s->f.A[L][c]);
+        if (s->f.ov[L]) (void)hipFree(s->f.ov[L]);
         if (s->d_h[L]) (void)hipFree(s->d_h[L]);
         if (s->d_hi[L]) (void)hipFree(s->d_hi[L]);
     }
     if (s->d_scal) (void)hipFree(s->d_scal);
     if (s->h_scal) (void)hipHostFree(s->h_scal);
     if (s->d_part) (void)hipFree(s->d_part);
+    if (s->d_npart) (void)hipFree(s->d_npart);
+    if (s->d_slots) (void)hipFree(s->d_slots);
     if (s->d_dense) (void)hipFree(s->d_dense);
     if (s->bm_mem) (void)hipFree(s->bm_mem);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -719,8 +946,8 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     SG_TRY(upload(*s, &s->d_ovslot, s->ovslot));
     SG_TRY(upload(*s, &s->d_cov, s->cov));
     for (int L = 0; L < kNfmg; ++L) {
-        for (int c = 0; c < NCODE; ++c) SG_HIP(hipMalloc((void **)&s->A[L][c], sizeof(double) * nblk * lenof(L)));
-        SG_HIP(hipMalloc((void **)&s->ov[L], sizeof(double) * std::max(1, s->nslots) * nof(L) * nof(L)));
+        for (int c = 0; c < NCODE; ++c) SG_HIP(hipMalloc((void **)&s->d.A[L][c], sizeof(double) * nblk * lenof(L)));
+        SG_HIP(hipMalloc((void **)&s->d.ov[L], sizeof(double) * std::max(1, s->nslots) * nof(L) * nof(L)));
         for (int l = 0; l < nlev; ++l) {
             s->hl[L][l] = std::ldexp(h0, L - l);          // fmg_data.F90:288 CellWidth * 2^fmglev
             s->hil[L][l] = 1.0 / s->hl[L][l];
@@ -736,6 +963,8 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     s->P_base = pow2ge(s->nbase);
     SG_HIP(hipMalloc((void **)&s->d_part, sizeof(double) * std::max<long>(2L * s->P_all, 16L * s->P_base)));
     SG_HIP(hipMalloc((void **)&s->d_iters, sizeof(int)));
+    SG_HIP(hipMalloc((void **)&s->d_npart, sizeof(double) * 3 * 1024));
+    SG_HIP(hipMalloc((void **)&s->d_slots, sizeof(double) * kMaxSlots));
     // cubic restriction weights St (restriction.F90:38-63)
     {
         const double ic = 1.5;
@@ -757,7 +986,7 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
         int nl = 0;
         while ((1 << nl) < NG) ++nl;
         nl += 1;
-        SG_REQUIRE(nl <= kBaseMaxLev, "sg_amr_create: base grid %d too large", NG);
+        SG_REQUIRE(nl <= kBaseMaxLev && NG <= kBaseMaxN, "sg_amr_create: base grid %d too large", NG);
         long total = 2L * (NG + 2) * (NG + 2) * (NG + 2);
         for (int l = 0; l < nl; ++l) { long w = (NG >> l) + 2; total += 4 * w * w * w; }
         SG_HIP(hipMalloc((void **)&s->bm_mem, sizeof(double) * total));
@@ -780,7 +1009,20 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
             M.RHS[l] = q; q += len;
             M.RES[l] = q; q += len;
             M.RHO[l] = q; q += len;
+            for (int a = 0; a < 4; ++a) M.lds_off[l][a] = -1;
         }
+        // coarsest levels first: all four arrays up to 8^3, then U and RHS of larger levels
+        for (int l = nl - 1; l >= 1; --l) {
+            long w = M.n[l] + 2, len = w * w * w;
+            int na = M.n[l] <= 8 ? 4 : 2;
+            if ((s->bm_lds + na * len) * 8 > kBaseLdsBytes) break;
+            for (int a = 0; a < na; ++a) {
+                M.lds_off[l][a] = (int)s->bm_lds;
+                s->bm_lds += len;
+            }
+        }
+        SG_HIP(hipFuncSetAttribute((const void *)k_base_mg, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(s->bm_lds * 8)));
     }
     SG_HIP(hipDeviceSynchronize());
     return SG_OK;
@@ -792,31 +1034,31 @@ int op(sg_amr &s, int opc, int L, int a, int b, int c, int flag, double *ret)
     SG_REQUIRE(a >= 0 && a < NCODE && b >= 0 && b < NCODE && c >= 0 && c < NCODE, "sg_amr_op: bad array code");
     double r = 0.0;
     switch (opc) {
-    case SG_AMR_OP_C2P: SG_TRY(c2p(s, L, a, flag)); break;
-    case SG_AMR_OP_GHOSTFIX: SG_TRY(ghostcell_fix(s, L, a, flag)); break;
-    case SG_AMR_OP_FILL0: SG_TRY(fill0_faces(s, L, a, 0, s.nlev)); break;
-    case SG_AMR_OP_EXTRAP: SG_TRY(extrap_faces(s, L, a, 0, s.nlev)); break;
+    case SG_AMR_OP_C2P: SG_TRY(c2p<double>(s, L, a, flag)); break;
+    case SG_AMR_OP_GHOSTFIX: SG_TRY(ghostcell_fix<double>(s, L, a, flag)); break;
+    case SG_AMR_OP_FILL0: SG_TRY(fill0_faces<double>(s, L, a, 0, s.nlev)); break;
+    case SG_AMR_OP_EXTRAP: SG_TRY(extrap_faces<double>(s, L, a, 0, s.nlev)); break;
     case SG_AMR_OP_RELAX:
-        SG_TRY(poisson_relax(s, L, a, b, flag));
+        SG_TRY(poisson_relax<double>(s, L, a, b, flag, true));
         SG_TRY(read_scalars(s));
         r = s.h_scal[0];
         break;
-    case SG_AMR_OP_RESID: SG_TRY(poisson_resid(s, L, a, b, c, flag)); break;
+    case SG_AMR_OP_RESID: SG_TRY(poisson_resid<double>(s, L, a, b, c, flag)); break;
     case SG_AMR_OP_RSTRCT:
         SG_REQUIRE(L >= 1, "sg_amr_op: restriction needs FMG level >= 1");
-        SG_TRY(fmg_rstrct(s, L, a, b));
+        SG_TRY(fmg_rstrct<double>(s, L, a, b));
         break;
     case SG_AMR_OP_INTERP:
         SG_REQUIRE(L <= 1, "sg_amr_op: interpolation needs FMG level <= 1");
-        SG_TRY(fmg_interp(s, L, a, b, flag));
+        SG_TRY(fmg_interp<double>(s, L, a, b, flag));
         break;
     case SG_AMR_OP_ADDINT:
         SG_REQUIRE(L <= 1, "sg_amr_op: interpolation needs FMG level <= 1");
-        SG_TRY(fmg_addint(s, L, a, b, c));
+        SG_TRY(fmg_addint<double>(s, L, a, b, c));
         break;
-    case SG_AMR_OP_VMG_FAS: SG_TRY(vmg_fas(s, a)); break;
+    case SG_AMR_OP_VMG_FAS: SG_TRY(vmg_fas<double>(s, a)); break;
     case SG_AMR_OP_FMG_LIN:
-        SG_TRY(fmg_lin(s));
+        SG_TRY(fmg_lin<double>(s));
         SG_TRY(read_scalars(s));
         r = s.h_scal[0];
         break;
@@ -867,7 +1109,9 @@ int sg_amr_set_zero_average(sg_amr *ctx, int on)
 int sg_amr_solve_dev(sg_amr *ctx, double *psi, const double *src, double *g, const sg_params *p, sg_stats *st)
 {
     SG_REQUIRE(ctx && psi && src && p && st, "sg_amr_solve_dev: null argument");
-    SG_REQUIRE(p->precision == SG_PREC_FP64, "sg_amr_solve_dev: only SG_PREC_FP64 is provided for AMR hierarchies");
+    SG_REQUIRE(p->precision == SG_PREC_FP64 || p->precision == SG_PREC_MIXED, "sg_amr_solve_dev: unknown precision %d",
+               p->precision);
+    if (p->precision == SG_PREC_MIXED) return solve_mixed(*ctx, psi, src, g, *p, *st);
     return solve(*ctx, psi, src, g, *p, *st);
 }
 
@@ -891,7 +1135,7 @@ int sg_amr_set_array(sg_amr *ctx, int fmglev, int code, const double *host)
 {
     SG_REQUIRE(ctx && host, "sg_amr_set_array: null argument");
     SG_REQUIRE(fmglev >= 0 && fmglev < kNfmg && code >= 0 && code < NCODE, "sg_amr_set_array: bad level/code");
-    SG_HIP(hipMemcpyAsync(ctx->A[fmglev][code], host, sizeof(double) * ctx->nblk * lenof(fmglev), hipMemcpyHostToDevice,
+    SG_HIP(hipMemcpyAsync(ctx->d.A[fmglev][code], host, sizeof(double) * ctx->nblk * lenof(fmglev), hipMemcpyHostToDevice,
                           ctx->stream));
     SG_HIP(hipStreamSynchronize(ctx->stream));
     return SG_OK;
@@ -901,7 +1145,7 @@ int sg_amr_get_array(sg_amr *ctx, int fmglev, int code, double *host)
 {
     SG_REQUIRE(ctx && host, "sg_amr_get_array: null argument");
     SG_REQUIRE(fmglev >= 0 && fmglev < kNfmg && code >= 0 && code < NCODE, "sg_amr_get_array: bad level/code");
-    SG_HIP(hipMemcpyAsync(host, ctx->A[fmglev][code], sizeof(double) * ctx->nblk * lenof(fmglev), hipMemcpyDeviceToHost,
+    SG_HIP(hipMemcpyAsync(host, ctx->d.A[fmglev][code], sizeof(double) * ctx->nblk * lenof(fmglev), hipMemcpyDeviceToHost,
                           ctx->stream));
     SG_HIP(hipStreamSynchronize(ctx->stream));
     return SG_OK;

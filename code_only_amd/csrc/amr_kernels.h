@@ -54,75 +54,77 @@ __device__ __forceinline__ void cubic_stencil(int f, int lo, int hi, int idx[4],
     idx[0] = c0; idx[1] = c1; idx[2] = c2; idx[3] = c3;
 }
 
-__device__ __forceinline__ double comb4(const double a[4], double v0, double v1, double v2, double v3)
+template <typename T> __device__ __forceinline__ T comb4(const double a[4], T v0, T v1, T v2, T v3)
 {
-    double s = a[0] * v0 + a[1] * v1;
-    s = s + a[2] * v2;
-    s = s + a[3] * v3;
+    T s = (T)a[0] * v0 + (T)a[1] * v1;
+    s = s + (T)a[2] * v2;
+    s = s + (T)a[3] * v3;
     return s;
 }
 
 struct Cubic64 { double w[64]; };   // St(a, b, c) of restriction.F90:38-63 at w[(c*4 + b)*4 + a]
 
 // restriction.F90:127-151
-template <int N> __device__ __forceinline__ double rstr_lin(const double *uf, int i, int j, int k)
+template <int N, typename T> __device__ __forceinline__ T rstr_lin(const T *uf, int i, int j, int k)
 {
     using B = Blk<N>;
-    double s = uf[B::gi(i, j, k)] + uf[B::gi(i + 1, j, k)];
+    T s = uf[B::gi(i, j, k)] + uf[B::gi(i + 1, j, k)];
     s = s + uf[B::gi(i, j + 1, k)];
     s = s + uf[B::gi(i, j, k + 1)];
     s = s + uf[B::gi(i + 1, j + 1, k)];
     s = s + uf[B::gi(i + 1, j, k + 1)];
     s = s + uf[B::gi(i, j + 1, k + 1)];
     s = s + uf[B::gi(i + 1, j + 1, k + 1)];
-    return s * (1.0 / 8.0);
+    return s * (T)(1.0 / 8.0);
 }
 
 // restriction.F90:71-119 (stencil i1-1 .. i1+2 per axis)
-template <int N> __device__ __forceinline__ double rstr_cub(const double *uf, int i1, int j1, int k1, const Cubic64 &st)
+template <int N, typename T> __device__ __forceinline__ T rstr_cub(const T *uf, int i1, int j1, int k1, const Cubic64 &st)
 {
     using B = Blk<N>;
-    double s = 0.0;
+    T s = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-                double t = uf[B::gi(i1 - 1 + a, j1 - 1 + b, k1 - 1 + c)] * st.w[(c * 4 + b) * 4 + a];
+                T t = uf[B::gi(i1 - 1 + a, j1 - 1 + b, k1 - 1 + c)] * (T)st.w[(c * 4 + b) * 4 + a];
                 s = (a == 0 && b == 0 && c == 0) ? t : s + t;
             }
     return s;
 }
 
 // interpolation.F90:222-247 on a coarse array addressed through cget(ic, jc, kc)
-template <typename G> __device__ __forceinline__ double trilinear_at(int i, int j, int k, G cget)
+template <typename G> __device__ __forceinline__ auto trilinear_at(int i, int j, int k, G cget) -> decltype(cget(0, 0, 0))
 {
+    using T = decltype(cget(0, 0, 0));
     int ic = ijkc(i), jc = ijkc(j), kc = ijkc(k);
     int ic1 = near_(i, ic), jc1 = near_(j, jc), kc1 = near_(k, kc);
-    double b = cget(ic1, jc, kc) + cget(ic, jc1, kc);
+    T b = cget(ic1, jc, kc) + cget(ic, jc1, kc);
     b = b + cget(ic, jc, kc1);
-    double e = cget(ic, jc1, kc1) + cget(ic1, jc, kc1);
+    T e = cget(ic, jc1, kc1) + cget(ic1, jc, kc1);
     e = e + cget(ic1, jc1, kc);
-    double s = 27.0 * cget(ic, jc, kc) + 9.0 * b;
-    s = s + 3.0 * e;
+    T s = (T)27.0 * cget(ic, jc, kc) + (T)9.0 * b;
+    s = s + (T)3.0 * e;
     s = s + cget(ic1, jc1, kc1);
-    return s / 64.0;
+    return s / (T)64.0;
 }
 
 // interpolation.F90:40-218 evaluated for one fine cell: x pass (bufx), y pass (bufxy), z pass,
 // each intermediate formed exactly as the three-pass loop forms it.  Coarse bounds -1..nc.
-template <typename G> __device__ __forceinline__ double tricubic_at(int i, int j, int k, int nc, G cget)
+template <typename G> __device__ __forceinline__ auto tricubic_at(int i, int j, int k, int nc, G cget) -> decltype(cget(0, 0, 0))
 {
+    using T = decltype(cget(0, 0, 0));
     int ix[4], jx[4], kx[4];
     double ax[4], ay[4], az[4];
     cubic_stencil(i, -1, nc, ix, ax);
     cubic_stencil(j, -1, nc, jx, ay);
     cubic_stencil(k, -1, nc, kx, az);
-    double bxy[4];
+    T bxy[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        double bx[4];
+        T bx[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b)
             bx[b] = comb4(ax, cget(ix[0], jx[b], kx[c]), cget(ix[1], jx[b], kx[c]), cget(ix[2], jx[b], kx[c]),
@@ -133,8 +135,8 @@ template <typename G> __device__ __forceinline__ double tricubic_at(int i, int j
 }
 
 // ---- fmg_converge_c2p_lev: each child of [b0, b0+nb) restricted into its parent's octant ----
-template <int N, bool CUBIC>
-__global__ void k_c2p(double *A, int b0, int nb, const int *parent, const int *octant, Cubic64 st)
+template <int N, bool CUBIC, typename T>
+__global__ void k_c2p(T *A, int b0, int nb, const int *parent, const int *octant, Cubic64 st)
 {
     using B = Blk<N>;
     constexpr int H = N / 2, HC = H * H * H;
@@ -142,19 +144,19 @@ __global__ void k_c2p(double *A, int b0, int nb, const int *parent, const int *o
     if (t >= (long)nb * HC) return;
     int b = b0 + (int)(t / HC), c = (int)(t % HC);
     int ic = c % H, jc = (c / H) % H, kc = c / (H * H);
-    const double *uf = A + (long)b * B::LEN;
-    double *uc = A + (long)parent[b] * B::LEN;
+    const T *uf = A + (long)b * B::LEN;
+    T *uc = A + (long)parent[b] * B::LEN;
     int o = octant[b];
     int oi = (o & 1) * H, oj = ((o >> 1) & 1) * H, ok = ((o >> 2) & 1) * H;
-    uc[B::gi(oi + ic, oj + jc, ok + kc)] = CUBIC ? rstr_cub<N>(uf, 2 * ic, 2 * jc, 2 * kc, st)
-                                                 : rstr_lin<N>(uf, 2 * ic, 2 * jc, 2 * kc);
+    uc[B::gi(oi + ic, oj + jc, ok + kc)] = CUBIC ? rstr_cub<N, T>(uf, 2 * ic, 2 * jc, 2 * kc, st)
+                                                 : rstr_lin<N, T>(uf, 2 * ic, 2 * jc, 2 * kc);
 }
 
 // ---- one direction of fmg_ghostcell_fix for the blocks [b0, b0+nb) of one AMR level:
 //      same-level faces copied (fmg_ghfix_samelev), parent-level faces interpolated
 //      (fmg_ghfix_parentlev: bilinear or separable cubic tangentially, quadratic normally) ----
-template <int N>
-__global__ void k_ghost(double *A, int b0, int nb, int d, int cubic, int kinds, const int *nbr, const int *nkind,
+template <int N, typename T>
+__global__ void k_ghost(T *A, int b0, int nb, int d, int cubic, int kinds, const int *nbr, const int *nkind,
                         const int *octant)
 {
     using B = Blk<N>;
@@ -166,8 +168,8 @@ __global__ void k_ghost(double *A, int b0, int nb, int d, int cubic, int kinds, 
     int f = 2 * d + lr;
     int kind = nkind[b * 6 + f];
     if (kind == NBR_NONE || (kind == NBR_SAME && !(kinds & 1)) || (kind == NBR_PARENT && !(kinds & 2))) return;
-    double *ud = A + (long)b * B::LEN;
-    const double *us = A + (long)nbr[b * 6 + f] * B::LEN;
+    T *ud = A + (long)b * B::LEN;
+    const T *us = A + (long)nbr[b * 6 + f] * B::LEN;
     const int ag = lr ? N : -1;
     if (kind == NBR_SAME) {
         ud[B::rot(d, ag, t1, t2)] = us[B::rot(d, lr ? 0 : N - 1, t1, t2)];
@@ -178,13 +180,13 @@ __global__ void k_ghost(double *A, int b0, int nb, int d, int cubic, int kinds, 
     const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
     const int an = lr ? 0 : N - 1;
     auto buf = [&](int c1, int c2) { return us[B::rot(d, an, c1 + e1, c2 + e2)]; };
-    double uH;
+    T uH;
     if (cubic && H + 2 >= 4) {
         int i1[4], i2[4];
         double a1[4], a2[4];
         cubic_stencil(t1, -1, H, i1, a1);
         cubic_stencil(t2, -1, H, i2, a2);
-        double bt[4];
+        T bt[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             bt[q] = comb4(a1, buf(i1[0], i2[q]), buf(i1[1], i2[q]), buf(i1[2], i2[q]), buf(i1[3], i2[q]));
@@ -192,21 +194,21 @@ __global__ void k_ghost(double *A, int b0, int nb, int d, int cubic, int kinds, 
     } else {
         int c1 = ijkc(t1), c2 = ijkc(t2);
         int c1n = near_(t1, c1), c2n = near_(t2, c2);
-        double S = 9.0 * buf(c1, c2) + buf(c1n, c2n);
-        S = S + 3.0 * buf(c1, c2n);
-        S = S + 3.0 * buf(c1n, c2);
-        uH = S / 16.0;
+        T S = (T)9.0 * buf(c1, c2) + buf(c1n, c2n);
+        S = S + (T)3.0 * buf(c1, c2n);
+        S = S + (T)3.0 * buf(c1n, c2);
+        uH = S / (T)16.0;
     }
     const int inner = lr ? -1 : 1;
-    double u0 = ud[B::rot(d, ag + inner, t1, t2)];
-    double uN = ud[B::rot(d, ag + 2 * inner, t1, t2)];
-    double q = 10.0 * u0 + 8.0 * uH;
-    q = q - 3.0 * uN;
-    ud[B::rot(d, ag, t1, t2)] = q / 15.0;
+    T u0 = ud[B::rot(d, ag + inner, t1, t2)];
+    T uN = ud[B::rot(d, ag + 2 * inner, t1, t2)];
+    T q = (T)10.0 * u0 + (T)8.0 * uH;
+    q = q - (T)3.0 * uN;
+    ud[B::rot(d, ag, t1, t2)] = q / (T)15.0;
 }
 
 // ---- fmg_boundary_fill0 on a list of physical faces (entries b*6+f) ----
-template <int N> __global__ void k_fill0_faces(double *A, const int *faces, int nf)
+template <int N, typename T> __global__ void k_fill0_faces(T *A, const int *faces, int nf)
 {
     using B = Blk<N>;
     constexpr int W = B::W;
@@ -215,11 +217,11 @@ template <int N> __global__ void k_fill0_faces(double *A, const int *faces, int 
     int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1;
     int e = faces[t / (W * W)];
     int b = e / 6, f = e % 6, d = f / 2;
-    A[(long)b * B::LEN + B::rot(d, (f & 1) ? N : -1, t1, t2)] = 0.0;
+    A[(long)b * B::LEN + B::rot(d, (f & 1) ? N : -1, t1, t2)] = (T)0;
 }
 
 // ---- fmg_boundary_extrap, the faces of direction d only ----
-template <int N> __global__ void k_extrap_faces(double *A, const int *faces, int nf, int d)
+template <int N, typename T> __global__ void k_extrap_faces(T *A, const int *faces, int nf, int d)
 {
     using B = Blk<N>;
     constexpr int W = B::W;
@@ -229,13 +231,13 @@ template <int N> __global__ void k_extrap_faces(double *A, const int *faces, int
     int e = faces[t / (W * W)];
     int b = e / 6, f = e % 6;
     if (f / 2 != d) return;
-    double *u = A + (long)b * B::LEN;
+    T *u = A + (long)b * B::LEN;
     u[B::rot(d, (f & 1) ? N : -1, t1, t2)] = -u[B::rot(d, (f & 1) ? N - 1 : 0, t1, t2)];
 }
 
 // ---- fmg_reflux_do: fine face fluxes averaged 2x2 into the coarse neighbour's override ----
-template <int N>
-__global__ void k_reflux(const double *A, double *ov, const int *cf, const int *cf_slot, int ncf,
+template <int N, typename T>
+__global__ void k_reflux(const T *A, T *ov, const int *cf, const int *cf_slot, int ncf,
                          const double *hi_lev, const int *level, const int *octant)
 {
     using B = Blk<N>;
@@ -244,27 +246,33 @@ __global__ void k_reflux(const double *A, double *ov, const int *cf, const int *
     if (t >= (long)ncf * H * H) return;
     int c1 = (int)(t % H), c2 = (int)((t / H) % H), e = (int)(t / (H * H));
     int b = cf[e] / 6, f = cf[e] % 6, d = f / 2, lr = f & 1;
-    const double hi = hi_lev[level[b]];
-    const double *u = A + (long)b * B::LEN;
+    const T hi = (T)hi_lev[level[b]];
+    const T *u = A + (long)b * B::LEN;
     const int af = lr ? N - 1 : -1;
     auto ff = [&](int a1, int a2) { return (u[B::rot(d, af + 1, a1, a2)] - u[B::rot(d, af, a1, a2)]) * hi; };
     const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
     const int o = octant[b];
     const int off1 = ((o >> ta) & 1) * H, off2 = ((o >> tb) & 1) * H;
     int f1 = 2 * c1, f2 = 2 * c2;
-    double v = ff(f1, f2) + ff(f1, f2 + 1);
+    T v = ff(f1, f2) + ff(f1, f2 + 1);
     v = v + ff(f1 + 1, f2);
     v = v + ff(f1 + 1, f2 + 1);
-    ov[(long)cf_slot[e] * N * N + (off1 + c1) + N * (off2 + c2)] = v * 0.25;
+    ov[(long)cf_slot[e] * N * N + (off1 + c1) + N * (off2 + c2)] = v * (T)0.25;
 }
 
 // Face-flux view of one block: fluxes from the block's values, or the override on reflux faces.
-template <int N> struct Flux {
-    const double *u;
+// With nb (the neighbour table of a block) set, same-level faces read the neighbour block's
+// boundary cell directly instead of the block's own ghost cell ("halo-free" form): the value a
+// same-level ghost copy (fmg_ghfix_samelev) would have placed there.
+template <int N, typename T> struct Flux {
+    const T *u;
     const int *slot;     // ovslot + 6 b, or nullptr
-    const double *ov;
-    double hi;
-    __device__ double ov_or(int f, int t1, int t2, double computed) const
+    const T *ov;
+    T hi;
+    const T *A = nullptr;       // halo-free form: all blocks of the level
+    const int *nb = nullptr;    //                 nbr + 6 b
+    const int *kind = nullptr;  //                 nkind + 6 b
+    __device__ T ov_or(int f, int t1, int t2, T computed) const
     {
         if (slot) {
             int s = slot[f];
@@ -272,18 +280,32 @@ template <int N> struct Flux {
         }
         return computed;
     }
-    // sum of flux differences at (i, j, k) in the reference's operand order
-    __device__ double div(int i, int j, int k) const
+    // value across face f (d = f / 2) at tangential (t1, t2): own ghost, or the neighbour's cell
+    __device__ T across(int f, int t1, int t2) const
     {
         using B = Blk<N>;
-        const double uc = u[B::gi(i, j, k)];
-        double fxp = (i == N - 1) ? ov_or(1, j, k, (u[B::gi(i + 1, j, k)] - uc) * hi) : (u[B::gi(i + 1, j, k)] - uc) * hi;
-        double fxm = (i == 0) ? ov_or(0, j, k, (uc - u[B::gi(i - 1, j, k)]) * hi) : (uc - u[B::gi(i - 1, j, k)]) * hi;
-        double fyp = (j == N - 1) ? ov_or(3, i, k, (u[B::gi(i, j + 1, k)] - uc) * hi) : (u[B::gi(i, j + 1, k)] - uc) * hi;
-        double fym = (j == 0) ? ov_or(2, i, k, (uc - u[B::gi(i, j - 1, k)]) * hi) : (uc - u[B::gi(i, j - 1, k)]) * hi;
-        double fzp = (k == N - 1) ? ov_or(5, i, j, (u[B::gi(i, j, k + 1)] - uc) * hi) : (u[B::gi(i, j, k + 1)] - uc) * hi;
-        double fzm = (k == 0) ? ov_or(4, i, j, (uc - u[B::gi(i, j, k - 1)]) * hi) : (uc - u[B::gi(i, j, k - 1)]) * hi;
-        double v = fxp - fxm;
+        const int d = f >> 1, lr = f & 1;
+        if (nb && kind[f] == NBR_SAME) return A[(long)nb[f] * B::LEN + B::rot(d, lr ? 0 : N - 1, t1, t2)];
+        return u[B::rot(d, lr ? N : -1, t1, t2)];
+    }
+    // sum of flux differences at (i, j, k) in the reference's operand order
+    __device__ T div(int i, int j, int k) const
+    {
+        using B = Blk<N>;
+        const T uc = u[B::gi(i, j, k)];
+        const T xp = (i == N - 1) ? across(1, j, k) : u[B::gi(i + 1, j, k)];
+        const T xm = (i == 0) ? across(0, j, k) : u[B::gi(i - 1, j, k)];
+        const T yp = (j == N - 1) ? across(3, i, k) : u[B::gi(i, j + 1, k)];
+        const T ym = (j == 0) ? across(2, i, k) : u[B::gi(i, j - 1, k)];
+        const T zp = (k == N - 1) ? across(5, i, j) : u[B::gi(i, j, k + 1)];
+        const T zm = (k == 0) ? across(4, i, j) : u[B::gi(i, j, k - 1)];
+        T fxp = (i == N - 1) ? ov_or(1, j, k, (xp - uc) * hi) : (xp - uc) * hi;
+        T fxm = (i == 0) ? ov_or(0, j, k, (uc - xm) * hi) : (uc - xm) * hi;
+        T fyp = (j == N - 1) ? ov_or(3, i, k, (yp - uc) * hi) : (yp - uc) * hi;
+        T fym = (j == 0) ? ov_or(2, i, k, (uc - ym) * hi) : (uc - ym) * hi;
+        T fzp = (k == N - 1) ? ov_or(5, i, j, (zp - uc) * hi) : (zp - uc) * hi;
+        T fzm = (k == 0) ? ov_or(4, i, j, (uc - zm) * hi) : (uc - zm) * hi;
+        T v = fxp - fxm;
         v = v + fyp;
         v = v - fym;
         v = v + fzp;
@@ -294,11 +316,27 @@ template <int N> struct Flux {
 
 __device__ __forceinline__ int pick_block(const int *list, int b0, int i) { return list ? list[i] : b0 + i; }
 
+// Max reductions of the smoother go to kMaxSlots addresses (workgroup index modulo kMaxSlots), so
+// that thousands of workgroups do not serialise on one atomic; k_max_slots folds them.
+constexpr int kMaxSlots = 64;
+
+__global__ void k_zero_slots(double *slots)
+{
+    if (threadIdx.x < kMaxSlots) slots[threadIdx.x] = 0.0;
+}
+
+__global__ void k_max_slots(const double *slots, double *out)
+{
+    double v = threadIdx.x < kMaxSlots ? slots[threadIdx.x] : 0.0;
+    v = wave_max(v);
+    if (threadIdx.x == 0) *out = v;
+}
+
 // ---- fmg_poisson_relax / vmg_poisson_relax: one colour pass over the listed blocks ----
-template <int N>
-__global__ void k_relax(double *Au, const double *Arhs, const int *list, int b0, int nb, int ipass,
+template <int N, typename T>
+__global__ void k_relax(T *Au, const T *Arhs, const int *list, int b0, int nb, int ipass,
                         const double *h_lev, const double *hi_lev, const int *level, const int *ovslot,
-                        const double *ov, double *resmax)
+                        const T *ov, double *resmax, const int *nbr = nullptr, const int *nkind = nullptr)
 {
     using B = Blk<N>;
     constexpr int HN = (N + 1) / 2, PER = N * N * HN;
@@ -310,22 +348,23 @@ __global__ void k_relax(double *Au, const double *Arhs, const int *list, int b0,
         int i = 2 * ih + ((j + k + ipass) & 1);
         if (i < N) {
             const int l = level[b];
-            const double h = h_lev[l];
-            double *u = Au + (long)b * B::LEN;
-            Flux<N> F{u, ovslot ? ovslot + 6 * b : nullptr, ov, hi_lev[l]};
-            double r = F.div(i, j, k) - Arhs[(long)b * B::LEN + B::gi(i, j, k)] * h;
-            double resh2 = r * h;
-            u[B::gi(i, j, k)] = u[B::gi(i, j, k)] + (1.0 / 6.0) * resh2;
-            a = fabs(resh2);
+            const T h = (T)h_lev[l];
+            T *u = Au + (long)b * B::LEN;
+            Flux<N, T> F{u, ovslot ? ovslot + 6 * b : nullptr, ov, (T)hi_lev[l]};
+            if (nbr) { F.A = Au; F.nb = nbr + 6 * b; F.kind = nkind + 6 * b; }
+            T r = F.div(i, j, k) - Arhs[(long)b * B::LEN + B::gi(i, j, k)] * h;
+            T resh2 = r * h;
+            u[B::gi(i, j, k)] = u[B::gi(i, j, k)] + (T)(1.0 / 6.0) * resh2;
+            a = (double)fabs(resh2);
         }
     }
-    block_atomic_max(resmax, a);
+    if (resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
 }
 
 // ---- fmg_poisson_resid: res = rhs - div F / h on the listed blocks ----
-template <int N>
-__global__ void k_resid(double *Ares, const double *Au, const double *Arhs, const int *list, int nb,
-                        const double *hi_lev, const int *level, const int *ovslot, const double *ov)
+template <int N, typename T>
+__global__ void k_resid(T *Ares, const T *Au, const T *Arhs, const int *list, int nb,
+                        const double *hi_lev, const int *level, const int *ovslot, const T *ov)
 {
     using B = Blk<N>;
     constexpr int PER = N * N * N;
@@ -334,46 +373,46 @@ __global__ void k_resid(double *Ares, const double *Au, const double *Arhs, cons
     int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
     int b = list[t / PER];
     const int l = level[b];
-    Flux<N> F{Au + (long)b * B::LEN, ovslot + 6 * b, ov, hi_lev[l]};
+    Flux<N, T> F{Au + (long)b * B::LEN, ovslot + 6 * b, ov, (T)hi_lev[l]};
     const long q = (long)b * B::LEN + B::gi(i, j, k);
-    Ares[q] = Arhs[q] - F.div(i, j, k) * hi_lev[l];
+    Ares[q] = Arhs[q] - F.div(i, j, k) * (T)hi_lev[l];
 }
 
 // ---- fmg_rstrct (linear): coarse level with NC cells from the fine level with 2 NC ----
-template <int NC> __global__ void k_rstrct(double *Ac, const double *Af, const int *list, int nb)
+template <int NC, typename T> __global__ void k_rstrct(T *Ac, const T *Af, const int *list, int nb)
 {
     constexpr int PER = NC * NC * NC;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * PER) return;
     int i = (int)(t % NC), j = (int)((t / NC) % NC), k = (int)((t / (NC * NC)) % NC);
     int b = list[t / PER];
-    Ac[(long)b * Blk<NC>::LEN + Blk<NC>::gi(i, j, k)] = rstr_lin<2 * NC>(Af + (long)b * Blk<2 * NC>::LEN, 2 * i, 2 * j, 2 * k);
+    Ac[(long)b * Blk<NC>::LEN + Blk<NC>::gi(i, j, k)] = rstr_lin<2 * NC, T>(Af + (long)b * Blk<2 * NC>::LEN, 2 * i, 2 * j, 2 * k);
 }
 
 // ---- fmg_interp_cubic / fmg_interp_linear: fine interior of each listed block from its own
 //      coarser FMG level (ghosts included) ----
-template <int NF, bool CUBIC> __global__ void k_interp(double *Af, const double *Ac, const int *list, int nb)
+template <int NF, bool CUBIC, typename T> __global__ void k_interp(T *Af, const T *Ac, const int *list, int nb)
 {
     constexpr int NC = NF / 2, PER = NF * NF * NF;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * PER) return;
     int i = (int)(t % NF), j = (int)((t / NF) % NF), k = (int)((t / (NF * NF)) % NF);
     int b = list[t / PER];
-    const double *uc = Ac + (long)b * Blk<NC>::LEN;
-    auto cget = [&](int a, int bb, int c) { return uc[Blk<NC>::gi(a, bb, c)]; };
-    double v = (CUBIC && NC + 2 >= 4) ? tricubic_at(i, j, k, NC, cget) : trilinear_at(i, j, k, cget);
+    const T *uc = Ac + (long)b * Blk<NC>::LEN;
+    auto cget = [&](int a, int bb, int c) -> T { return uc[Blk<NC>::gi(a, bb, c)]; };
+    T v = (CUBIC && NC + 2 >= 4) ? tricubic_at(i, j, k, NC, cget) : trilinear_at(i, j, k, cget);
     Af[(long)b * Blk<NF>::LEN + Blk<NF>::gi(i, j, k)] = v;
 }
 
 // ---- whole-array block operations (ghosts included) ----
-__global__ void k_blocks_fill0(double *A, const int *list, int b0, int nb, int len)
+template <typename T> __global__ void k_blocks_fill0(T *A, const int *list, int b0, int nb, int len)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * len) return;
-    A[(long)pick_block(list, b0, (int)(t / len)) * len + t % len] = 0.0;
+    A[(long)pick_block(list, b0, (int)(t / len)) * len + t % len] = (T)0;
 }
 
-__global__ void k_blocks_copy(double *O, const double *I, const int *list, int b0, int nb, int len)
+template <typename T> __global__ void k_blocks_copy(T *O, const T *I, const int *list, int b0, int nb, int len)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * len) return;
@@ -381,7 +420,7 @@ __global__ void k_blocks_copy(double *O, const double *I, const int *list, int b
     O[q] = I[q];
 }
 
-__global__ void k_blocks_add(double *A, const double *D, const int *list, int b0, int nb, int len)
+template <typename T> __global__ void k_blocks_add(T *A, const T *D, const int *list, int b0, int nb, int len)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * len) return;
@@ -389,7 +428,7 @@ __global__ void k_blocks_add(double *A, const double *D, const int *list, int b0
     A[q] = A[q] + D[q];
 }
 
-__global__ void k_blocks_sub(double *O, const double *A, const double *C, int b0, int nb, int len)
+template <typename T> __global__ void k_blocks_sub(T *O, const T *A, const T *C, int b0, int nb, int len)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * len) return;
@@ -398,23 +437,24 @@ __global__ void k_blocks_sub(double *O, const double *A, const double *C, int b0
 }
 
 // ---- vmg_poisson_tau, fine part: tau = 0, then tau = -(div F)/h inside, all blocks of a level ----
-template <int N> __global__ void k_tau_fine(double *Atau, const double *Au, int b0, int nb, double hi)
+template <int N, typename T> __global__ void k_tau_fine(T *Atau, const T *Au, int b0, int nb, double hi_d)
 {
     using B = Blk<N>;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * B::LEN) return;
     int b = b0 + (int)(t / B::LEN), q = (int)(t % B::LEN);
     int i = q % B::W - 1, j = (q / B::W) % B::W - 1, k = q / (B::W * B::W) - 1;
-    double v = 0.0;
+    const T hi = (T)hi_d;
+    T v = 0;
     if (i >= 0 && i < N && j >= 0 && j < N && k >= 0 && k < N) {
-        Flux<N> F{Au + (long)b * B::LEN, nullptr, nullptr, hi};
+        Flux<N, T> F{Au + (long)b * B::LEN, nullptr, nullptr, hi};
         v = -(F.div(i, j, k) * hi);
     }
     Atau[(long)b * B::LEN + q] = v;
 }
 
 // ---- vmg_poisson_tau, coarse part: tau += (div F)/h on the listed (covered) blocks ----
-template <int N> __global__ void k_tau_add(double *Atau, const double *Au, const int *list, int nb, double hi)
+template <int N, typename T> __global__ void k_tau_add(T *Atau, const T *Au, const int *list, int nb, double hi_d)
 {
     using B = Blk<N>;
     constexpr int PER = N * N * N;
@@ -422,13 +462,14 @@ template <int N> __global__ void k_tau_add(double *Atau, const double *Au, const
     if (t >= (long)nb * PER) return;
     int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
     int b = list[t / PER];
-    Flux<N> F{Au + (long)b * B::LEN, nullptr, nullptr, hi};
+    const T hi = (T)hi_d;
+    Flux<N, T> F{Au + (long)b * B::LEN, nullptr, nullptr, hi};
     long q = (long)b * B::LEN + B::gi(i, j, k);
     Atau[q] = Atau[q] + F.div(i, j, k) * hi;
 }
 
 // ---- vmg_tau_crop: a child face meeting the parent level zeroes the parent's boundary plane ----
-template <int N> __global__ void k_tau_crop(double *Atau, const int *cf, int ncf, const int *parent)
+template <int N, typename T> __global__ void k_tau_crop(T *Atau, const int *cf, int ncf, const int *parent)
 {
     using B = Blk<N>;
     constexpr int W = B::W;
@@ -436,11 +477,11 @@ template <int N> __global__ void k_tau_crop(double *Atau, const int *cf, int ncf
     if (t >= (long)ncf * W * W) return;
     int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1, e = cf[t / (W * W)];
     int b = e / 6, f = e % 6;
-    Atau[(long)parent[b] * B::LEN + B::rot(f / 2, (f & 1) ? N - 1 : 0, t1, t2)] = 0.0;
+    Atau[(long)parent[b] * B::LEN + B::rot(f / 2, (f & 1) ? N - 1 : 0, t1, t2)] = (T)0;
 }
 
 // ---- vmg_interp_p2c_lev (trilinear): child interiors of [b0, b0+nb) from the parent's octant ----
-template <int N> __global__ void k_p2c(double *Af, const double *Ac, int b0, int nb, const int *parent, const int *octant)
+template <int N, typename T> __global__ void k_p2c(T *Af, const T *Ac, int b0, int nb, const int *parent, const int *octant)
 {
     using B = Blk<N>;
     constexpr int H = N / 2, PER = N * N * N;
@@ -448,10 +489,10 @@ template <int N> __global__ void k_p2c(double *Af, const double *Ac, int b0, int
     if (t >= (long)nb * PER) return;
     int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
     int b = b0 + (int)(t / PER);
-    const double *uc = Ac + (long)parent[b] * B::LEN;
+    const T *uc = Ac + (long)parent[b] * B::LEN;
     int o = octant[b];
     int oi = (o & 1) * H, oj = ((o >> 1) & 1) * H, ok = ((o >> 2) & 1) * H;
-    auto cget = [&](int a, int bb, int c) { return uc[B::gi(oi + a, oj + bb, ok + c)]; };
+    auto cget = [&](int a, int bb, int c) -> T { return uc[B::gi(oi + a, oj + bb, ok + c)]; };
     Af[(long)b * B::LEN + B::gi(i, j, k)] = trilinear_at(i, j, k, cget);
 }
 
@@ -476,7 +517,7 @@ __global__ void k_psi2g(double *g, const double *Au, int nblk, const double *hi_
         gz[q] = -(u[B::gi(i, j, k + 1)] - u[B::gi(i, j, k - 1)]) * hi * 0.5;
         return;
     }
-    Flux<N> F{u, ovslot + 6 * b, ov, hi};
+    Flux<N, double> F{u, ovslot + 6 * b, ov, hi};
     const double uc = u[B::gi(i, j, k)];
     double fxp = (u[B::gi(i + 1, j, k)] - uc) * hi, fxm = (uc - u[B::gi(i - 1, j, k)]) * hi;
     double fyp = (u[B::gi(i, j + 1, k)] - uc) * hi, fym = (uc - u[B::gi(i, j - 1, k)]) * hi;
@@ -492,31 +533,85 @@ __global__ void k_psi2g(double *g, const double *Au, int nblk, const double *hi_
     gz[q] = -(fzp + fzm) * 0.5;
 }
 
-// ---- norms over the listed blocks' interiors: max |a| h^2 (fmg_get_h2absmax) -> out[0],
-//      max |a| -> out[1], sum a^2 dv -> out[2] ----
+// ---- norms over the listed blocks' interiors, two stages with a fixed order: each of G
+//      workgroups reduces a grid-stride share into part[g], part[G + g], part[2G + g]
+//      (max |a| h^2 as fmg_get_h2absmax, max |a|, sum a^2 dv), then one workgroup reduces those ----
 template <int N>
-__global__ void k_norms(const double *A, const int *list, int nb, const double *h_lev, const int *level, double *out)
+__global__ void k_norms_part(const double *A, const int *list, int nb, const double *h_lev, const int *level,
+                             double *part)
 {
     using B = Blk<N>;
     constexpr int PER = N * N * N;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double m2 = 0.0, m = 0.0, s = 0.0;
-    if (t < (long)nb * PER) {
+    const long total = (long)nb * PER, G = gridDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += G * blockDim.x) {
         int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
         int b = list[t / PER];
         double h = h_lev[level[b]];
         double v = A[(long)b * B::LEN + B::gi(i, j, k)];
-        m = fabs(v);
-        m2 = m * (h * h);
-        s = v * v * ((h * h) * h);
+        double a = fabs(v);
+        m = fmax(m, a);
+        m2 = fmax(m2, a * (h * h));
+        s += v * v * ((h * h) * h);
     }
-    double mm = m2;
-    block_max_sum(mm, s);
+    block_max_sum(m2, s);
+    double dummy = 0.0;
+    block_max_sum(m, dummy);
     if (threadIdx.x == 0) {
-        atomic_max_nonneg(out + 0, mm);
-        atomicAdd(out + 2, s);
+        part[blockIdx.x] = m2;
+        part[G + blockIdx.x] = m;
+        part[2 * G + blockIdx.x] = s;
     }
-    block_atomic_max(out + 1, m);
+}
+
+__global__ void k_norms_final(const double *part, int G, double *out)
+{
+    double m2 = 0.0, m = 0.0, s = 0.0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) {
+        m2 = fmax(m2, part[g]);
+        m = fmax(m, part[G + g]);
+        s += part[2 * G + g];
+    }
+    block_max_sum(m2, s);
+    double dummy = 0.0;
+    block_max_sum(m, dummy);
+    if (threadIdx.x == 0) { out[0] = m2; out[1] = m; out[2] = s; }
+}
+
+// ---- 8^3 blocks, one workgroup (256 threads) per block: the block's ghosted array is staged in
+//      LDS with coalesced loads, then one colour of fmg_poisson_relax / the residual is formed from
+//      LDS (same operand order as k_relax / k_resid) ----
+template <bool RELAX, typename T>
+__global__ void __launch_bounds__(256) k_blk8(T *Aout, T *Au, const T *Arhs, const int *list, int ipass,
+                                             const double *h_lev, const double *hi_lev, const int *level,
+                                             const int *ovslot, const T *ov, double *resmax)
+{
+    using B = Blk<8>;
+    __shared__ T su[B::LEN];
+    const int b = list[blockIdx.x];
+    const T *ug = Au + (long)b * B::LEN;
+    for (int q = threadIdx.x; q < B::LEN; q += 256) su[q] = ug[q];
+    __syncthreads();
+    const int l = level[b];
+    const T h = (T)h_lev[l], hi = (T)hi_lev[l];
+    Flux<8, T> F{su, ovslot + 6 * b, ov, hi};
+    double a = 0.0;
+    if (RELAX) {
+        const int ih = threadIdx.x & 3, j = (threadIdx.x >> 2) & 7, k = threadIdx.x >> 5;
+        const int i = 2 * ih + ((j + k + ipass) & 1);
+        const long q = (long)b * B::LEN + B::gi(i, j, k);
+        T r = F.div(i, j, k) - Arhs[q] * h;
+        T resh2 = r * h;
+        Au[q] = su[B::gi(i, j, k)] + (T)(1.0 / 6.0) * resh2;
+        a = (double)fabs(resh2);
+    } else {
+        for (int c = threadIdx.x; c < 512; c += 256) {
+            const int i = c & 7, j = (c >> 3) & 7, k = c >> 6;
+            const long q = (long)b * B::LEN + B::gi(i, j, k);
+            Aout[q] = Arhs[q] - F.div(i, j, k) * hi;
+        }
+    }
+    if (RELAX && resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
 }
 
 // ---- single-workgroup helpers (blockDim.x threads, one workgroup) ----
@@ -533,11 +628,11 @@ __device__ inline void wg_tree_sum(double *a, int m, int P)
 }
 
 // vmg_zeroAverage (vmg.F90:890-915): average of the base blocks' interiors removed from them.
-template <int N> __global__ void k_zero_average(double *A, int nbase, int P, double *scratch)
+template <int N, typename T> __global__ void k_zero_average(T *A, int nbase, int P, double *scratch)
 {
     using B = Blk<N>;
     for (int b = threadIdx.x; b < nbase; b += blockDim.x) {
-        const double *a = A + (long)b * B::LEN;
+        const T *a = A + (long)b * B::LEN;
         double t = 0.0;
         for (int k = 0; k < N; ++k)
             for (int j = 0; j < N; ++j)
@@ -550,7 +645,7 @@ template <int N> __global__ void k_zero_average(double *A, int nbase, int P, dou
     for (int q = threadIdx.x; q < nbase * N * N * N; q += blockDim.x) {
         int b = q / (N * N * N), c = q % (N * N * N);
         long e = (long)b * B::LEN + B::gi(c % N, (c / N) % N, c / (N * N));
-        A[e] = A[e] - ave;
+        A[e] = (T)((double)A[e] - ave);
     }
 }
 
@@ -711,7 +806,7 @@ template <int N> __global__ void k_sphere_ghost(double *A, const int *faces, int
 }
 
 // ---- base grid: gather of the base blocks' interiors (mg_dataPrepare) and scatter back ----
-template <int N> __global__ void k_base_gather(double *psi, double *src, const double *Au, const double *Arhs,
+template <int N, typename T> __global__ void k_base_gather(T *psi, T *src, const T *Au, const T *Arhs,
                                               int nbase, const int *ijk, int NG)
 {
     using B = Blk<N>;
@@ -725,7 +820,7 @@ template <int N> __global__ void k_base_gather(double *psi, double *src, const d
     src[q] = Arhs[(long)b * B::LEN + B::gi(i, j, k)];
 }
 
-template <int N> __global__ void k_base_scatter(double *Au, const double *psi, int nbase, const int *ijk, int NG)
+template <int N, typename T> __global__ void k_base_scatter(T *Au, const T *psi, int nbase, const int *ijk, int NG)
 {
     using B = Blk<N>;
     constexpr int PER = N * N * N;
@@ -735,6 +830,23 @@ template <int N> __global__ void k_base_scatter(double *Au, const double *psi, i
     long W = NG + 2;
     long q = ((long)(ijk[3 * b + 2] * N + k + 1) * W + (ijk[3 * b + 1] * N + j + 1)) * W + (ijk[3 * b] * N + i + 1);
     Au[(long)b * B::LEN + B::gi(i, j, k)] = psi[q];
+}
+
+// ---- mixed precision: FP64 residual -> FP32 inner source, FP64 potential += FP32 correction ----
+__global__ void k_cast_blocks(float *O, const double *I, const int *list, int nb, int len)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * len) return;
+    long q = (long)pick_block(list, 0, (int)(t / len)) * len + t % len;
+    O[q] = (float)I[q];
+}
+
+__global__ void k_add_cast_blocks(double *A, const float *D, const int *list, int nb, int len)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * len) return;
+    long q = (long)pick_block(list, 0, (int)(t / len)) * len + t % len;
+    A[q] = A[q] + (double)D[q];
 }
 
 } // namespace amr

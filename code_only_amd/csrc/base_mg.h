@@ -22,16 +22,19 @@ struct BaseMG {
     double h[kBaseMaxLev];
     double *U[kBaseMaxLev], *RHS[kBaseMaxLev], *RES[kBaseMaxLev], *RHO[kBaseMaxLev];
     double *psi, *src;              // level-0 potential (in/out) and source
+    int lds_off[kBaseMaxLev][4];    // LDS offset of level l's U, RHS, RES, RHO, or -1 when in global memory
     int dirichlet;
     double tol;                     // Resmaxg / rhomax stopping ratio
     int maxit;
     int *iters;                     // out: iterations taken
 };
 
-__device__ __forceinline__ long bgi(int n, int i, int j, int k)
+__device__ __forceinline__ int bgi(int n, int i, int j, int k)
 {
-    return ((long)(k + 1) * (n + 2) + (j + 1)) * (n + 2) + (i + 1);
+    return ((k + 1) * (n + 2) + (j + 1)) * (n + 2) + (i + 1);
 }
+
+__device__ __forceinline__ int ilog2_pow2(int n) { return 31 - __clz(n); }
 
 __device__ __forceinline__ void wg_sync() { __syncthreads(); }
 
@@ -117,40 +120,68 @@ __device__ inline double wg_max(double v)
     return r;
 }
 
-// mg_poisson_relax (mg_poisson.F90:32-79); returns Resmaxg = max|resh2| / h^2
-__device__ inline double bmg_relax(double *u, const double *rhs, int n, double h, int dirichlet)
+// mg_poisson_relax (mg_poisson.F90:32-79); returns Resmaxg = max|resh2| / h^2 when want_max
+// (the workgroup reduction costs three barriers, and only the last sweep's value is used).
+// Each thread loads the stencils of up to four cells of the colour before it updates any of them:
+// cells of one colour are never face neighbours, so the loads see exactly the values a sequential
+// sweep reads, and four cells' latencies overlap.
+__device__ inline double bmg_relax(double *u, const double *__restrict__ rhs, int n, double h, int dirichlet,
+                                  bool want_max)
 {
     const double sixth = 1.0 / 6.0, h2 = h * h;
-    const int hn = (n + 1) / 2;
+    const int hn = (n + 1) / 2, ncell = n * n * hn, lg = ilog2_pow2(n), lgh = ilog2_pow2(hn);
     double m = 0.0;
     for (int ipass = 1; ipass <= 2; ++ipass) {
         bmg_bu(u, n, dirichlet);
-        for (int t = threadIdx.x; t < n * n * hn; t += blockDim.x) {
-            int ih = t % hn, j = (t / hn) % n, k = t / (hn * n);
-            int i = 2 * ih + ((j + k + ipass) & 1);
-            if (i >= n) continue;
-            double r = u[bgi(n, i + 1, j, k)] + u[bgi(n, i - 1, j, k)];
-            r = r + u[bgi(n, i, j + 1, k)];
-            r = r + u[bgi(n, i, j - 1, k)];
-            r = r + u[bgi(n, i, j, k + 1)];
-            r = r + u[bgi(n, i, j, k - 1)];
-            r = r - 6.0 * u[bgi(n, i, j, k)];
-            r = r - h2 * rhs[bgi(n, i, j, k)];
-            u[bgi(n, i, j, k)] = u[bgi(n, i, j, k)] + sixth * r;
-            m = fmax(m, fabs(r));
+        for (int t0 = threadIdx.x; t0 < ncell; t0 += 4 * blockDim.x) {
+            double v[4][8];
+            int q[4];
+            bool ok[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                int t = t0 + c * blockDim.x;
+                int ih = t & (hn - 1), j = (t >> lgh) & (n - 1), k = t >> (lgh + lg);
+                int i = 2 * ih + ((j + k + ipass) & 1);
+                ok[c] = t < ncell && i < n;
+                if (!ok[c]) { i = 0; j = 0; k = 0; }
+                q[c] = bgi(n, i, j, k);
+                v[c][0] = u[bgi(n, i + 1, j, k)];
+                v[c][1] = u[bgi(n, i - 1, j, k)];
+                v[c][2] = u[bgi(n, i, j + 1, k)];
+                v[c][3] = u[bgi(n, i, j - 1, k)];
+                v[c][4] = u[bgi(n, i, j, k + 1)];
+                v[c][5] = u[bgi(n, i, j, k - 1)];
+                v[c][6] = u[q[c]];
+                v[c][7] = rhs[q[c]];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (!ok[c]) continue;
+                double r = v[c][0] + v[c][1];
+                r = r + v[c][2];
+                r = r + v[c][3];
+                r = r + v[c][4];
+                r = r + v[c][5];
+                r = r - 6.0 * v[c][6];
+                r = r - h2 * v[c][7];
+                u[q[c]] = v[c][6] + sixth * r;
+                m = fmax(m, fabs(r));
+            }
         }
         wg_sync();
     }
-    return wg_max(m) / h2;
+    return want_max ? wg_max(m) / h2 : 0.0;
 }
 
 // mg_poisson_resid (mg_poisson.F90:5-28)
-__device__ inline void bmg_resid(double *res, double *u, const double *rhs, int n, double h, int dirichlet)
+__device__ inline void bmg_resid(double *__restrict__ res, double *u, const double *__restrict__ rhs, int n, double h,
+                                 int dirichlet)
 {
     const double h2i = 1.0 / (h * h);
+    const int lg = ilog2_pow2(n);
     bmg_bu(u, n, dirichlet);
     for (int t = threadIdx.x; t < n * n * n; t += blockDim.x) {
-        int i = t % n, j = (t / n) % n, k = t / (n * n);
+        int i = t & (n - 1), j = (t >> lg) & (n - 1), k = t >> (2 * lg);
         double s = u[bgi(n, i + 1, j, k)] + u[bgi(n, i - 1, j, k)];
         s = s + u[bgi(n, i, j + 1, k)];
         s = s + u[bgi(n, i, j - 1, k)];
@@ -162,11 +193,11 @@ __device__ inline void bmg_resid(double *res, double *u, const double *rhs, int 
     wg_sync();
 }
 
-__device__ inline void bmg_rstrct(double *uc, const double *uf, int nc)
+__device__ inline void bmg_rstrct(double *__restrict__ uc, const double *__restrict__ uf, int nc)
 {
-    const int nf = 2 * nc;
+    const int nf = 2 * nc, lg = ilog2_pow2(nc);
     for (int t = threadIdx.x; t < nc * nc * nc; t += blockDim.x) {
-        int ic = t % nc, jc = (t / nc) % nc, kc = t / (nc * nc);
+        int ic = t & (nc - 1), jc = (t >> lg) & (nc - 1), kc = t >> (2 * lg);
         int i = 2 * ic, j = 2 * jc, k = 2 * kc;
         double s = uf[bgi(nf, i, j, k)] + uf[bgi(nf, i + 1, j, k)];
         s = s + uf[bgi(nf, i, j + 1, k)];
@@ -180,16 +211,43 @@ __device__ inline void bmg_rstrct(double *uc, const double *uf, int nc)
     wg_sync();
 }
 
-// mg_interp (mg.F90:608-633)
+// mg_interp (mg.F90:608-633).  The cubic Lagrange stencils depend only on the fine index, so
+// they are formed once per call into LDS (same function, same values) instead of per cell.
+constexpr int kBaseMaxN = 64;
 __device__ inline void bmg_interp(double *uf, double *uc, int nf, int cubic, int dirichlet)
 {
+    __shared__ int s_idx[kBaseMaxN][4];
+    __shared__ double s_a[kBaseMaxN][4];
     const int nc = nf / 2 > 0 ? nf / 2 : 1;
+    const bool use_cubic = cubic && nc + 2 >= 4;
     bmg_extrap(uc, nc, dirichlet);
     bmg_bu(uc, nc, dirichlet);
+    if (use_cubic) {
+        for (int f = threadIdx.x; f < nf; f += blockDim.x) cubic_stencil(f, -1, nc, s_idx[f], s_a[f]);
+        wg_sync();
+    }
     auto cget = [&](int a, int b, int c) { return uc[bgi(nc, a, b, c)]; };
+    const int lg = ilog2_pow2(nf);
     for (int t = threadIdx.x; t < nf * nf * nf; t += blockDim.x) {
-        int i = t % nf, j = (t / nf) % nf, k = t / (nf * nf);
-        uf[bgi(nf, i, j, k)] = (cubic && nc + 2 >= 4) ? tricubic_at(i, j, k, nc, cget) : trilinear_at(i, j, k, cget);
+        int i = t & (nf - 1), j = (t >> lg) & (nf - 1), k = t >> (2 * lg);
+        double v;
+        if (use_cubic) {
+            double bxy[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                double bx[4];
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    bx[b] = comb4(s_a[i], cget(s_idx[i][0], s_idx[j][b], s_idx[k][c]),
+                                  cget(s_idx[i][1], s_idx[j][b], s_idx[k][c]), cget(s_idx[i][2], s_idx[j][b], s_idx[k][c]),
+                                  cget(s_idx[i][3], s_idx[j][b], s_idx[k][c]));
+                bxy[c] = comb4(s_a[j], bx[0], bx[1], bx[2], bx[3]);
+            }
+            v = comb4(s_a[k], bxy[0], bxy[1], bxy[2], bxy[3]);
+        } else {
+            v = trilinear_at(i, j, k, cget);
+        }
+        uf[bgi(nf, i, j, k)] = v;
     }
     wg_sync();
     bmg_fill0(uf, nf, dirichlet);
@@ -219,60 +277,83 @@ __device__ inline void bmg_add(double *o, const double *a, int n)
     wg_sync();
 }
 
-// mg_lin (mg.F90:106-218), Ncycle = 2, Npre = Npost = 2; returns the last Resmaxg
-__device__ inline double bmg_lin(const BaseMG &M)
+// Level arrays: a = 0 U, 1 RHS, 2 RES, 3 RHO; levels with an LDS offset live in LDS.
+__device__ __forceinline__ double *lv(const BaseMG &M, double *lds, int l, int a)
+{
+    const int o = M.lds_off[l][a];
+    if (o >= 0) return lds + o;
+    return a == 0 ? M.U[l] : (a == 1 ? M.RHS[l] : (a == 2 ? M.RES[l] : M.RHO[l]));
+}
+
+// mg_lin (mg.F90:106-218), Ncycle = 2, Npre = Npost = 2; returns the last sweep's Resmaxg
+__device__ inline double bmg_lin(const BaseMG &M, double *lds)
 {
     const int lmax = M.nlev - 1, bc = M.dirichlet;
     double resmaxg = 0.0;
-    for (int l = 1; l <= lmax; ++l) bmg_rstrct(M.RHO[l], M.RHO[l - 1], M.n[l]);
-    bmg_zero(M.U[lmax], M.n[lmax]);
-    for (int it = 0; it < 2; ++it) resmaxg = bmg_relax(M.U[lmax], M.RHO[lmax], M.n[lmax], M.h[lmax], bc);
+    for (int l = 1; l <= lmax; ++l) bmg_rstrct(lv(M, lds, l, 3), lv(M, lds, l - 1, 3), M.n[l]);
+    bmg_zero(lv(M, lds, lmax, 0), M.n[lmax]);
+    for (int it = 0; it < 2; ++it) {
+        double r = bmg_relax(lv(M, lds, lmax, 0), lv(M, lds, lmax, 3), M.n[lmax], M.h[lmax], bc, lmax == 0 && it == 1);
+        if (lmax == 0 && it == 1) resmaxg = r;
+    }
     for (int ml = lmax - 1; ml >= 0; --ml) {
-        bmg_interp(M.U[ml], M.U[ml + 1], M.n[ml], 1, bc);
-        bmg_copy(M.RHS[ml], M.RHO[ml], M.n[ml]);
+        bmg_interp(lv(M, lds, ml, 0), lv(M, lds, ml + 1, 0), M.n[ml], 1, bc);
+        bmg_copy(lv(M, lds, ml, 1), lv(M, lds, ml, 3), M.n[ml]);
         for (int jc = 0; jc < 2; ++jc) {
             for (int v = ml; v <= lmax - 1; ++v) {
-                for (int p = 0; p < 2; ++p) resmaxg = bmg_relax(M.U[v], M.RHS[v], M.n[v], M.h[v], bc);
-                bmg_resid(M.RES[v], M.U[v], M.RHS[v], M.n[v], M.h[v], bc);
-                bmg_rstrct(M.RHS[v + 1], M.RES[v], M.n[v + 1]);
-                bmg_zero(M.U[v + 1], M.n[v + 1]);
+                for (int p = 0; p < 2; ++p) bmg_relax(lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc, false);
+                bmg_resid(lv(M, lds, v, 2), lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc);
+                bmg_rstrct(lv(M, lds, v + 1, 1), lv(M, lds, v, 2), M.n[v + 1]);
+                bmg_zero(lv(M, lds, v + 1, 0), M.n[v + 1]);
             }
-            for (int it = 0; it < 2; ++it) resmaxg = bmg_relax(M.U[lmax], M.RHS[lmax], M.n[lmax], M.h[lmax], bc);
+            for (int it = 0; it < 2; ++it)
+                bmg_relax(lv(M, lds, lmax, 0), lv(M, lds, lmax, 1), M.n[lmax], M.h[lmax], bc, false);
             for (int v = lmax - 1; v >= ml; --v) {
-                bmg_interp(M.RES[v], M.U[v + 1], M.n[v], 0, bc);
-                bmg_add(M.U[v], M.RES[v], M.n[v]);
-                for (int p = 0; p < 2; ++p) resmaxg = bmg_relax(M.U[v], M.RHS[v], M.n[v], M.h[v], bc);
+                bmg_interp(lv(M, lds, v, 2), lv(M, lds, v + 1, 0), M.n[v], 0, bc);
+                bmg_add(lv(M, lds, v, 0), lv(M, lds, v, 2), M.n[v]);
+                for (int p = 0; p < 2; ++p) {
+                    bool last = ml == 0 && jc == 1 && v == 0 && p == 1;
+                    double r = bmg_relax(lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc, last);
+                    if (last) resmaxg = r;
+                }
             }
         }
     }
     return resmaxg;
 }
 
-// mg_cycle (mg.F90:42-94) with the reference's stopping test Resmaxg / rhomax < tol.
-__global__ void __launch_bounds__(1024) k_base_mg(BaseMG M)
+// Levels below the finest keep their arrays in LDS as far as kBaseLdsBytes allows (all four on
+// levels of at most 8^3 cells, U and RHS on 16^3); the finest level stays in global memory.
+constexpr long kBaseLdsBytes = 150 * 1024;
+constexpr int kBaseThreads = 1024;
+
+// mg_cycle (mg.F90:42-94) with the reference's stopping test Resmaxg / rhomax < tol.  The final
+// residual that mg_cycle's caller never reads is not recomputed.
+__global__ void __launch_bounds__(kBaseThreads) k_base_mg(BaseMG M)
 {
+    extern __shared__ double lds[];
     const int n = M.n[0], bc = M.dirichlet;
+    double *rho0 = lv(M, lds, 0, 3);
     bmg_fill0(M.psi, n, bc);
     bmg_bu(M.psi, n, bc);
     double rhomax = 0.0;
     int it = 0;
     for (;;) {
-        bmg_resid(M.RHO[0], M.psi, M.src, n, M.h[0], bc);
-        double m = 0.0;
-        for (int t = threadIdx.x; t < n * n * n; t += blockDim.x) {
-            int i = t % n, j = (t / n) % n, k = t / (n * n);
-            m = fmax(m, fabs(M.RHO[0][bgi(n, i, j, k)]));
+        bmg_resid(rho0, M.psi, M.src, n, M.h[0], bc);
+        if (it == 0) {
+            double m = 0.0;
+            const int lg = ilog2_pow2(n);
+            for (int t = threadIdx.x; t < n * n * n; t += blockDim.x) {
+                int i = t & (n - 1), j = (t >> lg) & (n - 1), k = t >> (2 * lg);
+                m = fmax(m, fabs(rho0[bgi(n, i, j, k)]));
+            }
+            rhomax = wg_max(m);
         }
-        m = wg_max(m);
-        if (it == 0) rhomax = m;
         if (it >= M.maxit) break;
-        double resmaxg = bmg_lin(M);
-        bmg_add(M.psi, M.U[0], n);
+        double resmaxg = bmg_lin(M, lds);
+        bmg_add(M.psi, lv(M, lds, 0, 0), n);
         ++it;
-        if ((rhomax > 0 ? resmaxg / rhomax : 0.0) < M.tol) {
-            bmg_resid(M.RHO[0], M.psi, M.src, n, M.h[0], bc);
-            break;
-        }
+        if ((rhomax > 0 ? resmaxg / rhomax : 0.0) < M.tol) break;
     }
     if (threadIdx.x == 0 && M.iters) *M.iters = it;
 }

@@ -72,6 +72,7 @@ struct sg_uniform {
     int tile_fs[kMaxLev] = {};  // autotuned: source staged in LDS
     float tune_ms[kMaxLev][20] = {};
     double *psi_dev = nullptr, *src_dev = nullptr;   // staging for the host-pointer entry point
+    double *adj_src = nullptr, *adj_part = nullptr;  // adjoint: projected source, mean partials
     double *psi_alt = nullptr;                         // second phi buffer of the fused outer step
     double *pmax = nullptr, *psum = nullptr;          // norm partials
     int npart = 0;
@@ -685,6 +686,8 @@ static void destroy(sg_uniform *c)
     if (c->psi_dev) (void)hipFree(c->psi_dev);
     if (c->psi_alt) (void)hipFree(c->psi_alt);
     if (c->src_dev) (void)hipFree(c->src_dev);
+    if (c->adj_src) (void)hipFree(c->adj_src);
+    if (c->adj_part) (void)hipFree(c->adj_part);
     if (c->pmax) (void)hipFree(c->pmax);
     if (c->psum) (void)hipFree(c->psum);
     if (c->dnorm) (void)hipFree(c->dnorm);
@@ -826,6 +829,57 @@ int sg_uniform_gradient_dev(sg_uniform *c, const double *psi, double *g)
     SG_CHECK_LAUNCH();
     SG_HIP(hipStreamSynchronize(c->stream));
     return SG_OK;
+}
+
+int sg_uniform_gradient_sample_dev(sg_uniform *c, const double *psi, const int *cells, int npts, double *out)
+{
+    SG_REQUIRE(c && psi && cells && out && npts >= 0, "bad argument");
+    if (npts == 0) return SG_OK;
+    hipLaunchKernelGGL(sg::k_grad_sample, dim3(sg::nblocks(npts)), dim3(sg::kBlock), 0, c->stream, psi, cells, npts,
+                       sg::ilog2(c->n), 1.0 / c->h, out);
+    SG_CHECK_LAUNCH();
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_uniform_gradient_sample_adjoint_dev(sg_uniform *c, const int *cells, const double *w, int npts, double *out)
+{
+    SG_REQUIRE(c && cells && w && out && npts >= 0, "bad argument");
+    SG_HIP(hipMemsetAsync(out, 0, sizeof(double) * sg::n3(c->n), c->stream));
+    if (npts > 0) {
+        hipLaunchKernelGGL(sg::k_grad_sample_adjoint, dim3(sg::nblocks(npts)), dim3(sg::kBlock), 0, c->stream, cells, w,
+                           npts, sg::ilog2(c->n), 1.0 / c->h, out);
+        SG_CHECK_LAUNCH();
+    }
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+// d loss / d src of the solve src -> phi (phi = L^+ P src, P the zero-mean projection, L the
+// periodic 7-point operator, which is symmetric): grad = P L^+ P adj, by the same multigrid solver.
+int sg_uniform_adjoint_dev(sg_uniform *c, const double *adj, double *grad, const sg_params *p, sg_stats *st)
+{
+    SG_REQUIRE(c && adj && grad && p && st, "null argument");
+    const long len = sg::n3(c->n);
+    if (!c->adj_src) {
+        SG_HIP(hipMalloc(&c->adj_src, sizeof(double) * len));
+        SG_HIP(hipMalloc(&c->adj_part, sizeof(double) * sg::kMeanGroups));
+    }
+    auto project = [&](const double *a, double *out) -> int {
+        hipLaunchKernelGGL(sg::k_sum_part, dim3(sg::kMeanGroups), dim3(256), 0, c->stream, a, len, c->adj_part);
+        SG_CHECK_LAUNCH();
+        hipLaunchKernelGGL(sg::k_sub_mean, dim3(1024), dim3(256), 0, c->stream, a, out, len, (const double *)c->adj_part,
+                           sg::kMeanGroups);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
+    };
+    SG_TRY(project(adj, c->adj_src));
+    SG_HIP(hipMemsetAsync(grad, 0, sizeof(double) * len, c->stream));
+    const int rc = sg::uniform_cycle(c, grad, c->adj_src, p, st);
+    if (rc != SG_OK && rc != SG_ERR_NOT_CONVERGED) return rc;
+    SG_TRY(project(grad, grad));
+    SG_HIP(hipStreamSynchronize(c->stream));
+    return rc;
 }
 
 int sg_uniform_relax_dev(sg_uniform *c, int prec, void *u, const void *rhs, int n, double h, double *resmaxg)

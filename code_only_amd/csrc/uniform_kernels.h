@@ -322,4 +322,62 @@ __global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g, const 
     g[2 * n3 + c] = -(u[(wrap_p(k, n) * n + j) * n + i] - u[(wrap_m(k, n) * n + j) * n + i]) * hi * 0.5;
 }
 
+// ---- adjoint helpers (periodic grid, n = 2^lg) ----
+// Sampled centred gradient of phi at cells[p] (linear index (k n + j) n + i):
+// out[3p + d] = (phi(c + e_d) - phi(c - e_d)) / h * 0.5, the negative of fmg_psi2g's g
+// (fmg_poisson.F90:29-31).
+__global__ void k_grad_sample(const double *__restrict__ u, const int *__restrict__ cells, int npts, int lg, double hi,
+                              double *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int n = 1 << lg, c = cells[p];
+    const int i = c & (n - 1), j = (c >> lg) & (n - 1), k = c >> (2 * lg);
+    const int row = (k * n + j) * n;
+    out[3 * p + 0] = (u[row + wrap_p(i, n)] - u[row + wrap_m(i, n)]) * hi * 0.5;
+    out[3 * p + 1] = (u[(k * n + wrap_p(j, n)) * n + i] - u[(k * n + wrap_m(j, n)) * n + i]) * hi * 0.5;
+    out[3 * p + 2] = (u[(wrap_p(k, n) * n + j) * n + i] - u[(wrap_m(k, n) * n + j) * n + i]) * hi * 0.5;
+}
+
+// Transpose of k_grad_sample: out += sum_p sum_d w[3p + d] (delta(c_p + e_d) - delta(c_p - e_d)) / h * 0.5
+// (scatter with atomics; out must be cleared by the caller).
+__global__ void k_grad_sample_adjoint(const int *__restrict__ cells, const double *__restrict__ w, int npts, int lg,
+                                      double hi, double *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int n = 1 << lg, c = cells[p];
+    const int i = c & (n - 1), j = (c >> lg) & (n - 1), k = c >> (2 * lg);
+    const int row = (k * n + j) * n;
+    const double wx = w[3 * p] * hi * 0.5, wy = w[3 * p + 1] * hi * 0.5, wz = w[3 * p + 2] * hi * 0.5;
+    atomicAdd(out + row + wrap_p(i, n), wx);
+    atomicAdd(out + row + wrap_m(i, n), -wx);
+    atomicAdd(out + (k * n + wrap_p(j, n)) * n + i, wy);
+    atomicAdd(out + (k * n + wrap_m(j, n)) * n + i, -wy);
+    atomicAdd(out + (wrap_p(k, n) * n + j) * n + i, wz);
+    atomicAdd(out + (wrap_m(k, n) * n + j) * n + i, -wz);
+}
+
+// Mean removal (the projection P onto zero-mean fields): per-workgroup partial sums over a
+// grid-stride share, then every thread of k_sub_mean folds the partials in the same order.
+constexpr int kMeanGroups = 256;
+__global__ void k_sum_part(const double *__restrict__ a, long len, double *__restrict__ part)
+{
+    double s = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < len; t += (long)gridDim.x * blockDim.x) s += a[t];
+    double m = 0.0;
+    block_max_sum(m, s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_sub_mean(const double *__restrict__ a, double *__restrict__ out, long len, const double *__restrict__ part,
+                           int nparts)
+{
+    double s = 0.0;
+    for (int q = 0; q < nparts; ++q) s += part[q];
+    const double mean = s / (double)len;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < len; t += (long)gridDim.x * blockDim.x)
+        out[t] = a[t] - mean;
+}
+
 } // namespace sg

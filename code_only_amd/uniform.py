@@ -71,6 +71,38 @@ class UniformSolver:
         _lib.check(self._lib.sg_uniform_gradient_dev(self._ctx, _ptr(psi), _ptr(g)))
         return g
 
+    def adjoint(self, adj, tol=1e-10, maxit=50, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_MIXED,
+                use_graph=True, fused=True):
+        """d loss / d src = P L^+ P adj for the periodic solve (sg_uniform_adjoint_dev)."""
+        n = self.n
+        self._check(adj, shape=(n, n, n))
+        grad = torch.empty_like(adj)
+        p = _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=precision,
+                                use_graph=int(bool(use_graph)), fused=int(bool(fused)))
+        st = _lib.SgStats()
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_adjoint_dev(self._ctx, _ptr(adj), _ptr(grad), ctypes.byref(p), ctypes.byref(st)))
+        return grad, st.as_dict()
+
+    def gradient_sample(self, psi, cells):
+        """(npts, 3): centred grad psi at linear cell indices ``cells`` (int32 GPU tensor)."""
+        self._check(psi, shape=(self.n,) * 3)
+        self._check(cells, dtype=torch.int32)
+        out = torch.empty((cells.numel(), 3), dtype=torch.float64, device=psi.device)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_gradient_sample_dev(self._ctx, _ptr(psi), _ptr(cells), cells.numel(), _ptr(out)))
+        return out
+
+    def gradient_sample_adjoint(self, cells, w):
+        """Transpose of gradient_sample: the n^3 field sum_p w_p . d(grad psi(c_p))/d psi."""
+        self._check(cells, dtype=torch.int32)
+        self._check(w, shape=(cells.numel(), 3))
+        out = torch.empty((self.n,) * 3, dtype=torch.float64, device=w.device)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_uniform_gradient_sample_adjoint_dev(self._ctx, _ptr(cells), _ptr(w), cells.numel(),
+                                                                     _ptr(out)))
+        return out
+
     # single operators (parity tests)
     def relax(self, u, rhs, h, prec=SG_PREC_FP64):
         r = ctypes.c_double()

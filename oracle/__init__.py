@@ -186,8 +186,11 @@ class Amr:
             lib().orc_amr_set_zero_average(self.ptr, int(zero_average))
 
     def close(self):
-        if self.ptr:
-            lib().orc_amr_destroy(self.ptr)
+        if getattr(self, "ptr", None):
+            try:
+                lib().orc_amr_destroy(self.ptr)
+            except Exception:
+                pass
             self.ptr = None
 
     def __del__(self):

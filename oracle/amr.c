@@ -73,6 +73,7 @@ struct orc_amr {
     double resh2maxg;       /* Resh2maxg of the last fmg_relax */
     double resmaxg[ORC_AMR_MAXLEV];
     int zero_average;       /* ZERO_AVERAGE_DENSITY */
+    long base_calls, base_iters; /* statistics of the base-grid solves */
 };
 
 /* ------------------------------------------------------------------------------------------ */
@@ -708,6 +709,8 @@ static void vmg_slvsml(orc_amr *s, int L, int ju, int jrhs)
     orc_set_bc(s->bc == ORC_AMR_SPHERE ? ORC_BC_DIRICHLET : ORC_BC_PERIODIC);
     orc_mg_cycle(psi, src, N, hof(s, 0, L), &prm, &st);
     orc_set_bc(ORC_BC_PERIODIC);
+    s->base_calls += 1;
+    s->base_iters += st.iterations;
     for (int b = lev_begin(s, 0); b < lev_end(s, 0); ++b) {
         const int *p = s->ijk + 3 * b;
         double *u = arr(s, L, ju, b);
@@ -1081,6 +1084,7 @@ size_t orc_amr_block_len(const orc_amr *s, int L) { return s->bl[L]; }
 int orc_amr_get_fill0_level(const orc_amr *s) { return s->fill0_level; }
 void orc_amr_set_fill0_level(orc_amr *s, int v) { s->fill0_level = v; }
 double orc_amr_resh2maxg(const orc_amr *s) { return s->resh2maxg; }
+void orc_amr_base_stats(const orc_amr *s, long *calls, long *iters) { *calls = s->base_calls; *iters = s->base_iters; }
 
 /* Copy a whole state array ([nblk][(n+2)^3]) in or out. */
 void orc_amr_set_array(orc_amr *s, int L, int code, const double *src)

@@ -89,6 +89,7 @@ size_t orc_amr_block_len(const orc_amr *s, int L);
 int orc_amr_get_fill0_level(const orc_amr *s);
 void orc_amr_set_fill0_level(orc_amr *s, int v);
 double orc_amr_resh2maxg(const orc_amr *s);
+void orc_amr_base_stats(const orc_amr *s, long *calls, long *iters);
 void orc_amr_set_array(orc_amr *s, int L, int code, const double *src);
 void orc_amr_get_array(const orc_amr *s, int L, int code, double *dst);
 double orc_amr_op(orc_amr *s, int op, int L, int a, int b, int c, int flag);
