@@ -91,6 +91,12 @@ SIGNATURES = {
     "sg_amr_get_fill0_level": (_I, [_VP, _IP]),
     "sg_amr_set_fill0_level": (_I, [_VP, _I]),
     "sg_amr_op": (_I, [_VP, _I, _I, _I, _I, _I, _I, _DP]),
+    "sg_batch_create": (_I, [ctypes.POINTER(_VP), _I, _IP, _DP, _I]),
+    "sg_batch_destroy": (_I, [_VP]),
+    "sg_batch_solve_dev": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(SgParams),
+                                ctypes.POINTER(SgStats)]),
+    "sg_batch_tile_info": (_I, [_VP, _I, _IP, _IP, _IP, _IP, ctypes.POINTER(ctypes.c_float), _I]),
+    "sg_batch_levels": (_I, [_VP, _I, _IP, _IP]),
 }
 
 _LIB = None

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libsg.so")
 
-HIP_SOURCES = ["uniform.hip", "amr.hip"]
+HIP_SOURCES = ["uniform.hip", "amr.hip", "batch.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

@@ -95,6 +95,24 @@ template <int TX_, int TY_, int TZ_, int NT_> struct Tile {
     static constexpr int TX = TX_, TY = TY_, TZ = TZ_, NT = NT_;
 };
 
+// Tile shapes the autotuners choose from (all give identical values), each with 1 or 2 sweeps
+// per launch.
+using Tile0 = Tile<32, 16, 16, 1024>;
+using Tile1 = Tile<16, 8, 8, 1024>;
+using Tile2 = Tile<16, 8, 8, 256>;
+using Tile3 = Tile<64, 8, 8, 512>;
+using Tile4 = Tile<8, 8, 8, 512>;
+constexpr int kNumTiles = 5;
+constexpr int kMaxSweepsPerLaunch = 2;
+static const int kTileDims[kNumTiles][4] = {
+    {32, 16, 16, 1024}, {16, 8, 8, 1024}, {16, 8, 8, 256}, {64, 8, 8, 512}, {8, 8, 8, 512}};
+
+inline bool tile_fits(int cfg, int n)
+{
+    const int *d = kTileDims[cfg];
+    return n >= d[0] && n >= d[1] && n >= d[2] && n % d[0] == 0 && n % d[1] == 0 && n % d[2] == 0;
+}
+
 // One colour of the sweep over the LDS region shrunk by margin M: cells with
 // (gx + gy + gz) % 2 == colour, where colour 1 is the reference's first pass (ipass = 1).  The
 // source values are fetched for all of a thread's cells before any update.
@@ -177,12 +195,32 @@ template <class TL, int NS, bool FS> constexpr int sweep_lds_bytes()
 }
 template <class TL, int NS, bool FS> constexpr bool sweep_fits_lds() { return sweep_lds_bytes<TL, NS, FS>() <= 163840; }
 
+template <class TL> inline bool fits_lds(int ns, bool fs)
+{
+    if (ns == 2) return fs ? sweep_fits_lds<TL, 2, true>() : sweep_fits_lds<TL, 2, false>();
+    return fs ? sweep_fits_lds<TL, 1, true>() : sweep_fits_lds<TL, 1, false>();
+}
+
+// whether (tile shape, sweeps per launch, staging) is instantiated (fits the 160 KB of LDS)
+inline bool cand_ok(int cfg, int ns, bool fs)
+{
+    switch (cfg) {
+    case 0: return fits_lds<Tile0>(ns, fs);
+    case 1: return fits_lds<Tile1>(ns, fs);
+    case 2: return fits_lds<Tile2>(ns, fs);
+    case 3: return fits_lds<Tile3>(ns, fs);
+    default: return fits_lds<Tile4>(ns, fs);
+    }
+}
+
 // NS full sweeps (temporal blocking: halo 2 per sweep, +1 for the residual).  FS stages the source
 // on the same LDS region as u, loaded together with u, so the colour passes touch LDS only.
+// Body of the fused sweep for tile t of an n^3 level (n even, divisible by the tile extents);
+// called by every thread of a workgroup with the same t.
 template <class TL, int NS, bool ZERO, bool PROLONG, bool RESID, bool FS>
-__global__ void __launch_bounds__(TL::NT)
-k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const float *__restrict__ f, int n, float h2,
-             float h2i, const float *__restrict__ uc, float *__restrict__ rc, double *resmax)
+__device__ __forceinline__ void sweep_tile(const float *__restrict__ uin, float *__restrict__ uout,
+                                           const float *__restrict__ f, int n, float h2, float h2i,
+                                           const float *__restrict__ uc, float *__restrict__ rc, double *resmax, int t)
 {
     constexpr int H = 2 * NS + (RESID ? 1 : 0);
     constexpr int TX = TL::TX, TY = TL::TY, TZ = TL::TZ, NT = TL::NT;
@@ -193,10 +231,7 @@ k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const floa
     __shared__ float su[LX * LY * LZ];
     __shared__ float sf[FS ? LX * LY * LZ : 1];
     __shared__ float sc[PROLONG ? CX * CY * CZ : 1];
-    const int ntx = n / TX, nty = n / TY, ntz = n / TZ;
-    const int ntiles = ntx * nty * ntz;
-    const int t = xcd_tile(ntiles);
-    if (t >= ntiles) return;  // uniform per workgroup, before any barrier
+    const int ntx = n / TX, nty = n / TY;
     const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
     const int x0 = tx * TX - H, y0 = ty * TY - H, z0 = tz * TZ - H;
     const int nc = n >> 1;
@@ -325,21 +360,29 @@ k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const floa
     if (resmax) block_atomic_max(resmax, amax);
 }
 
+template <class TL, int NS, bool ZERO, bool PROLONG, bool RESID, bool FS>
+__global__ void __launch_bounds__(TL::NT)
+k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const float *__restrict__ f, int n, float h2,
+             float h2i, const float *__restrict__ uc, float *__restrict__ rc, double *resmax)
+{
+    const int ntiles = (n / TL::TX) * (n / TL::TY) * (n / TL::TZ);
+    const int t = xcd_tile(ntiles);
+    if (t >= ntiles) return;  // uniform per workgroup, before any barrier
+    sweep_tile<TL, NS, ZERO, PROLONG, RESID, FS>(uin, uout, f, n, h2, h2i, uc, rc, resmax, t);
+}
+
 // FMG tricubic prolongation uf = P3 uc (interpolation.F90:98-217) on an LDS tile: the coarse cells
 // any stencil of the fine tile can touch are staged, then the x, y and z passes run on LDS exactly
 // as the reference's bufx / bufxy sweeps.  Fine tile FX*FY*FZ, nf >= FX, nf % FY == nf % FZ == 0.
 template <int FX, int FY, int FZ, int NT>
-__global__ void __launch_bounds__(NT) k_interp_cubic_tile(float *__restrict__ uf, const float *__restrict__ uc, int nf)
+__device__ __forceinline__ void interp_cubic_tile(float *__restrict__ uf, const float *__restrict__ uc, int nf, int t)
 {
     constexpr int CX = FX / 2 + 4, CY = FY / 2 + 4, CZ = FZ / 2 + 4;
     __shared__ float sc[CX * CY * CZ];
     __shared__ float bx[CZ * CY * FX];
     __shared__ float bxy[CZ * FY * FX];
     const int nc = nf >> 1;
-    const int ntx = nf / FX, nty = nf / FY, ntz = nf / FZ;
-    const int ntiles = ntx * nty * ntz;
-    const int t = xcd_tile(ntiles);
-    if (t >= ntiles) return;
+    const int ntx = nf / FX, nty = nf / FY;
     const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
     const int fx0 = tx * FX, fy0 = ty * FY, fz0 = tz * FZ;
     const int bxo = fx0 / 2 - 2, byo = fy0 / 2 - 2, bzo = fz0 / 2 - 2;  // ghosted coarse index of local 0
@@ -402,6 +445,15 @@ __global__ void __launch_bounds__(NT) k_interp_cubic_tile(float *__restrict__ uf
         uf[((fz0 + z) * nf + (fy0 + y)) * nf + (fx0 + x)] =
             cubic4<float>(twz[z], p[tiz[z][0] * PL], p[tiz[z][1] * PL], p[tiz[z][2] * PL], p[tiz[z][3] * PL]);
     }
+}
+
+template <int FX, int FY, int FZ, int NT>
+__global__ void __launch_bounds__(NT) k_interp_cubic_tile(float *__restrict__ uf, const float *__restrict__ uc, int nf)
+{
+    const int ntiles = (nf / FX) * (nf / FY) * (nf / FZ);
+    const int t = xcd_tile(ntiles);
+    if (t >= ntiles) return;
+    interp_cubic_tile<FX, FY, FZ, NT>(uf, uc, nf, t);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -498,7 +550,7 @@ __device__ __forceinline__ void c_interp(float *Uf, const float *Uc, int nf, int
 // mode 1: one V-cycle from local level 0 with zero initial guess (mg.F90:179-210).
 // resmax receives max|resh2| of the final smoothing call when non-null (only meaningful when the
 // held level 0 is the finest level of the problem).
-__global__ void __launch_bounds__(kCoarseThreads)
+static __global__ void __launch_bounds__(kCoarseThreads)
 k_coarse(const float *__restrict__ rhs_in, float *__restrict__ u_out, CoarseLevels L, int mode, int ncycle,
          int npre, int npost, double *resmax)
 {

@@ -225,24 +225,6 @@ template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int
     return SG_OK;
 }
 
-// Tile shapes the autotuner chooses from (all give identical values), each with 1 or 2 sweeps
-// per launch.
-using Tile0 = Tile<32, 16, 16, 1024>;
-using Tile1 = Tile<16, 8, 8, 1024>;
-using Tile2 = Tile<16, 8, 8, 256>;
-using Tile3 = Tile<64, 8, 8, 512>;
-using Tile4 = Tile<8, 8, 8, 512>;
-constexpr int kNumTiles = 5;
-constexpr int kMaxSweepsPerLaunch = 2;
-static const int kTileDims[kNumTiles][4] = {
-    {32, 16, 16, 1024}, {16, 8, 8, 1024}, {16, 8, 8, 256}, {64, 8, 8, 512}, {8, 8, 8, 512}};
-
-static bool tile_fits(int cfg, int n)
-{
-    const int *d = kTileDims[cfg];
-    return n >= d[0] && n >= d[1] && n >= d[2] && n % d[0] == 0 && n % d[1] == 0 && n % d[2] == 0;
-}
-
 template <class TL, int NS, bool FS>
 static void sweep_variant(hipStream_t s, const float *uin, float *uout, const float *f, int n, float h2, float h2i,
                           bool zero, bool prolong, bool resid, const float *uc, float *rc, double *resmax)
@@ -276,24 +258,6 @@ static void sweep_ns(hipStream_t s, int ns, bool fs, const float *uin, float *uo
             sweep_variant<TL, 1, true>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax);
     } else {
         sweep_variant<TL, 1, false>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax);
-    }
-}
-
-template <class TL> static bool fits_lds(int ns, bool fs)
-{
-    if (ns == 2) return fs ? sweep_fits_lds<TL, 2, true>() : sweep_fits_lds<TL, 2, false>();
-    return fs ? sweep_fits_lds<TL, 1, true>() : sweep_fits_lds<TL, 1, false>();
-}
-
-// whether (tile shape, sweeps per launch, staging) is instantiated (fits the 160 KB of LDS)
-static bool cand_ok(int cfg, int ns, bool fs)
-{
-    switch (cfg) {
-    case 0: return fits_lds<Tile0>(ns, fs);
-    case 1: return fits_lds<Tile1>(ns, fs);
-    case 2: return fits_lds<Tile2>(ns, fs);
-    case 3: return fits_lds<Tile3>(ns, fs);
-    default: return fits_lds<Tile4>(ns, fs);
     }
 }
 

@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(256) k_update_resid(const double *__restrict__
 }
 
 // Second stage of the norm: one block reduces np partials in a fixed order -> out[0]=max, out[1]=sum.
-__global__ void __launch_bounds__(1024) k_norm_finalize(const double *pmax, const double *psum, int np,
+static __global__ void __launch_bounds__(1024) k_norm_finalize(const double *pmax, const double *psum, int np,
                                                         double *out)
 {
     double mx = 0.0, sm = 0.0;
@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(256) k_update(double *__restrict__ psi, const 
 }
 
 // fmg_poisson.F90:29-31: g_d = -(u(+1) - u(-1)) * hi * 0.5, periodic.
-__global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g, const double *__restrict__ u, int lg,
+static __global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g, const double *__restrict__ u, int lg,
                                                   double hi)
 {
     const int n = 1 << lg, n3 = 1 << (3 * lg), c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g, const 
 // Sampled centred gradient of phi at cells[p] (linear index (k n + j) n + i):
 // out[3p + d] = (phi(c + e_d) - phi(c - e_d)) / h * 0.5, the negative of fmg_psi2g's g
 // (fmg_poisson.F90:29-31).
-__global__ void k_grad_sample(const double *__restrict__ u, const int *__restrict__ cells, int npts, int lg, double hi,
+static __global__ void k_grad_sample(const double *__restrict__ u, const int *__restrict__ cells, int npts, int lg, double hi,
                               double *__restrict__ out)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -341,7 +341,7 @@ __global__ void k_grad_sample(const double *__restrict__ u, const int *__restric
 
 // Transpose of k_grad_sample: out += sum_p sum_d w[3p + d] (delta(c_p + e_d) - delta(c_p - e_d)) / h * 0.5
 // (scatter with atomics; out must be cleared by the caller).
-__global__ void k_grad_sample_adjoint(const int *__restrict__ cells, const double *__restrict__ w, int npts, int lg,
+static __global__ void k_grad_sample_adjoint(const int *__restrict__ cells, const double *__restrict__ w, int npts, int lg,
                                       double hi, double *__restrict__ out)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,7 +361,7 @@ __global__ void k_grad_sample_adjoint(const int *__restrict__ cells, const doubl
 // Mean removal (the projection P onto zero-mean fields): per-workgroup partial sums over a
 // grid-stride share, then every thread of k_sub_mean folds the partials in the same order.
 constexpr int kMeanGroups = 256;
-__global__ void k_sum_part(const double *__restrict__ a, long len, double *__restrict__ part)
+static __global__ void k_sum_part(const double *__restrict__ a, long len, double *__restrict__ part)
 {
     double s = 0.0;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < len; t += (long)gridDim.x * blockDim.x) s += a[t];
@@ -370,7 +370,7 @@ __global__ void k_sum_part(const double *__restrict__ a, long len, double *__res
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void k_sub_mean(const double *__restrict__ a, double *__restrict__ out, long len, const double *__restrict__ part,
+static __global__ void k_sub_mean(const double *__restrict__ a, double *__restrict__ out, long len, const double *__restrict__ part,
                            int nparts)
 {
     double s = 0.0;

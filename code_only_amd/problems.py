@@ -70,3 +70,53 @@ def sis_source(hier, seed=0, perturb=0.05):
     rho = to_blocks(hier, lambda x, y, z: 1.0 / (x * x + y * y + z * z + rc * rc))
     s = np.sqrt(np.log1p(perturb * perturb))
     return rho * rng.lognormal(-0.5 * s * s, s, size=rho.shape)
+
+
+BATCH_SIZES = (32, 64, 96, 128, 192)
+
+
+def clumpy_rhs_torch(n, seed=0, nclump=8, sigma_cells=4.0, device="cuda"):
+    """clumpy_rhs evaluated with torch on ``device`` (same centres and amplitudes; exp and sums
+    rounded by the device, so values agree with clumpy_rhs to a few ulp, not bitwise)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(size=(nclump, 3))
+    amps = rng.lognormal(mean=0.0, sigma=1.0, size=nclump)
+    h = 1.0 / n
+    sig = sigma_cells * h
+    x = (torch.arange(n, dtype=torch.float64, device=device) + 0.5) * h
+    rho = torch.full((n, n, n), 1.0, dtype=torch.float64, device=device)
+    for c in range(nclump):
+        g = []
+        for d in range(3):
+            dd = x - float(centres[c, d])
+            dd = dd - torch.round(dd)
+            g.append(torch.exp(-dd * dd / (2.0 * sig * sig)))
+        gx, gy, gz = g
+        rho += float(amps[c]) * (gz[:, None, None] * gy[None, :, None] * gx[None, None, :])
+    return rho - rho.mean()
+
+
+BATCH_SIZES = (32, 64, 96, 128, 192)
+
+
+def batch_spec(nprob=48, seed=0, sizes=BATCH_SIZES):
+    """The uneven batch: problem q has n_q drawn uniformly from ``sizes``, 1..16 clumps (uniform)
+    and its own seed.  Returns (list of n, list of clump counts, list of seeds)."""
+    rng = np.random.default_rng(seed)
+    ns = [int(x) for x in rng.choice(np.asarray(sizes), size=nprob)]
+    ncl = [int(x) for x in rng.integers(1, 17, size=nprob)]
+    seeds = [int(x) for x in rng.integers(0, 2 ** 31 - 1, size=nprob)]
+    return ns, ncl, seeds
+
+
+def batch_problems(nprob=48, seed=0, sizes=BATCH_SIZES, sigma_cells=4.0, device=None):
+    """Right-hand sides of the uneven batch (numpy, or torch tensors on ``device``).
+    Returns (list of n, list of rhs)."""
+    ns, ncl, seeds = batch_spec(nprob, seed, sizes)
+    if device is None:
+        rhs = [clumpy_rhs(n, seed=s, nclump=k, sigma_cells=sigma_cells) for n, s, k in zip(ns, seeds, ncl)]
+    else:
+        rhs = [clumpy_rhs_torch(n, seed=s, nclump=k, sigma_cells=sigma_cells, device=device)
+               for n, s, k in zip(ns, seeds, ncl)]
+    return ns, rhs

@@ -61,6 +61,9 @@ def lib():
         L.orc_mg_cycle_dense.argtypes = [dp, dp, ci, cd, ctypes.POINTER(OrcParams), ctypes.POINTER(OrcStats)]
         L.orc_mg_cycle_dense.restype = ci
         L.orc_gradient_dense.argtypes = [dp, dp, ci, cd]
+        L.orc_exact_basis.argtypes = [ci, cd, dp, dp]
+        L.orc_exact_periodic_dense.argtypes = [dp, dp, ci, cd]
+        L.orc_exact_periodic_dense.restype = ci
         L.orc_cubic_stencil.argtypes = [ci, ci, ci, ctypes.POINTER(ci), dp]
         L.orc_rstrct_cubic_weights.argtypes = [dp]
         L.orc_set_bc.argtypes = [ci]
@@ -143,12 +146,32 @@ def mg_cycle(src, h, psi=None, tol=1e-10, maxit=50, mode=STOP_TRUE_RESIDUAL, ncy
     st = OrcStats()
     lib().orc_set_bc(bc)
     try:
-        lib().orc_mg_cycle_dense(_p(psi), _p(src), n, h, ctypes.byref(p), ctypes.byref(st))
+        r = lib().orc_mg_cycle_dense(_p(psi), _p(src), n, h, ctypes.byref(p), ctypes.byref(st))
     finally:
         lib().orc_set_bc(BC_PERIODIC)
+    if r < 0:
+        raise ValueError(f"mg_cycle: unsupported grid n = {n} (n must be 2^a, or 3 * 2^a when periodic)")
     stats = dict(iterations=st.iterations, status=st.status, rel_linf=st.rel_linf, rel_l2=st.rel_l2,
                  resmaxg=st.resmaxg, hist=[st.hist[i] for i in range(min(st.iterations + 1, HIST))])
     return psi, stats
+
+
+def exact_basis(n, h):
+    """Hartley basis Q (n x n) and eigenvalues lam of the periodic 1-D second difference / h^2."""
+    Q = np.zeros((n, n))
+    lam = np.zeros(n)
+    lib().orc_exact_basis(n, h, _p(Q), _p(lam))
+    return Q, lam
+
+
+def exact_periodic(rhs, h):
+    """Exact zero-mean solution of the periodic 7-point problem (the coarsest-level solve of
+    n = 3 * 2^a grids)."""
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    u = np.zeros_like(rhs)
+    if lib().orc_exact_periodic_dense(_p(u), _p(rhs), rhs.shape[0], h) != 0:
+        raise ValueError("exact_periodic: n must be in [1, 64]")
+    return u
 
 
 def gradient(u, h):

@@ -23,6 +23,9 @@
  *   orc_mg_interp          mg.F90:608-633                          (mg_interp)
  *   orc_mg_lin             mg.F90:106-218                          (mg_lin)
  *   orc_mg_cycle           mg.F90:42-94                            (mg_cycle)
+ *   orc_exact_periodic     no reference counterpart: the coarsest-level solve of grids with
+ *                          n = 3 * 2^a (see orc_mg_setup), whose odd 3^3 level neither halves
+ *                          further nor admits a red-black ordering under periodic wrap
  *   orc_gradient           fmg_poisson.F90:29-31                   (centred g = -grad phi)
  */
 #include <math.h>
@@ -301,7 +304,7 @@ void orc_interp_tricubic(double *uf, const double *uc, int nf)
 /* ------------------------------------------------------------------------------------------ */
 
 typedef struct {
-    int nlev;              /* levels 0..nlev-1, n_l = n >> l, last level is 1^3 */
+    int nlev;              /* levels 0..nlev-1, n_l = n >> l, last level is 1^3 or 3^3 */
     int n[32];
     double h[32];
     double *U[32], *RHS[32], *RES[32], *RHO[32];
@@ -313,9 +316,16 @@ static double *zalloc(int n) { return (double *)calloc(orc_ghosted_len(n), sizeo
 static int orc_mg_setup(orc_mg *m, int n, double h)
 {
     memset(m, 0, sizeof(*m));
-    /* mg_data.F90:57-75: MG_LevelMax = log2(boxsize) (+0.5 rounding); level l has n/2^l cells. */
-    int lmax = (int)(log((double)n) / log(2.0) + 0.5);
-    if ((1 << lmax) != n) return -1;
+    /* mg_data.F90:57-75: MG_LevelMax = log2(boxsize) (+0.5 rounding); level l has n/2^l cells.
+     * That formula only describes n = 2^a (for n = 96 it yields a 1-cell level from a 3-cell one
+     * and then a 0-cell level).  Extension for the batched uniform problems: n = 3 * 2^a halves
+     * while even, down to a 3^3 coarsest level, which orc_exact_periodic solves exactly. */
+    int lmax = 0;
+    while ((n >> lmax) % 2 == 0) ++lmax;
+    const int nc = n >> lmax;
+    if (nc != 1 && nc != 3) return -1;
+    if (nc == 3 && g_bc != ORC_BC_PERIODIC) return -1;
+    if (lmax + 1 > 32) return -1;
     m->nlev = lmax + 1;
     for (int l = 0; l <= lmax; ++l) {
         m->n[l] = n >> l;
@@ -325,6 +335,75 @@ static int orc_mg_setup(orc_mg *m, int n, double h)
         m->RES[l] = zalloc(m->n[l]);
         m->RHO[l] = zalloc(m->n[l]);
     }
+    return 0;
+}
+
+/* Exact solve of (sum6 - 6u)/h^2 = rhs on a periodic n^3 grid (the mean of u set to zero).  The
+ * 1-D periodic second difference is diagonalised by the Hartley basis
+ *     Q[m][k] = (cos(2 pi mk/n) + sin(2 pi mk/n)) / sqrt(n),   lam[m] = (2 cos(2 pi m/n) - 2) / h^2,
+ * which is symmetric and orthonormal (Q Q = I), so u = Q3 (Lambda^+ (Q3 rhs)) with Q3 applied as
+ * three axis passes, each output line accumulated k = 0 .. n-1 in order; Lambda = (lam_i + lam_j)
+ * + lam_k, and the zero mode is set to zero.  u's ghosts are refreshed afterwards. */
+void orc_exact_basis(int n, double h, double *Q, double *lam)
+{
+    const double pi = 3.14159265358979323846;
+    for (int m = 0; m < n; ++m) {
+        for (int k = 0; k < n; ++k) {
+            const double a = 2.0 * pi * (double)((m * k) % n) / (double)n;
+            Q[m * n + k] = (cos(a) + sin(a)) / sqrt((double)n);
+        }
+        lam[m] = (2.0 * cos(2.0 * pi * (double)m / (double)n) - 2.0) / (h * h);
+    }
+}
+
+static void exact_pass(double *g, const double *Q, int n, int axis)
+{
+    double x[64], y[64];
+    for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a) {
+            for (int k = 0; k < n; ++k) {
+                const size_t q = axis == 0 ? ((size_t)b * n + a) * n + k
+                               : axis == 1 ? ((size_t)b * n + k) * n + a : ((size_t)k * n + b) * n + a;
+                x[k] = g[q];
+            }
+            for (int m = 0; m < n; ++m) {
+                double acc = 0.0;
+                for (int k = 0; k < n; ++k) acc = acc + Q[m * n + k] * x[k];
+                y[m] = acc;
+            }
+            for (int m = 0; m < n; ++m) {
+                const size_t q = axis == 0 ? ((size_t)b * n + a) * n + m
+                               : axis == 1 ? ((size_t)b * n + m) * n + a : ((size_t)m * n + b) * n + a;
+                g[q] = y[m];
+            }
+        }
+}
+
+int orc_exact_periodic(double *u, const double *rhs, int n, double h)
+{
+    if (n < 1 || n > 64) return -1;
+    double *Q = (double *)malloc(sizeof(double) * n * n), *lam = (double *)malloc(sizeof(double) * n);
+    double *g = (double *)malloc(sizeof(double) * n * n * n);
+    orc_exact_basis(n, h, Q, lam);
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) g[((size_t)k * n + j) * n + i] = rhs[GI(n, i, j, k)];
+    for (int a = 0; a < 3; ++a) exact_pass(g, Q, n, a);
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const double den = (lam[i] + lam[j]) + lam[k];
+                double *v = g + ((size_t)k * n + j) * n + i;
+                *v = den != 0.0 ? *v / den : 0.0;
+            }
+    for (int a = 0; a < 3; ++a) exact_pass(g, Q, n, a);
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) u[GI(n, i, j, k)] = g[((size_t)k * n + j) * n + i];
+    orc_boundary_u(u, n);
+    free(Q);
+    free(lam);
+    free(g);
     return 0;
 }
 
@@ -355,9 +434,13 @@ static void orc_mg_relax(orc_mg *m, int l, double *u, const double *rhs)
     m->resmaxg = orc_relax(u, rhs, m->n[l], m->h[l]);
 }
 
-/* mg.F90:540-546 */
+/* mg.F90:540-546 (two sweeps); a 3^3 coarsest level is solved exactly instead. */
 static void orc_mg_slvsml(orc_mg *m, int l, double *u, const double *rhs)
 {
+    if (m->n[l] == 3) {
+        orc_exact_periodic(u, rhs, 3, m->h[l]);
+        return;
+    }
     for (int it = 0; it < 2; ++it) orc_mg_relax(m, l, u, rhs);
 }
 
@@ -549,4 +632,15 @@ void orc_gradient_dense(double *g, const double *u, int n, double h)
     orc_to_ghosted(ug, u, n);
     orc_gradient(g, ug, n, h);
     free(ug);
+}
+
+int orc_exact_periodic_dense(double *u, const double *rhs, int n, double h)
+{
+    size_t len = orc_ghosted_len(n);
+    double *ug = (double *)calloc(len, sizeof(double)), *rg = (double *)calloc(len, sizeof(double));
+    orc_to_ghosted(rg, rhs, n);
+    int r = orc_exact_periodic(ug, rg, n, h);
+    orc_from_ghosted(u, ug, n);
+    free(ug); free(rg);
+    return r;
 }

@@ -44,6 +44,9 @@ void orc_cubic_stencil(int f, int lo, int hi, int idx[4], double a[4]);
 void orc_interp_tricubic(double *uf, const double *uc, int nf);
 void orc_mg_interp(double *uf, double *uc, int nf, int cubic);
 int orc_mg_cycle(double *psi, const double *src, int n, double h, const orc_params *p, orc_stats *st);
+void orc_exact_basis(int n, double h, double *Q, double *lam);
+int orc_exact_periodic(double *u, const double *rhs, int n, double h);
+int orc_exact_periodic_dense(double *u, const double *rhs, int n, double h);
 void orc_gradient(double *g, double *u, int n, double h);
 
 void orc_to_ghosted(double *ug, const double *dense, int n);

@@ -187,3 +187,46 @@ def test_clumpy_rhs_is_zero_mean():
     f = problems.clumpy_rhs(32, seed=0)
     assert abs(f.mean()) < 1e-14
     assert f.shape == (32, 32, 32)
+
+
+# ---- n = 3 * 2^a (the batched uneven problems): levels halve down to 3^3, solved exactly ----
+
+@pytest.mark.parametrize("n", [3, 5, 6, 12])
+def test_exact_periodic_solve_matches_fft(n):
+    """The Hartley-basis solve is the exact zero-mean periodic solution (FFT), for odd n too."""
+    rng = np.random.default_rng(n)
+    f = rng.standard_normal((n, n, n))
+    f -= f.mean()
+    h = 1.0 / n
+    u = oracle.exact_periodic(f, h)
+    ref = fp.solve(f, h)
+    assert np.abs(u - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert abs(u.mean()) < 1e-14 * np.abs(ref).max()
+
+
+def test_exact_basis_is_orthonormal_and_diagonalises():
+    """Q Q = I (symmetric, orthonormal) and Q D Q = diag(lam) for the periodic second difference D."""
+    n, h = 3, 0.25
+    Q, lam = oracle.exact_basis(n, h)
+    assert np.allclose(Q, Q.T, atol=1e-15) and np.allclose(Q @ Q, np.eye(n), atol=1e-14)
+    D = (np.roll(np.eye(n), 1, 0) + np.roll(np.eye(n), -1, 0) - 2 * np.eye(n)) / h ** 2
+    assert np.allclose(Q @ D @ Q, np.diag(lam), atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [6, 24, 48])
+def test_mg_cycle_non_power_of_two_matches_fft(n):
+    f = problems.clumpy_rhs(n, seed=n, nclump=5, sigma_cells=2.0)
+    h = 1.0 / n
+    psi, st = oracle.mg_cycle(f, h, tol=1e-13, maxit=30)
+    assert st["status"] == 0
+    ref = fp.solve(f, h)
+    assert np.linalg.norm((psi - psi.mean()) - ref) / np.linalg.norm(ref) < 1e-12
+    # a 3^3 coarsest level solved exactly keeps the FMG contraction of the power-of-two case
+    hist = st["hist"]
+    assert all(hist[i + 1] < 1e-2 * hist[i] for i in range(min(len(hist) - 2, 3)))
+
+
+@pytest.mark.parametrize("n,bc", [(80, oracle.BC_PERIODIC), (48, oracle.BC_DIRICHLET)])
+def test_mg_cycle_rejects_unsupported_grids(n, bc):
+    with pytest.raises(ValueError):
+        oracle.mg_cycle(np.zeros((n, n, n)), 1.0 / n, bc=bc)
