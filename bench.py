@@ -34,6 +34,10 @@ def parse():
     ap.add_argument("--smoother-reps", type=int, default=50)
     ap.add_argument("--no-amr", action="store_true", help="skip the AMR hierarchy solve")
     ap.add_argument("--amr-steps", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem uneven batch")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--section", choices=["amr", "batch"], default=None,
+                    help="run only the AMR or the batch section and print its JSON (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (reference reading for HBM counters)")
     return ap.parse_args()
@@ -151,6 +155,77 @@ def bench_amr(a, dev):
     return out
 
 
+def bench_batch(a, dev):
+    """48 independent periodic problems, sizes drawn uniformly from 32, 64, 96, 128, 192 (seeded),
+    each a clumpy density with 1..16 clumps, all solved together to rel. residual 1e-10: the
+    mixed-precision batch (fused FP32 inner solve, graph-replayed) against the plain FP64 batch."""
+    import numpy as np
+    import torch
+    from code_only_amd import problems
+    from code_only_amd.batch import SG_PREC_FP64, SG_PREC_MIXED, BatchSolver
+
+    ns, rhs = problems.batch_problems(48, seed=0, device=dev)
+    cells = sum(n ** 3 for n in ns)
+    log(f"[bench] batch: 48 problems, sizes {sorted(set(ns))}, {cells} cells")
+    solver = BatchSolver(ns)
+
+    def run(prec):
+        psis = [torch.zeros_like(r) for r in rhs]
+        st = solver.solve(psis, rhs, tol=a.tol, maxit=30, precision=prec)  # warm-up (autotune, plan, graph)
+        times = []
+        for _ in range(max(1, a.batch_steps)):
+            for p in psis:
+                p.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = solver.solve(psis, rhs, tol=a.tol, maxit=30, precision=prec)
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(times)), st, psis
+
+    ms64, st64, p64 = run(SG_PREC_FP64)
+    log(f"[bench] batch fp64: {ms64:.2f} ms")
+    msmx, stmx, pmx = run(SG_PREC_MIXED)
+    log(f"[bench] batch mixed: {msmx:.2f} ms")
+    worst = 0.0
+    for q in range(len(ns)):
+        x, y = pmx[q] - pmx[q].mean(), p64[q] - p64[q].mean()
+        worst = max(worst, (torch.linalg.norm(x - y) / torch.linalg.norm(y)).item())
+    out = {"workload": "48 periodic problems, n uniform in {32, 64, 96, 128, 192} (seed 0), 1-16 clumps each",
+           "sizes": ns, "cells": cells,
+           "fp64_gpu_ms": round(ms64, 3), "mixed_gpu_ms": round(msmx, 3),
+           "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
+           "iterations": {"mixed": sorted(set(s["iterations"] for s in stmx)),
+                          "fp64": sorted(set(s["iterations"] for s in st64))},
+           "max_rel_linf": {"mixed": max(s["rel_linf"] for s in stmx), "fp64": max(s["rel_linf"] for s in st64)},
+           "all_converged": all(s["status"] == 0 for s in stmx + st64),
+           "mixed_vs_fp64_gpu_worst_psi_rel_l2": worst,
+           "tiles": {str(v): solver.tile_info(v) for v in range(4)}}
+    for v in out["tiles"].values():
+        v.pop("ms", None)
+    log(f"[bench] batch: mixed vs fp64 worst rel L2 {worst:.2e}, speedup {ms64 / msmx:.2f}")
+    if not a.no_cpu:
+        import oracle
+        log("[bench] batch CPU oracle (FP64, 1 core) on all 48 problems")
+        host = [r.cpu().numpy() for r in rhs]
+        t0 = time.perf_counter()
+        sols = [oracle.mg_cycle(f, 1.0 / n, tol=a.tol, maxit=30) for f, n in zip(host, ns)]
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        bit = all(np.array_equal(p64[q].cpu().numpy(), sols[q][0]) for q in range(len(ns)))
+        wm = 0.0
+        for q in range(len(ns)):
+            po = sols[q][0] - sols[q][0].mean()
+            pm = pmx[q].cpu().numpy()
+            pm = pm - pm.mean()
+            wm = max(wm, float(np.linalg.norm(pm - po) / np.linalg.norm(po)))
+        out["cpu_oracle_ms"] = round(cpu_ms, 1)
+        out["fp64_gpu_bitwise_vs_oracle"] = bit
+        out["mixed_vs_oracle_worst_psi_rel_l2"] = wm
+        log(f"[bench] batch cpu oracle {cpu_ms:.0f} ms; fp64 bitwise {bit}; mixed vs oracle worst {wm:.2e}")
+    solver.close()
+    return out
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -169,6 +244,10 @@ def main():
     from code_only_amd import _lib
 
     dev = torch.device("cuda:0")
+    if a.section:
+        res = bench_amr(a, dev) if a.section == "amr" else bench_batch(a, dev)
+        print(json.dumps({"section": a.section, **res}), flush=True)
+        return
     n = a.n
     h = 1.0 / n
     log(f"[bench] generating clumpy rhs n={n}")
@@ -288,6 +367,8 @@ def main():
     }
     if not a.no_amr:
         out["amr"] = bench_amr(a, dev)
+    if not a.no_batch:
+        out["batch"] = bench_batch(a, dev)
     print(json.dumps(out), flush=True)
     solver.close()
 

@@ -40,26 +40,35 @@ def build_hip(verbose=False):
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
 FLANG = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
-FORTRAN_MODULES = ["sg_capi.F90", "mg_gpu.F90"]
-FORTRAN_TESTS = [os.path.join(ROOT, "tests", "fortran", "mg_drop_in_test.F90")]
+TESTS_FORTRAN = os.path.join(ROOT, "tests", "fortran")
+# Sources compiled in order (module dependencies first).  fmg_gpu.F90 uses the host's module
+# `grid`; here that is the test harness's (tests/fortran/grid_host.F90), in the reference's build
+# it is the reference's own grid module.
+FORTRAN_SOURCES = [os.path.join(FORTRAN_DIR, "sg_capi.F90"), os.path.join(FORTRAN_DIR, "mg_gpu.F90"),
+                   os.path.join(TESTS_FORTRAN, "grid_host.F90"), os.path.join(FORTRAN_DIR, "fmg_gpu.F90")]
+# test driver -> objects it links
+FORTRAN_TESTS = {
+    os.path.join(TESTS_FORTRAN, "mg_drop_in_test.F90"): ["sg_capi", "mg_gpu"],
+    os.path.join(TESTS_FORTRAN, "fmg_drop_in_test.F90"): ["sg_capi", "grid_host", "fmg_gpu"],
+}
 
 
 def build_fortran(verbose=False):
-    """Compile the Fortran drop-in modules and the Fortran test driver, linked against libsg.so."""
+    """Compile the Fortran drop-in modules and the Fortran test drivers, linked against libsg.so."""
     os.makedirs(FORTRAN_BUILD, exist_ok=True)
-    objs = []
-    for src in FORTRAN_MODULES:
-        obj = os.path.join(FORTRAN_BUILD, src.replace(".F90", ".o"))
-        cmd = [FLANG, "-O2", "-fPIC", "-c", os.path.join(FORTRAN_DIR, src), "-module-dir", FORTRAN_BUILD,
-               "-o", obj]
+    objs = {}
+    for src in FORTRAN_SOURCES:
+        name = os.path.basename(src).replace(".F90", "")
+        obj = os.path.join(FORTRAN_BUILD, name + ".o")
+        cmd = [FLANG, "-O2", "-fPIC", "-c", src, "-module-dir", FORTRAN_BUILD, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+        objs[name] = obj
     exes = []
-    for src in FORTRAN_TESTS:
+    for src, deps in FORTRAN_TESTS.items():
         exe = os.path.join(FORTRAN_BUILD, os.path.basename(src).replace(".F90", ""))
-        cmd = [FLANG, "-O2", src, "-I", FORTRAN_BUILD] + objs + [
+        cmd = [FLANG, "-O2", src, "-I", FORTRAN_BUILD] + [objs[d] for d in deps] + [
             "-L", HERE, "-lsg", "-Wl,-rpath," + HERE + ",-rpath,$ORIGIN/../..", "-o", exe]
         if verbose:
             print(" ".join(cmd), flush=True)

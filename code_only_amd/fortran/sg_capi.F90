@@ -9,6 +9,7 @@ module sg_capi
        SG_ERR_NOT_CONVERGED = 3, SG_ERR_NONFINITE = 4
   integer(c_int),parameter :: SG_STOP_REFERENCE = 0, SG_STOP_TRUE_RESIDUAL = 1
   integer(c_int),parameter :: SG_PREC_FP64 = 1, SG_PREC_MIXED = 2
+  integer(c_int),parameter :: SG_BC_PERIODIC = 0, SG_BC_SPHERE = 1
   integer,parameter :: SG_HIST = 64
 
   type, bind(C) :: sg_params
@@ -67,6 +68,41 @@ module sg_capi
        type(sg_stats),intent(OUT) :: st
        integer(c_int) :: rc
      end function sg_uniform_cycle_host
+
+     function sg_amr_create(ctx, nblk, nlev, level, ijk, parent, child, nbr, base_dims, h0, x0, bc) &
+          bind(C, name='sg_amr_create') result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr),intent(OUT) :: ctx
+       integer(c_int),value :: nblk, nlev, bc
+       integer(c_int),intent(IN) :: level(*), ijk(*), parent(*), child(*), nbr(*), base_dims(3)
+       real(c_double),value :: h0
+       real(c_double),intent(IN) :: x0(3)
+       integer(c_int) :: rc
+     end function sg_amr_create
+
+     function sg_amr_destroy(ctx) bind(C, name='sg_amr_destroy') result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr),value :: ctx
+       integer(c_int) :: rc
+     end function sg_amr_destroy
+
+     function sg_amr_set_zero_average(ctx, on) bind(C, name='sg_amr_set_zero_average') result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr),value :: ctx
+       integer(c_int),value :: on
+       integer(c_int) :: rc
+     end function sg_amr_set_zero_average
+
+     function sg_amr_solve_host(ctx, psi, src, g, p, st) bind(C, name='sg_amr_solve_host') result(rc)
+       import :: c_ptr, c_int, c_double, sg_params, sg_stats
+       type(c_ptr),value :: ctx
+       real(c_double),intent(INOUT) :: psi(*)
+       real(c_double),intent(IN) :: src(*)
+       real(c_double),intent(OUT) :: g(*)
+       type(sg_params),intent(IN) :: p
+       type(sg_stats),intent(OUT) :: st
+       integer(c_int) :: rc
+     end function sg_amr_solve_host
   end interface
 
 contains
