@@ -290,19 +290,25 @@ def main():
 
     tiles = {}
     dims = [(32, 16, 16, 1024), (16, 8, 8, 1024), (16, 8, 8, 256), (64, 8, 8, 512), (8, 8, 8, 512)]
+    sdims = [(64, 16, 32, 512), (32, 16, 32, 256), (64, 32, 32, 512)]
+    names = ["x".join(map(str, d[:3])) + f"/{d[3]}" for d in dims] + \
+            ["stream " + "x".join(map(str, d[:2])) + f" z{d[2]}/{d[3]}" for d in sdims]
     lev, m = 0, n
     while m > 16:
         cfg, ns, fs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        ms = (ctypes.c_float * 20)()
+        ms = (ctypes.c_float * 32)()
         _lib.check(solver._lib.sg_uniform_tile_info(solver._ctx, lev, ctypes.byref(cfg), ctypes.byref(ns),
-                                                    ctypes.byref(fs), ms, 20))
+                                                    ctypes.byref(fs), ms, 32))
         cand = {}
-        for q in range(4 * len(dims)):
-            d, k, st = dims[q // 4], 1 + (q // 2) % 2, q % 2
+        for q in range(4 * len(dims) + len(sdims)):
+            if q < 4 * len(dims):
+                key = names[q // 4] + f"/{1 + (q // 2) % 2}sw" + ("/fLDS" if q % 2 else "")
+            else:
+                key = names[len(dims) + q - 4 * len(dims)]
             if ms[q] > 0:
-                cand["x".join(map(str, d[:3])) + f"/{d[3]}/{k}sw" + ("/fLDS" if st else "")] = round(ms[q] * 1e3, 1)
-        tiles[str(m)] = {"tile": "x".join(map(str, dims[cfg.value][:3])), "threads": dims[cfg.value][3],
-                         "sweeps_per_launch": ns.value, "source_in_lds": bool(fs.value), "us_per_2_sweeps": cand}
+                cand[key] = round(ms[q] * 1e3, 1)
+        tiles[str(m)] = {"shape": names[cfg.value], "sweeps_per_launch": ns.value, "source_in_lds": bool(fs.value),
+                         "us_per_2_sweeps": cand}
         lev += 1
         m //= 2
     log(f"[bench] autotuned tiles {json.dumps(tiles)}")
@@ -310,7 +316,7 @@ def main():
     # HBM bytes per launch of the roofline kernel, from the committed rocprofv3 counter summary
     traffic = None
     prof = os.path.join(ROOT, "profiles", "uniform256_hbm_traffic.md")
-    if n == 256 and tiles.get("256", {}).get("tile") == "32x16x16" and os.path.exists(prof):
+    if n == 256 and tiles.get("256", {}).get("shape") == "32x16x16/1024" and os.path.exists(prof):
         for line in open(prof):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = int(line.split(":")[1])
@@ -359,7 +365,7 @@ def main():
         "phi_err_vs_cpu_oracle": err_vs_oracle,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "fine-level FP32 smoother sweep (both colours), 32x16x16 LDS tile",
+                     "kernel": "fine-level FP32 smoother sweep (both colours), " + tiles.get(str(n), {}).get("shape", "?"),
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "ms_per_launch": ms_sweep.value},
         "cpu_baseline": cpu,

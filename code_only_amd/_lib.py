@@ -78,6 +78,7 @@ SIGNATURES = {
     "sg_uniform_time_smoother": (_I, [_VP, _I, _I, _DP]),
     "sg_uniform_tile_info": (_I, [_VP, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                   ctypes.POINTER(ctypes.c_float), _I]),
+    "sg_uniform_set_tile": (_I, [_VP, _I, _I, _I, _I]),
     "sg_uniform_adjoint_dev": (_I, [_VP, _VP, _VP, ctypes.POINTER(SgParams), ctypes.POINTER(SgStats)]),
     "sg_uniform_gradient_sample_dev": (_I, [_VP, _VP, _VP, _I, _VP]),
     "sg_uniform_gradient_sample_adjoint_dev": (_I, [_VP, _VP, _VP, _I, _VP]),

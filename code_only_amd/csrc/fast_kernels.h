@@ -371,6 +371,262 @@ k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const floa
     sweep_tile<TL, NS, ZERO, PROLONG, RESID, FS>(uin, uout, f, n, h2, h2i, uc, rc, resmax, t);
 }
 
+// ------------------------------------------------------------------------------------------------
+// Streaming form of the fused sweep: a workgroup owns an SX x SY column of the grid and a chunk of
+// ZC planes, and marches up the chunk with the planes it needs in an LDS ring.  Step p updates
+// the first colour (i+j+k odd, the reference's ipass 1) of plane p and then the second colour of
+// plane p-1, whose first-colour neighbours on planes p-2, p-1 and p are then all new; plane p-1 is
+// final and is stored.  The next plane's u and f are loaded into registers before the step's
+// arithmetic and written into the ring after it, so global latency overlaps LDS work.  Halo: H = 2
+// cells in x and y (first colour on margin 1, second on margin 2), H planes below and above the
+// chunk; with RESID, H = 3 and the residual of each final plane is formed one step later and
+// restricted pairwise (restriction.F90:132-147 operand order).  Every cell update sees the same
+// operands as the two-launch colour passes, so the values are those of the tile kernel, bit for bit.
+// ------------------------------------------------------------------------------------------------
+template <int SX_, int SY_, int ZC_, int NT_> struct StreamShape {
+    static constexpr int SX = SX_, SY = SY_, ZC = ZC_, NT = NT_;
+};
+
+template <class SS, bool PROLONG, bool RESID> struct StreamGeom {
+    static constexpr int H = RESID ? 3 : 2;
+    static constexpr int PX = SS::SX + 2 * H, PY = SS::SY + 2 * H, PP = PX * PY;
+    static constexpr int R = RESID ? 5 : 4;                     // ring slots
+    static constexpr int CX = PX / 2 + 2, CY = PY / 2 + 2, CP = CX * CY;  // staged coarse plane
+    static constexpr int IL = (PP + SS::NT - 1) / SS::NT;      // plane cells per thread
+    static constexpr int ICL = (CP + SS::NT - 1) / SS::NT;
+    static constexpr int lds_bytes = 4 * (2 * R * PP + (PROLONG ? 4 * CP : 0) + (RESID ? SS::SX * SS::SY : 0));
+};
+
+template <class SS, bool PROLONG, bool RESID> constexpr bool stream_fits_lds()
+{
+    return StreamGeom<SS, PROLONG, RESID>::lds_bytes + 1024 <= 163840;
+}
+
+__device__ __forceinline__ int pmod(int a, int m) { const int r = a % m; return r < 0 ? r + m : r; }
+__device__ __forceinline__ int fdiv2(int a) { return a >> 1; }  // floor(a / 2), arithmetic shift
+
+template <class SS, bool ZERO, bool PROLONG, bool RESID>
+__device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, float *__restrict__ uout,
+                                             const float *__restrict__ f, int n, float h2, float h2i,
+                                             const float *__restrict__ uc, float *__restrict__ rc, double *resmax, int t)
+{
+    using G = StreamGeom<SS, PROLONG, RESID>;
+    constexpr int SX = SS::SX, SY = SS::SY, ZC = SS::ZC, NT = SS::NT;
+    constexpr int H = G::H, PX = G::PX, PY = G::PY, PP = G::PP, R = G::R, CX = G::CX, CP = G::CP;
+    static_assert(SX % 2 == 0 && SY % 2 == 0 && ZC % 2 == 0, "even extents");
+    __shared__ float su[R * PP];
+    __shared__ float sf[R * PP];
+    __shared__ float scz[PROLONG ? 4 * CP : 1];
+    __shared__ float srp[RESID ? SX * SY : 1];
+    const int ntx = n / SX, nty = n / SY;
+    const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+    const int x0 = tx * SX, y0 = ty * SY, z0 = tz * ZC;
+    const int nc = n >> 1;
+    const int tid = threadIdx.x;
+    double amax = 0.0;
+
+    float ru[G::IL], rf[G::IL], rcz[PROLONG ? G::ICL : 1];
+    auto gaddr = [&](int q, int k) {
+        const int x = q % PX, y = q / PX;
+        return (wrapn(k, n) * n + wrapn(y0 - H + y, n)) * n + wrapn(x0 - H + x, n);
+    };
+    auto fetch = [&](int k) {
+#pragma unroll
+        for (int it = 0; it < G::IL; ++it) {
+            const int q = tid + it * NT;
+            if (PP % NT == 0 || q < PP) {
+                const int g = gaddr(q, k);
+                ru[it] = ZERO ? 0.0f : uin[g];
+                rf[it] = f[g];
+            }
+        }
+    };
+    // coarse plane c of the staged region: coarse x from (x0 - H)/2 - 1, y from (y0 - H)/2 - 1
+    auto fetch_coarse = [&](int c) {
+#pragma unroll
+        for (int it = 0; it < G::ICL; ++it) {
+            const int q = tid + it * NT;
+            if (CP % NT == 0 || q < CP) {
+                const int x = q % CX, y = q / CX;
+                rcz[it] = uc[(wrapn(c, nc) * nc + wrapn(fdiv2(y0 - H) - 1 + y, nc)) * nc + wrapn(fdiv2(x0 - H) - 1 + x, nc)];
+            }
+        }
+    };
+    auto put_coarse = [&](int c) {
+        float *dst = scz + pmod(c, 4) * CP;
+#pragma unroll
+        for (int it = 0; it < G::ICL; ++it) {
+            const int q = tid + it * NT;
+            if (CP % NT == 0 || q < CP) dst[q] = rcz[it];
+        }
+    };
+    // write the fetched plane k into its ring slot (adding the trilinear correction with PROLONG)
+    auto put = [&](int k) {
+        float *du = su + pmod(k, R) * PP, *df = sf + pmod(k, R) * PP;
+        const int kc = fdiv2(k), kc1 = kc + 2 * (k & 1) - 1;
+        const float *c0 = scz + pmod(kc, 4) * CP, *c1 = scz + pmod(kc1, 4) * CP;
+#pragma unroll
+        for (int it = 0; it < G::IL; ++it) {
+            const int q = tid + it * NT;
+            if (PP % NT == 0 || q < PP) {
+                float v = ru[it];
+                if (PROLONG) {
+                    const int x = q % PX, y = q / PX;
+                    const int ic = (x >> 1) + 1, jc = (y >> 1) + 1;
+                    const int ic1 = ic + 2 * (x & 1) - 1, jc1 = jc + 2 * (y & 1) - 1;
+                    v = v + trilinear_value<float>(c0[jc * CX + ic], c0[jc * CX + ic1], c0[jc1 * CX + ic], c1[jc * CX + ic],
+                                                   c1[jc1 * CX + ic], c1[jc * CX + ic1], c0[jc1 * CX + ic1],
+                                                   c1[jc1 * CX + ic1]);
+                }
+                du[q] = v;
+                df[q] = rf[it];
+            }
+        }
+    };
+    // one colour of plane k on the region shrunk by margin M; colour 1: i+j+k odd
+    auto colour = [&](int k, int M, int col, bool out) {
+        const int RX = (PX - 2 * M) / 2, RY = PY - 2 * M, CNT = RX * RY;
+        float *u = su + pmod(k, R) * PP;
+        const float *um = su + pmod(k - 1, R) * PP, *up = su + pmod(k + 1, R) * PP;
+        const float *fk = sf + pmod(k, R) * PP;
+        for (int q = tid; q < CNT; q += NT) {
+            const int xh = q % RX, y = M + q / RX;
+            // parity of the global cell (x0 - H + x) + (y0 - H + y) + k must be col (odd for col 1)
+            const int d = (x0 + y0 + y + k + col + M) & 1;
+            const int x = M + 2 * xh + d;
+            const int c = y * PX + x;
+            float sres;
+            u[c] = relax_value<float>(u[c], u[c + 1], u[c - 1], u[c + PX], u[c - PX], up[c], um[c], fk[c], h2, &sres);
+            if (out && x >= H && x < H + SX && y >= H && y < H + SY) amax = fmax(amax, fabs((double)sres));
+        }
+    };
+    const int zlo = z0 - H, zhi = z0 + ZC + H - 1;  // planes held at some point
+    // prologue: planes zlo .. zlo + 2 (and the coarse planes they need)
+    if (PROLONG) {
+        for (int c = fdiv2(zlo) - 1; c <= fdiv2(zlo + 2) + 1; ++c) {
+            fetch_coarse(c);
+            put_coarse(c);
+        }
+        __syncthreads();
+    }
+    for (int k = zlo; k <= zlo + 2; ++k) {
+        fetch(k);
+        put(k);
+    }
+    __syncthreads();
+    int cmax = fdiv2(zlo + 2) + 1;  // highest coarse plane staged
+    for (int p = zlo + 1; p <= zhi - 1; ++p) {
+        const int knext = p + 2;
+        const bool more = knext <= zhi;
+        const bool cnew = PROLONG && more && fdiv2(knext) + 1 > cmax;
+        if (more) fetch(knext);
+        if (cnew) fetch_coarse(fdiv2(knext) + 1);
+        // first colour of plane p (margin 1)
+        colour(p, 1, 1, p >= z0 && p < z0 + ZC);
+        __syncthreads();
+        // second colour of plane p - 1 (margin 2): final
+        // (planes zlo + 2 .. zhi - 2: the core planes, and with RESID one more on each side)
+        if (p - 1 >= zlo + 2) colour(p - 1, 2, 0, p - 1 >= z0 && p - 1 < z0 + ZC);
+        __syncthreads();
+        if (p - 1 >= z0 && p - 1 < z0 + ZC) {
+            const float *u = su + pmod(p - 1, R) * PP;
+            const int gz = wrapn(p - 1, n);
+            for (int q = tid; q < SX * SY; q += NT) {
+                const int x = q % SX, y = q / SX;
+                uout[(gz * n + y0 + y) * n + x0 + x] = u[(y + H) * PX + x + H];
+            }
+        }
+        if (RESID && p - 2 >= z0 && p - 2 < z0 + ZC) {
+            const int kq = p - 2;
+            const float *u = su + pmod(kq, R) * PP, *um = su + pmod(kq - 1, R) * PP, *up = su + pmod(kq + 1, R) * PP;
+            const float *fk = sf + pmod(kq, R) * PP;
+            auto res = [&](int x, int y, const float *ua, const float *ul, const float *uh, const float *fa) {
+                const int c = (y + H) * PX + x + H;
+                return resid_value<float>(ua[c], ua[c + 1], ua[c - 1], ua[c + PX], ua[c - PX], uh[c], ul[c], fa[c], h2i);
+            };
+            if ((kq & 1) == 0) {
+                for (int q = tid; q < SX * SY; q += NT) srp[q] = res(q % SX, q / SX, u, um, up, fk);
+            } else {
+                for (int q = tid; q < (SX / 2) * (SY / 2); q += NT) {
+                    const int cx = q % (SX / 2), cy = q / (SX / 2);
+                    float r[8];
+#pragma unroll
+                    for (int o = 0; o < 8; ++o) {
+                        const int a = o & 1, b = (o >> 1) & 1, dz = (o >> 2) & 1;
+                        const int x = 2 * cx + a, y = 2 * cy + b;
+                        r[o] = dz ? res(x, y, u, um, up, fk) : srp[y * SX + x];
+                    }
+                    const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy, gcz = wrapn(kq, n) / 2;
+                    rc[(gcz * nc + gcy) * nc + gcx] = restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
+                }
+            }
+            __syncthreads();
+        }
+        if (cnew) {
+            put_coarse(fdiv2(knext) + 1);
+            cmax = fdiv2(knext) + 1;
+            __syncthreads();
+        }
+        if (more) put(knext);
+        __syncthreads();
+    }
+    if (resmax) block_atomic_max(resmax, amax);
+}
+
+template <class SS, bool ZERO, bool PROLONG, bool RESID>
+__global__ void __launch_bounds__(SS::NT)
+k_sweep_stream(const float *__restrict__ uin, float *__restrict__ uout, const float *__restrict__ f, int n, float h2,
+               float h2i, const float *__restrict__ uc, float *__restrict__ rc, double *resmax)
+{
+    const int ntiles = (n / SS::SX) * (n / SS::SY) * (n / SS::ZC);
+    const int t = xcd_tile(ntiles);
+    if (t >= ntiles) return;
+    sweep_stream<SS, ZERO, PROLONG, RESID>(uin, uout, f, n, h2, h2i, uc, rc, resmax, t);
+}
+
+// Streaming shapes the autotuners choose from beside the tiles (one sweep per launch, source always
+// staged): column SX x SY, chunk of ZC planes, NT threads.
+using Stream0 = StreamShape<64, 16, 32, 512>;
+using Stream1 = StreamShape<32, 16, 32, 256>;
+using Stream2 = StreamShape<64, 32, 32, 512>;
+constexpr int kNumStream = 3;
+static const int kStreamDims[kNumStream][4] = {{64, 16, 32, 512}, {32, 16, 32, 256}, {64, 32, 32, 512}};
+static_assert(stream_fits_lds<Stream2, true, false>() && stream_fits_lds<Stream2, false, true>(), "stream LDS");
+
+// sweep configurations: 0 .. kNumTiles-1 tiles, kNumTiles .. kNumTiles+kNumStream-1 streams
+constexpr int kNumCfg = kNumTiles + kNumStream;
+constexpr int kNumCand = kNumTiles * kMaxSweepsPerLaunch * 2 + kNumStream;
+
+inline bool cfg_fits(int cfg, int n)
+{
+    if (cfg < kNumTiles) return tile_fits(cfg, n);
+    const int *d = kStreamDims[cfg - kNumTiles];
+    return n >= d[0] && n >= d[1] && n >= d[2] && n % d[0] == 0 && n % d[1] == 0 && n % d[2] == 0;
+}
+
+// whether configuration cfg with (ns, fs) exists: streams take one sweep per launch and always
+// stage the source (reported as fs = 0)
+inline bool cfg_ok(int cfg, int ns, bool fs)
+{
+    if (cfg < kNumTiles) return cand_ok(cfg, ns, fs);
+    return cfg < kNumCfg && ns == 1 && !fs;
+}
+
+// autotuner candidate index -> (cfg, ns, fs)
+inline void cand_decode(int cand, int &cfg, int &ns, bool &fs)
+{
+    constexpr int nt = kNumTiles * kMaxSweepsPerLaunch * 2;
+    if (cand < nt) {
+        cfg = cand / (2 * kMaxSweepsPerLaunch);
+        ns = 1 + (cand / 2) % kMaxSweepsPerLaunch;
+        fs = cand % 2;
+    } else {
+        cfg = kNumTiles + (cand - nt);
+        ns = 1;
+        fs = false;
+    }
+}
+
 // FMG tricubic prolongation uf = P3 uc (interpolation.F90:98-217) on an LDS tile: the coarse cells
 // any stencil of the fine tile can touch are staged, then the x, y and z passes run on LDS exactly
 // as the reference's bufx / bufxy sweeps.  Fine tile FX*FY*FZ, nf >= FX, nf % FY == nf % FZ == 0.

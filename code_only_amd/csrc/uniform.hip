@@ -70,7 +70,7 @@ struct sg_uniform {
     int tile_cfg[kMaxLev] = {}; // autotuned sweep tile shape per fine level
     int tile_ns[kMaxLev] = {};  // autotuned sweeps per launch per fine level
     int tile_fs[kMaxLev] = {};  // autotuned: source staged in LDS
-    float tune_ms[kMaxLev][20] = {};
+    float tune_ms[kMaxLev][32] = {};
     double *psi_dev = nullptr, *src_dev = nullptr;   // staging for the host-pointer entry point
     double *adj_src = nullptr, *adj_part = nullptr;  // adjoint: projected source, mean partials
     double *psi_alt = nullptr;                         // second phi buffer of the fused outer step
@@ -263,15 +263,36 @@ static void sweep_ns(hipStream_t s, int ns, bool fs, const float *uin, float *uo
 
 // ns (1 or 2) fused red-black sweeps of a fine level (n >= 32) with tile shape cfg, the source
 // staged in LDS when fs: uin -> uout.
+template <class SS>
+static void stream_variant(hipStream_t s, const float *uin, float *uout, const float *f, int n, float h2, float h2i,
+                           bool zero, bool prolong, bool resid, const float *uc, float *rc, double *resmax)
+{
+    const int ntiles = (n / SS::SX) * (n / SS::SY) * (n / SS::ZC);
+    const dim3 grid(8 * ((ntiles + 7) / 8)), block(SS::NT);
+    if (resid && zero)
+        hipLaunchKernelGGL((k_sweep_stream<SS, true, false, true>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else if (resid)
+        hipLaunchKernelGGL((k_sweep_stream<SS, false, false, true>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else if (prolong)
+        hipLaunchKernelGGL((k_sweep_stream<SS, false, true, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else if (zero)
+        hipLaunchKernelGGL((k_sweep_stream<SS, true, false, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+    else
+        hipLaunchKernelGGL((k_sweep_stream<SS, false, false, false>), grid, block, 0, s, uin, uout, f, n, h2, h2i, uc, rc, resmax);
+}
+
 static int launch_sweep_cfg(hipStream_t s, int cfg, int ns, bool fs, const float *uin, float *uout, const float *f,
                             int n, double h, bool zero, bool prolong, bool resid, const float *uc, float *rc,
                             double *resmax)
 {
-    SG_REQUIRE(cfg >= 0 && cfg < kNumTiles && tile_fits(cfg, n), "tile shape %d does not fit n = %d", cfg, n);
+    SG_REQUIRE(cfg >= 0 && cfg < kNumCfg && cfg_fits(cfg, n), "sweep shape %d does not fit n = %d", cfg, n);
     SG_REQUIRE(ns >= 1 && ns <= kMaxSweepsPerLaunch, "bad sweeps per launch %d", ns);
-    SG_REQUIRE(cand_ok(cfg, ns, fs), "tile shape %d with %d sweeps (staged %d) exceeds LDS", cfg, ns, (int)fs);
+    SG_REQUIRE(cfg_ok(cfg, ns, fs), "sweep shape %d with %d sweeps (staged %d) does not exist", cfg, ns, (int)fs);
     const float h2 = (float)(h * h), h2i = (float)(1.0 / (h * h));
     switch (cfg) {
+    case kNumTiles + 0: stream_variant<Stream0>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 1: stream_variant<Stream1>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 2: stream_variant<Stream2>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 0: sweep_ns<Tile0>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 1: sweep_ns<Tile1>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 2: sweep_ns<Tile2>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
@@ -326,10 +347,12 @@ static int autotune_tiles(sg_uniform *c)
         c->tile_ns[l] = 1;
         if (n <= kCoarseMaxN) continue;
         float best = 1e30f;
-        for (int cand = 0; cand < kNumTiles * kMaxSweepsPerLaunch * 2 && rc == SG_OK; ++cand) {
-            const int cfg = cand / (2 * kMaxSweepsPerLaunch), ns = 1 + (cand / 2) % kMaxSweepsPerLaunch;
-            const bool fs = cand % 2;
-            if (!tile_fits(cfg, n) || !cand_ok(cfg, ns, fs)) continue;
+        for (int cand = 0; cand < kNumCand && rc == SG_OK; ++cand) {
+            int cfg, ns;
+            bool fs;
+            cand_decode(cand, cfg, ns, fs);
+            c->tune_ms[l][cand] = -1.f;
+            if (!cfg_fits(cfg, n) || !cfg_ok(cfg, ns, fs)) continue;
             {
                 const int launches = 2 / ns;             // two sweeps
                 const int reps = n >= 256 ? 3 : 6;       // timed repetitions of two sweeps
@@ -912,13 +935,27 @@ int sg_uniform_time_smoother(sg_uniform *c, int prec, int reps, double *ms_per_s
     return SG_OK;
 }
 
+int sg_uniform_set_tile(sg_uniform *c, int level, int cfg, int ns, int fs)
+{
+    SG_REQUIRE(c && (c->precisions & SG_PREC_MIXED), "context lacks the mixed-precision arrays");
+    SG_REQUIRE(level >= 0 && level < c->nlev && c->nl[level] > sg::kCoarseMaxN,
+               "level %d is not a fused-sweep level", level);
+    SG_REQUIRE(cfg >= 0 && cfg < sg::kNumCfg && sg::cfg_fits(cfg, c->nl[level]) && sg::cfg_ok(cfg, ns, fs != 0),
+               "sweep configuration (%d, %d, %d) does not fit level %d (n = %d)", cfg, ns, fs, level, c->nl[level]);
+    c->tile_cfg[level] = cfg;
+    c->tile_ns[level] = ns;
+    c->tile_fs[level] = fs != 0;
+    sg::drop_graphs(c);
+    return SG_OK;
+}
+
 int sg_uniform_tile_info(sg_uniform *c, int level, int *cfg, int *ns, int *fs, float *ms, int nms)
 {
     SG_REQUIRE(c && cfg && ns && fs && level >= 0 && level < c->nlev, "bad argument");
     *cfg = c->tile_cfg[level];
     *ns = c->tile_ns[level];
     *fs = c->tile_fs[level];
-    for (int q = 0; ms && q < nms && q < sg::kNumTiles * sg::kMaxSweepsPerLaunch * 2; ++q) ms[q] = c->tune_ms[level][q];
+    for (int q = 0; ms && q < nms && q < sg::kNumCand; ++q) ms[q] = c->tune_ms[level][q];
     return SG_OK;
 }
 

@@ -103,6 +103,18 @@ class UniformSolver:
                                                                      _ptr(out)))
         return out
 
+    def tile_info(self, level):
+        """Autotuned sweep configuration of fused level ``level`` and all candidate timings."""
+        cfg, ns, fs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_float * 32)()
+        _lib.check(self._lib.sg_uniform_tile_info(self._ctx, level, ctypes.byref(cfg), ctypes.byref(ns),
+                                                  ctypes.byref(fs), ms, 32))
+        return dict(cfg=cfg.value, ns=ns.value, fs=fs.value, ms=list(ms))
+
+    def set_tile(self, level, cfg, ns=1, fs=0):
+        """Force the sweep configuration of fused level ``level`` (values do not change)."""
+        _lib.check(self._lib.sg_uniform_set_tile(self._ctx, level, cfg, ns, fs))
+
     # single operators (parity tests)
     def relax(self, u, rhs, h, prec=SG_PREC_FP64):
         r = ctypes.c_double()

@@ -207,6 +207,40 @@ def test_fused_mixed_equals_unfused_bitwise(gpu, n, use_graph):
     s.close()
 
 
+STREAM_CFGS = [5, 6, 7]   # streaming sweep shapes (64x16, 32x16, 64x32 columns, 32-plane chunks)
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+@pytest.mark.parametrize("cfg", STREAM_CFGS + [0, 4])
+def test_every_sweep_shape_bitwise(gpu, n, cfg):
+    """Each sweep configuration, forced on every fused level where it fits, reproduces the unfused
+    FP32 solve bit for bit (its plain, zero-start, prolongation and residual variants all run)."""
+    f = problems.clumpy_rhs(n, seed=13)
+    s = UniformSolver(n, precisions=SG_PREC_MIXED)
+    ft = T(f, gpu)
+    forced = 0
+    lev, m = 0, n
+    while m > 16:
+        try:
+            s.set_tile(lev, cfg, 1, 0)
+            forced += 1
+        except Exception:
+            pass
+        lev += 1
+        m //= 2
+    if not forced:
+        pytest.skip(f"configuration {cfg} fits no level of n = {n}")
+    res = []
+    for fused in (False, True):
+        psi = torch.zeros_like(ft)
+        st = s.cycle(psi, ft, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, fused=fused,
+                     stop_mode=SG_STOP_REFERENCE, raise_on_fail=False)
+        res.append((psi, st))
+    assert torch.equal(res[0][0], res[1][0])
+    assert res[0][1]["resmaxg"] == res[1][1]["resmaxg"]
+    s.close()
+
+
 @pytest.mark.parametrize("npre,npost,ncycle", [(1, 1, 1), (3, 2, 2), (2, 3, 1)])
 def test_fused_other_cycle_shapes_bitwise(gpu, npre, npost, ncycle):
     n = 64
