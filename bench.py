@@ -290,9 +290,10 @@ def main():
 
     tiles = {}
     dims = [(32, 16, 16, 1024), (16, 8, 8, 1024), (16, 8, 8, 256), (64, 8, 8, 512), (8, 8, 8, 512)]
-    sdims = [(64, 16, 32, 512), (32, 16, 32, 256), (64, 32, 32, 512)]
+    sdims = [(64, 16, 32, 512, 1), (64, 16, 32, 512, 2), (64, 16, 16, 512, 2), (32, 16, 32, 256, 2),
+             (64, 32, 32, 512, 2)]
     names = ["x".join(map(str, d[:3])) + f"/{d[3]}" for d in dims] + \
-            ["stream " + "x".join(map(str, d[:2])) + f" z{d[2]}/{d[3]}" for d in sdims]
+            ["stream " + "x".join(map(str, d[:2])) + f" z{d[2]}/{d[3]}/{d[4]}pl" for d in sdims]
     lev, m = 0, n
     while m > 16:
         cfg, ns, fs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -313,13 +314,16 @@ def main():
         m //= 2
     log(f"[bench] autotuned tiles {json.dumps(tiles)}")
 
-    # HBM bytes per launch of the roofline kernel, from the committed rocprofv3 counter summary
+    # HBM bytes per launch of the roofline kernel (the autotuned shape at this size), from the
+    # committed rocprofv3 counter summary: lines "traffic_bytes_per_launch[<shape>]: <bytes>"
     traffic = None
     prof = os.path.join(ROOT, "profiles", "uniform256_hbm_traffic.md")
-    if n == 256 and tiles.get("256", {}).get("shape") == "32x16x16/1024" and os.path.exists(prof):
+    shape = tiles.get(str(n), {}).get("shape")
+    if n == 256 and shape and os.path.exists(prof):
+        key = f"traffic_bytes_per_launch[{shape}]:"
         for line in open(prof):
-            if line.startswith("traffic_bytes_per_launch:"):
-                traffic = int(line.split(":")[1])
+            if line.startswith(key):
+                traffic = int(line[len(key):])
 
     p64 = psi64 - psi64.mean()
     pmx = psimx - psimx.mean()
