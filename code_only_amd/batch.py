@@ -71,11 +71,11 @@ class BatchSolver:
 
     def tile_info(self, level):
         cfg, ns, fs, fx = (ctypes.c_int() for _ in range(4))
-        ms = (ctypes.c_float * 20)()
+        ms = (ctypes.c_float * 32)()
         _lib.check(self._lib.sg_batch_tile_info(self._ctx, level, ctypes.byref(cfg), ctypes.byref(ns),
-                                                ctypes.byref(fs), ctypes.byref(fx), ms, 20))
+                                                ctypes.byref(fs), ctypes.byref(fx), ms, 32))
         return dict(cfg=cfg.value, sweeps_per_launch=ns.value, source_in_lds=bool(fs.value),
-                    cubic_fx=fx.value, ms=[ms[i] for i in range(20)])
+                    cubic_fx=fx.value, ms=[ms[i] for i in range(32)])
 
     def levels(self, q):
         nl, lc = ctypes.c_int(), ctypes.c_int()

@@ -66,6 +66,8 @@ struct sg_amr {
     int have_f = 0;
     BaseDirect bd{};
     float *bd_mem = nullptr;
+    float *bd_grid = nullptr;   // two dense n^3 grids of the multi-workgroup form
+    int *bd_inv = nullptr;      // base block number by position
     size_t bd_lds_bytes = 0;
     int bd_ok = 0;
     hipGraphExec_t graph = nullptr;   // mixed solve: one outer iteration, captured once
@@ -167,10 +169,28 @@ template <typename T> int ghost_pass(sg_amr &s, int l, int L, int d, int code, i
     SG_NDISPATCH(L, ghost_pass_n, s, l, L, d, code, cubic, kinds);
 }
 
-template <typename T> int ghostcell_fix(sg_amr &s, int L, int code, int cubic)
+// all three directions of one level in one launch (same values as the three staged passes)
+template <int N, typename T> int ghost_level_n(sg_amr &s, int l, int L, int code, int cubic)
 {
-    for (int l = 0; l < s.nlev; ++l)
-        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, L, d, code, cubic, K_BOTH));
+    int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+    return go(s, k_ghost_level<N, T>, (long)nb * GhostCells<N>::COUNT, ar<T>(s).A[L][code], b0, nb, cubic, (const int *)s.d_nbr,
+              (const int *)s.d_nkind, (const int *)s.d_octant);
+}
+template <typename T> int ghost_level(sg_amr &s, int l, int L, int code, int cubic)
+{
+    SG_NDISPATCH(L, ghost_level_n, s, l, L, code, cubic);
+}
+
+// fmg_ghostcell_fix (fmg_ghostcell.F90:43-56): levels coarse to fine, one launch per (level,
+// direction).  `fused` fills each level in one launch instead (same values; measured slower on
+// MI355X than the three staged launches, so the solves do not use it).
+template <typename T> int ghostcell_fix(sg_amr &s, int L, int code, int cubic, bool fused = false)
+{
+    for (int l = 0; l < s.nlev; ++l) {
+        if (fused) SG_TRY(ghost_level<T>(s, l, L, code, cubic));
+        else
+            for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, L, d, code, cubic, K_BOTH));
+    }
     return SG_OK;
 }
 
@@ -387,14 +407,17 @@ template <typename T> int vmg_gridboundary(sg_amr &s, int l, int code)
     return fill0_faces<T>(s, LV, code, l, l + 1);
 }
 
-// vmg_poisson_relax (vmg_poisson.F90:88-145).  The inner (FP32) solve reads same-level neighbours
-// directly instead of refreshing the ghost layer before each colour (the values the smoother sees
-// are the same; the ghost arrays are left as the last refresh made them).
-template <typename T> int vmg_relax(sg_amr &s, int l, int ju, int jrhs)
+// vmg_poisson_relax (vmg_poisson.F90:88-145): a ghost refresh (vmg_ghostcell) before each colour.
+// The inner (FP32) solve reads same-level neighbours directly instead, so a colour needs no refresh;
+// but the ghosts the reference's last refresh leaves are read later (vmg_sub over whole arrays,
+// p2c's edge and corner ghosts, the restriction's ghost passes), so with `last` the refresh before
+// the final colour is kept: each refresh overwrites all the previous one wrote, so the ghost state
+// afterwards is the reference's, with one refresh instead of two per call.
+template <typename T> int vmg_relax(sg_amr &s, int l, int ju, int jrhs, bool last = true)
 {
     int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
     for (int ipass = 1; ipass <= 2; ++ipass) {
-        if (!kInner<T>) SG_TRY(vmg_ghostcell<T>(s, l, ju));
+        if (!kInner<T> || (last && ipass == 2)) SG_TRY(vmg_ghostcell<T>(s, l, ju));
         SG_TRY(relax_pass<T>(s, LV, ju, jrhs, nullptr, b0, nb, ipass, nullptr, nullptr, kInner<T>));
     }
     return SG_OK;
@@ -431,6 +454,21 @@ template <typename T> int vmg_interp(sg_amr &s, int l, int juf, int juc)
 // (FP32) solve: the same base problem solved exactly by separable transforms in one workgroup.
 template <typename T> int vmg_slvsml(sg_amr &s, int ju, int jrhs)
 {
+    if (kInner<T> && s.bd_ok && s.bd_grid) {
+        // gather, three forward passes (the third divided by the eigenvalue sums), three back
+        // (the last scattered into the base blocks): the values of the one-workgroup form
+        const int n = s.bd.n, n3 = n * n * n;
+        float *g0 = s.bd_grid, *g1 = s.bd_grid + n3;
+        float *Au = s.f.A[LV][ju];
+        const int *inv = s.bd_inv;
+        SG_TRY(go(s, k_bd_gather, (long)s.nbase * 8, g0, (const float *)s.f.A[LV][jrhs], s.nbase, (const int *)s.d_ijk, n));
+        SG_TRY(go(s, k_bd_pass<0, 0>, (long)n3, g1, (const float *)g0, s.bd, Au, inv));
+        SG_TRY(go(s, k_bd_pass<1, 0>, (long)n3, g0, (const float *)g1, s.bd, Au, inv));
+        SG_TRY(go(s, k_bd_pass<2, 1>, (long)n3, g1, (const float *)g0, s.bd, Au, inv));
+        SG_TRY(go(s, k_bd_pass<0, 0>, (long)n3, g0, (const float *)g1, s.bd, Au, inv));
+        SG_TRY(go(s, k_bd_pass<1, 0>, (long)n3, g1, (const float *)g0, s.bd, Au, inv));
+        return go(s, k_bd_pass<2, 2>, (long)n3, g0, (const float *)g1, s.bd, Au, inv);
+    }
     if (kInner<T> && s.bd_ok) {
         hipLaunchKernelGGL(k_base_direct, dim3(1), dim3(kDirectThreads), s.bd_lds_bytes, s.stream, s.f.A[LV][ju],
                            (const float *)s.f.A[LV][jrhs], s.nbase, (const int *)s.d_ijk, s.bd);
@@ -438,7 +476,7 @@ template <typename T> int vmg_slvsml(sg_amr &s, int ju, int jrhs)
         return SG_OK;
     }
     if (kInner<T> || !s.base_mg_ok) {
-        for (int it = 0; it < 10; ++it) SG_TRY(vmg_relax<T>(s, 0, ju, jrhs));
+        for (int it = 0; it < 10; ++it) SG_TRY(vmg_relax<T>(s, 0, ju, jrhs, it == 9));
         return SG_OK;
     }
     const int NG = s.bm.n[0];
@@ -462,7 +500,7 @@ template <typename T> int vmg_fas(sg_amr &s, int jrho)
     for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy<T>(s, l, C_RHS, C_RHO));
     for (int v = s.nlev - 1; v >= 1; --v) {
         SG_TRY(vmg_gridboundary<T>(s, v, C_U));
-        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS, p == 1));
         // vmg_rstrct(v-1, U, cubic): per direction parent-level then same-level faces (one
         // launch: the two write disjoint faces and read nothing the other writes)
         for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, v, LV, d, C_U, 1, K_BOTH));
@@ -481,7 +519,7 @@ template <typename T> int vmg_fas(sg_amr &s, int jrho)
         SG_TRY(vmg_interp<T>(s, v, C_RES, C_RES));
         SG_TRY(lev_add<T>(s, v, C_U, C_RES, false));
         SG_TRY(vmg_gridboundary<T>(s, v, C_U));
-        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS, p == 1));
     }
     return SG_OK;
 }
@@ -701,6 +739,13 @@ int ensure_mixed(sg_amr &s)
         s.bd_lds_bytes = base_direct_lds_bytes(n);
         SG_HIP(hipFuncSetAttribute((const void *)k_base_direct, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)s.bd_lds_bytes));
+        // multi-workgroup form: scratch grids and the position -> base block map
+        std::vector<int> inv((size_t)bx * bx * bx, -1);
+        for (int b = 0; b < s.nbase; ++b) inv[((size_t)s.ijk[3 * b + 2] * bx + s.ijk[3 * b + 1]) * bx + s.ijk[3 * b]] = b;
+        for (int v : inv) SG_REQUIRE(v >= 0, "base level does not cover its box");
+        SG_HIP(hipMalloc((void **)&s.bd_grid, sizeof(float) * 2 * (size_t)n * n * n));
+        SG_HIP(hipMalloc((void **)&s.bd_inv, sizeof(int) * inv.size()));
+        SG_HIP(hipMemcpy(s.bd_inv, inv.data(), sizeof(int) * inv.size(), hipMemcpyHostToDevice));
     }
     s.have_f = 1;
     return SG_OK;
@@ -818,6 +863,8 @@ void release(sg_amr *s)
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     if (s->bd_mem) (void)hipFree(s->bd_mem);
+    if (s->bd_grid) (void)hipFree(s->bd_grid);
+    if (s->bd_inv) (void)hipFree(s->bd_inv);
     for (int L = 0; L < kNfmg; ++L) {
         for (int c = 0; c < NCODE; ++c) if (s->d.A[L][c]) (void)hipFree(s->d.A[L][c]);
         if (s->d.ov[L]) (void)hipFree(s->d.ov[L]);
@@ -1035,7 +1082,7 @@ int op(sg_amr &s, int opc, int L, int a, int b, int c, int flag, double *ret)
     double r = 0.0;
     switch (opc) {
     case SG_AMR_OP_C2P: SG_TRY(c2p<double>(s, L, a, flag)); break;
-    case SG_AMR_OP_GHOSTFIX: SG_TRY(ghostcell_fix<double>(s, L, a, flag)); break;
+    case SG_AMR_OP_GHOSTFIX: SG_TRY(ghostcell_fix<double>(s, L, a, flag & 1, (flag & 2) != 0)); break;
     case SG_AMR_OP_FILL0: SG_TRY(fill0_faces<double>(s, L, a, 0, s.nlev)); break;
     case SG_AMR_OP_EXTRAP: SG_TRY(extrap_faces<double>(s, L, a, 0, s.nlev)); break;
     case SG_AMR_OP_RELAX:

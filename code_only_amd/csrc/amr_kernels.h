@@ -207,6 +207,137 @@ __global__ void k_ghost(T *A, int b0, int nb, int d, int cubic, int kinds, const
     ud[B::rot(d, ag, t1, t2)] = q / (T)15.0;
 }
 
+// ---- all three directions of fmg_ghostcell_fix for one AMR level in one launch ----
+// The staged fill (x, then y, then z; parent-level then same-level faces in each) leaves in every
+// ghost cell the value of the last pass that writes it: the highest direction among the cell's
+// ghost coordinates whose face is not a physical boundary.  S<D>(b, cell) is the cell's value after
+// the passes of directions <= D of this level; a pass-D value needs only S<D-1> values (a
+// same-level copy reads the neighbour's layer, a parent-level interpolation reads the coarser
+// level, complete, and the block's own cells at the two layers next to the face).  The recursion
+// reads from memory only interior cells and ghost cells whose every ghost face is a physical
+// boundary -- cells no pass writes, since physical faces of neighbouring blocks lie on the same
+// domain face -- so each thread computes its cell's final value without reading a value another
+// thread writes.  Same values as the three staged launches, bit for bit; on MI355X slower than
+// them (the recursion's dependent loads and divergence cost more than the two launches it saves),
+// so the solves keep the staged launches and this form is reached through the operator interface.
+template <int N, typename T> struct GhostLevel {
+    using B = Blk<N>;
+    static constexpr int H = N / 2;
+    T *A;
+    const int *nbr, *nkind, *octant;
+    int cubic;
+
+    template <int D> __device__ T S(int b, int i, int j, int k) const
+    {
+        if constexpr (D < 0) {
+            return A[(long)b * B::LEN + B::gi(i, j, k)];
+        } else {
+            const int c = D == 0 ? i : (D == 1 ? j : k);
+            if (c == -1 || c == N) {
+                const int lr = c == N ? 1 : 0;
+                const int f = b * 6 + 2 * D + lr;
+                const int kind = nkind[f];
+                if (kind == NBR_SAME) {
+                    const int m = nbr[f], layer = lr ? 0 : N - 1;
+                    if (D == 0) return S<D - 1>(m, layer, j, k);
+                    if (D == 1) return S<D - 1>(m, i, layer, k);
+                    return S<D - 1>(m, i, j, layer);
+                }
+                if (kind == NBR_PARENT) return P<D>(b, lr, i, j, k);
+            }
+            return S<D - 1>(b, i, j, k);
+        }
+    }
+
+    // fmg_ghfix_parentlev value of ghost cell (i, j, k) on face (D, lr) of block b (the formulas
+    // of k_ghost's parent branch)
+    template <int D> __device__ T P(int b, int lr, int i, int j, int k) const
+    {
+        constexpr int ta = D == 0 ? 1 : 0, tb = D == 2 ? 1 : 2;
+        const int t1 = D == 0 ? j : i, t2 = D == 2 ? j : k;
+        const int o = octant[b];
+        const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
+        const int an = lr ? 0 : N - 1;
+        const T *us = A + (long)nbr[b * 6 + 2 * D + lr] * B::LEN;
+        auto buf = [&](int c1, int c2) { return us[B::rot(D, an, c1 + e1, c2 + e2)]; };
+        T uH;
+        if (cubic && H + 2 >= 4) {
+            int i1[4], i2[4];
+            double a1[4], a2[4];
+            cubic_stencil(t1, -1, H, i1, a1);
+            cubic_stencil(t2, -1, H, i2, a2);
+            T bt[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                bt[q] = comb4(a1, buf(i1[0], i2[q]), buf(i1[1], i2[q]), buf(i1[2], i2[q]), buf(i1[3], i2[q]));
+            uH = comb4(a2, bt[0], bt[1], bt[2], bt[3]);
+        } else {
+            int c1 = ijkc(t1), c2 = ijkc(t2);
+            int c1n = near_(t1, c1), c2n = near_(t2, c2);
+            T Sv = (T)9.0 * buf(c1, c2) + buf(c1n, c2n);
+            Sv = Sv + (T)3.0 * buf(c1, c2n);
+            Sv = Sv + (T)3.0 * buf(c1n, c2);
+            uH = Sv / (T)16.0;
+        }
+        const int ag = lr ? N : -1, inner = lr ? -1 : 1;
+        auto own = [&](int a) {
+            if (D == 0) return S<D - 1>(b, a, t1, t2);
+            if (D == 1) return S<D - 1>(b, t1, a, t2);
+            return S<D - 1>(b, t1, t2, a);
+        };
+        const T u0 = own(ag + inner), uN = own(ag + 2 * inner);
+        T q = (T)10.0 * u0 + (T)8.0 * uH;
+        q = q - (T)3.0 * uN;
+        return q / (T)15.0;
+    }
+};
+
+// ghost cells of a block, each once: the two x slabs whole, the y slabs without x ghosts, the z
+// slabs without x and y ghosts
+template <int N> struct GhostCells {
+    static constexpr int W = N + 2, NX = 2 * W * W, NY = 2 * N * W, NZ = 2 * N * N, COUNT = NX + NY + NZ;
+    __device__ static void at(int g, int &i, int &j, int &k)
+    {
+        if (g < NX) {
+            const int r = g % (W * W);
+            i = g < W * W ? -1 : N;
+            j = r % W - 1;
+            k = r / W - 1;
+        } else if (g < NX + NY) {
+            const int h = g - NX, r = h % (N * W);
+            j = h < N * W ? -1 : N;
+            i = r % N;
+            k = r / N - 1;
+        } else {
+            const int h = g - NX - NY, r = h % (N * N);
+            k = h < N * N ? -1 : N;
+            i = r % N;
+            j = r / N;
+        }
+    }
+};
+
+template <int N, typename T>
+__global__ void k_ghost_level(T *A, int b0, int nb, int cubic, const int *nbr, const int *nkind, const int *octant)
+{
+    using B = Blk<N>;
+    using GC = GhostCells<N>;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * GC::COUNT) return;
+    const int b = b0 + (int)(t / GC::COUNT);
+    int i, j, k;
+    GC::at((int)(t % GC::COUNT), i, j, k);
+    const int c = B::gi(i, j, k);
+    const int gc[3] = {i, j, k};
+    bool written = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (gc[d] == -1 || gc[d] == N) written = written || nkind[b * 6 + 2 * d + (gc[d] == N ? 1 : 0)] != NBR_NONE;
+    if (!written) return;
+    const GhostLevel<N, T> G{A, nbr, nkind, octant, cubic};
+    A[(long)b * B::LEN + c] = G.template S<2>(b, i, j, k);
+}
+
 // ---- fmg_boundary_fill0 on a list of physical faces (entries b*6+f) ----
 template <int N, typename T> __global__ void k_fill0_faces(T *A, const int *faces, int nf)
 {

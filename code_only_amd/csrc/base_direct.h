@@ -84,5 +84,49 @@ __global__ void __launch_bounds__(kDirectThreads) k_base_direct(float *Au, const
     }
 }
 
+// ---- the same solve spread over the chip: gather, six axis passes (one thread per output value,
+// each accumulated k = 0 .. n-1 in order as above, so the values are those of k_base_direct), the
+// division by the eigenvalue sum folded into the third pass and the scatter into the sixth ----
+__global__ void k_bd_gather(float *g, const float *Arhs, int nbase, const int *ijk, int n)
+{
+    using B = Blk<2>;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbase * 8) return;
+    const int b = t >> 3, c = t & 7, i = c & 1, j = (c >> 1) & 1, k = c >> 2;
+    const int gi = ijk[3 * b] * 2 + i, gj = ijk[3 * b + 1] * 2 + j, gk = ijk[3 * b + 2] * 2 + k;
+    g[(gk * n + gj) * n + gi] = Arhs[(long)b * B::LEN + B::gi(i, j, k)];
+}
+
+// out[line, m] = sum_k Q[m][k] in[line, k] along axis AXIS of the dense n^3 grid (k, j, i order).
+// MODE 1: divided by lam_i + lam_j + lam_k (0 where that is 0); MODE 2: written to the base blocks.
+template <int AXIS, int MODE>
+__global__ void k_bd_pass(float *out, const float *in, BaseDirect D, float *Au, const int *inv)
+{
+    using B = Blk<2>;
+    const int n = D.n;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n * n) return;
+    const int i = t % n, j = (t / n) % n, k = t / (n * n);
+    const int m = AXIS == 0 ? i : (AXIS == 1 ? j : k);
+    const int stride = AXIS == 0 ? 1 : (AXIS == 1 ? n : n * n);
+    const int base = t - m * stride;
+    const float *q = D.Q + m * n;
+    float acc = 0.f;
+    for (int kk = 0; kk < n; ++kk) acc += q[kk] * in[base + kk * stride];
+    if (MODE == 1) {
+        const float den = D.lam[i] + D.lam[j] + D.lam[k];
+        acc = den != 0.f ? acc / den : 0.f;
+    }
+    if (MODE == 2) {
+        // base block of cell (i, j, k): blocks are 2^3 cells; inv maps block position
+        // (i/2, j/2, k/2) to its block number
+        const int nb = n / 2;
+        const int b = inv[((k >> 1) * nb + (j >> 1)) * nb + (i >> 1)];
+        Au[(long)b * B::LEN + B::gi(i & 1, j & 1, k & 1)] = acc;
+    } else {
+        out[t] = acc;
+    }
+}
+
 } // namespace amr
 } // namespace sg

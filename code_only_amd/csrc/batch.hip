@@ -66,7 +66,7 @@ struct sg_batch {
     double *d_norm = nullptr, *h_norm = nullptr;    // [2p] max|r|, [2p+1] sum r^2
     double *d_snorm = nullptr, *h_snorm = nullptr;  // same for the source
     int tile_cfg[kBLev] = {}, tile_ns[kBLev] = {}, tile_fs[kBLev] = {}, cubic_fx[kBLev] = {};
-    float tune_ms[kBLev][20] = {};
+    float tune_ms[kBLev][32] = {};
     bool tuned = false;
     Plan plan[2];  // [0] FP64, [1] mixed
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -313,18 +313,34 @@ void sweep_ns(sg_batch *c, int ns, bool fs, const BL<float> &L, int tot, int cin
     }
 }
 
+template <class SS>
+void stream_variant(sg_batch *c, const BL<float> &L, int tot, int cin, int cout, int cuc, bool zero, bool prolong,
+                    bool resid)
+{
+    const dim3 grid(8 * ((tot + 7) / 8)), block(SS::NT);
+    if (resid && zero)
+        hipLaunchKernelGGL((kb_stream<SS, true, false, true>), grid, block, 0, c->stream, L, tot, cin, cout, cuc);
+    else if (resid)
+        hipLaunchKernelGGL((kb_stream<SS, false, false, true>), grid, block, 0, c->stream, L, tot, cin, cout, cuc);
+    else if (prolong)
+        hipLaunchKernelGGL((kb_stream<SS, false, true, false>), grid, block, 0, c->stream, L, tot, cin, cout, cuc);
+    else if (zero)
+        hipLaunchKernelGGL((kb_stream<SS, true, false, false>), grid, block, 0, c->stream, L, tot, cin, cout, cuc);
+    else
+        hipLaunchKernelGGL((kb_stream<SS, false, false, false>), grid, block, 0, c->stream, L, tot, cin, cout, cuc);
+}
+
 // ns fused sweeps of level l of the listed problems with tile shape cfg: buffer cin -> cout;
 // zero: start from u = 0; prolong: add the trilinear correction from level l + 1's buffer cuc;
 // resid: restrict the residual into level l + 1's RHS.
 int op_sweep_cfg(sg_batch *c, const Pairs &pl, int cfg, int ns, bool fs, int cin, int cout, int cuc, bool zero,
                  bool prolong, bool resid)
 {
-    SG_REQUIRE(cfg >= 0 && cfg < kNumTiles && cand_ok(cfg, ns, fs), "bad sweep configuration %d/%d/%d", cfg, ns,
-               (int)fs);
-    const int *td = kTileDims[cfg];
+    SG_REQUIRE(cfg >= 0 && cfg < kNumCfg && cfg_ok(cfg, ns, fs), "bad sweep configuration %d/%d/%d", cfg, ns, (int)fs);
+    const int *td = cfg < kNumTiles ? kTileDims[cfg] : kStreamDims[cfg - kNumTiles];
     for (auto &q : pl) {
         SG_REQUIRE(q.second < c->lc[q.first], "batch schedule: fused sweep on a coarse-kernel level");
-        SG_REQUIRE(tile_fits(cfg, c->h_pl32[q.first * kBLev + q.second].n), "tile %d does not fit n = %d", cfg,
+        SG_REQUIRE(cfg_fits(cfg, c->h_pl32[q.first * kBLev + q.second].n), "sweep shape %d does not fit n = %d", cfg,
                    c->h_pl32[q.first * kBLev + q.second].n);
         SG_REQUIRE(c->h_pl32[q.first * kBLev + q.second].n > 2 * (2 * ns + 1), "tile halo wider than the grid");
     }
@@ -337,6 +353,11 @@ int op_sweep_cfg(sg_batch *c, const Pairs &pl, int cfg, int ns, bool fs, int cin
     if (!cnt) return SG_OK;
     const BL<float> L = bl<float>(c, d, cnt);
     switch (cfg) {
+    case kNumTiles + 0: stream_variant<Stream0>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 1: stream_variant<Stream1>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 2: stream_variant<Stream2>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 3: stream_variant<Stream3>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 4: stream_variant<Stream4>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case 0: sweep_ns<Tile0>(c, ns, fs, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case 1: sweep_ns<Tile1>(c, ns, fs, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case 2: sweep_ns<Tile2>(c, ns, fs, L, tot, cin, cout, cuc, zero, prolong, resid); break;
@@ -572,11 +593,12 @@ int autotune(sg_batch *c)
         if (c->cubic_fx[v] == 0) { set_error("no tricubic tile fits level %d", v); rc = SG_ERR_ARG; break; }
         c->tile_cfg[v] = -1;
         float best = 1e30f;
-        for (int cand = 0; cand < kNumTiles * kMaxSweepsPerLaunch * 2 && rc == SG_OK; ++cand) {
-            const int cfg = cand / (2 * kMaxSweepsPerLaunch), ns = 1 + (cand / 2) % kMaxSweepsPerLaunch;
-            const bool fs = cand % 2;
-            bool fits = cand_ok(cfg, ns, fs);
-            for (auto &q : Mv) fits = fits && tile_fits(cfg, c->h_pl32[q.first * kBLev + v].n);
+        for (int cand = 0; cand < kNumCand && rc == SG_OK; ++cand) {
+            int cfg, ns;
+            bool fs;
+            cand_decode(cand, cfg, ns, fs);
+            bool fits = cfg_ok(cfg, ns, fs);
+            for (auto &q : Mv) fits = fits && cfg_fits(cfg, c->h_pl32[q.first * kBLev + v].n);
             c->tune_ms[v][cand] = -1.f;
             if (!fits) continue;
             // record this launch's item list into a scratch plan, upload, time it
@@ -938,7 +960,7 @@ int sg_batch_tile_info(sg_batch *c, int level, int *cfg, int *ns, int *fs, int *
     if (fs) *fs = c->tile_fs[level];
     if (cubic_fx) *cubic_fx = c->cubic_fx[level];
     if (ms)
-        for (int q = 0; q < nms && q < 20; ++q) ms[q] = c->tune_ms[level][q];
+        for (int q = 0; q < nms && q < kNumCand; ++q) ms[q] = c->tune_ms[level][q];
     return SG_OK;
 }
 

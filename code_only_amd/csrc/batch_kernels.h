@@ -365,6 +365,19 @@ __global__ void __launch_bounds__(TL::NT) kb_sweep(BL<float> L, int total, int c
                                                  RESID ? (&q + 1)->a[CRHS] : nullptr, nullptr, c.lb);
 }
 
+// The streaming form of the same sweep (one launch = one sweep).
+template <class SS, bool ZERO, bool PROLONG, bool RESID>
+__global__ void __launch_bounds__(SS::NT) kb_stream(BL<float> L, int total, int cin, int cout, int cuc)
+{
+    Cell<float> c;
+    if (!locate(L, xcd_block(total), c)) return;
+    const PL<float> &q = *c.q;
+    const int l = c.it->l;
+    sweep_stream<SS, ZERO, PROLONG, RESID>(q.a[cin], q.a[cout], q.a[CRHS], q.n, q.h2, q.h2i,
+                                           PROLONG ? coarse_src(q, l, cuc) : nullptr,
+                                           RESID ? (&q + 1)->a[CRHS] : nullptr, nullptr, c.lb);
+}
+
 // FMG tricubic prolongation into U of level l from level l + 1 (tile body of the single-grid
 // solver).
 template <int FX, int FY, int FZ, int NT>

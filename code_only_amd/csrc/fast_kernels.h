@@ -383,18 +383,22 @@ k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const floa
 // restricted pairwise (restriction.F90:132-147 operand order).  Every cell update sees the same
 // operands as the two-launch colour passes, so the values are those of the tile kernel, bit for bit.
 // ------------------------------------------------------------------------------------------------
-template <int SX_, int SY_, int ZC_, int NT_> struct StreamShape {
-    static constexpr int SX = SX_, SY = SY_, ZC = ZC_, NT = NT_;
+template <int SX_, int SY_, int ZC_, int NT_, int NP_> struct StreamShape {
+    static constexpr int SX = SX_, SY = SY_, ZC = ZC_, NT = NT_, NP = NP_;  // NP planes per step
 };
 
 template <class SS, bool PROLONG, bool RESID> struct StreamGeom {
+    static constexpr int NP = SS::NP;
     static constexpr int H = RESID ? 3 : 2;
     static constexpr int PX = SS::SX + 2 * H, PY = SS::SY + 2 * H, PP = PX * PY;
-    static constexpr int R = RESID ? 5 : 4;                     // ring slots
+    // ring slots: the step's planes, the ones it reads below and the ones it prefetches
+    static constexpr int R = 2 * NP + (RESID ? 3 : 2);
     static constexpr int CX = PX / 2 + 2, CY = PY / 2 + 2, CP = CX * CY;  // staged coarse plane
     static constexpr int IL = (PP + SS::NT - 1) / SS::NT;      // plane cells per thread
     static constexpr int ICL = (CP + SS::NT - 1) / SS::NT;
-    static constexpr int lds_bytes = 4 * (2 * R * PP + (PROLONG ? 4 * CP : 0) + (RESID ? SS::SX * SS::SY : 0));
+    static constexpr bool PAIRS = RESID && NP == 2;            // each step restricts a whole pair
+    static constexpr int lds_bytes =
+        4 * (2 * R * PP + (PROLONG ? 4 * CP : 0) + (RESID && !PAIRS ? SS::SX * SS::SY : 0));
 };
 
 template <class SS, bool PROLONG, bool RESID> constexpr bool stream_fits_lds()
@@ -411,13 +415,13 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
                                              const float *__restrict__ uc, float *__restrict__ rc, double *resmax, int t)
 {
     using G = StreamGeom<SS, PROLONG, RESID>;
-    constexpr int SX = SS::SX, SY = SS::SY, ZC = SS::ZC, NT = SS::NT;
+    constexpr int SX = SS::SX, SY = SS::SY, ZC = SS::ZC, NT = SS::NT, NP = SS::NP;
     constexpr int H = G::H, PX = G::PX, PY = G::PY, PP = G::PP, R = G::R, CX = G::CX, CP = G::CP;
-    static_assert(SX % 2 == 0 && SY % 2 == 0 && ZC % 2 == 0, "even extents");
+    static_assert(SX % 2 == 0 && SY % 2 == 0 && ZC % 2 == 0 && (NP == 1 || NP == 2), "stream shape");
     __shared__ float su[R * PP];
     __shared__ float sf[R * PP];
     __shared__ float scz[PROLONG ? 4 * CP : 1];
-    __shared__ float srp[RESID ? SX * SY : 1];
+    __shared__ float srp[RESID && !G::PAIRS ? SX * SY : 1];
     const int ntx = n / SX, nty = n / SY;
     const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
     const int x0 = tx * SX, y0 = ty * SY, z0 = tz * ZC;
@@ -425,43 +429,41 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
     const int tid = threadIdx.x;
     double amax = 0.0;
 
-    float ru[G::IL], rf[G::IL], rcz[PROLONG ? G::ICL : 1];
-    auto gaddr = [&](int q, int k) {
-        const int x = q % PX, y = q / PX;
-        return (wrapn(k, n) * n + wrapn(y0 - H + y, n)) * n + wrapn(x0 - H + x, n);
-    };
-    auto fetch = [&](int k) {
+    float ru[NP][G::IL], rf[NP][G::IL], rcz[PROLONG ? 2 : 1][PROLONG ? G::ICL : 1];
+    auto fetch = [&](int k, int slot) {
 #pragma unroll
         for (int it = 0; it < G::IL; ++it) {
             const int q = tid + it * NT;
             if (PP % NT == 0 || q < PP) {
-                const int g = gaddr(q, k);
-                ru[it] = ZERO ? 0.0f : uin[g];
-                rf[it] = f[g];
+                const int x = q % PX, y = q / PX;
+                const int g = (wrapn(k, n) * n + wrapn(y0 - H + y, n)) * n + wrapn(x0 - H + x, n);
+                ru[slot][it] = ZERO ? 0.0f : uin[g];
+                rf[slot][it] = f[g];
             }
         }
     };
     // coarse plane c of the staged region: coarse x from (x0 - H)/2 - 1, y from (y0 - H)/2 - 1
-    auto fetch_coarse = [&](int c) {
+    auto fetch_coarse = [&](int c, int slot) {
 #pragma unroll
         for (int it = 0; it < G::ICL; ++it) {
             const int q = tid + it * NT;
             if (CP % NT == 0 || q < CP) {
                 const int x = q % CX, y = q / CX;
-                rcz[it] = uc[(wrapn(c, nc) * nc + wrapn(fdiv2(y0 - H) - 1 + y, nc)) * nc + wrapn(fdiv2(x0 - H) - 1 + x, nc)];
+                rcz[slot][it] =
+                    uc[(wrapn(c, nc) * nc + wrapn(fdiv2(y0 - H) - 1 + y, nc)) * nc + wrapn(fdiv2(x0 - H) - 1 + x, nc)];
             }
         }
     };
-    auto put_coarse = [&](int c) {
+    auto put_coarse = [&](int c, int slot) {
         float *dst = scz + pmod(c, 4) * CP;
 #pragma unroll
         for (int it = 0; it < G::ICL; ++it) {
             const int q = tid + it * NT;
-            if (CP % NT == 0 || q < CP) dst[q] = rcz[it];
+            if (CP % NT == 0 || q < CP) dst[q] = rcz[slot][it];
         }
     };
     // write the fetched plane k into its ring slot (adding the trilinear correction with PROLONG)
-    auto put = [&](int k) {
+    auto put = [&](int k, int slot) {
         float *du = su + pmod(k, R) * PP, *df = sf + pmod(k, R) * PP;
         const int kc = fdiv2(k), kc1 = kc + 2 * (k & 1) - 1;
         const float *c0 = scz + pmod(kc, 4) * CP, *c1 = scz + pmod(kc1, 4) * CP;
@@ -469,7 +471,7 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
         for (int it = 0; it < G::IL; ++it) {
             const int q = tid + it * NT;
             if (PP % NT == 0 || q < PP) {
-                float v = ru[it];
+                float v = ru[slot][it];
                 if (PROLONG) {
                     const int x = q % PX, y = q / PX;
                     const int ic = (x >> 1) + 1, jc = (y >> 1) + 1;
@@ -479,7 +481,7 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
                                                    c1[jc1 * CX + ic1]);
                 }
                 du[q] = v;
-                df[q] = rf[it];
+                df[q] = rf[slot][it];
             }
         }
     };
@@ -500,74 +502,113 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
             if (out && x >= H && x < H + SX && y >= H && y < H + SY) amax = fmax(amax, fabs((double)sres));
         }
     };
+    auto resid_at = [&](int kq, int x, int y) {
+        const float *u = su + pmod(kq, R) * PP, *um = su + pmod(kq - 1, R) * PP, *up = su + pmod(kq + 1, R) * PP;
+        const float *fk = sf + pmod(kq, R) * PP;
+        const int c = (y + H) * PX + x + H;
+        return resid_value<float>(u[c], u[c + 1], u[c - 1], u[c + PX], u[c - PX], up[c], um[c], fk[c], h2i);
+    };
+    // coarse cell (cx, cy) of the pair (kq - 1, kq), kq odd: 4 residuals of the even plane from
+    // `even` (or recomputed), 4 of the odd plane recomputed
+    auto restrict_pair = [&](int kq, bool from_srp) {
+        for (int q = tid; q < (SX / 2) * (SY / 2); q += NT) {
+            const int cx = q % (SX / 2), cy = q / (SX / 2);
+            float r[8];
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                const int a = o & 1, b = (o >> 1) & 1, dz = (o >> 2) & 1;
+                const int x = 2 * cx + a, y = 2 * cy + b;
+                r[o] = dz ? resid_at(kq, x, y) : (from_srp ? srp[y * SX + x] : resid_at(kq - 1, x, y));
+            }
+            const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy, gcz = wrapn(kq, n) / 2;
+            rc[(gcz * nc + gcy) * nc + gcx] = restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
+        }
+    };
     const int zlo = z0 - H, zhi = z0 + ZC + H - 1;  // planes held at some point
-    // prologue: planes zlo .. zlo + 2 (and the coarse planes they need)
+    // prologue: planes zlo .. zlo + NP + 1 (and the coarse planes they need)
+    int cmax = fdiv2(zlo + NP + 1) + 1;  // highest coarse plane staged
     if (PROLONG) {
-        for (int c = fdiv2(zlo) - 1; c <= fdiv2(zlo + 2) + 1; ++c) {
-            fetch_coarse(c);
-            put_coarse(c);
+        for (int c = fdiv2(zlo) - 1; c <= cmax; ++c) {
+            fetch_coarse(c, 0);
+            put_coarse(c, 0);
         }
         __syncthreads();
     }
-    for (int k = zlo; k <= zlo + 2; ++k) {
-        fetch(k);
-        put(k);
+    for (int k = zlo; k <= zlo + NP + 1; ++k) {
+        fetch(k, 0);
+        put(k, 0);
     }
     __syncthreads();
-    int cmax = fdiv2(zlo + 2) + 1;  // highest coarse plane staged
-    for (int p = zlo + 1; p <= zhi - 1; ++p) {
-        const int knext = p + 2;
-        const bool more = knext <= zhi;
-        const bool cnew = PROLONG && more && fdiv2(knext) + 1 > cmax;
-        if (more) fetch(knext);
-        if (cnew) fetch_coarse(fdiv2(knext) + 1);
-        // first colour of plane p (margin 1)
-        colour(p, 1, 1, p >= z0 && p < z0 + ZC);
-        __syncthreads();
-        // second colour of plane p - 1 (margin 2): final
-        // (planes zlo + 2 .. zhi - 2: the core planes, and with RESID one more on each side)
-        if (p - 1 >= zlo + 2) colour(p - 1, 2, 0, p - 1 >= z0 && p - 1 < z0 + ZC);
-        __syncthreads();
-        if (p - 1 >= z0 && p - 1 < z0 + ZC) {
-            const float *u = su + pmod(p - 1, R) * PP;
-            const int gz = wrapn(p - 1, n);
-            for (int q = tid; q < SX * SY; q += NT) {
-                const int x = q % SX, y = q / SX;
-                uout[(gz * n + y0 + y) * n + x0 + x] = u[(y + H) * PX + x + H];
-            }
-        }
-        if (RESID && p - 2 >= z0 && p - 2 < z0 + ZC) {
-            const int kq = p - 2;
-            const float *u = su + pmod(kq, R) * PP, *um = su + pmod(kq - 1, R) * PP, *up = su + pmod(kq + 1, R) * PP;
-            const float *fk = sf + pmod(kq, R) * PP;
-            auto res = [&](int x, int y, const float *ua, const float *ul, const float *uh, const float *fa) {
-                const int c = (y + H) * PX + x + H;
-                return resid_value<float>(ua[c], ua[c + 1], ua[c - 1], ua[c + PX], ua[c - PX], uh[c], ul[c], fa[c], h2i);
-            };
-            if ((kq & 1) == 0) {
-                for (int q = tid; q < SX * SY; q += NT) srp[q] = res(q % SX, q / SX, u, um, up, fk);
-            } else {
-                for (int q = tid; q < (SX / 2) * (SY / 2); q += NT) {
-                    const int cx = q % (SX / 2), cy = q / (SX / 2);
-                    float r[8];
+    for (int pa = zlo + 1; pa <= zhi - 1; pa += NP) {
+        // prefetch planes pa + NP + 1 .. pa + 2 NP (and new coarse planes) into registers
+        const int kn0 = pa + NP + 1;
+        int nnew = 0;
 #pragma unroll
-                    for (int o = 0; o < 8; ++o) {
-                        const int a = o & 1, b = (o >> 1) & 1, dz = (o >> 2) & 1;
-                        const int x = 2 * cx + a, y = 2 * cy + b;
-                        r[o] = dz ? res(x, y, u, um, up, fk) : srp[y * SX + x];
-                    }
-                    const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy, gcz = wrapn(kq, n) / 2;
-                    rc[(gcz * nc + gcy) * nc + gcx] = restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
+        for (int j = 0; j < NP; ++j)
+            if (kn0 + j <= zhi) fetch(kn0 + j, j);
+        const int klast = kn0 + NP - 1 < zhi ? kn0 + NP - 1 : zhi;
+        const int cneed = PROLONG ? fdiv2(klast) + 1 : 0;
+        if (PROLONG) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (cmax + 1 + j <= cneed) {
+                    fetch_coarse(cmax + 1 + j, j);
+                    nnew = j + 1;
+                }
+        }
+        // first colour of planes pa .. pa + NP - 1 (margin 1)
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int k = pa + j;
+            if (k <= zhi - 1) colour(k, 1, 1, k >= z0 && k < z0 + ZC);
+        }
+        __syncthreads();
+        // second colour of planes pa - 1 .. pa + NP - 2 (margin 2): final
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int k = pa - 1 + j;
+            if (k >= zlo + 2 && k <= zhi - 2) colour(k, 2, 0, k >= z0 && k < z0 + ZC);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int k = pa - 1 + j;
+            if (k >= z0 && k < z0 + ZC) {
+                const float *u = su + pmod(k, R) * PP;
+                const int gz = wrapn(k, n);
+                for (int q = tid; q < SX * SY; q += NT) {
+                    const int x = q % SX, y = q / SX;
+                    uout[(gz * n + y0 + y) * n + x0 + x] = u[(y + H) * PX + x + H];
                 }
             }
+        }
+        if (RESID) {
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const int kq = pa - 2 + j;
+                if (kq < z0 || kq >= z0 + ZC) continue;
+                any = true;
+                if (G::PAIRS) {
+                    if (kq & 1) restrict_pair(kq, false);
+                } else if ((kq & 1) == 0) {
+                    for (int q = tid; q < SX * SY; q += NT) srp[q] = resid_at(kq, q % SX, q / SX);
+                } else {
+                    restrict_pair(kq, true);
+                }
+            }
+            if (any) __syncthreads();
+        }
+        if (PROLONG && nnew) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (j < nnew) put_coarse(cmax + 1 + j, j);
+            cmax += nnew;
             __syncthreads();
         }
-        if (cnew) {
-            put_coarse(fdiv2(knext) + 1);
-            cmax = fdiv2(knext) + 1;
-            __syncthreads();
-        }
-        if (more) put(knext);
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+            if (kn0 + j <= zhi) put(kn0 + j, j);
         __syncthreads();
     }
     if (resmax) block_atomic_max(resmax, amax);
@@ -586,12 +627,15 @@ k_sweep_stream(const float *__restrict__ uin, float *__restrict__ uout, const fl
 
 // Streaming shapes the autotuners choose from beside the tiles (one sweep per launch, source always
 // staged): column SX x SY, chunk of ZC planes, NT threads.
-using Stream0 = StreamShape<64, 16, 32, 512>;
-using Stream1 = StreamShape<32, 16, 32, 256>;
-using Stream2 = StreamShape<64, 32, 32, 512>;
-constexpr int kNumStream = 3;
-static const int kStreamDims[kNumStream][4] = {{64, 16, 32, 512}, {32, 16, 32, 256}, {64, 32, 32, 512}};
-static_assert(stream_fits_lds<Stream2, true, false>() && stream_fits_lds<Stream2, false, true>(), "stream LDS");
+using Stream0 = StreamShape<64, 16, 32, 512, 1>;
+using Stream1 = StreamShape<64, 16, 32, 512, 2>;
+using Stream2 = StreamShape<64, 16, 16, 512, 2>;
+using Stream3 = StreamShape<32, 16, 32, 256, 2>;
+using Stream4 = StreamShape<64, 32, 32, 512, 2>;
+constexpr int kNumStream = 5;
+static const int kStreamDims[kNumStream][5] = {
+    {64, 16, 32, 512, 1}, {64, 16, 32, 512, 2}, {64, 16, 16, 512, 2}, {32, 16, 32, 256, 2}, {64, 32, 32, 512, 2}};
+static_assert(stream_fits_lds<Stream4, true, false>() && stream_fits_lds<Stream4, false, true>(), "stream LDS");
 
 // sweep configurations: 0 .. kNumTiles-1 tiles, kNumTiles .. kNumTiles+kNumStream-1 streams
 constexpr int kNumCfg = kNumTiles + kNumStream;

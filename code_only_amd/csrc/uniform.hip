@@ -293,6 +293,8 @@ static int launch_sweep_cfg(hipStream_t s, int cfg, int ns, bool fs, const float
     case kNumTiles + 0: stream_variant<Stream0>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case kNumTiles + 1: stream_variant<Stream1>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case kNumTiles + 2: stream_variant<Stream2>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 3: stream_variant<Stream3>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 4: stream_variant<Stream4>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 0: sweep_ns<Tile0>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 1: sweep_ns<Tile1>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 2: sweep_ns<Tile2>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;

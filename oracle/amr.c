@@ -1102,7 +1102,7 @@ double orc_amr_op(orc_amr *s, int op, int L, int a, int b, int c, int flag)
 {
     switch (op) {
     case ORC_AMR_OP_C2P: c2p(s, L, a, flag); return 0.0;
-    case ORC_AMR_OP_GHOSTFIX: ghostcell_fix(s, L, a, flag); return 0.0;
+    case ORC_AMR_OP_GHOSTFIX: ghostcell_fix(s, L, a, flag & 1); return 0.0;   /* bit 2: GPU-side form only */
     case ORC_AMR_OP_FILL0: fill0_faces(s, L, a, 0, s->nlev); return 0.0;
     case ORC_AMR_OP_EXTRAP: extrap_faces(s, L, a, 0, s->nlev); return 0.0;
     case ORC_AMR_OP_RELAX: poisson_relax(s, L, a, b, flag); return s->resh2maxg;

@@ -207,7 +207,7 @@ def test_fused_mixed_equals_unfused_bitwise(gpu, n, use_graph):
     s.close()
 
 
-STREAM_CFGS = [5, 6, 7]   # streaming sweep shapes (64x16, 32x16, 64x32 columns, 32-plane chunks)
+STREAM_CFGS = [5, 6, 7, 8, 9]   # streaming sweep shapes (one or two planes per step)
 
 
 @pytest.mark.parametrize("n", [32, 64, 128])
