@@ -319,6 +319,8 @@ def main():
     traffic = None
     prof = os.path.join(ROOT, "profiles", "uniform256_hbm_traffic.md")
     shape = tiles.get(str(n), {}).get("shape")
+    if shape and tiles[str(n)]["source_in_lds"]:
+        shape += "/fLDS"
     if n == 256 and shape and os.path.exists(prof):
         key = f"traffic_bytes_per_launch[{shape}]:"
         for line in open(prof):
@@ -369,7 +371,7 @@ def main():
         "phi_err_vs_cpu_oracle": err_vs_oracle,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "fine-level FP32 smoother sweep (both colours), " + tiles.get(str(n), {}).get("shape", "?"),
+                     "kernel": "fine-level FP32 smoother sweep (both colours), " + (shape or "?"),
                      "algorithmic_bytes_per_launch": sweep_bytes,
                      "ms_per_launch": ms_sweep.value},
         "cpu_baseline": cpu,

@@ -21,6 +21,9 @@ SG_STOP_TRUE_RESIDUAL = 1
 SG_PREC_FP64 = 1
 SG_PREC_MIXED = 2
 
+SG_COARSE_EXACT = 0
+SG_COARSE_VCYCLE = 1
+
 SG_HIST = 64
 
 SG_BC_PERIODIC = 0
@@ -35,7 +38,7 @@ class SgParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxit", ctypes.c_int), ("stop_mode", ctypes.c_int),
                 ("precision", ctypes.c_int), ("ncycle", ctypes.c_int), ("npre", ctypes.c_int),
                 ("npost", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("fused", ctypes.c_int)]
+                ("fused", ctypes.c_int), ("coarse", ctypes.c_int)]
 
 
 class SgStats(ctypes.Structure):

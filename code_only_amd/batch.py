@@ -7,9 +7,10 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import SG_ERR_NONFINITE, SG_ERR_NOT_CONVERGED, SG_PREC_FP64, SG_PREC_MIXED, SG_STOP_TRUE_RESIDUAL
+from ._lib import (SG_COARSE_EXACT, SG_COARSE_VCYCLE, SG_ERR_NONFINITE, SG_ERR_NOT_CONVERGED, SG_PREC_FP64,
+                   SG_PREC_MIXED, SG_STOP_TRUE_RESIDUAL)
 
-__all__ = ["BatchSolver", "SG_PREC_FP64", "SG_PREC_MIXED"]
+__all__ = ["BatchSolver", "SG_PREC_FP64", "SG_PREC_MIXED", "SG_COARSE_EXACT", "SG_COARSE_VCYCLE"]
 
 
 class BatchSolver:
@@ -40,8 +41,9 @@ class BatchSolver:
             pass
 
     def solve(self, psis, srcs, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, ncycle=2, npre=2, npost=2,
-              use_graph=True, raise_on_fail=True):
-        """Solve every problem in place (psis[q] in = initial guess).  Returns per-problem stats."""
+              use_graph=True, raise_on_fail=True, coarse=SG_COARSE_EXACT):
+        """Solve every problem in place (psis[q] in = initial guess).  Returns per-problem stats.
+        coarse: as UniformSolver.cycle's (mixed precision only)."""
         npb = len(self.sizes)
         if len(psis) != npb or len(srcs) != npb:
             raise ValueError(f"expected {npb} psi and src tensors")
@@ -51,7 +53,7 @@ class BatchSolver:
                 if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and tuple(t.shape) == (n,) * 3):
                     raise ValueError(f"problem {q}: expected a contiguous float64 GPU tensor of shape {(n,) * 3}")
         p = _lib.default_params(tol=tol, maxit=maxit, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=precision,
-                                ncycle=ncycle, npre=npre, npost=npost, use_graph=int(bool(use_graph)))
+                                ncycle=ncycle, npre=npre, npost=npost, use_graph=int(bool(use_graph)), coarse=coarse)
         psi_arr = (ctypes.c_void_p * npb)(*[t.data_ptr() for t in psis])
         src_arr = (ctypes.c_void_p * npb)(*[t.data_ptr() for t in srcs])
         st = (_lib.SgStats * npb)()

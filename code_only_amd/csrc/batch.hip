@@ -35,7 +35,7 @@ namespace {
 
 struct Plan {
     bool valid = false;
-    int ncycle = -1, npre = -1, npost = -1;
+    int ncycle = -1, npre = -1, npost = -1, coarse = -1;
     std::vector<Item> items;                // item lists of all launches, concatenated
     std::vector<std::pair<int, int>> recs;  // (offset, count) per launch, in schedule order
     size_t iter_begin = 0;                  // first record of the outer iteration
@@ -391,7 +391,7 @@ int op_interp_cubic_tile(sg_batch *c, const Pairs &pl, int fx, int cuc)
     return SG_OK;
 }
 
-int op_coarse(sg_batch *c, const Pairs &pl, int mode, int ncycle, int npre, int npost)
+int op_coarse(sg_batch *c, const Pairs &pl, int mode, int ncycle, int npre, int npost, bool exact)
 {
     for (auto &q : pl) SG_REQUIRE(q.second == c->lc[q.first], "batch schedule: coarse kernel off its level");
     const Item *d;
@@ -399,15 +399,16 @@ int op_coarse(sg_batch *c, const Pairs &pl, int mode, int ncycle, int npre, int 
     SG_TRY(take(c, pl, [](int, int) { return 1; }, &d, &cnt, &tot));
     if (!cnt) return SG_OK;
     hipLaunchKernelGGL(kb_coarse, dim3(tot), dim3(kCoarseThreads), 0, c->stream, bl<float>(c, d, cnt),
-                       (const CoarseDesc *)c->d_desc, mode, ncycle, npre, npost);
+                       (const CoarseDesc *)c->d_desc, mode, ncycle, npre, npost, exact ? 1 : 0);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
 
 // mg_lin in FP32 with the fused kernels (the single-grid fused mg_lin, stepped over all problems
 // with the same fine level index at once).  c->ecode receives the buffer code that holds level 0's
-// correction for problems with at least one fine level.
-int fast_mg_lin(sg_batch *c, int ncycle, int npre, int npost)
+// correction for problems with at least one fine level.  exact: the first level of <= 16^3 cells
+// of a problem with finer levels is solved exactly (as the single-grid solver's SG_COARSE_EXACT).
+int fast_mg_lin(sg_batch *c, int ncycle, int npre, int npost, bool exact)
 {
     const int maxlc = c->maxlc;
     auto fine_at = [&](int l, int minlc) {  // problems with lc > minlc, at level l
@@ -416,11 +417,11 @@ int fast_mg_lin(sg_batch *c, int ncycle, int npre, int npost)
     auto coarse_of = [&](auto pred) { return members(c, [&](int p) { return pred(p) ? c->lc[p] : -1; }); };
     for (int l = 1; l <= maxlc; ++l)
         SG_TRY(op_restrict<float>(c, members(c, [&](int p) { return c->lc[p] >= l ? l : -1; }), CRHS, CRHS));
-    SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] == 0; }), 0, ncycle, npre, npost));
+    SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] == 0; }), 0, ncycle, npre, npost, exact));
     int cur[kBLev] = {};
     auto code = [](int b) { return b ? CU1 : CU; };
     for (int ml = maxlc - 1; ml >= 0; --ml) {
-        SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] == ml + 1; }), 0, ncycle, npre, npost));
+        SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] == ml + 1; }), 0, ncycle, npre, npost, exact));
         const Pairs M = fine_at(ml, ml);
         SG_TRY(op_interp_cubic_tile(c, M, c->cubic_fx[ml], code(cur[ml + 1])));
         cur[ml] = 0;
@@ -436,7 +437,7 @@ int fast_mg_lin(sg_batch *c, int ncycle, int npre, int npost)
                     r -= k;
                 }
             }
-            SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] > ml; }), 1, ncycle, npre, npost));
+            SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] > ml; }), 1, ncycle, npre, npost, exact));
             for (int v = maxlc - 1; v >= ml; --v) {
                 const Pairs Mv = fine_at(v, v);
                 for (int r = npost, first = 1; r > 0; first = 0) {
@@ -504,7 +505,7 @@ int run_init(sg_batch *c, bool mixed)
 int run_iteration(sg_batch *c, bool mixed, const sg_params *p, int parity)
 {
     if (mixed) {
-        SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost));
+        SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost, p->coarse == SG_COARSE_EXACT));
         return op_outer_resid<float>(c, parity, true, c->ecode);
     }
     SG_TRY(plain_mg_lin(c, p->ncycle, p->npre, p->npost));
@@ -515,7 +516,8 @@ int run_iteration(sg_batch *c, bool mixed, const sg_params *p, int parity)
 int ensure_plan(sg_batch *c, bool mixed, const sg_params *p)
 {
     Plan &P = c->plan[mixed ? 1 : 0];
-    if (P.valid && P.ncycle == p->ncycle && P.npre == p->npre && P.npost == p->npost) return SG_OK;
+    if (P.valid && P.ncycle == p->ncycle && P.npre == p->npre && P.npost == p->npost && P.coarse == p->coarse)
+        return SG_OK;
     if (P.d_items) (void)hipFree(P.d_items);
     P = Plan();
     c->cur = &P;
@@ -532,6 +534,7 @@ int ensure_plan(sg_batch *c, bool mixed, const sg_params *p)
     P.ncycle = p->ncycle;
     P.npre = p->npre;
     P.npost = p->npost;
+    P.coarse = p->coarse;
     P.valid = true;
     ++c->plan_gen;
     return SG_OK;
@@ -672,6 +675,7 @@ int solve(sg_batch *c, double *const *psi, const double *const *src, const sg_pa
     SG_REQUIRE(p->tol >= 0.0, "bad tol");
     SG_REQUIRE(p->ncycle >= 1 && p->ncycle <= 64 && p->npre >= 1 && p->npre <= 64 && p->npost >= 1 && p->npost <= 64,
                "bad cycle shape");
+    SG_REQUIRE(p->coarse == SG_COARSE_EXACT || p->coarse == SG_COARSE_VCYCLE, "bad coarse %d", p->coarse);
     const bool mixed = p->precision == SG_PREC_MIXED;
     for (int q = 0; q < c->np; ++q) SG_REQUIRE(psi[q] && src[q], "null array of problem %d", q);
     if (mixed) SG_TRY(autotune(c));
@@ -761,20 +765,6 @@ void destroy(sg_batch *c)
     delete c;
 }
 
-// Hartley basis of the periodic 3-point second difference and its eigenvalues at spacing h,
-// formed exactly as the CPU restatement forms them.
-void exact_basis(int n, double h, double *Q, double *lam)
-{
-    const double pi = 3.14159265358979323846;
-    for (int m = 0; m < n; ++m) {
-        for (int k = 0; k < n; ++k) {
-            const double a = 2.0 * pi * (double)((m * k) % n) / (double)n;
-            Q[m * n + k] = (std::cos(a) + std::sin(a)) / std::sqrt((double)n);
-        }
-        lam[m] = (2.0 * std::cos(2.0 * pi * (double)m / (double)n) - 2.0) / (h * h);
-    }
-}
-
 int create(sg_batch **out, int np, const int *ns, const double *hs, int precisions)
 {
     SG_REQUIRE(out, "null context pointer");
@@ -801,7 +791,7 @@ int create(sg_batch **out, int np, const int *ns, const double *hs, int precisio
     c->h_pl32.assign((size_t)np * kBLev, PL<float>{});
     c->h_desc.assign(np, CoarseDesc{});
     std::vector<double> lam3(3 * np, 0.0), q3(9, 0.0), lamtmp(3);
-    exact_basis(3, 1.0, q3.data(), lamtmp.data());
+    hartley_basis(3, 1.0, q3.data(), lamtmp.data());
     auto fail = [&](int rc) { destroy(c); return rc; };
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -852,7 +842,7 @@ int create(sg_batch **out, int np, const int *ns, const double *hs, int precisio
             }
             if (m == 3) {
                 double Qd[9], ld[3];
-                exact_basis(3, hl, Qd, ld);
+                hartley_basis(3, hl, Qd, ld);
                 for (int e = 0; e < 3; ++e) lam3[3 * q + e] = ld[e];
             }
         }
@@ -872,10 +862,23 @@ int create(sg_batch **out, int np, const int *ns, const double *hs, int precisio
             off += m * m * m;
             if (m == 3) {
                 double Qd[9], ld[3];
-                exact_basis(3, hl, Qd, ld);
+                hartley_basis(3, hl, Qd, ld);
                 for (int e = 0; e < 9; ++e) D.q3[e] = (float)Qd[e];
                 for (int e = 0; e < 3; ++e) D.lam3[e] = (float)ld[e];
             }
+        }
+        // exact solve of held level 0 when the problem has finer levels (SG_COARSE_EXACT)
+        D.exact = lc > 0 ? 1 : 0;
+        if (D.exact) {
+            const int m = ns[q] >> lc;
+            if (m > kCoarseMaxN) {
+                set_error("problem %d: first coarse level %d^3 exceeds the exact solve's size", q, m);
+                return fail(SG_ERR_ARG);
+            }
+            double Qd[kCoarseMaxN * kCoarseMaxN], ld[kCoarseMaxN];
+            hartley_basis(m, hs[q] * (double)(1 << lc), Qd, ld);
+            for (int e = 0; e < m * m; ++e) D.qx[e] = (float)Qd[e];
+            for (int e = 0; e < m; ++e) D.lamx[e] = (float)ld[e];
         }
         if (off > kCoarseCellsB || D.nlev > 8) {
             set_error("problem %d: coarse levels exceed the coarse kernel's capacity", q);

@@ -398,6 +398,8 @@ struct CoarseDesc {
     int off[8];
     FastDiv dn[8], dnh[8];
     float q3[9], lam3[3];  // exact 3^3 solve (coarsest n == 3)
+    int exact;             // held level 0 lies below the problem's level 0: exact solve available
+    float qx[kCoarseMaxN * kCoarseMaxN], lamx[kCoarseMaxN];  // its Hartley basis and eigenvalues
 };
 
 constexpr int kCoarseCellsB = kCoarseCells;  // 16^3 + 8^3 + ... + 1 >= 12^3 + 6^3 + 3^3
@@ -498,9 +500,10 @@ __device__ __forceinline__ void cb_slvsml(float *U, const float *R, const Coarse
 
 // One workgroup per item (level l = the problem's first coarse level).  mode 0: FMG over the held
 // levels from level 0's RHS (mg.F90:142-215 restricted to them); mode 1: one V-cycle from held
-// level 0 with zero initial guess (mg.F90:179-210).
+// level 0 with zero initial guess (mg.F90:179-210).  exact: both modes solve held level 0 exactly
+// (coarse_exact) for the problems whose descriptor allows it.
 __global__ void __launch_bounds__(kCoarseThreads)
-kb_coarse(BL<float> L, const CoarseDesc *descs, int mode, int ncycle, int npre, int npost)
+kb_coarse(BL<float> L, const CoarseDesc *descs, int mode, int ncycle, int npre, int npost, int exact)
 {
     Cell<float> c;
     if (!locate(L, (int)blockIdx.x, c)) return;
@@ -509,6 +512,10 @@ kb_coarse(BL<float> L, const CoarseDesc *descs, int mode, int ncycle, int npre, 
     const int lmax = D.nlev - 1;
     const int n0 = D.n[0];
     const float *rhs_in = c.q->a[CRHS];
+    if (exact && D.exact) {
+        coarse_exact(sU, sR, D.qx, D.lamx, rhs_in, c.q->a[CU], n0);
+        return;
+    }
     for (int q = threadIdx.x; q < n0 * n0 * n0; q += kCoarseThreads) sR[q] = rhs_in[q];
     __syncthreads();
     auto U = [&](int l) { return sU + D.off[l]; };

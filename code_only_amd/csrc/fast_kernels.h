@@ -16,12 +16,15 @@
 //    reference's three separable passes, on an LDS-staged coarse tile.
 //  * k_coarse: every level of <= 16^3 cells in one workgroup's LDS: the FMG part of mg_lin
 //    (mg.F90:142-215) from the coarsest level up to the first fine level, or one V-cycle from that
-//    level down and back (the bottom of each fine-level V-cycle).
+//    level down and back (the bottom of each fine-level V-cycle); or, in both places, the exact
+//    solve of the first level of <= 16^3 cells by Hartley transforms (coarse_exact).
 //
 // Every cell value is produced by the same helpers (relax_value, resid_value, restrict_value,
 // trilinear_value, cubic_stencil) as the one-kernel-per-loop path, in the same order, so the fused
 // solve reproduces the unfused FP32 solve exactly.
 #pragma once
+#include <cmath>
+
 #include "uniform_kernels.h"
 
 namespace sg {
@@ -764,9 +767,93 @@ struct CoarseLevels {
     int n[8], lg[8];        // cells per side and its log2
     float h2[8], h2i[8];
     int off[8];             // offset of level l inside each LDS array
+    int exact;              // 1: held level 0 solved exactly instead (coarse_exact)
+    float Q[kCoarseMaxN * kCoarseMaxN], lam[kCoarseMaxN];  // its Hartley basis and eigenvalues
 };
 
 constexpr int kCoarseCells = 16 * 16 * 16 + 8 * 8 * 8 + 4 * 4 * 4 + 2 * 2 * 2 + 1;
+
+// Hartley basis of the periodic 3-point second difference on n cells, Q[m n + k] =
+// (cos a + sin a) / sqrt(n) with a = 2 pi ((m k) mod n) / n, and its eigenvalues
+// lam[m] = (2 cos(2 pi m / n) - 2) / h^2 (host; the CPU restatement forms them the same way).
+inline void hartley_basis(int n, double h, double *Q, double *lam)
+{
+    const double pi = 3.14159265358979323846;
+    for (int m = 0; m < n; ++m) {
+        for (int k = 0; k < n; ++k) {
+            const double a = 2.0 * pi * (double)((m * k) % n) / (double)n;
+            Q[m * n + k] = (std::cos(a) + std::sin(a)) / std::sqrt((double)n);
+        }
+        lam[m] = (2.0 * std::cos(2.0 * pi * (double)m / (double)n) - 2.0) / (h * h);
+    }
+}
+
+// ---- exact solve of a periodic level of n^3 cells (n <= 16) in one workgroup ---------------------
+// The coarse-grid problem L u = f that the V-cycle bottom and the FMG start approximate (mg.F90:
+// 142-210 below the first fine level) is solved exactly instead: the 7-point operator separates,
+// the symmetric orthonormal Hartley basis Q diagonalises the periodic second difference along
+// each axis, so u = Q (Lambda^-1 (Q f)) with Lambda = (lam_i + lam_j) + lam_k and the zero mode set
+// to zero (the zero-mean solution).  Three axis passes serve as forward and inverse transform.
+// The grid sits in LDS with row pitch n + 1; in each axis pass every grid line goes to
+// kCoarseThreads / n^2 threads, which read the whole line into registers and then each write
+// every (that many)-th entry of Q times it, accumulated k = 0 .. n-1 in order.
+__device__ inline void ce_pass(float *g, const float *Q, int n, int axis)
+{
+    const int P = n + 1, nlines = n * n, parts = kCoarseThreads / nlines;
+    const int t = threadIdx.x, line = t % nlines, part = t / nlines;
+    const int a = line % n, b = line / n;
+    int base, stride;
+    if (axis == 0) { base = (b * n + a) * P; stride = 1; }        // line (j, k) = (a, b)
+    else if (axis == 1) { base = b * n * P + a; stride = P; }     // line (i, k)
+    else { base = b * P + a; stride = n * P; }                    // line (i, j)
+    const bool act = part < parts;
+    float x[kCoarseMaxN];
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < kCoarseMaxN; ++k) x[k] = k < n ? g[base + k * stride] : 0.f;
+    }
+    __syncthreads();
+    if (act) {
+        for (int m = part; m < n; m += parts) {
+            const float *q = Q + m * n;
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < kCoarseMaxN; ++k)
+                if (k < n) acc += q[k] * x[k];
+            g[base + m * stride] = acc;
+        }
+    }
+    __syncthreads();
+}
+
+// u_out := exact solution for rhs_in (dense n^3 arrays, i fastest).  g: n^2 (n + 1) floats of LDS,
+// s: n^2 + n floats of LDS for the basis; Qg, lamg: basis and eigenvalues (any address space).
+__device__ inline void coarse_exact(float *g, float *s, const float *Qg, const float *lamg, const float *rhs_in,
+                                    float *u_out, int n)
+{
+    const int P = n + 1, n3 = n * n * n;
+    float *Q = s, *lam = s + n * n;
+    for (int t = threadIdx.x; t < n * n; t += kCoarseThreads) Q[t] = Qg[t];
+    for (int t = threadIdx.x; t < n; t += kCoarseThreads) lam[t] = lamg[t];
+    for (int c = threadIdx.x; c < n3; c += kCoarseThreads) {
+        const int i = c % n, j = (c / n) % n, k = c / (n * n);
+        g[(k * n + j) * P + i] = rhs_in[c];
+    }
+    __syncthreads();
+    for (int a = 0; a < 3; ++a) ce_pass(g, Q, n, a);
+    for (int c = threadIdx.x; c < n3; c += kCoarseThreads) {
+        const int i = c % n, j = (c / n) % n, k = c / (n * n);
+        const float den = (lam[i] + lam[j]) + lam[k];
+        float &v = g[(k * n + j) * P + i];
+        v = den != 0.f ? v / den : 0.f;
+    }
+    __syncthreads();
+    for (int a = 0; a < 3; ++a) ce_pass(g, Q, n, a);
+    for (int c = threadIdx.x; c < n3; c += kCoarseThreads) {
+        const int i = c % n, j = (c / n) % n, k = c / (n * n);
+        u_out[c] = g[(k * n + j) * P + i];
+    }
+}
 
 __device__ __forceinline__ void c_relax(float *U, const float *R, int n, int lg, float h2, double *resmax)
 {
@@ -849,14 +936,22 @@ __device__ __forceinline__ void c_interp(float *Uf, const float *Uc, int nf, int
 // mode 0: FMG over all held levels from RHO of local level 0 (mg.F90:142-215 restricted to them);
 // mode 1: one V-cycle from local level 0 with zero initial guess (mg.F90:179-210).
 // resmax receives max|resh2| of the final smoothing call when non-null (only meaningful when the
-// held level 0 is the finest level of the problem).
+// held level 0 is the finest level of the problem).  With L.exact both modes solve held level 0
+// exactly (coarse_exact).
 static __global__ void __launch_bounds__(kCoarseThreads)
 k_coarse(const float *__restrict__ rhs_in, float *__restrict__ u_out, CoarseLevels L, int mode, int ncycle,
          int npre, int npost, double *resmax)
 {
     __shared__ float sU[kCoarseCells], sR[kCoarseCells];
+    static_assert(kCoarseMaxN * kCoarseMaxN * (kCoarseMaxN + 1) <= kCoarseCells &&
+                      kCoarseMaxN * kCoarseMaxN + kCoarseMaxN <= kCoarseCells,
+                  "exact coarse solve exceeds the coarse kernel's LDS");
     const int lmax = L.nlev - 1;
     const int n0 = L.n[0];
+    if (L.exact) {
+        coarse_exact(sU, sR, L.Q, L.lam, rhs_in, u_out, n0);
+        return;
+    }
     for (int c = threadIdx.x; c < n0 * n0 * n0; c += kCoarseThreads) sR[c] = rhs_in[c];
     __syncthreads();
     auto U = [&](int l) { return sU + L.off[l]; };

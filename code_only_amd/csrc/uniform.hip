@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fast_kernels.h"
@@ -71,6 +72,8 @@ struct sg_uniform {
     int tile_ns[kMaxLev] = {};  // autotuned sweeps per launch per fine level
     int tile_fs[kMaxLev] = {};  // autotuned: source staged in LDS
     float tune_ms[kMaxLev][32] = {};
+    CoarseLevels coarse;       // levels held by k_coarse, with the exact-solve basis of the first
+    int lc = 0;                // first level of <= 16^3 cells
     double *psi_dev = nullptr, *src_dev = nullptr;   // staging for the host-pointer entry point
     double *adj_src = nullptr, *adj_part = nullptr;  // adjoint: projected source, mean partials
     double *psi_alt = nullptr;                         // second phi buffer of the fused outer step
@@ -87,7 +90,7 @@ struct sg_uniform {
         const double *pin = nullptr;
         double *pout = nullptr;
         const double *src = nullptr;
-        int ncycle = -1, npre = -1, npost = -1, fused = -1, stop = -1;
+        int ncycle = -1, npre = -1, npost = -1, fused = -1, stop = -1, coarse = -1;
     } graph[2];
 };
 
@@ -193,14 +196,31 @@ template <typename T> static int launch_interp(hipStream_t s, T *uf, const T *uc
 // within one mg_lin call (the next call restricts RHO afresh from level 0), so RHS and RHO share
 // storage here; every kernel reads the same values as in the reference.
 // resmax: receives max|resh2| of the last smoothing call (the one mg_cycle's test reads).
-template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax)
+// exact (FP32 only, and only when the first level of <= 16^3 cells lies below level 0): the levels
+// end at that level, solved exactly by the coarse kernel where the reference sweeps its coarsest
+// level (mg_slvsml, mg.F90:540-546) -- the fused path's coarse-grid solve, so both paths keep
+// the same values.
+static int launch_coarse_k(sg_uniform *c, bool exact, const float *rhs, float *u, int mode, int ncycle, int npre,
+                           int npost, double *resmax);
+
+template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax,
+                                        bool exact = false)
 {
     Levels<T> &A = levels<T>(c);
     hipStream_t s = c->stream;
-    const int lmax = c->nlev - 1;
+    exact = exact && std::is_same<T, float>::value && c->coarse.exact;
+    const int lmax = exact ? c->lc : c->nlev - 1;
+    auto bottom = [&]() -> int {
+        if constexpr (std::is_same<T, float>::value) {
+            if (exact) return launch_coarse_k(c, true, A.RHO[lmax], A.U[lmax], 0, ncycle, npre, npost, nullptr);
+        }
+        for (int it = 0; it < 2; ++it)
+            SG_TRY(launch_relax<T>(s, A.U[lmax], A.RHO[lmax], c->nl[lmax], c->hl[lmax], nullptr));
+        return SG_OK;
+    };
     for (int l = 1; l <= lmax; ++l) SG_TRY(launch_restrict<T>(s, A.RHO[l], A.RHO[l - 1], c->nl[l]));
     SG_TRY(launch_fill<T>(s, A.U[lmax], n3(c->nl[lmax])));
-    for (int it = 0; it < 2; ++it) SG_TRY(launch_relax<T>(s, A.U[lmax], A.RHO[lmax], c->nl[lmax], c->hl[lmax], nullptr));
+    SG_TRY(bottom());
     for (int ml = lmax - 1; ml >= 0; --ml) {
         SG_TRY(launch_interp<T>(s, A.U[ml], A.U[ml + 1], c->nl[ml], true, false));
         for (int jc = 0; jc < ncycle; ++jc) {
@@ -211,8 +231,7 @@ template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int
                 SG_TRY(launch_restrict<T>(s, A.RHO[v + 1], A.RES[v], c->nl[v + 1]));
                 SG_TRY(launch_fill<T>(s, A.U[v + 1], n3(c->nl[v + 1])));
             }
-            for (int it = 0; it < 2; ++it)
-                SG_TRY(launch_relax<T>(s, A.U[lmax], A.RHO[lmax], c->nl[lmax], c->hl[lmax], nullptr));
+            SG_TRY(bottom());
             for (int v = lmax - 1; v >= ml; --v) {
                 SG_TRY(launch_interp<T>(s, A.U[v], A.U[v + 1], c->nl[v], false, true));
                 for (int p = 0; p < npost; ++p) {
@@ -391,29 +410,16 @@ static int autotune_tiles(sg_uniform *c)
     return rc;
 }
 
-static int launch_coarse(sg_uniform *c, const CoarseLevels &L, const float *rhs, float *u, int mode, int ncycle,
-                         int npre, int npost, double *resmax)
+// The levels of <= 16^3 cells held by k_coarse (from the first such level, lc) and the Hartley
+// basis of level lc for its exact solve, which applies only when lc > 0 (a grid of <= 16^3 cells
+// keeps the V-cycle: its smoothing also yields Resmaxg).
+static int coarse_levels(sg_uniform *c)
 {
-    hipLaunchKernelGGL(k_coarse, dim3(1), dim3(kCoarseThreads), 0, c->stream, rhs, u, L, mode, ncycle, npre, npost,
-                       resmax);
-    SG_CHECK_LAUNCH();
-    return SG_OK;
-}
-
-// mg_lin (mg.F90:106-218) in FP32 with fused kernels: the levels of <= 16^3 cells run inside one
-// single-workgroup kernel (k_coarse); each fine-level visit of a V-cycle is npre + npost fused
-// sweeps, the last pre-sweep carrying the residual and its restriction, the first post-sweep the
-// prolongated correction.  Same values as mg_lin<float>, bit for bit.  *u0 receives the buffer
-// that holds the level-0 correction.
-static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax, const float **u0)
-{
-    SG_REQUIRE(npre >= 1 && npost >= 1, "fused path needs npre, npost >= 1");
-    Levels<float> &A = c->f;
-    hipStream_t s = c->stream;
     const int lmax = c->nlev - 1;
     int lc = 0;
     while (c->nl[lc] > kCoarseMaxN) ++lc;
-    CoarseLevels L{};
+    CoarseLevels &L = c->coarse;
+    L = CoarseLevels{};
     L.nlev = lmax - lc + 1;
     SG_REQUIRE(L.nlev <= 8, "too many coarse levels");
     int off = 0;
@@ -428,10 +434,49 @@ static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *r
         off += c->nl[l] * c->nl[l] * c->nl[l];
     }
     SG_REQUIRE(off <= kCoarseCells, "coarse levels exceed LDS capacity");
+    L.exact = lc > 0;
+    if (L.exact) {
+        const int m = c->nl[lc];
+        double Q[kCoarseMaxN * kCoarseMaxN], lam[kCoarseMaxN];
+        hartley_basis(m, c->hl[lc], Q, lam);
+        for (int e = 0; e < m * m; ++e) L.Q[e] = (float)Q[e];
+        for (int e = 0; e < m; ++e) L.lam[e] = (float)lam[e];
+    }
+    c->lc = lc;
+    return SG_OK;
+}
+
+// k_coarse on the held levels; exact: solve level lc exactly (needs c->coarse.exact).
+static int launch_coarse_k(sg_uniform *c, bool exact, const float *rhs, float *u, int mode, int ncycle, int npre,
+                           int npost, double *resmax)
+{
+    SG_REQUIRE(!exact || c->coarse.exact, "exact coarse solve needs a level below the finest");
+    CoarseLevels L = c->coarse;
+    L.exact = exact ? 1 : 0;
+    hipLaunchKernelGGL(k_coarse, dim3(1), dim3(kCoarseThreads), 0, c->stream, rhs, u, L, mode, ncycle, npre, npost,
+                       resmax);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+// mg_lin (mg.F90:106-218) in FP32 with fused kernels: the levels of <= 16^3 cells run inside one
+// single-workgroup kernel (k_coarse); each fine-level visit of a V-cycle is npre + npost fused
+// sweeps, the last pre-sweep carrying the residual and its restriction, the first post-sweep the
+// prolongated correction.  Same values as mg_lin<float>, bit for bit (with the same `exact`: the
+// first level of <= 16^3 cells solved exactly instead of the V-cycle below it).  *u0 receives the
+// buffer that holds the level-0 correction.
+static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax, const float **u0,
+                       bool exact)
+{
+    SG_REQUIRE(npre >= 1 && npost >= 1, "fused path needs npre, npost >= 1");
+    Levels<float> &A = c->f;
+    hipStream_t s = c->stream;
+    const int lc = c->lc;
+    exact = exact && c->coarse.exact;
     int cur[kMaxLev] = {};
     auto buf = [&](int l, int b) -> float * { return b ? c->U1[l] : A.U[l]; };
     for (int l = 1; l <= lc; ++l) SG_TRY(launch_restrict<float>(s, A.RHO[l], A.RHO[l - 1], c->nl[l]));
-    SG_TRY(launch_coarse(c, L, A.RHO[lc], A.U[lc], 0, ncycle, npre, npost, lc == 0 ? resmax : nullptr));
+    SG_TRY(launch_coarse_k(c, exact, A.RHO[lc], A.U[lc], 0, ncycle, npre, npost, lc == 0 ? resmax : nullptr));
     cur[lc] = 0;
     for (int ml = lc - 1; ml >= 0; --ml) {
         SG_TRY(launch_interp_cubic_tile(s, A.U[ml], buf(ml + 1, cur[ml + 1]), c->nl[ml]));
@@ -449,7 +494,7 @@ static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *r
                     r -= k;
                 }
             }
-            SG_TRY(launch_coarse(c, L, A.RHO[lc], A.U[lc], 1, ncycle, npre, npost, nullptr));
+            SG_TRY(launch_coarse_k(c, exact, A.RHO[lc], A.U[lc], 1, ncycle, npre, npost, nullptr));
             cur[lc] = 0;
             for (int v = lc - 1; v >= ml; --v) {
                 // npost sweeps; the first launch adds the prolongated correction
@@ -520,8 +565,9 @@ static int outer_iteration(sg_uniform *c, const double *pin, double *pout, const
     SG_TRY(launch_fill<double>(c->stream, c->dnorm + 2, 1));
     if (p->precision == SG_PREC_MIXED) {
         const float *u0 = c->f.U[0];
-        if (p->fused) SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost, resmax, &u0));
-        else SG_TRY(mg_lin<float>(c, p->ncycle, p->npre, p->npost, resmax));
+        const bool exact = p->coarse == SG_COARSE_EXACT;
+        if (p->fused) SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost, resmax, &u0, exact));
+        else SG_TRY(mg_lin<float>(c, p->ncycle, p->npre, p->npost, resmax, exact));
         if (pin != pout) {
             SG_TRY(launch_update_resid(c, pin, u0, pout, src));
         } else {
@@ -550,7 +596,8 @@ static int run_outer_iteration(sg_uniform *c, const double *pin, double *pout, c
     if (!(p->use_graph && p->precision == SG_PREC_MIXED)) return outer_iteration(c, pin, pout, src, p);
     sg_uniform::Graph &g = c->graph[parity & 1];
     const bool same = g.exec && g.pin == pin && g.pout == pout && g.src == src && g.ncycle == p->ncycle &&
-                      g.npre == p->npre && g.npost == p->npost && g.fused == p->fused && g.stop == p->stop_mode;
+                      g.npre == p->npre && g.npost == p->npost && g.fused == p->fused && g.stop == p->stop_mode &&
+                      g.coarse == p->coarse;
     if (!same) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = sg_uniform::Graph();
@@ -573,6 +620,7 @@ static int run_outer_iteration(sg_uniform *c, const double *pin, double *pout, c
         g.npre = p->npre;
         g.npost = p->npost;
         g.fused = p->fused;
+        g.coarse = p->coarse;
         g.stop = p->stop_mode;
     }
     SG_HIP(hipGraphLaunch(g.exec, c->stream));
@@ -593,6 +641,7 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
     SG_REQUIRE(p->tol >= 0.0, "bad tol");
     SG_REQUIRE(!p->fused || p->precision != SG_PREC_MIXED || (p->npre >= 1 && p->npost >= 1),
                "fused mixed path needs npre, npost >= 1");
+    SG_REQUIRE(p->coarse == SG_COARSE_EXACT || p->coarse == SG_COARSE_VCYCLE, "bad coarse %d", p->coarse);
     std::memset(st, 0, sizeof(*st));
     const bool mixed = p->precision == SG_PREC_MIXED;
     SG_HIP(hipEventRecord(c->ev0, c->stream));
@@ -721,6 +770,7 @@ static int create(sg_uniform **out, int n, double h, int precisions)
             SG_HIP(hipMalloc(&c->dnorm, sizeof(double) * 8));
             SG_HIP(hipMemsetAsync(c->dnorm, 0, sizeof(double) * 8, c->stream));
             SG_HIP(hipHostMalloc(&c->hnorm, sizeof(double) * 8, hipHostMallocDefault));
+            SG_TRY(coarse_levels(c));
             if (precisions & SG_PREC_FP64) SG_TRY(alloc_levels<double>(c, c->d));
             if (precisions & SG_PREC_MIXED) {
                 SG_TRY(alloc_levels<float>(c, c->f));
@@ -763,6 +813,7 @@ void sg_default_params(sg_params *p)
     p->npost = 2;
     p->use_graph = 1;
     p->fused = 1;
+    p->coarse = SG_COARSE_EXACT;
 }
 
 int sg_uniform_create(sg_uniform **ctx, int n, double h, int precisions) { return sg::create(ctx, n, h, precisions); }

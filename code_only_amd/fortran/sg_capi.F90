@@ -10,6 +10,7 @@ module sg_capi
   integer(c_int),parameter :: SG_STOP_REFERENCE = 0, SG_STOP_TRUE_RESIDUAL = 1
   integer(c_int),parameter :: SG_PREC_FP64 = 1, SG_PREC_MIXED = 2
   integer(c_int),parameter :: SG_BC_PERIODIC = 0, SG_BC_SPHERE = 1
+  integer(c_int),parameter :: SG_COARSE_EXACT = 0, SG_COARSE_VCYCLE = 1
   integer,parameter :: SG_HIST = 64
 
   type, bind(C) :: sg_params
@@ -22,6 +23,7 @@ module sg_capi
      integer(c_int) :: npost
      integer(c_int) :: use_graph
      integer(c_int) :: fused
+     integer(c_int) :: coarse
   end type sg_params
 
   type, bind(C) :: sg_stats

@@ -8,10 +8,11 @@ import ctypes
 import torch
 
 from . import _lib
-from ._lib import (SG_ERR_NONFINITE, SG_ERR_NOT_CONVERGED, SG_PREC_FP64, SG_PREC_MIXED, SG_STOP_REFERENCE,
-                   SG_STOP_TRUE_RESIDUAL)
+from ._lib import (SG_COARSE_EXACT, SG_COARSE_VCYCLE, SG_ERR_NONFINITE, SG_ERR_NOT_CONVERGED, SG_PREC_FP64,
+                   SG_PREC_MIXED, SG_STOP_REFERENCE, SG_STOP_TRUE_RESIDUAL)
 
-__all__ = ["UniformSolver", "SG_PREC_FP64", "SG_PREC_MIXED", "SG_STOP_REFERENCE", "SG_STOP_TRUE_RESIDUAL"]
+__all__ = ["UniformSolver", "SG_PREC_FP64", "SG_PREC_MIXED", "SG_STOP_REFERENCE", "SG_STOP_TRUE_RESIDUAL",
+           "SG_COARSE_EXACT", "SG_COARSE_VCYCLE"]
 
 
 def _ptr(t):
@@ -48,13 +49,16 @@ class UniformSolver:
             raise ValueError(f"expected shape {shape}, got {tuple(t.shape)}")
 
     def cycle(self, psi, src, tol=1e-10, maxit=50, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_MIXED,
-              ncycle=2, npre=2, npost=2, use_graph=True, fused=True, raise_on_fail=True):
-        """mg_cycle(ju, jrho) (mg.F90:42-94): psi updated in place; returns stats dict."""
+              ncycle=2, npre=2, npost=2, use_graph=True, fused=True, raise_on_fail=True, coarse=SG_COARSE_EXACT):
+        """mg_cycle(ju, jrho) (mg.F90:42-94): psi updated in place; returns stats dict.  coarse
+        (mixed precision only): SG_COARSE_EXACT solves the first level of <= 16^3 cells exactly,
+        SG_COARSE_VCYCLE keeps the reference's levels down to 1^3."""
         n = self.n
         self._check(psi, shape=(n, n, n))
         self._check(src, shape=(n, n, n))
         p = _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=precision, ncycle=ncycle,
-                                npre=npre, npost=npost, use_graph=int(bool(use_graph)), fused=int(bool(fused)))
+                                npre=npre, npost=npost, use_graph=int(bool(use_graph)), fused=int(bool(fused)),
+                                coarse=coarse)
         st = _lib.SgStats()
         torch.cuda.synchronize()
         rc = self._lib.sg_uniform_cycle_dev(self._ctx, _ptr(psi), _ptr(src), ctypes.byref(p), ctypes.byref(st))
@@ -72,13 +76,13 @@ class UniformSolver:
         return g
 
     def adjoint(self, adj, tol=1e-10, maxit=50, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_MIXED,
-                use_graph=True, fused=True):
+                use_graph=True, fused=True, coarse=SG_COARSE_EXACT):
         """d loss / d src = P L^+ P adj for the periodic solve (sg_uniform_adjoint_dev)."""
         n = self.n
         self._check(adj, shape=(n, n, n))
         grad = torch.empty_like(adj)
         p = _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=precision,
-                                use_graph=int(bool(use_graph)), fused=int(bool(fused)))
+                                use_graph=int(bool(use_graph)), fused=int(bool(fused)), coarse=coarse)
         st = _lib.SgStats()
         torch.cuda.synchronize()
         _lib.check(self._lib.sg_uniform_adjoint_dev(self._ctx, _ptr(adj), _ptr(grad), ctypes.byref(p), ctypes.byref(st)))

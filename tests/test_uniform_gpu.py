@@ -11,8 +11,8 @@ import torch
 import oracle
 from oracle import fft_poisson as fp
 from code_only_amd import problems
-from code_only_amd.uniform import (SG_PREC_FP64, SG_PREC_MIXED, SG_STOP_REFERENCE, SG_STOP_TRUE_RESIDUAL,
-                                   UniformSolver)
+from code_only_amd.uniform import (SG_COARSE_EXACT, SG_COARSE_VCYCLE, SG_PREC_FP64, SG_PREC_MIXED,
+                                   SG_STOP_REFERENCE, SG_STOP_TRUE_RESIDUAL, UniformSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -189,21 +189,50 @@ def test_full_size_256_mixed_vs_plain(gpu):
 
 @pytest.mark.parametrize("n", [16, 32, 64, 128])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_fused_mixed_equals_unfused_bitwise(gpu, n, use_graph):
+@pytest.mark.parametrize("coarse", [SG_COARSE_EXACT, SG_COARSE_VCYCLE])
+def test_fused_mixed_equals_unfused_bitwise(gpu, n, use_graph, coarse):
     """The fused FP32 kernels (LDS tiles, single-workgroup coarse solve) reproduce the
-    one-kernel-per-loop FP32 inner solve bit for bit, including Resmaxg."""
+    one-kernel-per-loop FP32 inner solve bit for bit, including Resmaxg, with either coarse-grid
+    solve (exact at 16^3, or the reference's levels down to 1^3)."""
     f = problems.clumpy_rhs(n, seed=11)
     s = UniformSolver(n, precisions=SG_PREC_MIXED)
     ft = T(f, gpu)
     out = []
     for fused in (False, True):
         psi = torch.zeros_like(ft)
-        st = s.cycle(psi, ft, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, fused=fused, use_graph=use_graph)
+        st = s.cycle(psi, ft, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, fused=fused, use_graph=use_graph,
+                     coarse=coarse)
         out.append((psi, st))
     (p0, s0), (p1, s1) = out
     assert s0["iterations"] == s1["iterations"]
     assert s0["resmaxg"] == s1["resmaxg"]
     assert torch.equal(p0, p1)
+    s.close()
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_exact_coarse_solve_contracts_at_least_as_well(gpu, n):
+    """Solving the 16^3 level exactly (instead of the V-cycle below it) keeps the mixed solve's
+    contraction: the first outer iteration reduces the residual at least as much (up to 5 %) as
+    with the reference's coarse levels (later ones sit at the FP32 inner solve's round-off level),
+    it takes no more iterations, and the result matches the FP64 solve."""
+    f = problems.clumpy_rhs(n, seed=21)
+    s = UniformSolver(n, precisions=SG_PREC_MIXED | SG_PREC_FP64)
+    ft = T(f, gpu)
+    hs = {}
+    for coarse in (SG_COARSE_EXACT, SG_COARSE_VCYCLE):
+        psi = torch.zeros_like(ft)
+        st = s.cycle(psi, ft, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, coarse=coarse)
+        hs[coarse] = (st, psi - psi.mean())
+    p64 = torch.zeros_like(ft)
+    s.cycle(p64, ft, tol=1e-12, maxit=30, precision=SG_PREC_FP64)
+    p64 -= p64.mean()
+    ex, vc = hs[SG_COARSE_EXACT][0], hs[SG_COARSE_VCYCLE][0]
+    print("exact", ex["iterations"], ex["hist"], "vcycle", vc["iterations"], vc["hist"], flush=True)
+    assert ex["iterations"] <= vc["iterations"]
+    assert ex["hist"][1] <= 1.05 * vc["hist"][1]
+    e = ((hs[SG_COARSE_EXACT][1] - p64).norm() / p64.norm()).item()
+    assert e < 1e-9, e
     s.close()
 
 
