@@ -49,8 +49,9 @@ constexpr int kCubicFXs[3] = {32, 16, 8};
 struct sg_batch {
     int np = 0, precisions = 0;
     std::vector<int> n, nlev, lc;
+    std::vector<int> lb;   // bottom level of the exact-coarse cycle: lc, or a level of <= 64^3 cells
     std::vector<double> h;
-    int maxlev = 0, maxlc = 0;
+    int maxlev = 0, maxlc = 0, maxlb = 0;
     std::vector<void *> allocs;
     std::vector<PL<double>> h_pl64;
     std::vector<PL<float>> h_pl32;
@@ -404,24 +405,59 @@ int op_coarse(sg_batch *c, const Pairs &pl, int mode, int ncycle, int npre, int 
     return SG_OK;
 }
 
+// U := exact solution for RHS on each pair's level by the plane-parallel kernels, U1 as the
+// intermediate array (the problems' bottom levels of more than 16^3 cells).
+int op_exact_planes(sg_batch *c, const Pairs &pl)
+{
+    for (auto &q : pl) {
+        const int n = c->h_pl32[q.first * kBLev + q.second].n;
+        SG_REQUIRE(q.second == c->lb[q.first] && q.second != c->lc[q.first] && c->h_desc[q.first].qp &&
+                       n <= kPlaneMaxN && n % 4 == 0,
+                   "batch schedule: plane-parallel exact solve off its level");
+    }
+    const Item *d;
+    int cnt, tot;
+    auto planes = [&](int p, int l) { return c->h_pl32[p * kBLev + l].n; };
+    const CoarseDesc *D = (const CoarseDesc *)c->d_desc;
+    for (int step = 0; step < 3; ++step) {
+        SG_TRY(take(c, pl, planes, &d, &cnt, &tot));
+        if (!cnt) continue;
+        const BL<float> L = bl<float>(c, d, cnt);
+        const dim3 grid(8 * ((tot + 7) / 8)), block(kPlaneThreads);
+        if (step == 0) hipLaunchKernelGGL(kb_exact_xy, grid, block, 0, c->stream, L, tot, D, CRHS, CU1);
+        else if (step == 1) hipLaunchKernelGGL(kb_exact_z, grid, block, 0, c->stream, L, tot, D, CU1);
+        else hipLaunchKernelGGL(kb_exact_xy, grid, block, 0, c->stream, L, tot, D, CU1, CU);
+        SG_CHECK_LAUNCH();
+    }
+    return SG_OK;
+}
+
 // mg_lin in FP32 with the fused kernels (the single-grid fused mg_lin, stepped over all problems
 // with the same fine level index at once).  c->ecode receives the buffer code that holds level 0's
-// correction for problems with at least one fine level.  exact: the first level of <= 16^3 cells
-// of a problem with finer levels is solved exactly (as the single-grid solver's SG_COARSE_EXACT).
+// correction for problems with at least one fine level.  exact: the cycle of a problem with finer
+// levels ends at its level lb, solved exactly (as the single-grid solver's SG_COARSE_EXACT).
 int fast_mg_lin(sg_batch *c, int ncycle, int npre, int npost, bool exact)
 {
-    const int maxlc = c->maxlc;
-    auto fine_at = [&](int l, int minlc) {  // problems with lc > minlc, at level l
-        return members(c, [&](int p) { return c->lc[p] > minlc ? l : -1; });
+    const std::vector<int> &bot = exact ? c->lb : c->lc;  // level of each problem's coarse-grid solve
+    const int maxlc = exact ? c->maxlb : c->maxlc;
+    auto fine_at = [&](int l, int minlc) {  // problems with bottom level > minlc, at level l
+        return members(c, [&](int p) { return bot[p] > minlc ? l : -1; });
     };
-    auto coarse_of = [&](auto pred) { return members(c, [&](int p) { return pred(p) ? c->lc[p] : -1; }); };
+    // coarse-grid solve of the problems selected by pred: k_coarse, or the plane-parallel kernels
+    auto bottom = [&](auto pred, int mode) -> int {
+        auto planes = [&](int p) { return exact && c->lb[p] != c->lc[p]; };
+        SG_TRY(op_coarse(c, members(c, [&](int p) { return pred(p) && !planes(p) ? c->lc[p] : -1; }), mode, ncycle,
+                         npre, npost, exact));
+        if (exact) SG_TRY(op_exact_planes(c, members(c, [&](int p) { return pred(p) && planes(p) ? c->lb[p] : -1; })));
+        return SG_OK;
+    };
     for (int l = 1; l <= maxlc; ++l)
-        SG_TRY(op_restrict<float>(c, members(c, [&](int p) { return c->lc[p] >= l ? l : -1; }), CRHS, CRHS));
-    SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] == 0; }), 0, ncycle, npre, npost, exact));
+        SG_TRY(op_restrict<float>(c, members(c, [&](int p) { return bot[p] >= l ? l : -1; }), CRHS, CRHS));
+    SG_TRY(bottom([&](int p) { return bot[p] == 0; }, 0));
     int cur[kBLev] = {};
     auto code = [](int b) { return b ? CU1 : CU; };
     for (int ml = maxlc - 1; ml >= 0; --ml) {
-        SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] == ml + 1; }), 0, ncycle, npre, npost, exact));
+        SG_TRY(bottom([&](int p) { return bot[p] == ml + 1; }, 0));
         const Pairs M = fine_at(ml, ml);
         SG_TRY(op_interp_cubic_tile(c, M, c->cubic_fx[ml], code(cur[ml + 1])));
         cur[ml] = 0;
@@ -437,7 +473,7 @@ int fast_mg_lin(sg_batch *c, int ncycle, int npre, int npost, bool exact)
                     r -= k;
                 }
             }
-            SG_TRY(op_coarse(c, coarse_of([&](int p) { return c->lc[p] > ml; }), 1, ncycle, npre, npost, exact));
+            SG_TRY(bottom([&](int p) { return bot[p] > ml; }, 1));
             for (int v = maxlc - 1; v >= ml; --v) {
                 const Pairs Mv = fine_at(v, v);
                 for (int r = npost, first = 1; r > 0; first = 0) {
@@ -787,6 +823,7 @@ int create(sg_batch **out, int np, const int *ns, const double *hs, int precisio
     c->h.assign(hs, hs + np);
     c->nlev.resize(np);
     c->lc.resize(np);
+    c->lb.resize(np);
     c->h_pl64.assign((size_t)np * kBLev, PL<double>{});
     c->h_pl32.assign((size_t)np * kBLev, PL<float>{});
     c->h_desc.assign(np, CoarseDesc{});
@@ -812,6 +849,16 @@ int create(sg_batch **out, int np, const int *ns, const double *hs, int precisio
         while (lc < nl - 1 && (ns[q] >> lc) > kCoarseMaxN) ++lc;
         c->lc[q] = lc;
         c->maxlc = std::max(c->maxlc, lc);
+        // exact-coarse cycle: ends at the first level below level 0 with <= 64^3 cells when that
+        // has more than 16^3 (plane-parallel solve), else at lc (k_coarse)
+        int lbq = lc;
+        if (lc > 0) {
+            int l = 1;
+            while ((ns[q] >> l) > kPlaneMaxN) ++l;
+            if ((ns[q] >> l) > kCoarseMaxN) lbq = l;
+        }
+        c->lb[q] = lbq;
+        c->maxlb = std::max(c->maxlb, lbq);
         part += nblk_cells(cells(ns[q]));
         for (int l = 0; l < nl; ++l) {
             const int m = ns[q] >> l;
@@ -879,6 +926,24 @@ int create(sg_batch **out, int np, const int *ns, const double *hs, int precisio
             hartley_basis(m, hs[q] * (double)(1 << lc), Qd, ld);
             for (int e = 0; e < m * m; ++e) D.qx[e] = (float)Qd[e];
             for (int e = 0; e < m; ++e) D.lamx[e] = (float)ld[e];
+        }
+        D.qp = D.invp = nullptr;
+        if (c->lb[q] != lc && (precisions & SG_PREC_MIXED)) {
+            const int m = ns[q] >> c->lb[q];
+            std::vector<double> Qd((size_t)m * m), ld(m);
+            hartley_basis(m, hs[q] * (double)(1 << c->lb[q]), Qd.data(), ld.data());
+            std::vector<float> Qf(Qd.begin(), Qd.end()), inv((size_t)m * m * m);
+            exact_inverse_table(m, ld.data(), inv.data());
+            float *qp = nullptr, *invp = nullptr;
+            int rc = SG_OK;
+            if ((rc = dev_alloc(c, &qp, Qf.size())) || (rc = dev_alloc(c, &invp, inv.size()))) return fail(rc);
+            if (hipMemcpy(qp, Qf.data(), sizeof(float) * Qf.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(invp, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                set_error("exact-solve table upload failed");
+                return fail(SG_ERR_HIP);
+            }
+            D.qp = qp;
+            D.invp = invp;
         }
         if (off > kCoarseCellsB || D.nlev > 8) {
             set_error("problem %d: coarse levels exceed the coarse kernel's capacity", q);

@@ -14,6 +14,7 @@
 // solved alone.  Index arithmetic is general (division by n through a multiply-high reciprocal):
 // n need not be a power of two.
 #pragma once
+#include "exact_planes.h"
 #include "fast_kernels.h"
 
 namespace sg {
@@ -400,6 +401,8 @@ struct CoarseDesc {
     float q3[9], lam3[3];  // exact 3^3 solve (coarsest n == 3)
     int exact;             // held level 0 lies below the problem's level 0: exact solve available
     float qx[kCoarseMaxN * kCoarseMaxN], lamx[kCoarseMaxN];  // its Hartley basis and eigenvalues
+    const float *qp, *invp; // plane-parallel exact solve of the problem's bottom level (n <= 64):
+                            // Hartley basis and 1 / Lambda (device), or null
 };
 
 constexpr int kCoarseCellsB = kCoarseCells;  // 16^3 + 8^3 + ... + 1 >= 12^3 + 6^3 + 3^3
@@ -547,6 +550,24 @@ kb_coarse(BL<float> L, const CoarseDesc *descs, int mode, int ncycle, int npre, 
     }
     float *u_out = c.q->a[CU];
     for (int q = threadIdx.x; q < n0 * n0 * n0; q += kCoarseThreads) u_out[q] = sU[q];
+}
+
+// Plane-parallel exact solve (exact_planes.h) of each item's level: workgroup b of an item works on
+// plane (k_exact_xy) or row (k_exact_z) b of that problem's level.
+__global__ void __launch_bounds__(kPlaneThreads) kb_exact_xy(BL<float> L, int total, const CoarseDesc *descs, int cin,
+                                                             int cout)
+{
+    Cell<float> c;
+    if (!locate(L, xcd_block(total), c)) return;
+    exact_xy(c.q->a[cin], c.q->a[cout], descs[c.it->p].qp, c.q->n, c.lb);
+}
+
+__global__ void __launch_bounds__(kPlaneThreads) kb_exact_z(BL<float> L, int total, const CoarseDesc *descs, int cw)
+{
+    Cell<float> c;
+    if (!locate(L, xcd_block(total), c)) return;
+    const CoarseDesc &D = descs[c.it->p];
+    exact_z(c.q->a[cw], D.qp, D.invp, c.q->n, c.lb);
 }
 
 } // namespace batch

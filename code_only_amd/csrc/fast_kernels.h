@@ -794,53 +794,73 @@ inline void hartley_basis(int n, double h, double *Q, double *lam)
 // the symmetric orthonormal Hartley basis Q diagonalises the periodic second difference along
 // each axis, so u = Q (Lambda^-1 (Q f)) with Lambda = (lam_i + lam_j) + lam_k and the zero mode set
 // to zero (the zero-mean solution).  Three axis passes serve as forward and inverse transform.
-// The grid sits in LDS with row pitch n + 1; in each axis pass every grid line goes to
-// kCoarseThreads / n^2 threads, which read the whole line into registers and then each write
-// every (that many)-th entry of Q times it, accumulated k = 0 .. n-1 in order.
-__device__ inline void ce_pass(float *g, const float *Q, int n, int axis)
+// The grid sits in LDS with row pitch n + 1.  Thread t works on grid line t / 4 and owns the
+// outputs m = t % 4 + 4 r (r < 4) of every line it is given: its four rows of Q stay in registers
+// for all six passes, and the four threads of a line sit in adjacent lanes, so the line's reads
+// are broadcasts.  Each output is accumulated k = 0 .. n-1 in order with fused multiply-adds.
+constexpr int kCeParts = 4;
+static_assert(kCoarseMaxN * kCoarseMaxN * kCeParts <= kCoarseThreads, "one thread per (line, part)");
+static_assert(kCoarseMaxN <= 4 * kCeParts, "four outputs per thread");
+
+__device__ inline void ce_basis(float (&q)[4][kCoarseMaxN], const float *Q, int n)
 {
-    const int P = n + 1, nlines = n * n, parts = kCoarseThreads / nlines;
-    const int t = threadIdx.x, line = t % nlines, part = t / nlines;
+    const int part = threadIdx.x % kCeParts;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int m = part + kCeParts * r;
+#pragma unroll
+        for (int k = 0; k < kCoarseMaxN; ++k) q[r][k] = (m < n && k < n) ? Q[m * n + k] : 0.f;
+    }
+}
+
+__device__ inline void ce_pass(float *g, const float (&q)[4][kCoarseMaxN], int n, int axis)
+{
+    const int P = n + 1;
+    const int line = threadIdx.x / kCeParts, part = threadIdx.x % kCeParts;
     const int a = line % n, b = line / n;
     int base, stride;
     if (axis == 0) { base = (b * n + a) * P; stride = 1; }        // line (j, k) = (a, b)
     else if (axis == 1) { base = b * n * P + a; stride = P; }     // line (i, k)
     else { base = b * P + a; stride = n * P; }                    // line (i, j)
-    const bool act = part < parts;
+    const bool act = line < n * n;
     float x[kCoarseMaxN];
-    if (act) {
 #pragma unroll
-        for (int k = 0; k < kCoarseMaxN; ++k) x[k] = k < n ? g[base + k * stride] : 0.f;
-    }
+    for (int k = 0; k < kCoarseMaxN; ++k) x[k] = (act && k < n) ? g[base + k * stride] : 0.f;
     __syncthreads();
     if (act) {
-        for (int m = part; m < n; m += parts) {
-            const float *q = Q + m * n;
-            float acc = 0.f;
 #pragma unroll
-            for (int k = 0; k < kCoarseMaxN; ++k)
-                if (k < n) acc += q[k] * x[k];
-            g[base + m * stride] = acc;
+        for (int r = 0; r < 4; ++r) {
+            const int m = part + kCeParts * r;
+            if (m < n) {
+                float acc = 0.f;
+#pragma unroll
+                for (int k = 0; k < kCoarseMaxN; ++k)
+                    if (k < n) acc = __builtin_fmaf(q[r][k], x[k], acc);
+                g[base + m * stride] = acc;
+            }
         }
     }
     __syncthreads();
 }
 
 // u_out := exact solution for rhs_in (dense n^3 arrays, i fastest).  g: n^2 (n + 1) floats of LDS,
-// s: n^2 + n floats of LDS for the basis; Qg, lamg: basis and eigenvalues (any address space).
+// s: n^2 + n floats of LDS for the basis (staged there once, then read into registers as
+// broadcasts) and the eigenvalues; Qg, lamg: basis and eigenvalues (any address space).
 __device__ inline void coarse_exact(float *g, float *s, const float *Qg, const float *lamg, const float *rhs_in,
                                     float *u_out, int n)
 {
     const int P = n + 1, n3 = n * n * n;
-    float *Q = s, *lam = s + n * n;
-    for (int t = threadIdx.x; t < n * n; t += kCoarseThreads) Q[t] = Qg[t];
+    float *Qs = s, *lam = s + n * n;
+    for (int t = threadIdx.x; t < n * n; t += kCoarseThreads) Qs[t] = Qg[t];
     for (int t = threadIdx.x; t < n; t += kCoarseThreads) lam[t] = lamg[t];
     for (int c = threadIdx.x; c < n3; c += kCoarseThreads) {
         const int i = c % n, j = (c / n) % n, k = c / (n * n);
         g[(k * n + j) * P + i] = rhs_in[c];
     }
     __syncthreads();
-    for (int a = 0; a < 3; ++a) ce_pass(g, Q, n, a);
+    float q[4][kCoarseMaxN];
+    ce_basis(q, Qs, n);
+    for (int a = 0; a < 3; ++a) ce_pass(g, q, n, a);
     for (int c = threadIdx.x; c < n3; c += kCoarseThreads) {
         const int i = c % n, j = (c / n) % n, k = c / (n * n);
         const float den = (lam[i] + lam[j]) + lam[k];
@@ -848,7 +868,7 @@ __device__ inline void coarse_exact(float *g, float *s, const float *Qg, const f
         v = den != 0.f ? v / den : 0.f;
     }
     __syncthreads();
-    for (int a = 0; a < 3; ++a) ce_pass(g, Q, n, a);
+    for (int a = 0; a < 3; ++a) ce_pass(g, q, n, a);
     for (int c = threadIdx.x; c < n3; c += kCoarseThreads) {
         const int i = c % n, j = (c / n) % n, k = c / (n * n);
         u_out[c] = g[(k * n + j) * P + i];

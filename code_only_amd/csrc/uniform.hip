@@ -18,6 +18,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "exact_planes.h"
 #include "fast_kernels.h"
 #include "sg_common.h"
 #include "uniform_kernels.h"
@@ -74,6 +75,8 @@ struct sg_uniform {
     float tune_ms[kMaxLev][32] = {};
     CoarseLevels coarse;       // levels held by k_coarse, with the exact-solve basis of the first
     int lc = 0;                // first level of <= 16^3 cells
+    int lb = 0;                // bottom level of the exact-coarse cycle (lc, or a level of <= 64^3)
+    float *d_qx = nullptr, *d_invx = nullptr;  // Hartley basis of level lb, 1 / Lambda (planes)
     double *psi_dev = nullptr, *src_dev = nullptr;   // staging for the host-pointer entry point
     double *adj_src = nullptr, *adj_part = nullptr;  // adjoint: projected source, mean partials
     double *psi_alt = nullptr;                         // second phi buffer of the fused outer step
@@ -197,11 +200,10 @@ template <typename T> static int launch_interp(hipStream_t s, T *uf, const T *uc
 // storage here; every kernel reads the same values as in the reference.
 // resmax: receives max|resh2| of the last smoothing call (the one mg_cycle's test reads).
 // exact (FP32 only, and only when the first level of <= 16^3 cells lies below level 0): the levels
-// end at that level, solved exactly by the coarse kernel where the reference sweeps its coarsest
-// level (mg_slvsml, mg.F90:540-546) -- the fused path's coarse-grid solve, so both paths keep
-// the same values.
-static int launch_coarse_k(sg_uniform *c, bool exact, const float *rhs, float *u, int mode, int ncycle, int npre,
-                           int npost, double *resmax);
+// end at level lb (the first level below level 0 with <= 64^3 cells, or with <= 16^3), solved
+// exactly (exact_bottom) where the reference sweeps its coarsest level (mg_slvsml,
+// mg.F90:540-546) -- the fused path's coarse-grid solve, so both paths keep the same values.
+static int exact_bottom(sg_uniform *c, int mode, int ncycle, int npre, int npost);
 
 template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax,
                                         bool exact = false)
@@ -209,10 +211,10 @@ template <typename T> static int mg_lin(sg_uniform *c, int ncycle, int npre, int
     Levels<T> &A = levels<T>(c);
     hipStream_t s = c->stream;
     exact = exact && std::is_same<T, float>::value && c->coarse.exact;
-    const int lmax = exact ? c->lc : c->nlev - 1;
+    const int lmax = exact ? c->lb : c->nlev - 1;
     auto bottom = [&]() -> int {
         if constexpr (std::is_same<T, float>::value) {
-            if (exact) return launch_coarse_k(c, true, A.RHO[lmax], A.U[lmax], 0, ncycle, npre, npost, nullptr);
+            if (exact) return exact_bottom(c, 0, ncycle, npre, npost);
         }
         for (int it = 0; it < 2; ++it)
             SG_TRY(launch_relax<T>(s, A.U[lmax], A.RHO[lmax], c->nl[lmax], c->hl[lmax], nullptr));
@@ -443,6 +445,27 @@ static int coarse_levels(sg_uniform *c)
         for (int e = 0; e < m; ++e) L.lam[e] = (float)lam[e];
     }
     c->lc = lc;
+    // the exact-coarse cycle ends at the first level below level 0 with <= 64^3 cells; with more
+    // than 16^3 cells it is solved by the plane-parallel kernels, else by k_coarse
+    int lb = lc;
+    if (lc > 0) {
+        int l = 1;
+        while (c->nl[l] > kPlaneMaxN) ++l;
+        if (c->nl[l] > kCoarseMaxN) lb = l;
+    }
+    c->lb = lb;
+    if (lb != lc) {
+        const int m = c->nl[lb];
+        SG_REQUIRE(m % 4 == 0 && m <= kPlaneMaxN, "plane-parallel exact solve needs n %% 4 == 0, n <= 64");
+        std::vector<double> Q((size_t)m * m), lam(m);
+        hartley_basis(m, c->hl[lb], Q.data(), lam.data());
+        std::vector<float> Qf(Q.begin(), Q.end()), inv((size_t)m * m * m);
+        exact_inverse_table(m, lam.data(), inv.data());
+        SG_HIP(hipMalloc(&c->d_qx, sizeof(float) * Qf.size()));
+        SG_HIP(hipMalloc(&c->d_invx, sizeof(float) * inv.size()));
+        SG_HIP(hipMemcpy(c->d_qx, Qf.data(), sizeof(float) * Qf.size(), hipMemcpyHostToDevice));
+        SG_HIP(hipMemcpy(c->d_invx, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice));
+    }
     return SG_OK;
 }
 
@@ -459,24 +482,45 @@ static int launch_coarse_k(sg_uniform *c, bool exact, const float *rhs, float *u
     return SG_OK;
 }
 
+// U[lb] := exact solution for RHO[lb] (FP32): k_coarse when lb is the first level of <= 16^3
+// cells, else the three plane-parallel launches with U1[lb] as the intermediate array.
+static int exact_bottom(sg_uniform *c, int mode, int ncycle, int npre, int npost)
+{
+    const int lb = c->lb, n = c->nl[lb];
+    SG_REQUIRE(c->coarse.exact, "exact coarse solve needs a level below the finest");
+    if (lb == c->lc) return launch_coarse_k(c, true, c->f.RHO[lb], c->f.U[lb], mode, ncycle, npre, npost, nullptr);
+    SG_REQUIRE(n <= kPlaneMaxN && c->d_qx && c->d_invx && c->U1[lb], "plane-parallel exact solve not set up");
+    hipLaunchKernelGGL(k_exact_xy, dim3(n), dim3(kPlaneThreads), 0, c->stream, (const float *)c->f.RHO[lb], c->U1[lb],
+                       (const float *)c->d_qx, n);
+    SG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_exact_z, dim3(n), dim3(kPlaneThreads), 0, c->stream, c->U1[lb], (const float *)c->d_qx,
+                       (const float *)c->d_invx, n);
+    SG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_exact_xy, dim3(n), dim3(kPlaneThreads), 0, c->stream, (const float *)c->U1[lb], c->f.U[lb],
+                       (const float *)c->d_qx, n);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
 // mg_lin (mg.F90:106-218) in FP32 with fused kernels: the levels of <= 16^3 cells run inside one
 // single-workgroup kernel (k_coarse); each fine-level visit of a V-cycle is npre + npost fused
 // sweeps, the last pre-sweep carrying the residual and its restriction, the first post-sweep the
-// prolongated correction.  Same values as mg_lin<float>, bit for bit (with the same `exact`: the
-// first level of <= 16^3 cells solved exactly instead of the V-cycle below it).  *u0 receives the
-// buffer that holds the level-0 correction.
+// prolongated correction.  With `exact` the cycle instead ends at level lb, solved exactly
+// (exact_bottom).  Same values as mg_lin<float> with the same `exact`, bit for bit.  *u0 receives
+// the buffer that holds the level-0 correction.
 static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax, const float **u0,
                        bool exact)
 {
     SG_REQUIRE(npre >= 1 && npost >= 1, "fused path needs npre, npost >= 1");
     Levels<float> &A = c->f;
     hipStream_t s = c->stream;
-    const int lc = c->lc;
     exact = exact && c->coarse.exact;
+    const int lc = exact ? c->lb : c->lc;   // level of the coarse-grid solve
     int cur[kMaxLev] = {};
     auto buf = [&](int l, int b) -> float * { return b ? c->U1[l] : A.U[l]; };
     for (int l = 1; l <= lc; ++l) SG_TRY(launch_restrict<float>(s, A.RHO[l], A.RHO[l - 1], c->nl[l]));
-    SG_TRY(launch_coarse_k(c, exact, A.RHO[lc], A.U[lc], 0, ncycle, npre, npost, lc == 0 ? resmax : nullptr));
+    if (exact) SG_TRY(exact_bottom(c, 0, ncycle, npre, npost));
+    else SG_TRY(launch_coarse_k(c, false, A.RHO[lc], A.U[lc], 0, ncycle, npre, npost, lc == 0 ? resmax : nullptr));
     cur[lc] = 0;
     for (int ml = lc - 1; ml >= 0; --ml) {
         SG_TRY(launch_interp_cubic_tile(s, A.U[ml], buf(ml + 1, cur[ml + 1]), c->nl[ml]));
@@ -494,7 +538,8 @@ static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *r
                     r -= k;
                 }
             }
-            SG_TRY(launch_coarse_k(c, exact, A.RHO[lc], A.U[lc], 1, ncycle, npre, npost, nullptr));
+            if (exact) SG_TRY(exact_bottom(c, 1, ncycle, npre, npost));
+            else SG_TRY(launch_coarse_k(c, false, A.RHO[lc], A.U[lc], 1, ncycle, npre, npost, nullptr));
             cur[lc] = 0;
             for (int v = lc - 1; v >= ml; --v) {
                 // npost sweeps; the first launch adds the prolongated correction
@@ -723,6 +768,8 @@ static void destroy(sg_uniform *c)
         if (c->U1[l]) (void)hipFree(c->U1[l]);
     if (c->psi_dev) (void)hipFree(c->psi_dev);
     if (c->psi_alt) (void)hipFree(c->psi_alt);
+    if (c->d_qx) (void)hipFree(c->d_qx);
+    if (c->d_invx) (void)hipFree(c->d_invx);
     if (c->src_dev) (void)hipFree(c->src_dev);
     if (c->adj_src) (void)hipFree(c->adj_src);
     if (c->adj_part) (void)hipFree(c->adj_part);
