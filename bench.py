@@ -291,7 +291,8 @@ def main():
     tiles = {}
     dims = [(32, 16, 16, 1024), (16, 8, 8, 1024), (16, 8, 8, 256), (64, 8, 8, 512), (8, 8, 8, 512)]
     sdims = [(64, 16, 32, 512, 1), (64, 16, 32, 512, 2), (64, 16, 16, 512, 2), (32, 16, 32, 256, 2),
-             (64, 32, 32, 512, 2)]
+             (64, 32, 32, 512, 2), (64, 16, 32, 256, 1), (64, 32, 32, 512, 1), (128, 16, 32, 512, 1),
+             (32, 16, 32, 256, 1)]
     names = ["x".join(map(str, d[:3])) + f"/{d[3]}" for d in dims] + \
             ["stream " + "x".join(map(str, d[:2])) + f" z{d[2]}/{d[3]}/{d[4]}pl" for d in sdims]
     lev, m = 0, n
@@ -309,7 +310,7 @@ def main():
             if ms[q] > 0:
                 cand[key] = round(ms[q] * 1e3, 1)
         tiles[str(m)] = {"shape": names[cfg.value], "sweeps_per_launch": ns.value, "source_in_lds": bool(fs.value),
-                         "us_per_2_sweeps": cand}
+                         "us_per_vcycle_visit": cand}
         lev += 1
         m //= 2
     log(f"[bench] autotuned tiles {json.dumps(tiles)}")

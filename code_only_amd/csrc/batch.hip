@@ -359,6 +359,10 @@ int op_sweep_cfg(sg_batch *c, const Pairs &pl, int cfg, int ns, bool fs, int cin
     case kNumTiles + 2: stream_variant<Stream2>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case kNumTiles + 3: stream_variant<Stream3>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case kNumTiles + 4: stream_variant<Stream4>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 5: stream_variant<Stream5>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 6: stream_variant<Stream6>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 7: stream_variant<Stream7>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
+    case kNumTiles + 8: stream_variant<Stream8>(c, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case 0: sweep_ns<Tile0>(c, ns, fs, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case 1: sweep_ns<Tile1>(c, ns, fs, L, tot, cin, cout, cuc, zero, prolong, resid); break;
     case 2: sweep_ns<Tile2>(c, ns, fs, L, tot, cin, cout, cuc, zero, prolong, resid); break;
@@ -640,10 +644,29 @@ int autotune(sg_batch *c)
             for (auto &q : Mv) fits = fits && cfg_fits(cfg, c->h_pl32[q.first * kBLev + v].n);
             c->tune_ms[v][cand] = -1.f;
             if (!fits) continue;
-            // record this launch's item list into a scratch plan, upload, time it
+            // one fine-level visit of a V(2,2) cycle in the launches the cycle makes: two sweeps
+            // ending with the residual restriction, two starting with the prolongated correction.
+            // Record its item lists into a scratch plan, upload, time it.
+            int flip = 0;
+            auto visit = [&]() -> int {
+                auto one = [&](int k, bool prolong, bool resid) {
+                    const int r = op_sweep_cfg(c, Mv, cfg, k, fs, flip ? CU1 : CU, flip ? CU : CU1, CU, false, prolong,
+                                               resid);
+                    flip ^= 1;
+                    return r;
+                };
+                if (ns == 2) {
+                    SG_TRY(one(2, false, true));
+                    return one(2, true, false);
+                }
+                SG_TRY(one(1, false, false));
+                SG_TRY(one(1, false, true));
+                SG_TRY(one(1, true, false));
+                return one(1, false, false);
+            };
             tmp = Plan();
             c->recording = true;
-            rc = op_sweep_cfg(c, Mv, cfg, ns, fs, CU, CU1, CU, false, false, false);
+            rc = visit();
             c->recording = false;
             if (rc != SG_OK) break;
             if (hipMalloc(&tmp.d_items, sizeof(Item) * tmp.items.size()) != hipSuccess ||
@@ -652,13 +675,11 @@ int autotune(sg_batch *c)
                 rc = SG_ERR_HIP;
                 break;
             }
-            const int launches = 2 / ns, reps = 3;
-            int flip = 0;
-            for (int r = 0; r < (reps + 1) * launches && rc == SG_OK; ++r) {
-                if (r == launches && hipEventRecord(a, c->stream) != hipSuccess) rc = SG_ERR_HIP;
+            const int reps = 3;  // timed visits, after one untimed
+            for (int r = 0; r < reps + 1 && rc == SG_OK; ++r) {
+                if (r == 1 && hipEventRecord(a, c->stream) != hipSuccess) rc = SG_ERR_HIP;
                 c->cursor = 0;
-                if (rc == SG_OK) rc = op_sweep_cfg(c, Mv, cfg, ns, fs, flip ? CU1 : CU, flip ? CU : CU1, CU, false, false, false);
-                flip ^= 1;
+                if (rc == SG_OK) rc = visit();
             }
             float ms = 0.f;
             if (rc == SG_OK && (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||

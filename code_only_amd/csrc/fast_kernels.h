@@ -629,20 +629,28 @@ k_sweep_stream(const float *__restrict__ uin, float *__restrict__ uout, const fl
 }
 
 // Streaming shapes the autotuners choose from beside the tiles (one sweep per launch, source always
-// staged): column SX x SY, chunk of ZC planes, NT threads.
+// staged): column SX x SY, chunk of ZC planes, NT threads, NP planes per step.
 using Stream0 = StreamShape<64, 16, 32, 512, 1>;
 using Stream1 = StreamShape<64, 16, 32, 512, 2>;
 using Stream2 = StreamShape<64, 16, 16, 512, 2>;
 using Stream3 = StreamShape<32, 16, 32, 256, 2>;
 using Stream4 = StreamShape<64, 32, 32, 512, 2>;
-constexpr int kNumStream = 5;
+using Stream5 = StreamShape<64, 16, 32, 256, 1>;
+using Stream6 = StreamShape<64, 32, 32, 512, 1>;
+using Stream7 = StreamShape<128, 16, 32, 512, 1>;
+using Stream8 = StreamShape<32, 16, 32, 256, 1>;
+constexpr int kNumStream = 9;
 static const int kStreamDims[kNumStream][5] = {
-    {64, 16, 32, 512, 1}, {64, 16, 32, 512, 2}, {64, 16, 16, 512, 2}, {32, 16, 32, 256, 2}, {64, 32, 32, 512, 2}};
+    {64, 16, 32, 512, 1}, {64, 16, 32, 512, 2}, {64, 16, 16, 512, 2}, {32, 16, 32, 256, 2}, {64, 32, 32, 512, 2},
+    {64, 16, 32, 256, 1}, {64, 32, 32, 512, 1}, {128, 16, 32, 512, 1}, {32, 16, 32, 256, 1}};
 static_assert(stream_fits_lds<Stream4, true, false>() && stream_fits_lds<Stream4, false, true>(), "stream LDS");
+static_assert(stream_fits_lds<Stream6, true, false>() && stream_fits_lds<Stream6, false, true>(), "stream LDS");
+static_assert(stream_fits_lds<Stream7, true, false>() && stream_fits_lds<Stream7, false, true>(), "stream LDS");
 
 // sweep configurations: 0 .. kNumTiles-1 tiles, kNumTiles .. kNumTiles+kNumStream-1 streams
 constexpr int kNumCfg = kNumTiles + kNumStream;
 constexpr int kNumCand = kNumTiles * kMaxSweepsPerLaunch * 2 + kNumStream;
+static_assert(kNumCand <= 32, "autotuner timing tables hold 32 candidates");
 
 inline bool cfg_fits(int cfg, int n)
 {

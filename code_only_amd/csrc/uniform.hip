@@ -316,6 +316,10 @@ static int launch_sweep_cfg(hipStream_t s, int cfg, int ns, bool fs, const float
     case kNumTiles + 2: stream_variant<Stream2>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case kNumTiles + 3: stream_variant<Stream3>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case kNumTiles + 4: stream_variant<Stream4>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 5: stream_variant<Stream5>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 6: stream_variant<Stream6>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 7: stream_variant<Stream7>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
+    case kNumTiles + 8: stream_variant<Stream8>(s, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 0: sweep_ns<Tile0>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 1: sweep_ns<Tile1>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
     case 2: sweep_ns<Tile2>(s, ns, fs, uin, uout, f, n, h2, h2i, zero, prolong, resid, uc, rc, resmax); break;
@@ -356,8 +360,9 @@ static int launch_interp_cubic_tile(hipStream_t s, float *uf, const float *uc, i
 }
 
 // Autotuner: for every fine level, time each fitting (tile shape, sweeps per launch) on the
-// level's own arrays as ms per two sweeps, and keep the fastest.  Runs once per context; values do
-// not depend on the choice.
+// level's own arrays as ms per fine-level visit of a V(2,2) cycle -- two sweeps ending with the
+// residual restriction, two starting with the prolongated correction, in the launches the cycle
+// makes -- and keep the fastest.  Runs once per context; values do not depend on the choice.
 static int autotune_tiles(sg_uniform *c)
 {
     hipEvent_t a = nullptr, b = nullptr;
@@ -368,7 +373,7 @@ static int autotune_tiles(sg_uniform *c)
         const int n = c->nl[l];
         c->tile_cfg[l] = -1;
         c->tile_ns[l] = 1;
-        if (n <= kCoarseMaxN) continue;
+        if (n <= kCoarseMaxN || l + 1 >= c->nlev) continue;
         float best = 1e30f;
         for (int cand = 0; cand < kNumCand && rc == SG_OK; ++cand) {
             int cfg, ns;
@@ -376,33 +381,40 @@ static int autotune_tiles(sg_uniform *c)
             cand_decode(cand, cfg, ns, fs);
             c->tune_ms[l][cand] = -1.f;
             if (!cfg_fits(cfg, n) || !cfg_ok(cfg, ns, fs)) continue;
-            {
-                const int launches = 2 / ns;             // two sweeps
-                const int reps = n >= 256 ? 3 : 6;       // timed repetitions of two sweeps
-                int flip = 0;
-                for (int r = 0; r < (reps + 1) * launches && rc == SG_OK; ++r) {
-                    if (r == launches && hipEventRecord(a, c->stream) != hipSuccess) rc = SG_ERR_HIP;
-                    if (rc == SG_OK)
-                        rc = launch_sweep_cfg(c->stream, cfg, ns, fs, flip ? c->U1[l] : c->f.U[l],
-                                              flip ? c->f.U[l] : c->U1[l], c->f.RHO[l], n, c->hl[l], false, false,
-                                              false, nullptr, nullptr, nullptr);
-                    flip ^= 1;
+            const int reps = n >= 256 ? 3 : 6;  // timed visits, after one untimed
+            int flip = 0;
+            auto launch = [&](int k, bool prolong, bool resid) {
+                const int r = launch_sweep_cfg(c->stream, cfg, k, fs, flip ? c->U1[l] : c->f.U[l],
+                                               flip ? c->f.U[l] : c->U1[l], c->f.RHO[l], n, c->hl[l], false, prolong,
+                                               resid, prolong ? c->f.U[l + 1] : nullptr,
+                                               resid ? c->f.RHO[l + 1] : nullptr, nullptr);
+                flip ^= 1;
+                return r;
+            };
+            for (int r = 0; r < reps + 1 && rc == SG_OK; ++r) {
+                if (r == 1 && hipEventRecord(a, c->stream) != hipSuccess) rc = SG_ERR_HIP;
+                if (rc == SG_OK && ns == 1) {
+                    if ((rc = launch(1, false, false)) == SG_OK && (rc = launch(1, false, true)) == SG_OK &&
+                        (rc = launch(1, true, false)) == SG_OK)
+                        rc = launch(1, false, false);
+                } else if (rc == SG_OK) {
+                    if ((rc = launch(2, false, true)) == SG_OK) rc = launch(2, true, false);
                 }
-                if (rc != SG_OK) break;
-                float ms = 0.f;
-                if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-                    hipEventElapsedTime(&ms, a, b) != hipSuccess) {
-                    rc = SG_ERR_HIP;
-                    break;
-                }
-                ms /= reps;
-                c->tune_ms[l][cand] = ms;
-                if (ms < best) {
-                    best = ms;
-                    c->tile_cfg[l] = cfg;
-                    c->tile_ns[l] = ns;
-                    c->tile_fs[l] = fs;
-                }
+            }
+            if (rc != SG_OK) break;
+            float ms = 0.f;
+            if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                rc = SG_ERR_HIP;
+                break;
+            }
+            ms /= reps;
+            c->tune_ms[l][cand] = ms;
+            if (ms < best) {
+                best = ms;
+                c->tile_cfg[l] = cfg;
+                c->tile_ns[l] = ns;
+                c->tile_fs[l] = fs;
             }
         }
     }

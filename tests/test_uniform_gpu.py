@@ -236,7 +236,7 @@ def test_exact_coarse_solve_contracts_at_least_as_well(gpu, n):
     s.close()
 
 
-STREAM_CFGS = [5, 6, 7, 8, 9]   # streaming sweep shapes (one or two planes per step)
+STREAM_CFGS = list(range(5, 14))   # streaming sweep shapes (one or two planes per step)
 
 
 @pytest.mark.parametrize("n", [32, 64, 128])
