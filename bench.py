@@ -290,11 +290,11 @@ def main():
 
     tiles = {}
     dims = [(32, 16, 16, 1024), (16, 8, 8, 1024), (16, 8, 8, 256), (64, 8, 8, 512), (8, 8, 8, 512)]
-    sdims = [(64, 16, 32, 512, 1), (64, 16, 32, 512, 2), (64, 16, 16, 512, 2), (32, 16, 32, 256, 2),
-             (64, 32, 32, 512, 2), (64, 16, 32, 256, 1), (64, 32, 32, 512, 1), (128, 16, 32, 512, 1),
-             (32, 16, 32, 256, 1)]
+    # streaming shapes: (SX, SY, ZC, threads)
+    sdims = [(64, 16, 32, 512), (64, 16, 16, 512), (64, 32, 32, 512), (32, 16, 32, 256), (128, 16, 32, 512),
+             (64, 16, 32, 256), (64, 8, 32, 256), (32, 32, 32, 512), (64, 16, 64, 512)]
     names = ["x".join(map(str, d[:3])) + f"/{d[3]}" for d in dims] + \
-            ["stream " + "x".join(map(str, d[:2])) + f" z{d[2]}/{d[3]}/{d[4]}pl" for d in sdims]
+            ["stream " + "x".join(map(str, d[:2])) + f" z{d[2]}/{d[3]}" for d in sdims]
     lev, m = 0, n
     while m > 16:
         cfg, ns, fs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -376,32 +376,42 @@ k_sweep_tile(const float *__restrict__ uin, float *__restrict__ uout, const floa
 
 // ------------------------------------------------------------------------------------------------
 // Streaming form of the fused sweep: a workgroup owns an SX x SY column of the grid and a chunk of
-// ZC planes, and marches up the chunk with the planes it needs in an LDS ring.  Step p updates
-// the first colour (i+j+k odd, the reference's ipass 1) of plane p and then the second colour of
-// plane p-1, whose first-colour neighbours on planes p-2, p-1 and p are then all new; plane p-1 is
-// final and is stored.  The next plane's u and f are loaded into registers before the step's
-// arithmetic and written into the ring after it, so global latency overlaps LDS work.  Halo: H = 2
-// cells in x and y (first colour on margin 1, second on margin 2), H planes below and above the
-// chunk; with RESID, H = 3 and the residual of each final plane is formed one step later and
-// restricted pairwise (restriction.F90:132-147 operand order).  Every cell update sees the same
-// operands as the two-launch colour passes, so the values are those of the tile kernel, bit for bit.
+// ZC planes, and marches up the chunk with the planes it needs in an LDS ring of R planes (u and
+// f).  Step p:
+//   1. first colour (i+j+k odd, the reference's ipass 1) of plane p, on the column with margin 1;
+//   -- barrier --
+//   2. plane p + 2 (fetched into registers one step earlier, with the trilinear correction added
+//      for PROLONG) into the slot of plane p - 3 (p - 4 with RESID), plane p + 3 fetched into the
+//      registers; second colour of plane p - 1, margin 2, whose first-colour neighbours on planes
+//      p - 2, p - 1 and p are all new: plane p - 1 is final;
+//   -- barrier --
+//   3. plane p - 1 stored; with RESID, the residual of plane p - 2 (final, as are p - 3 and
+//      p - 1), restricted pairwise (restriction.F90:132-147 operand order).
+// Phase 3 overlaps the next step's first colour, which touches none of its planes.  Every
+// thread's cells and global offsets are fixed for the whole march and computed once; the march is
+// unrolled by R, so the ring slots are immediate LDS offsets.  Halo: H = 2 cells in x and y, H
+// planes below and above the chunk (H = 3 with RESID).  max|resh2| is kept in FP32 only when
+// asked for (the max of the FP32 values, widened once, equals the max of their widened values).
+// Every cell update sees the same operands as the two-launch colour passes, so the values are
+// those of the tile kernel, bit for bit.
 // ------------------------------------------------------------------------------------------------
-template <int SX_, int SY_, int ZC_, int NT_, int NP_> struct StreamShape {
-    static constexpr int SX = SX_, SY = SY_, ZC = ZC_, NT = NT_, NP = NP_;  // NP planes per step
+template <int SX_, int SY_, int ZC_, int NT_> struct StreamShape {
+    static constexpr int SX = SX_, SY = SY_, ZC = ZC_, NT = NT_;
 };
 
 template <class SS, bool PROLONG, bool RESID> struct StreamGeom {
-    static constexpr int NP = SS::NP;
     static constexpr int H = RESID ? 3 : 2;
     static constexpr int PX = SS::SX + 2 * H, PY = SS::SY + 2 * H, PP = PX * PY;
-    // ring slots: the step's planes, the ones it reads below and the ones it prefetches
-    static constexpr int R = 2 * NP + (RESID ? 3 : 2);
+    static constexpr int R = RESID ? 6 : 5;                    // ring slots
     static constexpr int CX = PX / 2 + 2, CY = PY / 2 + 2, CP = CX * CY;  // staged coarse plane
-    static constexpr int IL = (PP + SS::NT - 1) / SS::NT;      // plane cells per thread
-    static constexpr int ICL = (CP + SS::NT - 1) / SS::NT;
-    static constexpr bool PAIRS = RESID && NP == 2;            // each step restricts a whole pair
-    static constexpr int lds_bytes =
-        4 * (2 * R * PP + (PROLONG ? 4 * CP : 0) + (RESID && !PAIRS ? SS::SX * SS::SY : 0));
+    static constexpr int CR = 8;                               // coarse ring slots
+    static constexpr int IL = (PP + SS::NT - 1) / SS::NT;      // ring cells per thread
+    static constexpr int ICL = (CP + SS::NT - 1) / SS::NT;     // coarse cells per thread
+    static constexpr int IS = (SS::SX * SS::SY + SS::NT - 1) / SS::NT;  // core cells per thread
+    static constexpr int RX1 = (PX - 2) / 2, RY1 = PY - 2, C1 = RX1 * RY1, I1 = (C1 + SS::NT - 1) / SS::NT;
+    static constexpr int RX2 = (PX - 4) / 2, RY2 = PY - 4, C2 = RX2 * RY2, I2 = (C2 + SS::NT - 1) / SS::NT;
+    static constexpr int lds_bytes = 4 * (2 * R * PP + (PROLONG ? CR * CP : 0) + (RESID ? SS::SX * SS::SY : 0));
+    static_assert(IL <= 32 && I1 <= 32 && I2 <= 32 && IS <= 32, "per-thread bit masks hold 32 cells");
 };
 
 template <class SS, bool PROLONG, bool RESID> constexpr bool stream_fits_lds()
@@ -412,209 +422,240 @@ template <class SS, bool PROLONG, bool RESID> constexpr bool stream_fits_lds()
 __device__ __forceinline__ int pmod(int a, int m) { const int r = a % m; return r < 0 ? r + m : r; }
 __device__ __forceinline__ int fdiv2(int a) { return a >> 1; }  // floor(a / 2), arithmetic shift
 
+// Cells of one colour pass (margin M) owned by this thread: base offset y PX + M + 2 xh (the cell
+// is base + (y + k) % 2 on plane k, since x0, y0 and colour + M are even), y parity, and for
+// either x parity whether the cell is in the column's core.
+template <int N> struct PassCells {
+    int cb[N];
+    unsigned yb = 0u, valid = 0u, core0 = 0u, core1 = 0u;
+};
+
+template <int N, int RX, int RY, int CNT, int NT, int PX, int M, int H, int SX, int SY>
+__device__ __forceinline__ void pass_cells(PassCells<N> &P, int tid)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int q = tid + i * NT;
+        const int xh = q % RX, y = M + q / RX;
+        P.cb[i] = y * PX + M + 2 * xh;
+        if (q < CNT) P.valid |= 1u << i;
+        if (y & 1) P.yb |= 1u << i;
+        const bool yc = y >= H && y < H + SY;
+        if (yc && M + 2 * xh >= H && M + 2 * xh < H + SX) P.core0 |= 1u << i;
+        if (yc && M + 2 * xh + 1 >= H && M + 2 * xh + 1 < H + SX) P.core1 |= 1u << i;
+    }
+}
+
 template <class SS, bool ZERO, bool PROLONG, bool RESID>
 __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, float *__restrict__ uout,
                                              const float *__restrict__ f, int n, float h2, float h2i,
                                              const float *__restrict__ uc, float *__restrict__ rc, double *resmax, int t)
 {
     using G = StreamGeom<SS, PROLONG, RESID>;
-    constexpr int SX = SS::SX, SY = SS::SY, ZC = SS::ZC, NT = SS::NT, NP = SS::NP;
-    constexpr int H = G::H, PX = G::PX, PY = G::PY, PP = G::PP, R = G::R, CX = G::CX, CP = G::CP;
-    static_assert(SX % 2 == 0 && SY % 2 == 0 && ZC % 2 == 0 && (NP == 1 || NP == 2), "stream shape");
+    constexpr int SX = SS::SX, SY = SS::SY, ZC = SS::ZC, NT = SS::NT;
+    constexpr int H = G::H, PX = G::PX, PY = G::PY, PP = G::PP, R = G::R, CX = G::CX, CP = G::CP, CR = G::CR;
+    static_assert(SX % 2 == 0 && SY % 2 == 0 && ZC % 2 == 0, "stream shape");
     __shared__ float su[R * PP];
     __shared__ float sf[R * PP];
-    __shared__ float scz[PROLONG ? 4 * CP : 1];
-    __shared__ float srp[RESID && !G::PAIRS ? SX * SY : 1];
+    __shared__ float scz[PROLONG ? CR * CP : 1];
+    __shared__ float srp[RESID ? SX * SY : 1];
     const int ntx = n / SX, nty = n / SY;
     const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
     const int x0 = tx * SX, y0 = ty * SY, z0 = tz * ZC;
     const int nc = n >> 1;
     const int tid = threadIdx.x;
-    double amax = 0.0;
+    const bool want = resmax != nullptr;
+    float amax = 0.f;
 
-    float ru[NP][G::IL], rf[NP][G::IL], rcz[PROLONG ? 2 : 1][PROLONG ? G::ICL : 1];
-    auto fetch = [&](int k, int slot) {
+    // ---- per-thread geometry, fixed for the whole march ----
+    int goff[G::IL];         // global (y, x) offset of ring cell tid + it NT
+    int coff[PROLONG ? G::IL : 1][4];  // its trilinear stencil in the staged coarse plane
+#pragma unroll
+    for (int it = 0; it < G::IL; ++it) {
+        const int q = tid + it * NT;
+        const int x = q % PX, y = q / PX;
+        goff[it] = wrapn(y0 - H + y, n) * n + wrapn(x0 - H + x, n);
+        if (PROLONG) {
+            const int ic = (x >> 1) + 1, jc = (y >> 1) + 1;
+            const int ic1 = ic + 2 * (x & 1) - 1, jc1 = jc + 2 * (y & 1) - 1;
+            coff[PROLONG ? it : 0][0] = jc * CX + ic;
+            coff[PROLONG ? it : 0][1] = jc * CX + ic1;
+            coff[PROLONG ? it : 0][2] = jc1 * CX + ic;
+            coff[PROLONG ? it : 0][3] = jc1 * CX + ic1;
+        }
+    }
+    int cgoff[PROLONG ? G::ICL : 1];  // global (y, x) offset of coarse staged cell tid + it NT
+    if (PROLONG) {
+#pragma unroll
+        for (int it = 0; it < G::ICL; ++it) {
+            const int q = tid + it * NT;
+            const int x = q % CX, y = q / CX;
+            cgoff[PROLONG ? it : 0] = wrapn(fdiv2(y0 - H) - 1 + y, nc) * nc + wrapn(fdiv2(x0 - H) - 1 + x, nc);
+        }
+    }
+    PassCells<G::I1> P1;
+    PassCells<G::I2> P2;
+    pass_cells<G::I1, G::RX1, G::RY1, G::C1, NT, PX, 1, H, SX, SY>(P1, tid);
+    pass_cells<G::I2, G::RX2, G::RY2, G::C2, NT, PX, 2, H, SX, SY>(P2, tid);
+    int soff[G::IS], sl[G::IS];  // core cells: global (y, x) offset, ring offset
+#pragma unroll
+    for (int it = 0; it < G::IS; ++it) {
+        const int q = tid + it * NT;
+        const int x = q % SX, y = q / SX;
+        soff[it] = (y0 + y) * n + x0 + x;
+        sl[it] = (y + H) * PX + x + H;
+    }
+
+    float ru[G::IL], rf[G::IL], rcz[PROLONG ? G::ICL : 1];
+    auto fetch = [&](int k) {
+        const int base = wrapn(k, n) * n * n;
 #pragma unroll
         for (int it = 0; it < G::IL; ++it) {
-            const int q = tid + it * NT;
-            if (PP % NT == 0 || q < PP) {
-                const int x = q % PX, y = q / PX;
-                const int g = (wrapn(k, n) * n + wrapn(y0 - H + y, n)) * n + wrapn(x0 - H + x, n);
-                ru[slot][it] = ZERO ? 0.0f : uin[g];
-                rf[slot][it] = f[g];
+            if (PP % NT == 0 || tid + it * NT < PP) {
+                ru[it] = ZERO ? 0.0f : uin[base + goff[it]];
+                rf[it] = f[base + goff[it]];
             }
         }
     };
-    // coarse plane c of the staged region: coarse x from (x0 - H)/2 - 1, y from (y0 - H)/2 - 1
-    auto fetch_coarse = [&](int c, int slot) {
+    auto fetch_coarse = [&](int c) {
+        const int base = wrapn(c, nc) * nc * nc;
 #pragma unroll
-        for (int it = 0; it < G::ICL; ++it) {
-            const int q = tid + it * NT;
-            if (CP % NT == 0 || q < CP) {
-                const int x = q % CX, y = q / CX;
-                rcz[slot][it] =
-                    uc[(wrapn(c, nc) * nc + wrapn(fdiv2(y0 - H) - 1 + y, nc)) * nc + wrapn(fdiv2(x0 - H) - 1 + x, nc)];
-            }
-        }
+        for (int it = 0; it < G::ICL; ++it)
+            if (CP % NT == 0 || tid + it * NT < CP) rcz[PROLONG ? it : 0] = uc[base + cgoff[PROLONG ? it : 0]];
     };
-    auto put_coarse = [&](int c, int slot) {
-        float *dst = scz + pmod(c, 4) * CP;
+    auto put_coarse = [&](int c) {
+        float *dst = scz + (c & (CR - 1)) * CP;
 #pragma unroll
-        for (int it = 0; it < G::ICL; ++it) {
-            const int q = tid + it * NT;
-            if (CP % NT == 0 || q < CP) dst[q] = rcz[slot][it];
-        }
+        for (int it = 0; it < G::ICL; ++it)
+            if (CP % NT == 0 || tid + it * NT < CP) dst[tid + it * NT] = rcz[PROLONG ? it : 0];
     };
-    // write the fetched plane k into its ring slot (adding the trilinear correction with PROLONG)
+    // the fetched plane k into ring slot `slot` (adding the trilinear correction with PROLONG)
     auto put = [&](int k, int slot) {
-        float *du = su + pmod(k, R) * PP, *df = sf + pmod(k, R) * PP;
+        float *du = su + slot * PP, *df = sf + slot * PP;
         const int kc = fdiv2(k), kc1 = kc + 2 * (k & 1) - 1;
-        const float *c0 = scz + pmod(kc, 4) * CP, *c1 = scz + pmod(kc1, 4) * CP;
+        const float *c0 = scz + (kc & (CR - 1)) * CP, *c1 = scz + (kc1 & (CR - 1)) * CP;
 #pragma unroll
         for (int it = 0; it < G::IL; ++it) {
             const int q = tid + it * NT;
             if (PP % NT == 0 || q < PP) {
-                float v = ru[slot][it];
+                float v = ru[it];
                 if (PROLONG) {
-                    const int x = q % PX, y = q / PX;
-                    const int ic = (x >> 1) + 1, jc = (y >> 1) + 1;
-                    const int ic1 = ic + 2 * (x & 1) - 1, jc1 = jc + 2 * (y & 1) - 1;
-                    v = v + trilinear_value<float>(c0[jc * CX + ic], c0[jc * CX + ic1], c0[jc1 * CX + ic], c1[jc * CX + ic],
-                                                   c1[jc1 * CX + ic], c1[jc * CX + ic1], c0[jc1 * CX + ic1],
-                                                   c1[jc1 * CX + ic1]);
+                    const int *o = coff[PROLONG ? it : 0];
+                    v = v + trilinear_value<float>(c0[o[0]], c0[o[1]], c0[o[2]], c1[o[0]], c1[o[2]], c1[o[1]], c0[o[3]],
+                                                   c1[o[3]]);
                 }
                 du[q] = v;
-                df[q] = rf[slot][it];
+                df[q] = rf[it];
             }
         }
     };
-    // one colour of plane k on the region shrunk by margin M; colour 1: i+j+k odd
-    auto colour = [&](int k, int M, int col, bool out) {
-        const int RX = (PX - 2 * M) / 2, RY = PY - 2 * M, CNT = RX * RY;
-        float *u = su + pmod(k, R) * PP;
-        const float *um = su + pmod(k - 1, R) * PP, *up = su + pmod(k + 1, R) * PP;
-        const float *fk = sf + pmod(k, R) * PP;
-        for (int q = tid; q < CNT; q += NT) {
-            const int xh = q % RX, y = M + q / RX;
-            // parity of the global cell (x0 - H + x) + (y0 - H + y) + k must be col (odd for col 1)
-            const int d = (x0 + y0 + y + k + col + M) & 1;
-            const int x = M + 2 * xh + d;
-            const int c = y * PX + x;
-            float sres;
-            u[c] = relax_value<float>(u[c], u[c + 1], u[c - 1], u[c + PX], u[c - PX], up[c], um[c], fk[c], h2, &sres);
-            if (out && x >= H && x < H + SX && y >= H && y < H + SY) amax = fmax(amax, fabs((double)sres));
+    // one colour pass over the cells P on plane k: u (slot of k), um / up (k - 1 / k + 1), fk
+    auto colour = [&](auto &P, int k, float *u, const float *um, const float *up, const float *fk, bool out) {
+        constexpr int N = sizeof(P.cb) / sizeof(int);
+        const unsigned kb = (unsigned)(k & 1);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if (((P.valid >> i) & 1u) == 0u) continue;
+            const unsigned d = ((P.yb >> i) & 1u) ^ kb;
+            const int c = P.cb[i] + (int)d;
+            float s;
+            u[c] = relax_value<float>(u[c], u[c + 1], u[c - 1], u[c + PX], u[c - PX], up[c], um[c], fk[c], h2, &s);
+            if (want && out && ((((d ? P.core1 : P.core0) >> i) & 1u) != 0u)) amax = fmaxf(amax, fabsf(s));
         }
     };
-    auto resid_at = [&](int kq, int x, int y) {
-        const float *u = su + pmod(kq, R) * PP, *um = su + pmod(kq - 1, R) * PP, *up = su + pmod(kq + 1, R) * PP;
-        const float *fk = sf + pmod(kq, R) * PP;
+    auto resid_at = [&](int slot, int slotm, int slotp, int x, int y) {
+        const float *u = su + slot * PP, *um = su + slotm * PP, *up = su + slotp * PP;
+        const float *fk = sf + slot * PP;
         const int c = (y + H) * PX + x + H;
         return resid_value<float>(u[c], u[c + 1], u[c - 1], u[c + PX], u[c - PX], up[c], um[c], fk[c], h2i);
     };
-    // coarse cell (cx, cy) of the pair (kq - 1, kq), kq odd: 4 residuals of the even plane from
-    // `even` (or recomputed), 4 of the odd plane recomputed
-    auto restrict_pair = [&](int kq, bool from_srp) {
-        for (int q = tid; q < (SX / 2) * (SY / 2); q += NT) {
-            const int cx = q % (SX / 2), cy = q / (SX / 2);
-            float r[8];
-#pragma unroll
-            for (int o = 0; o < 8; ++o) {
-                const int a = o & 1, b = (o >> 1) & 1, dz = (o >> 2) & 1;
-                const int x = 2 * cx + a, y = 2 * cy + b;
-                r[o] = dz ? resid_at(kq, x, y) : (from_srp ? srp[y * SX + x] : resid_at(kq - 1, x, y));
-            }
-            const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy, gcz = wrapn(kq, n) / 2;
-            rc[(gcz * nc + gcy) * nc + gcx] = restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
-        }
-    };
+
     const int zlo = z0 - H, zhi = z0 + ZC + H - 1;  // planes held at some point
-    // prologue: planes zlo .. zlo + NP + 1 (and the coarse planes they need)
-    int cmax = fdiv2(zlo + NP + 1) + 1;  // highest coarse plane staged
+    // ---- prologue: coarse planes for fine planes zlo .. zlo + 3, fine planes zlo .. zlo + 2 into
+    // slots 0 .. 2, plane zlo + 3 (and the next coarse plane) into registers ----
+    auto cneed = [&](int k) { return fdiv2(k) + 1; };  // highest coarse plane put(k) reads
+    int cmax = 0;
     if (PROLONG) {
-        for (int c = fdiv2(zlo) - 1; c <= cmax; ++c) {
-            fetch_coarse(c, 0);
-            put_coarse(c, 0);
+        for (int c = fdiv2(zlo) - 1; c <= cneed(zlo + 3); ++c) {
+            fetch_coarse(c);
+            put_coarse(c);
         }
+        cmax = cneed(zlo + 3);
         __syncthreads();
     }
-    for (int k = zlo; k <= zlo + NP + 1; ++k) {
-        fetch(k, 0);
-        put(k, 0);
+    for (int k = zlo; k <= zlo + 2; ++k) {
+        fetch(k);
+        put(k, k - zlo);
+    }
+    fetch(zlo + 3);
+    bool cpend = false;  // a coarse plane (cmax + 1) waits in the registers
+    if (PROLONG && cneed(zlo + 4) > cmax) {
+        fetch_coarse(cmax + 1);
+        cpend = true;
     }
     __syncthreads();
-    for (int pa = zlo + 1; pa <= zhi - 1; pa += NP) {
-        // prefetch planes pa + NP + 1 .. pa + 2 NP (and new coarse planes) into registers
-        const int kn0 = pa + NP + 1;
-        int nnew = 0;
+    for (int pb = zlo + 1; pb <= zhi - 1; pb += R) {
 #pragma unroll
-        for (int j = 0; j < NP; ++j)
-            if (kn0 + j <= zhi) fetch(kn0 + j, j);
-        const int klast = kn0 + NP - 1 < zhi ? kn0 + NP - 1 : zhi;
-        const int cneed = PROLONG ? fdiv2(klast) + 1 : 0;
-        if (PROLONG) {
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                if (cmax + 1 + j <= cneed) {
-                    fetch_coarse(cmax + 1 + j, j);
-                    nnew = j + 1;
-                }
-        }
-        // first colour of planes pa .. pa + NP - 1 (margin 1)
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const int k = pa + j;
-            if (k <= zhi - 1) colour(k, 1, 1, k >= z0 && k < z0 + ZC);
-        }
-        __syncthreads();
-        // second colour of planes pa - 1 .. pa + NP - 2 (margin 2): final
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const int k = pa - 1 + j;
-            if (k >= zlo + 2 && k <= zhi - 2) colour(k, 2, 0, k >= z0 && k < z0 + ZC);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const int k = pa - 1 + j;
-            if (k >= z0 && k < z0 + ZC) {
-                const float *u = su + pmod(k, R) * PP;
-                const int gz = wrapn(k, n);
-                for (int q = tid; q < SX * SY; q += NT) {
-                    const int x = q % SX, y = q / SX;
-                    uout[(gz * n + y0 + y) * n + x0 + x] = u[(y + H) * PX + x + H];
-                }
-            }
-        }
-        if (RESID) {
-            bool any = false;
-#pragma unroll
-            for (int j = 0; j < NP; ++j) {
-                const int kq = pa - 2 + j;
-                if (kq < z0 || kq >= z0 + ZC) continue;
-                any = true;
-                if (G::PAIRS) {
-                    if (kq & 1) restrict_pair(kq, false);
-                } else if ((kq & 1) == 0) {
-                    for (int q = tid; q < SX * SY; q += NT) srp[q] = resid_at(kq, q % SX, q / SX);
-                } else {
-                    restrict_pair(kq, true);
-                }
-            }
-            if (any) __syncthreads();
-        }
-        if (PROLONG && nnew) {
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                if (j < nnew) put_coarse(cmax + 1 + j, j);
-            cmax += nnew;
+        for (int uu = 0; uu < R; ++uu) {
+            const int pa = pb + uu;
+            if (pa > zhi - 1) break;  // uniform per workgroup
+            // ring slot of plane pa + delta: (pa + delta - zlo) mod R, pa - zlo = 1 + uu (mod R)
+            auto S = [&](int delta) { return ((1 + uu + delta) % R + R) % R; };
+            // 1. first colour of plane pa (margin 1)
+            colour(P1, pa, su + S(0) * PP, su + S(-1) * PP, su + S(1) * PP, sf + S(0) * PP, pa >= z0 && pa < z0 + ZC);
             __syncthreads();
-        }
+            // 2. coarse plane for the next step's put, plane pa + 2 into its slot, fetches
+            if (PROLONG && cpend) {
+                put_coarse(cmax + 1);
+                ++cmax;
+                cpend = false;
+            }
+            if (pa + 2 <= zhi) put(pa + 2, S(2));
+            if (pa + 3 <= zhi) fetch(pa + 3);
+            if (PROLONG && pa + 3 <= zhi && cneed(pa + 4 <= zhi ? pa + 4 : zhi) > cmax) {
+                fetch_coarse(cmax + 1);
+                cpend = true;
+            }
+            //    second colour of plane pa - 1 (margin 2): final
+            if (pa - 1 >= zlo + 2 && pa - 1 <= zhi - 2)
+                colour(P2, pa - 1, su + S(-1) * PP, su + S(-2) * PP, su + S(0) * PP, sf + S(-1) * PP,
+                       pa - 1 >= z0 && pa - 1 < z0 + ZC);
+            __syncthreads();
+            // 3. store plane pa - 1; residual of plane pa - 2
+            if (pa - 1 >= z0 && pa - 1 < z0 + ZC) {
+                const float *u = su + S(-1) * PP;
+                const int gz = wrapn(pa - 1, n) * n * n;
 #pragma unroll
-        for (int j = 0; j < NP; ++j)
-            if (kn0 + j <= zhi) put(kn0 + j, j);
-        __syncthreads();
+                for (int it = 0; it < G::IS; ++it)
+                    if ((SX * SY) % NT == 0 || tid + it * NT < SX * SY) uout[gz + soff[it]] = u[sl[it]];
+            }
+            if (RESID) {
+                const int kq = pa - 2;
+                if (kq >= z0 && kq < z0 + ZC) {
+                    if ((kq & 1) == 0) {
+                        for (int q = tid; q < SX * SY; q += NT) srp[q] = resid_at(S(-2), S(-3), S(-1), q % SX, q / SX);
+                    } else {
+                        // coarse cell (cx, cy) of the pair (kq - 1, kq): 4 residuals of the even plane
+                        // from srp, 4 of the odd plane
+                        for (int q = tid; q < (SX / 2) * (SY / 2); q += NT) {
+                            const int cx = q % (SX / 2), cy = q / (SX / 2);
+                            float r[8];
+#pragma unroll
+                            for (int o = 0; o < 8; ++o) {
+                                const int a = o & 1, b = (o >> 1) & 1, dz = (o >> 2) & 1;
+                                const int x = 2 * cx + a, y = 2 * cy + b;
+                                r[o] = dz ? resid_at(S(-2), S(-3), S(-1), x, y) : srp[y * SX + x];
+                            }
+                            const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy, gcz = wrapn(kq, n) / 2;
+                            rc[(gcz * nc + gcy) * nc + gcx] =
+                                restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
+                        }
+                    }
+                }
+            }
+        }
     }
-    if (resmax) block_atomic_max(resmax, amax);
+    if (want) block_atomic_max(resmax, (double)amax);
 }
 
 template <class SS, bool ZERO, bool PROLONG, bool RESID>
@@ -629,23 +670,22 @@ k_sweep_stream(const float *__restrict__ uin, float *__restrict__ uout, const fl
 }
 
 // Streaming shapes the autotuners choose from beside the tiles (one sweep per launch, source always
-// staged): column SX x SY, chunk of ZC planes, NT threads, NP planes per step.
-using Stream0 = StreamShape<64, 16, 32, 512, 1>;
-using Stream1 = StreamShape<64, 16, 32, 512, 2>;
-using Stream2 = StreamShape<64, 16, 16, 512, 2>;
-using Stream3 = StreamShape<32, 16, 32, 256, 2>;
-using Stream4 = StreamShape<64, 32, 32, 512, 2>;
-using Stream5 = StreamShape<64, 16, 32, 256, 1>;
-using Stream6 = StreamShape<64, 32, 32, 512, 1>;
-using Stream7 = StreamShape<128, 16, 32, 512, 1>;
-using Stream8 = StreamShape<32, 16, 32, 256, 1>;
+// staged): column SX x SY, chunk of ZC planes, NT threads.
+using Stream0 = StreamShape<64, 16, 32, 512>;
+using Stream1 = StreamShape<64, 16, 16, 512>;
+using Stream2 = StreamShape<64, 32, 32, 512>;
+using Stream3 = StreamShape<32, 16, 32, 256>;
+using Stream4 = StreamShape<128, 16, 32, 512>;
+using Stream5 = StreamShape<64, 16, 32, 256>;
+using Stream6 = StreamShape<64, 8, 32, 256>;
+using Stream7 = StreamShape<32, 32, 32, 512>;
+using Stream8 = StreamShape<64, 16, 64, 512>;
 constexpr int kNumStream = 9;
-static const int kStreamDims[kNumStream][5] = {
-    {64, 16, 32, 512, 1}, {64, 16, 32, 512, 2}, {64, 16, 16, 512, 2}, {32, 16, 32, 256, 2}, {64, 32, 32, 512, 2},
-    {64, 16, 32, 256, 1}, {64, 32, 32, 512, 1}, {128, 16, 32, 512, 1}, {32, 16, 32, 256, 1}};
+static const int kStreamDims[kNumStream][4] = {
+    {64, 16, 32, 512}, {64, 16, 16, 512}, {64, 32, 32, 512}, {32, 16, 32, 256}, {128, 16, 32, 512},
+    {64, 16, 32, 256}, {64, 8, 32, 256},  {32, 32, 32, 512}, {64, 16, 64, 512}};
+static_assert(stream_fits_lds<Stream2, true, false>() && stream_fits_lds<Stream2, false, true>(), "stream LDS");
 static_assert(stream_fits_lds<Stream4, true, false>() && stream_fits_lds<Stream4, false, true>(), "stream LDS");
-static_assert(stream_fits_lds<Stream6, true, false>() && stream_fits_lds<Stream6, false, true>(), "stream LDS");
-static_assert(stream_fits_lds<Stream7, true, false>() && stream_fits_lds<Stream7, false, true>(), "stream LDS");
 
 // sweep configurations: 0 .. kNumTiles-1 tiles, kNumTiles .. kNumTiles+kNumStream-1 streams
 constexpr int kNumCfg = kNumTiles + kNumStream;
