@@ -292,7 +292,7 @@ def main():
     dims = [(32, 16, 16, 1024), (16, 8, 8, 1024), (16, 8, 8, 256), (64, 8, 8, 512), (8, 8, 8, 512)]
     # streaming shapes: (SX, SY, ZC, threads)
     sdims = [(64, 16, 32, 512), (64, 16, 16, 512), (64, 32, 32, 512), (32, 16, 32, 256), (128, 16, 32, 512),
-             (64, 16, 32, 256), (64, 8, 32, 256), (32, 32, 32, 512), (64, 16, 64, 512)]
+             (32, 16, 16, 256), (32, 8, 16, 256), (32, 32, 32, 512), (16, 16, 16, 256)]
     names = ["x".join(map(str, d[:3])) + f"/{d[3]}" for d in dims] + \
             ["stream " + "x".join(map(str, d[:2])) + f" z{d[2]}/{d[3]}" for d in sdims]
     lev, m = 0, n

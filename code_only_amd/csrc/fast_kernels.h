@@ -676,14 +676,14 @@ using Stream1 = StreamShape<64, 16, 16, 512>;
 using Stream2 = StreamShape<64, 32, 32, 512>;
 using Stream3 = StreamShape<32, 16, 32, 256>;
 using Stream4 = StreamShape<128, 16, 32, 512>;
-using Stream5 = StreamShape<64, 16, 32, 256>;
-using Stream6 = StreamShape<64, 8, 32, 256>;
+using Stream5 = StreamShape<32, 16, 16, 256>;
+using Stream6 = StreamShape<32, 8, 16, 256>;
 using Stream7 = StreamShape<32, 32, 32, 512>;
-using Stream8 = StreamShape<64, 16, 64, 512>;
+using Stream8 = StreamShape<16, 16, 16, 256>;
 constexpr int kNumStream = 9;
 static const int kStreamDims[kNumStream][4] = {
     {64, 16, 32, 512}, {64, 16, 16, 512}, {64, 32, 32, 512}, {32, 16, 32, 256}, {128, 16, 32, 512},
-    {64, 16, 32, 256}, {64, 8, 32, 256},  {32, 32, 32, 512}, {64, 16, 64, 512}};
+    {32, 16, 16, 256}, {32, 8, 16, 256},  {32, 32, 32, 512}, {16, 16, 16, 256}};
 static_assert(stream_fits_lds<Stream2, true, false>() && stream_fits_lds<Stream2, false, true>(), "stream LDS");
 static_assert(stream_fits_lds<Stream4, true, false>() && stream_fits_lds<Stream4, false, true>(), "stream LDS");
 
