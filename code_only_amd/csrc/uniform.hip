@@ -588,18 +588,19 @@ static int launch_update_resid(sg_uniform *c, const double *pin, const float *e,
 {
     const int n = c->n, lg = ilog2(n);
     const long cells = n3(n);
-    int zc = (int)std::max<long>(4, std::min<long>(32, cells / (256L * kNormBlocks)));
+    int zc = (int)std::max<long>(4, std::min<long>(32, cells / ((long)kUrThreads * kNormBlocks)));
     zc = std::min(zc, n);
-    const dim3 grid(n / 64, n / 4, n / zc);
+    SG_REQUIRE(n >= kUrTX && n % kUrTX == 0 && n % kUrTY == 0 && n % zc == 0, "update+residual: bad n = %d", n);
+    const dim3 grid(n / kUrTX, n / kUrTY, n / zc);
     const int nb = (int)(grid.x * grid.y * grid.z);
     SG_REQUIRE(nb <= c->npart, "norm partial buffer too small");
     const double h2i = 1.0 / (c->h * c->h);
     if (e)
-        hipLaunchKernelGGL(k_update_resid<true>, grid, dim3(256), 0, c->stream, pin, e, pout, src, c->f.RHO[0], lg, h2i,
-                           zc, c->pmax, c->psum);
+        hipLaunchKernelGGL(k_update_resid<true>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
+                           lg, h2i, zc, c->pmax, c->psum);
     else
-        hipLaunchKernelGGL(k_update_resid<false>, grid, dim3(256), 0, c->stream, pin, e, pout, src, c->f.RHO[0], lg,
-                           h2i, zc, c->pmax, c->psum);
+        hipLaunchKernelGGL(k_update_resid<false>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
+                           lg, h2i, zc, c->pmax, c->psum);
     SG_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, nb, c->dnorm);
     SG_CHECK_LAUNCH();

@@ -144,36 +144,63 @@ __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin
 
 // Outer FP64 step of the mixed solve, fused: pout = pin + e (mg.F90:87) and
 // r = f - L pout (mg_poisson.F90:17-27) stored as FP32 for the next inner solve, with the norm
-// partials of r.  Neighbour values are formed as pin + e on the fly, exactly the values pout
-// receives, so the result equals the separate update and residual kernels bit for bit; pin and
-// pout must differ.  UPD = false: r = f - L pin only.  One thread per (i, j) column of a 64 x 4
-// patch marching through zc planes, the z neighbours kept in registers.  n = 2^lg >= 64.
+// partials of r.  Neighbour values are formed as pin + e, exactly the values pout receives, so the
+// result equals the separate update and residual kernels bit for bit; pin and pout must differ.
+// UPD = false: r = f - L pin only.  A workgroup owns a 64 x 8 patch of (i, j) columns and marches
+// through zc planes: each thread forms its cell of the next plane (and, on the patch edge, one
+// halo cell in x and/or y) while the current plane sits in a double-buffered LDS plane with a
+// one-cell halo, so every value is loaded from memory once and one barrier per plane suffices; the
+// z neighbours stay in registers.  n = 2^lg >= 64.
+constexpr int kUrTX = 64, kUrTY = 8, kUrThreads = kUrTX * kUrTY;
+
 template <bool UPD>
-__global__ void __launch_bounds__(256) k_update_resid(const double *__restrict__ pin, const float *__restrict__ e,
-                                                      double *__restrict__ pout, const double *__restrict__ f,
-                                                      float *__restrict__ r32, int lg, double h2i, int zc,
-                                                      double *pmax, double *psum)
+__global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__restrict__ pin, const float *__restrict__ e,
+                                                             double *__restrict__ pout, const double *__restrict__ f,
+                                                             float *__restrict__ r32, int lg, double h2i, int zc,
+                                                             double *pmax, double *psum)
 {
+    constexpr int TX = kUrTX, TY = kUrTY, SP = TX + 2;
+    __shared__ double sp[2][(TY + 2) * SP];
     const int n = 1 << lg, m = n - 1;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int tx = (int)threadIdx.x % TX, ty = (int)threadIdx.x / TX;
+    const int i = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
     const int z0 = blockIdx.z * zc;
     auto at = [&](int ii, int jj, int kk) { return (kk << (2 * lg)) | (jj << lg) | ii; };
     auto P = [&](int q) -> double { return UPD ? pin[q] + (double)e[q] : pin[q]; };
-    const int ip = (i + 1) & m, im = (i - 1) & m, jp = (j + 1) & m, jm = (j - 1) & m;
+    // this thread's halo cells: x (left edge of the patch: i - 1, right edge: i + 1), y likewise
+    const bool hx = tx == 0 || tx == TX - 1, hy = ty == 0 || ty == TY - 1;
+    const int ih = tx == 0 ? (i - 1) & m : (i + 1) & m, jh = ty == 0 ? (j - 1) & m : (j + 1) & m;
+    const int sx = tx == 0 ? (ty + 1) * SP : (ty + 1) * SP + TX + 1;  // its slot in the LDS plane
+    const int sy = ty == 0 ? tx + 1 : (TY + 1) * SP + tx + 1;
+    const int sc = (ty + 1) * SP + tx + 1;
     double zm = P(at(i, j, (z0 - 1) & m)), zcen = P(at(i, j, z0));
+    double vx = hx ? P(at(ih, j, z0)) : 0.0, vy = hy ? P(at(i, jh, z0)) : 0.0;
     double mx = 0.0, sm = 0.0;
     for (int k = z0; k < z0 + zc; ++k) {
-        const double zp = P(at(i, j, (k + 1) & m));
+        double *s = sp[k & 1];
+        s[sc] = zcen;
+        if (hx) s[sx] = vx;
+        if (hy) s[sy] = vy;
+        // the next plane's values, loaded while this plane is used
+        const int kn = (k + 1) & m;
+        const double zp = P(at(i, j, kn));
+        double vxn = 0.0, vyn = 0.0;
+        if (k + 1 < z0 + zc) {
+            if (hx) vxn = P(at(ih, j, kn));
+            if (hy) vyn = P(at(i, jh, kn));
+        }
         const int c = at(i, j, k);
-        const double rv = resid_value<double>(zcen, P(at(ip, j, k)), P(at(im, j, k)), P(at(i, jp, k)),
-                                              P(at(i, jm, k)), zp, zm, f[c], h2i);
+        const double fc = f[c];
+        __syncthreads();
+        const double rv = resid_value<double>(zcen, s[sc + 1], s[sc - 1], s[sc + SP], s[sc - SP], zp, zm, fc, h2i);
         r32[c] = (float)rv;
         if (UPD) pout[c] = zcen;
         mx = fmax(mx, fabs(rv));
         sm += rv * rv;
         zm = zcen;
         zcen = zp;
+        vx = vxn;
+        vy = vyn;
     }
     block_max_sum(mx, sm);
     if (threadIdx.x == 0) {
