@@ -200,14 +200,42 @@ template <typename T> int op_fill0(sg_batch *c, const Pairs &pl, int code)
 }
 
 // mg_interp (mg.F90:608-633) of each listed fine level from the next coarser one: tricubic for
-// the FMG step when the coarse grid plus ghosts is at least 4 wide, trilinear otherwise.
+// the FMG step when the coarse grid plus ghosts is at least 4 wide, trilinear otherwise.  The
+// tricubic levels that an LDS tile shape (x extent 32, 16 or 8, widest first; 8 x 8 in y, z)
+// divides take the tiled form, the others the per-cell form (same values).
 template <typename T> int op_interp(sg_batch *c, const Pairs &pl, int cdst, int csrc, bool cubic, bool add)
 {
-    Pairs pc, pln;
+    Pairs pc, pln, pt[3];
     for (auto &q : pl) {
         SG_REQUIRE(q.second + 1 < c->nlev[q.first], "batch schedule: interpolation from a missing level");
-        const int nc = pl_n<T>(c, q.first, q.second + 1);
-        (cubic && nc + 2 >= 4 ? pc : pln).push_back(q);
+        const int nc = pl_n<T>(c, q.first, q.second + 1), n = pl_n<T>(c, q.first, q.second);
+        if (!(cubic && nc + 2 >= 4 && !add)) {
+            (cubic && nc + 2 >= 4 ? pc : pln).push_back(q);
+            continue;
+        }
+        int g = -1;
+        for (int e = 0; e < 3 && g < 0; ++e)
+            if (n >= kCubicFXs[e] && n % kCubicFXs[e] == 0 && n % 8 == 0) g = e;
+        (g < 0 ? pc : pt[g]).push_back(q);
+    }
+    for (int e = 0; e < 3; ++e) {
+        const int fx = kCubicFXs[e];
+        const Item *d;
+        int cnt, tot;
+        SG_TRY(take(c, pt[e], [&](int p, int l) {
+            const int n = pl_n<T>(c, p, l);
+            return (n / fx) * (n / 8) * (n / 8);
+        }, &d, &cnt, &tot));
+        if (!cnt) continue;
+        const BL<T> L = bl<T>(c, d, cnt);
+        const dim3 grid(8 * ((tot + 7) / 8)), block(256);
+        if (fx == 32)
+            hipLaunchKernelGGL((kb_interp_cubic_tile_codes<32, 8, 8, 256, T>), grid, block, 0, c->stream, L, tot, cdst, csrc);
+        else if (fx == 16)
+            hipLaunchKernelGGL((kb_interp_cubic_tile_codes<16, 8, 8, 256, T>), grid, block, 0, c->stream, L, tot, cdst, csrc);
+        else
+            hipLaunchKernelGGL((kb_interp_cubic_tile_codes<8, 8, 8, 256, T>), grid, block, 0, c->stream, L, tot, cdst, csrc);
+        SG_CHECK_LAUNCH();
     }
     for (int pass = 0; pass < 2; ++pass) {
         const Pairs &pp = pass == 0 ? pc : pln;

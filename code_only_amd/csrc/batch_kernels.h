@@ -390,6 +390,17 @@ __global__ void __launch_bounds__(NT) kb_interp_cubic_tile(BL<float> L, int tota
     interp_cubic_tile<FX, FY, FZ, NT>(q.a[CU], coarse_src(q, c.it->l, cuc), q.n, c.lb);
 }
 
+// The same tile body from code csrc of level l + 1 into code cdst of level l, in either precision:
+// mg_interp's FMG step (mg.F90:608-633) in the plain FP64 and the unfused FP32 schedules.
+template <int FX, int FY, int FZ, int NT, typename T>
+__global__ void __launch_bounds__(NT) kb_interp_cubic_tile_codes(BL<T> L, int total, int cdst, int csrc)
+{
+    Cell<T> c;
+    if (!locate(L, xcd_block(total), c)) return;
+    const PL<T> &q = *c.q;
+    interp_cubic_tile<FX, FY, FZ, NT, T>(q.a[cdst], (c.q + 1)->a[csrc], q.n, c.lb);
+}
+
 // Levels of <= 16^3 cells of one problem, resident in one workgroup's LDS (the coarse kernel of
 // the single-grid solver, for any even n and a coarsest level of 1^3 or 3^3).
 struct CoarseDesc {
@@ -552,8 +563,8 @@ kb_coarse(BL<float> L, const CoarseDesc *descs, int mode, int ncycle, int npre, 
     for (int q = threadIdx.x; q < n0 * n0 * n0; q += kCoarseThreads) u_out[q] = sU[q];
 }
 
-// Plane-parallel exact solve (exact_planes.h) of each item's level: workgroup b of an item works on
-// plane (k_exact_xy) or row (k_exact_z) b of that problem's level.
+// Plane-parallel exact solve (exact_planes.h, whole form) of each item's level: workgroup b of an
+// item works on plane (k_exact_xy) or row (k_exact_z) b of that problem's level.
 __global__ void __launch_bounds__(kPlaneThreads) kb_exact_xy(BL<float> L, int total, const CoarseDesc *descs, int cin,
                                                              int cout)
 {

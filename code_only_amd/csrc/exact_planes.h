@@ -6,18 +6,29 @@
 // difference, Q[m n + k] = (cos a + sin a) / sqrt(n), a = 2 pi ((m k) mod n) / n, diagonalises it
 // along each axis with eigenvalues lam[m] = (2 cos(2 pi m / n) - 2) / h^2, so
 //     u = Q_xyz (Lambda^-1 (Q_xyz f)),  Lambda = (lam_i + lam_j) + lam_k,
-// the zero mode set to zero (the zero-mean solution).  Three launches of n workgroups each:
-//   k_exact_xy  (plane k):  x pass, then y pass             f -> w
-//   k_exact_z   (row j):    z pass, divide by Lambda, z pass w -> w
-//   k_exact_xy  (plane k):  x pass, then y pass             w -> u
-// Each workgroup stages its n x n plane in LDS with the transformed index k leading ([k][l], rows
-// of 68 floats: 16-byte aligned) and Q beside it (Q is symmetric, so its row k is column k).
-// Threads t and t + 256 compute the same 4 x 4 block of outputs -- m in [4 mb, 4 mb + 4) of the
-// lines l in [4 lb, 4 lb + 4) -- over the first and the second half of k: per k one float4 of Q
-// and one float4 of line values feed 16 fused multiply-adds in k order, and the block is
-// (first half) + (second half).  It is written back either as [m][l] or transposed as [l][m],
-// whichever layout the next pass reads.  The division by Lambda is a product with 1 / Lambda,
-// tabulated once per level (formed in double precision, zero for the zero mode).
+// the zero mode set to zero (the zero-mean solution).  Three launches:
+//   xy  (plane k):  x pass, then y pass             f -> w
+//   z   (row j):    z pass, times 1 / Lambda, z pass w -> w
+//   xy  (plane k):  x pass, then y pass             w -> u
+// Every transformed value is sum_k Q[m][k] v[k] formed as two chains of fused multiply-adds in k
+// order, one over the first and one over the second half of k, added at the end:
+// (first half) + (second half).  The division by Lambda is a product with 1 / Lambda, tabulated
+// once per level (formed in double precision, zero for the zero mode).
+//
+// Two launch forms give the same values bit for bit:
+// * whole (k_exact_xy, k_exact_z; the batched solver): one 512-thread workgroup per plane (row).
+//   The plane sits in LDS with the transformed index leading ([k][l], rows of 68 floats) and Q
+//   beside it (Q is symmetric, so its row k is column k).  Threads t and t + 256 form the same
+//   4 x 4 block of outputs -- m in [4 mb, 4 mb + 4) of the lines l in [4 lb, 4 lb + 4) -- over the
+//   two halves of k: per k one float4 of Q and one float4 of line values feed 16 multiply-adds.
+//   Few workgroups of much work each: the form for many planes at once.
+// * split (k_exact_xy_split, k_exact_z_split; the single-grid solver): G = plane_groups(n)
+//   workgroups of 256 threads per plane (row), split by the range of the first transformed index.
+//   Workgroup g of plane k forms the x transform X[mi][j] only for its n / G frequencies mi, then
+//   the y transform of those columns; workgroup g of row j transforms its n / G columns i along
+//   z, in place.  A thread forms four outputs (four consecutive frequencies) from one float4 of Q
+//   per k.  G times as many workgroups of 1 / G of the work: the form for one level, whose n
+//   planes alone would leave most of the chip idle.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,22 +41,32 @@ constexpr int kPlanePitch = kPlaneMaxN + 4;
 constexpr int kPlaneLoads = kPlaneMaxN * kPlaneMaxN / kPlaneThreads;  // plane values per thread
 static_assert((kPlaneMaxN / 4) * (kPlaneMaxN / 4) <= kPlaneBlocks, "one thread pair per 4 x 4 block");
 
+constexpr int kSplitThreads = 256;
+constexpr int kSplitLoads = kPlaneMaxN * kPlaneMaxN / kSplitThreads;
+
+// Workgroups per plane (row) of the split form: 4 when n / 4 is a multiple of 4, else 2 or 1
+// likewise (each workgroup's frequency range a multiple of 4); n % 4 == 0.
+__host__ __device__ constexpr int plane_groups(int n) { return n % 16 == 0 ? 4 : (n % 8 == 0 ? 2 : 1); }
+
 // Q (n x n, row-major, global) into LDS rows of pitch kPlanePitch; every load issued first.
-__device__ __forceinline__ void ep_stage_q(float *Qs, const float *Q, int n)
+template <int NT> __device__ __forceinline__ void ep_stage_q(float *Qs, const float *Q, int n)
 {
-    float v[kPlaneLoads];
+    constexpr int LD = kPlaneMaxN * kPlaneMaxN / NT;
+    float v[LD];
 #pragma unroll
-    for (int it = 0; it < kPlaneLoads; ++it) {
-        const int c = (int)threadIdx.x + it * kPlaneThreads;
+    for (int it = 0; it < LD; ++it) {
+        const int c = (int)threadIdx.x + it * NT;
         v[it] = 0.f;
         if (c < n * n) v[it] = Q[c];
     }
 #pragma unroll
-    for (int it = 0; it < kPlaneLoads; ++it) {
-        const int c = (int)threadIdx.x + it * kPlaneThreads;
+    for (int it = 0; it < LD; ++it) {
+        const int c = (int)threadIdx.x + it * NT;
         if (c < n * n) Qs[(c / n) * kPlanePitch + c % n] = v[it];
     }
 }
+
+// ---- whole form ----------------------------------------------------------------------------------
 
 // S ([k][l], n x n) := Q S, written as [m][l] (transpose == false) or [l][m].  n % 8 == 0 or
 // n % 4 == 0 with n / 2 even; part: scratch of kPlaneBlocks * 16 floats for the second halves.
@@ -122,7 +143,7 @@ __device__ __forceinline__ void exact_xy(const float *__restrict__ in, float *__
         v[it] = 0.f;
         if (c < n * n) v[it] = in[base + c];
     }
-    ep_stage_q(Qs, Q, n);
+    ep_stage_q<kPlaneThreads>(Qs, Q, n);
     // x lines (rows j) with the x index i leading: S[i][j]
 #pragma unroll
     for (int it = 0; it < kPlaneLoads; ++it) {
@@ -154,7 +175,7 @@ __device__ __forceinline__ void exact_z(float *__restrict__ w, const float *Q, c
             d[it] = inv[g];
         }
     }
-    ep_stage_q(Qs, Q, n);
+    ep_stage_q<kPlaneThreads>(Qs, Q, n);
 #pragma unroll
     for (int it = 0; it < kPlaneLoads; ++it) {
         const int c = (int)threadIdx.x + it * kPlaneThreads;
@@ -183,6 +204,140 @@ static __global__ void __launch_bounds__(kPlaneThreads) k_exact_xy(const float *
 static __global__ void __launch_bounds__(kPlaneThreads) k_exact_z(float *w, const float *Q, const float *inv, int n)
 {
     exact_z(w, Q, inv, n, (int)blockIdx.x);
+}
+
+// ---- split form ----------------------------------------------------------------------------------
+
+// acc[r] = sum_k Q[m0 + r][k] * V[k * sk + col * sc] (r < 4): (k < n/2 chain) + (k >= n/2 chain).
+__device__ __forceinline__ void ep_dot4(const float *Qs, const float *V, int sk, int sc, int n, int m0, int col,
+                                        float (&acc)[4])
+{
+    float lo[4] = {0.f, 0.f, 0.f, 0.f}, hi[4] = {0.f, 0.f, 0.f, 0.f};
+    const float *v = V + col * sc;
+    const int kh = n / 2;
+    for (int k = 0; k < kh; ++k) {
+        const float4 q0 = *reinterpret_cast<const float4 *>(Qs + k * kPlanePitch + m0);
+        const float4 q1 = *reinterpret_cast<const float4 *>(Qs + (k + kh) * kPlanePitch + m0);
+        const float x0 = v[k * sk], x1 = v[(k + kh) * sk];
+        lo[0] = __builtin_fmaf(q0.x, x0, lo[0]);
+        lo[1] = __builtin_fmaf(q0.y, x0, lo[1]);
+        lo[2] = __builtin_fmaf(q0.z, x0, lo[2]);
+        lo[3] = __builtin_fmaf(q0.w, x0, lo[3]);
+        hi[0] = __builtin_fmaf(q1.x, x1, hi[0]);
+        hi[1] = __builtin_fmaf(q1.y, x1, hi[1]);
+        hi[2] = __builtin_fmaf(q1.z, x1, hi[2]);
+        hi[3] = __builtin_fmaf(q1.w, x1, hi[3]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = lo[r] + hi[r];
+}
+
+// Workgroup wg = plane * G + g of the dense n^3 array in (i fastest), G = plane_groups(n): for the
+// x frequencies mi in [g n/G, (g+1) n/G), X[mi][j] = sum_i Q[mi][i] in[plane][j][i], then
+// out[plane][mj][mi] = sum_j Q[mj][j] X[mi][j].
+__device__ __forceinline__ void exact_xy_split(const float *__restrict__ in, float *__restrict__ out, const float *Q,
+                                               int n, int wg)
+{
+    constexpr int PS = kPlaneMaxN + 1;                                      // plane rows [j][i], odd pitch
+    __shared__ __align__(16) float Qs[kPlaneMaxN * kPlanePitch];
+    __shared__ float S[kPlaneMaxN * PS];
+    __shared__ __align__(16) float Xs[kPlaneMaxN * kPlanePitch];
+    const int G = plane_groups(n), mq = n / G, plane = wg / G, g = wg % G;
+    const int px = mq + 4;                                                  // X rows [j][mi local]
+    const long base = (long)plane * n * n;
+    float v[kSplitLoads];
+#pragma unroll
+    for (int it = 0; it < kSplitLoads; ++it) {
+        const int c = (int)threadIdx.x + it * kSplitThreads;
+        v[it] = 0.f;
+        if (c < n * n) v[it] = in[base + c];
+    }
+    ep_stage_q<kSplitThreads>(Qs, Q, n);
+#pragma unroll
+    for (int it = 0; it < kSplitLoads; ++it) {
+        const int c = (int)threadIdx.x + it * kSplitThreads;
+        if (c < n * n) S[(c / n) * PS + c % n] = v[it];
+    }
+    __syncthreads();
+    // x pass: task (line j, block b of four local frequencies)
+    for (int t = (int)threadIdx.x; t < n * (mq / 4); t += kSplitThreads) {
+        const int j = t % n, b = t / n;
+        float acc[4];
+        ep_dot4(Qs, S, 1, PS, n, g * mq + 4 * b, j, acc);
+        *reinterpret_cast<float4 *>(Xs + j * px + 4 * b) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    }
+    __syncthreads();
+    // y pass: task (local frequency mi, block of four mj)
+    for (int t = (int)threadIdx.x; t < mq * (n / 4); t += kSplitThreads) {
+        const int mi = t % mq, mj0 = 4 * (t / mq);
+        float acc[4];
+        ep_dot4(Qs, Xs, px, 1, n, mj0, mi, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[base + (long)(mj0 + r) * n + g * mq + mi] = acc[r];
+    }
+}
+
+// Workgroup wg = row * G + g of the dense n^3 array w: for the columns i in [g n/G, (g+1) n/G)
+// of row j, Z[mk][i] = sum_k Q[mk][k] w[k][j][i], times inv[mk][j][i] (1 / Lambda, dense n^3 like
+// w), then w[k][j][i] = sum_mk Q[k][mk] Z[mk][i].
+__device__ __forceinline__ void exact_z_split(float *__restrict__ w, const float *Q, const float *inv, int n, int wg)
+{
+    constexpr int PC = kPlaneMaxN;                                          // column block rows [k][i local]
+    __shared__ __align__(16) float Qs[kPlaneMaxN * kPlanePitch];
+    __shared__ float S[kPlaneMaxN * PC];
+    __shared__ float Zs[kPlaneMaxN * PC];
+    const int G = plane_groups(n), mq = n / G, j = wg / G, g = wg % G;
+    const long nn = (long)n * n, col0 = (long)j * n + g * mq;
+    const int tasks = mq * (n / 4);
+    float v[kSplitLoads];
+#pragma unroll
+    for (int it = 0; it < kSplitLoads; ++it) {
+        const int c = (int)threadIdx.x + it * kSplitThreads;
+        v[it] = 0.f;
+        if (c < n * mq) v[it] = w[(c / mq) * nn + col0 + c % mq];
+    }
+    // 1 / Lambda of the thread's first task, loaded ahead
+    const int t0 = (int)threadIdx.x;
+    float d0[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d0[r] = t0 < tasks ? inv[(4 * (t0 / mq) + r) * nn + col0 + t0 % mq] : 0.f;
+    ep_stage_q<kSplitThreads>(Qs, Q, n);
+#pragma unroll
+    for (int it = 0; it < kSplitLoads; ++it) {
+        const int c = (int)threadIdx.x + it * kSplitThreads;
+        if (c < n * mq) S[(c / mq) * PC + c % mq] = v[it];
+    }
+    __syncthreads();
+    for (int t = t0; t < tasks; t += kSplitThreads) {
+        const int ii = t % mq, m0 = 4 * (t / mq);
+        float acc[4];
+        ep_dot4(Qs, S, PC, 1, n, m0, ii, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float d = t == t0 ? d0[r] : inv[(m0 + r) * nn + col0 + ii];
+            Zs[(m0 + r) * PC + ii] = acc[r] * d;
+        }
+    }
+    __syncthreads();
+    for (int t = t0; t < tasks; t += kSplitThreads) {
+        const int ii = t % mq, m0 = 4 * (t / mq);
+        float acc[4];
+        ep_dot4(Qs, Zs, PC, 1, n, m0, ii, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w[(m0 + r) * nn + col0 + ii] = acc[r];
+    }
+}
+
+static __global__ void __launch_bounds__(kSplitThreads) k_exact_xy_split(const float *in, float *out, const float *Q,
+                                                                         int n)
+{
+    exact_xy_split(in, out, Q, n, (int)blockIdx.x);
+}
+
+static __global__ void __launch_bounds__(kSplitThreads) k_exact_z_split(float *w, const float *Q, const float *inv,
+                                                                        int n)
+{
+    exact_z_split(w, Q, inv, n, (int)blockIdx.x);
 }
 
 // 1 / ((lam_i + lam_j) + lam_k) for the dense n^3 grid (0 where the sum is 0), host, double.

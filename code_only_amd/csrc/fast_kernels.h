@@ -725,13 +725,15 @@ inline void cand_decode(int cand, int &cfg, int &ns, bool &fs)
 // FMG tricubic prolongation uf = P3 uc (interpolation.F90:98-217) on an LDS tile: the coarse cells
 // any stencil of the fine tile can touch are staged, then the x, y and z passes run on LDS exactly
 // as the reference's bufx / bufxy sweeps.  Fine tile FX*FY*FZ, nf >= FX, nf % FY == nf % FZ == 0.
-template <int FX, int FY, int FZ, int NT>
-__device__ __forceinline__ void interp_cubic_tile(float *__restrict__ uf, const float *__restrict__ uc, int nf, int t)
+// T = float (mixed path) or double (plain path): the weights are rounded to T as k_interp_cubic
+// rounds them, and every value is formed in the same operand order, so both give its results.
+template <int FX, int FY, int FZ, int NT, typename T = float>
+__device__ __forceinline__ void interp_cubic_tile(T *__restrict__ uf, const T *__restrict__ uc, int nf, int t)
 {
     constexpr int CX = FX / 2 + 4, CY = FY / 2 + 4, CZ = FZ / 2 + 4;
-    __shared__ float sc[CX * CY * CZ];
-    __shared__ float bx[CZ * CY * FX];
-    __shared__ float bxy[CZ * FY * FX];
+    __shared__ T sc[CX * CY * CZ];
+    __shared__ T bx[CZ * CY * FX];
+    __shared__ T bxy[CZ * FY * FX];
     const int nc = nf >> 1;
     const int ntx = nf / FX, nty = nf / FY;
     const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
@@ -739,7 +741,7 @@ __device__ __forceinline__ void interp_cubic_tile(float *__restrict__ uf, const 
     const int bxo = fx0 / 2 - 2, byo = fy0 / 2 - 2, bzo = fz0 / 2 - 2;  // ghosted coarse index of local 0
     {
         constexpr int CNT = CX * CY * CZ, IT = (CNT + NT - 1) / NT;
-        float v[IT];
+        T v[IT];
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const int q = threadIdx.x + it * NT;
@@ -756,55 +758,55 @@ __device__ __forceinline__ void interp_cubic_tile(float *__restrict__ uf, const 
     }
     // per-axis stencil tables (interpolation.F90:106-127), computed once per tile
     __shared__ int tix[FX][4], tiy[FY][4], tiz[FZ][4];
-    __shared__ float twx[FX][4], twy[FY][4], twz[FZ][4];
+    __shared__ T twx[FX][4], twy[FY][4], twz[FZ][4];
     if (threadIdx.x < FX + FY + FZ) {
         const int q = threadIdx.x;
         int idx[4];
         double a[4];
         if (q < FX) {
             cubic_stencil(fx0 + q, -1, nc, idx, a);
-            for (int e = 0; e < 4; ++e) { tix[q][e] = idx[e] - bxo; twx[q][e] = (float)a[e]; }
+            for (int e = 0; e < 4; ++e) { tix[q][e] = idx[e] - bxo; twx[q][e] = (T)a[e]; }
         } else if (q < FX + FY) {
             cubic_stencil(fy0 + q - FX, -1, nc, idx, a);
-            for (int e = 0; e < 4; ++e) { tiy[q - FX][e] = idx[e] - byo; twy[q - FX][e] = (float)a[e]; }
+            for (int e = 0; e < 4; ++e) { tiy[q - FX][e] = idx[e] - byo; twy[q - FX][e] = (T)a[e]; }
         } else {
             cubic_stencil(fz0 + q - FX - FY, -1, nc, idx, a);
-            for (int e = 0; e < 4; ++e) { tiz[q - FX - FY][e] = idx[e] - bzo; twz[q - FX - FY][e] = (float)a[e]; }
+            for (int e = 0; e < 4; ++e) { tiz[q - FX - FY][e] = idx[e] - bzo; twz[q - FX - FY][e] = (T)a[e]; }
         }
     }
     __syncthreads();
     // x pass: bx(fx, jc, kc) over the staged coarse y, z range
     for (int q = threadIdx.x; q < CZ * CY * FX; q += NT) {
         const int x = q % FX, r = q / FX;
-        const float *row = sc + r * CX;
-        bx[q] = cubic4<float>(twx[x], row[tix[x][0]], row[tix[x][1]], row[tix[x][2]], row[tix[x][3]]);
+        const T *row = sc + r * CX;
+        bx[q] = cubic4<T>(twx[x], row[tix[x][0]], row[tix[x][1]], row[tix[x][2]], row[tix[x][3]]);
     }
     __syncthreads();
     // y pass: bxy(fx, fy, kc)
     for (int q = threadIdx.x; q < CZ * FY * FX; q += NT) {
         const int x = q % FX, y = (q / FX) % FY, z = q / (FX * FY);
-        const float *col = bx + z * CY * FX + x;
-        bxy[q] = cubic4<float>(twy[y], col[tiy[y][0] * FX], col[tiy[y][1] * FX], col[tiy[y][2] * FX],
+        const T *col = bx + z * CY * FX + x;
+        bxy[q] = cubic4<T>(twy[y], col[tiy[y][0] * FX], col[tiy[y][1] * FX], col[tiy[y][2] * FX],
                                col[tiy[y][3] * FX]);
     }
     __syncthreads();
     // z pass: uf(fx, fy, fz)
     for (int q = threadIdx.x; q < FZ * FY * FX; q += NT) {
         const int x = q % FX, y = (q / FX) % FY, z = q / (FX * FY);
-        const float *p = bxy + y * FX + x;
+        const T *p = bxy + y * FX + x;
         constexpr int PL = FY * FX;
         uf[((fz0 + z) * nf + (fy0 + y)) * nf + (fx0 + x)] =
-            cubic4<float>(twz[z], p[tiz[z][0] * PL], p[tiz[z][1] * PL], p[tiz[z][2] * PL], p[tiz[z][3] * PL]);
+            cubic4<T>(twz[z], p[tiz[z][0] * PL], p[tiz[z][1] * PL], p[tiz[z][2] * PL], p[tiz[z][3] * PL]);
     }
 }
 
-template <int FX, int FY, int FZ, int NT>
-__global__ void __launch_bounds__(NT) k_interp_cubic_tile(float *__restrict__ uf, const float *__restrict__ uc, int nf)
+template <int FX, int FY, int FZ, int NT, typename T = float>
+__global__ void __launch_bounds__(NT) k_interp_cubic_tile(T *__restrict__ uf, const T *__restrict__ uc, int nf)
 {
     const int ntiles = (nf / FX) * (nf / FY) * (nf / FZ);
     const int t = xcd_tile(ntiles);
     if (t >= ntiles) return;
-    interp_cubic_tile<FX, FY, FZ, NT>(uf, uc, nf, t);
+    interp_cubic_tile<FX, FY, FZ, NT, T>(uf, uc, nf, t);
 }
 
 // ------------------------------------------------------------------------------------------------

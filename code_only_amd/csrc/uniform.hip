@@ -175,13 +175,29 @@ template <typename T> static int launch_restrict(hipStream_t s, T *uc, const T *
     return SG_OK;
 }
 
+constexpr int kCubicFX = 32, kCubicFY = 8, kCubicFZ = 8, kCubicNT = 256;
+
+// FMG tricubic prolongation of a fine level n >= 32 through LDS tiles.
+template <typename T> static int launch_interp_cubic_tile(hipStream_t s, T *uf, const T *uc, int nf)
+{
+    SG_REQUIRE(nf >= kCubicFX && nf % kCubicFX == 0 && nf % kCubicFY == 0 && nf % kCubicFZ == 0,
+               "tiled tricubic needs nf >= 32, got %d", nf);
+    const int ntiles = (nf / kCubicFX) * (nf / kCubicFY) * (nf / kCubicFZ);
+    hipLaunchKernelGGL((k_interp_cubic_tile<kCubicFX, kCubicFY, kCubicFZ, kCubicNT, T>), dim3(8 * ((ntiles + 7) / 8)),
+                       dim3(kCubicNT), 0, s, uf, uc, nf);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
 // mg_interp (mg.F90:608-633) on a periodic grid: the ghost fills become index wrap, so only the
 // prolongation itself remains.  Tricubic needs a coarse array of >= 4 cells with ghosts
-// (interpolation.F90:87-96); narrower ones fall back to trilinear.
+// (interpolation.F90:87-96); narrower ones fall back to trilinear.  Fine levels of >= 32 cells
+// take the LDS-tiled tricubic (same values as the per-cell form).
 template <typename T> static int launch_interp(hipStream_t s, T *uf, const T *uc, int nf, bool cubic, bool add)
 {
     const int nc = nf / 2;
     const unsigned nb = nblocks(n3(nf));
+    if (cubic && !add && nf >= kCubicFX) return launch_interp_cubic_tile<T>(s, uf, uc, nf);
     if (cubic && nc + 2 >= 4 && !add) {
         hipLaunchKernelGGL(k_interp_cubic<T>, dim3(nb), dim3(kBlock), 0, s, uf, uc, ilog2(nf));
     } else if (add) {
@@ -345,20 +361,6 @@ static int launch_sweep(sg_uniform *c, int l, int ns, const float *uin, float *u
                             c->hl[l], zero, prolong, resid, uc, rc, resmax);
 }
 
-constexpr int kCubicFX = 32, kCubicFY = 8, kCubicFZ = 8, kCubicNT = 256;
-
-// FMG tricubic prolongation of a fine level n >= 32 through LDS tiles.
-static int launch_interp_cubic_tile(hipStream_t s, float *uf, const float *uc, int nf)
-{
-    SG_REQUIRE(nf >= kCubicFX && nf % kCubicFX == 0 && nf % kCubicFY == 0 && nf % kCubicFZ == 0,
-               "tiled tricubic needs nf >= 32, got %d", nf);
-    const int ntiles = (nf / kCubicFX) * (nf / kCubicFY) * (nf / kCubicFZ);
-    hipLaunchKernelGGL((k_interp_cubic_tile<kCubicFX, kCubicFY, kCubicFZ, kCubicNT>), dim3(8 * ((ntiles + 7) / 8)),
-                       dim3(kCubicNT), 0, s, uf, uc, nf);
-    SG_CHECK_LAUNCH();
-    return SG_OK;
-}
-
 // Autotuner: for every fine level, time each fitting (tile shape, sweeps per launch) on the
 // level's own arrays as ms per fine-level visit of a V(2,2) cycle -- two sweeps ending with the
 // residual restriction, two starting with the prolongated correction, in the launches the cycle
@@ -502,13 +504,14 @@ static int exact_bottom(sg_uniform *c, int mode, int ncycle, int npre, int npost
     SG_REQUIRE(c->coarse.exact, "exact coarse solve needs a level below the finest");
     if (lb == c->lc) return launch_coarse_k(c, true, c->f.RHO[lb], c->f.U[lb], mode, ncycle, npre, npost, nullptr);
     SG_REQUIRE(n <= kPlaneMaxN && c->d_qx && c->d_invx && c->U1[lb], "plane-parallel exact solve not set up");
-    hipLaunchKernelGGL(k_exact_xy, dim3(n), dim3(kPlaneThreads), 0, c->stream, (const float *)c->f.RHO[lb], c->U1[lb],
+    const dim3 grid(n * plane_groups(n)), block(kSplitThreads);
+    hipLaunchKernelGGL(k_exact_xy_split, grid, block, 0, c->stream, (const float *)c->f.RHO[lb], c->U1[lb],
                        (const float *)c->d_qx, n);
     SG_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_exact_z, dim3(n), dim3(kPlaneThreads), 0, c->stream, c->U1[lb], (const float *)c->d_qx,
+    hipLaunchKernelGGL(k_exact_z_split, grid, block, 0, c->stream, c->U1[lb], (const float *)c->d_qx,
                        (const float *)c->d_invx, n);
     SG_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_exact_xy, dim3(n), dim3(kPlaneThreads), 0, c->stream, (const float *)c->U1[lb], c->f.U[lb],
+    hipLaunchKernelGGL(k_exact_xy_split, grid, block, 0, c->stream, (const float *)c->U1[lb], c->f.U[lb],
                        (const float *)c->d_qx, n);
     SG_CHECK_LAUNCH();
     return SG_OK;
