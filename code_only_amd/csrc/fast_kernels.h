@@ -635,20 +635,26 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
                     if ((kq & 1) == 0) {
                         for (int q = tid; q < SX * SY; q += NT) srp[q] = resid_at(S(-2), S(-3), S(-1), q % SX, q / SX);
                     } else {
-                        // coarse cell (cx, cy) of the pair (kq - 1, kq): 4 residuals of the even plane
-                        // from srp, 4 of the odd plane
-                        for (int q = tid; q < (SX / 2) * (SY / 2); q += NT) {
-                            const int cx = q % (SX / 2), cy = q / (SX / 2);
-                            float r[8];
-#pragma unroll
-                            for (int o = 0; o < 8; ++o) {
-                                const int a = o & 1, b = (o >> 1) & 1, dz = (o >> 2) & 1;
-                                const int x = 2 * cx + a, y = 2 * cy + b;
-                                r[o] = dz ? resid_at(S(-2), S(-3), S(-1), x, y) : srp[y * SX + x];
+                        // coarse cell (cx, cy) of the pair (kq - 1, kq), shared by two adjacent lanes:
+                        // lane b forms the two residuals of row 2 cy + b on the odd plane and takes
+                        // that row's two of the even plane from srp; the pair swaps its four values
+                        // and lane 0 restricts the eight (NT is a multiple of the wave size, so both
+                        // lanes of a pair are in one wave and in the same loop trip)
+                        for (int q = tid; q < SX * SY / 2; q += NT) {
+                            const int b = q & 1, cx = (q >> 1) % (SX / 2), cy = (q >> 1) / (SX / 2);
+                            const int y = 2 * cy + b;
+                            const float e0 = srp[y * SX + 2 * cx], e1 = srp[y * SX + 2 * cx + 1];
+                            const float o0 = resid_at(S(-2), S(-3), S(-1), 2 * cx, y);
+                            const float o1 = resid_at(S(-2), S(-3), S(-1), 2 * cx + 1, y);
+                            const float pe0 = __shfl_xor(e0, 1), pe1 = __shfl_xor(e1, 1);
+                            const float po0 = __shfl_xor(o0, 1), po1 = __shfl_xor(o1, 1);
+                            if (b == 0) {
+                                // r[a + 2 b + 4 dz]: own row is b = 0, the partner's b = 1
+                                const float r[8] = {e0, e1, pe0, pe1, o0, o1, po0, po1};
+                                const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy, gcz = wrapn(kq, n) / 2;
+                                rc[(gcz * nc + gcy) * nc + gcx] =
+                                    restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
                             }
-                            const int gcx = x0 / 2 + cx, gcy = y0 / 2 + cy, gcz = wrapn(kq, n) / 2;
-                            rc[(gcz * nc + gcy) * nc + gcx] =
-                                restrict_value<float>(r[0], r[1], r[2], r[4], r[3], r[5], r[6], r[7]);
                         }
                     }
                 }

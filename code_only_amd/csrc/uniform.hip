@@ -85,7 +85,7 @@ struct sg_uniform {
     double *dnorm = nullptr;   // [0] max|r| [1] sum r^2 [2] resmaxg*h^2 [3] max|f| [4] sum f^2
     double *hnorm = nullptr;   // pinned host mirror of dnorm
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evn = nullptr;
     // graphs of one mixed-precision outer iteration (one per phi buffer parity), keyed on the
     // arguments they were captured with
     struct Graph {
@@ -580,10 +580,17 @@ template <typename Tu> static int launch_update(sg_uniform *c, double *psi, cons
     return SG_OK;
 }
 
+// The norms to the host.  The host polls an event rather than blocking in a stream synchronize:
+// the outer loop waits on this between iterations, and polling wakes it as soon as the copy is
+// done (the GPU idles until the next iteration is launched).
 static int read_norms(sg_uniform *c)
 {
     SG_HIP(hipMemcpyAsync(c->hnorm, c->dnorm, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    SG_HIP(hipStreamSynchronize(c->stream));
+    SG_HIP(hipEventRecord(c->evn, c->stream));
+    hipError_t q;
+    while ((q = hipEventQuery(c->evn)) == hipErrorNotReady) {
+    }
+    SG_HIP(q);
     return SG_OK;
 }
 
@@ -795,6 +802,7 @@ static void destroy(sg_uniform *c)
     if (c->hnorm) (void)hipHostFree(c->hnorm);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evn) (void)hipEventDestroy(c->evn);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -827,6 +835,7 @@ static int create(sg_uniform **out, int n, double h, int precisions)
     if ((rc = [&]() -> int {
             SG_HIP(hipEventCreate(&c->ev0));
             SG_HIP(hipEventCreate(&c->ev1));
+            SG_HIP(hipEventCreateWithFlags(&c->evn, hipEventDisableTiming));
             c->npart = (int)nblocks(n3(n));
             SG_HIP(hipMalloc(&c->pmax, sizeof(double) * c->npart));
             SG_HIP(hipMalloc(&c->psum, sizeof(double) * c->npart));

@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin
 // through zc planes: each thread forms its cell of the next plane (and, on the patch edge, one
 // halo cell in x and/or y) while the current plane sits in a double-buffered LDS plane with a
 // one-cell halo, so every value is loaded from memory once and one barrier per plane suffices; the
-// z neighbours stay in registers.  n = 2^lg >= 64.
+// z neighbours and the next plane's source value stay in registers.  n = 2^lg >= 64.
 constexpr int kUrTX = 64, kUrTY = 8, kUrThreads = kUrTX * kUrTY;
 
 template <bool UPD>
@@ -175,6 +175,7 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
     const int sc = (ty + 1) * SP + tx + 1;
     double zm = P(at(i, j, (z0 - 1) & m)), zcen = P(at(i, j, z0));
     double vx = hx ? P(at(ih, j, z0)) : 0.0, vy = hy ? P(at(i, jh, z0)) : 0.0;
+    double fc = f[at(i, j, z0)];
     double mx = 0.0, sm = 0.0;
     for (int k = z0; k < z0 + zc; ++k) {
         double *s = sp[k & 1];
@@ -184,13 +185,13 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
         // the next plane's values, loaded while this plane is used
         const int kn = (k + 1) & m;
         const double zp = P(at(i, j, kn));
-        double vxn = 0.0, vyn = 0.0;
+        double vxn = 0.0, vyn = 0.0, fn = 0.0;
         if (k + 1 < z0 + zc) {
             if (hx) vxn = P(at(ih, j, kn));
             if (hy) vyn = P(at(i, jh, kn));
+            fn = f[at(i, j, kn)];
         }
         const int c = at(i, j, k);
-        const double fc = f[c];
         __syncthreads();
         const double rv = resid_value<double>(zcen, s[sc + 1], s[sc - 1], s[sc + SP], s[sc - SP], zp, zm, fc, h2i);
         r32[c] = (float)rv;
@@ -201,6 +202,7 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
         zcen = zp;
         vx = vxn;
         vy = vyn;
+        fc = fn;
     }
     block_max_sum(mx, sm);
     if (threadIdx.x == 0) {
