@@ -522,9 +522,10 @@ static int exact_bottom(sg_uniform *c, int mode, int ncycle, int npre, int npost
 // sweeps, the last pre-sweep carrying the residual and its restriction, the first post-sweep the
 // prolongated correction.  With `exact` the cycle instead ends at level lb, solved exactly
 // (exact_bottom).  Same values as mg_lin<float> with the same `exact`, bit for bit.  *u0 receives
-// the buffer that holds the level-0 correction.
+// the buffer that holds the level-0 correction.  rhs1_ready: RHO[1] already holds the restriction
+// of RHO[0] (formed with the residual by k_update_resid).
 static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *resmax, const float **u0,
-                       bool exact)
+                       bool exact, bool rhs1_ready)
 {
     SG_REQUIRE(npre >= 1 && npost >= 1, "fused path needs npre, npost >= 1");
     Levels<float> &A = c->f;
@@ -533,7 +534,7 @@ static int fast_mg_lin(sg_uniform *c, int ncycle, int npre, int npost, double *r
     const int lc = exact ? c->lb : c->lc;   // level of the coarse-grid solve
     int cur[kMaxLev] = {};
     auto buf = [&](int l, int b) -> float * { return b ? c->U1[l] : A.U[l]; };
-    for (int l = 1; l <= lc; ++l) SG_TRY(launch_restrict<float>(s, A.RHO[l], A.RHO[l - 1], c->nl[l]));
+    for (int l = rhs1_ready ? 2 : 1; l <= lc; ++l) SG_TRY(launch_restrict<float>(s, A.RHO[l], A.RHO[l - 1], c->nl[l]));
     if (exact) SG_TRY(exact_bottom(c, 0, ncycle, npre, npost));
     else SG_TRY(launch_coarse_k(c, false, A.RHO[lc], A.U[lc], 0, ncycle, npre, npost, lc == 0 ? resmax : nullptr));
     cur[lc] = 0;
@@ -604,13 +605,15 @@ static int launch_update_resid(sg_uniform *c, const double *pin, const float *e,
     const dim3 grid(n / kUrTX, n / kUrTY, n / zc);
     const int nb = (int)(grid.x * grid.y * grid.z);
     SG_REQUIRE(nb <= c->npart, "norm partial buffer too small");
+    SG_REQUIRE(c->nlev > 1 && zc % 2 == 0, "update+residual: no level to restrict into");
     const double h2i = 1.0 / (c->h * c->h);
+    // the residual restricted into RHO[1] here: the fused inner solve starts from it
     if (e)
         hipLaunchKernelGGL(k_update_resid<true>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
-                           lg, h2i, zc, c->pmax, c->psum);
+                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1]);
     else
         hipLaunchKernelGGL(k_update_resid<false>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
-                           lg, h2i, zc, c->pmax, c->psum);
+                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1]);
     SG_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, nb, c->dnorm);
     SG_CHECK_LAUNCH();
@@ -634,7 +637,8 @@ static int outer_iteration(sg_uniform *c, const double *pin, double *pout, const
     if (p->precision == SG_PREC_MIXED) {
         const float *u0 = c->f.U[0];
         const bool exact = p->coarse == SG_COARSE_EXACT;
-        if (p->fused) SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost, resmax, &u0, exact));
+        // with pin != pout every residual came from launch_update_resid, which restricted it too
+        if (p->fused) SG_TRY(fast_mg_lin(c, p->ncycle, p->npre, p->npost, resmax, &u0, exact, pin != pout));
         else SG_TRY(mg_lin<float>(c, p->ncycle, p->npre, p->npost, resmax, exact));
         if (pin != pout) {
             SG_TRY(launch_update_resid(c, pin, u0, pout, src));

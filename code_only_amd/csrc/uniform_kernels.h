@@ -146,7 +146,10 @@ __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin
 // r = f - L pout (mg_poisson.F90:17-27) stored as FP32 for the next inner solve, with the norm
 // partials of r.  Neighbour values are formed as pin + e, exactly the values pout receives, so the
 // result equals the separate update and residual kernels bit for bit; pin and pout must differ.
-// UPD = false: r = f - L pin only.  A workgroup owns a 64 x 8 patch of (i, j) columns and marches
+// UPD = false: r = f - L pin only.  With rc, the FP32 residual is also restricted into rc (the
+// next level's right-hand side, restriction.F90:132-147 operand order, as k_restrict forms it):
+// each plane's FP32 residuals go to one of four LDS planes, and the restriction of planes
+// (k - 2, k - 1) runs at plane k, after its barrier.  A workgroup owns a 64 x 8 patch of (i, j) columns and marches
 // through zc planes: each thread forms its cell of the next plane (and, on the patch edge, one
 // halo cell in x and/or y) while the current plane sits in a double-buffered LDS plane with a
 // one-cell halo, so every value is loaded from memory once and one barrier per plane suffices; the
@@ -157,16 +160,22 @@ template <bool UPD>
 __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__restrict__ pin, const float *__restrict__ e,
                                                              double *__restrict__ pout, const double *__restrict__ f,
                                                              float *__restrict__ r32, int lg, double h2i, int zc,
-                                                             double *pmax, double *psum)
+                                                             double *pmax, double *psum, float *__restrict__ rc)
 {
     constexpr int TX = kUrTX, TY = kUrTY, SP = TX + 2;
     __shared__ double sp[2][(TY + 2) * SP];
+    __shared__ float sr[4][TY * TX];
     const int n = 1 << lg, m = n - 1;
     const int tx = (int)threadIdx.x % TX, ty = (int)threadIdx.x / TX;
     const int i = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
     const int z0 = blockIdx.z * zc;
     auto at = [&](int ii, int jj, int kk) { return (kk << (2 * lg)) | (jj << lg) | ii; };
-    auto P = [&](int q) -> double { return UPD ? pin[q] + (double)e[q] : pin[q]; };
+    // pin, e, f and pout stream through once per outer iteration: non-temporal, so that the
+    // last-level cache keeps r32, which the next inner solve reads first
+    auto P = [&](int q) -> double {
+        return UPD ? __builtin_nontemporal_load(pin + q) + (double)__builtin_nontemporal_load(e + q)
+                   : __builtin_nontemporal_load(pin + q);
+    };
     // this thread's halo cells: x (left edge of the patch: i - 1, right edge: i + 1), y likewise
     const bool hx = tx == 0 || tx == TX - 1, hy = ty == 0 || ty == TY - 1;
     const int ih = tx == 0 ? (i - 1) & m : (i + 1) & m, jh = ty == 0 ? (j - 1) & m : (j + 1) & m;
@@ -175,8 +184,20 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
     const int sc = (ty + 1) * SP + tx + 1;
     double zm = P(at(i, j, (z0 - 1) & m)), zcen = P(at(i, j, z0));
     double vx = hx ? P(at(ih, j, z0)) : 0.0, vy = hy ? P(at(i, jh, z0)) : 0.0;
-    double fc = f[at(i, j, z0)];
+    double fc = __builtin_nontemporal_load(f + at(i, j, z0));
     double mx = 0.0, sm = 0.0;
+    // restriction of the FP32 residuals of planes (k0, k0 + 1) of the patch into rc
+    const int lgc = lg - 1;
+    auto restrict_pair = [&](int k0) {
+        const int t = (int)threadIdx.x;
+        if (t < (TX / 2) * (TY / 2)) {
+            const int cx = t % (TX / 2), cy = t / (TX / 2);
+            const float *a = sr[k0 & 3] + 2 * cy * TX + 2 * cx, *b = sr[(k0 + 1) & 3] + 2 * cy * TX + 2 * cx;
+            const int ic = blockIdx.x * (TX / 2) + cx, jc = blockIdx.y * (TY / 2) + cy, kc = (k0 & m) >> 1;
+            rc[(kc << (2 * lgc)) | (jc << lgc) | ic] =
+                restrict_value<float>(a[0], a[1], a[TX], b[0], a[TX + 1], b[1], b[TX], b[TX + 1]);
+        }
+    };
     for (int k = z0; k < z0 + zc; ++k) {
         double *s = sp[k & 1];
         s[sc] = zcen;
@@ -189,13 +210,17 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
         if (k + 1 < z0 + zc) {
             if (hx) vxn = P(at(ih, j, kn));
             if (hy) vyn = P(at(i, jh, kn));
-            fn = f[at(i, j, kn)];
+            fn = __builtin_nontemporal_load(f + at(i, j, kn));
         }
         const int c = at(i, j, k);
         __syncthreads();
         const double rv = resid_value<double>(zcen, s[sc + 1], s[sc - 1], s[sc + SP], s[sc - SP], zp, zm, fc, h2i);
         r32[c] = (float)rv;
-        if (UPD) pout[c] = zcen;
+        if (UPD) __builtin_nontemporal_store(zcen, pout + c);
+        if (rc) {
+            sr[k & 3][ty * TX + tx] = (float)rv;
+            if (((k - z0) & 1) == 0 && k >= z0 + 2) restrict_pair(k - 2);
+        }
         mx = fmax(mx, fabs(rv));
         sm += rv * rv;
         zm = zcen;
@@ -203,6 +228,10 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
         vx = vxn;
         vy = vyn;
         fc = fn;
+    }
+    if (rc) {
+        __syncthreads();
+        restrict_pair(z0 + zc - 2);
     }
     block_max_sum(mx, sm);
     if (threadIdx.x == 0) {
