@@ -175,16 +175,19 @@ template <typename T> static int launch_restrict(hipStream_t s, T *uc, const T *
     return SG_OK;
 }
 
-constexpr int kCubicFX = 32, kCubicFY = 8, kCubicFZ = 8, kCubicNT = 256;
+constexpr int kCubicFX = 32;
 
-// FMG tricubic prolongation of a fine level n >= 32 through LDS tiles.
+// FMG tricubic prolongation of a fine level n >= 32 through LDS tiles: 32 x 16 x 16 fine cells per
+// 512-thread workgroup in FP32, 32 x 8 x 8 per 256 threads in FP64 (its LDS bound).
 template <typename T> static int launch_interp_cubic_tile(hipStream_t s, T *uf, const T *uc, int nf)
 {
-    SG_REQUIRE(nf >= kCubicFX && nf % kCubicFX == 0 && nf % kCubicFY == 0 && nf % kCubicFZ == 0,
+    constexpr bool F = std::is_same<T, float>::value;
+    constexpr int FY = F ? 16 : 8, FZ = F ? 16 : 8, NT = F ? 512 : 256;
+    SG_REQUIRE(nf >= kCubicFX && nf % kCubicFX == 0 && nf % FY == 0 && nf % FZ == 0,
                "tiled tricubic needs nf >= 32, got %d", nf);
-    const int ntiles = (nf / kCubicFX) * (nf / kCubicFY) * (nf / kCubicFZ);
-    hipLaunchKernelGGL((k_interp_cubic_tile<kCubicFX, kCubicFY, kCubicFZ, kCubicNT, T>), dim3(8 * ((ntiles + 7) / 8)),
-                       dim3(kCubicNT), 0, s, uf, uc, nf);
+    const int ntiles = (nf / kCubicFX) * (nf / FY) * (nf / FZ);
+    hipLaunchKernelGGL((k_interp_cubic_tile<kCubicFX, FY, FZ, NT, T>), dim3(8 * ((ntiles + 7) / 8)), dim3(NT), 0, s,
+                       uf, uc, nf);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
