@@ -455,6 +455,7 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
     constexpr int SX = SS::SX, SY = SS::SY, ZC = SS::ZC, NT = SS::NT;
     constexpr int H = G::H, PX = G::PX, PY = G::PY, PP = G::PP, R = G::R, CX = G::CX, CP = G::CP, CR = G::CR;
     static_assert(SX % 2 == 0 && SY % 2 == 0 && ZC % 2 == 0, "stream shape");
+    static_assert(!PROLONG || H % 2 == 0, "the coarse-value window starts on an even fine plane");
     __shared__ float su[R * PP];
     __shared__ float sf[R * PP];
     __shared__ float scz[PROLONG ? CR * CP : 1];
@@ -529,20 +530,45 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
         for (int it = 0; it < G::ICL; ++it)
             if (CP % NT == 0 || tid + it * NT < CP) dst[tid + it * NT] = rcz[PROLONG ? it : 0];
     };
-    // the fetched plane k into ring slot `slot` (adding the trilinear correction with PROLONG)
+    // PROLONG: the four coarse values of each ring cell's trilinear stencil on the two coarse
+    // planes a put reads, held in registers: win[0] plane m, win[1] plane m + 1 for the fine
+    // planes 2m + 1 and 2m + 2, so each coarse plane is read from LDS once per thread, not 4 times
+    constexpr int WI = PROLONG ? G::IL : 1;
+    float win[2][WI][4];
+    auto win_load = [&](int w, int c) {
+        const float *src = scz + (c & (CR - 1)) * CP;
+#pragma unroll
+        for (int it = 0; it < WI; ++it)
+            if (PP % NT == 0 || tid + it * NT < PP)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) win[w][it][e] = src[coff[PROLONG ? it : 0][e]];
+    };
+    // the fetched plane k into ring slot `slot` (adding the trilinear correction with PROLONG:
+    // coarse planes kc = floor(k / 2) and kc1 = kc - 1 (k even) or kc + 1 (k odd))
     auto put = [&](int k, int slot) {
         float *du = su + slot * PP, *df = sf + slot * PP;
-        const int kc = fdiv2(k), kc1 = kc + 2 * (k & 1) - 1;
-        const float *c0 = scz + (kc & (CR - 1)) * CP, *c1 = scz + (kc1 & (CR - 1)) * CP;
+        const bool odd = (k & 1) != 0;
+        if (PROLONG && odd) {
+#pragma unroll
+            for (int it = 0; it < WI; ++it)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) win[0][it][e] = win[1][it][e];
+            win_load(1, fdiv2(k) + 1);
+        }
 #pragma unroll
         for (int it = 0; it < G::IL; ++it) {
             const int q = tid + it * NT;
             if (PP % NT == 0 || q < PP) {
                 float v = ru[it];
                 if (PROLONG) {
-                    const int *o = coff[PROLONG ? it : 0];
-                    v = v + trilinear_value<float>(c0[o[0]], c0[o[1]], c0[o[2]], c1[o[0]], c1[o[2]], c1[o[1]], c0[o[3]],
-                                                   c1[o[3]]);
+                    const int w = PROLONG ? it : 0;
+                    float a[4], b[4];  // plane kc, plane kc1
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        a[e] = odd ? win[0][w][e] : win[1][w][e];
+                        b[e] = odd ? win[1][w][e] : win[0][w][e];
+                    }
+                    v = v + trilinear_value<float>(a[0], a[1], a[2], b[0], b[2], b[1], a[3], b[3]);
                 }
                 du[q] = v;
                 df[q] = rf[it];
@@ -582,6 +608,11 @@ __device__ __forceinline__ void sweep_stream(const float *__restrict__ uin, floa
         }
         cmax = cneed(zlo + 3);
         __syncthreads();
+    }
+    if (PROLONG) {
+        // zlo is even (H = 2, z0 a multiple of the even ZC): put(zlo) reads planes zlo/2 - 1, zlo/2
+        win_load(0, fdiv2(zlo) - 1);
+        win_load(1, fdiv2(zlo));
     }
     for (int k = zlo; k <= zlo + 2; ++k) {
         fetch(k);
