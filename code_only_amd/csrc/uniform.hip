@@ -82,6 +82,7 @@ struct sg_uniform {
     double *psi_alt = nullptr;                         // second phi buffer of the fused outer step
     double *pmax = nullptr, *psum = nullptr;          // norm partials
     int npart = 0;
+    int last_odd = 0;          // the previous fused mixed solve took an odd number of iterations
     double *dnorm = nullptr;   // [0] max|r| [1] sum r^2 [2] resmaxg*h^2 [3] max|f| [4] sum f^2
     double *hnorm = nullptr;   // pinned host mirror of dnorm
     hipStream_t stream = nullptr;
@@ -598,7 +599,10 @@ static int read_norms(sg_uniform *c)
     return SG_OK;
 }
 
-static int launch_update_resid(sg_uniform *c, const double *pin, const float *e, double *pout, const double *src)
+// e == nullptr: the first residual (no update); pout, when given, receives a copy of pin, and with
+// fnorms max|src| and sum src^2 land in dnorm[3..4].
+static int launch_update_resid(sg_uniform *c, const double *pin, const float *e, double *pout, const double *src,
+                               bool fnorms = false)
 {
     const int n = c->n, lg = ilog2(n);
     const long cells = n3(n);
@@ -611,15 +615,21 @@ static int launch_update_resid(sg_uniform *c, const double *pin, const float *e,
     SG_REQUIRE(c->nlev > 1 && zc % 2 == 0, "update+residual: no level to restrict into");
     const double h2i = 1.0 / (c->h * c->h);
     // the residual restricted into RHO[1] here: the fused inner solve starts from it
+    double *fpmax = fnorms ? c->pmax + c->npart : nullptr, *fpsum = fnorms ? c->psum + c->npart : nullptr;
     if (e)
         hipLaunchKernelGGL(k_update_resid<true>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
-                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1]);
+                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1], fpmax, fpsum);
     else
         hipLaunchKernelGGL(k_update_resid<false>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
-                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1]);
+                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1], fpmax, fpsum);
     SG_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, nb, c->dnorm);
     SG_CHECK_LAUNCH();
+    if (fnorms) {
+        hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, (const double *)fpmax,
+                           (const double *)fpsum, nb, c->dnorm + 3);
+        SG_CHECK_LAUNCH();
+    }
     return SG_OK;
 }
 
@@ -720,12 +730,19 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
     std::memset(st, 0, sizeof(*st));
     const bool mixed = p->precision == SG_PREC_MIXED;
     SG_HIP(hipEventRecord(c->ev0, c->stream));
-    SG_TRY(launch_norm(c, src, n3(c->n), c->dnorm + 3));
     const bool pp = pingpong(c, p);
-    if (pp) SG_TRY(launch_update_resid(c, psi, nullptr, nullptr, src));
-    else if (mixed) SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
-    else SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
-    double *cur = psi;  // buffer holding the current phi (psi or psi_alt)
+    // With the fused update the iterations alternate between psi and psi_alt, so an odd count ends
+    // in psi_alt and costs a final copy.  When the previous solve on this context took an odd
+    // count, the first residual step also copies psi into psi_alt and the iterations start there.
+    const bool start_alt = pp && c->last_odd;
+    if (pp) {
+        SG_TRY(launch_update_resid(c, psi, nullptr, start_alt ? c->psi_alt : nullptr, src, true));
+    } else {
+        SG_TRY(launch_norm(c, src, n3(c->n), c->dnorm + 3));
+        if (mixed) SG_TRY((launch_resid<double, float>(c, c->f.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+        else SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
+    }
+    double *cur = start_alt ? c->psi_alt : psi;  // buffer holding the current phi (psi or psi_alt)
     SG_TRY(read_norms(c));
     const double fmax = c->hnorm[3], fl2 = std::sqrt(c->hnorm[4]);
     const double rhomax = c->hnorm[0];
@@ -752,6 +769,7 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
         ++it;
         st->resmaxg = c->hnorm[2] / (c->h * c->h);
     }
+    if (pp) c->last_odd = it & 1;
     if (cur != psi) SG_HIP(hipMemcpyAsync(psi, cur, sizeof(double) * n3(c->n), hipMemcpyDeviceToDevice, c->stream));
     SG_HIP(hipEventRecord(c->ev1, c->stream));
     SG_HIP(hipEventSynchronize(c->ev1));
@@ -844,8 +862,9 @@ static int create(sg_uniform **out, int n, double h, int precisions)
             SG_HIP(hipEventCreate(&c->ev1));
             SG_HIP(hipEventCreateWithFlags(&c->evn, hipEventDisableTiming));
             c->npart = (int)nblocks(n3(n));
-            SG_HIP(hipMalloc(&c->pmax, sizeof(double) * c->npart));
-            SG_HIP(hipMalloc(&c->psum, sizeof(double) * c->npart));
+            // two sets of partials: the residual's and (first residual step) the source's
+            SG_HIP(hipMalloc(&c->pmax, sizeof(double) * 2 * c->npart));
+            SG_HIP(hipMalloc(&c->psum, sizeof(double) * 2 * c->npart));
             SG_HIP(hipMalloc(&c->dnorm, sizeof(double) * 8));
             SG_HIP(hipMemsetAsync(c->dnorm, 0, sizeof(double) * 8, c->stream));
             SG_HIP(hipHostMalloc(&c->hnorm, sizeof(double) * 8, hipHostMallocDefault));

@@ -146,7 +146,8 @@ __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin
 // r = f - L pout (mg_poisson.F90:17-27) stored as FP32 for the next inner solve, with the norm
 // partials of r.  Neighbour values are formed as pin + e, exactly the values pout receives, so the
 // result equals the separate update and residual kernels bit for bit; pin and pout must differ.
-// UPD = false: r = f - L pin only.  With rc, the FP32 residual is also restricted into rc (the
+// UPD = false: r = f - L pin only, and pout (when given) a copy of pin.  With fpmax, the partials
+// of max|f| and sum f^2 as well.  With rc, the FP32 residual is also restricted into rc (the
 // next level's right-hand side, restriction.F90:132-147 operand order, as k_restrict forms it):
 // each plane's FP32 residuals go to one of four LDS planes, and the restriction of planes
 // (k - 2, k - 1) runs at plane k, after its barrier.  A workgroup owns a 64 x 8 patch of (i, j) columns and marches
@@ -160,7 +161,8 @@ template <bool UPD>
 __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__restrict__ pin, const float *__restrict__ e,
                                                              double *__restrict__ pout, const double *__restrict__ f,
                                                              float *__restrict__ r32, int lg, double h2i, int zc,
-                                                             double *pmax, double *psum, float *__restrict__ rc)
+                                                             double *pmax, double *psum, float *__restrict__ rc,
+                                                             double *fpmax, double *fpsum)
 {
     constexpr int TX = kUrTX, TY = kUrTY, SP = TX + 2;
     __shared__ double sp[2][(TY + 2) * SP];
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
     double zm = P(at(i, j, (z0 - 1) & m)), zcen = P(at(i, j, z0));
     double vx = hx ? P(at(ih, j, z0)) : 0.0, vy = hy ? P(at(i, jh, z0)) : 0.0;
     double fc = __builtin_nontemporal_load(f + at(i, j, z0));
-    double mx = 0.0, sm = 0.0;
+    double mx = 0.0, sm = 0.0, fmx = 0.0, fsm = 0.0;
     // restriction of the FP32 residuals of planes (k0, k0 + 1) of the patch into rc
     const int lgc = lg - 1;
     auto restrict_pair = [&](int k0) {
@@ -216,7 +218,11 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
         __syncthreads();
         const double rv = resid_value<double>(zcen, s[sc + 1], s[sc - 1], s[sc + SP], s[sc - SP], zp, zm, fc, h2i);
         r32[c] = (float)rv;
-        if (UPD) __builtin_nontemporal_store(zcen, pout + c);
+        if (pout) __builtin_nontemporal_store(zcen, pout + c);
+        if (fpmax) {
+            fmx = fmax(fmx, fabs(fc));
+            fsm += fc * fc;
+        }
         if (rc) {
             sr[k & 3][ty * TX + tx] = (float)rv;
             if (((k - z0) & 1) == 0 && k >= z0 + 2) restrict_pair(k - 2);
@@ -233,11 +239,18 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
         __syncthreads();
         restrict_pair(z0 + zc - 2);
     }
+    const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     block_max_sum(mx, sm);
     if (threadIdx.x == 0) {
-        const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         pmax[b] = mx;
         psum[b] = sm;
+    }
+    if (fpmax) {
+        block_max_sum(fmx, fsm);
+        if (threadIdx.x == 0) {
+            fpmax[b] = fmx;
+            fpsum[b] = fsm;
+        }
     }
 }
 

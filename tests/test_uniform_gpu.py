@@ -210,6 +210,31 @@ def test_fused_mixed_equals_unfused_bitwise(gpu, n, use_graph, coarse):
     s.close()
 
 
+@pytest.mark.parametrize("n", [64, 128])
+def test_repeated_fused_solves_bitwise(gpu, n):
+    """Consecutive fused mixed solves on one context start their phi ping-pong from psi or from its
+    copy in the second buffer (chosen by the previous solve's iteration parity, so that an odd count
+    ends in psi); every solve gives the same phi, statistics and Resmaxg bit for bit, from a zero
+    and from a non-zero initial guess."""
+    f = problems.clumpy_rhs(n, seed=5)
+    ft = T(f, gpu)
+    g0 = T(0.01 * rand(n, 6), gpu)
+    s = UniformSolver(n, precisions=SG_PREC_MIXED)
+    for init in (torch.zeros_like(ft), g0):
+        outs = []
+        for _ in range(3):
+            psi = init.clone()
+            st = s.cycle(psi, ft, tol=1e-10, maxit=30, precision=SG_PREC_MIXED)
+            outs.append((psi, st))
+        (p0, s0) = outs[0]
+        assert s0["status"] == 0
+        for p1, s1 in outs[1:]:
+            assert s1["iterations"] == s0["iterations"]
+            assert s1["resmaxg"] == s0["resmaxg"] and s1["rel_linf"] == s0["rel_linf"]
+            assert torch.equal(p1, p0)
+    s.close()
+
+
 @pytest.mark.parametrize("n", [32, 128])
 def test_exact_coarse_solve_contracts_at_least_as_well(gpu, n):
     """Solving the 16^3 level exactly (instead of the V-cycle below it) keeps the mixed solve's
