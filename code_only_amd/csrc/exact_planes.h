@@ -23,12 +23,13 @@
 //   two halves of k: per k one float4 of Q and one float4 of line values feed 16 multiply-adds.
 //   Few workgroups of much work each: the form for many planes at once.
 // * split (k_exact_xy_split, k_exact_z_split; the single-grid solver): G = plane_groups(n)
-//   workgroups of 256 threads per plane (row), split by the range of the first transformed index.
+//   workgroups of 512 threads per plane (row), split by the range of the first transformed index.
 //   Workgroup g of plane k forms the x transform X[mi][j] only for its n / G frequencies mi, then
 //   the y transform of those columns; workgroup g of row j transforms its n / G columns i along
-//   z, in place.  A thread forms four outputs (four consecutive frequencies) from one float4 of Q
-//   per k.  G times as many workgroups of 1 / G of the work: the form for one level, whose n
-//   planes alone would leave most of the chip idle.
+//   z, in place.  A task is four outputs (four consecutive frequencies), formed from one float4 of
+//   Q per k by two threads, one per half of k.  G times as many workgroups of 1 / G of the work,
+//   and two waves per SIMD on the serial chains: the form for one level, whose n planes alone
+//   would leave most of the chip idle.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -40,9 +41,6 @@ constexpr int kPlaneBlocks = kPlaneThreads / 2;   // 4 x 4 output blocks per pla
 constexpr int kPlanePitch = kPlaneMaxN + 4;
 constexpr int kPlaneLoads = kPlaneMaxN * kPlaneMaxN / kPlaneThreads;  // plane values per thread
 static_assert((kPlaneMaxN / 4) * (kPlaneMaxN / 4) <= kPlaneBlocks, "one thread pair per 4 x 4 block");
-
-constexpr int kSplitThreads = 256;
-constexpr int kSplitLoads = kPlaneMaxN * kPlaneMaxN / kSplitThreads;
 
 // Workgroups per plane (row) of the split form: 4 when n / 4 is a multiple of 4, else 2 or 1
 // likewise (each workgroup's frequency range a multiple of 4); n % 4 == 0.
@@ -208,28 +206,58 @@ static __global__ void __launch_bounds__(kPlaneThreads) k_exact_z(float *w, cons
 
 // ---- split form ----------------------------------------------------------------------------------
 
-// acc[r] = sum_k Q[m0 + r][k] * V[k * sk + col * sc] (r < 4): (k < n/2 chain) + (k >= n/2 chain).
-__device__ __forceinline__ void ep_dot4(const float *Qs, const float *V, int sk, int sc, int n, int m0, int col,
-                                        float (&acc)[4])
+// Chain h (0: k < n/2, 1: k >= n/2) of acc[r] = sum_k Q[m0 + r][k] * V[k * sk + col * sc], r < 4.
+__device__ __forceinline__ void ep_dot4_half(const float *Qs, const float *V, int sk, int sc, int n, int m0, int col,
+                                             int h, float (&acc)[4])
 {
-    float lo[4] = {0.f, 0.f, 0.f, 0.f}, hi[4] = {0.f, 0.f, 0.f, 0.f};
+    acc[0] = acc[1] = acc[2] = acc[3] = 0.f;
     const float *v = V + col * sc;
-    const int kh = n / 2;
-    for (int k = 0; k < kh; ++k) {
+    const int kh = n / 2, k0 = h * kh;
+    for (int k = k0; k < k0 + kh; k += 2) {
         const float4 q0 = *reinterpret_cast<const float4 *>(Qs + k * kPlanePitch + m0);
-        const float4 q1 = *reinterpret_cast<const float4 *>(Qs + (k + kh) * kPlanePitch + m0);
-        const float x0 = v[k * sk], x1 = v[(k + kh) * sk];
-        lo[0] = __builtin_fmaf(q0.x, x0, lo[0]);
-        lo[1] = __builtin_fmaf(q0.y, x0, lo[1]);
-        lo[2] = __builtin_fmaf(q0.z, x0, lo[2]);
-        lo[3] = __builtin_fmaf(q0.w, x0, lo[3]);
-        hi[0] = __builtin_fmaf(q1.x, x1, hi[0]);
-        hi[1] = __builtin_fmaf(q1.y, x1, hi[1]);
-        hi[2] = __builtin_fmaf(q1.z, x1, hi[2]);
-        hi[3] = __builtin_fmaf(q1.w, x1, hi[3]);
+        const float4 q1 = *reinterpret_cast<const float4 *>(Qs + (k + 1) * kPlanePitch + m0);
+        const float x0 = v[k * sk], x1 = v[(k + 1) * sk];
+        acc[0] = __builtin_fmaf(q0.x, x0, acc[0]);
+        acc[1] = __builtin_fmaf(q0.y, x0, acc[1]);
+        acc[2] = __builtin_fmaf(q0.z, x0, acc[2]);
+        acc[3] = __builtin_fmaf(q0.w, x0, acc[3]);
+        acc[0] = __builtin_fmaf(q1.x, x1, acc[0]);
+        acc[1] = __builtin_fmaf(q1.y, x1, acc[1]);
+        acc[2] = __builtin_fmaf(q1.z, x1, acc[2]);
+        acc[3] = __builtin_fmaf(q1.w, x1, acc[3]);
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] = lo[r] + hi[r];
+}
+
+// One pass of the split form over `tasks` tasks of four outputs each: thread t < kSplitTasks forms
+// the first-half chain of task t (+ trips of kSplitTasks), thread t + kSplitTasks the second-half
+// chain, which it hands over through `part`; fin(task, acc) receives (first half) + (second half).
+// Every thread runs the same trips (each holds a barrier).
+constexpr int kSplitTasks = 256;
+constexpr int kSplitThreads = 2 * kSplitTasks;
+constexpr int kSplitLoads = kPlaneMaxN * kPlaneMaxN / kSplitThreads;
+
+template <typename Task, typename Fin>
+__device__ __forceinline__ void ep_split_pass(const float *Qs, float *part, int n, int tasks, Task task, Fin fin)
+{
+    const int h = (int)threadIdx.x / kSplitTasks, tl = (int)threadIdx.x % kSplitTasks;
+    for (int base = 0; base < tasks; base += kSplitTasks) {
+        const int t = base + tl;
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        if (t < tasks) {
+            const float *V;
+            int sk, sc, m0, col;
+            task(t, V, sk, sc, m0, col);
+            ep_dot4_half(Qs, V, sk, sc, n, m0, col, h, acc);
+            if (h == 1) *reinterpret_cast<float4 *>(part + 4 * tl) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        }
+        __syncthreads();
+        if (t < tasks && h == 0) {
+            const float4 q = *reinterpret_cast<const float4 *>(part + 4 * tl);
+            acc[0] += q.x; acc[1] += q.y; acc[2] += q.z; acc[3] += q.w;
+            fin(t, acc);
+        }
+        __syncthreads();
+    }
 }
 
 // Workgroup wg = plane * G + g of the dense n^3 array in (i fastest), G = plane_groups(n): for the
@@ -242,6 +270,7 @@ __device__ __forceinline__ void exact_xy_split(const float *__restrict__ in, flo
     __shared__ __align__(16) float Qs[kPlaneMaxN * kPlanePitch];
     __shared__ float S[kPlaneMaxN * PS];
     __shared__ __align__(16) float Xs[kPlaneMaxN * kPlanePitch];
+    __shared__ __align__(16) float part[4 * kSplitTasks];
     const int G = plane_groups(n), mq = n / G, plane = wg / G, g = wg % G;
     const int px = mq + 4;                                                  // X rows [j][mi local]
     const long base = (long)plane * n * n;
@@ -260,21 +289,25 @@ __device__ __forceinline__ void exact_xy_split(const float *__restrict__ in, flo
     }
     __syncthreads();
     // x pass: task (line j, block b of four local frequencies)
-    for (int t = (int)threadIdx.x; t < n * (mq / 4); t += kSplitThreads) {
-        const int j = t % n, b = t / n;
-        float acc[4];
-        ep_dot4(Qs, S, 1, PS, n, g * mq + 4 * b, j, acc);
-        *reinterpret_cast<float4 *>(Xs + j * px + 4 * b) = make_float4(acc[0], acc[1], acc[2], acc[3]);
-    }
-    __syncthreads();
+    ep_split_pass(
+        Qs, part, n, n * (mq / 4),
+        [&](int t, const float *&V, int &sk, int &sc, int &m0, int &col) {
+            V = S; sk = 1; sc = PS; m0 = g * mq + 4 * (t / n); col = t % n;
+        },
+        [&](int t, const float (&acc)[4]) {
+            *reinterpret_cast<float4 *>(Xs + (t % n) * px + 4 * (t / n)) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        });
     // y pass: task (local frequency mi, block of four mj)
-    for (int t = (int)threadIdx.x; t < mq * (n / 4); t += kSplitThreads) {
-        const int mi = t % mq, mj0 = 4 * (t / mq);
-        float acc[4];
-        ep_dot4(Qs, Xs, px, 1, n, mj0, mi, acc);
+    ep_split_pass(
+        Qs, part, n, mq * (n / 4),
+        [&](int t, const float *&V, int &sk, int &sc, int &m0, int &col) {
+            V = Xs; sk = px; sc = 1; m0 = 4 * (t / mq); col = t % mq;
+        },
+        [&](int t, const float (&acc)[4]) {
+            const int mi = t % mq, mj0 = 4 * (t / mq);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) out[base + (long)(mj0 + r) * n + g * mq + mi] = acc[r];
-    }
+            for (int r = 0; r < 4; ++r) out[base + (long)(mj0 + r) * n + g * mq + mi] = acc[r];
+        });
 }
 
 // Workgroup wg = row * G + g of the dense n^3 array w: for the columns i in [g n/G, (g+1) n/G)
@@ -286,6 +319,7 @@ __device__ __forceinline__ void exact_z_split(float *__restrict__ w, const float
     __shared__ __align__(16) float Qs[kPlaneMaxN * kPlanePitch];
     __shared__ float S[kPlaneMaxN * PC];
     __shared__ float Zs[kPlaneMaxN * PC];
+    __shared__ __align__(16) float part[4 * kSplitTasks];
     const int G = plane_groups(n), mq = n / G, j = wg / G, g = wg % G;
     const long nn = (long)n * n, col0 = (long)j * n + g * mq;
     const int tasks = mq * (n / 4);
@@ -296,11 +330,13 @@ __device__ __forceinline__ void exact_z_split(float *__restrict__ w, const float
         v[it] = 0.f;
         if (c < n * mq) v[it] = w[(c / mq) * nn + col0 + c % mq];
     }
-    // 1 / Lambda of the thread's first task, loaded ahead
-    const int t0 = (int)threadIdx.x;
+    // 1 / Lambda of the first trip's task of this thread (used by the first-half threads), ahead
+    const int t0 = (int)threadIdx.x % kSplitTasks;
+    const bool lo = (int)threadIdx.x < kSplitTasks;
     float d0[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) d0[r] = t0 < tasks ? inv[(4 * (t0 / mq) + r) * nn + col0 + t0 % mq] : 0.f;
+    for (int r = 0; r < 4; ++r)
+        d0[r] = lo && t0 < tasks ? inv[(4 * (t0 / mq) + r) * nn + col0 + t0 % mq] : 0.f;
     ep_stage_q<kSplitThreads>(Qs, Q, n);
 #pragma unroll
     for (int it = 0; it < kSplitLoads; ++it) {
@@ -308,24 +344,24 @@ __device__ __forceinline__ void exact_z_split(float *__restrict__ w, const float
         if (c < n * mq) S[(c / mq) * PC + c % mq] = v[it];
     }
     __syncthreads();
-    for (int t = t0; t < tasks; t += kSplitThreads) {
+    auto task_of = [&](const float *V) {
+        return [=](int t, const float *&Vo, int &sk, int &sc, int &m0, int &col) {
+            Vo = V; sk = PC; sc = 1; m0 = 4 * (t / mq); col = t % mq;
+        };
+    };
+    ep_split_pass(Qs, part, n, tasks, task_of(S), [&](int t, const float (&acc)[4]) {
         const int ii = t % mq, m0 = 4 * (t / mq);
-        float acc[4];
-        ep_dot4(Qs, S, PC, 1, n, m0, ii, acc);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const float d = t == t0 ? d0[r] : inv[(m0 + r) * nn + col0 + ii];
             Zs[(m0 + r) * PC + ii] = acc[r] * d;
         }
-    }
-    __syncthreads();
-    for (int t = t0; t < tasks; t += kSplitThreads) {
+    });
+    ep_split_pass(Qs, part, n, tasks, task_of(Zs), [&](int t, const float (&acc)[4]) {
         const int ii = t % mq, m0 = 4 * (t / mq);
-        float acc[4];
-        ep_dot4(Qs, Zs, PC, 1, n, m0, ii, acc);
 #pragma unroll
         for (int r = 0; r < 4; ++r) w[(m0 + r) * nn + col0 + ii] = acc[r];
-    }
+    });
 }
 
 static __global__ void __launch_bounds__(kSplitThreads) k_exact_xy_split(const float *in, float *out, const float *Q,
