@@ -85,6 +85,7 @@ struct sg_uniform {
     int last_odd = 0;          // the previous fused mixed solve took an odd number of iterations
     double *dnorm = nullptr;   // [0] max|r| [1] sum r^2 [2] resmaxg*h^2 [3] max|f| [4] sum f^2
     double *hnorm = nullptr;   // pinned host mirror of dnorm
+    double *hnorm_dev = nullptr;  // its device-side address (hipHostGetDevicePointer)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evn = nullptr;
     // graphs of one mixed-precision outer iteration (one per phi buffer parity), keyed on the
@@ -585,12 +586,13 @@ template <typename Tu> static int launch_update(sg_uniform *c, double *psi, cons
     return SG_OK;
 }
 
-// The norms to the host.  The host polls an event rather than blocking in a stream synchronize:
-// the outer loop waits on this between iterations, and polling wakes it as soon as the copy is
-// done (the GPU idles until the next iteration is launched).
-static int read_norms(sg_uniform *c)
+// The norms to the host (published: the last kernel already wrote them into hnorm).  The host
+// polls an event rather than blocking in a stream synchronize: the outer loop waits on this
+// between iterations, and polling wakes it as soon as the values are there (the GPU idles until
+// the next iteration is launched).
+static int read_norms(sg_uniform *c, bool published = false)
 {
-    SG_HIP(hipMemcpyAsync(c->hnorm, c->dnorm, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (!published) SG_HIP(hipMemcpyAsync(c->hnorm, c->dnorm, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SG_HIP(hipEventRecord(c->evn, c->stream));
     hipError_t q;
     while ((q = hipEventQuery(c->evn)) == hipErrorNotReady) {
@@ -623,13 +625,14 @@ static int launch_update_resid(sg_uniform *c, const double *pin, const float *e,
         hipLaunchKernelGGL(k_update_resid<false>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
                            lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1], fpmax, fpsum);
     SG_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, nb, c->dnorm);
-    SG_CHECK_LAUNCH();
     if (fnorms) {
         hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, (const double *)fpmax,
-                           (const double *)fpsum, nb, c->dnorm + 3);
+                           (const double *)fpsum, nb, c->dnorm + 3, (double *)nullptr);
         SG_CHECK_LAUNCH();
     }
+    // the residual's norms, published with Resmaxg and the source norms straight into hnorm
+    hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(1024), 0, c->stream, c->pmax, c->psum, nb, c->dnorm, c->hnorm_dev);
+    SG_CHECK_LAUNCH();
     return SG_OK;
 }
 
@@ -743,7 +746,7 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
         else SG_TRY((launch_resid<double, double>(c, c->d.RHO[0], psi, src, c->n, c->h, c->dnorm)));
     }
     double *cur = start_alt ? c->psi_alt : psi;  // buffer holding the current phi (psi or psi_alt)
-    SG_TRY(read_norms(c));
+    SG_TRY(read_norms(c, pp));
     const double fmax = c->hnorm[3], fl2 = std::sqrt(c->hnorm[4]);
     const double rhomax = c->hnorm[0];
     int it = 0;
@@ -765,7 +768,7 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
         double *next = pp ? (cur == psi ? c->psi_alt : psi) : cur;
         SG_TRY(run_outer_iteration(c, cur, next, src, p, cur == psi ? 0 : 1));
         cur = next;
-        SG_TRY(read_norms(c));
+        SG_TRY(read_norms(c, pp));
         ++it;
         st->resmaxg = c->hnorm[2] / (c->h * c->h);
     }
@@ -868,6 +871,7 @@ static int create(sg_uniform **out, int n, double h, int precisions)
             SG_HIP(hipMalloc(&c->dnorm, sizeof(double) * 8));
             SG_HIP(hipMemsetAsync(c->dnorm, 0, sizeof(double) * 8, c->stream));
             SG_HIP(hipHostMalloc(&c->hnorm, sizeof(double) * 8, hipHostMallocDefault));
+            SG_HIP(hipHostGetDevicePointer((void **)&c->hnorm_dev, c->hnorm, 0));
             SG_TRY(coarse_levels(c));
             if (precisions & SG_PREC_FP64) SG_TRY(alloc_levels<double>(c, c->d));
             if (precisions & SG_PREC_MIXED) {

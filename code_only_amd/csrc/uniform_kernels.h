@@ -255,13 +255,25 @@ __global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__res
 }
 
 // Second stage of the norm: one block reduces np partials in a fixed order -> out[0]=max, out[1]=sum.
+// With host (pinned host memory): host[0..1] = out[0..1] and host[2..4] = out[2..4] as well, so
+// the host reads the five norm values without a separate copy (out[2..4] are final by then).
 static __global__ void __launch_bounds__(1024) k_norm_finalize(const double *pmax, const double *psum, int np,
-                                                        double *out)
+                                                        double *out, double *host = nullptr)
 {
     double mx = 0.0, sm = 0.0;
     for (int q = threadIdx.x; q < np; q += blockDim.x) { mx = fmax(mx, pmax[q]); sm += psum[q]; }
     block_max_sum(mx, sm);
-    if (threadIdx.x == 0) { out[0] = mx; out[1] = sm; }
+    if (threadIdx.x == 0) {
+        out[0] = mx;
+        out[1] = sm;
+        if (host) {
+            host[0] = mx;
+            host[1] = sm;
+            host[2] = out[2];
+            host[3] = out[3];
+            host[4] = out[4];
+        }
+    }
 }
 
 // restriction.F90:132-147, one thread per coarse cell (nc = 2^lgc).
