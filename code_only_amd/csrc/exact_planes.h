@@ -42,9 +42,9 @@ constexpr int kPlanePitch = kPlaneMaxN + 4;
 constexpr int kPlaneLoads = kPlaneMaxN * kPlaneMaxN / kPlaneThreads;  // plane values per thread
 static_assert((kPlaneMaxN / 4) * (kPlaneMaxN / 4) <= kPlaneBlocks, "one thread pair per 4 x 4 block");
 
-// Workgroups per plane (row) of the split form: 4 when n / 4 is a multiple of 4, else 2 or 1
-// likewise (each workgroup's frequency range a multiple of 4); n % 4 == 0.
-__host__ __device__ constexpr int plane_groups(int n) { return n % 16 == 0 ? 4 : (n % 8 == 0 ? 2 : 1); }
+// Workgroups per plane (row) of the split form, each owning n / 4 frequencies (a multiple of 4):
+// the split form takes n % 16 == 0 (the single-grid levels are powers of two >= 32).
+__host__ __device__ constexpr int plane_groups(int) { return 4; }
 
 // Q (n x n, row-major, global) into LDS rows of pitch kPlanePitch; every load issued first.
 template <int NT> __device__ __forceinline__ void ep_stage_q(float *Qs, const float *Q, int n)

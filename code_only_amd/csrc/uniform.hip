@@ -475,7 +475,7 @@ static int coarse_levels(sg_uniform *c)
     c->lb = lb;
     if (lb != lc) {
         const int m = c->nl[lb];
-        SG_REQUIRE(m % 4 == 0 && m <= kPlaneMaxN, "plane-parallel exact solve needs n %% 4 == 0, n <= 64");
+        SG_REQUIRE(m % 16 == 0 && m <= kPlaneMaxN, "plane-parallel exact solve needs n %% 16 == 0, n <= 64");
         std::vector<double> Q((size_t)m * m), lam(m);
         hartley_basis(m, c->hl[lb], Q.data(), lam.data());
         std::vector<float> Qf(Q.begin(), Q.end()), inv((size_t)m * m * m);
@@ -508,7 +508,8 @@ static int exact_bottom(sg_uniform *c, int mode, int ncycle, int npre, int npost
     const int lb = c->lb, n = c->nl[lb];
     SG_REQUIRE(c->coarse.exact, "exact coarse solve needs a level below the finest");
     if (lb == c->lc) return launch_coarse_k(c, true, c->f.RHO[lb], c->f.U[lb], mode, ncycle, npre, npost, nullptr);
-    SG_REQUIRE(n <= kPlaneMaxN && c->d_qx && c->d_invx && c->U1[lb], "plane-parallel exact solve not set up");
+    SG_REQUIRE(n <= kPlaneMaxN && n % 16 == 0 && c->d_qx && c->d_invx && c->U1[lb],
+               "plane-parallel exact solve not set up (n = %d)", n);
     const dim3 grid(n * plane_groups(n)), block(kSplitThreads);
     hipLaunchKernelGGL(k_exact_xy_split, grid, block, 0, c->stream, (const float *)c->f.RHO[lb], c->U1[lb],
                        (const float *)c->d_qx, n);
