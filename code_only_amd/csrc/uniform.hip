@@ -609,21 +609,30 @@ static int launch_update_resid(sg_uniform *c, const double *pin, const float *e,
 {
     const int n = c->n, lg = ilog2(n);
     const long cells = n3(n);
-    int zc = (int)std::max<long>(4, std::min<long>(32, cells / ((long)kUrThreads * kNormBlocks)));
+    // z chunk: about kNormBlocks patches of 512 cells per plane
+    int zc = (int)std::max<long>(4, std::min<long>(32, cells / (512L * kNormBlocks)));
     zc = std::min(zc, n);
-    SG_REQUIRE(n >= kUrTX && n % kUrTX == 0 && n % kUrTY == 0 && n % zc == 0, "update+residual: bad n = %d", n);
-    const dim3 grid(n / kUrTX, n / kUrTY, n / zc);
+    const bool wide = n % 256 == 0;  // 256 x 4 patches, else 64 x 8
+    const int tx = wide ? 256 : 64, ty = wide ? 4 : 8;
+    SG_REQUIRE(n >= 64 && n % tx == 0 && n % ty == 0 && n % zc == 0, "update+residual: bad n = %d", n);
+    const dim3 grid(n / tx, n / ty, n / zc), block(tx * ty);
     const int nb = (int)(grid.x * grid.y * grid.z);
     SG_REQUIRE(nb <= c->npart, "norm partial buffer too small");
     SG_REQUIRE(c->nlev > 1 && zc % 2 == 0, "update+residual: no level to restrict into");
     const double h2i = 1.0 / (c->h * c->h);
     // the residual restricted into RHO[1] here: the fused inner solve starts from it
     double *fpmax = fnorms ? c->pmax + c->npart : nullptr, *fpsum = fnorms ? c->psum + c->npart : nullptr;
-    if (e)
-        hipLaunchKernelGGL(k_update_resid<true>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
+    if (e && wide)
+        hipLaunchKernelGGL((k_update_resid<true, 256, 4>), grid, block, 0, c->stream, pin, e, pout, src, c->f.RHO[0],
+                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1], fpmax, fpsum);
+    else if (e)
+        hipLaunchKernelGGL((k_update_resid<true, 64, 8>), grid, block, 0, c->stream, pin, e, pout, src, c->f.RHO[0],
+                           lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1], fpmax, fpsum);
+    else if (wide)
+        hipLaunchKernelGGL((k_update_resid<false, 256, 4>), grid, block, 0, c->stream, pin, e, pout, src, c->f.RHO[0],
                            lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1], fpmax, fpsum);
     else
-        hipLaunchKernelGGL(k_update_resid<false>, grid, dim3(kUrThreads), 0, c->stream, pin, e, pout, src, c->f.RHO[0],
+        hipLaunchKernelGGL((k_update_resid<false, 64, 8>), grid, block, 0, c->stream, pin, e, pout, src, c->f.RHO[0],
                            lg, h2i, zc, c->pmax, c->psum, c->f.RHO[1], fpmax, fpsum);
     SG_CHECK_LAUNCH();
     if (fnorms) {

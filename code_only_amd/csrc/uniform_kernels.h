@@ -154,17 +154,19 @@ __global__ void __launch_bounds__(256) k_resid(Tout *__restrict__ res, const Tin
 // through zc planes: each thread forms its cell of the next plane (and, on the patch edge, one
 // halo cell in x and/or y) while the current plane sits in a double-buffered LDS plane with a
 // one-cell halo, so every value is loaded from memory once and one barrier per plane suffices; the
-// z neighbours and the next plane's source value stay in registers.  n = 2^lg >= 64.
-constexpr int kUrTX = 64, kUrTY = 8, kUrThreads = kUrTX * kUrTY;
-
-template <bool UPD>
-__global__ void __launch_bounds__(kUrThreads) k_update_resid(const double *__restrict__ pin, const float *__restrict__ e,
-                                                             double *__restrict__ pout, const double *__restrict__ f,
-                                                             float *__restrict__ r32, int lg, double h2i, int zc,
-                                                             double *pmax, double *psum, float *__restrict__ rc,
-                                                             double *fpmax, double *fpsum)
+// z neighbours and the next plane's source value stay in registers.  n = 2^lg >= 64.  Patches of
+// 64 x 8 columns, or 256 x 4 when n % 256 == 0: an x-halo cell costs a whole cache line of pin
+// and of e, so patches as wide as the grid (whose x neighbours wrap into the patch) trade those
+// for a second y-halo row per four.
+template <bool UPD, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_update_resid(const double *__restrict__ pin, const float *__restrict__ e,
+                                                          double *__restrict__ pout, const double *__restrict__ f,
+                                                          float *__restrict__ r32, int lg, double h2i, int zc,
+                                                          double *pmax, double *psum, float *__restrict__ rc,
+                                                          double *fpmax, double *fpsum)
 {
-    constexpr int TX = kUrTX, TY = kUrTY, SP = TX + 2;
+    constexpr int SP = TX + 2;
+    static_assert(TY % 2 == 0 && TX % 2 == 0 && TX * TY <= 1024, "update+residual patch");
     __shared__ double sp[2][(TY + 2) * SP];
     __shared__ float sr[4][TY * TX];
     const int n = 1 << lg, m = n - 1;

@@ -210,6 +210,25 @@ def test_fused_mixed_equals_unfused_bitwise(gpu, n, use_graph, coarse):
     s.close()
 
 
+def test_fused_mixed_equals_unfused_bitwise_256(gpu):
+    """At 256^3 the fused update + residual step runs in 256 x 4 patches (grid-wide rows); it still
+    reproduces the separate update, residual and restriction kernels bit for bit."""
+    n = 256
+    f = problems.clumpy_rhs(n, seed=3)
+    s = UniformSolver(n, precisions=SG_PREC_MIXED)
+    ft = T(f, gpu)
+    out = []
+    for fused in (False, True):
+        psi = torch.zeros_like(ft)
+        st = s.cycle(psi, ft, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, fused=fused)
+        out.append((psi, st))
+    (p0, s0), (p1, s1) = out
+    assert s0["iterations"] == s1["iterations"]
+    assert s0["resmaxg"] == s1["resmaxg"]
+    assert torch.equal(p0, p1)
+    s.close()
+
+
 @pytest.mark.parametrize("n", [64, 128])
 def test_repeated_fused_solves_bitwise(gpu, n):
     """Consecutive fused mixed solves on one context start their phi ping-pong from psi or from its
