@@ -544,17 +544,31 @@ int op_src_norms(sg_batch *c)
 template <typename Te> int op_outer_resid(sg_batch *c, int parity, bool upd, int ce)
 {
     const Pairs all = members(c, [](int) { return 0; });
+    // the z-marching form when every problem's n is a multiple of 32 (chunks of 16 planes)
+    bool zform = true;
+    for (auto &q : all) zform = zform && c->n[q.first] % kOzX == 0 && c->n[q.first] % 16 == 0;
+    const int zc = 16;
     const Item *d;
     int cnt, tot;
-    SG_TRY(take(c, all, [&](int p, int) { return nblk_cells(cells(c->n[p])); }, &d, &cnt, &tot));
+    SG_TRY(take(c, all, [&](int p, int) {
+        const int n = c->n[p];
+        return zform ? (n / kOzX) * (n / kOzY) * (n / zc) : nblk_cells(cells(n));
+    }, &d, &cnt, &tot));
     if (!cnt) return SG_OK;
     const BL<Te> L = bl<Te>(c, d, cnt);
-    if (upd)
-        hipLaunchKernelGGL((kb_outer_resid<Te, true>), dim3(tot), dim3(kThreads), 0, c->stream, L,
-                           (const Outer *)c->d_outer, parity, ce, c->d_pmax, c->d_psum);
+    const Outer *O = (const Outer *)c->d_outer;
+    if (zform && upd)
+        hipLaunchKernelGGL((kb_outer_resid_z<Te, true>), dim3(tot), dim3(kOzX * kOzY), 0, c->stream, L, O, parity, ce,
+                           zc, c->d_pmax, c->d_psum);
+    else if (zform)
+        hipLaunchKernelGGL((kb_outer_resid_z<Te, false>), dim3(tot), dim3(kOzX * kOzY), 0, c->stream, L, O, parity, ce,
+                           zc, c->d_pmax, c->d_psum);
+    else if (upd)
+        hipLaunchKernelGGL((kb_outer_resid<Te, true>), dim3(tot), dim3(kThreads), 0, c->stream, L, O, parity, ce,
+                           c->d_pmax, c->d_psum);
     else
-        hipLaunchKernelGGL((kb_outer_resid<Te, false>), dim3(tot), dim3(kThreads), 0, c->stream, L,
-                           (const Outer *)c->d_outer, parity, ce, c->d_pmax, c->d_psum);
+        hipLaunchKernelGGL((kb_outer_resid<Te, false>), dim3(tot), dim3(kThreads), 0, c->stream, L, O, parity, ce,
+                           c->d_pmax, c->d_psum);
     SG_CHECK_LAUNCH();
     hipLaunchKernelGGL(kb_norm_final, dim3(cnt), dim3(1024), 0, c->stream, d, (const int *)c->d_active,
                        (const double *)c->d_pmax, (const double *)c->d_psum, c->d_norm);

@@ -328,6 +328,75 @@ kb_outer_resid(BL<Te> L, const Outer *O, int parity, int ce, double *pmax, doubl
     if (threadIdx.x == 0) { pmax[blockIdx.x] = mx; psum[blockIdx.x] = sm; }
 }
 
+// The same outer step marching along z: workgroup b of an item owns a kOzX x kOzY patch of (i, j)
+// columns and zc planes of that problem (n % 32 == 0, n % zc == 0).  Each thread forms its cell
+// of the next plane (and, on the patch edge, one x and/or y halo cell: the periodic neighbours)
+// while the current plane sits in a double-buffered LDS plane with a one-cell halo, the z
+// neighbours in registers: every value of pin + e is loaded once per patch instead of once per
+// stencil use.  Same operands, in the same order, as kb_outer_resid, so the same values.
+constexpr int kOzX = 32, kOzY = 8;
+
+template <typename Te, bool UPD>
+__global__ void __launch_bounds__(kOzX * kOzY)
+kb_outer_resid_z(BL<Te> L, const Outer *O, int parity, int ce, int zc, double *pmax, double *psum)
+{
+    constexpr int TX = kOzX, TY = kOzY, SP = TX + 2;
+    __shared__ double sp[2][(TY + 2) * SP];
+    Cell<Te> c;
+    if (!locate(L, (int)blockIdx.x, c)) return;   // uniform per workgroup
+    const PL<Te> &q = *c.q;
+    const int p = c.it->p, n = q.n;
+    const double *pin = parity ? O->alt[p] : O->psi[p];
+    double *pout = parity ? O->psi[p] : O->alt[p];
+    const Te *e = q.a[q.lc == 0 ? CU : ce];
+    const double *f = O->src[p];
+    Te *r = q.a[CRHS];
+    const int nbx = n / TX, nby = n / TY;
+    const int bx = c.lb % nbx, by = (c.lb / nbx) % nby, bz = c.lb / (nbx * nby);
+    const int tx = (int)threadIdx.x % TX, ty = (int)threadIdx.x / TX;
+    const int i = bx * TX + tx, j = by * TY + ty, z0 = bz * zc;
+    auto at = [&](int ii, int jj, int kk) { return (kk * n + jj) * n + ii; };
+    auto wrapz = [&](int a) { return a < 0 ? a + n : (a >= n ? a - n : a); };
+    auto P = [&](int x) -> double { return UPD ? pin[x] + (double)e[x] : pin[x]; };
+    const bool hx = tx == 0 || tx == TX - 1, hy = ty == 0 || ty == TY - 1;
+    const int ih = tx == 0 ? wrapz(i - 1) : wrapz(i + 1), jh = ty == 0 ? wrapz(j - 1) : wrapz(j + 1);
+    const int sx = tx == 0 ? (ty + 1) * SP : (ty + 1) * SP + TX + 1;
+    const int sy = ty == 0 ? tx + 1 : (TY + 1) * SP + tx + 1;
+    const int sc = (ty + 1) * SP + tx + 1;
+    double zm = P(at(i, j, wrapz(z0 - 1))), zcen = P(at(i, j, z0));
+    double vx = hx ? P(at(ih, j, z0)) : 0.0, vy = hy ? P(at(i, jh, z0)) : 0.0;
+    double fc = f[at(i, j, z0)];
+    double mx = 0.0, sm = 0.0;
+    for (int k = z0; k < z0 + zc; ++k) {
+        double *s = sp[k & 1];
+        s[sc] = zcen;
+        if (hx) s[sx] = vx;
+        if (hy) s[sy] = vy;
+        const int kn = wrapz(k + 1);
+        const double zp = P(at(i, j, kn));
+        double vxn = 0.0, vyn = 0.0, fn = 0.0;
+        if (k + 1 < z0 + zc) {
+            if (hx) vxn = P(at(ih, j, kn));
+            if (hy) vyn = P(at(i, jh, kn));
+            fn = f[at(i, j, kn)];
+        }
+        const int cc = at(i, j, k);
+        __syncthreads();
+        const double rv = resid_value<double>(zcen, s[sc + 1], s[sc - 1], s[sc + SP], s[sc - SP], zp, zm, fc, O->h2i[p]);
+        r[cc] = (Te)rv;
+        if (UPD) pout[cc] = zcen;
+        mx = fmax(mx, fabs(rv));
+        sm += rv * rv;
+        zm = zcen;
+        zcen = zp;
+        vx = vxn;
+        vy = vyn;
+        fc = fn;
+    }
+    block_max_sum(mx, sm);
+    if (threadIdx.x == 0) { pmax[blockIdx.x] = mx; psum[blockIdx.x] = sm; }
+}
+
 // Per-problem second stage of a norm: workgroup w reduces item w's partials in order.
 __global__ void __launch_bounds__(1024) kb_norm_final(const Item *items, const int *active, const double *pmax,
                                                      const double *psum, double *out)

@@ -55,6 +55,21 @@ def test_fp64_batch_bitwise_vs_oracle(gpu):
     b.close()
 
 
+def test_fp64_batch_z_marching_outer_step_bitwise_vs_oracle(gpu):
+    """When every problem's n is a multiple of 32 the outer update + residual runs as z-marching
+    patches; the FP64 batch still equals the oracle bit for bit."""
+    sizes = [32, 96, 64]
+    srcs = rhs_list(sizes, seed=4)
+    b = BatchSolver(sizes, precisions=SG_PREC_FP64)
+    psis, st = b.solve_host(srcs, tol=1e-10, maxit=30, precision=SG_PREC_FP64)
+    for q, n in enumerate(sizes):
+        po, so = oracle.mg_cycle(srcs[q], 1.0 / n, tol=1e-10, maxit=30)
+        assert st[q]["status"] == 0 and st[q]["iterations"] == so["iterations"]
+        assert st[q]["rel_linf"] == so["rel_linf"]
+        assert np.array_equal(psis[q], po), f"n = {n}"
+    b.close()
+
+
 @pytest.mark.parametrize("coarse", [SG_COARSE_EXACT, SG_COARSE_VCYCLE])
 def test_mixed_batch_power_of_two_equals_single_grid_solver(gpu, coarse):
     sizes = [64, 16, 32, 128, 8]
