@@ -557,12 +557,13 @@ template <typename Te> int op_outer_resid(sg_batch *c, int parity, bool upd, int
     if (!cnt) return SG_OK;
     const BL<Te> L = bl<Te>(c, d, cnt);
     const Outer *O = (const Outer *)c->d_outer;
+    const dim3 zgrid(8 * ((tot + 7) / 8)), zblock(kOzX * kOzY);
     if (zform && upd)
-        hipLaunchKernelGGL((kb_outer_resid_z<Te, true>), dim3(tot), dim3(kOzX * kOzY), 0, c->stream, L, O, parity, ce,
-                           zc, c->d_pmax, c->d_psum);
+        hipLaunchKernelGGL((kb_outer_resid_z<Te, true>), zgrid, zblock, 0, c->stream, L, tot, O, parity, ce, zc,
+                           c->d_pmax, c->d_psum);
     else if (zform)
-        hipLaunchKernelGGL((kb_outer_resid_z<Te, false>), dim3(tot), dim3(kOzX * kOzY), 0, c->stream, L, O, parity, ce,
-                           zc, c->d_pmax, c->d_psum);
+        hipLaunchKernelGGL((kb_outer_resid_z<Te, false>), zgrid, zblock, 0, c->stream, L, tot, O, parity, ce, zc,
+                           c->d_pmax, c->d_psum);
     else if (upd)
         hipLaunchKernelGGL((kb_outer_resid<Te, true>), dim3(tot), dim3(kThreads), 0, c->stream, L, O, parity, ce,
                            c->d_pmax, c->d_psum);

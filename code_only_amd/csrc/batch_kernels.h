@@ -338,12 +338,15 @@ constexpr int kOzX = 32, kOzY = 8;
 
 template <typename Te, bool UPD>
 __global__ void __launch_bounds__(kOzX * kOzY)
-kb_outer_resid_z(BL<Te> L, const Outer *O, int parity, int ce, int zc, double *pmax, double *psum)
+kb_outer_resid_z(BL<Te> L, int total, const Outer *O, int parity, int ce, int zc, double *pmax, double *psum)
 {
     constexpr int TX = kOzX, TY = kOzY, SP = TX + 2;
     __shared__ double sp[2][(TY + 2) * SP];
     Cell<Te> c;
-    if (!locate(L, (int)blockIdx.x, c)) return;   // uniform per workgroup
+    // XCD-aware numbering: neighbouring patches (consecutive in x) share an XCD, so the cache
+    // lines an x-halo cell pulls in are the ones its neighbour loads anyway
+    const int wb = xcd_block(total);
+    if (!locate(L, wb, c)) return;   // uniform per workgroup
     const PL<Te> &q = *c.q;
     const int p = c.it->p, n = q.n;
     const double *pin = parity ? O->alt[p] : O->psi[p];
@@ -394,7 +397,7 @@ kb_outer_resid_z(BL<Te> L, const Outer *O, int parity, int ce, int zc, double *p
         fc = fn;
     }
     block_max_sum(mx, sm);
-    if (threadIdx.x == 0) { pmax[blockIdx.x] = mx; psum[blockIdx.x] = sm; }
+    if (threadIdx.x == 0) { pmax[wb] = mx; psum[wb] = sm; }
 }
 
 // Per-problem second stage of a norm: workgroup w reduces item w's partials in order.
