@@ -171,8 +171,15 @@ __global__ void __launch_bounds__(TX * TY) k_update_resid(const double *__restri
     __shared__ float sr[4][TY * TX];
     const int n = 1 << lg, m = n - 1;
     const int tx = (int)threadIdx.x % TX, ty = (int)threadIdx.x / TX;
-    const int i = blockIdx.x * TX + tx, j = blockIdx.y * TY + ty;
-    const int z0 = blockIdx.z * zc;
+    // XCD-aware patch numbering: the hardware deals consecutive workgroups round-robin to the 8
+    // XCDs; renumbered, each XCD takes a contiguous run of patches (neighbours in y), so the halo
+    // rows a patch loads are mostly lines its XCD's L2 already holds
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y, nbk = gx * gy * (int)gridDim.z;
+    const int hb = ((int)blockIdx.z * gy + (int)blockIdx.y) * gx + (int)blockIdx.x;
+    const int vb = nbk % 8 == 0 ? (hb % 8) * (nbk / 8) + hb / 8 : hb;
+    const int pbx = vb % gx, pby = (vb / gx) % gy, pbz = vb / (gx * gy);
+    const int i = pbx * TX + tx, j = pby * TY + ty;
+    const int z0 = pbz * zc;
     auto at = [&](int ii, int jj, int kk) { return (kk << (2 * lg)) | (jj << lg) | ii; };
     // pin, e, f and pout stream through once per outer iteration: non-temporal, so that the
     // last-level cache keeps r32, which the next inner solve reads first
@@ -197,7 +204,7 @@ __global__ void __launch_bounds__(TX * TY) k_update_resid(const double *__restri
         if (t < (TX / 2) * (TY / 2)) {
             const int cx = t % (TX / 2), cy = t / (TX / 2);
             const float *a = sr[k0 & 3] + 2 * cy * TX + 2 * cx, *b = sr[(k0 + 1) & 3] + 2 * cy * TX + 2 * cx;
-            const int ic = blockIdx.x * (TX / 2) + cx, jc = blockIdx.y * (TY / 2) + cy, kc = (k0 & m) >> 1;
+            const int ic = pbx * (TX / 2) + cx, jc = pby * (TY / 2) + cy, kc = (k0 & m) >> 1;
             rc[(kc << (2 * lgc)) | (jc << lgc) | ic] =
                 restrict_value<float>(a[0], a[1], a[TX], b[0], a[TX + 1], b[1], b[TX], b[TX + 1]);
         }
@@ -241,7 +248,7 @@ __global__ void __launch_bounds__(TX * TY) k_update_resid(const double *__restri
         __syncthreads();
         restrict_pair(z0 + zc - 2);
     }
-    const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const int b = vb;
     block_max_sum(mx, sm);
     if (threadIdx.x == 0) {
         pmax[b] = mx;
