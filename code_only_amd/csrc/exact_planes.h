@@ -364,16 +364,24 @@ __device__ __forceinline__ void exact_z_split(float *__restrict__ w, const float
     });
 }
 
+// XCD-aware numbering: the hardware deals consecutive workgroups round-robin to the 8 XCDs; the
+// G workgroups of one plane (which all stage that plane and Q) get consecutive numbers on one XCD.
+__device__ __forceinline__ int ep_xcd_block()
+{
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    return nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+}
+
 static __global__ void __launch_bounds__(kSplitThreads) k_exact_xy_split(const float *in, float *out, const float *Q,
                                                                          int n)
 {
-    exact_xy_split(in, out, Q, n, (int)blockIdx.x);
+    exact_xy_split(in, out, Q, n, ep_xcd_block());
 }
 
 static __global__ void __launch_bounds__(kSplitThreads) k_exact_z_split(float *w, const float *Q, const float *inv,
                                                                         int n)
 {
-    exact_z_split(w, Q, inv, n, (int)blockIdx.x);
+    exact_z_split(w, Q, inv, n, ep_xcd_block());
 }
 
 // 1 / ((lam_i + lam_j) + lam_k) for the dense n^3 grid (0 where the sum is 0), host, double.
