@@ -324,11 +324,18 @@ template <typename T> int interp_blocks(sg_amr &s, int Lf, int iuf, int iuc, int
     int na = (int)s.active.size();
     const int *al = s.d_active;
     auto &R = ar<T>(s);
-    if (Lf == 0) {
-        if (cubic) return go(s, k_interp<8, true, T>, (long)na * 512, R.A[0][iuf], (const T *)R.A[1][iuc], al, na);
-        return go(s, k_interp<8, false, T>, (long)na * 512, R.A[0][iuf], (const T *)R.A[1][iuc], al, na);
+    if (cubic && na > 0) {
+        // tricubic: one workgroup per block, the separable passes staged in LDS
+        if (Lf == 0)
+            hipLaunchKernelGGL((k_interp_cubic_blk<8, T>), dim3(na), dim3(256), 0, s.stream, R.A[0][iuf],
+                               (const T *)R.A[1][iuc], al, na);
+        else
+            hipLaunchKernelGGL((k_interp_cubic_blk<4, T>), dim3(na), dim3(256), 0, s.stream, R.A[1][iuf],
+                               (const T *)R.A[2][iuc], al, na);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
     }
-    if (cubic) return go(s, k_interp<4, true, T>, (long)na * 64, R.A[1][iuf], (const T *)R.A[2][iuc], al, na);
+    if (Lf == 0) return go(s, k_interp<8, false, T>, (long)na * 512, R.A[0][iuf], (const T *)R.A[1][iuc], al, na);
     return go(s, k_interp<4, false, T>, (long)na * 64, R.A[1][iuf], (const T *)R.A[2][iuc], al, na);
 }
 

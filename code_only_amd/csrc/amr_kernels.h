@@ -535,6 +535,56 @@ template <int NF, bool CUBIC, typename T> __global__ void k_interp(T *Af, const 
     Af[(long)b * Blk<NF>::LEN + Blk<NF>::gi(i, j, k)] = v;
 }
 
+// The tricubic case of k_interp with one 256-thread workgroup per listed block: the coarse block
+// with its ghosts is staged in LDS and the three separable passes of interpolation.F90:40-218
+// (bufx, bufxy, then z) run over it, so each intermediate is formed once instead of once per fine
+// cell that uses it.  Every intermediate is the value tricubic_at forms, so the results are the
+// same bit for bit.
+template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cubic_blk(T *Af, const T *Ac, const int *list,
+                                                                                     int nb)
+{
+    constexpr int NC = NF / 2, W = NC + 2;   // coarse cells -1 .. NC per axis
+    static_assert(NC + 2 >= 4, "the tricubic stencil needs four coarse cells");
+    __shared__ T sc[W * W * W];
+    __shared__ T bx[W * W * NF];    // [kc][jc][i]
+    __shared__ T bxy[W * NF * NF];  // [kc][j][i]
+    __shared__ int sidx[NF][4];     // stencil index + 1 (ghosted offset)
+    __shared__ double swt[NF][4];
+    if ((int)blockIdx.x >= nb) return;
+    const int b = list[blockIdx.x];
+    const T *uc = Ac + (long)b * Blk<NC>::LEN;
+    for (int q = (int)threadIdx.x; q < W * W * W; q += (int)blockDim.x) sc[q] = uc[q];
+    if ((int)threadIdx.x < NF) {
+        int idx[4];
+        double a[4];
+        cubic_stencil((int)threadIdx.x, -1, NC, idx, a);
+        for (int e = 0; e < 4; ++e) {
+            sidx[threadIdx.x][e] = idx[e] + 1;
+            swt[threadIdx.x][e] = a[e];
+        }
+    }
+    __syncthreads();
+    for (int q = (int)threadIdx.x; q < W * W * NF; q += (int)blockDim.x) {
+        const int i = q % NF, r = q / NF;  // row r = kc * W + jc (ghosted)
+        const T *row = sc + r * W;
+        bx[q] = comb4<T>(swt[i], row[sidx[i][0]], row[sidx[i][1]], row[sidx[i][2]], row[sidx[i][3]]);
+    }
+    __syncthreads();
+    for (int q = (int)threadIdx.x; q < W * NF * NF; q += (int)blockDim.x) {
+        const int i = q % NF, j = (q / NF) % NF, kc = q / (NF * NF);
+        const T *col = bx + kc * W * NF + i;
+        bxy[q] = comb4<T>(swt[j], col[sidx[j][0] * NF], col[sidx[j][1] * NF], col[sidx[j][2] * NF], col[sidx[j][3] * NF]);
+    }
+    __syncthreads();
+    T *uf = Af + (long)b * Blk<NF>::LEN;
+    for (int q = (int)threadIdx.x; q < NF * NF * NF; q += (int)blockDim.x) {
+        const int i = q % NF, j = (q / NF) % NF, k = q / (NF * NF);
+        const T *p = bxy + j * NF + i;
+        uf[Blk<NF>::gi(i, j, k)] = comb4<T>(swt[k], p[sidx[k][0] * NF * NF], p[sidx[k][1] * NF * NF],
+                                            p[sidx[k][2] * NF * NF], p[sidx[k][3] * NF * NF]);
+    }
+}
+
 // ---- whole-array block operations (ghosts included) ----
 template <typename T> __global__ void k_blocks_fill0(T *A, const int *list, int b0, int nb, int len)
 {
