@@ -44,11 +44,11 @@ class SgParams(ctypes.Structure):
 class SgStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("rel_linf", ctypes.c_double),
                 ("rel_l2", ctypes.c_double), ("resmaxg", ctypes.c_double), ("ms", ctypes.c_double),
-                ("hist", ctypes.c_double * SG_HIST)]
+                ("hist", ctypes.c_double * SG_HIST), ("rel_h2", ctypes.c_double)]
 
     def as_dict(self):
         return dict(iterations=self.iterations, status=self.status, rel_linf=self.rel_linf,
-                    rel_l2=self.rel_l2, resmaxg=self.resmaxg, ms=self.ms,
+                    rel_l2=self.rel_l2, rel_h2=self.rel_h2, resmaxg=self.resmaxg, ms=self.ms,
                     hist=[self.hist[i] for i in range(min(self.iterations + 1, SG_HIST))])
 
 

@@ -10,8 +10,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (SG_BC_PERIODIC, SG_BC_SPHERE, SG_PREC_FP64, SG_PREC_MIXED, SG_STOP_REFERENCE,
-                   SG_STOP_TRUE_RESIDUAL)
+from ._lib import (SG_BC_PERIODIC, SG_BC_SPHERE, SG_ERR_NONFINITE, SG_ERR_NOT_CONVERGED, SG_PREC_FP64,
+                   SG_PREC_MIXED, SG_STOP_REFERENCE, SG_STOP_TRUE_RESIDUAL)
 
 __all__ = ["AmrSolver", "SG_BC_PERIODIC", "SG_BC_SPHERE", "SG_PREC_FP64", "SG_PREC_MIXED", "SG_STOP_REFERENCE",
            "SG_STOP_TRUE_RESIDUAL"]
@@ -44,6 +44,10 @@ class AmrSolver:
         if zero_average is not None:
             _lib.check(self._lib.sg_amr_set_zero_average(self._ctx, int(bool(zero_average))))
 
+    def set_zero_average(self, on):
+        """ZERO_AVERAGE_DENSITY on or off for later solves (fmg_data_if.F90:53-78, vmg.F90:213-216)."""
+        _lib.check(self._lib.sg_amr_set_zero_average(self._ctx, int(bool(on))))
+
     def close(self):
         if getattr(self, "_ctx", None):
             self._lib.sg_amr_destroy(self._ctx)
@@ -61,8 +65,10 @@ class AmrSolver:
                                    use_graph=int(bool(use_graph)))
 
     def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64,
-              use_graph=True):
-        """fmg_poisson on torch float64 GPU tensors [nblk, 8, 8, 8]; psi in place, g [nblk, 3, 8, 8, 8]."""
+              use_graph=True, raise_on_fail=True):
+        """fmg_poisson on torch float64 GPU tensors [nblk, 8, 8, 8]; psi in place, g [nblk, 3, 8, 8, 8].
+        With raise_on_fail False, SG_ERR_NOT_CONVERGED / SG_ERR_NONFINITE come back in the stats'
+        status instead of an exception."""
         import torch
         for t in (psi, src) + ((g,) if g is not None else ()):
             if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
@@ -75,18 +81,20 @@ class AmrSolver:
         _lib.check(self._lib.sg_amr_solve_dev(self._ctx, ctypes.c_void_p(psi.data_ptr()),
                                               ctypes.c_void_p(src.data_ptr()),
                                               ctypes.c_void_p(g.data_ptr()) if g is not None else None,
-                                              ctypes.byref(p), ctypes.byref(st)))
+                                              ctypes.byref(p), ctypes.byref(st)),
+                   allow=() if raise_on_fail else (SG_ERR_NOT_CONVERGED, SG_ERR_NONFINITE))
         return st.as_dict()
 
     def solve_host(self, src, psi=None, want_g=True, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL,
-                   precision=SG_PREC_FP64, use_graph=True):
+                   precision=SG_PREC_FP64, use_graph=True, raise_on_fail=True):
         src = np.ascontiguousarray(src, np.float64)
         psi = np.zeros_like(src) if psi is None else np.ascontiguousarray(psi, np.float64).copy()
         g = np.zeros((self.h.nblk, 3, 8, 8, 8)) if want_g else None
         p = self._params(tol, maxit, stop_mode, precision, use_graph)
         st = _lib.SgStats()
         _lib.check(self._lib.sg_amr_solve_host(self._ctx, _dp(psi), _dp(src), _dp(g) if want_g else None,
-                                               ctypes.byref(p), ctypes.byref(st)))
+                                               ctypes.byref(p), ctypes.byref(st)),
+                   allow=() if raise_on_fail else (SG_ERR_NOT_CONVERGED, SG_ERR_NONFINITE))
         return psi, g, st.as_dict()
 
     # ---- operator-level access (parity tests) ----

@@ -26,10 +26,28 @@ HIP_FLAGS = [
 ]
 
 
+HIP_BUILD = os.path.join(CSRC, "_build")
+
+
 def build_hip(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    """Each source compiled to an object in its own hipcc process (in parallel), then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(HIP_BUILD, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(HIP_BUILD, src.replace(".hip", ".o"))
+        cmd = [HIPCC] + compile_flags + ["-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=len(HIP_SOURCES)) as pool:
+        objs = list(pool.map(compile_one, HIP_SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", CSRC] + srcs + ["-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

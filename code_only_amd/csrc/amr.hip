@@ -71,6 +71,7 @@ struct sg_amr {
     size_t bd_lds_bytes = 0;
     int bd_ok = 0;
     hipGraphExec_t graph = nullptr;   // mixed solve: one outer iteration, captured once
+    int graph_zero_average = -1;      // the zero_average setting the graph was captured with
     double hl[kNfmg][kMaxLev] = {}, hil[kNfmg][kMaxLev] = {};
     double *d_h[kNfmg] = {}, *d_hi[kNfmg] = {};
     double *d_scal = nullptr;   // [0] Resh2maxg [1] scratch max [2..4] norms [5] mean [8..23] multipole
@@ -662,22 +663,27 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     SG_TRY(norms(s, C_SRC));
     SG_TRY(read_scalars(s));
     const double resh2max = s.h_scal[2], srcmax = s.h_scal[3], srcsq = s.h_scal[4];
-    int nmax = p.maxit;
+    // The max norms drop NaN (fmax); the sum of squares keeps it, so test both.
+    const bool src_finite = std::isfinite(resh2max) && std::isfinite(srcmax) && std::isfinite(srcsq);
+    int nmax = src_finite ? p.maxit : 0;
     if (std::fabs(resh2max) <= 2.2250738585072014e-308) nmax = 0;
     double prev = HUGE_VAL;
-    int it = 0, status = 1;
+    int it = 0, status = SG_ERR_NOT_CONVERGED;
     auto measure = [&](int slot) -> int {
         SG_TRY(norms(s, C_RHO));
         SG_TRY(read_scalars(s));
         st.rel_linf = srcmax > 0 ? s.h_scal[3] / srcmax : 0.0;
         st.rel_l2 = srcsq > 0 ? std::sqrt(s.h_scal[4] / srcsq) : 0.0;
+        st.rel_h2 = resh2max > 0 ? s.h_scal[2] / resh2max : 0.0;
+        if (!std::isfinite(s.h_scal[3]) || !std::isfinite(s.h_scal[4])) st.rel_linf = st.rel_l2 = NAN;
         if (slot >= 0 && slot < 64) st.hist[slot] = st.rel_linf;
         return SG_OK;
     };
     for (;;) {
         SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
         SG_TRY(measure(it));
-        if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = 0; break; }
+        if (!src_finite || !std::isfinite(st.rel_l2)) { status = SG_ERR_NONFINITE; break; }
+        if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = SG_OK; break; }
         if (it >= nmax) break;
         SG_TRY(fmg_lin<double>(s));
         SG_TRY(fmg_add<double>(s, 0, C_PSI, C_U));
@@ -686,15 +692,16 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
         double ratio = s.h_scal[0] / resh2max;
         st.resmaxg = ratio;
         if (p.stop_mode == SG_STOP_REFERENCE) {
-            if (ratio < p.tol) { status = 0; break; }
-            if (ratio > prev * 0.5) break;
+            if (ratio < p.tol) { status = SG_OK; break; }
+            if (ratio > prev * 0.5) break;   // stall (fmg.F90:193)
             prev = ratio;
         }
     }
-    if (nmax == 0) status = 0;
-    if (p.stop_mode == SG_STOP_REFERENCE) {
+    if (nmax == 0 && src_finite) status = SG_OK;
+    if (p.stop_mode == SG_STOP_REFERENCE && status != SG_ERR_NONFINITE) {
         SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
         SG_TRY(measure(-1));
+        if (!std::isfinite(st.rel_l2)) status = SG_ERR_NONFINITE;
     }
     st.iterations = it;
     st.status = status;
@@ -711,7 +718,7 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     float ms = 0.f;
     SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
     st.ms = ms;
-    return SG_OK;
+    return status;
 }
 
 // ---- mixed precision: FP64 defect correction around the FP32 inner fmg_lin ----
@@ -794,30 +801,37 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
     SG_TRY(norms(s, C_SRC));
     SG_TRY(read_scalars(s));
     const double resh2max = s.h_scal[2], srcmax = s.h_scal[3], srcsq = s.h_scal[4];
-    int nmax = p.maxit;
+    // The max norms drop NaN (fmax); the sum of squares keeps it, so test both.
+    const bool src_finite = std::isfinite(resh2max) && std::isfinite(srcmax) && std::isfinite(srcsq);
+    int nmax = src_finite ? p.maxit : 0;
     if (std::fabs(resh2max) <= 2.2250738585072014e-308) nmax = 0;
     SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
     SG_TRY(norms(s, C_RHO));
     SG_TRY(read_scalars(s));
     double prev = HUGE_VAL;
-    int it = 0, status = 1;
+    int it = 0, status = SG_ERR_NOT_CONVERGED;
     for (;;) {
         st.rel_linf = srcmax > 0 ? s.h_scal[3] / srcmax : 0.0;
         st.rel_l2 = srcsq > 0 ? std::sqrt(s.h_scal[4] / srcsq) : 0.0;
+        st.rel_h2 = resh2max > 0 ? s.h_scal[2] / resh2max : 0.0;
+        if (!std::isfinite(s.h_scal[3]) || !std::isfinite(s.h_scal[4])) st.rel_linf = st.rel_l2 = NAN;
         if (it < 64) st.hist[it] = st.rel_linf;
-        if (!std::isfinite(st.rel_linf)) { status = SG_ERR_NONFINITE; break; }
-        if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = 0; break; }
+        if (!src_finite || !std::isfinite(st.rel_l2)) { status = SG_ERR_NONFINITE; break; }
+        if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = SG_OK; break; }
         if (it >= nmax) break;
         if (p.use_graph) {
-            if (!s.graph) {
+            if (!s.graph || s.graph_zero_average != s.zero_average) {
+                if (s.graph) (void)hipGraphExecDestroy(s.graph);
+                s.graph = nullptr;
                 hipGraph_t gr = nullptr;
                 SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
                 int rc = mixed_iteration(s);
                 hipError_t e = hipStreamEndCapture(s.stream, &gr);
+                if (rc == SG_OK && e == hipSuccess) e = hipGraphInstantiate(&s.graph, gr, nullptr, nullptr, 0);
+                if (gr) (void)hipGraphDestroy(gr);
                 if (rc != SG_OK) return rc;
                 SG_HIP(e);
-                SG_HIP(hipGraphInstantiate(&s.graph, gr, nullptr, nullptr, 0));
-                SG_HIP(hipGraphDestroy(gr));
+                s.graph_zero_average = s.zero_average;
             }
             SG_HIP(hipGraphLaunch(s.graph, s.stream));
         } else {
@@ -830,12 +844,14 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
         if (p.stop_mode == SG_STOP_REFERENCE) {
             st.rel_linf = srcmax > 0 ? s.h_scal[3] / srcmax : 0.0;
             st.rel_l2 = srcsq > 0 ? std::sqrt(s.h_scal[4] / srcsq) : 0.0;
-            if (ratio < p.tol) { status = 0; break; }
-            if (ratio > prev * 0.5) break;
+            st.rel_h2 = resh2max > 0 ? s.h_scal[2] / resh2max : 0.0;
+            if (!std::isfinite(s.h_scal[3]) || !std::isfinite(s.h_scal[4])) { status = SG_ERR_NONFINITE; break; }
+            if (ratio < p.tol) { status = SG_OK; break; }
+            if (ratio > prev * 0.5) break;   // stall (fmg.F90:193)
             prev = ratio;
         }
     }
-    if (nmax == 0) status = 0;
+    if (nmax == 0 && src_finite) status = SG_OK;
     st.iterations = it;
     st.status = status;
     SG_TRY(fmg_copy<double>(s, 0, C_U, C_PSI, 1));
@@ -851,7 +867,7 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
     float ms = 0.f;
     SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
     st.ms = ms;
-    return status == SG_ERR_NONFINITE ? SG_ERR_NONFINITE : SG_OK;
+    return status;
 }
 
 template <typename T> int upload(sg_amr &s, T **dptr, const std::vector<T> &v)
@@ -899,6 +915,8 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     SG_REQUIRE(nblk > 0 && nlev > 0 && nlev <= kMaxLev, "sg_amr_create: bad sizes nblk=%d nlev=%d", nblk, nlev);
     SG_REQUIRE(bc == SG_BC_PERIODIC || bc == SG_BC_SPHERE, "sg_amr_create: unknown boundary %d", bc);
     SG_REQUIRE(h0 > 0, "sg_amr_create: h0 must be positive");
+    for (int a = 0; a < 3; ++a)
+        SG_REQUIRE(base_dims[a] > 0 && base_dims[a] <= 4096, "sg_amr_create: bad base_dims[%d] = %d", a, base_dims[a]);
     s->nblk = nblk;
     s->nlev = nlev;
     s->bc = bc;
@@ -920,6 +938,10 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
         SG_REQUIRE(b == 0 || level[b] >= level[b - 1], "sg_amr_create: blocks must be ordered by level");
         SG_REQUIRE((level[b] == 0) == (parent[b] < 0), "sg_amr_create: block %d parent/level mismatch", b);
         s->lev_start[level[b] + 1] = b + 1;
+        for (int a = 0; a < 3; ++a)
+            SG_REQUIRE(ijk[3 * b + a] >= 0 && ijk[3 * b + a] < (base_dims[a] << level[b]),
+                       "sg_amr_create: block %d position %d out of [0, %d) on axis %d", b, ijk[3 * b + a],
+                       base_dims[a] << level[b], a);
         s->covered[b] = child[8 * b] >= 0;
         if (parent[b] >= 0) {
             SG_REQUIRE(parent[b] < nblk && level[parent[b]] == level[b] - 1, "sg_amr_create: bad parent of %d", b);
@@ -1121,6 +1143,11 @@ int op(sg_amr &s, int opc, int L, int a, int b, int c, int flag, double *ret)
         SG_TRY(sphere_bc(s, a, b));
         break;
     case SG_AMR_OP_REMOVE_MEAN: SG_TRY(remove_leaf_mean(s, a)); break;
+    case SG_AMR_OP_VMG_TAU:
+        SG_REQUIRE(L == LV, "sg_amr_op: vmg_tau runs at the coarsest FMG level %d", LV);
+        SG_REQUIRE(flag >= 0 && flag < s.nlev, "sg_amr_op: vmg_tau AMR level %d out of range", flag);
+        SG_TRY(vmg_tau<double>(s, flag, a, b));
+        break;
     default: SG_REQUIRE(false, "sg_amr_op: unknown operation %d", opc);
     }
     SG_HIP(hipStreamSynchronize(s.stream));
@@ -1157,6 +1184,11 @@ int sg_amr_set_zero_average(sg_amr *ctx, int on)
 {
     SG_REQUIRE(ctx, "sg_amr_set_zero_average: null context");
     ctx->zero_average = on ? 1 : 0;
+    // the captured mixed iteration bakes the setting in (vmg_fas's zero average): recapture
+    if (ctx->graph && ctx->graph_zero_average != ctx->zero_average) {
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
     return SG_OK;
 }
 
@@ -1165,6 +1197,10 @@ int sg_amr_solve_dev(sg_amr *ctx, double *psi, const double *src, double *g, con
     SG_REQUIRE(ctx && psi && src && p && st, "sg_amr_solve_dev: null argument");
     SG_REQUIRE(p->precision == SG_PREC_FP64 || p->precision == SG_PREC_MIXED, "sg_amr_solve_dev: unknown precision %d",
                p->precision);
+    SG_REQUIRE(p->maxit >= 0 && p->maxit <= 1000, "sg_amr_solve_dev: bad maxit %d", p->maxit);
+    SG_REQUIRE(p->stop_mode == SG_STOP_REFERENCE || p->stop_mode == SG_STOP_TRUE_RESIDUAL,
+               "sg_amr_solve_dev: bad stop_mode %d", p->stop_mode);
+    SG_REQUIRE(p->tol >= 0.0, "sg_amr_solve_dev: bad tol");
     if (p->precision == SG_PREC_MIXED) return solve_mixed(*ctx, psi, src, g, *p, *st);
     return solve(*ctx, psi, src, g, *p, *st);
 }
@@ -1178,11 +1214,12 @@ int sg_amr_solve_host(sg_amr *ctx, double *psi, const double *src, double *g, co
     double *dpsi = s.d_dense, *dsrc = dpsi + n, *dg = dsrc + n;
     SG_HIP(hipMemcpyAsync(dpsi, psi, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
     SG_HIP(hipMemcpyAsync(dsrc, src, sizeof(double) * n, hipMemcpyHostToDevice, s.stream));
-    SG_TRY(sg_amr_solve_dev(ctx, dpsi, dsrc, g ? dg : nullptr, p, st));
+    const int rc = sg_amr_solve_dev(ctx, dpsi, dsrc, g ? dg : nullptr, p, st);
+    if (rc != SG_OK && rc != SG_ERR_NOT_CONVERGED && rc != SG_ERR_NONFINITE) return rc;
     SG_HIP(hipMemcpyAsync(psi, dpsi, sizeof(double) * n, hipMemcpyDeviceToHost, s.stream));
     if (g) SG_HIP(hipMemcpyAsync(g, dg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s.stream));
     SG_HIP(hipStreamSynchronize(s.stream));
-    return SG_OK;
+    return rc;
 }
 
 int sg_amr_set_array(sg_amr *ctx, int fmglev, int code, const double *host)

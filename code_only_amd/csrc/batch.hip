@@ -806,6 +806,7 @@ int solve(sg_batch *c, double *const *psi, const double *const *src, const sg_pa
             s.iterations = it;
             s.rel_linf = fmax > 0 ? rmax / fmax : 0.0;
             s.rel_l2 = fl2 > 0 ? rl2 / fl2 : 0.0;
+            s.rel_h2 = s.rel_linf;   // one cell width per problem: the h^2 weights cancel
             if (it < SG_HIST) s.hist[it] = s.rel_linf;
             if (!std::isfinite(rmax) || !std::isfinite(rl2)) { s.status = SG_ERR_NONFINITE; done[q] = 1; }
             else if (s.rel_linf < p->tol) { s.status = SG_OK; done[q] = 1; }

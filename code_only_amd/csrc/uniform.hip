@@ -765,6 +765,7 @@ static int uniform_cycle(sg_uniform *c, double *psi, const double *src, const sg
         const double rmax = c->hnorm[0], rl2 = std::sqrt(c->hnorm[1]);
         st->rel_linf = fmax > 0 ? rmax / fmax : 0.0;
         st->rel_l2 = fl2 > 0 ? rl2 / fl2 : 0.0;
+        st->rel_h2 = st->rel_linf;   // one cell width: the h^2 weights cancel
         if (it < SG_HIST) st->hist[it] = st->rel_linf;
         if (!std::isfinite(rmax) || !std::isfinite(rl2)) { status = SG_ERR_NONFINITE; break; }
         if (p->stop_mode == SG_STOP_TRUE_RESIDUAL && st->rel_linf < p->tol) { status = SG_OK; break; }

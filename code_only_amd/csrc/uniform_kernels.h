@@ -409,9 +409,11 @@ static __global__ void __launch_bounds__(256) k_gradient(double *__restrict__ g,
     if (c >= n3) return;
     const int i = c & (n - 1), j = (c >> lg) & (n - 1), k = c >> (2 * lg);
     const int row = (k * n + j) * n;
+    // component offsets in 64 bits: 2 n^3 exceeds INT_MAX at n = 1024
+    const size_t o1 = (size_t)n3, o2 = 2 * (size_t)n3;
     g[c] = -(u[row + wrap_p(i, n)] - u[row + wrap_m(i, n)]) * hi * 0.5;
-    g[n3 + c] = -(u[(k * n + wrap_p(j, n)) * n + i] - u[(k * n + wrap_m(j, n)) * n + i]) * hi * 0.5;
-    g[2 * n3 + c] = -(u[(wrap_p(k, n) * n + j) * n + i] - u[(wrap_m(k, n) * n + j) * n + i]) * hi * 0.5;
+    g[o1 + c] = -(u[(k * n + wrap_p(j, n)) * n + i] - u[(k * n + wrap_m(j, n)) * n + i]) * hi * 0.5;
+    g[o2 + c] = -(u[(wrap_p(k, n) * n + j) * n + i] - u[(wrap_m(k, n) * n + j) * n + i]) * hi * 0.5;
 }
 
 // ---- adjoint helpers (periodic grid, n = 2^lg) ----

@@ -34,6 +34,7 @@ module sg_capi
      real(c_double) :: resmaxg
      real(c_double) :: ms
      real(c_double) :: hist(SG_HIST)
+     real(c_double) :: rel_h2
   end type sg_stats
 
   interface

@@ -28,7 +28,7 @@ class OrcParams(ctypes.Structure):
 class OrcAmrStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int),
                 ("rel_linf", ctypes.c_double), ("rel_l2", ctypes.c_double),
-                ("ratio", ctypes.c_double), ("hist", ctypes.c_double * HIST)]
+                ("ratio", ctypes.c_double), ("hist", ctypes.c_double * HIST), ("rel_h2", ctypes.c_double)]
 
 
 class OrcStats(ctypes.Structure):
@@ -184,7 +184,7 @@ def gradient(u, h):
 # ---- AMR (fmg.F90 / vmg.F90) ---------------------------------------------------------------
 U, RES, RHS, RHO, SRC, PSI = range(6)
 OP_C2P, OP_GHOSTFIX, OP_FILL0, OP_EXTRAP, OP_RELAX, OP_RESID, OP_RSTRCT, OP_INTERP, OP_ADDINT, \
-    OP_VMG_FAS, OP_FMG_LIN, OP_SPHERE_BC, OP_REMOVE_MEAN = range(1, 14)
+    OP_VMG_FAS, OP_FMG_LIN, OP_SPHERE_BC, OP_REMOVE_MEAN, OP_VMG_TAU = range(1, 15)
 
 
 def _ip(a):
@@ -207,6 +207,9 @@ class Amr:
             raise ValueError("orc_amr_create rejected the hierarchy")
         if zero_average is not None:
             lib().orc_amr_set_zero_average(self.ptr, int(zero_average))
+
+    def set_zero_average(self, on):
+        lib().orc_amr_set_zero_average(self.ptr, int(bool(on)))
 
     def close(self):
         if getattr(self, "ptr", None):
@@ -254,6 +257,6 @@ class Amr:
         lib().orc_amr_solve(self.ptr, _p(psi), _p(src), _p(g) if want_g else None, ctypes.byref(p),
                             ctypes.byref(st))
         stats = dict(iterations=st.iterations, status=st.status, rel_linf=st.rel_linf,
-                     rel_l2=st.rel_l2, ratio=st.ratio,
+                     rel_l2=st.rel_l2, rel_h2=st.rel_h2, ratio=st.ratio,
                      hist=[st.hist[i] for i in range(min(st.iterations + 1, HIST))])
         return psi, g, stats

@@ -1114,13 +1114,18 @@ double orc_amr_op(orc_amr *s, int op, int L, int a, int b, int c, int flag)
     case ORC_AMR_OP_FMG_LIN: fmg_lin(s); return s->resh2maxg;
     case ORC_AMR_OP_SPHERE_BC: orc_amr_sphere_bc(s, a, b); return 0.0;
     case ORC_AMR_OP_REMOVE_MEAN: remove_leaf_mean(s, a); return 0.0;
+    case ORC_AMR_OP_VMG_TAU:   /* vmg_tau(AMR level flag, jtau a, ju b) at FMG level L (vmg.F90:440-458) */
+        if (flag < 0 || flag >= s->nlev) return -1.0;
+        vmg_tau(s, L, flag, a, b);
+        return 0.0;
     default: return -1.0;
     }
 }
 
 /* fmg_cycle (fmg.F90:134-233) for the Poisson equation.  psi: [nblk][n^3] initial guess in,
  * potential out; src: [nblk][n^3] source 4 pi G rho (mean removed here when zero_average is on,
- * as fmg_make_physvar does); g: [nblk][3][n^3] out (may be NULL).  Returns 0 when converged. */
+ * as fmg_make_physvar does); g: [nblk][3][n^3] out (may be NULL).  Returns ORC_OK when converged,
+ * ORC_NOT_CONVERGED when the loop ended on maxit or on the stall test. */
 int orc_amr_solve(orc_amr *s, double *psi, const double *src, double *g, const orc_params *p, orc_amr_stats *st)
 {
     const int n = s->nb;
@@ -1153,15 +1158,16 @@ int orc_amr_solve(orc_amr *s, double *psi, const double *src, double *g, const o
     int nmax = p->maxit;
     if (fabs(resh2max) <= 2.2250738585072014e-308) nmax = 0;
     double prev = HUGE_VAL;
-    int it = 0, status = 1;
+    int it = 0, status = ORC_NOT_CONVERGED;
     for (;;) {
         poisson_resid(s, 0, C_RHO, C_PSI, C_SRC, 0);
         double rmax, rsq;
         leaf_norms(s, C_RHO, &rmax, &rsq);
         st->rel_linf = srcmax > 0 ? rmax / srcmax : 0.0;
         st->rel_l2 = srcsq > 0 ? sqrt(rsq / srcsq) : 0.0;
+        st->rel_h2 = resh2max > 0 ? h2absmax(s, 0, C_RHO) / resh2max : 0.0;
         if (it < ORC_HIST) st->hist[it] = st->rel_linf;
-        if (p->mode == ORC_STOP_TRUE_RESIDUAL && st->rel_linf < p->tol) { status = 0; break; }
+        if (p->mode == ORC_STOP_TRUE_RESIDUAL && st->rel_linf < p->tol) { status = ORC_OK; break; }
         if (it >= nmax) break;
         fmg_lin(s);
         fmg_add(s, 0, C_PSI, C_U);
@@ -1169,18 +1175,19 @@ int orc_amr_solve(orc_amr *s, double *psi, const double *src, double *g, const o
         double ratio = s->resh2maxg / resh2max;
         st->ratio = ratio;
         if (p->mode == ORC_STOP_REFERENCE) {
-            if (ratio < p->tol) { status = 0; break; }
+            if (ratio < p->tol) { status = ORC_OK; break; }
             if (ratio > prev * 0.5) break;   /* stall test, fmg.F90:188 */
             prev = ratio;
         }
     }
-    if (nmax == 0) status = 0;
+    if (nmax == 0) status = ORC_OK;
     if (p->mode == ORC_STOP_REFERENCE) {
         poisson_resid(s, 0, C_RHO, C_PSI, C_SRC, 0);
         double rmax, rsq;
         leaf_norms(s, C_RHO, &rmax, &rsq);
         st->rel_linf = srcmax > 0 ? rmax / srcmax : 0.0;
         st->rel_l2 = srcsq > 0 ? sqrt(rsq / srcsq) : 0.0;
+        st->rel_h2 = resh2max > 0 ? h2absmax(s, 0, C_RHO) / resh2max : 0.0;
     }
     st->iterations = it;
     st->status = status;

@@ -70,7 +70,7 @@ void orc_gradient_dense(double *g, const double *u, int n, double h);
 enum {
     ORC_AMR_OP_C2P = 1, ORC_AMR_OP_GHOSTFIX, ORC_AMR_OP_FILL0, ORC_AMR_OP_EXTRAP, ORC_AMR_OP_RELAX,
     ORC_AMR_OP_RESID, ORC_AMR_OP_RSTRCT, ORC_AMR_OP_INTERP, ORC_AMR_OP_ADDINT, ORC_AMR_OP_VMG_FAS,
-    ORC_AMR_OP_FMG_LIN, ORC_AMR_OP_SPHERE_BC, ORC_AMR_OP_REMOVE_MEAN
+    ORC_AMR_OP_FMG_LIN, ORC_AMR_OP_SPHERE_BC, ORC_AMR_OP_REMOVE_MEAN, ORC_AMR_OP_VMG_TAU
 };
 
 typedef struct {
@@ -80,7 +80,12 @@ typedef struct {
     double rel_l2;       /* volume-weighted L2 ratio over leaf cells */
     double ratio;        /* Resh2maxg / h2absmax(src) of the last iteration (fmg.F90:183) */
     double hist[ORC_HIST];
+    double rel_h2;       /* max_l(max|src - L psi| h_l^2) / max_l(max|src| h_l^2) (fmg.F90:172, 1017-1039) */
 } orc_amr_stats;
+
+/* status values, as the library's SG_OK / SG_ERR_NOT_CONVERGED */
+#define ORC_OK 0
+#define ORC_NOT_CONVERGED 3
 
 typedef struct orc_amr orc_amr;
 orc_amr *orc_amr_create(int nblk, int nlev, const int *level, const int *ijk, const int *parent,

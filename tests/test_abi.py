@@ -93,3 +93,22 @@ def test_amr_create_rejects_improper_tables_without_touching_the_gpu():
     # a periodic hierarchy may not have open faces
     assert create(bc=_lib.SG_BC_PERIODIC) == _lib.SG_ERR_ARG
     assert b"open face" in lib.sg_last_error()
+
+
+def test_amr_create_rejects_block_positions_outside_the_level_box():
+    """ijk must lie in [0, base_dims << level) before any table built from it is indexed."""
+    import numpy as np
+    from code_only_amd import _lib
+    from code_only_amd import amr_hierarchy as ah
+    lib = _lib.load()
+    h = ah.nested(base=4, levels=2, bc=ah.SPHERE)
+    ip = lambda a: np.ascontiguousarray(a, np.int32).ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    x0 = np.asarray(h.x0, np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    for b, bad in ((0, -1), (0, 4), (int(np.nonzero(h.level == 1)[0][0]), 8)):
+        ijk = np.ascontiguousarray(h.ijk, np.int32).copy()
+        ijk[b, 0] = bad
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (h.level, ijk, h.parent, h.child, h.nbr, h.base_dims)]
+        ctx = ctypes.c_void_p()
+        rc = lib.sg_amr_create(ctypes.byref(ctx), h.nblk, h.nlev, *[ip(a) for a in arrs], h.h0, x0,
+                               _lib.SG_BC_SPHERE)
+        assert rc == _lib.SG_ERR_ARG and b"out of [0" in lib.sg_last_error(), (b, bad, lib.sg_last_error())

@@ -313,3 +313,183 @@ def test_fmg_cycle_converges_to_1e10(bc):
     # the reference's own stopping test (fmg.F90:183-190)
     psi2, _, st2 = A.solve(src, tol=1e-3, maxit=10, mode=oracle.STOP_REFERENCE)
     assert st2["status"] == 0 and st2["ratio"] < 1e-3
+
+
+# ---- vmg_poisson_tau / vmg_tau_crop (vmg_poisson.F90:6-55, vmg.F90:440-506) and the
+#      FmgLevel_fill0 skip of fmg_poisson_flux (fmg_poisson.F90:71-89) ----
+
+LV = 2   # coarsest FMG level: 2^3 cells per block, where vmg_fas works
+
+
+def ghosted_coords(h, L):
+    """Cell-centre coordinates of every block's ghosted FMG-level-L array, each [nblk, w, w, w]
+    in (k, j, i) order, w = n + 2 with n = 8 >> L (ghost cells at index -1 and n included)."""
+    n = B >> L
+    idx = np.arange(-1, n + 1)
+    hl = h.h0 * 2.0 ** L / 2.0 ** h.level.astype(float)          # cell width per block
+    out = []
+    for a in range(3):
+        g = h.ijk[:, a, None] * n + idx[None, :]                  # [nblk, w]
+        out.append(h.x0[a] + (g + 0.5) * hl[:, None])
+    x, y, z = out
+    w = n + 2
+    return (np.broadcast_to(x[:, None, None, :], (h.nblk, w, w, w)),
+            np.broadcast_to(y[:, None, :, None], (h.nblk, w, w, w)),
+            np.broadcast_to(z[:, :, None, None], (h.nblk, w, w, w)))
+
+
+def nkind_parent(h):
+    """[nblk, 6] True where the face meets the parent level (NeighborParentLevel)."""
+    m = h.nbr
+    lv = np.where(m >= 0, h.level[np.maximum(m, 0)], -1)
+    return (m >= 0) & (lv == h.level[:, None] - 1)
+
+
+def crop_mask(h, lc):
+    """[nblk, 2, 2, 2] interior cells of level-lc blocks that vmg_tau_crop zeroes: for every child
+    face meeting the parent level, the parent's boundary plane on that side (vmg.F90:472-503)."""
+    mask = np.zeros((h.nblk, 2, 2, 2), bool)
+    pk = nkind_parent(h)
+    for b in np.nonzero(h.level == lc + 1)[0]:
+        for f in np.nonzero(pk[b])[0]:
+            d, lr = divmod(int(f), 2)
+            sl = [slice(None)] * 3
+            sl[2 - d] = 1 if lr else 0                      # (k, j, i) order
+            mask[(h.parent[b],) + tuple(sl)] = True
+    return mask
+
+
+def tau_after(h, u, lc, sentinel=7.0):
+    """Oracle vmg_tau at AMR level lc on the field u (ghosted LV arrays): U <- u, cubic c2p of U at
+    every level (as vmg_fas does before its tau), RES <- sentinel, then tau into RES."""
+    A = oracle.Amr(h)
+    try:
+        A.set(LV, oracle.U, u)
+        A.op(oracle.OP_C2P, LV, oracle.U, flag=1)
+        A.set(LV, oracle.RES, np.full_like(u, sentinel))
+        A.op(oracle.OP_VMG_TAU, LV, oracle.RES, oracle.U, flag=lc)
+        return A.get(LV, oracle.RES)[:, 1:3, 1:3, 1:3], A.get(LV, oracle.U)
+    finally:
+        A.close()
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_vmg_tau_vanishes_for_cubic_fields(levels):
+    """A cubic field: the cubic restriction (-1, 9, 9, -1)/16 and the 7-point operator are exact
+    for cubics and the 8-average of the (linear) Laplacian is exact, so tau = L_c(R u) - R(L_f u)
+    is zero on every covered parent cell; non-covered cells are left as they were."""
+    h = ah.nested(base=4, levels=levels, bc=ah.SPHERE)
+    x, y, z = ghosted_coords(h, LV)
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal(10)
+    u = (c[0] * x ** 3 + c[1] * y ** 3 + c[2] * z ** 3 + c[3] * x * y * z + c[4] * x ** 2 * y
+         + c[5] * y * z ** 2 + c[6] * x ** 2 + c[7] * y + c[8] * z + c[9])
+    for lc in range(levels - 1):
+        tau, _ = tau_after(h, u, lc)
+        lvl = h.level == lc
+        cov = lvl & ~h.leaf
+        scale = np.abs(u).max() / (h.h0 * 2.0 ** LV / 2.0 ** (lc + 1)) ** 2
+        assert np.abs(tau[cov]).max() < 1e-11 * scale, (lc, np.abs(tau[cov]).max(), scale)
+        assert np.all(tau[lvl & h.leaf] == 7.0)
+        # a sign flip of either term would leave |Lap u| ~ |c| here, far above the bound
+        assert scale * 1e-11 < 1e-3
+
+
+@pytest.mark.parametrize("axes", [1, 3])
+def test_vmg_tau_of_quartic_is_three_h_squared_per_axis(axes):
+    """u = x^4 (+ y^4 + z^4): the cubic restriction of x^4 is X^4 - 9 h^4/16 (a constant shift
+    the coarse operator ignores), L_c(R u) = 12 X^2 + 2 H^2 = 12 X^2 + 8 h^2, and the 8-average of
+    L_f u = 12 x^2 + 2 h^2 is 12 X^2 + 5 h^2, so tau = 3 h_f^2 per quartic axis.  Checked on the
+    covered cells whose six neighbours are covered cells too (outside the crop), together with the
+    crop: exactly zero on the parent planes that children meeting the parent level mark, and only
+    there (the planes derived here from the block tables)."""
+    h = ah.nested(base=8, levels=2, bc=ah.SPHERE)
+    x, y, z = ghosted_coords(h, LV)
+    u = x ** 4 + (y ** 4 + z ** 4 if axes == 3 else 0.0)
+    tau, U = tau_after(h, u, 0)
+    hf = h.h0 * 2.0 ** LV / 2.0
+    cov = np.nonzero((h.level == 0) & ~h.leaf)[0]
+    crop = crop_mask(h, 0)
+    # covered cells of level 0 on the dense base grid: neighbours all covered <=> interior of the
+    # covered box
+    ext = np.asarray(h.base_dims) * 2
+    covgrid = np.zeros((ext[2], ext[1], ext[0]), bool)
+    for b in cov:
+        i, j, k = h.ijk[b] * 2
+        covgrid[k:k + 2, j:j + 2, i:i + 2] = True
+    inner = covgrid.copy()
+    for ax in range(3):
+        inner &= np.roll(covgrid, 1, ax) & np.roll(covgrid, -1, ax)
+    n_checked = 0
+    for b in cov:
+        i, j, k = h.ijk[b] * 2
+        sub_inner = inner[k:k + 2, j:j + 2, i:i + 2]
+        t = tau[b]
+        assert np.all(t[crop[b]] == 0.0), b
+        want = 3.0 * hf * hf * axes
+        sel = sub_inner & ~crop[b]
+        assert np.allclose(t[sel], want, rtol=1e-9, atol=0), (b, t[sel], want)
+        n_checked += int(sel.sum())
+    # the crop marks exactly the covered box's outer cell layer, which no interior cell is on
+    box = np.zeros_like(covgrid)
+    for b in cov:
+        i, j, k = h.ijk[b] * 2
+        box[k:k + 2, j:j + 2, i:i + 2] = crop[b]
+    assert np.array_equal(box, covgrid & ~inner)
+    assert n_checked == int(inner.sum()) > 0
+    # and the restriction really was the cubic one (X^4 - 9 h^4 / 16 per quartic axis)
+    Xc = ghosted_coords(h, LV)
+    b = cov[0]
+    Xs = [c[b, 1:3, 1:3, 1:3] for c in Xc]
+    want_u = Xs[0] ** 4 - 9 * hf ** 4 / 16 + ((Xs[1] ** 4 + Xs[2] ** 4 - 2 * 9 * hf ** 4 / 16) if axes == 3 else 0)
+    assert np.allclose(U[b, 1:3, 1:3, 1:3], want_u, rtol=0, atol=1e-14)
+
+
+def _active(h):
+    """fmg_skip_grid: covered blocks are skipped except on the finest level (none are covered there)."""
+    return h.leaf | (h.level == h.nlev - 1)
+
+
+@pytest.mark.parametrize("which", ["periodic", "sphere"])
+@pytest.mark.parametrize("L", [0, 1])
+def test_fill0_skip_is_value_neutral_from_a_zero_guess(which, L):
+    """fmg_lin sets FmgLevel_fill0 = v+1 right after fmg_fill0 zeroes U on the non-covered blocks of
+    FMG level v+1, ghosts included (fmg.F90:336-340, 847-859); fmg_poisson_flux then skips its ghost
+    refresh once (fmg_poisson.F90:71-89), because on a zero guess the refresh rewrites the same
+    zeros.  So from such a state (covered blocks holding anything), one relax with the skip armed
+    ends in the same non-covered state, bit for bit, as one without it, and the flag is cleared.
+    From a non-zero guess the two differ, which shows that the skip does skip."""
+    h = (ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC) if which == "periodic"
+         else ah.nested(base=4, levels=3, bc=ah.SPHERE))
+    act = _active(h)
+    rng = np.random.default_rng(21)
+    n = B >> L
+    state = {(LL, c): rng.standard_normal((h.nblk, (B >> LL) + 2, (B >> LL) + 2, (B >> LL) + 2))
+             for LL in range(3) for c in range(6)}
+
+    def run(armed, zero_guess):
+        A = oracle.Amr(h)
+        try:
+            for (LL, c), a in state.items():
+                a = a.copy()
+                if zero_guess and LL == L and c == oracle.U:
+                    a[act] = 0.0
+                A.set(LL, c, a)
+            A.fill0_level = L if armed else -1
+            r = A.op(oracle.OP_RELAX, L, oracle.U, oracle.RHS, flag=1)
+            assert A.fill0_level == -1
+            return r, {k: A.get(*k) for k in state}
+        finally:
+            A.close()
+
+    ra, sa = run(False, True)
+    rb, sb = run(True, True)
+    assert ra == rb
+    for k in state:
+        assert np.array_equal(sa[k][act], sb[k][act]), k
+        # covered blocks' interiors are recomputed by the second colour pass's restriction
+        assert np.array_equal(sa[k][~act][:, 1:n + 1, 1:n + 1, 1:n + 1] if k[0] == L else sa[k][~act],
+                              sb[k][~act][:, 1:n + 1, 1:n + 1, 1:n + 1] if k[0] == L else sb[k][~act]), k
+    _, sc = run(False, False)
+    _, sd = run(True, False)
+    assert not np.array_equal(sc[(L, oracle.U)][act], sd[(L, oracle.U)][act])
