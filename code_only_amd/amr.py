@@ -60,12 +60,19 @@ class AmrSolver:
             pass
 
     @staticmethod
-    def _params(tol, maxit, stop_mode, precision=SG_PREC_FP64, use_graph=True):
+    def _params(tol, maxit, stop_mode, precision=SG_PREC_FP64, use_graph=True, fused=True):
         return _lib.default_params(tol=tol, maxit=maxit, stop_mode=stop_mode, precision=precision,
-                                   use_graph=int(bool(use_graph)))
+                                   use_graph=int(bool(use_graph)), fused=int(bool(fused)))
+
+    def info(self):
+        """Context facts: FMG levels whose mixed-solve refreshes run fused (bit mask), kernel nodes of the
+        captured mixed outer iteration, blocks, coarse-fine faces."""
+        v = (ctypes.c_int * 4)()
+        _lib.check(self._lib.sg_amr_info(self._ctx, v, 4))
+        return dict(fused_levels=v[0], graph_nodes=v[1], nblk=v[2], cf_faces=v[3])
 
     def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64,
-              use_graph=True, raise_on_fail=True):
+              use_graph=True, raise_on_fail=True, fused=True):
         """fmg_poisson on torch float64 GPU tensors [nblk, 8, 8, 8]; psi in place, g [nblk, 3, 8, 8, 8].
         With raise_on_fail False, SG_ERR_NOT_CONVERGED / SG_ERR_NONFINITE come back in the stats'
         status instead of an exception."""
@@ -75,7 +82,7 @@ class AmrSolver:
                 raise ValueError("expected contiguous float64 GPU tensors")
         if tuple(psi.shape) != (self.h.nblk, 8, 8, 8) or psi.shape != src.shape:
             raise ValueError("psi and src must be [nblk, 8, 8, 8]")
-        p = self._params(tol, maxit, stop_mode, precision, use_graph)
+        p = self._params(tol, maxit, stop_mode, precision, use_graph, fused)
         st = _lib.SgStats()
         torch.cuda.synchronize()
         _lib.check(self._lib.sg_amr_solve_dev(self._ctx, ctypes.c_void_p(psi.data_ptr()),
@@ -86,11 +93,11 @@ class AmrSolver:
         return st.as_dict()
 
     def solve_host(self, src, psi=None, want_g=True, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL,
-                   precision=SG_PREC_FP64, use_graph=True, raise_on_fail=True):
+                   precision=SG_PREC_FP64, use_graph=True, raise_on_fail=True, fused=True):
         src = np.ascontiguousarray(src, np.float64)
         psi = np.zeros_like(src) if psi is None else np.ascontiguousarray(psi, np.float64).copy()
         g = np.zeros((self.h.nblk, 3, 8, 8, 8)) if want_g else None
-        p = self._params(tol, maxit, stop_mode, precision, use_graph)
+        p = self._params(tol, maxit, stop_mode, precision, use_graph, fused)
         st = _lib.SgStats()
         _lib.check(self._lib.sg_amr_solve_host(self._ctx, _dp(psi), _dp(src), _dp(g) if want_g else None,
                                                ctypes.byref(p), ctypes.byref(st)),

@@ -62,11 +62,16 @@ TESTS_FORTRAN = os.path.join(ROOT, "tests", "fortran")
 # Sources compiled in order (module dependencies first).  fmg_gpu.F90 uses the host's module
 # `grid`; here that is the test harness's (tests/fortran/grid_host.F90), in the reference's build
 # it is the reference's own grid module.
-FORTRAN_SOURCES = [os.path.join(FORTRAN_DIR, "sg_capi.F90"), os.path.join(FORTRAN_DIR, "mg_gpu.F90"),
-                   os.path.join(TESTS_FORTRAN, "grid_host.F90"), os.path.join(FORTRAN_DIR, "fmg_gpu.F90")]
+# fmg_data_host.F90 likewise stands in for the reference's fmg_data module, which mg_data reads the
+# base blocks' FMG arrays from.
+FORTRAN_SOURCES = [os.path.join(FORTRAN_DIR, "sg_capi.F90"), os.path.join(TESTS_FORTRAN, "grid_host.F90"),
+                   os.path.join(TESTS_FORTRAN, "fmg_data_host.F90"), os.path.join(FORTRAN_DIR, "mg_data_gpu.F90"),
+                   os.path.join(FORTRAN_DIR, "mg_gpu.F90"), os.path.join(FORTRAN_DIR, "fmg_gpu.F90")]
 # test driver -> objects it links
+_MG_OBJS = ["sg_capi", "grid_host", "fmg_data_host", "mg_data_gpu", "mg_gpu"]
 FORTRAN_TESTS = {
-    os.path.join(TESTS_FORTRAN, "mg_drop_in_test.F90"): ["sg_capi", "mg_gpu"],
+    os.path.join(TESTS_FORTRAN, "mg_drop_in_test.F90"): _MG_OBJS,
+    os.path.join(TESTS_FORTRAN, "mg_data_drop_in_test.F90"): _MG_OBJS,
     os.path.join(TESTS_FORTRAN, "fmg_drop_in_test.F90"): ["sg_capi", "grid_host", "fmg_gpu"],
 }
 

@@ -59,6 +59,13 @@ struct sg_amr {
     int *d_level = nullptr, *d_ijk = nullptr, *d_parent = nullptr, *d_octant = nullptr, *d_nbr = nullptr;
     int *d_nkind = nullptr, *d_covered = nullptr, *d_skip = nullptr, *d_active = nullptr, *d_phys = nullptr;
     int *d_cf = nullptr, *d_cf_slot = nullptr, *d_ovslot = nullptr, *d_cov = nullptr;
+    // fused refresh (mixed solve): covered blocks' face layers read by same-level leaf neighbours
+    // (entries b*6+f), the child table, and per FMG level whether the hierarchy admits the fused
+    // form (stage_check); fused_on is set for the duration of a mixed solve that asks for it
+    std::vector<int> covf;
+    int *d_covf = nullptr, *d_child = nullptr;
+    int fused_ok[kNfmg] = {};
+    int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
     Arrays<double> d;
@@ -72,6 +79,8 @@ struct sg_amr {
     int bd_ok = 0;
     hipGraphExec_t graph = nullptr;   // mixed solve: one outer iteration, captured once
     int graph_zero_average = -1;      // the zero_average setting the graph was captured with
+    int graph_fused = -1;             // and the fused-refresh setting
+    size_t graph_nodes = 0;           // kernel nodes of the captured outer iteration
     double hl[kNfmg][kMaxLev] = {}, hil[kNfmg][kMaxLev] = {};
     double *d_h[kNfmg] = {}, *d_hi[kNfmg] = {};
     double *d_scal = nullptr;   // [0] Resh2maxg [1] scratch max [2..4] norms [5] mean [8..23] multipole
@@ -232,8 +241,32 @@ template <typename T> int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, r
 
 // fmg_poisson_flux (fmg_poisson.F90:53-124): the face fluxes themselves are formed by their
 // consumers; what remains here is the ghost refresh and the reflux overrides.
+// One launch in place of fmg_poisson_flux's staged refresh and reflux (k_prep): same values in
+// every cell the following colour pass or residual reads (they read same-level neighbours'
+// cells directly).  The FmgLevel_fill0 skip is not needed: from the zero guess it guards, the
+// fused refresh computes the same zeros the skipped refresh would leave.  Physical faces keep
+// their stored values (zero in the inner solve, whose every refresh zeroes them; the multipole
+// values in the outer residual, which refreshes without zeroing).
+template <int N, typename T> int prep_n(sg_amr &s, int L, int ju)
+{
+    constexpr int H = N / 2;
+    const int ncov = (int)s.covf.size(), ncf = (int)s.cf.size();
+    const long work = (long)ncov * N * N + (long)ncf * H * H;
+    return go(s, k_prep<N, T>, work, ar<T>(s).A[L][ju], ar<T>(s).ov[L], (const int *)s.d_covf, ncov,
+              (const int *)s.d_cf, (const int *)s.d_cf_slot, ncf, (const double *)s.d_hi[L], (const int *)s.d_level,
+              (const int *)s.d_octant, (const int *)s.d_nbr, (const int *)s.d_nkind, (const int *)s.d_child,
+              (const int *)s.d_covered);
+}
+
+inline bool fused(const sg_amr &s, int L) { return s.fused_on && L < kNfmg - 1 && s.fused_ok[L]; }
+
 template <typename T> int poisson_flux(sg_amr &s, int L, int ju, int fill0flag)
 {
+    if (fused(s, L)) {
+        s.fill0_level = -1;
+        if (L == 0) return prep_n<8, T>(s, L, ju);
+        return prep_n<4, T>(s, L, ju);
+    }
     if (s.fill0_level != L) {
         SG_TRY(c2p<T>(s, L, ju, 0));
         SG_TRY(ghostcell_fix<T>(s, L, ju, 0));
@@ -271,14 +304,17 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
     }
     for (int ipass = 1; ipass <= 2; ++ipass) {
         SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
+        const bool hf = fused(s, L);
         if (L == 0) {
             hipLaunchKernelGGL((k_blk8<true, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, (T *)nullptr,
                                ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_active, ipass,
                                (const double *)s.d_h[0], (const double *)s.d_hi[0], (const int *)s.d_level,
-                               (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], slots);
+                               (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], slots,
+                               hf ? (const int *)s.d_nbr : (const int *)nullptr,
+                               hf ? (const int *)s.d_nkind : (const int *)nullptr);
             SG_CHECK_LAUNCH();
         } else {
-            SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_active, 0, (int)s.active.size(), ipass, s.d_ovslot, slots));
+            SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_active, 0, (int)s.active.size(), ipass, s.d_ovslot, slots, hf));
         }
     }
     if (want_max) {
@@ -291,19 +327,23 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
 template <int N, typename T> int resid_n(sg_amr &s, int L, int jres, int ju, int jrhs)
 {
     int na = (int)s.active.size();
+    const bool hf = fused(s, L);
     return go(s, k_resid<N, T>, (long)na * N * N * N, ar<T>(s).A[L][jres], (const T *)ar<T>(s).A[L][ju],
               (const T *)ar<T>(s).A[L][jrhs], (const int *)s.d_active, na, (const double *)s.d_hi[L],
-              (const int *)s.d_level, (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[L]);
+              (const int *)s.d_level, (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[L],
+              hf ? (const int *)s.d_nbr : (const int *)nullptr, hf ? (const int *)s.d_nkind : (const int *)nullptr);
 }
 
 template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int jrhs, int fill0flag)
 {
     SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
     if (L == 0) {
+        const bool hf = fused(s, 0);
         hipLaunchKernelGGL((k_blk8<false, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, ar<T>(s).A[0][jres],
                            ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_active, 0,
                            (const double *)s.d_h[0], (const double *)s.d_hi[0], (const int *)s.d_level,
-                           (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], (double *)nullptr);
+                           (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], (double *)nullptr,
+                           hf ? (const int *)s.d_nbr : (const int *)nullptr, hf ? (const int *)s.d_nkind : (const int *)nullptr);
         SG_CHECK_LAUNCH();
         return SG_OK;
     }
@@ -805,6 +845,12 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
     const bool src_finite = std::isfinite(resh2max) && std::isfinite(srcmax) && std::isfinite(srcsq);
     int nmax = src_finite ? p.maxit : 0;
     if (std::fabs(resh2max) <= 2.2250738585072014e-308) nmax = 0;
+    // fused refreshes in the outer residual and the inner fmg_lin (same values as the staged ones)
+    struct FusedScope {
+        sg_amr &s;
+        FusedScope(sg_amr &s_, int on) : s(s_) { s.fused_on = on; }
+        ~FusedScope() { s.fused_on = 0; }
+    } fused_scope(s, p.fused ? 1 : 0);
     SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
     SG_TRY(norms(s, C_RHO));
     SG_TRY(read_scalars(s));
@@ -820,18 +866,20 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
         if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = SG_OK; break; }
         if (it >= nmax) break;
         if (p.use_graph) {
-            if (!s.graph || s.graph_zero_average != s.zero_average) {
+            if (!s.graph || s.graph_zero_average != s.zero_average || s.graph_fused != s.fused_on) {
                 if (s.graph) (void)hipGraphExecDestroy(s.graph);
                 s.graph = nullptr;
                 hipGraph_t gr = nullptr;
                 SG_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
                 int rc = mixed_iteration(s);
                 hipError_t e = hipStreamEndCapture(s.stream, &gr);
+                if (rc == SG_OK && e == hipSuccess) e = hipGraphGetNodes(gr, nullptr, &s.graph_nodes);
                 if (rc == SG_OK && e == hipSuccess) e = hipGraphInstantiate(&s.graph, gr, nullptr, nullptr, 0);
                 if (gr) (void)hipGraphDestroy(gr);
                 if (rc != SG_OK) return rc;
                 SG_HIP(e);
                 s.graph_zero_average = s.zero_average;
+                s.graph_fused = s.fused_on;
             }
             SG_HIP(hipGraphLaunch(s.graph, s.stream));
         } else {
@@ -854,6 +902,7 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
     if (nmax == 0 && src_finite) status = SG_OK;
     st.iterations = it;
     st.status = status;
+    s.fused_on = 0;   // fmg_psi2g reads the stored ghosts: the staged refresh below writes them
     SG_TRY(fmg_copy<double>(s, 0, C_U, C_PSI, 1));
     SG_TRY(c2p<double>(s, 0, C_U, 0));
     SG_TRY(go(s, k_blocks_to_dense, (long)s.nblk * 512, psi, (const double *)s.d.A[0][C_U], s.nblk));
@@ -870,6 +919,64 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
     return status;
 }
 
+// Host mirror of the control flow of Staged<N, T>::S<D> (amr_kernels.h): false when evaluating the
+// cell (i, j, k) of block b would meet a coarse-fine face, or read a covered cell whose child in
+// that octant is covered itself (the fused refresh restricts leaf children only).
+bool staged_reach(const sg_amr &s, int N, int b, int i, int j, int k)
+{
+    int p[3] = {i, j, k};
+    for (int D = 2; D >= 0; --D) {
+        if (p[D] == -1 || p[D] == N) {
+            const int f = 6 * b + 2 * D + (p[D] == N ? 1 : 0);
+            if (s.nkind[f] == NBR_PARENT) return false;
+            if (s.nkind[f] == NBR_SAME) {
+                p[D] = p[D] == N ? 0 : N - 1;
+                b = s.nbr[f];
+            }
+        }
+    }
+    const bool interior = p[0] >= 0 && p[0] < N && p[1] >= 0 && p[1] < N && p[2] >= 0 && p[2] < N;
+    if (interior && s.covered[b]) {
+        const int H = N / 2, o = (p[0] >= H ? 1 : 0) + (p[1] >= H ? 2 : 0) + (p[2] >= H ? 4 : 0);
+        if (s.covered[s.child[8 * b + o]]) return false;
+    }
+    return true;
+}
+
+// Whether the fused refresh reproduces the staged one at an FMG level with N cells per block side:
+// every coarse value a coarse-fine ghost needs, and every covered face cell a colour pass reads,
+// must be reachable without a coarse-fine face and with leaf children (staged_reach).
+bool stage_check(const sg_amr &s, int N)
+{
+    const int H = N / 2;
+    for (size_t e = 0; e < s.cf.size(); ++e) {
+        const int b = s.cf[e] / 6, f = s.cf[e] % 6, d = f / 2, lr = f & 1;
+        const int m = s.nbr[6 * b + f], o = s.octant[b];
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
+        for (int x = -1; x <= H; ++x)
+            for (int y = -1; y <= H; ++y) {
+                int p[3];
+                p[d] = lr ? 0 : N - 1;
+                p[ta] = x + e1;
+                p[tb] = y + e2;
+                if (!staged_reach(s, N, m, p[0], p[1], p[2])) return false;
+            }
+    }
+    for (int e : s.covf) {
+        const int b = e / 6, f = e % 6, d = f / 2, a = (f & 1) ? N - 1 : 0;
+        for (int t2 = 0; t2 < N; ++t2)
+            for (int t1 = 0; t1 < N; ++t1) {
+                int p[3];
+                p[d] = a;
+                p[d == 0 ? 1 : 0] = t1;
+                p[d == 2 ? 1 : 2] = t2;
+                if (!staged_reach(s, N, b, p[0], p[1], p[2])) return false;
+            }
+    }
+    return true;
+}
+
 template <typename T> int upload(sg_amr &s, T **dptr, const std::vector<T> &v)
 {
     size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
@@ -882,7 +989,8 @@ void release(sg_amr *s)
 {
     if (!s) return;
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
-                 s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters};
+                 s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters,
+                 s->d_covf, s->d_child};
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     if (s->bd_mem) (void)hipFree(s->bd_mem);
@@ -1003,6 +1111,12 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     s->phys_off[nlev] = (int)s->phys.size();
     s->cf_off[nlev] = (int)s->cf.size();
     s->cov_off[nlev] = (int)s->cov.size();
+    for (int b = 0; b < nblk; ++b) {
+        if (!s->covered[b]) continue;
+        for (int f = 0; f < 6; ++f)
+            if (s->nkind[6 * b + f] == NBR_SAME && !s->covered[nbr[6 * b + f]]) s->covf.push_back(6 * b + f);
+    }
+    for (int L = 0; L < kNfmg - 1; ++L) s->fused_ok[L] = stage_check(*s, 8 >> L) ? 1 : 0;
     // device tables
     SG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     SG_HIP(hipEventCreate(&s->ev0));
@@ -1021,6 +1135,8 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     SG_TRY(upload(*s, &s->d_cf_slot, s->cf_slot));
     SG_TRY(upload(*s, &s->d_ovslot, s->ovslot));
     SG_TRY(upload(*s, &s->d_cov, s->cov));
+    SG_TRY(upload(*s, &s->d_covf, s->covf));
+    SG_TRY(upload(*s, &s->d_child, s->child));
     for (int L = 0; L < kNfmg; ++L) {
         for (int c = 0; c < NCODE; ++c) SG_HIP(hipMalloc((void **)&s->d.A[L][c], sizeof(double) * nblk * lenof(L)));
         SG_HIP(hipMalloc((void **)&s->d.ov[L], sizeof(double) * std::max(1, s->nslots) * nof(L) * nof(L)));
@@ -1253,6 +1369,15 @@ int sg_amr_set_fill0_level(sg_amr *ctx, int v)
 {
     SG_REQUIRE(ctx, "sg_amr_set_fill0_level: null context");
     ctx->fill0_level = v;
+    return SG_OK;
+}
+
+int sg_amr_info(sg_amr *ctx, int *info, int n)
+{
+    SG_REQUIRE(ctx && info && n >= 0, "sg_amr_info: null argument");
+    const long v[4] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
+                       (long)ctx->cf.size()};
+    for (int q = 0; q < n; ++q) info[q] = q < 4 ? (int)v[q] : 0;
     return SG_OK;
 }
 

@@ -338,6 +338,123 @@ __global__ void k_ghost_level(T *A, int b0, int nb, int cubic, const int *nbr, c
     A[(long)b * B::LEN + c] = G.template S<2>(b, i, j, k);
 }
 
+// ---- fused refresh of the mixed solve (fmg_poisson_flux without its staged launches) ----
+// The values a refresh (fmg_converge_c2p, then the staged fmg_ghostcell_fix) leaves in the cells
+// that a colour pass or a residual reads, evaluated directly from the blocks' current interiors:
+// a covered block's interior cell as the linear restriction of its child's cells (the child must
+// be a leaf), a ghost cell by the staged passes' rule (the value of the last direction pass that
+// writes it: a same-level copy of the neighbour's value after the earlier passes, or, on a
+// physical face, the stored value).  Valid when none of the cells needed meets a coarse-fine face
+// in that recursion and every covered cell needed has a leaf child; amr.hip checks both per
+// hierarchy on the host (stage_check) before using it.
+template <int N, typename T> struct Staged {
+    using B = Blk<N>;
+    static constexpr int H = N / 2;
+    const T *A;
+    const int *nbr, *nkind, *child, *covered;
+
+    __device__ T cell(int b, int i, int j, int k) const
+    {
+        if (!covered[b]) return A[(long)b * B::LEN + B::gi(i, j, k)];
+        const int o = (i >= H ? 1 : 0) + (j >= H ? 2 : 0) + (k >= H ? 4 : 0);
+        const T *uf = A + (long)child[8 * b + o] * B::LEN;
+        return rstr_lin<N, T>(uf, 2 * (i - (o & 1) * H), 2 * (j - ((o >> 1) & 1) * H), 2 * (k - ((o >> 2) & 1) * H));
+    }
+
+    template <int D> __device__ T S(int b, int i, int j, int k) const
+    {
+        if constexpr (D < 0) {
+            if (i >= 0 && i < N && j >= 0 && j < N && k >= 0 && k < N) return cell(b, i, j, k);
+            return A[(long)b * B::LEN + B::gi(i, j, k)];
+        } else {
+            const int c = D == 0 ? i : (D == 1 ? j : k);
+            if (c == -1 || c == N) {
+                const int f = b * 6 + 2 * D + (c == N ? 1 : 0);
+                if (nkind[f] == NBR_SAME) {
+                    const int m = nbr[f], layer = c == N ? 0 : N - 1;
+                    if (D == 0) return S<D - 1>(m, layer, j, k);
+                    if (D == 1) return S<D - 1>(m, i, layer, k);
+                    return S<D - 1>(m, i, j, layer);
+                }
+                // coarse-fine faces are excluded by the host check; physical faces keep the value
+            }
+            return S<D - 1>(b, i, j, k);
+        }
+    }
+};
+
+// One launch per refresh: work items [0, ncov * N^2) write the listed covered blocks' face layers
+// (entries b*6+f: the layer next to face f, whose neighbour is a same-level leaf that a colour pass
+// reads it from); items [ncov N^2, + ncf H^2) each take one coarse face cell of a coarse-fine face:
+// the four fine ghost cells behind it (fmg_ghfix_parentlev: bilinear (9,3,3,1)/16 tangentially,
+// then (10 u0 + 8 uH - 3 uN)/15 normally, the operand order of k_ghost) and the reflux override of
+// the coarse neighbour there (the 2x2 average of the fine face fluxes, the order of k_reflux).
+// Reads only leaf interiors and physical ghost cells, which no item writes.
+template <int N, typename T>
+__global__ void k_prep(T *A, T *ov, const int *covf, int ncov, const int *cf, const int *cf_slot, int ncf,
+                       const double *hi_lev, const int *level, const int *octant, const int *nbr, const int *nkind,
+                       const int *child, const int *covered)
+{
+    using B = Blk<N>;
+    constexpr int H = N / 2;
+    const Staged<N, T> St{A, nbr, nkind, child, covered};
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n1 = (long)ncov * N * N;
+    if (t < n1) {
+        const int e = covf[t / (N * N)], t1 = (int)(t % N), t2 = (int)((t / N) % N);
+        const int b = e / 6, f = e % 6, d = f >> 1, a = (f & 1) ? N - 1 : 0;
+        int i, j, k;
+        if (d == 0) { i = a; j = t1; k = t2; } else if (d == 1) { i = t1; j = a; k = t2; } else { i = t1; j = t2; k = a; }
+        A[(long)b * B::LEN + B::gi(i, j, k)] = St.cell(b, i, j, k);
+        return;
+    }
+    t -= n1;
+    if (t >= (long)ncf * H * H) return;
+    const int c1 = (int)(t % H), c2 = (int)((t / H) % H), e = (int)(t / (H * H));
+    const int b = cf[e] / 6, f = cf[e] % 6, d = f >> 1, lr = f & 1;
+    const int m = nbr[6 * b + f];
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int o = octant[b];
+    const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
+    const int an = lr ? 0 : N - 1, ag = lr ? N : -1, inner = lr ? -1 : 1;
+    auto buf = [&](int x, int y) -> T {
+        int p[3];
+        p[d] = an;
+        p[ta] = x + e1;
+        p[tb] = y + e2;
+        return St.template S<2>(m, p[0], p[1], p[2]);
+    };
+    T *u = A + (long)b * B::LEN;
+    const T hi = (T)hi_lev[level[b]];
+    T ffv[2][2];
+#pragma unroll
+    for (int q2 = 0; q2 < 2; ++q2)
+#pragma unroll
+        for (int q1 = 0; q1 < 2; ++q1) {
+            const int s1 = 2 * c1 + q1, s2 = 2 * c2 + q2;
+            const int cc1 = ijkc(s1), cc2 = ijkc(s2);
+            const int c1n = near_(s1, cc1), c2n = near_(s2, cc2);
+            T Sv = (T)9.0 * buf(cc1, cc2) + buf(c1n, c2n);
+            Sv = Sv + (T)3.0 * buf(cc1, c2n);
+            Sv = Sv + (T)3.0 * buf(c1n, cc2);
+            const T uH = Sv / (T)16.0;
+            const T u0 = u[B::rot(d, ag + inner, s1, s2)];
+            const T uN = u[B::rot(d, ag + 2 * inner, s1, s2)];
+            T q = (T)10.0 * u0 + (T)8.0 * uH;
+            q = q - (T)3.0 * uN;
+            const T g = q / (T)15.0;
+            u[B::rot(d, ag, s1, s2)] = g;
+            // face flux (u(af + 1) - u(af)) hi with af = -1 (left face) or N - 1 (right face)
+            ffv[q1][q2] = lr ? (g - u0) * hi : (u0 - g) * hi;
+        }
+    T v = ffv[0][0] + ffv[0][1];
+    v = v + ffv[1][0];
+    v = v + ffv[1][1];
+    const int ta2 = d == 0 ? 1 : 0, tb2 = d == 2 ? 1 : 2;
+    const int off1 = ((o >> ta2) & 1) * H, off2 = ((o >> tb2) & 1) * H;
+    ov[(long)cf_slot[e] * N * N + (off1 + c1) + N * (off2 + c2)] = v * (T)0.25;
+}
+
 // ---- fmg_boundary_fill0 on a list of physical faces (entries b*6+f) ----
 template <int N, typename T> __global__ void k_fill0_faces(T *A, const int *faces, int nf)
 {
@@ -495,7 +612,8 @@ __global__ void k_relax(T *Au, const T *Arhs, const int *list, int b0, int nb, i
 // ---- fmg_poisson_resid: res = rhs - div F / h on the listed blocks ----
 template <int N, typename T>
 __global__ void k_resid(T *Ares, const T *Au, const T *Arhs, const int *list, int nb,
-                        const double *hi_lev, const int *level, const int *ovslot, const T *ov)
+                        const double *hi_lev, const int *level, const int *ovslot, const T *ov,
+                        const int *nbr = nullptr, const int *nkind = nullptr)
 {
     using B = Blk<N>;
     constexpr int PER = N * N * N;
@@ -505,6 +623,7 @@ __global__ void k_resid(T *Ares, const T *Au, const T *Arhs, const int *list, in
     int b = list[t / PER];
     const int l = level[b];
     Flux<N, T> F{Au + (long)b * B::LEN, ovslot + 6 * b, ov, (T)hi_lev[l]};
+    if (nbr) { F.A = Au; F.nb = nbr + 6 * b; F.kind = nkind + 6 * b; }
     const long q = (long)b * B::LEN + B::gi(i, j, k);
     Ares[q] = Arhs[q] - F.div(i, j, k) * (T)hi_lev[l];
 }
@@ -765,13 +884,25 @@ __global__ void k_norms_final(const double *part, int G, double *out)
 template <bool RELAX, typename T>
 __global__ void __launch_bounds__(256) k_blk8(T *Aout, T *Au, const T *Arhs, const int *list, int ipass,
                                              const double *h_lev, const double *hi_lev, const int *level,
-                                             const int *ovslot, const T *ov, double *resmax)
+                                             const int *ovslot, const T *ov, double *resmax,
+                                             const int *nbr = nullptr, const int *nkind = nullptr)
 {
     using B = Blk<8>;
     __shared__ T su[B::LEN];
     const int b = list[blockIdx.x];
     const T *ug = Au + (long)b * B::LEN;
     for (int q = threadIdx.x; q < B::LEN; q += 256) su[q] = ug[q];
+    if (nbr) {
+        // halo-free form: the same-level faces' ghost cells taken from the neighbours' boundary
+        // layers (what fmg_ghfix_samelev would copy there); only the face cells the 7-point
+        // operator reads, not edges or corners.  The neighbour's cells next to this block's
+        // colour-ipass cells have the other colour, so a concurrent pass does not change them.
+        for (int q = threadIdx.x; q < 6 * 64; q += 256) {
+            const int f = q >> 6, t1 = q & 7, t2 = (q >> 3) & 7, d = f >> 1, lr = f & 1;
+            if (nkind[6 * b + f] != NBR_SAME) continue;
+            su[B::rot(d, lr ? 8 : -1, t1, t2)] = Au[(long)nbr[6 * b + f] * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
+        }
+    }
     __syncthreads();
     const int l = level[b];
     const T h = (T)h_lev[l], hi = (T)hi_lev[l];

@@ -1,15 +1,16 @@
 !-------------------------------------------------------------------------
 ! Drop-in for the reference's uniform multigrid module `mg` (mg.F90:8-14):
 ! same module name and public entry points (mg_init, mg_cycle, mg_lin,
-! mg_fill0, mg_boundary_fill0, mg_finalize), same array codes (IU, IRHO,
-! IPSI, ISRC, ... as fmg_data.F90:89) and the same ghosted storage
-! u(-1:n,-1:n,-1:n,0:0) per code (mg_data.F90:398-417).  The solve itself runs
-! on the GPU through libsg's C ABI (sg_capi).  Only the level-0 arrays are held
-! on the host: coarser levels live on the device for the whole run.
+! mg_fill0, mg_boundary_fill0, mg_finalize), working on module mg_data's
+! arrays (mg_data_gpu.F90: same codes as fmg_data.F90:89, same ghosted storage
+! u(-1:n,-1:n,-1:n,0:0) per code, filled by mg_dataPrepare).  The solve itself
+! runs on the GPU through libsg's C ABI (sg_capi).  Only the level-0 arrays are
+! held on the host: coarser levels live on the device for the whole run.
 !
 ! Differences from the reference, by design:
-!  * the domain size and cell width come from mg_set_domain instead of the
-!    config.h macros NGI_BASE*NI (mg_data.F90:58-75);
+!  * the domain size and cell width come from the host's module grid
+!    (GidBase, CellWidth) or from mg_set_domain, instead of the config.h
+!    macros NGI_BASE*NI (mg_data.F90:58-75);
 !  * the stopping test is selectable (mg_set_solver); the default is the
 !    reference's Resmaxg/rhomax < 1e-2 with at most 10 iterations
 !    (mg.F90:50-52,91).
@@ -19,31 +20,19 @@
 module mg
   use iso_c_binding
   use sg_capi
+  use mg_data
   implicit none
   private
-  integer,parameter,public :: IDBG=-1, IU=0, IRES=1, IRHS=2, IRHO=3, IFLX=4, ISRC=5, IPSI=6, IRUF=7
-  integer,parameter :: ICODEMIN = -1, ICODEMAX = 7
-  integer,save :: NBase = 0, N = 0
-  real(kind=8),save :: H0 = 0.d0, H = 0.d0
-  integer,save :: FMG_Level = -1
   type(c_ptr),save :: Ctx = c_null_ptr
-  real(kind=8),save,pointer :: Arr(:,:,:,:,:) => null()   ! (-1:N,-1:N,-1:N,0:0,ICODEMIN:ICODEMAX)
+  integer,save :: CtxLevel = -1
   type(sg_params),save :: Params
   logical,save :: ParamsSet = .false.
   type(sg_stats),save :: LastStats
   real(kind=8),save,public :: Resmaxg = 0.d0   ! mg.F90:12
+  public :: IDBG, IU, IRES, IRHS, IRHO, IFLX, ISRC, IPSI, IRUF
   public :: mg_lin, mg_cycle, mg_boundary_fill0, mg_fill0, mg_nonlin, mg_nonlin_vcycle, mg_init, mg_finalize
   public :: mg_set_domain, mg_set_solver, mg_get_arrp, mg_last_iterations, mg_last_residual
 contains
-  !-------------------------------------------------------------------------
-  ! base grid: nbase cells per side (power of two), cell width h0 at fmglev 0
-  !-------------------------------------------------------------------------
-  subroutine mg_set_domain(ncells, width)
-    integer,intent(IN) :: ncells
-    real(kind=8),intent(IN) :: width
-    NBase = ncells
-    H0 = width
-  end subroutine mg_set_domain
   !-------------------------------------------------------------------------
   ! stopping test and arithmetic of mg_cycle
   !-------------------------------------------------------------------------
@@ -58,51 +47,34 @@ contains
     Params%precision = precision
   end subroutine mg_set_solver
   !-------------------------------------------------------------------------
-  ! mg.F90:525-528 / mg_data.F90:45-90
+  ! mg.F90:525-528: mg_data_init, then the device context of that grid
   !-------------------------------------------------------------------------
   subroutine mg_init(fmglev)
     integer,intent(IN) :: fmglev
-    if (fmglev == FMG_Level) return
-    if (FMG_Level >= 0) call mg_finalize
-    if (NBase <= 0) then
-       print *, '*** mg_init: call mg_set_domain first'
-       stop 1
-    end if
-    FMG_Level = fmglev
-    N = NBase / 2**fmglev
-    H = H0 * 2**fmglev
+    if (fmglev == CtxLevel .and. c_associated(Ctx)) return
+    call release_ctx
+    call mg_data_init(fmglev)
     if (.not. ParamsSet) then
        call sg_default_params(Params)
        ParamsSet = .true.
     end if
-    call sg_check(sg_uniform_create(Ctx, int(N, c_int), real(H, c_double), SG_PREC_FP64 + SG_PREC_MIXED), 'mg_init')
-    allocate(Arr(-1:N, -1:N, -1:N, 0:0, ICODEMIN:ICODEMAX))
-    Arr = 0.d0
+    call sg_check(sg_uniform_create(Ctx, int(Nmg, c_int), real(Hmg, c_double), SG_PREC_FP64 + SG_PREC_MIXED), 'mg_init')
+    CtxLevel = fmglev
   end subroutine mg_init
+
+  subroutine release_ctx
+    integer(c_int) :: rc
+    if (c_associated(Ctx)) rc = sg_uniform_destroy(Ctx)
+    Ctx = c_null_ptr
+    CtxLevel = -1
+  end subroutine release_ctx
   !-------------------------------------------------------------------------
   ! mg.F90:532-536
   !-------------------------------------------------------------------------
   subroutine mg_finalize
-    integer(c_int) :: rc
-    if (c_associated(Ctx)) rc = sg_uniform_destroy(Ctx)
-    Ctx = c_null_ptr
-    if (associated(Arr)) deallocate(Arr)
-    nullify(Arr)
-    FMG_Level = -1
+    call release_ctx
+    call mg_data_finalize
   end subroutine mg_finalize
-  !-------------------------------------------------------------------------
-  ! level-0 array of a code (mg_data.F90:472-504); coarser levels are on the GPU
-  !-------------------------------------------------------------------------
-  function mg_get_arrp(mglev, icode) result(arrp)
-    integer,intent(IN) :: mglev, icode
-    real(kind=8),pointer,dimension(:,:,:,:) :: arrp
-    if (mglev /= 0 .or. icode < ICODEMIN .or. icode > ICODEMAX .or. .not. associated(Arr)) then
-       print *, '*** This code is not supported', mglev, icode
-       stop 1
-    end if
-    ! keep the reference's bounds u(-1:n,-1:n,-1:n,0:0): a plain section pointer would start at 1
-    arrp(-1:, -1:, -1:, 0:) => Arr(:,:,:,:,icode)
-  end function mg_get_arrp
   !-------------------------------------------------------------------------
   ! mg.F90:654-660
   !-------------------------------------------------------------------------
@@ -128,8 +100,9 @@ contains
     integer,intent(IN) :: ju, jrho
     real(kind=8),pointer,dimension(:,:,:,:) :: u, f
     integer(c_int) :: rc
-    u => mg_get_arrp(0, ju)
-    f => mg_get_arrp(0, jrho)
+    call need_ctx('mg_cycle')
+    u => mg_get_arrp(MG_LevelMin, ju)
+    f => mg_get_arrp(MG_LevelMin, jrho)
     rc = sg_uniform_cycle_host(Ctx, u, f, Params, LastStats)
     if (rc /= SG_ERR_NOT_CONVERGED) call sg_check(rc, 'mg_cycle')
     Resmaxg = LastStats%resmaxg
@@ -141,17 +114,26 @@ contains
     type(sg_params) :: p
     real(kind=8),pointer,dimension(:,:,:,:) :: u, f
     integer(c_int) :: rc
+    call need_ctx('mg_lin')
     p = Params
     p%maxit = 1
     p%tol = 0.d0
     p%stop_mode = SG_STOP_TRUE_RESIDUAL
-    u => mg_get_arrp(0, IU)
-    f => mg_get_arrp(0, IRHO)
+    u => mg_get_arrp(MG_LevelMin, IU)
+    f => mg_get_arrp(MG_LevelMin, IRHO)
     u = 0.d0
     rc = sg_uniform_cycle_host(Ctx, u, f, p, LastStats)
     if (rc /= SG_ERR_NOT_CONVERGED) call sg_check(rc, 'mg_lin')
     Resmaxg = LastStats%resmaxg
   end subroutine mg_lin
+
+  subroutine need_ctx(where)
+    character(len=*),intent(IN) :: where
+    if (.not. c_associated(Ctx)) then
+       print *, '*** ', where, ': call mg_init first'
+       stop 1
+    end if
+  end subroutine need_ctx
 
   integer function mg_last_iterations()
     mg_last_iterations = LastStats%iterations

@@ -5,7 +5,8 @@
 ! ParentGid, ChildGid, NeighborGid (same-level neighbours, Undefi
 ! elsewhere), CellWidth, BlockMem(gid)%u(-2:9,-2:9,-2:9,0:NM-1) and the
 ! component pointers get_Ucomp / get_U1comp (lower bounds -2, as slice3
-! returns them).  Array sizes are set at run time by grid_host_alloc.  It
+! returns them), and the base-block tables GidBase / RankBase (grid.F90:54).
+! Array sizes are set at run time by grid_host_alloc / grid_host_base.  It
 ! exists so that the drop-in module `fmg` can be compiled and driven as the
 ! reference's time step would drive it; it is not part of the library.
 !-------------------------------------------------------------------------
@@ -23,6 +24,7 @@ module grid
   integer,save,allocatable :: GidList(:,:), GidListMax(:)
   integer,save,allocatable :: ParentGid(:,:), ChildGid(:,:,:,:,:), NeighborGid(:,:,:,:)
   real(kind=8),save,allocatable :: CellWidth(:,:)
+  integer,save,allocatable :: GidBase(:,:,:), RankBase(:,:,:)
   integer,save :: LevelMax = Undefi
 contains
   subroutine grid_host_alloc(ngid, nl, nm)
@@ -41,6 +43,22 @@ contains
     GidList = Undefi; GidListMax = Undefi
     ParentGid = Undefi; ChildGid = Undefi; NeighborGid = Undefi
   end subroutine grid_host_alloc
+
+  ! base blocks nx x ny x nz, numbered x fastest, all on rank 0
+  subroutine grid_host_base(nx, ny, nz)
+    integer,intent(IN) :: nx, ny, nz
+    integer :: i, j, k
+    if (allocated(GidBase)) deallocate(GidBase, RankBase)
+    allocate(GidBase(0:nx-1, 0:ny-1, 0:nz-1), RankBase(0:nx-1, 0:ny-1, 0:nz-1))
+    do k = 0, nz - 1
+       do j = 0, ny - 1
+          do i = 0, nx - 1
+             GidBase(i, j, k) = (k * ny + j) * nx + i
+          end do
+       end do
+    end do
+    RankBase = 0
+  end subroutine grid_host_base
 
   function get_Ucomp(ncomp, gid) result(ptr)
     integer,intent(IN) :: ncomp, gid

@@ -285,3 +285,46 @@ def test_zero_average_toggle_recaptures_the_graph(pair):
         g.set_zero_average(True)
     assert se["iterations"] == sg["iterations"] == 2
     assert np.array_equal(pe, pg)
+
+
+# ---- fused refresh of the mixed solve ----
+
+@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_fused_refresh_equals_staged(which, graph, gpu):
+    """The mixed solve with one-launch refreshes (k_prep + halo-free colour passes) against the
+    same solve with the staged refresh launches: bit for bit the same iterates (potential,
+    acceleration, residual history), since every cell a colour pass or residual reads gets the
+    value the staged fill gives it.  The nested hierarchies must admit the fused form."""
+    h = {"nested-sphere": lambda: ah.nested(base=4, levels=3, bc=ah.SPHERE),
+         "nested-periodic": lambda: ah.nested(base=4, levels=3, bc=ah.PERIODIC),
+         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC)}[which]()
+    g = AmrSolver(h)
+    info = g.info()
+    print(which, info, flush=True)
+    if which.startswith("nested"):
+        assert info["fused_levels"] == 3
+    src = problems.amr_source(h, seed=17)
+    ps, gs, ss = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=graph, fused=False)
+    pf, gf, sf = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=graph, fused=True)
+    print(ss["iterations"], sf["iterations"], ss["hist"], sf["hist"], flush=True)
+    assert ss["iterations"] == sf["iterations"] and ss["hist"] == sf["hist"]
+    assert np.array_equal(ps, pf) and np.array_equal(gs, gf)
+    g.close()
+
+
+def test_fused_refresh_full_sphere_hierarchy_launches(gpu):
+    """16^3 base blocks, five levels: the fused form applies at FMG levels 0 and 1, reproduces
+    the staged mixed solve bit for bit, and needs fewer kernel nodes per outer iteration."""
+    h = ah.nested(base=16, levels=5, bc=ah.SPHERE)
+    src = problems.sis_source(h, seed=1)
+    g = AmrSolver(h)
+    ps, gs, ss = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, fused=False)
+    n_staged = g.info()["graph_nodes"]
+    pf, gf, sf = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, fused=True)
+    info = g.info()
+    print("nodes staged", n_staged, "fused", info["graph_nodes"], "ms", ss["ms"], sf["ms"], flush=True)
+    assert info["fused_levels"] == 3
+    assert ss["iterations"] == sf["iterations"] and ss["hist"] == sf["hist"]
+    assert np.array_equal(ps, pf) and np.array_equal(gs, gf)
+    assert info["graph_nodes"] < n_staged
