@@ -295,7 +295,11 @@ int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int 
 
 // fmg_poisson_relax (fmg_poisson.F90:130-213).  With want_max, Resh2maxg lands in d_scal[0]; the
 // reference overwrites Resh2maxg in every call but reads only the last one of fmg_lin.
-template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int fill0flag, bool want_max)
+// staged_last: the refresh before the second colour runs staged even where the fused form applies,
+// so that every ghost cell afterwards holds the reference's value (the fused refresh writes only
+// the cells a colour pass reads; a later cubic fmg_converge_c2p reads the others).
+template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int fill0flag, bool want_max,
+                                        bool staged_last = false)
 {
     double *slots = want_max ? s.d_slots : nullptr;
     if (want_max) {
@@ -303,7 +307,14 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
         SG_CHECK_LAUNCH();
     }
     for (int ipass = 1; ipass <= 2; ++ipass) {
-        SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
+        if (staged_last && ipass == 2 && s.fused_on) {
+            s.fused_on = 0;
+            const int rc = poisson_flux<T>(s, L, ju, fill0flag);
+            s.fused_on = 1;
+            SG_TRY(rc);
+        } else {
+            SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
+        }
         const bool hf = fused(s, L);
         if (L == 0) {
             hipLaunchKernelGGL((k_blk8<true, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, (T *)nullptr,
@@ -593,8 +604,12 @@ template <typename T> int fmg_lin(sg_amr &s)
             SG_TRY(vmg_fas<T>(s, C_RHS));
             for (int v = Lmax - 1; v >= Lf; --v) {
                 SG_TRY(fmg_addint<T>(s, v, C_U, C_U, C_RES));
+                // the next FMG stage's cubic fmg_converge_c2p at FMG level Lf reads the ghosts of
+                // the last relaxation there
+                const bool last = jc == 1 && v == Lf && Lf >= 1;
                 for (int p = 0; p < 2; ++p)
-                    SG_TRY(poisson_relax<T>(s, v, C_U, C_RHS, 1, Lf == 0 && jc == 1 && v == 0 && p == 1));
+                    SG_TRY(poisson_relax<T>(s, v, C_U, C_RHS, 1, Lf == 0 && jc == 1 && v == 0 && p == 1,
+                                            last && p == 1));
             }
         }
     }
@@ -1225,6 +1240,13 @@ int op(sg_amr &s, int opc, int L, int a, int b, int c, int flag, double *ret)
     SG_REQUIRE(L >= 0 && L < kNfmg, "sg_amr_op: FMG level %d out of range", L);
     SG_REQUIRE(a >= 0 && a < NCODE && b >= 0 && b < NCODE && c >= 0 && c < NCODE, "sg_amr_op: bad array code");
     double r = 0.0;
+    // relax / resid with flag & 4: the fused refresh of the mixed solve (where the hierarchy admits it)
+    struct FusedScope {
+        sg_amr &s;
+        FusedScope(sg_amr &s_, int on) : s(s_) { s.fused_on = on; }
+        ~FusedScope() { s.fused_on = 0; }
+    } fused_scope(s, (opc == SG_AMR_OP_RELAX || opc == SG_AMR_OP_RESID || opc == SG_AMR_OP_FMG_LIN) && (flag & 4) ? 1 : 0);
+    if (opc == SG_AMR_OP_RELAX || opc == SG_AMR_OP_RESID) flag &= 1;
     switch (opc) {
     case SG_AMR_OP_C2P: SG_TRY(c2p<double>(s, L, a, flag)); break;
     case SG_AMR_OP_GHOSTFIX: SG_TRY(ghostcell_fix<double>(s, L, a, flag & 1, (flag & 2) != 0)); break;

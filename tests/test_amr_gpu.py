@@ -227,7 +227,9 @@ def test_nonfinite_source_is_reported(pair, precision):
     from code_only_amd import _lib
     h, o, g = pair["periodic"]
     src = problems.amr_source(h, seed=13)
-    src[3, 2, 1, 0] = np.nan
+    # a leaf block: a covered block's source is overwritten by the restriction of its children
+    b = int(np.flatnonzero(h.leaf)[0])
+    src[b, 2, 1, 0] = np.nan
     psi, _, st = g.solve_host(src, tol=1e-10, maxit=5, precision=precision, raise_on_fail=False)
     print("nan", precision, st["status"], st["iterations"], st["rel_linf"], flush=True)
     assert st["status"] == _lib.SG_ERR_NONFINITE
@@ -328,3 +330,171 @@ def test_fused_refresh_full_sphere_hierarchy_launches(gpu):
     assert ss["iterations"] == sf["iterations"] and ss["hist"] == sf["hist"]
     assert np.array_equal(ps, pf) and np.array_equal(gs, gf)
     assert info["graph_nodes"] < n_staged
+
+
+def _zero_physical_ghosts(h, a, L):
+    """Zero the ghost layers on physical faces (what every refresh of the inner solve leaves)."""
+    n = 8 >> L
+    for b in range(h.nblk):
+        for f in range(6):
+            if h.nbr[b, f] >= 0:
+                continue
+            d, lr = f // 2, f % 2
+            sl = [slice(None)] * 3
+            sl[2 - d] = n + 1 if lr else 0   # arrays are [k][j][i]
+            a[(b,) + tuple(sl)] = 0.0
+
+
+@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic"])
+@pytest.mark.parametrize("L", [0, 1])
+@pytest.mark.parametrize("opname", ["relax", "resid"])
+def test_fused_operator_equals_staged(which, L, opname, gpu):
+    """One fmg_poisson_relax / fmg_poisson_resid with the fused refresh against the oracle's staged
+    one, from a random state whose physical ghosts are zero: the interiors of the leaf blocks (and
+    the residual) must agree bit for bit."""
+    h = {"nested-sphere": lambda: ah.nested(base=4, levels=3, bc=ah.SPHERE),
+         "nested-periodic": lambda: ah.nested(base=4, levels=3, bc=ah.PERIODIC),
+         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC)}[which]()
+    o, g = oracle.Amr(h), AmrSolver(h)
+    try:
+        if not (g.info()["fused_levels"] >> L) & 1:
+            pytest.skip("hierarchy does not admit the fused form at this level")
+        rng = np.random.default_rng(3 + L)
+        n = 8 >> L
+        for code in range(6):
+            a = rng.standard_normal((h.nblk, n + 2, n + 2, n + 2))
+            _zero_physical_ghosts(h, a, L)
+            o.set(L, code, a)
+            g.set(L, code, a)
+        if opname == "relax":
+            o.op(OPS.OP_RELAX, L, OPS.U, OPS.RHS, 0, 1)
+            g.op(OPS.OP_RELAX, L, OPS.U, OPS.RHS, 0, 1 | 4)
+            code = OPS.U
+        else:
+            o.op(OPS.OP_RESID, L, OPS.RES, OPS.U, OPS.RHS, 1)
+            g.op(OPS.OP_RESID, L, OPS.RES, OPS.U, OPS.RHS, 1 | 4)
+            code = OPS.RES
+        a, b = o.get(L, code)[:, 1:-1, 1:-1, 1:-1], g.get(L, code)[:, 1:-1, 1:-1, 1:-1]
+        leaf = np.asarray(h.leaf, bool)
+        bad = np.argwhere((a != b) & leaf[:, None, None, None])
+        if len(bad):
+            lines = []
+            for blk, k, j, i in bad[:12]:
+                kinds = ["-" if h.nbr[blk, f] < 0 else ("S" if h.level[h.nbr[blk, f]] == h.level[blk] else "P")
+                         for f in range(6)]
+                lines.append(f"blk {blk} lev {h.level[blk]} cell ({i},{j},{k}) faces {''.join(kinds)} "
+                             f"oracle {a[blk, k, j, i]:.6e} gpu {b[blk, k, j, i]:.6e}")
+            blocks = sorted(set(int(x) for x in bad[:, 0]))
+            raise AssertionError(f"{len(bad)} leaf cells differ in {len(blocks)} blocks {blocks[:20]}\n" + "\n".join(lines))
+    finally:
+        o.close()
+        g.close()
+
+
+@pytest.mark.parametrize("which", ["nested-periodic", "random-periodic"])
+def test_fused_fmg_lin_equals_staged(which, gpu):
+    """A whole fmg_lin with the fused refreshes at FMG levels 0 and 1 against the oracle's staged
+    fmg_lin, from a random state with zero physical ghosts: the leaf interiors of every code at every
+    FMG level agree bit for bit."""
+    h = {"nested-periodic": lambda: ah.nested(base=4, levels=3, bc=ah.PERIODIC),
+         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC)}[which]()
+    o, g = oracle.Amr(h), AmrSolver(h)
+    try:
+        rng = np.random.default_rng(11)
+        for L in range(3):
+            n = 8 >> L
+            for code in range(6):
+                a = rng.standard_normal((h.nblk, n + 2, n + 2, n + 2))
+                _zero_physical_ghosts(h, a, L)
+                o.set(L, code, a)
+                g.set(L, code, a)
+        o.fill0_level = -1
+        g.fill0_level = -1
+        ro = o.op(OPS.OP_FMG_LIN)
+        rg = g.op(OPS.OP_FMG_LIN, flag=4)
+        leaf = np.asarray(h.leaf, bool)
+        report = []
+        for L in range(3):
+            for code in range(6):
+                a = o.get(L, code)[:, 1:-1, 1:-1, 1:-1]
+                b = g.get(L, code)[:, 1:-1, 1:-1, 1:-1]
+                bad = np.argwhere((a != b) & leaf[:, None, None, None])
+                if len(bad):
+                    report.append(f"L{L} code {code}: {len(bad)} leaf cells in {len(set(bad[:, 0].tolist()))} blocks, "
+                                  f"max {np.abs(a - b)[leaf].max():.3e}")
+        print("\n".join(report), ro, rg, flush=True)
+        assert not report, "\n".join(report)
+        assert ro == rg
+    finally:
+        o.close()
+        g.close()
+
+
+SEQUENCES = {
+    "relax-relax-resid-L0": [("relax", 0), ("relax", 0), ("resid", 0)],
+    "relax-relax-resid-L1": [("relax", 1), ("relax", 1), ("resid", 1)],
+    "interp-relax-L0": [("interp", 0), ("relax", 0), ("relax", 0)],
+    "addint-relax-L1": [("addint", 1), ("relax", 1), ("relax", 1)],
+    "addint-relax-L0": [("addint", 0), ("relax", 0), ("relax", 0)],
+    "down-leg-L0": [("relax", 0), ("relax", 0), ("resid", 0), ("rstrct", 1), ("fill0U", 1), ("relax", 1),
+                    ("relax", 1), ("resid", 1)],
+    "down-vmg-up-L1": [("relax", 1), ("relax", 1), ("resid", 1), ("rstrct", 2), ("fill0U", 2), ("vmg", 2),
+                       ("addint", 1), ("relax", 1), ("relax", 1)],
+}
+
+
+@pytest.mark.parametrize("seq", sorted(SEQUENCES))
+def test_fused_sequence_equals_staged(seq, gpu):
+    """Short sequences of fmg_lin's operations, the relaxations and residuals with the fused refresh,
+    against the oracle's staged ones: leaf interiors of every array after every step."""
+    h = ah.nested(base=4, levels=3, bc=ah.PERIODIC)
+    o, g = oracle.Amr(h), AmrSolver(h)
+    try:
+        rng = np.random.default_rng(21)
+        for L in range(3):
+            n = 8 >> L
+            for code in range(6):
+                a = rng.standard_normal((h.nblk, n + 2, n + 2, n + 2))
+                o.set(L, code, a)
+                g.set(L, code, a)
+        leaf = np.asarray(h.leaf, bool)
+        for step, (what, L) in enumerate(SEQUENCES[seq]):
+            if what == "relax":
+                o.op(OPS.OP_RELAX, L, OPS.U, OPS.RHS, 0, 1)
+                g.op(OPS.OP_RELAX, L, OPS.U, OPS.RHS, 0, 1 | 4)
+            elif what == "resid":
+                o.op(OPS.OP_RESID, L, OPS.RES, OPS.U, OPS.RHS, 1)
+                g.op(OPS.OP_RESID, L, OPS.RES, OPS.U, OPS.RHS, 1 | 4)
+            elif what == "interp":
+                o.op(OPS.OP_INTERP, L, OPS.U, OPS.U, 0, 1)
+                g.op(OPS.OP_INTERP, L, OPS.U, OPS.U, 0, 1)
+            elif what == "addint":
+                o.op(OPS.OP_ADDINT, L, OPS.U, OPS.U, OPS.RES)
+                g.op(OPS.OP_ADDINT, L, OPS.U, OPS.U, OPS.RES)
+            elif what == "rstrct":
+                o.op(OPS.OP_RSTRCT, L, OPS.RHS, OPS.RES)
+                g.op(OPS.OP_RSTRCT, L, OPS.RHS, OPS.RES)
+            elif what == "fill0U":
+                # fmg_fill0 (whole arrays of the non-skipped blocks) and the FmgLevel_fill0 mark
+                a = o.get(L, OPS.U)
+                a[leaf] = 0.0
+                o.set(L, OPS.U, a)
+                g.set(L, OPS.U, a)
+                o.fill0_level = L
+                g.fill0_level = L
+            elif what == "vmg":
+                o.op(OPS.OP_VMG_FAS, L, OPS.RHS)
+                g.op(OPS.OP_VMG_FAS, L, OPS.RHS)
+            report = []
+            for LL in range(3):
+                for code in range(6):
+                    a = o.get(LL, code)[:, 1:-1, 1:-1, 1:-1]
+                    b = g.get(LL, code)[:, 1:-1, 1:-1, 1:-1]
+                    bad = np.argwhere((a != b) & leaf[:, None, None, None])
+                    if len(bad):
+                        report.append(f"L{LL} code {code}: {len(bad)} cells, blocks {sorted(set(bad[:, 0].tolist()))[:8]}, "
+                                      f"first {bad[0].tolist()}")
+            assert not report, f"{seq} step {step} ({what} L{L}):\n" + "\n".join(report)
+    finally:
+        o.close()
+        g.close()
