@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -61,10 +62,19 @@ struct sg_amr {
     int *d_cf = nullptr, *d_cf_slot = nullptr, *d_ovslot = nullptr, *d_cov = nullptr;
     // fused refresh (mixed solve): covered blocks' face layers read by same-level leaf neighbours
     // (entries b*6+f), the child table, and per FMG level whether the hierarchy admits the fused
-    // form (stage_check); fused_on is set for the duration of a mixed solve that asks for it
+    // form (build_prep_tables); fused_on is set for the duration of a mixed solve that asks for it
     std::vector<int> covf;
     int *d_covf = nullptr, *d_child = nullptr;
+    // k_prep's source tables per fused FMG level (build_prep_tables): covered face-layer cells
+    // {destination, restriction origin}, and per coarse-fine face the sources of the coarse
+    // neighbour's face layer; d_cfrec holds {fine block, face + 8 octant, override slot, level}
+    int *d_prep_cov[kNfmg] = {}, *d_prep_src[kNfmg] = {}, *d_cfrec = nullptr;
+    int prep_ncov[kNfmg] = {};
     int fused_ok[kNfmg] = {};
+    // per-block records (kRec ints, amr_kernels.h) of the relaxation / residual kernels, by block
+    // number (d_brec) and in the order of the active list (d_arec); cell widths per FMG level
+    int *d_brec = nullptr, *d_arec = nullptr;
+    LevelScalars hs[kNfmg] = {}, his[kNfmg] = {};
     int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
@@ -250,12 +260,12 @@ template <typename T> int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, r
 template <int N, typename T> int prep_n(sg_amr &s, int L, int ju)
 {
     constexpr int H = N / 2;
-    const int ncov = (int)s.covf.size(), ncf = (int)s.cf.size();
-    const long work = (long)ncov * N * N + (long)ncf * H * H;
-    return go(s, k_prep<N, T>, work, ar<T>(s).A[L][ju], ar<T>(s).ov[L], (const int *)s.d_covf, ncov,
-              (const int *)s.d_cf, (const int *)s.d_cf_slot, ncf, (const double *)s.d_hi[L], (const int *)s.d_level,
-              (const int *)s.d_octant, (const int *)s.d_nbr, (const int *)s.d_nkind, (const int *)s.d_child,
-              (const int *)s.d_covered);
+    const int ncov = s.prep_ncov[L], ncf = (int)s.cf.size();
+    const long work = (long)ncov + (long)ncf * H * H;
+    LevelScalars hil{};
+    for (int l = 0; l < kMaxLev; ++l) hil.v[l] = s.hil[L][l];
+    return go(s, k_prep<N, T>, work, ar<T>(s).A[L][ju], ar<T>(s).ov[L], (const int2 *)s.d_prep_cov[L], ncov,
+              (const int4 *)s.d_cfrec, (const int *)s.d_prep_src[L], ncf, hil);
 }
 
 inline bool fused(const sg_amr &s, int L) { return s.fused_on && L < kNfmg - 1 && s.fused_ok[L]; }
@@ -276,21 +286,20 @@ template <typename T> int poisson_flux(sg_amr &s, int L, int ju, int fill0flag)
     return reflux<T>(s, L, ju);
 }
 
+// one colour pass over nrec blocks given by their records (rec), with the reflux overrides or not
 template <int N, typename T>
-int relax_pass_n(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int nb, int ipass, const int *ovslot,
+int relax_pass_n(sg_amr &s, int L, int ju, int jrhs, const int *rec, int nrec, int ipass, bool with_ov,
                  double *resmax, bool halo_free)
 {
-    long work = (long)nb * N * N * ((N + 1) / 2);
-    return go(s, k_relax<N, T>, work, ar<T>(s).A[L][ju], (const T *)ar<T>(s).A[L][jrhs], list, b0, nb, ipass,
-              (const double *)s.d_h[L], (const double *)s.d_hi[L], (const int *)s.d_level, ovslot,
-              (const T *)ar<T>(s).ov[L], resmax, halo_free ? (const int *)s.d_nbr : (const int *)nullptr,
-              halo_free ? (const int *)s.d_nkind : (const int *)nullptr);
+    long work = (long)nrec * N * N * ((N + 1) / 2);
+    return go(s, k_relax<N, T>, work, ar<T>(s).A[L][ju], (const T *)ar<T>(s).A[L][jrhs], rec, nrec, ipass, s.hs[L],
+              s.his[L], with_ov ? (const T *)ar<T>(s).ov[L] : (const T *)nullptr, resmax, halo_free ? 1 : 0);
 }
 template <typename T>
-int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *list, int b0, int nb, int ipass, const int *ovslot,
+int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *rec, int nrec, int ipass, bool with_ov,
                double *resmax, bool halo_free = false)
 {
-    SG_NDISPATCH(L, relax_pass_n, s, L, ju, jrhs, list, b0, nb, ipass, ovslot, resmax, halo_free);
+    SG_NDISPATCH(L, relax_pass_n, s, L, ju, jrhs, rec, nrec, ipass, with_ov, resmax, halo_free);
 }
 
 // fmg_poisson_relax (fmg_poisson.F90:130-213).  With want_max, Resh2maxg lands in d_scal[0]; the
@@ -318,14 +327,11 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
         const bool hf = fused(s, L);
         if (L == 0) {
             hipLaunchKernelGGL((k_blk8<true, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, (T *)nullptr,
-                               ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_active, ipass,
-                               (const double *)s.d_h[0], (const double *)s.d_hi[0], (const int *)s.d_level,
-                               (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], slots,
-                               hf ? (const int *)s.d_nbr : (const int *)nullptr,
-                               hf ? (const int *)s.d_nkind : (const int *)nullptr);
+                               ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_arec, ipass,
+                               s.hs[0], s.his[0], (const T *)ar<T>(s).ov[0], slots, hf ? 1 : 0);
             SG_CHECK_LAUNCH();
         } else {
-            SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_active, 0, (int)s.active.size(), ipass, s.d_ovslot, slots, hf));
+            SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_arec, (int)s.active.size(), ipass, true, slots, hf));
         }
     }
     if (want_max) {
@@ -340,9 +346,8 @@ template <int N, typename T> int resid_n(sg_amr &s, int L, int jres, int ju, int
     int na = (int)s.active.size();
     const bool hf = fused(s, L);
     return go(s, k_resid<N, T>, (long)na * N * N * N, ar<T>(s).A[L][jres], (const T *)ar<T>(s).A[L][ju],
-              (const T *)ar<T>(s).A[L][jrhs], (const int *)s.d_active, na, (const double *)s.d_hi[L],
-              (const int *)s.d_level, (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[L],
-              hf ? (const int *)s.d_nbr : (const int *)nullptr, hf ? (const int *)s.d_nkind : (const int *)nullptr);
+              (const T *)ar<T>(s).A[L][jrhs], (const int *)s.d_arec, na, s.his[L], (const T *)ar<T>(s).ov[L],
+              hf ? 1 : 0);
 }
 
 template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int jrhs, int fill0flag)
@@ -351,10 +356,8 @@ template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int 
     if (L == 0) {
         const bool hf = fused(s, 0);
         hipLaunchKernelGGL((k_blk8<false, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, ar<T>(s).A[0][jres],
-                           ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_active, 0,
-                           (const double *)s.d_h[0], (const double *)s.d_hi[0], (const int *)s.d_level,
-                           (const int *)s.d_ovslot, (const T *)ar<T>(s).ov[0], (double *)nullptr,
-                           hf ? (const int *)s.d_nbr : (const int *)nullptr, hf ? (const int *)s.d_nkind : (const int *)nullptr);
+                           ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_arec, 0, s.hs[0],
+                           s.his[0], (const T *)ar<T>(s).ov[0], (double *)nullptr, hf ? 1 : 0);
         SG_CHECK_LAUNCH();
         return SG_OK;
     }
@@ -477,7 +480,7 @@ template <typename T> int vmg_relax(sg_amr &s, int l, int ju, int jrhs, bool las
     int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
     for (int ipass = 1; ipass <= 2; ++ipass) {
         if (!kInner<T> || (last && ipass == 2)) SG_TRY(vmg_ghostcell<T>(s, l, ju));
-        SG_TRY(relax_pass<T>(s, LV, ju, jrhs, nullptr, b0, nb, ipass, nullptr, nullptr, kInner<T>));
+        SG_TRY(relax_pass<T>(s, LV, ju, jrhs, s.d_brec + (long)kRec * b0, nb, ipass, false, nullptr, kInner<T>));
     }
     return SG_OK;
 }
@@ -934,10 +937,16 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
     return status;
 }
 
-// Host mirror of the control flow of Staged<N, T>::S<D> (amr_kernels.h): false when evaluating the
-// cell (i, j, k) of block b would meet a coarse-fine face, or read a covered cell whose child in
-// that octant is covered itself (the fused refresh restricts leaf children only).
-bool staged_reach(const sg_amr &s, int N, int b, int i, int j, int k)
+// Host mirror of the staged refresh's rule (fmg_converge_c2p, then fmg_ghostcell_fix levels coarse
+// to fine, x, y, z): the stored cell the value a refresh leaves in cell (i, j, k) of block b comes
+// from, at an FMG level with N cells per block side, encoded as a k_prep source (amr_kernels.h).
+// The last direction pass writing a ghost cell is the highest direction among its ghost
+// coordinates whose face is not physical; a same-level pass copies the neighbour's layer as the
+// earlier passes left it, so the cell is followed through the neighbours from z down to x.  False
+// where the fused refresh cannot reproduce the value: a coarse-fine face on the way (the value
+// would come from fmg_ghfix_parentlev), or a covered cell whose child there is covered itself (the
+// restriction of a restriction).
+bool prep_source(const sg_amr &s, int N, int b, int i, int j, int k, int &src)
 {
     int p[3] = {i, j, k};
     for (int D = 2; D >= 0; --D) {
@@ -950,34 +959,33 @@ bool staged_reach(const sg_amr &s, int N, int b, int i, int j, int k)
             }
         }
     }
+    const long W = N + 2, LEN = W * W * W;
+    auto gi = [W](int a, int bb, int c) { return ((long)(c + 1) * W + (bb + 1)) * W + (a + 1); };
     const bool interior = p[0] >= 0 && p[0] < N && p[1] >= 0 && p[1] < N && p[2] >= 0 && p[2] < N;
     if (interior && s.covered[b]) {
         const int H = N / 2, o = (p[0] >= H ? 1 : 0) + (p[1] >= H ? 2 : 0) + (p[2] >= H ? 4 : 0);
-        if (s.covered[s.child[8 * b + o]]) return false;
+        const int c = s.child[8 * b + o];
+        if (s.covered[c]) return false;
+        const long org = (long)c * LEN + gi(2 * (p[0] - (o & 1) * H), 2 * (p[1] - ((o >> 1) & 1) * H),
+                                            2 * (p[2] - ((o >> 2) & 1) * H));
+        src = -(int)org - 1;
+        return true;
     }
+    src = (int)((long)b * LEN + gi(p[0], p[1], p[2]));
     return true;
 }
 
-// Whether the fused refresh reproduces the staged one at an FMG level with N cells per block side:
-// every coarse value a coarse-fine ghost needs, and every covered face cell a colour pass reads,
-// must be reachable without a coarse-fine face and with leaf children (staged_reach).
-bool stage_check(const sg_amr &s, int N)
+// k_prep's tables at FMG level L; false (the solves then keep the staged refresh there) where some
+// value a colour pass or residual reads cannot be reproduced: every coarse value a coarse-fine ghost
+// interpolates from, and every covered face-layer cell a same-level leaf reads.
+bool build_prep_tables(sg_amr &s, int L, std::vector<int> &cov, std::vector<int> &tab)
 {
-    const int H = N / 2;
-    for (size_t e = 0; e < s.cf.size(); ++e) {
-        const int b = s.cf[e] / 6, f = s.cf[e] % 6, d = f / 2, lr = f & 1;
-        const int m = s.nbr[6 * b + f], o = s.octant[b];
-        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-        const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
-        for (int x = -1; x <= H; ++x)
-            for (int y = -1; y <= H; ++y) {
-                int p[3];
-                p[d] = lr ? 0 : N - 1;
-                p[ta] = x + e1;
-                p[tb] = y + e2;
-                if (!staged_reach(s, N, m, p[0], p[1], p[2])) return false;
-            }
-    }
+    const int N = 8 >> L, H = N / 2;
+    const long W = N + 2, LEN = W * W * W;
+    if ((long)s.nblk * LEN >= (long)INT32_MAX) return false;   // sources are 32-bit offsets
+    auto gi = [W](int a, int bb, int c) { return ((long)(c + 1) * W + (bb + 1)) * W + (a + 1); };
+    cov.clear();
+    tab.clear();
     for (int e : s.covf) {
         const int b = e / 6, f = e % 6, d = f / 2, a = (f & 1) ? N - 1 : 0;
         for (int t2 = 0; t2 < N; ++t2)
@@ -986,7 +994,26 @@ bool stage_check(const sg_amr &s, int N)
                 p[d] = a;
                 p[d == 0 ? 1 : 0] = t1;
                 p[d == 2 ? 1 : 2] = t2;
-                if (!staged_reach(s, N, b, p[0], p[1], p[2])) return false;
+                int src = 0;
+                if (!prep_source(s, N, b, p[0], p[1], p[2], src) || src >= 0) return false;
+                cov.push_back((int)((long)b * LEN + gi(p[0], p[1], p[2])));
+                cov.push_back(-src - 1);
+            }
+    }
+    for (size_t e = 0; e < s.cf.size(); ++e) {
+        const int b = s.cf[e] / 6, f = s.cf[e] % 6, d = f / 2, lr = f & 1;
+        const int m = s.nbr[6 * b + f], o = s.octant[b];
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
+        for (int y = -1; y <= H; ++y)
+            for (int x = -1; x <= H; ++x) {
+                int p[3];
+                p[d] = lr ? 0 : N - 1;
+                p[ta] = x + e1;
+                p[tb] = y + e2;
+                int src = 0;
+                if (!prep_source(s, N, m, p[0], p[1], p[2], src)) return false;
+                tab.push_back(src);
             }
     }
     return true;
@@ -1005,8 +1032,12 @@ void release(sg_amr *s)
     if (!s) return;
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
                  s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters,
-                 s->d_covf, s->d_child};
+                 s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec};
     for (int *q : ip) if (q) (void)hipFree(q);
+    for (int L = 0; L < kNfmg; ++L) {
+        if (s->d_prep_cov[L]) (void)hipFree(s->d_prep_cov[L]);
+        if (s->d_prep_src[L]) (void)hipFree(s->d_prep_src[L]);
+    }
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
     if (s->bd_mem) (void)hipFree(s->bd_mem);
     if (s->bd_grid) (void)hipFree(s->bd_grid);
@@ -1131,7 +1162,8 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
         for (int f = 0; f < 6; ++f)
             if (s->nkind[6 * b + f] == NBR_SAME && !s->covered[nbr[6 * b + f]]) s->covf.push_back(6 * b + f);
     }
-    for (int L = 0; L < kNfmg - 1; ++L) s->fused_ok[L] = stage_check(*s, 8 >> L) ? 1 : 0;
+    std::vector<int> prep_cov[kNfmg], prep_src[kNfmg];
+    for (int L = 0; L < kNfmg - 1; ++L) s->fused_ok[L] = build_prep_tables(*s, L, prep_cov[L], prep_src[L]) ? 1 : 0;
     // device tables
     SG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     SG_HIP(hipEventCreate(&s->ev0));
@@ -1152,12 +1184,45 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     SG_TRY(upload(*s, &s->d_cov, s->cov));
     SG_TRY(upload(*s, &s->d_covf, s->covf));
     SG_TRY(upload(*s, &s->d_child, s->child));
+    {
+        std::vector<int> brec((size_t)kRec * nblk, 0), arec;
+        for (int b = 0; b < nblk; ++b) {
+            int *r = brec.data() + (size_t)kRec * b;
+            r[0] = b;
+            r[1] = level[b];
+            for (int f = 0; f < 6; ++f) {
+                r[2 + f] = s->nkind[6 * b + f] == NBR_SAME ? nbr[6 * b + f] : -1;
+                r[8 + f] = s->ovslot[6 * b + f];
+            }
+        }
+        for (int b : s->active) arec.insert(arec.end(), brec.begin() + (size_t)kRec * b, brec.begin() + (size_t)kRec * (b + 1));
+        SG_TRY(upload(*s, &s->d_brec, brec));
+        SG_TRY(upload(*s, &s->d_arec, arec));
+    }
+    {
+        std::vector<int> rec;
+        for (size_t e = 0; e < s->cf.size(); ++e) {
+            const int b = s->cf[e] / 6, f = s->cf[e] % 6;
+            rec.insert(rec.end(), {b, f + 8 * s->octant[b], s->cf_slot[e], s->level[b]});
+        }
+        SG_TRY(upload(*s, &s->d_cfrec, rec));
+        for (int L = 0; L < kNfmg - 1; ++L) {
+            if (!s->fused_ok[L]) continue;
+            s->prep_ncov[L] = (int)prep_cov[L].size() / 2;
+            SG_TRY(upload(*s, &s->d_prep_cov[L], prep_cov[L]));
+            SG_TRY(upload(*s, &s->d_prep_src[L], prep_src[L]));
+        }
+    }
     for (int L = 0; L < kNfmg; ++L) {
         for (int c = 0; c < NCODE; ++c) SG_HIP(hipMalloc((void **)&s->d.A[L][c], sizeof(double) * nblk * lenof(L)));
         SG_HIP(hipMalloc((void **)&s->d.ov[L], sizeof(double) * std::max(1, s->nslots) * nof(L) * nof(L)));
         for (int l = 0; l < nlev; ++l) {
             s->hl[L][l] = std::ldexp(h0, L - l);          // fmg_data.F90:288 CellWidth * 2^fmglev
             s->hil[L][l] = 1.0 / s->hl[L][l];
+        }
+        for (int l = 0; l < kMaxLev; ++l) {
+            s->hs[L].v[l] = s->hl[L][l];
+            s->his[L].v[l] = s->hil[L][l];
         }
         std::vector<double> hv(s->hl[L], s->hl[L] + kMaxLev), hiv(s->hil[L], s->hil[L] + kMaxLev);
         SG_TRY(upload(*s, &s->d_h[L], hv));

@@ -63,6 +63,7 @@ template <typename T> __device__ __forceinline__ T comb4(const double a[4], T v0
 }
 
 struct Cubic64 { double w[64]; };   // St(a, b, c) of restriction.F90:38-63 at w[(c*4 + b)*4 + a]
+struct LevelScalars { double v[16]; };   // one value per AMR level, passed by value
 
 // restriction.F90:127-151
 template <int N, typename T> __device__ __forceinline__ T rstr_lin(const T *uf, int i, int j, int k)
@@ -340,103 +341,79 @@ __global__ void k_ghost_level(T *A, int b0, int nb, int cubic, const int *nbr, c
 
 // ---- fused refresh of the mixed solve (fmg_poisson_flux without its staged launches) ----
 // The values a refresh (fmg_converge_c2p, then the staged fmg_ghostcell_fix) leaves in the cells
-// that a colour pass or a residual reads, evaluated directly from the blocks' current interiors:
-// a covered block's interior cell as the linear restriction of its child's cells (the child must
-// be a leaf), a ghost cell by the staged passes' rule (the value of the last direction pass that
-// writes it: a same-level copy of the neighbour's value after the earlier passes, or, on a
-// physical face, the stored value).  Valid when none of the cells needed meets a coarse-fine face
-// in that recursion and every covered cell needed has a leaf child; amr.hip checks both per
-// hierarchy on the host (stage_check) before using it.
-template <int N, typename T> struct Staged {
-    using B = Blk<N>;
-    static constexpr int H = N / 2;
-    const T *A;
-    const int *nbr, *nkind, *child, *covered;
+// that a colour pass or a residual reads, evaluated directly from the blocks' current interiors.
+// Which stored cell each such value comes from depends on the hierarchy only, so the host resolves
+// it once per context (amr.hip, build_prep_tables) into "sources": s >= 0 reads A[s] (a leaf's
+// interior cell, or a ghost cell on a physical face, which no refresh pass writes); s < 0 is a
+// covered block's interior cell, the linear restriction (restriction.F90:127-151) of its leaf
+// child's 2x2x2 cells whose first cell sits at A[-s - 1].
+template <int N, typename T> __device__ __forceinline__ T rstr_at(const T *A, long o)
+{
+    constexpr long W = N + 2, W2 = W * W;
+    const T *u = A + o;
+    T s = u[0] + u[1];
+    s = s + u[W];
+    s = s + u[W2];
+    s = s + u[1 + W];
+    s = s + u[1 + W2];
+    s = s + u[W + W2];
+    s = s + u[1 + W + W2];
+    return s * (T)(1.0 / 8.0);
+}
 
-    __device__ T cell(int b, int i, int j, int k) const
-    {
-        if (!covered[b]) return A[(long)b * B::LEN + B::gi(i, j, k)];
-        const int o = (i >= H ? 1 : 0) + (j >= H ? 2 : 0) + (k >= H ? 4 : 0);
-        const T *uf = A + (long)child[8 * b + o] * B::LEN;
-        return rstr_lin<N, T>(uf, 2 * (i - (o & 1) * H), 2 * (j - ((o >> 1) & 1) * H), 2 * (k - ((o >> 2) & 1) * H));
-    }
+template <int N, typename T> __device__ __forceinline__ T src_val(const T *A, int s)
+{
+    return s >= 0 ? A[s] : rstr_at<N, T>(A, -(long)s - 1);
+}
 
-    template <int D> __device__ T S(int b, int i, int j, int k) const
-    {
-        if constexpr (D < 0) {
-            if (i >= 0 && i < N && j >= 0 && j < N && k >= 0 && k < N) return cell(b, i, j, k);
-            return A[(long)b * B::LEN + B::gi(i, j, k)];
-        } else {
-            const int c = D == 0 ? i : (D == 1 ? j : k);
-            if (c == -1 || c == N) {
-                const int f = b * 6 + 2 * D + (c == N ? 1 : 0);
-                if (nkind[f] == NBR_SAME) {
-                    const int m = nbr[f], layer = c == N ? 0 : N - 1;
-                    if (D == 0) return S<D - 1>(m, layer, j, k);
-                    if (D == 1) return S<D - 1>(m, i, layer, k);
-                    return S<D - 1>(m, i, j, layer);
-                }
-                // coarse-fine faces are excluded by the host check; physical faces keep the value
-            }
-            return S<D - 1>(b, i, j, k);
-        }
-    }
-};
 
-// One launch per refresh: work items [0, ncov * N^2) write the listed covered blocks' face layers
-// (entries b*6+f: the layer next to face f, whose neighbour is a same-level leaf that a colour pass
-// reads it from); items [ncov N^2, + ncf H^2) each take one coarse face cell of a coarse-fine face:
-// the four fine ghost cells behind it (fmg_ghfix_parentlev: bilinear (9,3,3,1)/16 tangentially,
-// then (10 u0 + 8 uH - 3 uN)/15 normally, the operand order of k_ghost) and the reflux override of
-// the coarse neighbour there (the 2x2 average of the fine face fluxes, the order of k_reflux).
+// One launch per refresh.  Items [0, ncov): the covered blocks' face layers that same-level leaf
+// neighbours read (cov[t] = {destination, restriction origin}).  Items [ncov, + ncf H^2): one coarse
+// face cell (c1, c2) of coarse-fine face e each: the four fine ghost cells behind it
+// (fmg_ghfix_parentlev: bilinear (9,3,3,1)/16 tangentially, then (10 u0 + 8 uH - 3 uN)/15 normally,
+// the operand order of k_ghost) and the reflux override of the coarse neighbour there (the 2x2
+// average of the fine face fluxes, the order of k_reflux).  rec[e] = {fine block, face + 8 octant,
+// override slot, AMR level}; src + e (H+2)^2 holds the sources of the coarse neighbour's face layer
+// at tangential coarse positions -1..H (the fine block's quarter of it, ghosts included).
 // Reads only leaf interiors and physical ghost cells, which no item writes.
 template <int N, typename T>
-__global__ void k_prep(T *A, T *ov, const int *covf, int ncov, const int *cf, const int *cf_slot, int ncf,
-                       const double *hi_lev, const int *level, const int *octant, const int *nbr, const int *nkind,
-                       const int *child, const int *covered)
+__global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, const int *src, int ncf,
+                       LevelScalars hil)
 {
     using B = Blk<N>;
-    constexpr int H = N / 2;
-    const Staged<N, T> St{A, nbr, nkind, child, covered};
+    constexpr int H = N / 2, S = H + 2;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n1 = (long)ncov * N * N;
-    if (t < n1) {
-        const int e = covf[t / (N * N)], t1 = (int)(t % N), t2 = (int)((t / N) % N);
-        const int b = e / 6, f = e % 6, d = f >> 1, a = (f & 1) ? N - 1 : 0;
-        int i, j, k;
-        if (d == 0) { i = a; j = t1; k = t2; } else if (d == 1) { i = t1; j = a; k = t2; } else { i = t1; j = t2; k = a; }
-        A[(long)b * B::LEN + B::gi(i, j, k)] = St.cell(b, i, j, k);
+    if (t < ncov) {
+        const int2 p = cov[t];
+        A[p.x] = rstr_at<N, T>(A, p.y);
         return;
     }
-    t -= n1;
+    t -= ncov;
     if (t >= (long)ncf * H * H) return;
     const int c1 = (int)(t % H), c2 = (int)((t / H) % H), e = (int)(t / (H * H));
-    const int b = cf[e] / 6, f = cf[e] % 6, d = f >> 1, lr = f & 1;
-    const int m = nbr[6 * b + f];
-    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-    const int o = octant[b];
-    const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
-    const int an = lr ? 0 : N - 1, ag = lr ? N : -1, inner = lr ? -1 : 1;
-    auto buf = [&](int x, int y) -> T {
-        int p[3];
-        p[d] = an;
-        p[ta] = x + e1;
-        p[tb] = y + e2;
-        return St.template S<2>(m, p[0], p[1], p[2]);
-    };
+    const int *sr = src + (long)e * S * S;
+    // the 3x3 coarse neighbourhood the four fine ghosts interpolate from
+    T w[3][3];
+#pragma unroll
+    for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) w[y][x] = src_val<N, T>(A, sr[(c2 + y) * S + (c1 + x)]);
+    const int4 r = rec[e];
+    const int b = r.x, f = r.y & 7, o = r.y >> 3, d = f >> 1, lr = f & 1;
+    const int ag = lr ? N : -1, inner = lr ? -1 : 1;
     T *u = A + (long)b * B::LEN;
-    const T hi = (T)hi_lev[level[b]];
+    const T hi = (T)hil.v[r.w];
     T ffv[2][2];
 #pragma unroll
     for (int q2 = 0; q2 < 2; ++q2)
 #pragma unroll
         for (int q1 = 0; q1 < 2; ++q1) {
             const int s1 = 2 * c1 + q1, s2 = 2 * c2 + q2;
-            const int cc1 = ijkc(s1), cc2 = ijkc(s2);
-            const int c1n = near_(s1, cc1), c2n = near_(s2, cc2);
-            T Sv = (T)9.0 * buf(cc1, cc2) + buf(c1n, c2n);
-            Sv = Sv + (T)3.0 * buf(cc1, c2n);
-            Sv = Sv + (T)3.0 * buf(c1n, cc2);
+            // buf(cc, cc) is w[1][1]; the near neighbour is one coarse cell down (q = 0) or up (q = 1)
+            const int n1 = 2 * q1, n2 = 2 * q2;
+            T Sv = (T)9.0 * w[1][1] + w[n2][n1];
+            Sv = Sv + (T)3.0 * w[n2][1];
+            Sv = Sv + (T)3.0 * w[1][n1];
             const T uH = Sv / (T)16.0;
             const T u0 = u[B::rot(d, ag + inner, s1, s2)];
             const T uN = u[B::rot(d, ag + 2 * inner, s1, s2)];
@@ -450,9 +427,9 @@ __global__ void k_prep(T *A, T *ov, const int *covf, int ncov, const int *cf, co
     T v = ffv[0][0] + ffv[0][1];
     v = v + ffv[1][0];
     v = v + ffv[1][1];
-    const int ta2 = d == 0 ? 1 : 0, tb2 = d == 2 ? 1 : 2;
-    const int off1 = ((o >> ta2) & 1) * H, off2 = ((o >> tb2) & 1) * H;
-    ov[(long)cf_slot[e] * N * N + (off1 + c1) + N * (off2 + c2)] = v * (T)0.25;
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const int off1 = ((o >> ta) & 1) * H, off2 = ((o >> tb) & 1) * H;
+    ov[(long)r.z * N * N + (off1 + c1) + N * (off2 + c2)] = v * (T)0.25;
 }
 
 // ---- fmg_boundary_fill0 on a list of physical faces (entries b*6+f) ----
@@ -580,27 +557,93 @@ __global__ void k_max_slots(const double *slots, double *out)
     if (threadIdx.x == 0) *out = v;
 }
 
-// ---- fmg_poisson_relax / vmg_poisson_relax: one colour pass over the listed blocks ----
+// ---- per-block records of the relaxation and residual kernels ----
+// 16 ints per block: block number, AMR level, the same-level neighbour across each face (-1 where
+// the face meets a coarser level or the domain boundary), the reflux override slot of each face
+// (-1 where there is none), two unused.  One 64-byte load gives a workgroup or thread everything
+// it needs before its data loads; the cell widths come in by value (LevelScalars).
+constexpr int kRec = 16;
+struct BlockRec {
+    int b, l, nb[6], slot[6];
+    __device__ void load(const int *rec, long a)
+    {
+        const int4 *p = (const int4 *)(rec + kRec * a);
+        const int4 r0 = p[0], r1 = p[1], r2 = p[2], r3 = p[3];
+        b = r0.x; l = r0.y;
+        nb[0] = r0.z; nb[1] = r0.w; nb[2] = r1.x; nb[3] = r1.y; nb[4] = r1.z; nb[5] = r1.w;
+        slot[0] = r2.x; slot[1] = r2.y; slot[2] = r2.z; slot[3] = r2.w; slot[4] = r3.x; slot[5] = r3.y;
+    }
+};
+
+// Face-flux view of one block from its record: fluxes from the block's values, or the reflux
+// override on faces that have one (when ov is set).  With A set ("halo-free" form), same-level
+// faces read the neighbour block's boundary cell directly instead of the block's own ghost cell:
+// the value a same-level ghost copy (fmg_ghfix_samelev) would place there.
+template <int N, typename T> struct FluxR {
+    const T *u;
+    const T *A;
+    const T *ov;
+    T hi;
+    const BlockRec &r;
+    __device__ T ov_or(int f, int t1, int t2, T computed) const
+    {
+        if (ov && r.slot[f] >= 0) return ov[(long)r.slot[f] * N * N + t1 + N * t2];
+        return computed;
+    }
+    __device__ T across(int f, int t1, int t2) const
+    {
+        using B = Blk<N>;
+        const int d = f >> 1, lr = f & 1;
+        if (A && r.nb[f] >= 0) return A[(long)r.nb[f] * B::LEN + B::rot(d, lr ? 0 : N - 1, t1, t2)];
+        return u[B::rot(d, lr ? N : -1, t1, t2)];
+    }
+    // sum of flux differences at (i, j, k) in the reference's operand order
+    __device__ T div(int i, int j, int k) const
+    {
+        using B = Blk<N>;
+        const T uc = u[B::gi(i, j, k)];
+        const T xp = (i == N - 1) ? across(1, j, k) : u[B::gi(i + 1, j, k)];
+        const T xm = (i == 0) ? across(0, j, k) : u[B::gi(i - 1, j, k)];
+        const T yp = (j == N - 1) ? across(3, i, k) : u[B::gi(i, j + 1, k)];
+        const T ym = (j == 0) ? across(2, i, k) : u[B::gi(i, j - 1, k)];
+        const T zp = (k == N - 1) ? across(5, i, j) : u[B::gi(i, j, k + 1)];
+        const T zm = (k == 0) ? across(4, i, j) : u[B::gi(i, j, k - 1)];
+        T fxp = (i == N - 1) ? ov_or(1, j, k, (xp - uc) * hi) : (xp - uc) * hi;
+        T fxm = (i == 0) ? ov_or(0, j, k, (uc - xm) * hi) : (uc - xm) * hi;
+        T fyp = (j == N - 1) ? ov_or(3, i, k, (yp - uc) * hi) : (yp - uc) * hi;
+        T fym = (j == 0) ? ov_or(2, i, k, (uc - ym) * hi) : (uc - ym) * hi;
+        T fzp = (k == N - 1) ? ov_or(5, i, j, (zp - uc) * hi) : (zp - uc) * hi;
+        T fzm = (k == 0) ? ov_or(4, i, j, (uc - zm) * hi) : (uc - zm) * hi;
+        T v = fxp - fxm;
+        v = v + fyp;
+        v = v - fym;
+        v = v + fzp;
+        v = v - fzm;
+        return v;
+    }
+};
+
+// ---- fmg_poisson_relax / vmg_poisson_relax: one colour pass over the blocks of nrec records ----
+// (ov: the reflux overrides, or nullptr in the VMG, which has none; halo: read same-level
+// neighbours directly)
 template <int N, typename T>
-__global__ void k_relax(T *Au, const T *Arhs, const int *list, int b0, int nb, int ipass,
-                        const double *h_lev, const double *hi_lev, const int *level, const int *ovslot,
-                        const T *ov, double *resmax, const int *nbr = nullptr, const int *nkind = nullptr)
+__global__ void k_relax(T *Au, const T *Arhs, const int *rec, int nrec, int ipass, LevelScalars h_lev,
+                        LevelScalars hi_lev, const T *ov, double *resmax, int halo)
 {
     using B = Blk<N>;
     constexpr int HN = (N + 1) / 2, PER = N * N * HN;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double a = 0.0;
-    if (t < (long)nb * PER) {
+    if (t < (long)nrec * PER) {
         int ih = (int)(t % HN), j = (int)((t / HN) % N), k = (int)((t / (HN * N)) % N);
-        int b = pick_block(list, b0, (int)(t / PER));
+        BlockRec R;
+        R.load(rec, t / PER);
         int i = 2 * ih + ((j + k + ipass) & 1);
         if (i < N) {
-            const int l = level[b];
-            const T h = (T)h_lev[l];
-            T *u = Au + (long)b * B::LEN;
-            Flux<N, T> F{u, ovslot ? ovslot + 6 * b : nullptr, ov, (T)hi_lev[l]};
-            if (nbr) { F.A = Au; F.nb = nbr + 6 * b; F.kind = nkind + 6 * b; }
-            T r = F.div(i, j, k) - Arhs[(long)b * B::LEN + B::gi(i, j, k)] * h;
+            const T h = (T)h_lev.v[R.l];
+            T *u = Au + (long)R.b * B::LEN;
+            FluxR<N, T> F{u, halo ? Au : nullptr, ov, (T)hi_lev.v[R.l], R};
+            T r = F.div(i, j, k) - Arhs[(long)R.b * B::LEN + B::gi(i, j, k)] * h;
             T resh2 = r * h;
             u[B::gi(i, j, k)] = u[B::gi(i, j, k)] + (T)(1.0 / 6.0) * resh2;
             a = (double)fabs(resh2);
@@ -609,23 +652,22 @@ __global__ void k_relax(T *Au, const T *Arhs, const int *list, int b0, int nb, i
     if (resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
 }
 
-// ---- fmg_poisson_resid: res = rhs - div F / h on the listed blocks ----
+// ---- fmg_poisson_resid: res = rhs - div F / h on the blocks of nrec records ----
 template <int N, typename T>
-__global__ void k_resid(T *Ares, const T *Au, const T *Arhs, const int *list, int nb,
-                        const double *hi_lev, const int *level, const int *ovslot, const T *ov,
-                        const int *nbr = nullptr, const int *nkind = nullptr)
+__global__ void k_resid(T *Ares, const T *Au, const T *Arhs, const int *rec, int nrec, LevelScalars hi_lev,
+                        const T *ov, int halo)
 {
     using B = Blk<N>;
     constexpr int PER = N * N * N;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nb * PER) return;
+    if (t >= (long)nrec * PER) return;
     int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
-    int b = list[t / PER];
-    const int l = level[b];
-    Flux<N, T> F{Au + (long)b * B::LEN, ovslot + 6 * b, ov, (T)hi_lev[l]};
-    if (nbr) { F.A = Au; F.nb = nbr + 6 * b; F.kind = nkind + 6 * b; }
-    const long q = (long)b * B::LEN + B::gi(i, j, k);
-    Ares[q] = Arhs[q] - F.div(i, j, k) * (T)hi_lev[l];
+    BlockRec R;
+    R.load(rec, t / PER);
+    const T hi = (T)hi_lev.v[R.l];
+    FluxR<N, T> F{Au + (long)R.b * B::LEN, halo ? Au : nullptr, ov, hi, R};
+    const long q = (long)R.b * B::LEN + B::gi(i, j, k);
+    Ares[q] = Arhs[q] - F.div(i, j, k) * hi;
 }
 
 // ---- fmg_rstrct (linear): coarse level with NC cells from the fine level with 2 NC ----
@@ -878,50 +920,59 @@ __global__ void k_norms_final(const double *part, int G, double *out)
     if (threadIdx.x == 0) { out[0] = m2; out[1] = m; out[2] = s; }
 }
 
-// ---- 8^3 blocks, one workgroup (256 threads) per block: the block's ghosted array is staged in
+// ---- 8^3 blocks, one workgroup (256 threads) per record: the block's ghosted array is staged in
 //      LDS with coalesced loads, then one colour of fmg_poisson_relax / the residual is formed from
-//      LDS (same operand order as k_relax / k_resid) ----
+//      LDS (same operand order as k_relax / k_resid).  The record load is the only load the data
+//      loads wait for; the right-hand side is fetched before the barrier. ----
 template <bool RELAX, typename T>
-__global__ void __launch_bounds__(256) k_blk8(T *Aout, T *Au, const T *Arhs, const int *list, int ipass,
-                                             const double *h_lev, const double *hi_lev, const int *level,
-                                             const int *ovslot, const T *ov, double *resmax,
-                                             const int *nbr = nullptr, const int *nkind = nullptr)
+__global__ void __launch_bounds__(256) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
+                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
+                                             int halo)
 {
     using B = Blk<8>;
     __shared__ T su[B::LEN];
-    const int b = list[blockIdx.x];
-    const T *ug = Au + (long)b * B::LEN;
+    BlockRec R;
+    R.load(rec, blockIdx.x);
+    const T *ug = Au + (long)R.b * B::LEN;
     for (int q = threadIdx.x; q < B::LEN; q += 256) su[q] = ug[q];
-    if (nbr) {
+    // this thread's cells: one of the colour (relax) or two (residual)
+    const int ih = threadIdx.x & 3, j = (threadIdx.x >> 2) & 7, k = threadIdx.x >> 5;
+    const int i = 2 * ih + ((j + k + ipass) & 1);
+    const long q0 = (long)R.b * B::LEN + B::gi(i, j, k);
+    const long q1 = (long)R.b * B::LEN + B::gi(threadIdx.x & 7, (threadIdx.x >> 3) & 7, threadIdx.x >> 6);
+    const long q2 = q1 + B::gi(0, 0, 4) - B::gi(0, 0, 0);
+    T rhs0 = 0, rhs1 = 0;
+    if (RELAX) rhs0 = Arhs[q0];
+    else { rhs0 = Arhs[q1]; rhs1 = Arhs[q2]; }
+    if (halo) {
         // halo-free form: the same-level faces' ghost cells taken from the neighbours' boundary
         // layers (what fmg_ghfix_samelev would copy there); only the face cells the 7-point
         // operator reads, not edges or corners.  The neighbour's cells next to this block's
         // colour-ipass cells have the other colour, so a concurrent pass does not change them.
+        // The barrier orders these LDS writes after the block copy above, which writes the same
+        // ghost cells with their stored values.
+        __syncthreads();
         for (int q = threadIdx.x; q < 6 * 64; q += 256) {
             const int f = q >> 6, t1 = q & 7, t2 = (q >> 3) & 7, d = f >> 1, lr = f & 1;
-            if (nkind[6 * b + f] != NBR_SAME) continue;
-            su[B::rot(d, lr ? 8 : -1, t1, t2)] = Au[(long)nbr[6 * b + f] * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
+            const int m = R.nb[f];
+            if (m < 0) continue;
+            su[B::rot(d, lr ? 8 : -1, t1, t2)] = Au[(long)m * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
         }
     }
     __syncthreads();
-    const int l = level[b];
-    const T h = (T)h_lev[l], hi = (T)hi_lev[l];
-    Flux<8, T> F{su, ovslot + 6 * b, ov, hi};
+    const T h = (T)h_lev.v[R.l], hi = (T)hi_lev.v[R.l];
+    FluxR<8, T> F{su, nullptr, ov, hi, R};
     double a = 0.0;
     if (RELAX) {
-        const int ih = threadIdx.x & 3, j = (threadIdx.x >> 2) & 7, k = threadIdx.x >> 5;
-        const int i = 2 * ih + ((j + k + ipass) & 1);
-        const long q = (long)b * B::LEN + B::gi(i, j, k);
-        T r = F.div(i, j, k) - Arhs[q] * h;
+        T r = F.div(i, j, k) - rhs0 * h;
         T resh2 = r * h;
-        Au[q] = su[B::gi(i, j, k)] + (T)(1.0 / 6.0) * resh2;
+        Au[q0] = su[B::gi(i, j, k)] + (T)(1.0 / 6.0) * resh2;
         a = (double)fabs(resh2);
     } else {
-        for (int c = threadIdx.x; c < 512; c += 256) {
-            const int i = c & 7, j = (c >> 3) & 7, k = c >> 6;
-            const long q = (long)b * B::LEN + B::gi(i, j, k);
-            Aout[q] = Arhs[q] - F.div(i, j, k) * hi;
-        }
+        const int c = threadIdx.x;
+        Aout[q1] = rhs0 - F.div(c & 7, (c >> 3) & 7, c >> 6) * hi;
+        const int c2 = c + 256;
+        Aout[q2] = rhs1 - F.div(c2 & 7, (c2 >> 3) & 7, c2 >> 6) * hi;
     }
     if (RELAX && resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
 }

@@ -345,7 +345,7 @@ def _zero_physical_ghosts(h, a, L):
             a[(b,) + tuple(sl)] = 0.0
 
 
-@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic"])
+@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic", "nested-8-4", "nested-16-5"])
 @pytest.mark.parametrize("L", [0, 1])
 @pytest.mark.parametrize("opname", ["relax", "resid"])
 def test_fused_operator_equals_staged(which, L, opname, gpu):
@@ -354,7 +354,9 @@ def test_fused_operator_equals_staged(which, L, opname, gpu):
     the residual) must agree bit for bit."""
     h = {"nested-sphere": lambda: ah.nested(base=4, levels=3, bc=ah.SPHERE),
          "nested-periodic": lambda: ah.nested(base=4, levels=3, bc=ah.PERIODIC),
-         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC)}[which]()
+         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC),
+         "nested-8-4": lambda: ah.nested(base=8, levels=4, bc=ah.SPHERE),
+         "nested-16-5": lambda: ah.nested(base=16, levels=5, bc=ah.SPHERE)}[which]()
     o, g = oracle.Amr(h), AmrSolver(h)
     try:
         if not (g.info()["fused_levels"] >> L) & 1:
