@@ -66,10 +66,11 @@ class AmrSolver:
 
     def info(self):
         """Context facts: FMG levels whose mixed-solve refreshes run fused (bit mask), kernel nodes of the
-        captured mixed outer iteration, blocks, coarse-fine faces."""
-        v = (ctypes.c_int * 4)()
-        _lib.check(self._lib.sg_amr_info(self._ctx, v, 4))
-        return dict(fused_levels=v[0], graph_nodes=v[1], nblk=v[2], cf_faces=v[3])
+        captured mixed outer iteration, blocks, coarse-fine faces, AMR levels whose vmg_ghostcell
+        runs as one gather in the fused mode (bit mask)."""
+        v = (ctypes.c_int * 5)()
+        _lib.check(self._lib.sg_amr_info(self._ctx, v, 5))
+        return dict(fused_levels=v[0], graph_nodes=v[1], nblk=v[2], cf_faces=v[3], vmg_fill_levels=v[4])
 
     def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64,
               use_graph=True, raise_on_fail=True, fused=True):

@@ -75,6 +75,17 @@ struct sg_amr {
     // number (d_brec) and in the order of the active list (d_arec); cell widths per FMG level
     int *d_brec = nullptr, *d_arec = nullptr;
     LevelScalars hs[kNfmg] = {}, his[kNfmg] = {};
+    // vmg_ghostcell at FMG level 2 as one gather per AMR level where the hierarchy allows it
+    // (same_fill_pairs): {destination, source or -1} pairs, all levels in one table
+    int *d_vfill = nullptr;
+    int vfill_off[kMaxLev + 1] = {};
+    int vfill_ok[kMaxLev] = {};
+    // merged vmg_fas steps (vmg_fas_merged): per FMG-level-2 cell what vmg_ghostcell does to it
+    // (-2 untouched, -1 zero, else the cell it copies), and per covered block the mask of the 64
+    // cells vmg_tau_crop zeroes; usable when every level has its gather table
+    int *d_vmap = nullptr;
+    unsigned long long *d_crop = nullptr;
+    int vmg_merged_ok = 0;
     int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
@@ -457,12 +468,19 @@ template <typename T> int lev_add(sg_amr &s, int l, int a, int da, bool covered_
               (int)LENV);
 }
 
+// vmg_ghostcell (vmg.F90:728-742): same-level faces x, y, z, then zero on physical faces; in the
+// mixed solve's fused mode one gather (k_fill_pairs) where the hierarchy allows it
 template <typename T> int vmg_ghostcell(sg_amr &s, int l, int code)
 {
+    if (s.fused_on && s.vfill_ok[l]) {
+        const int e0 = s.vfill_off[l], ne = s.vfill_off[l + 1] - e0;
+        return go(s, k_fill_pairs<T>, (long)ne, ar<T>(s).A[LV][code], (const int2 *)s.d_vfill + e0, ne);
+    }
     for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, LV, d, code, 0, K_SAME));
     return fill0_faces<T>(s, LV, code, l, l + 1);
 }
 
+// vmg_gridboundary (vmg.F90:708-724): parent-level faces x, y, z (linear), then physical zero
 template <typename T> int vmg_gridboundary(sg_amr &s, int l, int code)
 {
     for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, LV, d, code, 0, K_PARENT));
@@ -550,9 +568,59 @@ template <typename T> int vmg_slvsml(sg_amr &s, int ju, int jrhs)
               (const int *)s.d_ijk, NG);
 }
 
+// vmg_fas with consecutive steps merged into single launches (the mixed solve's fused mode; the
+// same operations on every value, amr_kernels.h "merged steps of vmg_fas")
+template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
+{
+    auto &R = ar<T>(s);
+    T *U = R.A[LV][C_U], *RES = R.A[LV][C_RES], *RHS = R.A[LV][C_RHS], *RHO = R.A[LV][C_RHO], *PSI = R.A[LV][C_PSI];
+    const long nall = (long)s.nblk * LENV;
+    auto nlv = [&](int l) { return s.lev_start[l + 1] - s.lev_start[l]; };
+    if (jrho != C_RHO)
+        SG_TRY(go(s, k_blocks_copy<T>, nall, RHO, (const T *)R.A[LV][jrho], (const int *)nullptr, 0, s.nblk, (int)LENV));
+    for (int lc = s.nlev - 2; lc >= 0; --lc)   // fmg_converge_c2p of U and of RHO, finest first
+        SG_TRY(go(s, k_c2p_two<T>, (long)nlv(lc + 1), U, RHO, s.lev_start[lc + 1], nlv(lc + 1), (const int *)s.d_parent,
+                  (const int *)s.d_octant));
+    SG_TRY(fill0_faces<T>(s, LV, C_U, 0, s.nlev));
+    SG_TRY(go(s, k_blocks_copy<T>, nall, RHS, (const T *)RHO, (const int *)nullptr, 0, s.nblk, (int)LENV));
+    for (int v = s.nlev - 1; v >= 1; --v) {
+        SG_TRY(vmg_gridboundary<T>(s, v, C_U));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS, p == 1));
+        for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, v, LV, d, C_U, 1, K_BOTH));
+        // vmg_rstrct(v-1, U, cubic), vmg_copy(v-1, IRUF <- U), vmg_rstrct(v-1, RHS)
+        SG_TRY(go(s, k_vmg_rstr<T>, (long)nlv(v - 1) * LENV, U, PSI, RHS, s.lev_start[v - 1], nlv(v - 1),
+                  (const int *)s.d_child, (const int *)s.d_covered, s.st));
+        // vmg_poisson_tau + vmg_tau_crop (vmg_tau), then vmg_add_onov
+        SG_TRY(vmg_ghostcell<T>(s, v, C_U));
+        SG_TRY(go(s, k_tau_fine_c2p<T>, (long)nlv(v) * LENV, RES, (const T *)U, s.lev_start[v], nlv(v), s.hil[LV][v],
+                  (const int *)s.d_parent, (const int *)s.d_octant));
+        SG_TRY(vmg_ghostcell<T>(s, v - 1, C_U));
+        const int c0 = s.cov_off[v - 1], nc = s.cov_off[v] - c0;
+        SG_TRY(go(s, k_tau_finish<T>, (long)nc * LENV, RES, RHS, (const T *)U, (const int *)(s.d_cov + c0), nc,
+                  s.hil[LV][v - 1], (const unsigned long long *)(s.d_crop + c0)));
+    }
+    if (s.zero_average) SG_TRY(go1(s, k_zero_average<2, T>, R.A[LV][C_RHS], s.nbase, s.P_base, s.d_part));
+    SG_TRY(vmg_slvsml<T>(s, C_U, C_RHS));
+    for (int v = 1; v < s.nlev; ++v) {
+        // vmg_sub(v-1) with vmg_interp's vmg_ghostcell(v-1, RES), its extrapolation, then
+        // vmg_interp_p2c_lev with vmg_add(v), then vmg_interp's fill0
+        const long base = (long)s.lev_start[v - 1] * LENV;
+        SG_TRY(go(s, k_sub_fill<T>, (long)nlv(v - 1) * LENV, RES, (const T *)U, (const T *)PSI, base,
+                  (int)((long)nlv(v - 1) * LENV), (const int *)(s.d_vmap + base)));
+        SG_TRY(extrap_faces<T>(s, LV, C_RES, v - 1, v));
+        SG_TRY(go(s, k_p2c_add<T>, (long)nlv(v) * LENV, RES, U, s.lev_start[v], nlv(v), (const int *)s.d_parent,
+                  (const int *)s.d_octant));
+        SG_TRY(fill0_faces<T>(s, LV, C_RES, v - 1, v));
+        SG_TRY(vmg_gridboundary<T>(s, v, C_U));
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS, p == 1));
+    }
+    return SG_OK;
+}
+
 // vmg_fas (vmg.F90:116-243): Ncycle = 1, Npre = Npost = 2
 template <typename T> int vmg_fas(sg_amr &s, int jrho)
 {
+    if (s.fused_on && s.vmg_merged_ok) return vmg_fas_merged<T>(s, jrho);
     auto &R = ar<T>(s);
     if (jrho != C_RHO)
         for (int l = 0; l < s.nlev; ++l) SG_TRY(lev_copy<T>(s, l, C_RHO, jrho));
@@ -975,6 +1043,61 @@ bool prep_source(const sg_amr &s, int N, int b, int i, int j, int k, int &src)
     return true;
 }
 
+// vmg_ghostcell of AMR level l (N cells per block side) as {destination, source} pairs, appended
+// to `pairs`: the same-level passes x, y, z copy whole slabs, so the value a ghost cell ends with is
+// the stored cell reached by following its ghost coordinates through same-level faces from z down
+// to x (the last pass writing it reads what the earlier ones left), and a cell on a physical face
+// ends as zero (source -1).  False where a single gather would not reproduce the passes: a source
+// that the fill itself writes (a pass would read it before a later pass changes it).
+bool same_fill_pairs(const sg_amr &s, int N, int l, std::vector<int> &pairs)
+{
+    const int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+    const long W = N + 2, LEN = W * W * W;
+    if ((long)s.nblk * LEN >= (long)INT32_MAX) return false;
+    auto addr = [&](int b, int i, int j, int k) { return (long)b * LEN + ((long)(k + 1) * W + (j + 1)) * W + (i + 1); };
+    std::vector<char> target((size_t)nb * LEN, 0);
+    const size_t first = pairs.size();
+    for (int b = b0; b < b0 + nb; ++b)
+        for (int k = -1; k <= N; ++k)
+            for (int j = -1; j <= N; ++j)
+                for (int i = -1; i <= N; ++i) {
+                    int p[3] = {i, j, k};
+                    bool same = false, phys = false, ghost = false;
+                    for (int d = 0; d < 3; ++d)
+                        if (p[d] == -1 || p[d] == N) {
+                            ghost = true;
+                            const int kind = s.nkind[6 * b + 2 * d + (p[d] == N ? 1 : 0)];
+                            phys = phys || kind == NBR_NONE;
+                            same = same || kind == NBR_SAME;
+                        }
+                    if (!ghost || !(phys || same)) continue;
+                    const long dst = addr(b, i, j, k);
+                    target[(size_t)(dst - (long)b0 * LEN)] = 1;
+                    if (phys) {
+                        pairs.insert(pairs.end(), {(int)dst, -1});
+                        continue;
+                    }
+                    int m = b;
+                    for (int D = 2; D >= 0; --D)
+                        if (p[D] == -1 || p[D] == N) {
+                            const int f = 6 * m + 2 * D + (p[D] == N ? 1 : 0);
+                            if (s.nkind[f] == NBR_SAME) {
+                                p[D] = p[D] == N ? 0 : N - 1;
+                                m = s.nbr[f];
+                            }
+                        }
+                    pairs.insert(pairs.end(), {(int)dst, (int)addr(m, p[0], p[1], p[2])});
+                }
+    for (size_t e = first; e < pairs.size(); e += 2) {
+        const long rel = (long)pairs[e + 1] - (long)b0 * LEN;
+        if (pairs[e + 1] >= 0 && rel >= 0 && rel < (long)nb * LEN && target[(size_t)rel]) {
+            pairs.resize(first);
+            return false;
+        }
+    }
+    return true;
+}
+
 // k_prep's tables at FMG level L; false (the solves then keep the staged refresh there) where some
 // value a colour pass or residual reads cannot be reproduced: every coarse value a coarse-fine ghost
 // interpolates from, and every covered face-layer cell a same-level leaf reads.
@@ -1032,8 +1155,9 @@ void release(sg_amr *s)
     if (!s) return;
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
                  s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters,
-                 s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec};
+                 s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap};
     for (int *q : ip) if (q) (void)hipFree(q);
+    if (s->d_crop) (void)hipFree(s->d_crop);
     for (int L = 0; L < kNfmg; ++L) {
         if (s->d_prep_cov[L]) (void)hipFree(s->d_prep_cov[L]);
         if (s->d_prep_src[L]) (void)hipFree(s->d_prep_src[L]);
@@ -1164,6 +1288,38 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     }
     std::vector<int> prep_cov[kNfmg], prep_src[kNfmg];
     for (int L = 0; L < kNfmg - 1; ++L) s->fused_ok[L] = build_prep_tables(*s, L, prep_cov[L], prep_src[L]) ? 1 : 0;
+    std::vector<int> vfill;
+    for (int l = 0; l < nlev; ++l) {
+        s->vfill_off[l] = (int)vfill.size() / 2;
+        s->vfill_ok[l] = same_fill_pairs(*s, 2, l, vfill) ? 1 : 0;
+    }
+    s->vfill_off[nlev] = (int)vfill.size() / 2;
+    std::vector<int> vmap;
+    std::vector<unsigned long long> crop;
+    s->vmg_merged_ok = 1;
+    for (int l = 0; l < nlev; ++l) s->vmg_merged_ok &= s->vfill_ok[l];
+    if (s->vmg_merged_ok) {
+        // vmg_ghostcell's effect per cell, from the gather tables
+        vmap.assign((size_t)nblk * lenof(LV), -2);
+        for (size_t e = 0; e < vfill.size(); e += 2) vmap[(size_t)vfill[e]] = vfill[e + 1];
+        // vmg_tau_crop: a level-l block's face meeting level l-1 zeroes its parent's plane there
+        std::vector<int> covidx(nblk, -1);
+        for (size_t e = 0; e < s->cov.size(); ++e) covidx[s->cov[e]] = (int)e;
+        crop.assign(std::max<size_t>(1, s->cov.size()), 0ULL);
+        const int W = nof(LV) + 2;
+        for (size_t e = 0; e < s->cf.size(); ++e) {
+            const int b = s->cf[e] / 6, f = s->cf[e] % 6, d = f / 2, a = (f & 1) ? nof(LV) - 1 : 0;
+            unsigned long long &m = crop[(size_t)covidx[parent[b]]];
+            for (int t2 = -1; t2 <= nof(LV); ++t2)
+                for (int t1 = -1; t1 <= nof(LV); ++t1) {
+                    int p[3];
+                    p[d] = a;
+                    p[d == 0 ? 1 : 0] = t1;
+                    p[d == 2 ? 1 : 2] = t2;
+                    m |= 1ULL << (((p[2] + 1) * W + (p[1] + 1)) * W + (p[0] + 1));
+                }
+        }
+    }
     // device tables
     SG_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     SG_HIP(hipEventCreate(&s->ev0));
@@ -1206,6 +1362,11 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
             rec.insert(rec.end(), {b, f + 8 * s->octant[b], s->cf_slot[e], s->level[b]});
         }
         SG_TRY(upload(*s, &s->d_cfrec, rec));
+        SG_TRY(upload(*s, &s->d_vfill, vfill));
+        if (s->vmg_merged_ok) {
+            SG_TRY(upload(*s, &s->d_vmap, vmap));
+            SG_TRY(upload(*s, &s->d_crop, crop));
+        }
         for (int L = 0; L < kNfmg - 1; ++L) {
             if (!s->fused_ok[L]) continue;
             s->prep_ncov[L] = (int)prep_cov[L].size() / 2;
@@ -1462,9 +1623,12 @@ int sg_amr_set_fill0_level(sg_amr *ctx, int v)
 int sg_amr_info(sg_amr *ctx, int *info, int n)
 {
     SG_REQUIRE(ctx && info && n >= 0, "sg_amr_info: null argument");
-    const long v[4] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
-                       (long)ctx->cf.size()};
-    for (int q = 0; q < n; ++q) info[q] = q < 4 ? (int)v[q] : 0;
+    long vmg = 0;   // AMR levels whose vmg_ghostcell runs as one gather
+    for (int l = 0; l < ctx->nlev; ++l)
+        if (ctx->vfill_ok[l]) vmg |= 1L << l;
+    const long v[5] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
+                       (long)ctx->cf.size(), vmg};
+    for (int q = 0; q < n; ++q) info[q] = q < 5 ? (int)v[q] : 0;
     return SG_OK;
 }
 

@@ -318,6 +318,19 @@ template <int N> struct GhostCells {
     }
 };
 
+// vmg_ghostcell (vmg.F90:728-742) of one AMR level in one launch: every cell it writes is either a
+// copy of one stored cell (the same-level passes x, y, z only copy, so the value a ghost cell ends
+// with is the cell the copies lead back to) or, on a physical face, zero.  pairs[t] = {destination,
+// source or -1 for zero}, resolved by the host (amr.hip, same_fill_pairs), which also checks that no
+// source is a destination.
+template <typename T> __global__ void k_fill_pairs(T *A, const int2 *pairs, int n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int2 p = pairs[t];
+    A[p.x] = p.y >= 0 ? A[p.y] : (T)0;
+}
+
 template <int N, typename T>
 __global__ void k_ghost_level(T *A, int b0, int nb, int cubic, const int *nbr, const int *nkind, const int *octant)
 {
@@ -820,6 +833,155 @@ template <int N, typename T> __global__ void k_tau_crop(T *Atau, const int *cf, 
     int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1, e = cf[t / (W * W)];
     int b = e / 6, f = e % 6;
     Atau[(long)parent[b] * B::LEN + B::rot(f / 2, (f & 1) ? N - 1 : 0, t1, t2)] = (T)0;
+}
+
+// ---- merged steps of vmg_fas at FMG level 2 (2^3 cells per block) for the mixed solve's fused
+//      mode: each does, in one launch, what consecutive launches of the staged sequence do, with
+//      the same operations in the same order on every value ----
+namespace vmgm {
+using B2 = Blk<2>;
+__device__ __forceinline__ void cell_of(int c, int &i, int &j, int &k)
+{
+    i = c % B2::W - 1;
+    j = (c / B2::W) % B2::W - 1;
+    k = c / (B2::W * B2::W) - 1;
+}
+__device__ __forceinline__ bool interior(int i, int j, int k) { return i >= 0 && i < 2 && j >= 0 && j < 2 && k >= 0 && k < 2; }
+} // namespace vmgm
+
+// fmg_converge_c2p_lev of two arrays at once (children [b0, b0+nb) into their parents)
+template <typename T> __global__ void k_c2p_two(T *A1, T *A2, int b0, int nb, const int *parent, const int *octant)
+{
+    using B = Blk<2>;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb) return;
+    const int b = b0 + (int)t, o = octant[b];
+    const long dst = (long)parent[b] * B::LEN + B::gi(o & 1, (o >> 1) & 1, (o >> 2) & 1);
+    A1[dst] = rstr_lin<2, T>(A1 + (long)b * B::LEN, 0, 0, 0);
+    A2[dst] = rstr_lin<2, T>(A2 + (long)b * B::LEN, 0, 0, 0);
+}
+
+// vmg_rstrct(v-1, U, cubic) + vmg_copy(v-1, IRUF <- U) + vmg_rstrct(v-1, RHS) over the blocks
+// [b0, b0+nb) of level v-1: a covered block's interior cell gets its child's cubic restriction of
+// U (in U and in PSI, the IRUF slot) and linear restriction of RHS; every other cell copies U to PSI.
+template <typename T>
+__global__ void k_vmg_rstr(T *U, T *PSI, T *RHS, int b0, int nb, const int *child, const int *covered, Cubic64 st)
+{
+    using B = Blk<2>;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * B::LEN) return;
+    const int b = b0 + (int)(t / B::LEN), c = (int)(t % B::LEN);
+    int i, j, k;
+    vmgm::cell_of(c, i, j, k);
+    const long q = (long)b * B::LEN + c;
+    if (vmgm::interior(i, j, k) && covered[b]) {
+        const int o = i + 2 * j + 4 * k;
+        const long cb = (long)child[8 * b + o] * B::LEN;
+        const T v = rstr_cub<2, T>(U + cb, 0, 0, 0, st);
+        U[q] = v;
+        PSI[q] = v;
+        RHS[q] = rstr_lin<2, T>(RHS + cb, 0, 0, 0);
+        return;
+    }
+    PSI[q] = U[q];
+}
+
+// vmg_poisson_tau's fine part for level v (tau = 0, then -(div F)/h inside, into RES of the blocks
+// [b0, b0+nb)) + fmg_converge_c2p_lev of RES into the parents.  The thread of a block's first
+// interior cell also forms the restriction, from the eight values of its block (recomputed in the
+// same way, so equal to the stored ones).
+template <typename T> __global__ void k_tau_fine_c2p(T *RES, const T *U, int b0, int nb, double hi_d, const int *parent,
+                                                    const int *octant)
+{
+    using B = Blk<2>;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * B::LEN) return;
+    const int b = b0 + (int)(t / B::LEN), c = (int)(t % B::LEN);
+    int i, j, k;
+    vmgm::cell_of(c, i, j, k);
+    const T hi = (T)hi_d;
+    const long base = (long)b * B::LEN;
+    if (!vmgm::interior(i, j, k)) {
+        RES[base + c] = (T)0;
+        return;
+    }
+    Flux<2, T> F{U + base, nullptr, nullptr, hi};
+    RES[base + c] = -(F.div(i, j, k) * hi);
+    if (c != B::gi(0, 0, 0)) return;
+    T f[2][2][2];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) f[kk][jj][ii] = -(F.div(ii, jj, kk) * hi);
+    T sum = f[0][0][0] + f[0][0][1];
+    sum = sum + f[0][1][0];
+    sum = sum + f[1][0][0];
+    sum = sum + f[0][1][1];
+    sum = sum + f[1][0][1];
+    sum = sum + f[1][1][0];
+    sum = sum + f[1][1][1];
+    const int o = octant[b];
+    RES[(long)parent[b] * B::LEN + B::gi(o & 1, (o >> 1) & 1, (o >> 2) & 1)] = sum * (T)(1.0 / 8.0);
+}
+
+// vmg_poisson_tau's coarse part (tau += (div F)/h inside) + vmg_tau_crop (the planes of the
+// parent next to a child face that meets the parent's level set to zero; crop[e] is that mask of
+// the 64 cells of covered block list[e]) + vmg_add_onov (RHS += tau) on the covered blocks of one level.
+template <typename T> __global__ void k_tau_finish(T *RES, T *RHS, const T *U, const int *list, int nc, double hi_d,
+                                                  const unsigned long long *crop)
+{
+    using B = Blk<2>;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nc * B::LEN) return;
+    const int e = (int)(t / B::LEN), c = (int)(t % B::LEN), b = list[e];
+    int i, j, k;
+    vmgm::cell_of(c, i, j, k);
+    const long q = (long)b * B::LEN + c;
+    T tau = RES[q];
+    if (vmgm::interior(i, j, k)) {
+        Flux<2, T> F{U + (long)b * B::LEN, nullptr, nullptr, (T)hi_d};
+        tau = tau + F.div(i, j, k) * (T)hi_d;
+    }
+    if ((crop[e] >> c) & 1ULL) tau = (T)0;
+    RES[q] = tau;
+    RHS[q] = RHS[q] + tau;
+}
+
+// vmg_sub (RES = U - PSI over whole arrays of one level) + vmg_ghostcell of RES there: map[t] is -2
+// where the fill leaves the cell, -1 where it zeroes it (physical face), else the cell it copies.
+template <typename T> __global__ void k_sub_fill(T *RES, const T *U, const T *PSI, long base, int n, const int *map)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int m = map[t];
+    const long q = base + t;
+    RES[q] = m == -2 ? U[q] - PSI[q] : (m == -1 ? (T)0 : U[m] - PSI[m]);
+}
+
+// vmg_interp_p2c_lev (trilinear, child interiors from the parent's octant, its ghosts included) +
+// vmg_add (U += RES over whole arrays) for the blocks [b0, b0+nb)
+template <typename T> __global__ void k_p2c_add(T *RES, T *U, int b0, int nb, const int *parent, const int *octant)
+{
+    using B = Blk<2>;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * B::LEN) return;
+    const int b = b0 + (int)(t / B::LEN), c = (int)(t % B::LEN);
+    int i, j, k;
+    vmgm::cell_of(c, i, j, k);
+    const long q = (long)b * B::LEN + c;
+    if (vmgm::interior(i, j, k)) {
+        const T *uc = RES + (long)parent[b] * B::LEN;
+        const int o = octant[b];
+        const int oi = o & 1, oj = (o >> 1) & 1, ok = (o >> 2) & 1;
+        auto cget = [&](int a, int bb, int cc) -> T { return uc[B::gi(oi + a, oj + bb, ok + cc)]; };
+        const T r = trilinear_at(i, j, k, cget);
+        RES[q] = r;
+        U[q] = U[q] + r;
+        return;
+    }
+    U[q] = U[q] + RES[q];
 }
 
 // ---- vmg_interp_p2c_lev (trilinear): child interiors of [b0, b0+nb) from the parent's octant ----

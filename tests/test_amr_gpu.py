@@ -327,6 +327,7 @@ def test_fused_refresh_full_sphere_hierarchy_launches(gpu):
     info = g.info()
     print("nodes staged", n_staged, "fused", info["graph_nodes"], "ms", ss["ms"], sf["ms"], flush=True)
     assert info["fused_levels"] == 3
+    assert info["vmg_fill_levels"] == (1 << h.nlev) - 1
     assert ss["iterations"] == sf["iterations"] and ss["hist"] == sf["hist"]
     assert np.array_equal(ps, pf) and np.array_equal(gs, gf)
     assert info["graph_nodes"] < n_staged
