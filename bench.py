@@ -132,7 +132,8 @@ def bench_amr(a, dev):
            "fp64_history": st["hist"],
            "mixed_gpu_ms": round(msmx, 3), "mixed_iterations": stm["iterations"], "mixed_rel_linf": stm["rel_linf"],
            "mixed_rel_l2": stm["rel_l2"], "mixed_history": stm["hist"], "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
-           "mixed_vs_fp64_gpu": {"psi_rel_l2": err_p, "g_rel_l2": err_g}}
+           "mixed_vs_fp64_gpu": {"psi_rel_l2": err_p, "g_rel_l2": err_g}, "autotune": solver.tile_info(),
+           "context": solver.info()}
     if not a.no_cpu:
         import oracle
         log("[bench] AMR CPU oracle (FP64, 1 core) on the same hierarchy")

@@ -72,6 +72,18 @@ class AmrSolver:
         _lib.check(self._lib.sg_amr_info(self._ctx, v, 5))
         return dict(fused_levels=v[0], graph_nodes=v[1], nblk=v[2], cf_faces=v[3], vmg_fill_levels=v[4])
 
+    def tile_info(self):
+        """The autotuner's threads per 8^3 block of the FMG-level-0 relaxation / residual kernels and
+        the candidates' ms per launch."""
+        out = {}
+        for which, name in enumerate(("relax_fp32", "resid_fp32", "relax_fp64", "resid_fp64")):
+            c = ctypes.c_int()
+            ms = (ctypes.c_float * 3)()
+            _lib.check(self._lib.sg_amr_tile_info(self._ctx, which, ctypes.byref(c), ms, 3))
+            out[name] = {"threads_per_block": c.value,
+                         "ms_per_launch": {str(nt): round(ms[q], 5) for q, nt in enumerate((64, 128, 256)) if ms[q] > 0}}
+        return out
+
     def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64,
               use_graph=True, raise_on_fail=True, fused=True):
         """fmg_poisson on torch float64 GPU tensors [nblk, 8, 8, 8]; psi in place, g [nblk, 3, 8, 8, 8].

@@ -86,6 +86,10 @@ struct sg_amr {
     int *d_vmap = nullptr;
     unsigned long long *d_crop = nullptr;
     int vmg_merged_ok = 0;
+    // autotuned threads per block of the FMG-level-0 colour / residual kernel (k_blk8):
+    // [0] relax FP32, [1] residual FP32, [2] relax FP64, [3] residual FP64; candidate timings in ms
+    int blk8_nt[4] = {256, 256, 256, 256};
+    float blk8_ms[4][3] = {};
     int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
@@ -313,6 +317,62 @@ int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *rec, int nrec, int
     SG_NDISPATCH(L, relax_pass_n, s, L, ju, jrhs, rec, nrec, ipass, with_ov, resmax, halo_free);
 }
 
+constexpr int kBlk8Nt[3] = {64, 128, 256};
+
+// one launch of k_blk8 over the active blocks with nt threads per block
+template <bool RELAX, typename T>
+int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo)
+{
+    const dim3 g((unsigned)s.active.size());
+    const int *rec = s.d_arec;
+    if (nt == 64)
+        hipLaunchKernelGGL((k_blk8<RELAX, T, 64>), g, dim3(64), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0],
+                           ov, resmax, halo);
+    else if (nt == 128)
+        hipLaunchKernelGGL((k_blk8<RELAX, T, 128>), g, dim3(128), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
+                           s.his[0], ov, resmax, halo);
+    else
+        hipLaunchKernelGGL((k_blk8<RELAX, T, 256>), g, dim3(256), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
+                           s.his[0], ov, resmax, halo);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
+template <typename T> constexpr int kBlk8Slot = std::is_same<T, float>::value ? 0 : 2;
+
+// The autotuner of the FMG-level-0 kernels: each candidate's time over 4 launches (after one
+// warm-up) on the context's own arrays, relaxation and residual apart.  The values it leaves in U
+// and RES do not matter: every solve starts from a cleared state.
+template <typename T> int tune_blk8(sg_amr &s)
+{
+    if (s.active.empty()) return SG_OK;
+    auto &R = ar<T>(s);
+    const int halo = s.fused_ok[0];
+    for (int which = 0; which < 2; ++which) {
+        float best = 1e30f;
+        for (int c = 0; c < 3; ++c) {
+            const int nt = kBlk8Nt[c];
+            for (int rep = 0; rep < 5; ++rep) {
+                if (rep == 1) SG_HIP(hipEventRecord(s.ev0, s.stream));
+                if (which == 0)
+                    SG_TRY((blk8<true, T>(s, nt, (T *)nullptr, R.A[0][C_U], R.A[0][C_RHS], 1 + (rep & 1), R.ov[0], nullptr, halo)));
+                else
+                    SG_TRY((blk8<false, T>(s, nt, R.A[0][C_RES], R.A[0][C_U], R.A[0][C_RHS], 0, R.ov[0], nullptr, halo)));
+            }
+            SG_HIP(hipEventRecord(s.ev1, s.stream));
+            SG_HIP(hipEventSynchronize(s.ev1));
+            float ms = 0.f;
+            SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+            s.blk8_ms[kBlk8Slot<T> + which][c] = ms / 4;
+            if (ms < best) {
+                best = ms;
+                s.blk8_nt[kBlk8Slot<T> + which] = nt;
+            }
+        }
+    }
+    return SG_OK;
+}
+
 // fmg_poisson_relax (fmg_poisson.F90:130-213).  With want_max, Resh2maxg lands in d_scal[0]; the
 // reference overwrites Resh2maxg in every call but reads only the last one of fmg_lin.
 // staged_last: the refresh before the second colour runs staged even where the fused form applies,
@@ -337,10 +397,8 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
         }
         const bool hf = fused(s, L);
         if (L == 0) {
-            hipLaunchKernelGGL((k_blk8<true, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, (T *)nullptr,
-                               ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_arec, ipass,
-                               s.hs[0], s.his[0], (const T *)ar<T>(s).ov[0], slots, hf ? 1 : 0);
-            SG_CHECK_LAUNCH();
+            SG_TRY((blk8<true, T>(s, s.blk8_nt[kBlk8Slot<T>], (T *)nullptr, ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], ipass,
+                                  ar<T>(s).ov[0], slots, hf ? 1 : 0)));
         } else {
             SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_arec, (int)s.active.size(), ipass, true, slots, hf));
         }
@@ -366,11 +424,8 @@ template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int 
     SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
     if (L == 0) {
         const bool hf = fused(s, 0);
-        hipLaunchKernelGGL((k_blk8<false, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, ar<T>(s).A[0][jres],
-                           ar<T>(s).A[0][ju], (const T *)ar<T>(s).A[0][jrhs], (const int *)s.d_arec, 0, s.hs[0],
-                           s.his[0], (const T *)ar<T>(s).ov[0], (double *)nullptr, hf ? 1 : 0);
-        SG_CHECK_LAUNCH();
-        return SG_OK;
+        return blk8<false, T>(s, s.blk8_nt[kBlk8Slot<T> + 1], ar<T>(s).A[0][jres], ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], 0,
+                              ar<T>(s).ov[0], nullptr, hf ? 1 : 0);
     }
     SG_NDISPATCH(L, resid_n, s, L, jres, ju, jrhs);
 }
@@ -480,9 +535,18 @@ template <typename T> int vmg_ghostcell(sg_amr &s, int l, int code)
     return fill0_faces<T>(s, LV, code, l, l + 1);
 }
 
-// vmg_gridboundary (vmg.F90:708-724): parent-level faces x, y, z (linear), then physical zero
+// vmg_gridboundary (vmg.F90:708-724): parent-level faces x, y, z (linear), then physical zero; in
+// the mixed solve's fused mode one launch (k_parent_fill)
 template <typename T> int vmg_gridboundary(sg_amr &s, int l, int code)
 {
+    if (s.fused_on && l >= 1) {
+        const int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+        const int fill0 = s.phys_off[l + 1] > s.phys_off[l] ? 1 : 0;
+        hipLaunchKernelGGL(k_parent_fill<T>, dim3((unsigned)((nb + 7) / 8)), dim3(256), 0, s.stream, ar<T>(s).A[LV][code], b0,
+                           nb, fill0, (const int *)s.d_nbr, (const int *)s.d_nkind, (const int *)s.d_octant);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
+    }
     for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, LV, d, code, 0, K_PARENT));
     return fill0_faces<T>(s, LV, code, l, l + 1);
 }
@@ -888,7 +952,7 @@ int ensure_mixed(sg_amr &s)
         SG_HIP(hipMemcpy(s.bd_inv, inv.data(), sizeof(int) * inv.size(), hipMemcpyHostToDevice));
     }
     s.have_f = 1;
-    return SG_OK;
+    return tune_blk8<float>(s);
 }
 
 // One outer iteration: FP32 copy of the FP64 residual, inner fmg_lin in FP32, FP64 update
@@ -1457,6 +1521,7 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
         SG_HIP(hipFuncSetAttribute((const void *)k_base_mg, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(s->bm_lds * 8)));
     }
+    SG_TRY(tune_blk8<double>(*s));
     SG_HIP(hipDeviceSynchronize());
     return SG_OK;
 }
@@ -1629,6 +1694,15 @@ int sg_amr_info(sg_amr *ctx, int *info, int n)
     const long v[5] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
                        (long)ctx->cf.size(), vmg};
     for (int q = 0; q < n; ++q) info[q] = q < 5 ? (int)v[q] : 0;
+    return SG_OK;
+}
+
+int sg_amr_tile_info(sg_amr *ctx, int which, int *choice, float *ms, int nms)
+{
+    SG_REQUIRE(ctx && choice && (ms || nms == 0) && nms >= 0, "sg_amr_tile_info: null argument");
+    SG_REQUIRE(which >= 0 && which < 4, "sg_amr_tile_info: unknown kernel %d", which);
+    *choice = ctx->blk8_nt[which];
+    for (int q = 0; q < nms; ++q) ms[q] = q < 3 ? ctx->blk8_ms[which][q] : 0.f;
     return SG_OK;
 }
 

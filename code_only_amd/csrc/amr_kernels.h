@@ -849,6 +849,53 @@ __device__ __forceinline__ void cell_of(int c, int &i, int &j, int &k)
 __device__ __forceinline__ bool interior(int i, int j, int k) { return i >= 0 && i < 2 && j >= 0 && j < 2 && k >= 0 && k < 2; }
 } // namespace vmgm
 
+// vmg_gridboundary (vmg.F90:708-724) of the blocks [b0, b0+nb) in one launch: the parent-level
+// faces' passes x, y, z (linear, the formulas of k_ghost's parent branch) and then, with fill0, the
+// zero on physical faces.  A parent-level pass reads only the coarser level and the block's own
+// cells, never a same-level neighbour, so each block runs its passes on its own: 32 threads per
+// block (the two face slabs of the pass direction), a workgroup barrier between passes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_parent_fill(T *A, int b0, int nb, int fill0, const int *nbr, const int *nkind,
+                                                     const int *octant)
+{
+    using B = Blk<2>;
+    constexpr int N = 2, W = B::W, H = 1;
+    const int bl = (int)threadIdx.x >> 5, idx = (int)threadIdx.x & 31;
+    const int b = b0 + (int)blockIdx.x * 8 + bl;
+    const bool mine = b < b0 + nb;
+    const int lr = idx >> 4, t1 = (idx & 3) - 1, t2 = ((idx >> 2) & 3) - 1;
+    T *ud = A + (long)(mine ? b : b0) * B::LEN;
+    for (int d = 0; d < 3; ++d) {
+        const int f = 2 * d + lr;
+        if (mine && nkind[b * 6 + f] == NBR_PARENT) {
+            const T *us = A + (long)nbr[b * 6 + f] * B::LEN;
+            const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+            const int o = octant[b];
+            const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
+            const int an = lr ? 0 : N - 1;
+            auto buf = [&](int c1, int c2) { return us[B::rot(d, an, c1 + e1, c2 + e2)]; };
+            const int c1 = ijkc(t1), c2 = ijkc(t2);
+            const int c1n = near_(t1, c1), c2n = near_(t2, c2);
+            T S = (T)9.0 * buf(c1, c2) + buf(c1n, c2n);
+            S = S + (T)3.0 * buf(c1, c2n);
+            S = S + (T)3.0 * buf(c1n, c2);
+            const T uH = S / (T)16.0;
+            const int ag = lr ? N : -1, inner = lr ? -1 : 1;
+            const T u0 = ud[B::rot(d, ag + inner, t1, t2)];
+            const T uN = ud[B::rot(d, ag + 2 * inner, t1, t2)];
+            T q = (T)10.0 * u0 + (T)8.0 * uH;
+            q = q - (T)3.0 * uN;
+            ud[B::rot(d, ag, t1, t2)] = q / (T)15.0;
+        }
+        __syncthreads();
+    }
+    if (!fill0 || !mine) return;
+    for (int g = idx; g < 6 * W * W; g += 32) {
+        const int f = g / (W * W), r = g % (W * W);
+        if (nkind[b * 6 + f] == NBR_NONE) ud[B::rot(f >> 1, (f & 1) ? N : -1, r % W - 1, r / W - 1)] = (T)0;
+    }
+}
+
 // fmg_converge_c2p_lev of two arrays at once (children [b0, b0+nb) into their parents)
 template <typename T> __global__ void k_c2p_two(T *A1, T *A2, int b0, int nb, const int *parent, const int *octant)
 {
@@ -1082,30 +1129,53 @@ __global__ void k_norms_final(const double *part, int G, double *out)
     if (threadIdx.x == 0) { out[0] = m2; out[1] = m; out[2] = s; }
 }
 
-// ---- 8^3 blocks, one workgroup (256 threads) per record: the block's ghosted array is staged in
-//      LDS with coalesced loads, then one colour of fmg_poisson_relax / the residual is formed from
-//      LDS (same operand order as k_relax / k_resid).  The record load is the only load the data
-//      loads wait for; the right-hand side is fetched before the barrier. ----
-template <bool RELAX, typename T>
-__global__ void __launch_bounds__(256) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
-                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
-                                             int halo)
+// ---- 8^3 blocks, one workgroup of NT threads (64, 128 or 256) per record: the block's ghosted
+//      array is staged in LDS with coalesced loads, then one colour of fmg_poisson_relax / the
+//      residual is formed from LDS (same operand order as k_relax / k_resid).  The record load is
+//      the only load the data loads wait for; the right-hand side is fetched before the barrier.
+//      Fewer threads per block let a CU hold more blocks at once (the kernel waits on load latency
+//      more than on bandwidth); the AMR autotuner (amr.hip) picks NT per hierarchy. ----
+__device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
+{
+    // select chain: a runtime index into the record's array would put it in scratch memory
+    int m = R.nb[0];
+    m = f == 1 ? R.nb[1] : m;
+    m = f == 2 ? R.nb[2] : m;
+    m = f == 3 ? R.nb[3] : m;
+    m = f == 4 ? R.nb[4] : m;
+    m = f == 5 ? R.nb[5] : m;
+    return m;
+}
+
+template <bool RELAX, typename T, int NT>
+__global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
+                                            LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
+                                            int halo)
 {
     using B = Blk<8>;
+    constexpr int NC = RELAX ? 256 / NT : 512 / NT;   // cells per thread
     __shared__ T su[B::LEN];
     BlockRec R;
     R.load(rec, blockIdx.x);
     const T *ug = Au + (long)R.b * B::LEN;
-    for (int q = threadIdx.x; q < B::LEN; q += 256) su[q] = ug[q];
-    // this thread's cells: one of the colour (relax) or two (residual)
-    const int ih = threadIdx.x & 3, j = (threadIdx.x >> 2) & 7, k = threadIdx.x >> 5;
-    const int i = 2 * ih + ((j + k + ipass) & 1);
-    const long q0 = (long)R.b * B::LEN + B::gi(i, j, k);
-    const long q1 = (long)R.b * B::LEN + B::gi(threadIdx.x & 7, (threadIdx.x >> 3) & 7, threadIdx.x >> 6);
-    const long q2 = q1 + B::gi(0, 0, 4) - B::gi(0, 0, 0);
-    T rhs0 = 0, rhs1 = 0;
-    if (RELAX) rhs0 = Arhs[q0];
-    else { rhs0 = Arhs[q1]; rhs1 = Arhs[q2]; }
+    for (int q = threadIdx.x; q < B::LEN; q += NT) su[q] = ug[q];
+    // this thread's cells: colour cells c = threadIdx.x + r NT (relax) or cells c (residual)
+    int ci[NC], cj[NC], ck[NC];
+    T rhs[NC];
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        const int c = threadIdx.x + r * NT;
+        if (RELAX) {
+            cj[r] = (c >> 2) & 7;
+            ck[r] = c >> 5;
+            ci[r] = 2 * (c & 3) + ((cj[r] + ck[r] + ipass) & 1);
+        } else {
+            ci[r] = c & 7;
+            cj[r] = (c >> 3) & 7;
+            ck[r] = c >> 6;
+        }
+        rhs[r] = Arhs[(long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r])];
+    }
     if (halo) {
         // halo-free form: the same-level faces' ghost cells taken from the neighbours' boundary
         // layers (what fmg_ghfix_samelev would copy there); only the face cells the 7-point
@@ -1114,9 +1184,9 @@ __global__ void __launch_bounds__(256) k_blk8(T *Aout, T *Au, const T *Arhs, con
         // The barrier orders these LDS writes after the block copy above, which writes the same
         // ghost cells with their stored values.
         __syncthreads();
-        for (int q = threadIdx.x; q < 6 * 64; q += 256) {
+        for (int q = threadIdx.x; q < 6 * 64; q += NT) {
             const int f = q >> 6, t1 = q & 7, t2 = (q >> 3) & 7, d = f >> 1, lr = f & 1;
-            const int m = R.nb[f];
+            const int m = rec_nb(R, f);
             if (m < 0) continue;
             su[B::rot(d, lr ? 8 : -1, t1, t2)] = Au[(long)m * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
         }
@@ -1125,16 +1195,17 @@ __global__ void __launch_bounds__(256) k_blk8(T *Aout, T *Au, const T *Arhs, con
     const T h = (T)h_lev.v[R.l], hi = (T)hi_lev.v[R.l];
     FluxR<8, T> F{su, nullptr, ov, hi, R};
     double a = 0.0;
-    if (RELAX) {
-        T r = F.div(i, j, k) - rhs0 * h;
-        T resh2 = r * h;
-        Au[q0] = su[B::gi(i, j, k)] + (T)(1.0 / 6.0) * resh2;
-        a = (double)fabs(resh2);
-    } else {
-        const int c = threadIdx.x;
-        Aout[q1] = rhs0 - F.div(c & 7, (c >> 3) & 7, c >> 6) * hi;
-        const int c2 = c + 256;
-        Aout[q2] = rhs1 - F.div(c2 & 7, (c2 >> 3) & 7, c2 >> 6) * hi;
+#pragma unroll
+    for (int r = 0; r < NC; ++r) {
+        const long q = (long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r]);
+        if (RELAX) {
+            T res = F.div(ci[r], cj[r], ck[r]) - rhs[r] * h;
+            T resh2 = res * h;
+            Au[q] = su[B::gi(ci[r], cj[r], ck[r])] + (T)(1.0 / 6.0) * resh2;
+            a = fmax(a, (double)fabs(resh2));
+        } else {
+            Aout[q] = rhs[r] - F.div(ci[r], cj[r], ck[r]) * hi;
+        }
     }
     if (RELAX && resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
 }
