@@ -1,13 +1,22 @@
 """Benchmark of the self-gravity Poisson solve on one MI355X.
 
-Workload: periodic 256^3 unit box, clumpy density (1 + 8 seeded Gaussian clumps, sigma = 4 cells,
-lognormal amplitudes, mean subtracted; G = 1/(4 pi)), solved to relative residual
-max|f - L phi| / max|f| < 1e-10.  One step = one full solve from phi = 0 with the input already
-resident in HBM.  Reported value: ms per solve of the mixed-precision solver (FP32 V-cycles inside
-FP64 iterative refinement, hipGraph-replayed); the plain FP64 GPU solve is timed beside it, and the
-CPU oracle (1 core) on the same input as the CPU baseline.
+Headline workload (the metric's "mixed-prec+batched" configuration): the block-structured AMR
+hierarchy of the reference's fmg_poisson path -- 16^3 base blocks of 8^3 cells (128^3) plus four
+nested refined levels, each covering the central half-width of its parent (20 480 blocks,
+9 437 184 leaf cells), singular isothermal sphere source with a one-cell core and 5 % seeded
+lognormal perturbations, multipole (sphere) boundary -- solved to relative residual
+max|f - L phi| / max|f| < 1e-10.  One step = one full solve from phi = 0, inputs resident in HBM.
+`value` is ms per solve of the mixed-precision solver (FP32 inner fmg_lin, FP64 outer residual and
+update, hipGraph-replayed outer iteration); the plain FP64 GPU solve is timed beside it and the
+CPU oracle (FP64, 1 core) on the same input is the CPU baseline.  The residual is reported in the
+survey's three norms (L-inf, the reference's h^2-weighted L-inf of fmg.F90:1017-1039, volume-
+weighted L2).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+The other configurations run as sub-objects: "uniform" (periodic 256^3 clumpy density, the
+mg.F90 path), "batch" (48 uneven periodic problems), "adjoint" (gradient of a sampled-force loss on
+a periodic 128^3 grid) and "c0" (the periodic 64^3 CPU-runnable case).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only SECTION]
 """
 import argparse
 import json
@@ -26,18 +35,20 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=5, help="timed AMR solves (the headline steps)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256, help="uniform-grid size of the 'uniform' section")
+    ap.add_argument("--uniform-steps", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baselines")
     ap.add_argument("--smoother-reps", type=int, default=50)
-    ap.add_argument("--no-amr", action="store_true", help="skip the AMR hierarchy solve")
-    ap.add_argument("--amr-steps", type=int, default=3)
+    ap.add_argument("--no-uniform", action="store_true", help="skip the periodic 256^3 section")
     ap.add_argument("--no-batch", action="store_true", help="skip the 48-problem uneven batch")
+    ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and 64^3 sections")
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--section", choices=["amr", "batch"], default=None,
-                    help="run only the AMR or the batch section and print its JSON (profiling runs)")
+    ap.add_argument("--section", choices=["amr", "batch", "uniform", "adjoint", "c0"], default=None,
+                    help="run only one section and print its JSON (profiling runs)")
+    ap.add_argument("--amr-steps", type=int, default=None, help="alias of --steps for the AMR section")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (reference reading for HBM counters)")
     return ap.parse_args()
@@ -72,16 +83,34 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+AMR_WORKLOAD = ("sphere AMR: 128^3 base (16^3 blocks of 8^3), 4 nested levels each refining the central "
+                "half-width of its parent (20 480 blocks, 9 437 184 leaf cells), singular isothermal sphere "
+                "(1-cell core, 5% lognormal), multipole BC, rel. residual 1e-10")
+
+
+def _profile_traffic(name, kernel):
+    """Scaled HBM bytes per launch of `kernel` from a committed rocprofv3 counter summary
+    (lines "traffic_bytes_per_launch[<kernel>]: <bytes>")."""
+    prof = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(prof):
+        return None
+    key = f"traffic_bytes_per_launch[{kernel}]:"
+    for line in open(prof):
+        if line.startswith(key):
+            return int(line[len(key):])
+    return None
+
+
 def bench_amr(a, dev):
-    """Sphere hierarchy: 16^3 base blocks (128^3 cells), four nested levels each refining the
-    central half-width of its parent (5 x 4096 blocks of 8^3, 9 437 184 leaf cells), singular
-    isothermal sphere with a one-cell core and 5 % lognormal perturbations, multipole boundary."""
+    """The headline: the sphere hierarchy solved by the plain FP64 solver and by the mixed solver;
+    K timed mixed solves bracketed by device synchronisation."""
     import numpy as np
     import torch
     from code_only_amd import amr_hierarchy as ah
     from code_only_amd import problems
-    from code_only_amd.amr import AmrSolver
+    from code_only_amd.amr import SG_PREC_MIXED, AmrSolver
 
+    steps = a.amr_steps if a.amr_steps is not None else a.steps
     hier = ah.nested(base=16, levels=5, bc=ah.SPHERE)
     src = problems.sis_source(hier, seed=1)
     solver = AmrSolver(hier)
@@ -89,51 +118,61 @@ def bench_amr(a, dev):
     g_t = torch.zeros((hier.nblk, 3, 8, 8, 8), dtype=torch.float64, device=dev)
     p_t = torch.zeros_like(s_t)
     log(f"[bench] AMR sphere hierarchy: {hier.nblk} blocks, {hier.cells()} leaf cells, plain FP64")
-    st = None
-    for _ in range(1):
-        p_t.zero_()
-        st = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30)
-    times = []
-    for _ in range(max(1, a.amr_steps)):
-        p_t.zero_()
+
+    def run(prec, k, w):
+        st = None
+        for _ in range(max(1, w)):
+            p_t.zero_()
+            st = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30, precision=prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30)
+        for _ in range(k):
+            p_t.zero_()
+            st = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30, precision=prec)
         torch.cuda.synchronize()
-        times.append((time.perf_counter() - t0) * 1e3)
-    ms64 = float(np.median(times))
+        return (time.perf_counter() - t0) * 1e3 / k, st
+
+    ms64, st = run(1, max(1, min(steps, 3)), 1)
     log(f"[bench] AMR fp64: {ms64:.2f} ms/solve, {st['iterations']} it, rel {st['rel_linf']:.3e}")
-    p64 = p_t.clone()
-    g64 = g_t.clone()
-    # mixed precision: FP32 inner fmg_lin inside FP64 defect correction, one outer iteration replayed
-    # as a hipGraph
-    from code_only_amd.amr import SG_PREC_MIXED
-    stm = None
-    for _ in range(1):
-        p_t.zero_()
-        stm = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30, precision=SG_PREC_MIXED)
-    times = []
-    for _ in range(max(1, a.amr_steps)):
-        p_t.zero_()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        stm = solver.solve(p_t, s_t, g_t, tol=a.tol, maxit=30, precision=SG_PREC_MIXED)
-        torch.cuda.synchronize()
-        times.append((time.perf_counter() - t0) * 1e3)
-    msmx = float(np.median(times))
+    p64, g64 = p_t.clone(), g_t.clone()
+    msmx, stm = run(SG_PREC_MIXED, max(1, steps), a.warmup)
     err_p = ((p_t - p64).norm() / p64.norm()).item()
     err_g = ((g_t - g64).norm() / g64.norm()).item()
     log(f"[bench] AMR mixed: {msmx:.2f} ms/solve, {stm['iterations']} it, rel {stm['rel_linf']:.3e}, "
         f"psi err vs fp64 {err_p:.2e}")
-    out = {"workload": "sphere AMR: 128^3 base (16^3 blocks of 8^3), 4 nested levels of the central "
-                       "half-width, singular isothermal sphere (1-cell core, 5% lognormal), multipole BC",
-           "blocks": hier.nblk, "leaf_cells": hier.cells(), "fp64_gpu_ms": round(ms64, 3),
-           "fp64_iterations": st["iterations"], "fp64_rel_linf": st["rel_linf"], "fp64_rel_l2": st["rel_l2"],
-           "fp64_history": st["hist"],
-           "mixed_gpu_ms": round(msmx, 3), "mixed_iterations": stm["iterations"], "mixed_rel_linf": stm["rel_linf"],
-           "mixed_rel_l2": stm["rel_l2"], "mixed_history": stm["hist"], "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
-           "mixed_vs_fp64_gpu": {"psi_rel_l2": err_p, "g_rel_l2": err_g}, "autotune": solver.tile_info(),
-           "context": solver.info()}
+    info = solver.info()
+    # dominant kernel: the FMG-level-0 relaxation of the mixed solve (both colours of one sweep),
+    # HIP events on the library's stream; algorithmic bytes of a sweep = read u + read f + write u
+    # = 3 x 4 B per leaf cell (FP32)
+    ms_sweep = solver.time_smoother(SG_PREC_MIXED, a.smoother_reps)
+    tiles = solver.tile_info()
+    nt = tiles["relax_fp32"]["threads_per_block"]
+    sweep_bytes = 3 * 4 * hier.cells()
+    achieved = sweep_bytes / (ms_sweep * 1e-3) / 1e9
+    kname = f"sg::amr::k_blk8<true, float, {nt}>"
+    per_launch = _profile_traffic("amr_c2_hbm_traffic.md", kname)
+    traffic = 2 * per_launch if per_launch else None
+    log(f"[bench] AMR smoother sweep {ms_sweep * 1e3:.1f} us -> {achieved:.0f} GB/s algorithmic")
+    ms64_sweep = solver.time_smoother(1, max(5, a.smoother_reps // 5))
+    out = {
+        "workload": AMR_WORKLOAD, "blocks": hier.nblk, "leaf_cells": hier.cells(),
+        "mixed_gpu_ms": round(msmx, 3), "fp64_gpu_ms": round(ms64, 3), "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
+        "iterations": {"mixed": stm["iterations"], "fp64": st["iterations"]},
+        "rel_residual": {"mixed_linf": stm["rel_linf"], "mixed_h2_linf": stm["rel_h2"], "mixed_l2": stm["rel_l2"],
+                         "fp64_linf": st["rel_linf"], "fp64_h2_linf": st["rel_h2"], "fp64_l2": st["rel_l2"]},
+        "history": {"mixed": stm["hist"], "fp64": st["hist"]},
+        "mixed_vs_fp64_gpu": {"psi_rel_l2": err_p, "g_rel_l2": err_g},
+        "kernel_nodes_per_outer_iteration": info["graph_nodes"],
+        "fused": {"fmg_levels": info["fused_levels"], "vmg_gather_levels": info["vmg_fill_levels"]},
+        "autotune": tiles,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kname + " (FMG level 0 relaxation, one sweep = both colour launches)",
+                     "algorithmic_bytes_per_launch": sweep_bytes, "ms_per_launch": ms_sweep,
+                     "hbm_rate_gbs": round(traffic / (ms_sweep * 1e-3) / 1e9, 1) if traffic else None,
+                     "fp64_ms_per_sweep": ms64_sweep},
+    }
+    cpu = None
     if not a.no_cpu:
         import oracle
         log("[bench] AMR CPU oracle (FP64, 1 core) on the same hierarchy")
@@ -143,17 +182,18 @@ def bench_amr(a, dev):
         cpu_ms = (time.perf_counter() - t0) * 1e3
         pg, gg = p64.cpu().numpy(), g64.cpu().numpy()
         pm, gm = p_t.cpu().numpy(), g_t.cpu().numpy()
-        out["cpu_oracle_ms"] = round(cpu_ms, 1)
-        out["cpu_oracle_iterations"] = so["iterations"]
         out["fp64_gpu_vs_oracle"] = {"psi_rel_l2": float(np.linalg.norm(pg - po) / np.linalg.norm(po)),
                                      "g_rel_l2": float(np.linalg.norm(gg - go_) / np.linalg.norm(go_)),
                                      "bitwise": bool(np.array_equal(pg, po) and np.array_equal(gg, go_))}
         out["mixed_gpu_vs_oracle"] = {"psi_rel_l2": float(np.linalg.norm(pm - po) / np.linalg.norm(po)),
                                       "g_rel_l2": float(np.linalg.norm(gm - go_) / np.linalg.norm(go_))}
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms/solve", "cores": 1, "kind": "port",
+               "sample": f"one full solve of the same sphere hierarchy to rel. residual {a.tol:g} "
+                         f"({so['iterations']} outer iterations) by the FP64 C oracle on 1 core of {cpu_model()}"}
         log(f"[bench] AMR cpu oracle {cpu_ms:.0f} ms; parity {out['fp64_gpu_vs_oracle']}")
         o.close()
     solver.close()
-    return out
+    return out, cpu
 
 
 def bench_batch(a, dev):
@@ -227,16 +267,9 @@ def bench_batch(a, dev):
     return out
 
 
-def main():
-    a = parse()
-    if a.calibrate_copy:
-        calibrate_copy(a.calibrate_copy)
-        return
-    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build's solve runs on a single MI355X "
-              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def bench_uniform(a, dev):
+    """Periodic 256^3 clumpy density (the mg.F90 path): plain FP64 against the mixed solver, the
+    fine-level sweep's roofline, the autotuner's tile timings, the CPU oracle."""
     import numpy as np
     import torch
 
@@ -244,11 +277,6 @@ def main():
     from code_only_amd.uniform import SG_PREC_FP64, SG_PREC_MIXED, UniformSolver
     from code_only_amd import _lib
 
-    dev = torch.device("cuda:0")
-    if a.section:
-        res = bench_amr(a, dev) if a.section == "amr" else bench_batch(a, dev)
-        print(json.dumps({"section": a.section, **res}), flush=True)
-        return
     n = a.n
     h = 1.0 / n
     log(f"[bench] generating clumpy rhs n={n}")
@@ -272,10 +300,10 @@ def main():
         return ms, st, psi
 
     log("[bench] plain FP64 GPU solve")
-    ms64, st64, psi64 = solve(SG_PREC_FP64, max(1, a.steps // 2), 1)
+    ms64, st64, psi64 = solve(SG_PREC_FP64, max(1, a.uniform_steps // 2), 1)
     log(f"[bench] fp64: {ms64:.3f} ms/solve, {st64['iterations']} it, rel {st64['rel_linf']:.3e}")
     log("[bench] mixed-precision GPU solve")
-    msmx, stmx, psimx = solve(SG_PREC_MIXED, a.steps, a.warmup)
+    msmx, stmx, psimx = solve(SG_PREC_MIXED, a.uniform_steps, 2)
     log(f"[bench] mixed: {msmx:.3f} ms/solve, {stmx['iterations']} it, rel {stmx['rel_linf']:.3e}")
 
     # dominant kernel: the fine-level smoother of the mixed solve, timed with HIP events on the
@@ -366,25 +394,12 @@ def main():
         log(f"[bench] cpu oracle {cpu_ms:.0f} ms, mixed vs oracle rel L2 {err_vs_oracle:.2e}")
 
     out = {
-        "metric": METRIC,
-        "value": round(msmx, 4),
-        "unit": "ms/solve",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(msmx, 4),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 inner + f64 outer",
-        "data": "synthetic (seeded clumpy density)",
-        "config": {"workload": f"periodic {n}^3 clumpy-density Poisson solve to rel. residual {a.tol:g}",
-                   "n": n, "clumps": 8, "sigma_cells": 4, "seed": 0},
-        "fp64_gpu_ms": round(ms64, 4),
-        "speedup_vs_fp64_gpu": round(ms64 / msmx, 3),
+        "workload": f"periodic {n}^3 clumpy-density Poisson solve to rel. residual {a.tol:g}",
+        "n": n, "clumps": 8, "sigma_cells": 4, "seed": 0,
+        "mixed_gpu_ms": round(msmx, 4), "fp64_gpu_ms": round(ms64, 4),
+        "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
         "iterations": {"mixed": stmx["iterations"], "fp64": st64["iterations"]},
-        "rel_residual": {"mixed_linf": stmx["rel_linf"], "mixed_l2": stmx["rel_l2"],
-                         "fp64_linf": st64["rel_linf"]},
+        "rel_residual": {"mixed_linf": stmx["rel_linf"], "mixed_l2": stmx["rel_l2"], "fp64_linf": st64["rel_linf"]},
         "phi_err_vs_fp64_gpu": err_mx_vs_64,
         "phi_err_vs_cpu_oracle": err_vs_oracle,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -395,15 +410,148 @@ def main():
                      "cache_cold": {"achieved": round(achieved_cold, 1), "frac": round(achieved_cold / HBM_PEAK_GBS, 4),
                                     "ms_per_launch": ms_cold,
                                     "note": "Infinity Cache evicted by a 1 GiB fill before each sweep, median"}},
-        "cpu_baseline": cpu,
+        "cpu_oracle": cpu,
         "autotune": tiles,
     }
-    if not a.no_amr:
-        out["amr"] = bench_amr(a, dev)
+    solver.close()
+    return out
+
+
+def bench_adjoint(a, dev):
+    """Gradient of L = sum_p w_p . grad_h phi(c_p) (4096 seeded cells, w ~ N(0,1)^3) with respect to
+    rho on a periodic 128^3 grid: forward solve, then the backward pass (the hand-written adjoint
+    solve and the transpose of the sampled gradient), both in mixed precision."""
+    import numpy as np
+    import torch
+    from code_only_amd import problems
+    from code_only_amd.adjoint import sampled_gradient_loss
+    from code_only_amd.uniform import SG_PREC_FP64, SG_PREC_MIXED, UniformSolver
+
+    n = 128
+    rng = np.random.default_rng(4)
+    rho0 = problems.clumpy_density(n, seed=4, nclump=8)
+    cells = torch.tensor(rng.integers(0, n ** 3, size=4096).astype(np.int32), device=dev)
+    w = torch.tensor(rng.standard_normal((4096, 3)), device=dev)
+    s = UniformSolver(n, precisions=SG_PREC_FP64 | SG_PREC_MIXED)
+
+    def one(prec):
+        rho = torch.tensor(rho0, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = sampled_gradient_loss(rho, s, cells, w, tol=a.tol, precision=prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return (t1 - t0) * 1e3, (t2 - t1) * 1e3, rho.grad
+
+    one(SG_PREC_MIXED)
+    fw, bw, gm = [], [], None
+    for _ in range(3):
+        f, b, gm = one(SG_PREC_MIXED)
+        fw.append(f)
+        bw.append(b)
+    f64, b64, g64 = one(SG_PREC_FP64)
+    err = ((gm - g64).norm() / g64.norm()).item()
+    s.close()
+    out = {"workload": "periodic 128^3, L = sum over 4096 seeded cells of w . grad phi, dL/drho by the "
+                       "hand-written adjoint", "mixed_forward_ms": round(float(np.median(fw)), 3),
+           "mixed_adjoint_ms": round(float(np.median(bw)), 3), "fp64_forward_ms": round(f64, 3),
+           "fp64_adjoint_ms": round(b64, 3), "grad_mixed_vs_fp64_rel_l2": err}
+    log(f"[bench] adjoint 128^3: forward {out['mixed_forward_ms']} ms, adjoint {out['mixed_adjoint_ms']} ms (mixed)")
+    return out
+
+
+def bench_c0(a, dev):
+    """The periodic 64^3 case of the mg.F90 path: CPU oracle (FP64, 1 core), plain FP64 GPU and mixed."""
+    import numpy as np
+    import torch
+    from code_only_amd import problems
+    from code_only_amd.uniform import SG_PREC_FP64, SG_PREC_MIXED, UniformSolver
+
+    n = 64
+    f = problems.clumpy_rhs(n, seed=0)
+    ft = torch.tensor(f, device=dev)
+    s = UniformSolver(n, 1.0 / n, precisions=SG_PREC_FP64 | SG_PREC_MIXED)
+    res = {}
+    for name, prec in (("fp64_gpu_ms", SG_PREC_FP64), ("mixed_gpu_ms", SG_PREC_MIXED)):
+        psi = torch.zeros_like(ft)
+        st = s.cycle(psi, ft, tol=a.tol, maxit=60, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            psi.zero_()
+            st = s.cycle(psi, ft, tol=a.tol, maxit=60, precision=prec)
+        torch.cuda.synchronize()
+        res[name] = round((time.perf_counter() - t0) * 1e3 / 5, 4)
+        res[name.replace("_ms", "_rel_linf")] = st["rel_linf"]
+    s.close()
+    res["workload"] = "periodic 64^3 clumpy-density Poisson solve to rel. residual 1e-10"
+    if not a.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        _, so = oracle.mg_cycle(f, 1.0 / n, tol=a.tol, maxit=60)
+        res["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        res["cpu_oracle_iterations"] = so["iterations"]
+    log(f"[bench] 64^3: {res}")
+    return res
+
+
+def main():
+    a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build's solve runs on a single MI355X "
+              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    dev = torch.device("cuda:0")
+    if a.section:
+        if a.section == "amr":
+            res, cpu = bench_amr(a, dev)
+            res["cpu_baseline"] = cpu
+        else:
+            res = {"batch": bench_batch, "uniform": bench_uniform, "adjoint": bench_adjoint, "c0": bench_c0}[a.section](a, dev)
+        print(json.dumps({"section": a.section, **res}), flush=True)
+        return
+    amr, cpu = bench_amr(a, dev)
+    steps = a.amr_steps if a.amr_steps is not None else a.steps
+    out = {
+        "metric": METRIC,
+        "value": amr["mixed_gpu_ms"],
+        "unit": "ms/solve",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": amr["mixed_gpu_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner + f64 outer",
+        "data": "synthetic (seeded singular isothermal sphere with lognormal perturbations)",
+        "config": {"workload": AMR_WORKLOAD, "blocks": amr["blocks"], "leaf_cells": amr["leaf_cells"],
+                   "base_blocks": 4096, "levels": 5, "boundary": "sphere multipole", "tol": a.tol},
+        "fp64_gpu_ms": amr["fp64_gpu_ms"],
+        "speedup_vs_fp64_gpu": amr["speedup_mixed_vs_fp64"],
+        "iterations": amr["iterations"],
+        "rel_residual": amr["rel_residual"],
+        "roofline": amr["roofline"],
+        "cpu_baseline": cpu,
+        "amr": {k: v for k, v in amr.items() if k not in ("roofline",)},
+    }
+    if not a.no_uniform:
+        out["uniform"] = bench_uniform(a, dev)
     if not a.no_batch:
         out["batch"] = bench_batch(a, dev)
+    if not a.no_extra:
+        out["adjoint"] = bench_adjoint(a, dev)
+        out["c0"] = bench_c0(a, dev)
     print(json.dumps(out), flush=True)
-    solver.close()
 
 
 if __name__ == "__main__":

@@ -84,6 +84,13 @@ class AmrSolver:
                          "ms_per_launch": {str(nt): round(ms[q], 5) for q, nt in enumerate((64, 128, 256)) if ms[q] > 0}}
         return out
 
+    def time_smoother(self, precision=SG_PREC_MIXED, reps=20):
+        """ms per full red-black sweep (both colours) of the FMG-level-0 relaxation, HIP events on
+        the library's stream."""
+        ms = ctypes.c_double()
+        _lib.check(self._lib.sg_amr_time_smoother(self._ctx, precision, reps, ctypes.byref(ms)))
+        return ms.value
+
     def solve(self, psi, src, g=None, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64,
               use_graph=True, raise_on_fail=True, fused=True):
         """fmg_poisson on torch float64 GPU tensors [nblk, 8, 8, 8]; psi in place, g [nblk, 3, 8, 8, 8].

@@ -90,6 +90,8 @@ struct sg_amr {
     // [0] relax FP32, [1] residual FP32, [2] relax FP64, [3] residual FP64; candidate timings in ms
     int blk8_nt[4] = {256, 256, 256, 256};
     float blk8_ms[4][3] = {};
+    // FMG-level-0 face store of U (FP32, fused mode): 6 x 64 values per block (amr_kernels.h)
+    float *fb0 = nullptr;
     int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
@@ -272,6 +274,17 @@ template <typename T> int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, r
 // fused refresh computes the same zeros the skipped refresh would leave.  Physical faces keep
 // their stored values (zero in the inner solve, whose every refresh zeroes them; the multipole
 // values in the outer residual, which refreshes without zeroing).
+inline bool fused(const sg_amr &s, int L) { return s.fused_on && L < kNfmg - 1 && s.fused_ok[L]; }
+
+// the level-0 face store of U applies in the inner (FP32) solve's fused mode
+template <typename T> inline T *face_store(sg_amr &s, int L)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        if (L == 0 && fused(s, 0)) return s.fb0;
+    }
+    return nullptr;
+}
+
 template <int N, typename T> int prep_n(sg_amr &s, int L, int ju)
 {
     constexpr int H = N / 2;
@@ -279,11 +292,11 @@ template <int N, typename T> int prep_n(sg_amr &s, int L, int ju)
     const long work = (long)ncov + (long)ncf * H * H;
     LevelScalars hil{};
     for (int l = 0; l < kMaxLev; ++l) hil.v[l] = s.hil[L][l];
+    T *fb = ju == C_U ? face_store<T>(s, L) : nullptr;
     return go(s, k_prep<N, T>, work, ar<T>(s).A[L][ju], ar<T>(s).ov[L], (const int2 *)s.d_prep_cov[L], ncov,
-              (const int4 *)s.d_cfrec, (const int *)s.d_prep_src[L], ncf, hil);
+              (const int4 *)s.d_cfrec, (const int *)s.d_prep_src[L], ncf, hil, fb, (const int *)s.d_covf);
 }
 
-inline bool fused(const sg_amr &s, int L) { return s.fused_on && L < kNfmg - 1 && s.fused_ok[L]; }
 
 template <typename T> int poisson_flux(sg_amr &s, int L, int ju, int fill0flag)
 {
@@ -320,20 +333,23 @@ int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *rec, int nrec, int
 constexpr int kBlk8Nt[3] = {64, 128, 256};
 
 // one launch of k_blk8 over the active blocks with nt threads per block
+// (fb: the face store of Au, read for the neighbours' faces and, by the relaxation, written back)
 template <bool RELAX, typename T>
-int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo)
+int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo,
+         T *fb = nullptr)
 {
     const dim3 g((unsigned)s.active.size());
     const int *rec = s.d_arec;
+    T *fbout = RELAX ? fb : nullptr;
     if (nt == 64)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 64>), g, dim3(64), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0],
-                           ov, resmax, halo);
+                           ov, resmax, halo, (const T *)fb, fbout);
     else if (nt == 128)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 128>), g, dim3(128), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout);
     else
         hipLaunchKernelGGL((k_blk8<RELAX, T, 256>), g, dim3(256), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
@@ -348,6 +364,8 @@ template <typename T> int tune_blk8(sg_amr &s)
     if (s.active.empty()) return SG_OK;
     auto &R = ar<T>(s);
     const int halo = s.fused_ok[0];
+    T *fb = nullptr;
+    if constexpr (std::is_same<T, float>::value) fb = halo ? s.fb0 : nullptr;
     for (int which = 0; which < 2; ++which) {
         float best = 1e30f;
         for (int c = 0; c < 3; ++c) {
@@ -355,9 +373,11 @@ template <typename T> int tune_blk8(sg_amr &s)
             for (int rep = 0; rep < 5; ++rep) {
                 if (rep == 1) SG_HIP(hipEventRecord(s.ev0, s.stream));
                 if (which == 0)
-                    SG_TRY((blk8<true, T>(s, nt, (T *)nullptr, R.A[0][C_U], R.A[0][C_RHS], 1 + (rep & 1), R.ov[0], nullptr, halo)));
+                    SG_TRY((blk8<true, T>(s, nt, (T *)nullptr, R.A[0][C_U], R.A[0][C_RHS], 1 + (rep & 1), R.ov[0], nullptr, halo,
+                                          fb)));
                 else
-                    SG_TRY((blk8<false, T>(s, nt, R.A[0][C_RES], R.A[0][C_U], R.A[0][C_RHS], 0, R.ov[0], nullptr, halo)));
+                    SG_TRY((blk8<false, T>(s, nt, R.A[0][C_RES], R.A[0][C_U], R.A[0][C_RHS], 0, R.ov[0], nullptr, halo,
+                                           fb)));
             }
             SG_HIP(hipEventRecord(s.ev1, s.stream));
             SG_HIP(hipEventSynchronize(s.ev1));
@@ -398,7 +418,7 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
         const bool hf = fused(s, L);
         if (L == 0) {
             SG_TRY((blk8<true, T>(s, s.blk8_nt[kBlk8Slot<T>], (T *)nullptr, ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], ipass,
-                                  ar<T>(s).ov[0], slots, hf ? 1 : 0)));
+                                  ar<T>(s).ov[0], slots, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr)));
         } else {
             SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_arec, (int)s.active.size(), ipass, true, slots, hf));
         }
@@ -425,7 +445,7 @@ template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int 
     if (L == 0) {
         const bool hf = fused(s, 0);
         return blk8<false, T>(s, s.blk8_nt[kBlk8Slot<T> + 1], ar<T>(s).A[0][jres], ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], 0,
-                              ar<T>(s).ov[0], nullptr, hf ? 1 : 0);
+                              ar<T>(s).ov[0], nullptr, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr);
     }
     SG_NDISPATCH(L, resid_n, s, L, jres, ju, jrhs);
 }
@@ -449,7 +469,7 @@ template <typename T> int interp_blocks(sg_amr &s, int Lf, int iuf, int iuc, int
         // tricubic: one workgroup per block, the separable passes staged in LDS
         if (Lf == 0)
             hipLaunchKernelGGL((k_interp_cubic_blk<8, T>), dim3(na), dim3(256), 0, s.stream, R.A[0][iuf],
-                               (const T *)R.A[1][iuc], al, na);
+                               (const T *)R.A[1][iuc], al, na, iuf == C_U ? face_store<T>(s, 0) : (T *)nullptr);
         else
             hipLaunchKernelGGL((k_interp_cubic_blk<4, T>), dim3(na), dim3(256), 0, s.stream, R.A[1][iuf],
                                (const T *)R.A[2][iuc], al, na);
@@ -496,6 +516,13 @@ template <typename T> int fmg_add(sg_amr &s, int L, int a, int da)
 template <typename T> int fmg_addint(sg_amr &s, int Lf, int juf, int juc, int jres)
 {
     SG_TRY(fmg_interp<T>(s, Lf, jres, juc, 0));
+    T *fb = juf == C_U ? face_store<T>(s, Lf) : nullptr;
+    if (fb && !s.active.empty()) {
+        hipLaunchKernelGGL(k_add_blk8_fb<T>, dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, ar<T>(s).A[0][juf],
+                           (const T *)ar<T>(s).A[0][jres], (const int *)s.d_active, fb);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
+    }
     return fmg_add<T>(s, Lf, juf, jres);
 }
 
@@ -951,6 +978,8 @@ int ensure_mixed(sg_amr &s)
         SG_HIP(hipMalloc((void **)&s.bd_inv, sizeof(int) * inv.size()));
         SG_HIP(hipMemcpy(s.bd_inv, inv.data(), sizeof(int) * inv.size(), hipMemcpyHostToDevice));
     }
+    SG_HIP(hipMalloc((void **)&s.fb0, sizeof(float) * 6 * 64 * (size_t)s.nblk));
+    SG_HIP(hipMemset(s.fb0, 0, sizeof(float) * 6 * 64 * (size_t)s.nblk));
     s.have_f = 1;
     return tune_blk8<float>(s);
 }
@@ -1222,6 +1251,7 @@ void release(sg_amr *s)
                  s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap};
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->d_crop) (void)hipFree(s->d_crop);
+    if (s->fb0) (void)hipFree(s->fb0);
     for (int L = 0; L < kNfmg; ++L) {
         if (s->d_prep_cov[L]) (void)hipFree(s->d_prep_cov[L]);
         if (s->d_prep_src[L]) (void)hipFree(s->d_prep_src[L]);
@@ -1694,6 +1724,33 @@ int sg_amr_info(sg_amr *ctx, int *info, int n)
     const long v[5] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
                        (long)ctx->cf.size(), vmg};
     for (int q = 0; q < n; ++q) info[q] = q < 5 ? (int)v[q] : 0;
+    return SG_OK;
+}
+
+int sg_amr_time_smoother(sg_amr *ctx, int prec, int reps, double *ms_per_sweep)
+{
+    SG_REQUIRE(ctx && ms_per_sweep && reps > 0 && reps <= 100000, "sg_amr_time_smoother: bad argument");
+    SG_REQUIRE(prec == SG_PREC_FP64 || prec == SG_PREC_MIXED, "sg_amr_time_smoother: unknown precision %d", prec);
+    sg_amr &s = *ctx;
+    if (prec == SG_PREC_MIXED) SG_TRY(ensure_mixed(s));
+    // the solve's FMG-level-0 colour passes: same-level faces read directly where the mixed solve's
+    // fused mode does (FP32), from the stored ghosts in the plain FP64 path
+    const int halo = prec == SG_PREC_MIXED ? s.fused_ok[0] : 0;
+    SG_HIP(hipEventRecord(s.ev0, s.stream));
+    for (int r = 0; r < reps; ++r)
+        for (int ipass = 1; ipass <= 2; ++ipass) {
+            if (prec == SG_PREC_MIXED)
+                SG_TRY((blk8<true, float>(s, s.blk8_nt[0], (float *)nullptr, s.f.A[0][C_U], s.f.A[0][C_RHS], ipass, s.f.ov[0],
+                                          nullptr, halo, halo ? s.fb0 : nullptr)));
+            else
+                SG_TRY((blk8<true, double>(s, s.blk8_nt[2], (double *)nullptr, s.d.A[0][C_U], s.d.A[0][C_RHS], ipass,
+                                           s.d.ov[0], nullptr, 0)));
+        }
+    SG_HIP(hipEventRecord(s.ev1, s.stream));
+    SG_HIP(hipEventSynchronize(s.ev1));
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    *ms_per_sweep = (double)ms / reps;
     return SG_OK;
 }
 

@@ -33,6 +33,8 @@ template <int N> struct Blk {
     }
 };
 
+// FMG-level-0 face store (mixed solve, fused mode): face f of block b, tangential (t1, t2) as Blk::rot
+__device__ __forceinline__ long fb_at(int b, int f, int t1, int t2) { return ((long)b * 6 + f) * 64 + t1 + 8 * t2; }
 __device__ __forceinline__ int ijkc(int i) { return i >= 0 ? i / 2 : -((1 - i) / 2); }   // floor(i/2)
 __device__ __forceinline__ int near_(int f, int c) { return c + 2 * (((f % 2) + 2) % 2) - 1; }
 
@@ -391,14 +393,18 @@ template <int N, typename T> __device__ __forceinline__ T src_val(const T *A, in
 // Reads only leaf interiors and physical ghost cells, which no item writes.
 template <int N, typename T>
 __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, const int *src, int ncf,
-                       LevelScalars hil)
+                       LevelScalars hil, T *fbcov = nullptr, const int *covf = nullptr)
 {
     using B = Blk<N>;
     constexpr int H = N / 2, S = H + 2;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ncov) {
         const int2 p = cov[t];
-        A[p.x] = rstr_at<N, T>(A, p.y);
+        const T v = rstr_at<N, T>(A, p.y);
+        // the items of one covered face are its N^2 layer cells in (t2, t1) order: with a face
+        // store, the value goes to the face's slab there instead of to the block's layer
+        if (fbcov) fbcov[(long)covf[t / (N * N)] * (N * N) + t % (N * N)] = v;
+        else A[p.x] = v;
         return;
     }
     t -= ncov;
@@ -715,7 +721,7 @@ template <int NF, bool CUBIC, typename T> __global__ void k_interp(T *Af, const 
 // cell that uses it.  Every intermediate is the value tricubic_at forms, so the results are the
 // same bit for bit.
 template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cubic_blk(T *Af, const T *Ac, const int *list,
-                                                                                     int nb)
+                                                                                     int nb, T *fb = nullptr)
 {
     constexpr int NC = NF / 2, W = NC + 2;   // coarse cells -1 .. NC per axis
     static_assert(NC + 2 >= 4, "the tricubic stencil needs four coarse cells");
@@ -754,8 +760,18 @@ template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cu
     for (int q = (int)threadIdx.x; q < NF * NF * NF; q += (int)blockDim.x) {
         const int i = q % NF, j = (q / NF) % NF, k = q / (NF * NF);
         const T *p = bxy + j * NF + i;
-        uf[Blk<NF>::gi(i, j, k)] = comb4<T>(swt[k], p[sidx[k][0] * NF * NF], p[sidx[k][1] * NF * NF],
-                                            p[sidx[k][2] * NF * NF], p[sidx[k][3] * NF * NF]);
+        const T v = comb4<T>(swt[k], p[sidx[k][0] * NF * NF], p[sidx[k][1] * NF * NF], p[sidx[k][2] * NF * NF],
+                             p[sidx[k][3] * NF * NF]);
+        uf[Blk<NF>::gi(i, j, k)] = v;
+        if (NF == 8 && fb) {
+            // the face store: each boundary cell on the one to three faces it lies on
+            if (i == 0) fb[fb_at(b, 0, j, k)] = v;
+            if (i == NF - 1) fb[fb_at(b, 1, j, k)] = v;
+            if (j == 0) fb[fb_at(b, 2, i, k)] = v;
+            if (j == NF - 1) fb[fb_at(b, 3, i, k)] = v;
+            if (k == 0) fb[fb_at(b, 4, i, j)] = v;
+            if (k == NF - 1) fb[fb_at(b, 5, i, j)] = v;
+        }
     }
 }
 
@@ -1135,6 +1151,20 @@ __global__ void k_norms_final(const double *part, int G, double *out)
 //      the only load the data loads wait for; the right-hand side is fetched before the barrier.
 //      Fewer threads per block let a CU hold more blocks at once (the kernel waits on load latency
 //      more than on bandwidth); the AMR autotuner (amr.hip) picks NT per hierarchy. ----
+// ---- FMG-level-0 face store of the mixed solve's fused mode: per block and face, the block's
+//      boundary layer on that face, 64 contiguous values (t1 + 8 t2, tangential axes as Blk::rot),
+//      so that a neighbour's face is two cache lines instead of 64 scattered cells ----
+
+// the six faces of a block staged (ghosted) in LDS, into the store
+template <typename T> __device__ __forceinline__ void fb_write(T *fb, int b, const T *su, int nt)
+{
+    using B = Blk<8>;
+    for (int q = threadIdx.x; q < 6 * 64; q += nt) {
+        const int f = q >> 6, t1 = q & 7, t2 = (q >> 3) & 7;
+        fb[fb_at(b, f, t1, t2)] = su[B::rot(f >> 1, (f & 1) ? 7 : 0, t1, t2)];
+    }
+}
+
 __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
 {
     // select chain: a runtime index into the record's array would put it in scratch memory
@@ -1150,7 +1180,7 @@ __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
 template <bool RELAX, typename T, int NT>
 __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
-                                            int halo)
+                                            int halo, const T *fbin = nullptr, T *fbout = nullptr)
 {
     using B = Blk<8>;
     constexpr int NC = RELAX ? 256 / NT : 512 / NT;   // cells per thread
@@ -1184,11 +1214,13 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
         // The barrier orders these LDS writes after the block copy above, which writes the same
         // ghost cells with their stored values.
         __syncthreads();
+        // (from the face store when there is one: the neighbour's face f^1 faces this block)
         for (int q = threadIdx.x; q < 6 * 64; q += NT) {
             const int f = q >> 6, t1 = q & 7, t2 = (q >> 3) & 7, d = f >> 1, lr = f & 1;
             const int m = rec_nb(R, f);
             if (m < 0) continue;
-            su[B::rot(d, lr ? 8 : -1, t1, t2)] = Au[(long)m * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
+            su[B::rot(d, lr ? 8 : -1, t1, t2)] =
+                fbin ? fbin[fb_at(m, f ^ 1, t1, t2)] : Au[(long)m * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
         }
     }
     __syncthreads();
@@ -1201,13 +1233,38 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
         if (RELAX) {
             T res = F.div(ci[r], cj[r], ck[r]) - rhs[r] * h;
             T resh2 = res * h;
-            Au[q] = su[B::gi(ci[r], cj[r], ck[r])] + (T)(1.0 / 6.0) * resh2;
+            const T v = su[B::gi(ci[r], cj[r], ck[r])] + (T)(1.0 / 6.0) * resh2;
+            Au[q] = v;
+            // the colour's cells are read by no stencil of this pass, so LDS can take the new value
+            if (fbout) su[B::gi(ci[r], cj[r], ck[r])] = v;
             a = fmax(a, (double)fabs(resh2));
         } else {
             Aout[q] = rhs[r] - F.div(ci[r], cj[r], ck[r]) * hi;
         }
     }
     if (RELAX && resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
+    if (RELAX && fbout) {
+        __syncthreads();
+        fb_write(fbout, R.b, su, NT);
+    }
+}
+
+// fmg_add (U += D over whole arrays) of the listed 8^3 blocks, one workgroup per block, which also
+// writes the blocks' faces into the face store
+template <typename T>
+__global__ void __launch_bounds__(256) k_add_blk8_fb(T *A, const T *D, const int *list, T *fb)
+{
+    using B = Blk<8>;
+    __shared__ T su[B::LEN];
+    const int b = list[blockIdx.x];
+    const long base = (long)b * B::LEN;
+    for (int q = threadIdx.x; q < B::LEN; q += 256) {
+        const T v = A[base + q] + D[base + q];
+        A[base + q] = v;
+        su[q] = v;
+    }
+    __syncthreads();
+    fb_write(fb, b, su, 256);
 }
 
 // ---- single-workgroup helpers (blockDim.x threads, one workgroup) ----
