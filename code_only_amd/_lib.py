@@ -76,6 +76,7 @@ SIGNATURES = {
     "sg_uniform_gradient_dev": (_I, [_VP, _VP, _VP]),
     "sg_uniform_relax_dev": (_I, [_VP, _I, _VP, _VP, _I, _D, _DP]),
     "sg_uniform_resid_dev": (_I, [_VP, _I, _VP, _VP, _VP, _I, _D]),
+    "sg_uniform_relax_noalloc": (_I, [_I, _VP, _VP, _I, _D, _DP, _VP]),
     "sg_uniform_restrict_dev": (_I, [_VP, _I, _VP, _VP, _I]),
     "sg_uniform_interp_dev": (_I, [_VP, _I, _VP, _VP, _I, _I]),
     "sg_uniform_time_smoother": (_I, [_VP, _I, _I, _DP]),

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -908,6 +909,38 @@ static int create(sg_uniform **out, int n, double h, int precisions)
 // ----------------------------------------------------------------------------------------------
 // C ABI
 // ----------------------------------------------------------------------------------------------
+// ---- the allocation-free relaxation (sg_uniform_relax_noalloc) ----
+// Its only device memory beyond the caller's arrays is the Resmaxg accumulator, kept in this
+// file-scope array: kScratchDoubles doubles per GPU, held by every GPU the library runs on until the
+// process ends.  In host code `scratch` names a host-side stand-in, not a device address: the
+// address the kernels get is the one hipGetSymbolAddress returns for the current device, looked up
+// once per device and kept in g_scratch_ptr.
+constexpr int kScratchDoubles = 64;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static double *g_scratch_ptr[kScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_address(double **out)
+{
+    int dev = 0;
+    SG_HIP(hipGetDevice(&dev));
+    SG_REQUIRE(dev >= 0 && dev < kScratchDevices, "device %d beyond the %d the static scratch table covers", dev,
+               kScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            sg::set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return SG_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SG_OK;
+}
+
 extern "C" {
 
 const char *sg_last_error(void) { return sg::last_error(); }
@@ -1062,6 +1095,29 @@ int sg_uniform_resid_dev(sg_uniform *c, int prec, void *res, const void *u, cons
     else
         SG_REQUIRE(false, "bad precision");
     SG_HIP(hipStreamSynchronize(c->stream));
+    return SG_OK;
+}
+
+int sg_uniform_relax_noalloc(int prec, void *u, const void *rhs, int n, double h, double *resmaxg, void *stream)
+{
+    SG_REQUIRE(u && rhs, "sg_uniform_relax_noalloc: null argument");
+    SG_REQUIRE(n >= 1 && n <= sg::kMaxN && (n & (n - 1)) == 0,
+               "sg_uniform_relax_noalloc: n must be a power of two in [1, %d], got %d", sg::kMaxN, n);
+    SG_REQUIRE(h > 0.0, "sg_uniform_relax_noalloc: h must be positive");
+    SG_REQUIRE(prec == SG_PREC_FP64 || prec == SG_PREC_MIXED, "sg_uniform_relax_noalloc: bad precision %d", prec);
+    double *acc = nullptr;
+    SG_TRY(scratch_address(&acc));
+    hipStream_t s = (hipStream_t)stream;
+    // the accumulator starts at zero: a fill kernel (no memset node if the caller captures the stream)
+    SG_TRY(sg::launch_fill<double>(s, acc, 1));
+    if (prec == SG_PREC_FP64)
+        SG_TRY(sg::launch_relax<double>(s, (double *)u, (const double *)rhs, n, h, acc));
+    else
+        SG_TRY(sg::launch_relax<float>(s, (float *)u, (const float *)rhs, n, h, acc));
+    double m = 0.0;
+    SG_HIP(hipMemcpyAsync(&m, acc, sizeof(double), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));
+    if (resmaxg) *resmaxg = m / (h * h);
     return SG_OK;
 }
 
