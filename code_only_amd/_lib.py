@@ -99,6 +99,7 @@ SIGNATURES = {
     "sg_amr_info": (_I, [_VP, _IP, _I]),
     "sg_amr_tile_info": (_I, [_VP, _I, _IP, ctypes.POINTER(ctypes.c_float), _I]),
     "sg_amr_time_smoother": (_I, [_VP, _I, _I, _DP]),
+    "sg_amr_set_base_solver": (_I, [_VP, _I]),
     "sg_batch_create": (_I, [ctypes.POINTER(_VP), _I, _IP, _DP, _I]),
     "sg_batch_destroy": (_I, [_VP]),
     "sg_batch_solve_dev": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(SgParams),

@@ -84,6 +84,11 @@ class AmrSolver:
                          "ms_per_launch": {str(nt): round(ms[q], 5) for q, nt in enumerate((64, 128, 256)) if ms[q] > 0}}
         return out
 
+    def set_base_solver(self, chip):
+        """Base-grid mg_cycle of the plain FP64 solve spread over the chip (True, default) or in one
+        workgroup (False); the values are the same."""
+        _lib.check(self._lib.sg_amr_set_base_solver(self._ctx, 1 if chip else 0))
+
     def time_smoother(self, precision=SG_PREC_MIXED, reps=20):
         """ms per full red-black sweep (both colours) of the FMG-level-0 relaxation, HIP events on
         the library's stream."""

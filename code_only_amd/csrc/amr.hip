@@ -124,6 +124,10 @@ struct sg_amr {
     int *d_iters = nullptr;
     int base_mg_ok = 0;
     long bm_lds = 0;            // LDS doubles of the base-grid kernel
+    // the base-grid mg_cycle spread over the chip (base_mg_chip; 0: the one-workgroup k_base_mg):
+    // bm2 is bm with levels 0 and 1 in global memory (they cross launches)
+    int base_chip = 1;
+    BaseMG bm2{};
     int fill0_level = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -621,6 +625,89 @@ template <typename T> int vmg_interp(sg_amr &s, int l, int juf, int juc)
     return fill0_faces<T>(s, LV, juc, l - 1, l);
 }
 
+// The base grid's mg_cycle (mg.F90:42-94, Ncycle 2, Npre = Npost = 2, the reference's stopping
+// test Resmaxg / rhomax < tol) with the finest level's steps spread over the chip and the coarser
+// levels' stretches in one workgroup each (base_mg.h, k_bm_*): the same operations in the same
+// order as the one-workgroup k_base_mg, so the same values.  The stopping test reads two scalars
+// back per iteration.
+int read_scalars(sg_amr &s);
+
+int base_mg_chip(sg_amr &s)
+{
+    const BaseMG &M = s.bm2;
+    const int n0 = M.n[0], n1 = M.n[1], bc = M.dirichlet;
+    const double h0 = M.h[0];
+    double *U0 = M.U[0], *RHS0 = M.RHS[0], *RES0 = M.RES[0], *RHO0 = M.RHO[0];
+    double *U1 = M.U[1], *RHS1 = M.RHS[1], *RHO1 = M.RHO[1];
+    double *acc = s.d_scal + 1;
+    auto cube = [](int n) { return (long)(n + 2) * (n + 2) * (n + 2); };
+    auto ghosts = [&](double *u, int n, int mode) { return go(s, k_bm_ghosts, cube(n), u, n, mode); };
+    auto wrap = [&](double *u, int n) { return bc ? SG_OK : ghosts(u, n, 0); };   // mg_boundary_u
+    auto relax0 = [&](double *u, const double *rhs, double *mx) -> int {
+        const long ncell = (long)n0 * n0 * ((n0 + 1) / 2);
+        for (int ipass = 1; ipass <= 2; ++ipass) {
+            SG_TRY(wrap(u, n0));
+            SG_TRY(go(s, k_bm_relax, ncell, u, rhs, n0, h0, ipass, mx));
+        }
+        return SG_OK;
+    };
+    auto interp0 = [&](double *uf, double *uc, int cubic) -> int {   // bmg_interp for level 0 from level 1
+        SG_TRY(bc ? ghosts(uc, n1, 1) : ghosts(uc, n1, 0));
+        SG_TRY(go(s, k_bm_interp, (long)n0 * n0 * n0, uf, (const double *)uc, n0, cubic));
+        if (bc) {
+            SG_TRY(ghosts(uf, n0, 2));
+            return ghosts(uc, n1, 2);
+        }
+        SG_TRY(ghosts(uf, n0, 0));
+        return ghosts(uc, n1, 0);
+    };
+    auto coarse = [&](int part) {
+        hipLaunchKernelGGL(k_bm_coarse, dim3(1), dim3(kBaseThreads), sizeof(double) * s.bm_lds, s.stream, M, part);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
+    };
+    SG_TRY(bc ? ghosts(M.psi, n0, 2) : ghosts(M.psi, n0, 0));
+    double rhomax = 0.0;
+    int it = 0;
+    for (;;) {
+        SG_TRY(wrap(M.psi, n0));
+        if (it == 0) SG_TRY(set_scalar(s, 1, 0.0));
+        SG_TRY(go(s, k_bm_resid, (long)n0 * n0 * n0, RHO0, (const double *)M.psi, (const double *)M.src, n0, h0,
+                  it == 0 ? acc : (double *)nullptr));
+        if (it == 0) {
+            SG_TRY(read_scalars(s));
+            rhomax = s.h_scal[1];
+        }
+        if (it >= M.maxit) break;
+        // mg_lin (mg.F90:106-218)
+        SG_TRY(go(s, k_bm_rstrct, (long)n1 * n1 * n1, RHO1, (const double *)RHO0, n1));
+        SG_TRY(coarse(0));
+        SG_TRY(interp0(U0, U1, 1));
+        SG_TRY(go(s, k_bm_blas, cube(n0), RHS0, (const double *)RHO0, cube(n0), 1));
+        for (int jc = 0; jc < 2; ++jc) {
+            for (int p = 0; p < 2; ++p) SG_TRY(relax0(U0, RHS0, nullptr));
+            SG_TRY(wrap(U0, n0));
+            SG_TRY(go(s, k_bm_resid, (long)n0 * n0 * n0, RES0, (const double *)U0, (const double *)RHS0, n0, h0,
+                      (double *)nullptr));
+            SG_TRY(go(s, k_bm_rstrct, (long)n1 * n1 * n1, RHS1, (const double *)RES0, n1));
+            SG_TRY(coarse(1));
+            SG_TRY(interp0(RES0, U1, 0));
+            SG_TRY(go(s, k_bm_blas, cube(n0), U0, (const double *)RES0, cube(n0), 2));
+            for (int p = 0; p < 2; ++p) {
+                const bool last = jc == 1 && p == 1;
+                if (last) SG_TRY(set_scalar(s, 1, 0.0));
+                SG_TRY(relax0(U0, RHS0, last ? acc : nullptr));
+            }
+        }
+        SG_TRY(read_scalars(s));
+        const double resmaxg = s.h_scal[1] / (h0 * h0);
+        SG_TRY(go(s, k_bm_blas, cube(n0), M.psi, (const double *)U0, cube(n0), 2));
+        ++it;
+        if ((rhomax > 0 ? resmaxg / rhomax : 0.0) < M.tol) break;
+    }
+    return SG_OK;
+}
+
 // vmg_slvsml (vmg.F90:387-432).  FP64: the reference's mg_cycle on the gathered base grid.  Inner
 // (FP32) solve: the same base problem solved exactly by separable transforms in one workgroup.
 template <typename T> int vmg_slvsml(sg_amr &s, int ju, int jrhs)
@@ -653,8 +740,12 @@ template <typename T> int vmg_slvsml(sg_amr &s, int ju, int jrhs)
     const int NG = s.bm.n[0];
     SG_TRY(go(s, k_base_gather<2, double>, (long)s.nbase * 8, s.bm.psi, s.bm.src, (const double *)s.d.A[LV][ju],
               (const double *)s.d.A[LV][jrhs], s.nbase, (const int *)s.d_ijk, NG));
-    hipLaunchKernelGGL(k_base_mg, dim3(1), dim3(kBaseThreads), sizeof(double) * s.bm_lds, s.stream, s.bm);
-    SG_CHECK_LAUNCH();
+    if (s.base_chip && s.bm.nlev >= 3) {
+        SG_TRY(base_mg_chip(s));
+    } else {
+        hipLaunchKernelGGL(k_base_mg, dim3(1), dim3(kBaseThreads), sizeof(double) * s.bm_lds, s.stream, s.bm);
+        SG_CHECK_LAUNCH();
+    }
     return go(s, k_base_scatter<2, double>, (long)s.nbase * 8, s.d.A[LV][ju], (const double *)s.bm.psi, s.nbase,
               (const int *)s.d_ijk, NG);
 }
@@ -1550,6 +1641,11 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
         }
         SG_HIP(hipFuncSetAttribute((const void *)k_base_mg, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(s->bm_lds * 8)));
+        SG_HIP(hipFuncSetAttribute((const void *)k_bm_coarse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(s->bm_lds * 8)));
+        s->bm2 = M;
+        for (int l = 0; l < std::min(2, nl); ++l)
+            for (int a = 0; a < 4; ++a) s->bm2.lds_off[l][a] = -1;
     }
     SG_TRY(tune_blk8<double>(*s));
     SG_HIP(hipDeviceSynchronize());
@@ -1724,6 +1820,14 @@ int sg_amr_info(sg_amr *ctx, int *info, int n)
     const long v[5] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
                        (long)ctx->cf.size(), vmg};
     for (int q = 0; q < n; ++q) info[q] = q < 5 ? (int)v[q] : 0;
+    return SG_OK;
+}
+
+int sg_amr_set_base_solver(sg_amr *ctx, int chip)
+{
+    SG_REQUIRE(ctx, "sg_amr_set_base_solver: null context");
+    SG_REQUIRE(chip == 0 || chip == 1, "sg_amr_set_base_solver: mode must be 0 or 1, got %d", chip);
+    ctx->base_chip = chip;
     return SG_OK;
 }
 

@@ -358,5 +358,190 @@ __global__ void __launch_bounds__(kBaseThreads) k_base_mg(BaseMG M)
     if (threadIdx.x == 0 && M.iters) *M.iters = it;
 }
 
+// ---- the same mg_cycle spread over the chip: the finest level's steps as multi-workgroup launches
+//      (one thread per cell), the stretches on the coarser levels (levels >= 2 in LDS, level 1 in
+//      global memory, which crosses launches) as single-workgroup launches over the functions
+//      above; same operations in the same order on every value ----
+
+// ghost layers of level array u (n cells per side) in one launch: mode 0 wraps (periodic
+// mg_boundary_u: after the x, y, z passes each ghost holds the interior cell with every coordinate
+// wrapped), 1 extrapolates (mg_boundary_extrap's x, y, z passes leave (-1)^g times the clamped
+// interior cell, g the number of ghost coordinates), 2 zeroes (mg_boundary_fill0)
+__global__ void k_bm_ghosts(double *u, int n, int mode)
+{
+    const int W = n + 2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)W * W * W) return;
+    const int i = (int)(t % W) - 1, j = (int)((t / W) % W) - 1, k = (int)(t / ((long)W * W)) - 1;
+    const bool gi_ = i < 0 || i >= n, gj = j < 0 || j >= n, gk = k < 0 || k >= n;
+    if (!(gi_ || gj || gk)) return;
+    if (mode == 2) { u[t] = 0.0; return; }
+    if (mode == 0) {
+        u[t] = u[bgi(n, (i + n) % n, (j + n) % n, (k + n) % n)];
+        return;
+    }
+    const int ci = i < 0 ? 0 : (i >= n ? n - 1 : i), cj = j < 0 ? 0 : (j >= n ? n - 1 : j), ck = k < 0 ? 0 : (k >= n ? n - 1 : k);
+    const int g = (gi_ ? 1 : 0) + (gj ? 1 : 0) + (gk ? 1 : 0);
+    const double v = u[bgi(n, ci, cj, ck)];
+    u[t] = (g & 1) ? -v : v;
+}
+
+// one colour of bmg_relax; |r| maxima into *resmax (then / h^2 = Resmaxg)
+__global__ void k_bm_relax(double *u, const double *__restrict__ rhs, int n, double h, int ipass, double *resmax)
+{
+    const double sixth = 1.0 / 6.0, h2 = h * h;
+    const int hn = (n + 1) / 2, ncell = n * n * hn, lg = ilog2_pow2(n), lgh = ilog2_pow2(hn);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (t < ncell) {
+        const int ih = t & (hn - 1), j = (t >> lgh) & (n - 1), k = t >> (lgh + lg);
+        const int i = 2 * ih + ((j + k + ipass) & 1);
+        if (i < n) {
+            const int q = bgi(n, i, j, k);
+            double r = u[bgi(n, i + 1, j, k)] + u[bgi(n, i - 1, j, k)];
+            r = r + u[bgi(n, i, j + 1, k)];
+            r = r + u[bgi(n, i, j - 1, k)];
+            r = r + u[bgi(n, i, j, k + 1)];
+            r = r + u[bgi(n, i, j, k - 1)];
+            const double uc = u[q];
+            r = r - 6.0 * uc;
+            r = r - h2 * rhs[q];
+            u[q] = uc + sixth * r;
+            m = fabs(r);
+        }
+    }
+    if (resmax) block_atomic_max(resmax, m);
+}
+
+// bmg_resid without its boundary call; with amax, max |res| over the cells into *amax
+__global__ void k_bm_resid(double *__restrict__ res, const double *__restrict__ u, const double *__restrict__ rhs, int n,
+                           double h, double *amax)
+{
+    const double h2i = 1.0 / (h * h);
+    const int lg = ilog2_pow2(n);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (t < n * n * n) {
+        const int i = t & (n - 1), j = (t >> lg) & (n - 1), k = t >> (2 * lg);
+        double s = u[bgi(n, i + 1, j, k)] + u[bgi(n, i - 1, j, k)];
+        s = s + u[bgi(n, i, j + 1, k)];
+        s = s + u[bgi(n, i, j - 1, k)];
+        s = s + u[bgi(n, i, j, k + 1)];
+        s = s + u[bgi(n, i, j, k - 1)];
+        s = s - 6.0 * u[bgi(n, i, j, k)];
+        const double r = -(h2i * s) + rhs[bgi(n, i, j, k)];
+        res[bgi(n, i, j, k)] = r;
+        m = fabs(r);
+    }
+    if (amax) block_atomic_max(amax, m);
+}
+
+__global__ void k_bm_rstrct(double *__restrict__ uc, const double *__restrict__ uf, int nc)
+{
+    const int nf = 2 * nc, lg = ilog2_pow2(nc);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc * nc * nc) return;
+    const int ic = t & (nc - 1), jc = (t >> lg) & (nc - 1), kc = t >> (2 * lg);
+    const int i = 2 * ic, j = 2 * jc, k = 2 * kc;
+    double s = uf[bgi(nf, i, j, k)] + uf[bgi(nf, i + 1, j, k)];
+    s = s + uf[bgi(nf, i, j + 1, k)];
+    s = s + uf[bgi(nf, i, j, k + 1)];
+    s = s + uf[bgi(nf, i + 1, j + 1, k)];
+    s = s + uf[bgi(nf, i + 1, j, k + 1)];
+    s = s + uf[bgi(nf, i, j + 1, k + 1)];
+    s = s + uf[bgi(nf, i + 1, j + 1, k + 1)];
+    uc[bgi(nc, ic, jc, kc)] = s * (1.0 / 8.0);
+}
+
+// the interpolation of bmg_interp (its boundary calls are separate launches of k_bm_ghosts)
+__global__ void k_bm_interp(double *uf, const double *uc, int nf, int cubic)
+{
+    const int nc = nf / 2 > 0 ? nf / 2 : 1;
+    const bool use_cubic = cubic && nc + 2 >= 4;
+    const int lg = ilog2_pow2(nf);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nf * nf * nf) return;
+    const int i = t & (nf - 1), j = (t >> lg) & (nf - 1), k = t >> (2 * lg);
+    auto cget = [&](int a, int b, int c) { return uc[bgi(nc, a, b, c)]; };
+    double v;
+    if (use_cubic) {
+        int ix[4], jx[4], kx[4];
+        double ax[4], ay[4], az[4];
+        cubic_stencil(i, -1, nc, ix, ax);
+        cubic_stencil(j, -1, nc, jx, ay);
+        cubic_stencil(k, -1, nc, kx, az);
+        double bxy[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double bx[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                bx[b] = comb4(ax, cget(ix[0], jx[b], kx[c]), cget(ix[1], jx[b], kx[c]), cget(ix[2], jx[b], kx[c]),
+                              cget(ix[3], jx[b], kx[c]));
+            bxy[c] = comb4(ay, bx[0], bx[1], bx[2], bx[3]);
+        }
+        v = comb4(az, bxy[0], bxy[1], bxy[2], bxy[3]);
+    } else {
+        v = trilinear_at(i, j, k, cget);
+    }
+    uf[bgi(nf, i, j, k)] = v;
+}
+
+// whole ghosted arrays: op 0 o = 0, 1 o = a, 2 o = o + a
+__global__ void k_bm_blas(double *o, const double *a, long len, int op)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= len) return;
+    o[t] = op == 0 ? 0.0 : (op == 1 ? a[t] : o[t] + a[t]);
+}
+
+// mg_lin's stretches on the levels >= 1 in one workgroup (level 1 in global memory):
+// part 0: the restrictions of RHO to levels 2..lmax, the bottom relaxation and the FMG stages
+// ml = lmax-1 .. 1 (mg.F90:130-215 before the finest stage); part 1: one V-cycle's visit of the
+// levels below the finest: zero U(1), down from level 1, bottom, up to level 1
+__global__ void __launch_bounds__(kBaseThreads) k_bm_coarse(BaseMG M, int part)
+{
+    extern __shared__ double lds[];
+    const int lmax = M.nlev - 1, bc = M.dirichlet;
+    if (part == 0) {
+        for (int l = 2; l <= lmax; ++l) bmg_rstrct(lv(M, lds, l, 3), lv(M, lds, l - 1, 3), M.n[l]);
+        bmg_zero(lv(M, lds, lmax, 0), M.n[lmax]);
+        for (int it = 0; it < 2; ++it) bmg_relax(lv(M, lds, lmax, 0), lv(M, lds, lmax, 3), M.n[lmax], M.h[lmax], bc, false);
+        for (int ml = lmax - 1; ml >= 1; --ml) {
+            bmg_interp(lv(M, lds, ml, 0), lv(M, lds, ml + 1, 0), M.n[ml], 1, bc);
+            bmg_copy(lv(M, lds, ml, 1), lv(M, lds, ml, 3), M.n[ml]);
+            for (int jc = 0; jc < 2; ++jc) {
+                for (int v = ml; v <= lmax - 1; ++v) {
+                    for (int p = 0; p < 2; ++p) bmg_relax(lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc, false);
+                    bmg_resid(lv(M, lds, v, 2), lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc);
+                    bmg_rstrct(lv(M, lds, v + 1, 1), lv(M, lds, v, 2), M.n[v + 1]);
+                    bmg_zero(lv(M, lds, v + 1, 0), M.n[v + 1]);
+                }
+                for (int it = 0; it < 2; ++it)
+                    bmg_relax(lv(M, lds, lmax, 0), lv(M, lds, lmax, 1), M.n[lmax], M.h[lmax], bc, false);
+                for (int v = lmax - 1; v >= ml; --v) {
+                    bmg_interp(lv(M, lds, v, 2), lv(M, lds, v + 1, 0), M.n[v], 0, bc);
+                    bmg_add(lv(M, lds, v, 0), lv(M, lds, v, 2), M.n[v]);
+                    for (int p = 0; p < 2; ++p) bmg_relax(lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc, false);
+                }
+            }
+        }
+        return;
+    }
+    bmg_zero(lv(M, lds, 1, 0), M.n[1]);
+    for (int v = 1; v <= lmax - 1; ++v) {
+        for (int p = 0; p < 2; ++p) bmg_relax(lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc, false);
+        bmg_resid(lv(M, lds, v, 2), lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc);
+        bmg_rstrct(lv(M, lds, v + 1, 1), lv(M, lds, v, 2), M.n[v + 1]);
+        bmg_zero(lv(M, lds, v + 1, 0), M.n[v + 1]);
+    }
+    for (int it = 0; it < 2; ++it) bmg_relax(lv(M, lds, lmax, 0), lv(M, lds, lmax, 1), M.n[lmax], M.h[lmax], bc, false);
+    for (int v = lmax - 1; v >= 1; --v) {
+        bmg_interp(lv(M, lds, v, 2), lv(M, lds, v + 1, 0), M.n[v], 0, bc);
+        bmg_add(lv(M, lds, v, 0), lv(M, lds, v, 2), M.n[v]);
+        for (int p = 0; p < 2; ++p) bmg_relax(lv(M, lds, v, 0), lv(M, lds, v, 1), M.n[v], M.h[v], bc, false);
+    }
+}
+
 } // namespace amr
 } // namespace sg

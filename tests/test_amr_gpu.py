@@ -501,3 +501,23 @@ def test_fused_sequence_equals_staged(seq, gpu):
     finally:
         o.close()
         g.close()
+
+
+@pytest.mark.parametrize("which", ["nested-sphere", "random-periodic"])
+def test_base_solver_chip_equals_one_workgroup(which, gpu):
+    """The plain FP64 solve with the base grid's mg_cycle spread over the chip and in one
+    workgroup: the same values bit for bit (and the oracle's, which the other solve tests check)."""
+    h = {"nested-sphere": lambda: ah.nested(base=8, levels=3, bc=ah.SPHERE),
+         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC)}[which]()
+    g = AmrSolver(h)
+    src = problems.amr_source(h, seed=23)
+    try:
+        g.set_base_solver(True)
+        pc, gc, sc = g.solve_host(src, tol=1e-10, maxit=30)
+        g.set_base_solver(False)
+        pw, gw, sw = g.solve_host(src, tol=1e-10, maxit=30)
+        print(which, sc["iterations"], sc["ms"], sw["ms"], flush=True)
+        assert sc["iterations"] == sw["iterations"] and sc["hist"] == sw["hist"]
+        assert np.array_equal(pc, pw) and np.array_equal(gc, gw)
+    finally:
+        g.close()
