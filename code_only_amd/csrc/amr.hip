@@ -113,7 +113,7 @@ struct sg_amr {
     double *d_scal = nullptr;   // [0] Resh2maxg [1] scratch max [2..4] norms [5] mean [8..23] multipole
     double *h_scal = nullptr;   // pinned mirror
     double *d_part = nullptr;   // tree-sum scratch
-    double *d_npart = nullptr;  // norm partials [3][kNormGroups]
+    double *d_npart = nullptr;  // norm partials [3][max(kNormGroups, active blocks)]
     double *d_slots = nullptr;  // smoother max slots [kMaxSlots]
     int P_all = 1, P_base = 1;
     double *d_dense = nullptr;  // staging for host entry points
@@ -338,22 +338,23 @@ constexpr int kBlk8Nt[3] = {64, 128, 256};
 
 // one launch of k_blk8 over the active blocks with nt threads per block
 // (fb: the face store of Au, read for the neighbours' faces and, by the relaxation, written back)
+// (cast, npart: the residual's FP32 copy and norm partials, k_blk8)
 template <bool RELAX, typename T>
 int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo,
-         T *fb = nullptr)
+         T *fb = nullptr, float *cast = nullptr, double *npart = nullptr)
 {
     const dim3 g((unsigned)s.active.size());
     const int *rec = s.d_arec;
     T *fbout = RELAX ? fb : nullptr;
     if (nt == 64)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 64>), g, dim3(64), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0],
-                           ov, resmax, halo, (const T *)fb, fbout);
+                           ov, resmax, halo, (const T *)fb, fbout, cast, npart);
     else if (nt == 128)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 128>), g, dim3(128), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart);
     else
         hipLaunchKernelGGL((k_blk8<RELAX, T, 256>), g, dim3(256), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
@@ -767,7 +768,10 @@ template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
     SG_TRY(go(s, k_blocks_copy<T>, nall, RHS, (const T *)RHO, (const int *)nullptr, 0, s.nblk, (int)LENV));
     for (int v = s.nlev - 1; v >= 1; --v) {
         SG_TRY(vmg_gridboundary<T>(s, v, C_U));
-        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS, p == 1));
+        // the inner solve's relaxation keeps no refresh here: the parent- and same-level passes
+        // below rewrite every ghost cell it would write (from the relaxed interiors, the parent
+        // level and physical ghosts, none of which it writes)
+        for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS, p == 1 && !kInner<T>));
         for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, v, LV, d, C_U, 1, K_BOTH));
         // vmg_rstrct(v-1, U, cubic), vmg_copy(v-1, IRUF <- U), vmg_rstrct(v-1, RHS)
         SG_TRY(go(s, k_vmg_rstr<T>, (long)nlv(v - 1) * LENV, U, PSI, RHS, s.lev_start[v - 1], nlv(v - 1),
@@ -837,19 +841,27 @@ template <typename T> int vmg_fas(sg_amr &s, int jrho)
 }
 
 // fmg_lin (fmg.F90:270-377)
+// In the inner (FP32) solve's fused mode two steps are left out, both without effect on any value
+// read later: FMG level 0 reads its source from RHO, of which fmg_copy(0, RHS <- RHO) would make
+// a copy (nothing writes RHO or RHS at level 0 within fmg_lin); and the closing
+// fmg_converge_c2p(0, U) is not run, since the covered blocks' U it writes is read by no later
+// step (the update psi += U covers the leaf blocks, and every refresh forms covered cells from
+// their children).
 template <typename T> int fmg_lin(sg_amr &s)
 {
     const int Lmax = kNfmg - 1;
+    const bool lean = kInner<T> && s.fused_on;
+    auto rhs = [lean](int v) { return lean && v == 0 ? C_RHO : C_RHS; };
     for (int L = 1; L <= Lmax; ++L) SG_TRY(fmg_rstrct<T>(s, L, C_RHO, C_RHO));
     SG_TRY(fmg_fill0<T>(s, Lmax, C_U));
     SG_TRY(vmg_fas<T>(s, C_RHO));
     for (int Lf = Lmax - 1; Lf >= 0; --Lf) {
         SG_TRY(fmg_interp<T>(s, Lf, C_U, C_U, 1));
-        SG_TRY(fmg_copy<T>(s, Lf, C_RHS, C_RHO, 0));
+        if (!(lean && Lf == 0)) SG_TRY(fmg_copy<T>(s, Lf, C_RHS, C_RHO, 0));
         for (int jc = 0; jc < 2; ++jc) {
             for (int v = Lf; v <= Lmax - 1; ++v) {
-                for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax<T>(s, v, C_U, C_RHS, 1, false));
-                SG_TRY(poisson_resid<T>(s, v, C_RES, C_U, C_RHS, 1));
+                for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax<T>(s, v, C_U, rhs(v), 1, false));
+                SG_TRY(poisson_resid<T>(s, v, C_RES, C_U, rhs(v), 1));
                 SG_TRY(fmg_rstrct<T>(s, v + 1, C_RHS, C_RES));
                 SG_TRY(fmg_fill0<T>(s, v + 1, C_U));
                 s.fill0_level = v + 1;
@@ -861,12 +873,12 @@ template <typename T> int fmg_lin(sg_amr &s)
                 // the last relaxation there
                 const bool last = jc == 1 && v == Lf && Lf >= 1;
                 for (int p = 0; p < 2; ++p)
-                    SG_TRY(poisson_relax<T>(s, v, C_U, C_RHS, 1, Lf == 0 && jc == 1 && v == 0 && p == 1,
+                    SG_TRY(poisson_relax<T>(s, v, C_U, rhs(v), 1, Lf == 0 && jc == 1 && v == 0 && p == 1,
                                             last && p == 1));
             }
         }
     }
-    return c2p<T>(s, 0, C_U, 0);
+    return lean ? SG_OK : c2p<T>(s, 0, C_U, 0);
 }
 
 Geo geo(sg_amr &s) { return Geo{s.d_ijk, s.d_level, s.d_h[0], s.x0[0], s.x0[1], s.x0[2]}; }
@@ -1075,20 +1087,39 @@ int ensure_mixed(sg_amr &s)
     return tune_blk8<float>(s);
 }
 
+// The FP64 composite residual of the outer iteration (the reference operator: its refresh and
+// reflux) and its norms -> d_scal[2..4].  In the fused mode the residual kernel also writes the
+// inner solve's FP32 source (the leaf interiors, all the inner solve reads of it) and its blocks'
+// norm partials.
+int outer_resid(sg_amr &s)
+{
+    if (!s.fused_on || s.active.empty()) {
+        SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
+        return norms(s, C_RHO);
+    }
+    SG_TRY(poisson_flux<double>(s, 0, C_PSI, 0));
+    SG_TRY((blk8<false, double>(s, s.blk8_nt[kBlk8Slot<double> + 1], s.d.A[0][C_RHO], s.d.A[0][C_PSI], s.d.A[0][C_SRC], 0,
+                                s.d.ov[0], nullptr, fused(s, 0) ? 1 : 0, nullptr, s.f.A[0][C_RHO], s.d_npart)));
+    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(1024), 0, s.stream, (const double *)s.d_npart, (int)s.active.size(),
+                       s.d_scal + 2);
+    SG_CHECK_LAUNCH();
+    return SG_OK;
+}
+
 // One outer iteration: FP32 copy of the FP64 residual, inner fmg_lin in FP32, FP64 update
-// psi += e, next FP64 composite residual (the reference operator, staged ghosts and reflux) and its
-// norms.  Only launches: it is captured into a hipGraph once and replayed.
+// psi += e, next FP64 composite residual and its norms.  Only launches: it is captured into a
+// hipGraph once and replayed.
 int mixed_iteration(sg_amr &s)
 {
     const int nb = s.nblk;
-    SG_TRY(go(s, k_cast_blocks, (long)nb * lenof(0), s.f.A[0][C_RHO], (const double *)s.d.A[0][C_RHO], (const int *)nullptr,
-              nb, (int)lenof(0)));
+    if (!s.fused_on)
+        SG_TRY(go(s, k_cast_blocks, (long)nb * lenof(0), s.f.A[0][C_RHO], (const double *)s.d.A[0][C_RHO],
+                  (const int *)nullptr, nb, (int)lenof(0)));
     SG_TRY(fmg_lin<float>(s));
     const int na = (int)s.active.size();
     SG_TRY(go(s, k_add_cast_blocks, (long)na * lenof(0), s.d.A[0][C_PSI], (const float *)s.f.A[0][C_U],
               (const int *)s.d_active, na, (int)lenof(0)));
-    SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
-    return norms(s, C_RHO);
+    return outer_resid(s);
 }
 
 int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_params &p, sg_stats &st)
@@ -1121,8 +1152,7 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
         FusedScope(sg_amr &s_, int on) : s(s_) { s.fused_on = on; }
         ~FusedScope() { s.fused_on = 0; }
     } fused_scope(s, p.fused ? 1 : 0);
-    SG_TRY(poisson_resid<double>(s, 0, C_RHO, C_PSI, C_SRC, 0));
-    SG_TRY(norms(s, C_RHO));
+    SG_TRY(outer_resid(s));
     SG_TRY(read_scalars(s));
     double prev = HUGE_VAL;
     int it = 0, status = SG_ERR_NOT_CONVERGED;
@@ -1581,7 +1611,7 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     s->P_base = pow2ge(s->nbase);
     SG_HIP(hipMalloc((void **)&s->d_part, sizeof(double) * std::max<long>(2L * s->P_all, 16L * s->P_base)));
     SG_HIP(hipMalloc((void **)&s->d_iters, sizeof(int)));
-    SG_HIP(hipMalloc((void **)&s->d_npart, sizeof(double) * 3 * 1024));
+    SG_HIP(hipMalloc((void **)&s->d_npart, sizeof(double) * 3 * std::max<size_t>(kNormGroups, s->active.size())));
     SG_HIP(hipMalloc((void **)&s->d_slots, sizeof(double) * kMaxSlots));
     // cubic restriction weights St (restriction.F90:38-63)
     {

@@ -1177,10 +1177,14 @@ __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
     return m;
 }
 
+// The FP64 residual of the mixed solve can also write its FP32 copy (cast, same offsets) and its
+// block's norm partials (npart[g], npart[G + g], npart[2G + g] for workgroup g of G: the per-cell
+// terms of k_norms_part, so the two max norms are the same values; the sum adds in another order).
 template <bool RELAX, typename T, int NT>
 __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
-                                            int halo, const T *fbin = nullptr, T *fbout = nullptr)
+                                            int halo, const T *fbin = nullptr, T *fbout = nullptr,
+                                            float *cast = nullptr, double *npart = nullptr)
 {
     using B = Blk<8>;
     constexpr int NC = RELAX ? 256 / NT : 512 / NT;   // cells per thread
@@ -1226,7 +1230,8 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
     __syncthreads();
     const T h = (T)h_lev.v[R.l], hi = (T)hi_lev.v[R.l];
     FluxR<8, T> F{su, nullptr, ov, hi, R};
-    double a = 0.0;
+    double a = 0.0, m2 = 0.0, sq = 0.0;
+    const double hd = h_lev.v[R.l];
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
         const long q = (long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r]);
@@ -1239,7 +1244,28 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             if (fbout) su[B::gi(ci[r], cj[r], ck[r])] = v;
             a = fmax(a, (double)fabs(resh2));
         } else {
-            Aout[q] = rhs[r] - F.div(ci[r], cj[r], ck[r]) * hi;
+            const T v = rhs[r] - F.div(ci[r], cj[r], ck[r]) * hi;
+            Aout[q] = v;
+            if constexpr (std::is_same<T, double>::value) {
+                if (cast) cast[q] = (float)v;
+                if (npart) {
+                    const double av = fabs(v);
+                    a = fmax(a, av);
+                    m2 = fmax(m2, av * (hd * hd));
+                    sq += v * v * ((hd * hd) * hd);
+                }
+            }
+        }
+    }
+    if (!RELAX && npart) {
+        const int G = gridDim.x;
+        block_max_sum(m2, sq);
+        double dummy = 0.0;
+        block_max_sum(a, dummy);
+        if (threadIdx.x == 0) {
+            npart[blockIdx.x] = m2;
+            npart[G + blockIdx.x] = a;
+            npart[2 * G + blockIdx.x] = sq;
         }
     }
     if (RELAX && resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
