@@ -52,6 +52,19 @@ class SgStats(ctypes.Structure):
                     hist=[self.hist[i] for i in range(min(self.iterations + 1, SG_HIST))])
 
 
+SG_AMR_MAX_LEVELS = 16
+
+
+class SgBlockmem(ctypes.Structure):
+    _fields_ = [("ng", ctypes.c_int), ("ncomp", ctypes.c_int), ("stride", ctypes.c_longlong)] + \
+               [(n, ctypes.c_int) for n in ("mrho", "mvx", "mvy", "mvz", "mp", "mpsi", "mgx", "mgy", "mgz")]
+
+
+class SgTimeslice(ctypes.Structure):
+    _fields_ = [("source", ctypes.c_int), ("step_max", ctypes.c_double),
+                ("step", ctypes.c_double * SG_AMR_MAX_LEVELS), ("dstep", ctypes.c_double * SG_AMR_MAX_LEVELS)]
+
+
 class SgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libsg error {code}: {msg}")
@@ -100,6 +113,10 @@ SIGNATURES = {
     "sg_amr_tile_info": (_I, [_VP, _I, _IP, ctypes.POINTER(ctypes.c_float), _I]),
     "sg_amr_time_smoother": (_I, [_VP, _I, _I, _DP]),
     "sg_amr_set_base_solver": (_I, [_VP, _I]),
+    "sg_amr_set_blockmem": (_I, [_VP, ctypes.POINTER(SgBlockmem), ctypes.c_longlong, _IP, _IP, _IP]),
+    "sg_amr_solve_blockmem_dev": (_I, [_VP, _VP, ctypes.POINTER(SgTimeslice), _D, ctypes.POINTER(SgParams),
+                                       ctypes.POINTER(SgStats)]),
+    "sg_amr_source_g_dev": (_I, [_VP, _VP, _DP, _D]),
     "sg_batch_create": (_I, [ctypes.POINTER(_VP), _I, _IP, _DP, _I]),
     "sg_batch_destroy": (_I, [_VP]),
     "sg_batch_solve_dev": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(SgParams),

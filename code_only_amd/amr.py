@@ -129,6 +129,51 @@ class AmrSolver:
                    allow=() if raise_on_fail else (SG_ERR_NOT_CONVERGED, SG_ERR_NONFINITE))
         return psi, g, st.as_dict()
 
+    # ---- hydro block memory on the device (BlockMem(gid)%u, grid.F90:146-194) ----
+    def set_blockmem(self, layout, pool_slots, slot_u, slot_u1=None, slot_u2=None):
+        """Layout (dict of sg_blockmem's fields: ng, ncomp, stride, mrho, mvx, mvy, mvz, mp, mpsi, mgx,
+        mgy, mgz) and per block the pool slots of its own, predictor-stage and past arrays."""
+        lay = _lib.SgBlockmem(**{k: int(v) for k, v in layout.items()})
+        tabs = [None if a is None else np.ascontiguousarray(a, np.int32) for a in (slot_u, slot_u1, slot_u2)]
+        for a in tabs:
+            if a is not None and a.shape != (self.h.nblk,):
+                raise ValueError("slot tables hold one entry per block")
+        _lib.check(self._lib.sg_amr_set_blockmem(self._ctx, ctypes.byref(lay), int(pool_slots),
+                                                 *[None if a is None else _ip(a) for a in tabs]))
+
+    def solve_blockmem(self, pool, pi4g, source=0, step_max=0.0, step=(), dstep=(), tol=1e-10, maxit=30,
+                       stop_mode=SG_STOP_TRUE_RESIDUAL, precision=SG_PREC_FP64, use_graph=True, raise_on_fail=True,
+                       fused=True):
+        """fmg_poisson on a float64 GPU tensor holding the pool: stage psi and 4 pi G rho (time slice
+        `source` of fmg_prepare_interp), solve, write psi back over the FMG array's extent and g
+        over the interior."""
+        import torch
+        if not (pool.is_cuda and pool.dtype == torch.float64 and pool.is_contiguous()):
+            raise ValueError("expected a contiguous float64 GPU tensor")
+        ts = _lib.SgTimeslice(source=int(source), step_max=float(step_max))
+        for l, v in enumerate(step):
+            ts.step[l] = float(v)
+        for l, v in enumerate(dstep):
+            ts.dstep[l] = float(v)
+        p = self._params(tol, maxit, stop_mode, precision, use_graph, fused)
+        st = _lib.SgStats()
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_solve_blockmem_dev(self._ctx, ctypes.c_void_p(pool.data_ptr()), ctypes.byref(ts),
+                                                       float(pi4g), ctypes.byref(p), ctypes.byref(st)),
+                   allow=() if raise_on_fail else (SG_ERR_NOT_CONVERGED, SG_ERR_NONFINITE))
+        return st.as_dict()
+
+    def source_g(self, pool, dtime, gamma=1.4):
+        """source_g_all_level on the pool (float64 GPU tensor), dtime[l] = Dtime(l) per AMR level."""
+        import torch
+        if not (pool.is_cuda and pool.dtype == torch.float64 and pool.is_contiguous()):
+            raise ValueError("expected a contiguous float64 GPU tensor")
+        dt = np.ascontiguousarray(dtime, np.float64)
+        if dt.shape != (self.h.nlev,):
+            raise ValueError("dtime holds one value per AMR level")
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_source_g_dev(self._ctx, ctypes.c_void_p(pool.data_ptr()), _dp(dt), float(gamma)))
+
     # ---- operator-level access (parity tests) ----
     def get(self, L, code):
         n = 8 >> L

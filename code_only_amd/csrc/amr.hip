@@ -19,6 +19,7 @@
 #include "amr_kernels.h"
 #include "base_direct.h"
 #include "base_mg.h"
+#include "blockmem_kernels.h"
 #include "sg_common.h"
 
 namespace sg {
@@ -129,6 +130,10 @@ struct sg_amr {
     int base_chip = 1;
     BaseMG bm2{};
     int fill0_level = -1;
+    // hydro block memory (sg_amr_set_blockmem): its layout and the blocks' pool slots on the device
+    sg_blockmem bml{};
+    int have_bm = 0;
+    int *d_slot_u = nullptr, *d_slot_u1 = nullptr, *d_slot_u2 = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1369,7 +1374,8 @@ void release(sg_amr *s)
     if (!s) return;
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
                  s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters,
-                 s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap};
+                 s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap,
+                 s->d_slot_u, s->d_slot_u1, s->d_slot_u2};
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->d_crop) (void)hipFree(s->d_crop);
     if (s->fb0) (void)hipFree(s->fb0);
@@ -1901,6 +1907,96 @@ int sg_amr_op(sg_amr *ctx, int opc, int fmglev, int a, int b, int c, int flag, d
 {
     SG_REQUIRE(ctx, "sg_amr_op: null context");
     return op(*ctx, opc, fmglev, a, b, c, flag, ret);
+}
+
+// ---- hydro block memory on the device ----
+int sg_amr_set_blockmem(sg_amr *ctx, const sg_blockmem *lay, long long pool_slots, const int *slot_u,
+                        const int *slot_u1, const int *slot_u2)
+{
+    SG_REQUIRE(ctx && lay && slot_u, "sg_amr_set_blockmem: null argument");
+    sg_amr &s = *ctx;
+    const sg_blockmem L = *lay;
+    SG_REQUIRE(L.ng >= 1 && L.ng <= 8, "sg_amr_set_blockmem: ng = %d outside 1..8", L.ng);
+    SG_REQUIRE(L.ncomp >= 9 && L.ncomp <= 4096, "sg_amr_set_blockmem: ncomp = %d (needs >= 9 labels)", L.ncomp);
+    const long long W = 8 + 2 * L.ng;
+    SG_REQUIRE(L.stride >= (long long)L.ncomp * W * W * W, "sg_amr_set_blockmem: stride %lld < ncomp (8 + 2 ng)^3",
+               L.stride);
+    const int lab[9] = {L.mrho, L.mvx, L.mvy, L.mvz, L.mp, L.mpsi, L.mgx, L.mgy, L.mgz};
+    for (int a = 0; a < 9; ++a) {
+        SG_REQUIRE(lab[a] >= 0 && lab[a] < L.ncomp, "sg_amr_set_blockmem: label %d = %d outside 0..ncomp-1", a, lab[a]);
+        for (int c = 0; c < a; ++c) SG_REQUIRE(lab[a] != lab[c], "sg_amr_set_blockmem: labels %d and %d coincide", c, a);
+    }
+    SG_REQUIRE(pool_slots >= 1 && pool_slots <= (1LL << 31) && (double)pool_slots * (double)L.stride < 9.0e18,
+               "sg_amr_set_blockmem: bad pool_slots %lld", pool_slots);
+    const int *tabs[3] = {slot_u, slot_u1, slot_u2};
+    int **dst[3] = {&s.d_slot_u, &s.d_slot_u1, &s.d_slot_u2};
+    for (int q = 0; q < 3; ++q) {
+        if (*dst[q]) (void)hipFree(*dst[q]);
+        *dst[q] = nullptr;
+    }
+    s.have_bm = 0;
+    for (int q = 0; q < 3; ++q) {
+        if (!tabs[q]) continue;
+        for (int b = 0; b < s.nblk; ++b)
+            SG_REQUIRE(tabs[q][b] >= 0 && tabs[q][b] < pool_slots, "sg_amr_set_blockmem: slot table %d, block %d: %d",
+                       q, b, tabs[q][b]);
+        SG_TRY(upload(s, dst[q], std::vector<int>(tabs[q], tabs[q] + s.nblk)));
+    }
+    s.bml = L;
+    s.have_bm = 1;
+    return SG_OK;
+}
+
+int sg_amr_solve_blockmem_dev(sg_amr *ctx, double *pool, const sg_timeslice *ts, double pi4g, const sg_params *p,
+                              sg_stats *st)
+{
+    SG_REQUIRE(ctx && pool && ts && p && st, "sg_amr_solve_blockmem_dev: null argument");
+    sg_amr &s = *ctx;
+    SG_REQUIRE(s.have_bm, "sg_amr_solve_blockmem_dev: no block memory layout (sg_amr_set_blockmem)");
+    SG_REQUIRE(ts->source >= 0 && ts->source <= 3, "sg_amr_solve_blockmem_dev: unknown time slice %d", ts->source);
+    SG_REQUIRE(!(ts->source == 1 || ts->source == 3) || s.d_slot_u1,
+               "sg_amr_solve_blockmem_dev: time slice %d needs the predictor slots", ts->source);
+    SG_REQUIRE(!(ts->source == 2 || ts->source == 3) || s.d_slot_u2,
+               "sg_amr_solve_blockmem_dev: time slice %d needs the past slots", ts->source);
+    SliceCoef C{};
+    C.source = ts->source;
+    for (int l = 0; l < s.nlev; ++l) {
+        const double stp = ts->step_max, dstp = ts->dstep[l];
+        C.fine[l] = ts->step[l] == stp ? 1 : 0;
+        C.stp1[l] = stp - (ts->step[l] - ts->dstep[l]);
+        C.stp2[l] = ts->step[l] - stp;
+        C.dstp[l] = dstp;
+    }
+    const size_t n = (size_t)s.nblk * 512;
+    if (!s.d_dense) SG_HIP(hipMalloc((void **)&s.d_dense, sizeof(double) * 5 * n));
+    double *dpsi = s.d_dense, *dsrc = dpsi + n, *dg = dsrc + n;
+    SG_TRY(go(s, k_bm_stage, (long)n, dpsi, dsrc, (const double *)pool, (const int *)s.d_slot_u,
+              (const int *)s.d_slot_u1, (const int *)s.d_slot_u2, s.nblk, (const int *)s.d_level, s.bml, C, pi4g));
+    const int rc = sg_amr_solve_dev(ctx, dpsi, dsrc, dg, p, st);
+    if (rc != SG_OK && rc != SG_ERR_NOT_CONVERGED && rc != SG_ERR_NONFINITE) return rc;
+    SG_TRY(go(s, k_bm_restore, (long)s.nblk * 1000, pool, (const int *)s.d_slot_u, s.nblk, s.bml,
+              (const double *)s.d.A[0][C_U], (const double *)dg));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return rc;
+}
+
+int sg_amr_source_g_dev(sg_amr *ctx, double *pool, const double *dtime, double gamma)
+{
+    SG_REQUIRE(ctx && pool && dtime, "sg_amr_source_g_dev: null argument");
+    sg_amr &s = *ctx;
+    SG_REQUIRE(s.have_bm && s.d_slot_u1, "sg_amr_source_g_dev: no block memory layout with predictor slots");
+    SG_REQUIRE(std::isfinite(gamma) && gamma != 1.0, "sg_amr_source_g_dev: bad gamma");
+    LevelScalars dv{}, dt{};
+    for (int l = 0; l < s.nlev; ++l) {
+        const double h = s.hl[0][l];
+        dv.v[l] = h * h * h;   // get_dv (grid.F90:405-409)
+        dt.v[l] = dtime[l];
+    }
+    const long W = 8 + 2 * s.bml.ng;
+    SG_TRY(go(s, k_source_g, (long)s.nblk * W * W * W, pool, (const int *)s.d_slot_u, (const int *)s.d_slot_u1,
+              (const int *)s.d_level, s.nblk, s.bml, dv, dt, gamma));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return SG_OK;
 }
 
 } // extern "C"

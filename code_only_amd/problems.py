@@ -120,3 +120,41 @@ def batch_problems(nprob=48, seed=0, sizes=BATCH_SIZES, sigma_cells=4.0, device=
         rhs = [clumpy_rhs_torch(n, seed=s, nclump=k, sigma_cells=sigma_cells, device=device)
                for n, s, k in zip(ns, seeds, ncl)]
     return ns, rhs
+
+
+# Hydro block memory layouts (config.h:61-110): the self-gravitating adiabatic HD model without
+# chemistry (MODEL_ART == 0) and the one with the chemistry network (MODEL_ART == 1, METAL), both
+# with N_GHOST_CELL = 2 (config.h:135).  The second pads each slot to show the stride is honoured.
+BLOCKMEM_LAYOUTS = {
+    "hd": dict(ng=2, ncomp=9, stride=9 * 12 ** 3, mrho=0, mvx=1, mvy=2, mvz=3, mp=4, mpsi=5, mgx=6, mgy=7, mgz=8),
+    "art": dict(ng=2, ncomp=21, stride=21 * 12 ** 3 + 96, mrho=0, mvx=1, mvy=2, mvz=3, mp=4, mpsi=16, mgx=17,
+                mgy=18, mgz=19),
+}
+
+
+def blockmem_pool(hier, layout, seed=0, spare=5, src=None):
+    """A pool of hydro block arrays for every block of ``hier``: three slots per block (its own
+    state, the predictor stage, the past state) in a seeded random order among 3 nblk + spare
+    slots.  Returns (pool [nslots, stride] float64, slot_u, slot_u1, slot_u2).  Density is
+    positive everywhere (``src`` on the interiors of the own state when given, else a clumpy
+    field; seeded perturbations of it in the other two), velocities in [-1, 1], pressure in
+    [0.5, 1.5], the potential, g and any other component random normal."""
+    rng = np.random.default_rng(seed)
+    nb = hier.nblk
+    W = 8 + 2 * layout["ng"]
+    nslots = 3 * nb + spare
+    pool = rng.standard_normal((nslots, layout["stride"]))
+    perm = rng.permutation(nslots).astype(np.int32)
+    slot_u, slot_u1, slot_u2 = perm[:nb], perm[nb:2 * nb], perm[2 * nb:3 * nb]
+    comp = pool[:, :layout["ncomp"] * W ** 3].reshape(nslots, layout["ncomp"], W, W, W)
+    comp[:, layout["mrho"]] = rng.uniform(0.5, 2.0, (nslots, W, W, W))
+    for m in ("mvx", "mvy", "mvz"):
+        comp[:, layout[m]] = rng.uniform(-1.0, 1.0, (nslots, W, W, W))
+    comp[:, layout["mp"]] = rng.uniform(0.5, 1.5, (nslots, W, W, W))
+    rho = amr_source(hier, seed=seed + 1) if src is None else np.asarray(src, np.float64)
+    g0 = layout["ng"]
+    inner = (slice(g0, g0 + 8),) * 3
+    comp[(slot_u, layout["mrho"]) + inner] = rho
+    comp[(slot_u1, layout["mrho"]) + inner] = rho * rng.uniform(0.9, 1.1, rho.shape)
+    comp[(slot_u2, layout["mrho"]) + inner] = rho * rng.uniform(0.9, 1.1, rho.shape)
+    return pool, slot_u, slot_u1, slot_u2

@@ -86,8 +86,74 @@ def lib():
         L.orc_amr_solve.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.POINTER(OrcParams),
                                     ctypes.POINTER(OrcAmrStats)]
         L.orc_amr_solve.restype = ci
+        ip = ctypes.POINTER(ci)
+        bp, tp = ctypes.POINTER(OrcBlockmem), ctypes.POINTER(OrcTimeslice)
+        L.orc_bm_stage.argtypes = [dp, ip, ip, ip, ci, ip, bp, tp, cd, dp, dp]
+        L.orc_bm_restore.argtypes = [dp, ip, ci, bp, dp, dp]
+        L.orc_bm_source_g.argtypes = [dp, ip, ip, ci, ip, dp, dp, bp, cd]
         _LIB = L
     return _LIB
+
+
+# ---- hydro block memory on either side of the solve (blockmem.c) ----
+MAXLEV = 16
+
+
+class OrcBlockmem(ctypes.Structure):
+    _fields_ = [("ng", ctypes.c_int), ("ncomp", ctypes.c_int), ("stride", ctypes.c_long)] + \
+               [(n, ctypes.c_int) for n in ("mrho", "mvx", "mvy", "mvz", "mp", "mpsi", "mgx", "mgy", "mgz")]
+
+
+class OrcTimeslice(ctypes.Structure):
+    _fields_ = [("source", ctypes.c_int), ("step_max", ctypes.c_double),
+                ("step", ctypes.c_double * MAXLEV), ("dstep", ctypes.c_double * MAXLEV)]
+
+
+def blockmem(ng, ncomp, stride, **labels):
+    return OrcBlockmem(ng=ng, ncomp=ncomp, stride=stride, **labels)
+
+
+def timeslice(source, step_max=0.0, step=(), dstep=()):
+    t = OrcTimeslice(source=source, step_max=step_max)
+    for l, v in enumerate(step):
+        t.step[l] = v
+    for l, v in enumerate(dstep):
+        t.dstep[l] = v
+    return t
+
+
+def _slots(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def bm_stage(pool, lay, slot_u, level, ts, pi4g, slot_u1=None, slot_u2=None):
+    """fmg_make_physvar's Poisson staging: dense psi (U(MPSI)) and src (4 pi G slice(rho))."""
+    nblk = len(slot_u)
+    su, lv = _slots(slot_u), _slots(level)
+    s1 = _slots(slot_u1 if slot_u1 is not None else slot_u)
+    s2 = _slots(slot_u2 if slot_u2 is not None else slot_u)
+    psi, src = np.zeros((nblk, 8, 8, 8)), np.zeros((nblk, 8, 8, 8))
+    lib().orc_bm_stage(_p(pool), _ip(su), _ip(s1), _ip(s2), nblk, _ip(lv), ctypes.byref(lay), ctypes.byref(ts),
+                       float(pi4g), _p(psi), _p(src))
+    return psi, src
+
+
+def bm_restore(pool, lay, slot_u, u_blocks, g):
+    """fmg_restore_u((/MPSI/), IU) over the FMG array's extent and fmg_psi2g's g, in place."""
+    su = _slots(slot_u)
+    lib().orc_bm_restore(_p(pool), _ip(su), len(su), ctypes.byref(lay), _p(np.ascontiguousarray(u_blocks)),
+                         _p(np.ascontiguousarray(g)))
+
+
+def bm_source_g(pool, lay, slot_u, slot_u1, level, dv, dt, gamma):
+    """source_g_all_level (SINGLE_STEP) with the ideal-gas u2w / w2u, in place."""
+    su, s1, lv = _slots(slot_u), _slots(slot_u1), _slots(level)
+    dvv = np.zeros(MAXLEV)
+    dtv = np.zeros(MAXLEV)
+    dvv[:len(dv)] = dv
+    dtv[:len(dt)] = dt
+    lib().orc_bm_source_g(_p(pool), _ip(su), _ip(s1), len(su), _ip(lv), _p(dvv), _p(dtv), ctypes.byref(lay),
+                          float(gamma))
 
 
 def _p(a):

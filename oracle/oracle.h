@@ -104,4 +104,25 @@ double orc_amr_op(orc_amr *s, int op, int L, int a, int b, int c, int flag);
 void orc_amr_sphere_bc(orc_amr *s, int ju, int jrho);
 int orc_amr_solve(orc_amr *s, double *psi, const double *src, double *g, const orc_params *p, orc_amr_stats *st);
 
+/* ---- hydro block memory on either side of the solve (blockmem.c) ---- */
+typedef struct {
+    int ng, ncomp;       /* ghost layers (N_GHOST_CELL), components (NM) */
+    long stride;         /* doubles from one pool slot to the next */
+    int mrho, mvx, mvy, mvz, mp, mpsi, mgx, mgy, mgz;   /* component labels (config.h:63-71) */
+} orc_blockmem;
+
+typedef struct {
+    int source;          /* fmg_prepare_interp time slice of Rho: 0 current, 1 predictor, 2 past, 3 in time */
+    double step_max;     /* Step(LevelMax) */
+    double step[ORC_AMR_MAXLEV], dstep[ORC_AMR_MAXLEV];   /* Step(l), Dstep(l) */
+} orc_timeslice;
+
+void orc_bm_stage(const double *pool, const int *slot_u, const int *slot_u1, const int *slot_u2, int nblk,
+                  const int *level, const orc_blockmem *L, const orc_timeslice *ts, double pi4g, double *psi,
+                  double *src);
+void orc_bm_restore(double *pool, const int *slot_u, int nblk, const orc_blockmem *L, const double *u_blocks,
+                    const double *g);
+void orc_bm_source_g(double *pool, const int *slot_u, const int *slot_u1, int nblk, const int *level,
+                     const double *dv_lev, const double *dt_lev, const orc_blockmem *L, double gamma);
+
 #endif
