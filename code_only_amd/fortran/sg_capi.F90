@@ -37,6 +37,22 @@ module sg_capi
      real(c_double) :: rel_h2
   end type sg_stats
 
+  ! hydro block memory on the device (sg.h: sg_blockmem, sg_timeslice)
+  integer,parameter :: SG_AMR_MAX_LEVELS = 16
+  type, bind(C) :: sg_blockmem
+     integer(c_int) :: ng
+     integer(c_int) :: ncomp
+     integer(c_long_long) :: stride
+     integer(c_int) :: mrho, mvx, mvy, mvz, mp, mpsi, mgx, mgy, mgz
+  end type sg_blockmem
+
+  type, bind(C) :: sg_timeslice
+     integer(c_int) :: source
+     real(c_double) :: step_max
+     real(c_double) :: step(SG_AMR_MAX_LEVELS)
+     real(c_double) :: dstep(SG_AMR_MAX_LEVELS)
+  end type sg_timeslice
+
   interface
      function sg_last_error() bind(C, name='sg_last_error') result(p)
        import :: c_ptr
@@ -106,6 +122,37 @@ module sg_capi
        type(sg_stats),intent(OUT) :: st
        integer(c_int) :: rc
      end function sg_amr_solve_host
+
+     function sg_amr_set_blockmem(ctx, lay, pool_slots, slot_u, slot_u1, slot_u2) &
+          bind(C, name='sg_amr_set_blockmem') result(rc)
+       import :: c_ptr, c_int, c_long_long, sg_blockmem
+       type(c_ptr),value :: ctx
+       type(sg_blockmem),intent(IN) :: lay
+       integer(c_long_long),value :: pool_slots
+       integer(c_int),intent(IN) :: slot_u(*)
+       type(c_ptr),value :: slot_u1, slot_u2   ! c_loc of the tables, or c_null_ptr
+       integer(c_int) :: rc
+     end function sg_amr_set_blockmem
+
+     ! pool: the device address of the grids' arrays (from the GPU-resident hydro code)
+     function sg_amr_solve_blockmem_dev(ctx, pool, ts, pi4g, p, st) &
+          bind(C, name='sg_amr_solve_blockmem_dev') result(rc)
+       import :: c_ptr, c_int, c_double, sg_timeslice, sg_params, sg_stats
+       type(c_ptr),value :: ctx, pool
+       type(sg_timeslice),intent(IN) :: ts
+       real(c_double),value :: pi4g
+       type(sg_params),intent(IN) :: p
+       type(sg_stats),intent(OUT) :: st
+       integer(c_int) :: rc
+     end function sg_amr_solve_blockmem_dev
+
+     function sg_amr_source_g_dev(ctx, pool, dtime, gamma) bind(C, name='sg_amr_source_g_dev') result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr),value :: ctx, pool
+       real(c_double),intent(IN) :: dtime(*)
+       real(c_double),value :: gamma
+       integer(c_int) :: rc
+     end function sg_amr_source_g_dev
   end interface
 
 contains
