@@ -117,6 +117,9 @@ SIGNATURES = {
     "sg_amr_solve_blockmem_dev": (_I, [_VP, _VP, ctypes.POINTER(SgTimeslice), _D, ctypes.POINTER(SgParams),
                                        ctypes.POINTER(SgStats)]),
     "sg_amr_source_g_dev": (_I, [_VP, _VP, _DP, _D]),
+    "sg_amr_hydro_boundary_dev": (_I, [_VP, _VP, _I, _I]),
+    "sg_amr_fg2cg_dev": (_I, [_VP, _VP, _I, _D]),
+    "sg_amr_fluxcorrection_dev": (_I, [_VP, _VP, _VP, _I, _DP, _D]),
     "sg_batch_create": (_I, [ctypes.POINTER(_VP), _I, _IP, _DP, _I]),
     "sg_batch_destroy": (_I, [_VP]),
     "sg_batch_solve_dev": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(SgParams),

@@ -174,6 +174,41 @@ class AmrSolver:
         torch.cuda.synchronize()
         _lib.check(self._lib.sg_amr_source_g_dev(self._ctx, ctypes.c_void_p(pool.data_ptr()), _dp(dt), float(gamma)))
 
+    # ---- hydro steps on the pool (grid_boundary.F90, fg2cg.F90, reflux.F90) ----
+    @staticmethod
+    def _pool_ok(t):
+        import torch
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError("expected a contiguous float64 GPU tensor")
+
+    def hydro_boundary(self, pool, level, mode=0):
+        """boundary_grid(level, mode): mode 0 complete, 1 predictor, 2 corrector."""
+        import torch
+        self._pool_ok(pool)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_hydro_boundary_dev(self._ctx, ctypes.c_void_p(pool.data_ptr()), int(level), int(mode)))
+
+    def fg2cg(self, pool, level, gamma=1.4):
+        """fg2cg_u(level): the level's grids restricted into their parents."""
+        import torch
+        self._pool_ok(pool)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_fg2cg_dev(self._ctx, ctypes.c_void_p(pool.data_ptr()), int(level), float(gamma)))
+
+    def fluxcorrection(self, pool, fb, level, dtime, gamma=1.4):
+        """fluxcorrection(level) with the face-flux buffers fb [nblk, 2, 6, 9, 64] (GPU float64)."""
+        import torch
+        self._pool_ok(pool)
+        self._pool_ok(fb)
+        if tuple(fb.shape) != (self.h.nblk, 2, 6, 9, 64):
+            raise ValueError("fb must be [nblk, 2, 6, 9, 64]")
+        dt = np.ascontiguousarray(dtime, np.float64)
+        if dt.shape != (self.h.nlev,):
+            raise ValueError("dtime holds one value per AMR level")
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_fluxcorrection_dev(self._ctx, ctypes.c_void_p(pool.data_ptr()),
+                                                       ctypes.c_void_p(fb.data_ptr()), int(level), _dp(dt), float(gamma)))
+
     # ---- operator-level access (parity tests) ----
     def get(self, L, code):
         n = 8 >> L

@@ -134,6 +134,13 @@ struct sg_amr {
     sg_blockmem bml{};
     int have_bm = 0;
     int *d_slot_u = nullptr, *d_slot_u1 = nullptr, *d_slot_u2 = nullptr;
+    // the hydro steps' item tables (build_hydro_tables, on first use): exchange items per (level,
+    // direction), restriction items per fine level, refluxing items per coarse level grouped in
+    // rounds of disjoint coarse cells
+    int hydro_ready = 0;
+    int4 *d_hx = nullptr, *d_fg = nullptr, *d_rf = nullptr;
+    std::vector<int> hx_off, fg_off;
+    std::vector<std::vector<int>> rf_rounds;   // per coarse level: offsets of its rounds, + end
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1378,6 +1385,7 @@ void release(sg_amr *s)
                  s->d_slot_u, s->d_slot_u1, s->d_slot_u2};
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->d_crop) (void)hipFree(s->d_crop);
+    for (int4 *q : {s->d_hx, s->d_fg, s->d_rf}) if (q) (void)hipFree(q);
     if (s->fb0) (void)hipFree(s->fb0);
     for (int L = 0; L < kNfmg; ++L) {
         if (s->d_prep_cov[L]) (void)hipFree(s->d_prep_cov[L]);
@@ -1909,6 +1917,103 @@ int sg_amr_op(sg_amr *ctx, int opc, int fmglev, int a, int b, int c, int flag, d
     return op(*ctx, opc, fmglev, a, b, c, flag, ret);
 }
 
+// ---- hydro steps: item tables ----
+namespace {
+int upload4(sg_amr &s, int4 **dptr, const std::vector<int4> &v)
+{
+    SG_HIP(hipMalloc((void **)dptr, std::max<size_t>(1, v.size()) * sizeof(int4)));
+    if (!v.empty()) SG_HIP(hipMemcpy(*dptr, v.data(), v.size() * sizeof(int4), hipMemcpyHostToDevice));
+    return SG_OK;
+}
+
+int build_hydro_tables(sg_amr &s)
+{
+    if (s.hydro_ready) return SG_OK;
+    const int nl = s.nlev;
+    std::vector<int4> hx, fg, rf;
+    s.hx_off.assign(3 * nl + 1, 0);
+    for (int l = 0; l < nl; ++l)
+        for (int d = 0; d < 3; ++d) {
+            s.hx_off[3 * l + d] = (int)hx.size();
+            for (int b = s.lev_start[l]; b < s.lev_start[l + 1]; ++b)
+                for (int lr = 0; lr < 2; ++lr) {
+                    const int f = 2 * d + lr, m = s.nbr[6 * b + f];
+                    if (m >= 0 && s.level[m] == l) {
+                        hx.push_back(make_int4(b, lr, 0, m));
+                    } else if (l > 0) {
+                        const int g = s.nbr[6 * s.parent[b] + f];
+                        if (g >= 0 && s.level[g] == l - 1) hx.push_back(make_int4(b, lr, 1 | (s.octant[b] << 1), g));
+                    }
+                }
+        }
+    s.hx_off[3 * nl] = (int)hx.size();
+    s.fg_off.assign(nl + 1, 0);
+    for (int l = 0; l < nl; ++l) {
+        s.fg_off[l] = (int)fg.size();
+        if (l > 0)
+            for (int b = s.lev_start[l]; b < s.lev_start[l + 1]; ++b) fg.push_back(make_int4(b, s.parent[b], s.octant[b], 0));
+    }
+    s.fg_off[nl] = (int)fg.size();
+    // refluxing: items in the reference's order, each in the round after the last one that
+    // corrected any of its coarse cells
+    s.rf_rounds.assign(nl, {});
+    for (int l = 0; l + 1 < nl; ++l) {
+        std::vector<int4> items;
+        std::vector<int> round;
+        std::vector<int> last;   // per (coarse block of level l, cell)
+        const int c0 = s.lev_start[l], nc = s.lev_start[l + 1] - c0;
+        last.assign((size_t)nc * 512, 0);
+        int nround = 0;
+        for (int b = s.lev_start[l + 1]; b < s.lev_start[l + 2]; ++b)
+            for (int f = 0; f < 6; ++f) {
+                const int m = s.nbr[6 * b + f];
+                if (m >= 0 && s.level[m] == l + 1) continue;
+                const int g = s.nbr[6 * s.parent[b] + f];
+                if (g < 0 || s.level[g] != l) continue;
+                const int d = f >> 1, lr = f & 1, o = s.octant[b];
+                const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+                std::vector<size_t> cells;
+                int r = 0;
+                for (int c2 = 0; c2 < 4; ++c2)
+                    for (int c1 = 0; c1 < 4; ++c1) {
+                        int pc[3];
+                        pc[d] = lr == 0 ? 7 : 0;
+                        pc[ta] = c1 + ((o >> ta) & 1) * 4;
+                        pc[tb] = c2 + ((o >> tb) & 1) * 4;
+                        const size_t id = (size_t)(g - c0) * 512 + (size_t)((pc[2] * 8 + pc[1]) * 8 + pc[0]);
+                        cells.push_back(id);
+                        r = std::max(r, last[id]);
+                    }
+                ++r;
+                for (size_t id : cells) last[id] = r;
+                items.push_back(make_int4(b, f + 8 * o, g, 0));
+                round.push_back(r);
+                nround = std::max(nround, r);
+            }
+        std::vector<int> &off = s.rf_rounds[l];
+        for (int r = 1; r <= nround; ++r) {
+            off.push_back((int)rf.size());
+            for (size_t q = 0; q < items.size(); ++q)
+                if (round[q] == r) rf.push_back(items[q]);
+        }
+        off.push_back((int)rf.size());
+    }
+    SG_TRY(upload4(s, &s.d_hx, hx));
+    SG_TRY(upload4(s, &s.d_fg, fg));
+    SG_TRY(upload4(s, &s.d_rf, rf));
+    s.hydro_ready = 1;
+    return SG_OK;
+}
+
+int hydro_checks(sg_amr &s, const char *who)
+{
+    SG_REQUIRE(s.have_bm, "%s: no block memory layout (sg_amr_set_blockmem)", who);
+    return SG_OK;
+}
+
+int is_hd9(const sg_blockmem &L) { return L.ncomp == 9; }
+} // namespace
+
 // ---- hydro block memory on the device ----
 int sg_amr_set_blockmem(sg_amr *ctx, const sg_blockmem *lay, long long pool_slots, const int *slot_u,
                         const int *slot_u1, const int *slot_u2)
@@ -1995,6 +2100,69 @@ int sg_amr_source_g_dev(sg_amr *ctx, double *pool, const double *dtime, double g
     const long W = 8 + 2 * s.bml.ng;
     SG_TRY(go(s, k_source_g, (long)s.nblk * W * W * W, pool, (const int *)s.d_slot_u, (const int *)s.d_slot_u1,
               (const int *)s.d_level, s.nblk, s.bml, dv, dt, gamma));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return SG_OK;
+}
+
+int sg_amr_hydro_boundary_dev(sg_amr *ctx, double *pool, int level, int mode)
+{
+    SG_REQUIRE(ctx && pool, "sg_amr_hydro_boundary_dev: null argument");
+    sg_amr &s = *ctx;
+    SG_TRY(hydro_checks(s, "sg_amr_hydro_boundary_dev"));
+    SG_REQUIRE(level >= 0 && level < s.nlev, "sg_amr_hydro_boundary_dev: level %d outside 0..%d", level, s.nlev - 1);
+    SG_REQUIRE(mode >= 0 && mode <= 2, "sg_amr_hydro_boundary_dev: unknown step mode %d", mode);
+    SG_REQUIRE(s.bml.ng >= 2 && s.bml.ng % 2 == 0, "sg_amr_hydro_boundary_dev: ng = %d (needs an even ng >= 2)",
+               s.bml.ng);
+    SG_REQUIRE(level == 0 || (s.d_slot_u1 && s.d_slot_u2),
+               "sg_amr_hydro_boundary_dev: the parent level's pass needs the predictor and past slots");
+    SG_TRY(build_hydro_tables(s));
+    const long W = 8 + 2 * s.bml.ng, slab = (long)s.bml.ng * W * W * s.bml.ncomp;
+    auto pass = [&](int l, int d, int ntab, int same_only) -> int {
+        const int a = s.hx_off[3 * l + d], n = s.hx_off[3 * l + d + 1] - a;
+        if (n == 0) return SG_OK;
+        return go(s, k_hx_pass, (long)n * slab * ntab, pool, (const int4 *)(s.d_hx + a), n, (const int *)s.d_slot_u,
+                  (const int *)s.d_slot_u1, (const int *)s.d_slot_u2, ntab, same_only, mode, s.bml, d);
+    };
+    if (level >= 1)
+        for (int d = 0; d < 3; ++d) SG_TRY(pass(level - 1, d, 3, 1));
+    for (int d = 0; d < 3; ++d) SG_TRY(pass(level, d, 1, 0));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return SG_OK;
+}
+
+int sg_amr_fg2cg_dev(sg_amr *ctx, double *pool, int level, double gamma)
+{
+    SG_REQUIRE(ctx && pool, "sg_amr_fg2cg_dev: null argument");
+    sg_amr &s = *ctx;
+    SG_TRY(hydro_checks(s, "sg_amr_fg2cg_dev"));
+    SG_REQUIRE(is_hd9(s.bml), "sg_amr_fg2cg_dev: the ideal-gas restriction takes the nine-component layout");
+    SG_REQUIRE(level >= 0 && level < s.nlev, "sg_amr_fg2cg_dev: level %d outside 0..%d", level, s.nlev - 1);
+    SG_REQUIRE(std::isfinite(gamma) && gamma != 1.0, "sg_amr_fg2cg_dev: bad gamma");
+    if (level < 1) return SG_OK;   // fg2cg.F90:25
+    SG_TRY(build_hydro_tables(s));
+    const int a = s.fg_off[level], n = s.fg_off[level + 1] - a;
+    SG_TRY(go(s, k_fg2cg, (long)n * 64, pool, (const int4 *)(s.d_fg + a), n, (const int *)s.d_slot_u, s.bml, gamma));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return SG_OK;
+}
+
+int sg_amr_fluxcorrection_dev(sg_amr *ctx, double *pool, const double *fb, int level, const double *dtime, double gamma)
+{
+    SG_REQUIRE(ctx && pool && fb && dtime, "sg_amr_fluxcorrection_dev: null argument");
+    sg_amr &s = *ctx;
+    SG_TRY(hydro_checks(s, "sg_amr_fluxcorrection_dev"));
+    SG_REQUIRE(is_hd9(s.bml), "sg_amr_fluxcorrection_dev: the ideal-gas refluxing takes the nine-component layout");
+    SG_REQUIRE(std::isfinite(gamma) && gamma != 1.0, "sg_amr_fluxcorrection_dev: bad gamma");
+    if (level < 0 || level >= s.nlev - 1) return SG_OK;   // reflux.F90:118-119
+    SG_TRY(build_hydro_tables(s));
+    const double hf = s.hl[0][level + 1], hc = s.hl[0][level];
+    const double dsf = hf * hf, dsc = hc * hc, dv = hc * hc * hc;   // get_ds, get_dv (grid.F90:405-421)
+    const std::vector<int> &off = s.rf_rounds[level];
+    for (size_t r = 0; r + 1 < off.size(); ++r) {
+        const int n = off[r + 1] - off[r];
+        SG_TRY(go(s, k_reflux_round, (long)n * 16, pool, (const int4 *)(s.d_rf + off[r]), n, (const int *)s.d_slot_u, fb,
+                  s.bml, dsf, dsc, dv, dtime[level], gamma));
+    }
     SG_HIP(hipStreamSynchronize(s.stream));
     return SG_OK;
 }

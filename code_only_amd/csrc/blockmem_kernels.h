@@ -135,5 +135,201 @@ __global__ void k_source_g(double *pool, const int *su, const int *s1, const int
     u[L.mgz * W3] = wgz / wr;
 }
 
+// ---- hydro steps around the solve: ghost exchange (boundary_grid), fine-to-coarse restriction
+//      (fg2cg_u), refluxing (fluxcorrection).  Faces are 2 d + lr as the reference's
+//      encode_surf (grid.F90:461-465). ----
+
+// One direction pass of boundary_grid (grid_boundary.F90:65-105) over items {block, lr, kind |
+// parity << 1, source}: kind 0 fix_from_samelev (grid_boundary.F90:238-342), the ghost slab (ng
+// layers along d, the whole extent of the other two directions) copied from the same-level
+// neighbour's boundary slab, on the first ntab of the current, predictor and past arrays; kind 1
+// fix_from_parent (:347-405) on the current array: interp_space's minmod-limited linear
+// interpolation (:409-530) from the coarse grid across the face, whose array the step mode picks
+// (SINGLE_STEP interp_time, :622-639: past for the predictor step, predictor for the corrector
+// step, else current).  same_only skips kind 1 (the parent level's pass).  One thread per cell of
+// a slab per array and component; a pass writes only ghost slabs and reads only interior slabs
+// along d and coarse cells, so its items are independent.
+__device__ __forceinline__ double minmod(double a, double b)
+{
+    const double s = copysign(1.0, a);
+    return s * fmax(0.0, fmin(fabs(a), s * b));
+}
+
+__global__ void k_hx_pass(double *pool, const int4 *items, int nitems, const int *su, const int *s1, const int *s2,
+                          int ntab, int same_only, int mode, sg_blockmem L, int d)
+{
+    const int ng = L.ng, W = 8 + 2 * ng;
+    const long W3 = (long)W * W * W;
+    const long slab = (long)ng * W * W, per_tab = slab * L.ncomp, per_item = per_tab * ntab;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nitems * per_item) return;
+    const int it = (int)(t / per_item);
+    long rem = t - (long)it * per_item;
+    const int tab = (int)(rem / per_tab);
+    rem -= (long)tab * per_tab;
+    const int c = (int)(rem / slab);
+    rem -= (long)c * slab;
+    const int r = (int)(rem / (W * W)), t2 = (int)((rem / W) % W) - ng, t1 = (int)(rem % W) - ng;
+    const int4 e = items[it];
+    const int b = e.x, lr = e.y, kind = e.z & 1, par = e.z >> 1;
+    if (kind == 1 && (same_only || tab > 0)) return;
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    int p[3];
+    p[d] = lr == 0 ? -ng + r : 8 + r;
+    p[ta] = t1;
+    p[tb] = t2;
+    const int *tb_slots = tab == 0 ? su : (tab == 1 ? s1 : s2);
+    double *ud = pool + (long)tb_slots[b] * L.stride + c * W3;
+    if (kind == 0) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[d] = lr == 0 ? p[d] + 8 : p[d] - 8;
+        const double *us = pool + (long)tb_slots[e.w] * L.stride + c * W3;
+        ud[bm_cell(L, p[0], p[1], p[2])] = us[bm_cell(L, q[0], q[1], q[2])];
+        return;
+    }
+    // the coarse buffer Ics..Ice (grid_boundary.F90:182-234) and the fine slab's start, per axis
+    int ci[3];
+    for (int a = 0; a < 3; ++a) {
+        int bs, fs;
+        if (a == d) {
+            bs = lr == 0 ? 7 - ng / 2 : -ng / 2;
+            fs = lr == 0 ? -ng : 8;
+        } else {
+            bs = ((par >> a) & 1) ? 4 - ng : -ng;
+            fs = -ng;
+        }
+        ci[a] = (p[a] - fs) / 2 + bs + 1;   // IJKC (config.h:249) relative to the buffer
+    }
+    const int *cs = mode == 1 ? s2 : (mode == 2 ? s1 : su);
+    const double *bu = pool + (long)cs[e.w] * L.stride + c * W3;
+    const double v = bu[bm_cell(L, ci[0], ci[1], ci[2])];
+    double g3[3];
+    for (int a = 0; a < 3; ++a) {
+        int up[3] = {ci[0], ci[1], ci[2]}, dn[3] = {ci[0], ci[1], ci[2]};
+        up[a] += 1;
+        dn[a] -= 1;
+        const double dr = (bu[bm_cell(L, up[0], up[1], up[2])] - v) / 2.0;
+        const double dl = (v - bu[bm_cell(L, dn[0], dn[1], dn[2])]) / 2.0;
+        g3[a] = minmod(dr, dl);
+    }
+    double res = v + g3[0] * ((double)(p[0] & 1) - 0.5);
+    res = res + g3[1] * ((double)(p[1] & 1) - 0.5);
+    res = res + g3[2] * ((double)(p[2] & 1) - 0.5);
+    ud[bm_cell(L, p[0], p[1], p[2])] = res;
+}
+
+// the ideal-gas u2w / w2u of one cell (eos_adiabatic.F90:226-266) over the nine labelled
+// components, w in the order rho, rho v, E, psi, rho g
+__device__ __forceinline__ void u2w9(const double *u, long s, const sg_blockmem &L, double dv, double gamma, double w[9])
+{
+    const double rho = u[L.mrho * s], vx = u[L.mvx * s], vy = u[L.mvy * s], vz = u[L.mvz * s];
+    w[0] = rho * dv;
+    w[1] = vx * w[0];
+    w[2] = vy * w[0];
+    w[3] = vz * w[0];
+    double kin = vx * vx + vy * vy;
+    kin = kin + vz * vz;
+    w[4] = dv * (u[L.mp * s] / (gamma - 1.0) + rho * kin / 2.0);
+    w[5] = u[L.mpsi * s] * dv;
+    w[6] = u[L.mgx * s] * rho * dv;
+    w[7] = u[L.mgy * s] * rho * dv;
+    w[8] = u[L.mgz * s] * rho * dv;
+}
+
+__device__ __forceinline__ void w2u9(const double w[9], double *u, long s, const sg_blockmem &L, double dv, double gamma)
+{
+    const double r = w[0] / dv, x = w[1] / w[0], y = w[2] / w[0], z = w[3] / w[0];
+    double k2 = x * x + y * y;
+    k2 = k2 + z * z;
+    u[L.mrho * s] = r;
+    u[L.mvx * s] = x;
+    u[L.mvy * s] = y;
+    u[L.mvz * s] = z;
+    u[L.mp * s] = (w[4] / dv - 0.5 * r * k2) * (gamma - 1.0);
+    u[L.mpsi * s] = w[5] / dv;
+    u[L.mgx * s] = w[6] / w[0];
+    u[L.mgy * s] = w[7] / w[0];
+    u[L.mgz * s] = w[8] / w[0];
+}
+
+__device__ __forceinline__ int label9(const sg_blockmem &L, int q)
+{
+    int m = L.mrho;
+    m = q == 1 ? L.mvx : m;
+    m = q == 2 ? L.mvy : m;
+    m = q == 3 ? L.mvz : m;
+    m = q == 4 ? L.mp : m;
+    m = q == 5 ? L.mpsi : m;
+    m = q == 6 ? L.mgx : m;
+    m = q == 7 ? L.mgy : m;
+    m = q == 8 ? L.mgz : m;
+    return m;
+}
+
+// fg2cg_u (fg2cg.F90:22-194): items {fine block, parent, octant}; one thread per coarse cell of
+// the octant: u2w (dv 1) of the eight children, summed in restrict_u's order, w2u (dv 8) into the
+// parent
+__global__ void k_fg2cg(double *pool, const int4 *items, int nitems, const int *su, sg_blockmem L, double gamma)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nitems * 64) return;
+    const int4 e = items[t >> 6];
+    const int q = (int)(t & 63), ic = q & 3, jc = (q >> 2) & 3, kc = q >> 4;
+    const long W = 8 + 2 * L.ng, W3 = W * W * W;
+    const double *uf = pool + (long)su[e.x] * L.stride;
+    double *uc = pool + (long)su[e.y] * L.stride;
+    const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    double sum[9];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        double w[9];
+        u2w9(uf + bm_cell(L, 2 * ic + off[n][0], 2 * jc + off[n][1], 2 * kc + off[n][2]), W3, L, 1.0, gamma, w);
+#pragma unroll
+        for (int m = 0; m < 9; ++m) sum[m] = n == 0 ? w[m] : sum[m] + w[m];
+    }
+    const int o = e.z;
+    w2u9(sum, uc + bm_cell(L, ic + (o & 1) * 4, jc + ((o >> 1) & 1) * 4, kc + (o >> 2) * 4), W3, L, 8.0, gamma);
+}
+
+// fluxcorrection's com_restrict (reflux.F90:193-295) for one round of items {fine block, face +
+// 8 parity, coarse grid, -}: items of a round touch disjoint coarse cells, and rounds keep the
+// reference's order (grid by grid, face by face) for cells two faces correct.  One thread per
+// coarse cell of the face quarter: restrict_f's 2 x 2 sum (reflux.F90:300-388) times dsf / dsc,
+// u2w, w -/+= (that - the coarse boundary flux) dt ds, w2u.  fb[b][kind][face][m][64] (kind 0
+// _bnd, 1 _sum).
+__global__ void k_reflux_round(double *pool, const int4 *items, int nitems, const int *su, const double *fb,
+                               sg_blockmem L, double dsf, double dsc, double dv, double dt, double gamma)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nitems * 16) return;
+    const int4 e = items[t >> 4];
+    const int c1 = (int)(t & 3), c2 = (int)((t >> 2) & 3);
+    const int f = e.y & 7, par = e.y >> 3, d = f >> 1, lr = f & 1;
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const long W = 8 + 2 * L.ng, W3 = W * W * W;
+    int pc[3];
+    pc[d] = lr == 0 ? 7 : 0;
+    pc[ta] = c1 + ((par >> ta) & 1) * 4;
+    pc[tb] = c2 + ((par >> tb) & 1) * 4;
+    double *u = pool + (long)su[e.z] * L.stride + bm_cell(L, pc[0], pc[1], pc[2]);
+    const double *ff = fb + (((long)e.x * 2 + 1) * 6 + f) * 9 * 64;
+    const double *fcb = fb + (((long)e.z * 2 + 0) * 6 + (2 * d + 1 - lr)) * 9 * 64;
+    double w[9];
+    u2w9(u, W3, L, dv, gamma, w);
+    const int s1 = 2 * c1, s2 = 2 * c2;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int m = label9(L, q);
+        const double *a = ff + m * 64;
+        double sm = a[s1 + 8 * s2] + a[s1 + 8 * (s2 + 1)];
+        sm = sm + a[s1 + 1 + 8 * s2];
+        sm = sm + a[s1 + 1 + 8 * (s2 + 1)];
+        const double fr = sm * dsf / dsc;
+        const double corr = (fr - fcb[m * 64 + pc[ta] + 8 * pc[tb]]) * dt * dsc;
+        w[q] = lr == 0 ? w[q] - corr : w[q] + corr;
+    }
+    w2u9(w, u, W3, L, dv, gamma);
+}
+
 } // namespace amr
 } // namespace sg

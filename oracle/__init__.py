@@ -91,6 +91,9 @@ def lib():
         L.orc_bm_stage.argtypes = [dp, ip, ip, ip, ci, ip, bp, tp, cd, dp, dp]
         L.orc_bm_restore.argtypes = [dp, ip, ci, bp, dp, dp]
         L.orc_bm_source_g.argtypes = [dp, ip, ip, ci, ip, dp, dp, bp, cd]
+        L.orc_bm_grid_boundary.argtypes = [dp, ip, ip, ip, ci, ip, ip, ip, ip, bp, ci, ci]
+        L.orc_bm_fg2cg.argtypes = [dp, ip, ci, ip, ip, ip, bp, ci, cd]
+        L.orc_bm_fluxcorrection.argtypes = [dp, ip, dp, ci, ip, ip, ip, ip, dp, dp, bp, ci, cd]
         _LIB = L
     return _LIB
 
@@ -326,3 +329,32 @@ class Amr:
                      rel_l2=st.rel_l2, rel_h2=st.rel_h2, ratio=st.ratio,
                      hist=[st.hist[i] for i in range(min(st.iterations + 1, HIST))])
         return psi, g, stats
+
+
+def _tables(h):
+    return [np.ascontiguousarray(a, np.int32) for a in (h.level, h.parent, h.child, h.nbr)]
+
+
+def bm_grid_boundary(pool, lay, h, slot_u, slot_u1, slot_u2, level, mode):
+    """boundary_grid(level, mode) (SINGLE_STEP) on the pool, in place."""
+    lv, pa, ch, nb = _tables(h)
+    lib().orc_bm_grid_boundary(_p(pool), _ip(_slots(slot_u)), _ip(_slots(slot_u1)), _ip(_slots(slot_u2)), h.nblk,
+                               _ip(lv), _ip(pa), _ip(ch), _ip(nb), ctypes.byref(lay), int(level), int(mode))
+
+
+def bm_fg2cg(pool, lay, h, slot_u, level, gamma):
+    """fg2cg_u(level) on the pool, in place."""
+    lv, pa, ch, _ = _tables(h)
+    lib().orc_bm_fg2cg(_p(pool), _ip(_slots(slot_u)), h.nblk, _ip(lv), _ip(pa), _ip(ch), ctypes.byref(lay),
+                       int(level), float(gamma))
+
+
+def bm_fluxcorrection(pool, lay, h, slot_u, fb, level, dt, gamma):
+    """fluxcorrection(level) on the pool with the face-flux buffers fb [nblk][2][6][ncomp][64]."""
+    lv, pa, ch, nb = _tables(h)
+    hl = np.array([h.h(l) for l in range(h.nlev)] + [0.0] * (MAXLEV - h.nlev))
+    dtv = np.zeros(MAXLEV)
+    dtv[:len(dt)] = dt
+    lib().orc_bm_fluxcorrection(_p(pool), _ip(_slots(slot_u)), _p(np.ascontiguousarray(fb, np.float64)), h.nblk,
+                                _ip(lv), _ip(pa), _ip(ch), _ip(nb), _p(hl), _p(dtv), ctypes.byref(lay), int(level),
+                                float(gamma))

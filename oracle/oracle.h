@@ -124,5 +124,12 @@ void orc_bm_restore(double *pool, const int *slot_u, int nblk, const orc_blockme
                     const double *g);
 void orc_bm_source_g(double *pool, const int *slot_u, const int *slot_u1, int nblk, const int *level,
                      const double *dv_lev, const double *dt_lev, const orc_blockmem *L, double gamma);
+void orc_bm_grid_boundary(double *pool, const int *su, const int *s1, const int *s2, int nblk, const int *level,
+                          const int *parent, const int *child, const int *nbr, const orc_blockmem *L, int lev, int mode);
+void orc_bm_fg2cg(double *pool, const int *su, int nblk, const int *level, const int *parent, const int *child,
+                  const orc_blockmem *L, int lev, double gamma);
+void orc_bm_fluxcorrection(double *pool, const int *su, const double *fb, int nblk, const int *level,
+                           const int *parent, const int *child, const int *nbr, const double *h_lev,
+                           const double *dt_lev, const orc_blockmem *L, int lev, double gamma);
 
 #endif
