@@ -537,6 +537,22 @@ def bench_blockmem(a, dev):
     # written, the interior's density and velocity of the predictor stage read
     per_grid = 2 * 9 * W ** 3 * 8 + 4 * 512 * 8
     gbs = per_grid * nb / (src_ms * 1e-3) / 1e9
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) * 1e3 / reps
+    # the hydro steps around the solve, every level of the hierarchy (the reference calls them
+    # level by level in the time step)
+    hydro = {"boundary_grid_ms": [round(timed(lambda l=l: s.hydro_boundary(pool, l, 0)), 4) for l in range(h.nlev)],
+             "fg2cg_ms": [round(timed(lambda l=l: s.fg2cg(pool, l)), 4) for l in range(1, h.nlev)]}
+    fb = torch.randn((nb, 2, 6, 9, 64), dtype=torch.float64, device=dev, generator=gen)
+    hydro["fluxcorrection_ms"] = [round(timed(lambda l=l: s.fluxcorrection(pool, fb, l, dt)), 4)
+                                  for l in range(h.nlev - 1)]
+    del fb
     s.close()
     out = {"workload": "C2 sphere hierarchy (20 480 grids) in a device pool of ideal-gas HD arrays "
                        "(9 components, 12^3 with 2 ghost layers, 3 shuffled slots per grid, 7.6 GB)",
@@ -545,7 +561,8 @@ def bench_blockmem(a, dev):
            "iterations": [st_s["iterations"], st_d["iterations"]],
            "rel_linf": [st_s["rel_linf"], st_d["rel_linf"]],
            "source_g_ms": round(src_ms, 4), "source_g_bytes": per_grid * nb,
-           "source_g_gbs": round(gbs, 1), "source_g_frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+           "source_g_gbs": round(gbs, 1), "source_g_frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+           "hydro_steps_per_level": hydro}
     log(f"[bench] blockmem: {out}")
     return out
 

@@ -2117,11 +2117,16 @@ int sg_amr_hydro_boundary_dev(sg_amr *ctx, double *pool, int level, int mode)
                "sg_amr_hydro_boundary_dev: the parent level's pass needs the predictor and past slots");
     SG_TRY(build_hydro_tables(s));
     const long W = 8 + 2 * s.bml.ng, slab = (long)s.bml.ng * W * W * s.bml.ncomp;
+    SG_REQUIRE(3 * slab <= 65535L * 256, "sg_amr_hydro_boundary_dev: ncomp too large for the launch shape");
     auto pass = [&](int l, int d, int ntab, int same_only) -> int {
         const int a = s.hx_off[3 * l + d], n = s.hx_off[3 * l + d + 1] - a;
         if (n == 0) return SG_OK;
-        return go(s, k_hx_pass, (long)n * slab * ntab, pool, (const int4 *)(s.d_hx + a), n, (const int *)s.d_slot_u,
-                  (const int *)s.d_slot_u1, (const int *)s.d_slot_u2, ntab, same_only, mode, s.bml, d);
+        const dim3 grid((unsigned)n, (unsigned)((slab * ntab + 255) / 256));
+        hipLaunchKernelGGL(k_hx_pass, grid, dim3(256), 0, s.stream, pool, (const int4 *)(s.d_hx + a), n,
+                           (const int *)s.d_slot_u, (const int *)s.d_slot_u1, (const int *)s.d_slot_u2, ntab, same_only,
+                           mode, s.bml, d);
+        SG_CHECK_LAUNCH();
+        return SG_OK;
     };
     if (level >= 1)
         for (int d = 0; d < 3; ++d) SG_TRY(pass(level - 1, d, 3, 1));

@@ -146,9 +146,10 @@ __global__ void k_source_g(double *pool, const int *su, const int *s1, const int
 // fix_from_parent (:347-405) on the current array: interp_space's minmod-limited linear
 // interpolation (:409-530) from the coarse grid across the face, whose array the step mode picks
 // (SINGLE_STEP interp_time, :622-639: past for the predictor step, predictor for the corrector
-// step, else current).  same_only skips kind 1 (the parent level's pass).  One thread per cell of
-// a slab per array and component; a pass writes only ghost slabs and reads only interior slabs
-// along d and coarse cells, so its items are independent.
+// step, else current).  same_only skips kind 1 (the parent level's pass).  Grid (items, chunks):
+// one thread per cell of a slab per array and component, consecutive threads along the slab's
+// contiguous axis (i; for x slabs that is the ng-wide layer, then j); a pass writes only ghost
+// slabs and reads only interior slabs along d and coarse cells, so its items are independent.
 __device__ __forceinline__ double minmod(double a, double b)
 {
     const double s = copysign(1.0, a);
@@ -160,16 +161,24 @@ __global__ void k_hx_pass(double *pool, const int4 *items, int nitems, const int
 {
     const int ng = L.ng, W = 8 + 2 * ng;
     const long W3 = (long)W * W * W;
-    const long slab = (long)ng * W * W, per_tab = slab * L.ncomp, per_item = per_tab * ntab;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nitems * per_item) return;
-    const int it = (int)(t / per_item);
-    long rem = t - (long)it * per_item;
-    const int tab = (int)(rem / per_tab);
-    rem -= (long)tab * per_tab;
-    const int c = (int)(rem / slab);
-    rem -= (long)c * slab;
-    const int r = (int)(rem / (W * W)), t2 = (int)((rem / W) % W) - ng, t1 = (int)(rem % W) - ng;
+    const int slab = ng * W * W, per_tab = slab * L.ncomp;
+    const int it = blockIdx.x;
+    int rem = blockIdx.y * blockDim.x + threadIdx.x;
+    if (rem >= per_tab * ntab) return;
+    const int tab = rem / per_tab;
+    rem -= tab * per_tab;
+    const int c = rem / slab;
+    rem -= c * slab;
+    int r, t1, t2;
+    if (d == 0) {   // (t2, t1, r): r fastest
+        r = rem % ng;
+        t1 = (rem / ng) % W - ng;
+        t2 = rem / (ng * W) - ng;
+    } else {        // (r, t2, t1): t1 (= i) fastest
+        t1 = rem % W - ng;
+        t2 = (rem / W) % W - ng;
+        r = rem / (W * W);
+    }
     const int4 e = items[it];
     const int b = e.x, lr = e.y, kind = e.z & 1, par = e.z >> 1;
     if (kind == 1 && (same_only || tab > 0)) return;
