@@ -93,6 +93,9 @@ struct sg_amr {
     float blk8_ms[4][3] = {};
     // FMG-level-0 face store of U (FP32, fused mode): 6 x 64 values per block (amr_kernels.h)
     float *fb0 = nullptr;
+    // the inner solve's FMG-level-0 source in the fused mode, compact and colour-split
+    // (amr_kernels.h cc_index), 512 values per active block
+    float *rhs0c = nullptr;
     int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
@@ -353,27 +356,37 @@ constexpr int kBlk8Nt[3] = {64, 128, 256};
 // (cast, npart: the residual's FP32 copy and norm partials, k_blk8)
 template <bool RELAX, typename T>
 int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo,
-         T *fb = nullptr, float *cast = nullptr, double *npart = nullptr)
+         T *fb = nullptr, float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr)
 {
     const dim3 g((unsigned)s.active.size());
     const int *rec = s.d_arec;
     T *fbout = RELAX ? fb : nullptr;
     if (nt == 64)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 64>), g, dim3(64), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0],
-                           ov, resmax, halo, (const T *)fb, fbout, cast, npart);
+                           ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc);
     else if (nt == 128)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 128>), g, dim3(128), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc);
     else
         hipLaunchKernelGGL((k_blk8<RELAX, T, 256>), g, dim3(256), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
 
 template <typename T> constexpr int kBlk8Slot = std::is_same<T, float>::value ? 0 : 2;
 
-// The autotuner of the FMG-level-0 kernels: each candidate's time over 4 launches (after one
+// the inner solve's FMG-level-0 source in compact form, where the fused mode keeps it there (the
+// outer residual writes it; RHO at level 0 is read through it)
+template <typename T> inline const float *src0c(const sg_amr &s, int L, int jrhs)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        if (s.fused_on && L == 0 && jrhs == C_RHO) return s.rhs0c;
+    }
+    return nullptr;
+}
+
+// The autotuner of the FMG-level-0 kernels: each candidate's time over 16 launches (after one
 // warm-up) on the context's own arrays, relaxation and residual apart.  The values it leaves in U
 // and RES do not matter: every solve starts from a cleared state.
 template <typename T> int tune_blk8(sg_amr &s)
@@ -382,25 +395,29 @@ template <typename T> int tune_blk8(sg_amr &s)
     auto &R = ar<T>(s);
     const int halo = s.fused_ok[0];
     T *fb = nullptr;
-    if constexpr (std::is_same<T, float>::value) fb = halo ? s.fb0 : nullptr;
+    const float *rc = nullptr;   // the solve's FP32 passes read the compact source
+    if constexpr (std::is_same<T, float>::value) {
+        fb = halo ? s.fb0 : nullptr;
+        rc = s.rhs0c;
+    }
     for (int which = 0; which < 2; ++which) {
         float best = 1e30f;
         for (int c = 0; c < 3; ++c) {
             const int nt = kBlk8Nt[c];
-            for (int rep = 0; rep < 5; ++rep) {
+            for (int rep = 0; rep < 17; ++rep) {
                 if (rep == 1) SG_HIP(hipEventRecord(s.ev0, s.stream));
                 if (which == 0)
                     SG_TRY((blk8<true, T>(s, nt, (T *)nullptr, R.A[0][C_U], R.A[0][C_RHS], 1 + (rep & 1), R.ov[0], nullptr, halo,
-                                          fb)));
+                                          fb, nullptr, nullptr, rc)));
                 else
                     SG_TRY((blk8<false, T>(s, nt, R.A[0][C_RES], R.A[0][C_U], R.A[0][C_RHS], 0, R.ov[0], nullptr, halo,
-                                           fb)));
+                                           fb, nullptr, nullptr, rc)));
             }
             SG_HIP(hipEventRecord(s.ev1, s.stream));
             SG_HIP(hipEventSynchronize(s.ev1));
             float ms = 0.f;
             SG_HIP(hipEventElapsedTime(&ms, s.ev0, s.ev1));
-            s.blk8_ms[kBlk8Slot<T> + which][c] = ms / 4;
+            s.blk8_ms[kBlk8Slot<T> + which][c] = ms / 16;
             if (ms < best) {
                 best = ms;
                 s.blk8_nt[kBlk8Slot<T> + which] = nt;
@@ -435,7 +452,8 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
         const bool hf = fused(s, L);
         if (L == 0) {
             SG_TRY((blk8<true, T>(s, s.blk8_nt[kBlk8Slot<T>], (T *)nullptr, ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], ipass,
-                                  ar<T>(s).ov[0], slots, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr)));
+                                  ar<T>(s).ov[0], slots, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr, nullptr,
+                                  nullptr, src0c<T>(s, 0, jrhs))));
         } else {
             SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_arec, (int)s.active.size(), ipass, true, slots, hf));
         }
@@ -462,7 +480,8 @@ template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int 
     if (L == 0) {
         const bool hf = fused(s, 0);
         return blk8<false, T>(s, s.blk8_nt[kBlk8Slot<T> + 1], ar<T>(s).A[0][jres], ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], 0,
-                              ar<T>(s).ov[0], nullptr, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr);
+                              ar<T>(s).ov[0], nullptr, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr, nullptr,
+                              nullptr, src0c<T>(s, 0, jrhs));
     }
     SG_NDISPATCH(L, resid_n, s, L, jres, ju, jrhs);
 }
@@ -472,6 +491,8 @@ template <typename T> int fmg_rstrct(sg_amr &s, int Lc, int iuc, int iuf)
 {
     int na = (int)s.active.size();
     auto &R = ar<T>(s);
+    if (Lc == 1 && src0c<T>(s, 0, iuf))
+        return go(s, k_rstrct_cc, (long)na * 64, s.f.A[1][iuc], (const float *)s.rhs0c, (const int *)s.d_active, na);
     if (Lc == 1)
         return go(s, k_rstrct<4, T>, (long)na * 64, R.A[1][iuc], (const T *)R.A[0][iuf], (const int *)s.d_active, na);
     return go(s, k_rstrct<2, T>, (long)na * 8, R.A[2][iuc], (const T *)R.A[1][iuf], (const int *)s.d_active, na);
@@ -1095,6 +1116,8 @@ int ensure_mixed(sg_amr &s)
     }
     SG_HIP(hipMalloc((void **)&s.fb0, sizeof(float) * 6 * 64 * (size_t)s.nblk));
     SG_HIP(hipMemset(s.fb0, 0, sizeof(float) * 6 * 64 * (size_t)s.nblk));
+    SG_HIP(hipMalloc((void **)&s.rhs0c, sizeof(float) * 512 * std::max<size_t>(1, s.active.size())));
+    SG_HIP(hipMemset(s.rhs0c, 0, sizeof(float) * 512 * std::max<size_t>(1, s.active.size())));
     s.have_f = 1;
     return tune_blk8<float>(s);
 }
@@ -1111,7 +1134,7 @@ int outer_resid(sg_amr &s)
     }
     SG_TRY(poisson_flux<double>(s, 0, C_PSI, 0));
     SG_TRY((blk8<false, double>(s, s.blk8_nt[kBlk8Slot<double> + 1], s.d.A[0][C_RHO], s.d.A[0][C_PSI], s.d.A[0][C_SRC], 0,
-                                s.d.ov[0], nullptr, fused(s, 0) ? 1 : 0, nullptr, s.f.A[0][C_RHO], s.d_npart)));
+                                s.d.ov[0], nullptr, fused(s, 0) ? 1 : 0, nullptr, s.rhs0c, s.d_npart)));
     hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(1024), 0, s.stream, (const double *)s.d_npart, (int)s.active.size(),
                        s.d_scal + 2);
     SG_CHECK_LAUNCH();
@@ -1387,6 +1410,7 @@ void release(sg_amr *s)
     if (s->d_crop) (void)hipFree(s->d_crop);
     for (int4 *q : {s->d_hx, s->d_fg, s->d_rf}) if (q) (void)hipFree(q);
     if (s->fb0) (void)hipFree(s->fb0);
+    if (s->rhs0c) (void)hipFree(s->rhs0c);
     for (int L = 0; L < kNfmg; ++L) {
         if (s->d_prep_cov[L]) (void)hipFree(s->d_prep_cov[L]);
         if (s->d_prep_src[L]) (void)hipFree(s->d_prep_src[L]);
@@ -1889,7 +1913,7 @@ int sg_amr_time_smoother(sg_amr *ctx, int prec, int reps, double *ms_per_sweep)
         for (int ipass = 1; ipass <= 2; ++ipass) {
             if (prec == SG_PREC_MIXED)
                 SG_TRY((blk8<true, float>(s, s.blk8_nt[0], (float *)nullptr, s.f.A[0][C_U], s.f.A[0][C_RHS], ipass, s.f.ov[0],
-                                          nullptr, halo, halo ? s.fb0 : nullptr)));
+                                          nullptr, halo, halo ? s.fb0 : nullptr, nullptr, nullptr, s.rhs0c)));
             else
                 SG_TRY((blk8<true, double>(s, s.blk8_nt[2], (double *)nullptr, s.d.A[0][C_U], s.d.A[0][C_RHS], ipass,
                                            s.d.ov[0], nullptr, 0)));

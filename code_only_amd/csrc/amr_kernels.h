@@ -81,6 +81,28 @@ template <int N, typename T> __device__ __forceinline__ T rstr_lin(const T *uf, 
     return s * (T)(1.0 / 8.0);
 }
 
+// The inner solve's FMG-level-0 source in compact colour-split form: per block (in the order of
+// the active list) the 256 cells of the first colour (i + j + k odd), then the 256 of the second,
+// each colour in the order of k_blk8's colour threads, i/2 + 4 j + 32 k.  A colour pass reads its
+// 1 KB contiguously instead of picking its cells out of the ghosted 10^3 array.
+__device__ __forceinline__ int cc_index(int i, int j, int k)
+{
+    return ((i + j + k + 1) & 1) * 256 + (i >> 1) + 4 * j + 32 * k;
+}
+
+// restriction.F90:127-151 from the compact source: the operand order of rstr_lin
+__device__ __forceinline__ float rstr_lin_cc(const float *uc, int i, int j, int k)
+{
+    float s = uc[cc_index(i, j, k)] + uc[cc_index(i + 1, j, k)];
+    s = s + uc[cc_index(i, j + 1, k)];
+    s = s + uc[cc_index(i, j, k + 1)];
+    s = s + uc[cc_index(i + 1, j + 1, k)];
+    s = s + uc[cc_index(i + 1, j, k + 1)];
+    s = s + uc[cc_index(i, j + 1, k + 1)];
+    s = s + uc[cc_index(i + 1, j + 1, k + 1)];
+    return s * (1.0f / 8.0f);
+}
+
 // restriction.F90:71-119 (stencil i1-1 .. i1+2 per axis)
 template <int N, typename T> __device__ __forceinline__ T rstr_cub(const T *uf, int i1, int j1, int k1, const Cubic64 &st)
 {
@@ -700,6 +722,16 @@ template <int NC, typename T> __global__ void k_rstrct(T *Ac, const T *Af, const
     Ac[(long)b * Blk<NC>::LEN + Blk<NC>::gi(i, j, k)] = rstr_lin<2 * NC, T>(Af + (long)b * Blk<2 * NC>::LEN, 2 * i, 2 * j, 2 * k);
 }
 
+// fmg_rstrct from FMG level 0 to 1 with the level-0 source in compact form (src + a 512 for list
+// entry a)
+__global__ void k_rstrct_cc(float *Ac, const float *src, const int *list, int nb)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * 64) return;
+    const int i = (int)(t & 3), j = (int)((t >> 2) & 3), k = (int)((t >> 4) & 3), a = (int)(t >> 6);
+    Ac[(long)list[a] * Blk<4>::LEN + Blk<4>::gi(i, j, k)] = rstr_lin_cc(src + (long)a * 512, 2 * i, 2 * j, 2 * k);
+}
+
 // ---- fmg_interp_cubic / fmg_interp_linear: fine interior of each listed block from its own
 //      coarser FMG level (ghosts included) ----
 template <int NF, bool CUBIC, typename T> __global__ void k_interp(T *Af, const T *Ac, const int *list, int nb)
@@ -1177,14 +1209,15 @@ __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
     return m;
 }
 
-// The FP64 residual of the mixed solve can also write its FP32 copy (cast, same offsets) and its
-// block's norm partials (npart[g], npart[G + g], npart[2G + g] for workgroup g of G: the per-cell
+// The FP64 residual of the mixed solve can also write its FP32 copy (cast, in the compact
+// colour-split form of cc_index, which rhsc then feeds the FP32 passes from) and its block's
+// norm partials (npart[g], npart[G + g], npart[2G + g] for workgroup g of G: the per-cell
 // terms of k_norms_part, so the two max norms are the same values; the sum adds in another order).
 template <bool RELAX, typename T, int NT>
 __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
                                             int halo, const T *fbin = nullptr, T *fbout = nullptr,
-                                            float *cast = nullptr, double *npart = nullptr)
+                                            float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr)
 {
     using B = Blk<8>;
     constexpr int NC = RELAX ? 256 / NT : 512 / NT;   // cells per thread
@@ -1207,6 +1240,12 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             ci[r] = c & 7;
             cj[r] = (c >> 3) & 7;
             ck[r] = c >> 6;
+        }
+        if constexpr (std::is_same<T, float>::value) {
+            if (rhsc) {
+                rhs[r] = rhsc[(long)blockIdx.x * 512 + (RELAX ? (ipass - 1) * 256 + c : cc_index(ci[r], cj[r], ck[r]))];
+                continue;
+            }
         }
         rhs[r] = Arhs[(long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r])];
     }
@@ -1247,7 +1286,7 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             const T v = rhs[r] - F.div(ci[r], cj[r], ck[r]) * hi;
             Aout[q] = v;
             if constexpr (std::is_same<T, double>::value) {
-                if (cast) cast[q] = (float)v;
+                if (cast) cast[(long)blockIdx.x * 512 + cc_index(ci[r], cj[r], ck[r])] = (float)v;
                 if (npart) {
                     const double av = fabs(v);
                     a = fmax(a, av);
