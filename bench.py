@@ -137,10 +137,17 @@ def bench_amr(a, dev):
     solver.set_base_solver(False)
     ms64_wg, _ = run(1, 1, 1)
     solver.set_base_solver(True)
+    # and with the mixed solve's launch structure in FP64 (fused refresh, merged VMG steps: the
+    # same values), so that mixed / this is the part of the speed-up precision buys
+    solver.set_fp64_refresh(True)
+    ms64_fz, st_fz = run(1, max(1, min(steps, 3)), 1)
+    pfz = p_t.clone()
+    solver.set_fp64_refresh(False)
     ms64, st = run(1, max(1, min(steps, 3)), 1)
-    log(f"[bench] AMR fp64: {ms64:.2f} ms/solve ({ms64_wg:.2f} with the one-workgroup base solve), "
-        f"{st['iterations']} it, rel {st['rel_linf']:.3e}")
+    log(f"[bench] AMR fp64: {ms64:.2f} ms/solve ({ms64_wg:.2f} with the one-workgroup base solve, "
+        f"{ms64_fz:.2f} with the fused refresh), {st['iterations']} it, rel {st['rel_linf']:.3e}")
     p64, g64 = p_t.clone(), g_t.clone()
+    fz_same = bool(torch.equal(pfz, p64)) and st_fz["iterations"] == st["iterations"]
     msmx, stm = run(SG_PREC_MIXED, max(1, steps), a.warmup)
     err_p = ((p_t - p64).norm() / p64.norm()).item()
     err_g = ((g_t - g64).norm() / g64.norm()).item()
@@ -164,6 +171,8 @@ def bench_amr(a, dev):
         "workload": AMR_WORKLOAD, "blocks": hier.nblk, "leaf_cells": hier.cells(),
         "mixed_gpu_ms": round(msmx, 3), "fp64_gpu_ms": round(ms64, 3), "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
         "fp64_gpu_ms_one_workgroup_base_solve": round(ms64_wg, 3),
+        "fp64_gpu_ms_fused_refresh": round(ms64_fz, 3), "fp64_fused_refresh_bitwise_equal": fz_same,
+        "speedup_mixed_vs_fp64_fused_refresh": round(ms64_fz / msmx, 3),
         "iterations": {"mixed": stm["iterations"], "fp64": st["iterations"]},
         "rel_residual": {"mixed_linf": stm["rel_linf"], "mixed_h2_linf": stm["rel_h2"], "mixed_l2": stm["rel_l2"],
                          "fp64_linf": st["rel_linf"], "fp64_h2_linf": st["rel_h2"], "fp64_l2": st["rel_l2"]},
@@ -643,6 +652,8 @@ def main():
                    "base_blocks": 4096, "levels": 5, "boundary": "sphere multipole", "tol": a.tol},
         "fp64_gpu_ms": amr["fp64_gpu_ms"],
         "speedup_vs_fp64_gpu": amr["speedup_mixed_vs_fp64"],
+        "fp64_gpu_ms_fused_refresh": amr["fp64_gpu_ms_fused_refresh"],
+        "speedup_vs_fp64_gpu_fused_refresh": amr["speedup_mixed_vs_fp64_fused_refresh"],
         "iterations": amr["iterations"],
         "rel_residual": amr["rel_residual"],
         "roofline": amr["roofline"],

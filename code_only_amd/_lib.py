@@ -113,6 +113,7 @@ SIGNATURES = {
     "sg_amr_tile_info": (_I, [_VP, _I, _IP, ctypes.POINTER(ctypes.c_float), _I]),
     "sg_amr_time_smoother": (_I, [_VP, _I, _I, _DP]),
     "sg_amr_set_base_solver": (_I, [_VP, _I]),
+    "sg_amr_set_fp64_refresh": (_I, [_VP, _I]),
     "sg_amr_set_blockmem": (_I, [_VP, ctypes.POINTER(SgBlockmem), ctypes.c_longlong, _IP, _IP, _IP]),
     "sg_amr_solve_blockmem_dev": (_I, [_VP, _VP, ctypes.POINTER(SgTimeslice), _D, ctypes.POINTER(SgParams),
                                        ctypes.POINTER(SgStats)]),

@@ -89,6 +89,11 @@ class AmrSolver:
         workgroup (False); the values are the same."""
         _lib.check(self._lib.sg_amr_set_base_solver(self._ctx, 1 if chip else 0))
 
+    def set_fp64_refresh(self, fused):
+        """Plain FP64 solve with the staged refresh of the reference (False, default) or the mixed
+        solve's fused launch structure (True); the values are the same."""
+        _lib.check(self._lib.sg_amr_set_fp64_refresh(self._ctx, 1 if fused else 0))
+
     def time_smoother(self, precision=SG_PREC_MIXED, reps=20):
         """ms per full red-black sweep (both colours) of the FMG-level-0 relaxation, HIP events on
         the library's stream."""

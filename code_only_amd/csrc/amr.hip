@@ -131,6 +131,9 @@ struct sg_amr {
     // the base-grid mg_cycle spread over the chip (base_mg_chip; 0: the one-workgroup k_base_mg):
     // bm2 is bm with levels 0 and 1 in global memory (they cross launches)
     int base_chip = 1;
+    // the plain FP64 solve with the fused refresh and merged VMG steps (the mixed solve's launch
+    // structure, the same values as the reference's staged sequence); off by default
+    int fp64_fused = 0;
     BaseMG bm2{};
     int fill0_level = -1;
     // hydro block memory (sg_amr_set_blockmem): its layout and the blocks' pool slots on the device
@@ -1022,6 +1025,7 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     if (std::fabs(resh2max) <= 2.2250738585072014e-308) nmax = 0;
     double prev = HUGE_VAL;
     int it = 0, status = SG_ERR_NOT_CONVERGED;
+    s.fused_on = s.fp64_fused;   // reset below, before the epilogue reads the stored ghosts
     auto measure = [&](int slot) -> int {
         SG_TRY(norms(s, C_RHO));
         SG_TRY(read_scalars(s));
@@ -1058,6 +1062,7 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     }
     st.iterations = it;
     st.status = status;
+    s.fused_on = 0;
     SG_TRY(fmg_copy<double>(s, 0, C_U, C_PSI, 1));
     SG_TRY(c2p<double>(s, 0, C_U, 0));
     SG_TRY(go(s, k_blocks_to_dense, (long)s.nblk * 512, psi, (const double *)s.d.A[0][C_U], s.nblk));
@@ -1896,6 +1901,14 @@ int sg_amr_set_base_solver(sg_amr *ctx, int chip)
     SG_REQUIRE(ctx, "sg_amr_set_base_solver: null context");
     SG_REQUIRE(chip == 0 || chip == 1, "sg_amr_set_base_solver: mode must be 0 or 1, got %d", chip);
     ctx->base_chip = chip;
+    return SG_OK;
+}
+
+int sg_amr_set_fp64_refresh(sg_amr *ctx, int fused)
+{
+    SG_REQUIRE(ctx, "sg_amr_set_fp64_refresh: null context");
+    SG_REQUIRE(fused == 0 || fused == 1, "sg_amr_set_fp64_refresh: mode must be 0 or 1, got %d", fused);
+    ctx->fp64_fused = fused;
     return SG_OK;
 }
 
