@@ -2077,13 +2077,20 @@ int sg_amr_set_blockmem(sg_amr *ctx, const sg_blockmem *lay, long long pool_slot
         *dst[q] = nullptr;
     }
     s.have_bm = 0;
+    std::vector<int> all;   // every slot named, to check that no two arrays share one
     for (int q = 0; q < 3; ++q) {
         if (!tabs[q]) continue;
         for (int b = 0; b < s.nblk; ++b)
             SG_REQUIRE(tabs[q][b] >= 0 && tabs[q][b] < pool_slots, "sg_amr_set_blockmem: slot table %d, block %d: %d",
                        q, b, tabs[q][b]);
-        SG_TRY(upload(s, dst[q], std::vector<int>(tabs[q], tabs[q] + s.nblk)));
+        all.insert(all.end(), tabs[q], tabs[q] + s.nblk);
     }
+    std::sort(all.begin(), all.end());
+    SG_REQUIRE(std::adjacent_find(all.begin(), all.end()) == all.end(),
+               "sg_amr_set_blockmem: a pool slot is named twice (each grid's current, predictor and past arrays "
+               "need slots of their own)");
+    for (int q = 0; q < 3; ++q)
+        if (tabs[q]) SG_TRY(upload(s, dst[q], std::vector<int>(tabs[q], tabs[q] + s.nblk)));
     s.bml = L;
     s.have_bm = 1;
     return SG_OK;

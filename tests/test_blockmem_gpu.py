@@ -149,6 +149,12 @@ def test_blockmem_argument_checks(gpu):
         bad[3] = pool.shape[0]
         with pytest.raises(SgError):
             g.set_blockmem(lay, pool.shape[0], bad)
+        dup = su.copy()
+        dup[3] = dup[4]                                  # two grids in one slot
+        with pytest.raises(SgError):
+            g.set_blockmem(lay, pool.shape[0], dup)
+        with pytest.raises(SgError):
+            g.set_blockmem(lay, pool.shape[0], su, su)   # predictor arrays on the grids' own
         g.set_blockmem(lay, pool.shape[0], su)
         with pytest.raises(SgError):
             g.solve_blockmem(dpool, PI4G, source=1)   # no predictor slots
@@ -157,3 +163,34 @@ def test_blockmem_argument_checks(gpu):
         assert np.array_equal(dpool.cpu().numpy(), pool)
     finally:
         g.close()
+
+
+def test_blockmem_full_sphere_hierarchy(gpu):
+    """The C2 hierarchy (20 480 grids): the FP64 solve staged from the pool writes back exactly what
+    the dense entry point returns for the same staged input."""
+    import torch
+    h = ah.nested(base=16, levels=5, bc=ah.SPHERE)
+    lay = LAY["hd"]
+    # one slot per grid (the current state only: time slice 0 reads nothing else), shuffled
+    rng = np.random.default_rng(71)
+    pool = rng.standard_normal((h.nblk, lay["stride"]))
+    su = rng.permutation(h.nblk).astype(np.int32)
+    c0 = pool.reshape(h.nblk, 9, 12, 12, 12)
+    c0[(su, lay["mrho"]) + (slice(2, 10),) * 3] = problems.sis_source(h, seed=1)
+    L = oracle.blockmem(**lay)
+    psi0, src0 = oracle.bm_stage(pool, L, su, h.level, oracle.timeslice(0), PI4G)
+    g = AmrSolver(h)
+    try:
+        pd, gd, sd = g.solve_host(src0, psi=psi0, tol=1e-10, maxit=30)
+        g.set_blockmem(lay, pool.shape[0], su)
+        dpool = torch.from_numpy(pool).to(gpu)
+        sb = g.solve_blockmem(dpool, PI4G, tol=1e-10, maxit=30)
+        got = dpool.cpu().numpy()
+    finally:
+        g.close()
+    assert sb["iterations"] == sd["iterations"] and sb["rel_linf"] == sd["rel_linf"]
+    c = got[:, :9 * 12 ** 3].reshape(-1, 9, 12, 12, 12)
+    ins = (slice(2, 10),) * 3
+    assert np.array_equal(c[(su, lay["mpsi"]) + ins], pd)
+    for a, m in enumerate(("mgx", "mgy", "mgz")):
+        assert np.array_equal(c[(su, lay[m]) + ins], gd[:, a])
