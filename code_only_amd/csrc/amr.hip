@@ -96,6 +96,16 @@ struct sg_amr {
     // the inner solve's FMG-level-0 source in the fused mode, compact and colour-split
     // (amr_kernels.h cc_index), 512 values per active block
     float *rhs0c = nullptr;
+    // the inner solve's FMG-level-0 correction U in the fused mode, compact (512 values per active
+    // block, in active-list order), its residual (the same) and its ghost store (6 x 64 per active
+    // block: same-level faces written by the neighbours, coarse-fine ones by the refresh, physical
+    // ones zero)
+    float *u0c = nullptr;
+    float *res0c = nullptr;
+    float *gs0 = nullptr;
+    // its tables (build_compact_tables), present when cc_ok
+    int cc_ok = 0;
+    int *d_aidx = nullptr, *d_ptab = nullptr, *d_cov_cc = nullptr, *d_src_cc = nullptr;
     int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
@@ -307,6 +317,16 @@ template <typename T> inline T *face_store(sg_amr &s, int L)
     return nullptr;
 }
 
+// The inner solve's FMG-level-0 correction U is held in compact form (u0c with its ghost store gs0;
+// amr_kernels.h cc_index, gs_cell) where the fused form applies at FMG level 0 and the compact
+// tables could be built: every kernel that reads or writes it in fmg_lin then has a compact
+// variant, and the FP64 update adds its interiors.
+template <typename T> inline bool compact0(const sg_amr &s, int L, int ju)
+{
+    if constexpr (std::is_same<T, float>::value) return L == 0 && ju == C_U && fused(s, 0) && s.cc_ok && s.u0c;
+    return false;
+}
+
 template <int N, typename T> int prep_n(sg_amr &s, int L, int ju)
 {
     constexpr int H = N / 2;
@@ -315,6 +335,11 @@ template <int N, typename T> int prep_n(sg_amr &s, int L, int ju)
     LevelScalars hil{};
     for (int l = 0; l < kMaxLev; ++l) hil.v[l] = s.hil[L][l];
     T *fb = ju == C_U ? face_store<T>(s, L) : nullptr;
+    if constexpr (N == 8 && std::is_same<T, float>::value) {
+        if (compact0<T>(s, L, ju))
+            return go(s, k_prep<8, float, true>, work, s.u0c, s.f.ov[0], (const int2 *)s.d_cov_cc, ncov,
+                      (const int4 *)s.d_cfrec, (const int *)s.d_src_cc, ncf, hil, s.gs0, (const int *)s.d_aidx);
+    }
     return go(s, k_prep<N, T>, work, ar<T>(s).A[L][ju], ar<T>(s).ov[L], (const int2 *)s.d_prep_cov[L], ncov,
               (const int4 *)s.d_cfrec, (const int *)s.d_prep_src[L], ncf, hil, fb, (const int *)s.d_covf);
 }
@@ -357,22 +382,40 @@ constexpr int kBlk8Nt[3] = {64, 128, 256};
 // one launch of k_blk8 over the active blocks with nt threads per block
 // (fb: the face store of Au, read for the neighbours' faces and, by the relaxation, written back)
 // (cast, npart: the residual's FP32 copy and norm partials, k_blk8)
+// (gs: the compact level-0 form, FP32 only: Au and Aout compact, gs the ghost store, written
+// through the context's push table)
 template <bool RELAX, typename T>
 int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo,
-         T *fb = nullptr, float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr)
+         T *fb = nullptr, float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr,
+         T *gs = nullptr)
 {
     const dim3 g((unsigned)s.active.size());
     const int *rec = s.d_arec;
     T *fbout = RELAX ? fb : nullptr;
+    if constexpr (std::is_same<T, float>::value) {
+        if (gs) {
+            SG_REQUIRE(rhsc && s.d_ptab, "compact FMG-level-0 pass without its compact source or push table");
+            auto launch = [&](auto kern, int n) {
+                hipLaunchKernelGGL(kern, g, dim3(n), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0], ov,
+                                   resmax, 1, (const T *)nullptr, (T *)nullptr, cast, npart, rhsc, gs,
+                                   (const int *)s.d_ptab);
+            };
+            if (nt == 64) launch(k_blk8<RELAX, float, 64, true>, 64);
+            else if (nt == 128) launch(k_blk8<RELAX, float, 128, true>, 128);
+            else launch(k_blk8<RELAX, float, 256, true>, 256);
+            SG_CHECK_LAUNCH();
+            return SG_OK;
+        }
+    }
     if (nt == 64)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 64>), g, dim3(64), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0],
-                           ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc);
+                           ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
     else if (nt == 128)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 128>), g, dim3(128), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
     else
         hipLaunchKernelGGL((k_blk8<RELAX, T, 256>), g, dim3(256), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc);
+                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
@@ -397,11 +440,16 @@ template <typename T> int tune_blk8(sg_amr &s)
     if (s.active.empty()) return SG_OK;
     auto &R = ar<T>(s);
     const int halo = s.fused_ok[0];
-    T *fb = nullptr;
+    T *fb = nullptr, *U = R.A[0][C_U], *RES = R.A[0][C_RES], *gs = nullptr;
     const float *rc = nullptr;   // the solve's FP32 passes read the compact source
     if constexpr (std::is_same<T, float>::value) {
         fb = halo ? s.fb0 : nullptr;
         rc = s.rhs0c;
+        if (halo && s.cc_ok) {   // and, in the fused mode, keep U in compact form
+            U = s.u0c;
+            RES = s.res0c;
+            gs = s.gs0;
+        }
     }
     for (int which = 0; which < 2; ++which) {
         float best = 1e30f;
@@ -410,11 +458,11 @@ template <typename T> int tune_blk8(sg_amr &s)
             for (int rep = 0; rep < 17; ++rep) {
                 if (rep == 1) SG_HIP(hipEventRecord(s.ev0, s.stream));
                 if (which == 0)
-                    SG_TRY((blk8<true, T>(s, nt, (T *)nullptr, R.A[0][C_U], R.A[0][C_RHS], 1 + (rep & 1), R.ov[0], nullptr, halo,
-                                          fb, nullptr, nullptr, rc)));
+                    SG_TRY((blk8<true, T>(s, nt, (T *)nullptr, U, R.A[0][C_RHS], 1 + (rep & 1), R.ov[0], nullptr, halo,
+                                          fb, nullptr, nullptr, rc, gs)));
                 else
-                    SG_TRY((blk8<false, T>(s, nt, R.A[0][C_RES], R.A[0][C_U], R.A[0][C_RHS], 0, R.ov[0], nullptr, halo,
-                                           fb, nullptr, nullptr, rc)));
+                    SG_TRY((blk8<false, T>(s, nt, RES, U, R.A[0][C_RHS], 0, R.ov[0], nullptr, halo,
+                                           fb, nullptr, nullptr, rc, gs)));
             }
             SG_HIP(hipEventRecord(s.ev1, s.stream));
             SG_HIP(hipEventSynchronize(s.ev1));
@@ -453,7 +501,11 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
             SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
         }
         const bool hf = fused(s, L);
-        if (L == 0) {
+        if (L == 0 && compact0<T>(s, 0, ju)) {
+            SG_TRY((blk8<true, T>(s, s.blk8_nt[kBlk8Slot<T>], (T *)nullptr, (T *)s.u0c, (const T *)nullptr, ipass,
+                                  ar<T>(s).ov[0], slots, 1, (T *)nullptr, nullptr, nullptr, src0c<T>(s, 0, jrhs),
+                                  (T *)s.gs0)));
+        } else if (L == 0) {
             SG_TRY((blk8<true, T>(s, s.blk8_nt[kBlk8Slot<T>], (T *)nullptr, ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], ipass,
                                   ar<T>(s).ov[0], slots, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr, nullptr,
                                   nullptr, src0c<T>(s, 0, jrhs))));
@@ -480,6 +532,13 @@ template <int N, typename T> int resid_n(sg_amr &s, int L, int jres, int ju, int
 template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int jrhs, int fill0flag)
 {
     SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
+    if (L == 0 && compact0<T>(s, 0, ju)) {
+        // the residual in compact form (res0c), which the restriction to FMG level 1 reads
+        SG_REQUIRE(jres == C_RES, "compact FMG-level-0 residual goes to RES");
+        return blk8<false, T>(s, s.blk8_nt[kBlk8Slot<T> + 1], (T *)s.res0c, (T *)s.u0c, (const T *)nullptr, 0,
+                              ar<T>(s).ov[0], nullptr, 1, (T *)nullptr, nullptr, nullptr, src0c<T>(s, 0, jrhs),
+                              (T *)s.gs0);
+    }
     if (L == 0) {
         const bool hf = fused(s, 0);
         return blk8<false, T>(s, s.blk8_nt[kBlk8Slot<T> + 1], ar<T>(s).A[0][jres], ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], 0,
@@ -496,6 +555,8 @@ template <typename T> int fmg_rstrct(sg_amr &s, int Lc, int iuc, int iuf)
     auto &R = ar<T>(s);
     if (Lc == 1 && src0c<T>(s, 0, iuf))
         return go(s, k_rstrct_cc, (long)na * 64, s.f.A[1][iuc], (const float *)s.rhs0c, (const int *)s.d_active, na);
+    if (Lc == 1 && iuf == C_RES && compact0<T>(s, 0, C_U))
+        return go(s, k_rstrct_cc, (long)na * 64, s.f.A[1][iuc], (const float *)s.res0c, (const int *)s.d_active, na);
     if (Lc == 1)
         return go(s, k_rstrct<4, T>, (long)na * 64, R.A[1][iuc], (const T *)R.A[0][iuf], (const int *)s.d_active, na);
     return go(s, k_rstrct<2, T>, (long)na * 8, R.A[2][iuc], (const T *)R.A[1][iuf], (const int *)s.d_active, na);
@@ -528,6 +589,17 @@ template <typename T> int fmg_interp(sg_amr &s, int Lf, int iuf, int iuc, int cu
     SG_TRY(c2p<T>(s, Lc, iuc, cubic));
     SG_TRY(ghostcell_fix<T>(s, Lc, iuc, cubic));
     SG_TRY(extrap_faces<T>(s, Lc, iuc, 0, s.nlev));
+    if (cubic && compact0<T>(s, Lf, iuf)) {
+        // the compact level-0 U: interiors and face store; its physical faces are the ghost
+        // store's zeros, what fmg_boundary_fill0 would leave
+        if (!s.active.empty()) {
+            hipLaunchKernelGGL((k_interp_cubic_blk<8, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream,
+                               ar<T>(s).A[0][iuf], (const T *)ar<T>(s).A[1][iuc], (const int *)s.d_active,
+                               (int)s.active.size(), (T *)nullptr, (T *)s.u0c, (T *)s.gs0, (const int *)s.d_ptab);
+            SG_CHECK_LAUNCH();
+        }
+        return fill0_faces<T>(s, Lc, iuc, 0, s.nlev);
+    }
     SG_TRY(interp_blocks<T>(s, Lf, iuf, iuc, cubic));
     SG_TRY(fill0_faces<T>(s, Lc, iuc, 0, s.nlev));
     return fill0_faces<T>(s, Lf, iuf, 0, s.nlev);
@@ -556,6 +628,22 @@ template <typename T> int fmg_add(sg_amr &s, int L, int a, int da)
 
 template <typename T> int fmg_addint(sg_amr &s, int Lf, int juf, int juc, int jres)
 {
+    if constexpr (std::is_same<T, float>::value) {
+        if (compact0<T>(s, Lf, juf)) {
+            // fmg_interp's steps at FMG level 1, then the interpolation added to the compact U in
+            // one pass (the staged sequence's level-0 RES is not formed)
+            const int Lc = Lf + 1;
+            SG_TRY(c2p<T>(s, Lc, juc, 0));
+            SG_TRY(ghostcell_fix<T>(s, Lc, juc, 0));
+            SG_TRY(extrap_faces<T>(s, Lc, juc, 0, s.nlev));
+            if (!s.active.empty()) {
+                hipLaunchKernelGGL(k_addint_cc, dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, s.u0c,
+                                   (const float *)s.f.A[1][juc], (const int *)s.d_active, s.gs0, (const int *)s.d_ptab);
+                SG_CHECK_LAUNCH();
+            }
+            return fill0_faces<T>(s, Lc, juc, 0, s.nlev);
+        }
+    }
     SG_TRY(fmg_interp<T>(s, Lf, jres, juc, 0));
     T *fb = juf == C_U ? face_store<T>(s, Lf) : nullptr;
     if (fb && !s.active.empty()) {
@@ -1123,6 +1211,13 @@ int ensure_mixed(sg_amr &s)
     SG_HIP(hipMemset(s.fb0, 0, sizeof(float) * 6 * 64 * (size_t)s.nblk));
     SG_HIP(hipMalloc((void **)&s.rhs0c, sizeof(float) * 512 * std::max<size_t>(1, s.active.size())));
     SG_HIP(hipMemset(s.rhs0c, 0, sizeof(float) * 512 * std::max<size_t>(1, s.active.size())));
+    const size_t nac = std::max<size_t>(1, s.active.size());
+    SG_HIP(hipMalloc((void **)&s.u0c, sizeof(float) * 512 * nac));
+    SG_HIP(hipMemset(s.u0c, 0, sizeof(float) * 512 * nac));
+    SG_HIP(hipMalloc((void **)&s.res0c, sizeof(float) * 512 * nac));
+    SG_HIP(hipMemset(s.res0c, 0, sizeof(float) * 512 * nac));
+    SG_HIP(hipMalloc((void **)&s.gs0, sizeof(float) * 6 * 64 * nac));
+    SG_HIP(hipMemset(s.gs0, 0, sizeof(float) * 6 * 64 * nac));
     s.have_f = 1;
     return tune_blk8<float>(s);
 }
@@ -1157,8 +1252,11 @@ int mixed_iteration(sg_amr &s)
                   (const int *)nullptr, nb, (int)lenof(0)));
     SG_TRY(fmg_lin<float>(s));
     const int na = (int)s.active.size();
-    SG_TRY(go(s, k_add_cast_blocks, (long)na * lenof(0), s.d.A[0][C_PSI], (const float *)s.f.A[0][C_U],
-              (const int *)s.d_active, na, (int)lenof(0)));
+    if (compact0<float>(s, 0, C_U))
+        SG_TRY(go(s, k_add_cast_cc, (long)na * 512, s.d.A[0][C_PSI], (const float *)s.u0c, (const int *)s.d_active, na));
+    else
+        SG_TRY(go(s, k_add_cast_blocks, (long)na * lenof(0), s.d.A[0][C_PSI], (const float *)s.f.A[0][C_U],
+                  (const int *)s.d_active, na, (int)lenof(0)));
     return outer_resid(s);
 }
 
@@ -1396,6 +1494,81 @@ bool build_prep_tables(sg_amr &s, int L, std::vector<int> &cov, std::vector<int>
     return true;
 }
 
+// Tables of the compact FMG-level-0 form (amr_kernels.h gs_cell): the active-list position of each
+// block (-1 for skipped ones); per active block and face, the ghost-store face its boundary cells
+// go to (6 a' + face of the same-level active neighbour a' across it, else -1); k_prep's tables with
+// every cell named by its compact position: covered items {ghost-store cell of the leaf neighbour,
+// restriction origin a 512 + i + 8 j + 64 k}, sources v >= 0 a compact interior cell, -1 a physical
+// ghost (zero), v <= -2 a restriction with origin -(v + 2).  False where a named cell has no
+// compact position.
+bool build_compact_tables(const sg_amr &s, const std::vector<int> &cov, const std::vector<int> &src,
+                          std::vector<int> &aidx, std::vector<int> &ptab, std::vector<int> &cov_cc,
+                          std::vector<int> &src_cc)
+{
+    const long LEN = 1000;
+    const int na = (int)s.active.size();
+    aidx.assign(s.nblk, -1);
+    for (int a = 0; a < na; ++a) aidx[s.active[a]] = a;
+    ptab.assign((size_t)6 * std::max(1, na), -1);
+    for (int a = 0; a < na; ++a) {
+        const int b = s.active[a];
+        for (int f = 0; f < 6; ++f) {
+            const int m = s.nbr[6 * b + f];
+            if (s.nkind[6 * b + f] == NBR_SAME && aidx[m] >= 0) ptab[6 * a + f] = 6 * aidx[m] + (f ^ 1);
+        }
+    }
+    // a ghosted offset -> (block, i, j, k)
+    auto split = [LEN](long o, int &b, int &i, int &j, int &k) {
+        b = (int)(o / LEN);
+        const int r = (int)(o - (long)b * LEN);
+        i = r % 10 - 1;
+        j = (r / 10) % 10 - 1;
+        k = r / 100 - 1;
+    };
+    auto inside = [](int i, int j, int k) { return i >= 0 && i < 8 && j >= 0 && j < 8 && k >= 0 && k < 8; };
+    auto origin = [&](long o, int &v) {   // restriction origin in compact numbering
+        int b, i, j, k;
+        split(o, b, i, j, k);
+        if (aidx[b] < 0 || !inside(i, j, k) || (i | j | k) & 1) return false;
+        v = aidx[b] * 512 + i + 8 * j + 64 * k;
+        return true;
+    };
+    auto cc = [](int i, int j, int k) { return ((i + j + k + 1) & 1) * 256 + (i >> 1) + 4 * j + 32 * k; };
+    auto gcell = [](int F, int t1, int t2) { return ((((F & 1) ? 0 : 1) + t1 + t2) & 1) * 32 + (t1 >> 1) + 4 * t2; };
+    for (int e : s.cf)
+        if (aidx[e / 6] < 0) return false;   // k_prep names a coarse-fine item's block by its position
+    cov_cc.clear();
+    src_cc.clear();
+    for (size_t e = 0; e < s.covf.size(); ++e) {
+        const int f = s.covf[e] % 6, m = s.nbr[s.covf[e]];
+        if (aidx[m] < 0) return false;
+        for (int q = 0; q < 64; ++q) {
+            const int t1 = q & 7, t2 = q >> 3;
+            int org = 0;
+            if (!origin(cov[2 * (e * 64 + q) + 1], org)) return false;
+            cov_cc.push_back((6 * aidx[m] + (f ^ 1)) * 64 + gcell(f ^ 1, t1, t2));
+            cov_cc.push_back(org);
+        }
+    }
+    for (int v : src) {
+        if (v < 0) {
+            int org = 0;
+            if (!origin(-(long)v - 1, org)) return false;
+            src_cc.push_back(-org - 2);
+            continue;
+        }
+        int b, i, j, k;
+        split(v, b, i, j, k);
+        if (!inside(i, j, k)) {
+            src_cc.push_back(-1);
+            continue;
+        }
+        if (aidx[b] < 0) return false;
+        src_cc.push_back(aidx[b] * 512 + cc(i, j, k));
+    }
+    return true;
+}
+
 template <typename T> int upload(sg_amr &s, T **dptr, const std::vector<T> &v)
 {
     size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
@@ -1410,12 +1583,13 @@ void release(sg_amr *s)
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
                  s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters,
                  s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap,
-                 s->d_slot_u, s->d_slot_u1, s->d_slot_u2};
+                 s->d_slot_u, s->d_slot_u1, s->d_slot_u2, s->d_aidx, s->d_ptab, s->d_cov_cc, s->d_src_cc};
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->d_crop) (void)hipFree(s->d_crop);
     for (int4 *q : {s->d_hx, s->d_fg, s->d_rf}) if (q) (void)hipFree(q);
     if (s->fb0) (void)hipFree(s->fb0);
     if (s->rhs0c) (void)hipFree(s->rhs0c);
+    for (float *q : {s->u0c, s->res0c, s->gs0}) if (q) (void)hipFree(q);
     for (int L = 0; L < kNfmg; ++L) {
         if (s->d_prep_cov[L]) (void)hipFree(s->d_prep_cov[L]);
         if (s->d_prep_src[L]) (void)hipFree(s->d_prep_src[L]);
@@ -1630,6 +1804,14 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
             s->prep_ncov[L] = (int)prep_cov[L].size() / 2;
             SG_TRY(upload(*s, &s->d_prep_cov[L], prep_cov[L]));
             SG_TRY(upload(*s, &s->d_prep_src[L], prep_src[L]));
+        }
+        std::vector<int> aidx, ptab, cov_cc, src_cc;
+        if (s->fused_ok[0] && build_compact_tables(*s, prep_cov[0], prep_src[0], aidx, ptab, cov_cc, src_cc)) {
+            s->cc_ok = 1;
+            SG_TRY(upload(*s, &s->d_aidx, aidx));
+            SG_TRY(upload(*s, &s->d_ptab, ptab));
+            SG_TRY(upload(*s, &s->d_cov_cc, cov_cc));
+            SG_TRY(upload(*s, &s->d_src_cc, src_cc));
         }
     }
     for (int L = 0; L < kNfmg; ++L) {
@@ -1924,12 +2106,15 @@ int sg_amr_time_smoother(sg_amr *ctx, int prec, int reps, double *ms_per_sweep)
     SG_HIP(hipEventRecord(s.ev0, s.stream));
     for (int r = 0; r < reps; ++r)
         for (int ipass = 1; ipass <= 2; ++ipass) {
-            if (prec == SG_PREC_MIXED)
-                SG_TRY((blk8<true, float>(s, s.blk8_nt[0], (float *)nullptr, s.f.A[0][C_U], s.f.A[0][C_RHS], ipass, s.f.ov[0],
-                                          nullptr, halo, halo ? s.fb0 : nullptr, nullptr, nullptr, s.rhs0c)));
-            else
+            if (prec == SG_PREC_MIXED) {   // (the compact form of U, as the fused mode keeps it)
+                const bool cc = halo && s.cc_ok;
+                SG_TRY((blk8<true, float>(s, s.blk8_nt[0], (float *)nullptr, cc ? s.u0c : s.f.A[0][C_U],
+                                          s.f.A[0][C_RHS], ipass, s.f.ov[0], nullptr, halo, halo ? s.fb0 : nullptr, nullptr,
+                                          nullptr, s.rhs0c, cc ? s.gs0 : nullptr)));
+            } else {
                 SG_TRY((blk8<true, double>(s, s.blk8_nt[2], (double *)nullptr, s.d.A[0][C_U], s.d.A[0][C_RHS], ipass,
                                            s.d.ov[0], nullptr, 0)));
+            }
         }
     SG_HIP(hipEventRecord(s.ev1, s.stream));
     SG_HIP(hipEventSynchronize(s.ev1));

@@ -103,6 +103,59 @@ __device__ __forceinline__ float rstr_lin_cc(const float *uc, int i, int j, int 
     return s * (1.0f / 8.0f);
 }
 
+// The inner solve's FMG-level-0 correction U in the same compact form (active-list entry a at
+// 512 a): the interiors only.  The ghost values its colour passes read are held apart in a ghost
+// store (gs_cell): same-level faces written there by the neighbours, coarse-fine faces by the
+// refresh, physical faces left zero, the inner solve's boundary value (fmg_boundary_fill0).
+__device__ __forceinline__ void cc_ijk(int c, int &i, int &j, int &k)   // inverse of cc_index
+{
+    const int r = c & 255;
+    j = (r >> 2) & 7;
+    k = r >> 5;
+    i = 2 * (r & 3) + (((c >> 8) + j + k + 1) & 1);
+}
+
+// cell (a along axis d, (t1, t2) along the other two in x, y, z order) in the compact form
+__device__ __forceinline__ int cc_rot(int d, int a, int t1, int t2)
+{
+    return d == 0 ? cc_index(a, t1, t2) : (d == 1 ? cc_index(t1, a, t2) : cc_index(t1, t2, a));
+}
+
+// restriction.F90:127-151 over the compact interiors, the 2x2x2 cells at origin v = a 512 + i + 8 j
+// + 64 k (block a's cells (i, j, k) .. (i + 1, j + 1, k + 1))
+__device__ __forceinline__ float rstr_cc(const float *U, int v)
+{
+    const int r = v & 511;
+    return rstr_lin_cc(U + (long)(v >> 9) * 512, r & 7, (r >> 3) & 7, r >> 6);
+}
+
+// The ghost store of the compact form: per active block a and face F, the 64 ghost cells at
+// (a 6 + F) 64 in two halves by the cell's parity (i + j + k in the block's frame, the ghost
+// coordinate -1 or 8 included), each half in (t1 >> 1) + 4 t2 order.  A colour pass reads the half
+// its stencils need and its neighbours write the other half (the cells they change), so the pass
+// reads one 128-byte line per face and nothing it reads is written during it.
+__device__ __forceinline__ int gs_cell(int F, int t1, int t2)
+{
+    return ((((F & 1) ? 0 : 1) + t1 + t2) & 1) * 32 + (t1 >> 1) + 4 * t2;
+}
+
+// interior cell (i, j, k) with value v into the ghost stores of the same-level neighbours whose
+// ghost it is (pt: the block's six ptab entries)
+template <typename T> __device__ __forceinline__ void push_faces(T *gs, const int *pt, int i, int j, int k, T v)
+{
+    const int hp = ((i + j + k) & 1) * 32;
+    auto push = [&](int f, int t1, int t2) {
+        const int tg = pt[f];
+        if (tg >= 0) gs[(long)tg * 64 + hp + (t1 >> 1) + 4 * t2] = v;
+    };
+    if (i == 0) push(0, j, k);
+    if (i == 7) push(1, j, k);
+    if (j == 0) push(2, i, k);
+    if (j == 7) push(3, i, k);
+    if (k == 0) push(4, i, j);
+    if (k == 7) push(5, i, j);
+}
+
 // restriction.F90:71-119 (stencil i1-1 .. i1+2 per axis)
 template <int N, typename T> __device__ __forceinline__ T rstr_cub(const T *uf, int i1, int j1, int k1, const Cubic64 &st)
 {
@@ -413,15 +466,23 @@ template <int N, typename T> __device__ __forceinline__ T src_val(const T *A, in
 // override slot, AMR level}; src + e (H+2)^2 holds the sources of the coarse neighbour's face layer
 // at tangential coarse positions -1..H (the fine block's quarter of it, ghosts included).
 // Reads only leaf interiors and physical ghost cells, which no item writes.
-template <int N, typename T>
+// CC: the compact level-0 form of the inner solve (N = 8, T = float; A the compact interiors by
+// active-list position, the tables of build_compact_tables): covered items write their value to the
+// ghost store fbcov at cov[t].x, the fine ghosts go there too (covf: the blocks' active positions).
+template <int N, typename T, bool CC = false>
 __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, const int *src, int ncf,
                        LevelScalars hil, T *fbcov = nullptr, const int *covf = nullptr)
 {
     using B = Blk<N>;
     constexpr int H = N / 2, S = H + 2;
+    static_assert(!CC || (N == 8 && std::is_same<T, float>::value), "the compact form is FP32 8^3");
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ncov) {
         const int2 p = cov[t];
+        if constexpr (CC) {
+            fbcov[p.x] = rstr_cc(A, p.y);
+            return;
+        }
         const T v = rstr_at<N, T>(A, p.y);
         // the items of one covered face are its N^2 layer cells in (t2, t1) order: with a face
         // store, the value goes to the face's slab there instead of to the block's layer
@@ -438,11 +499,19 @@ __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, 
 #pragma unroll
     for (int y = 0; y < 3; ++y)
 #pragma unroll
-        for (int x = 0; x < 3; ++x) w[y][x] = src_val<N, T>(A, sr[(c2 + y) * S + (c1 + x)]);
+        for (int x = 0; x < 3; ++x) {
+            const int so = sr[(c2 + y) * S + (c1 + x)];
+            if constexpr (CC) {
+                w[y][x] = so >= 0 ? A[so] : (so == -1 ? (T)0 : rstr_cc(A, -so - 2));
+            } else {
+                w[y][x] = src_val<N, T>(A, so);
+            }
+        }
     const int4 r = rec[e];
     const int b = r.x, f = r.y & 7, o = r.y >> 3, d = f >> 1, lr = f & 1;
     const int ag = lr ? N : -1, inner = lr ? -1 : 1;
-    T *u = A + (long)b * B::LEN;
+    const int ab = CC ? covf[b] : b;   // the block's position in A
+    T *u = A + (long)ab * (CC ? 512 : B::LEN);
     const T hi = (T)hil.v[r.w];
     T ffv[2][2];
 #pragma unroll
@@ -456,12 +525,13 @@ __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, 
             Sv = Sv + (T)3.0 * w[n2][1];
             Sv = Sv + (T)3.0 * w[1][n1];
             const T uH = Sv / (T)16.0;
-            const T u0 = u[B::rot(d, ag + inner, s1, s2)];
-            const T uN = u[B::rot(d, ag + 2 * inner, s1, s2)];
+            const T u0 = u[CC ? cc_rot(d, ag + inner, s1, s2) : B::rot(d, ag + inner, s1, s2)];
+            const T uN = u[CC ? cc_rot(d, ag + 2 * inner, s1, s2) : B::rot(d, ag + 2 * inner, s1, s2)];
             T q = (T)10.0 * u0 + (T)8.0 * uH;
             q = q - (T)3.0 * uN;
             const T g = q / (T)15.0;
-            u[B::rot(d, ag, s1, s2)] = g;
+            if constexpr (CC) fbcov[((long)ab * 6 + f) * 64 + gs_cell(f, s1, s2)] = g;
+            else u[B::rot(d, ag, s1, s2)] = g;
             // face flux (u(af + 1) - u(af)) hi with af = -1 (left face) or N - 1 (right face)
             ffv[q1][q2] = lr ? (g - u0) * hi : (u0 - g) * hi;
         }
@@ -752,8 +822,12 @@ template <int NF, bool CUBIC, typename T> __global__ void k_interp(T *Af, const 
 // (bufx, bufxy, then z) run over it, so each intermediate is formed once instead of once per fine
 // cell that uses it.  Every intermediate is the value tricubic_at forms, so the results are the
 // same bit for bit.
+// (ucc: the fine blocks' interiors go there in compact form instead, list entry a at 512 a, and
+// their boundary cells into the same-level neighbours' ghost stores gs through ptab)
 template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cubic_blk(T *Af, const T *Ac, const int *list,
-                                                                                     int nb, T *fb = nullptr)
+                                                                                     int nb, T *fb = nullptr,
+                                                                                     T *ucc = nullptr, T *gs = nullptr,
+                                                                                     const int *ptab = nullptr)
 {
     constexpr int NC = NF / 2, W = NC + 2;   // coarse cells -1 .. NC per axis
     static_assert(NC + 2 >= 4, "the tricubic stencil needs four coarse cells");
@@ -794,7 +868,12 @@ template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cu
         const T *p = bxy + j * NF + i;
         const T v = comb4<T>(swt[k], p[sidx[k][0] * NF * NF], p[sidx[k][1] * NF * NF], p[sidx[k][2] * NF * NF],
                              p[sidx[k][3] * NF * NF]);
-        uf[Blk<NF>::gi(i, j, k)] = v;
+        if (NF == 8 && ucc) {
+            ucc[(long)blockIdx.x * 512 + cc_index(i, j, k)] = v;
+            push_faces(gs, ptab + blockIdx.x * 6, i, j, k, v);
+        } else {
+            uf[Blk<NF>::gi(i, j, k)] = v;
+        }
         if (NF == 8 && fb) {
             // the face store: each boundary cell on the one to three faces it lies on
             if (i == 0) fb[fb_at(b, 0, j, k)] = v;
@@ -1213,20 +1292,49 @@ __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
 // colour-split form of cc_index, which rhsc then feeds the FP32 passes from) and its block's
 // norm partials (npart[g], npart[G + g], npart[2G + g] for workgroup g of G: the per-cell
 // terms of k_norms_part, so the two max norms are the same values; the sum adds in another order).
-template <bool RELAX, typename T, int NT>
+//
+// CC: the compact level-0 form of the inner solve (T = float, halo-free, rhsc set), everything by
+// the workgroup's active-list position a: Au holds the interiors in compact form (512 a), the ghost
+// cells come from the ghost store gs (gs_cell), and the residual goes to Aout in compact form.  A
+// colour pass writes the boundary cells it changes into its same-level neighbours' ghost store
+// (gs + 64 ptab[6 a + f]), the half of it that no concurrent pass reads.
+template <bool RELAX, typename T, int NT, bool CC = false>
 __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
                                             int halo, const T *fbin = nullptr, T *fbout = nullptr,
-                                            float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr)
+                                            float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr,
+                                            T *gs = nullptr, const int *ptab = nullptr)
 {
     using B = Blk<8>;
+    static_assert(!CC || std::is_same<T, float>::value, "the compact form is FP32");
     constexpr int NC = RELAX ? 256 / NT : 512 / NT;   // cells per thread
     __shared__ T su[B::LEN];
+    const int wa = (int)blockIdx.x;
     BlockRec R;
-    R.load(rec, blockIdx.x);
-    const T *ug = Au + (long)R.b * B::LEN;
-    for (int q = threadIdx.x; q < B::LEN; q += NT) su[q] = ug[q];
-    // this thread's cells: colour cells c = threadIdx.x + r NT (relax) or cells c (residual)
+    R.load(rec, wa);
+    if constexpr (CC) {
+        // the interiors, then the ghost cells: for a colour pass only the parity its stencils read
+        const T *ug = Au + (long)wa * 512;
+        for (int q = threadIdx.x; q < 512; q += NT) {
+            int i, j, k;
+            cc_ijk(q, i, j, k);
+            su[B::gi(i, j, k)] = ug[q];
+        }
+        const T *gw = gs + (long)wa * 384;
+        constexpr int NG = RELAX ? 192 : 384;
+        for (int q = threadIdx.x; q < NG; q += NT) {
+            const int f = RELAX ? q >> 5 : q >> 6;
+            const int par = RELAX ? (ipass + 1) & 1 : (q >> 5) & 1, x = q & 31;
+            const int d = f >> 1, lr = f & 1, t2 = x >> 2;
+            const int t1 = 2 * (x & 3) + ((par + (lr ? 0 : 1) + t2) & 1);
+            su[B::rot(d, lr ? 8 : -1, t1, t2)] = gw[f * 64 + par * 32 + x];
+        }
+    } else {
+        const T *ug = Au + (long)R.b * B::LEN;
+        for (int q = threadIdx.x; q < B::LEN; q += NT) su[q] = ug[q];
+    }
+    // this thread's cells: colour cells c = threadIdx.x + r NT (relax) or cells c (residual, in
+    // compact order in the compact form)
     int ci[NC], cj[NC], ck[NC];
     T rhs[NC];
 #pragma unroll
@@ -1236,6 +1344,8 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             cj[r] = (c >> 2) & 7;
             ck[r] = c >> 5;
             ci[r] = 2 * (c & 3) + ((cj[r] + ck[r] + ipass) & 1);
+        } else if (CC) {
+            cc_ijk(c, ci[r], cj[r], ck[r]);
         } else {
             ci[r] = c & 7;
             cj[r] = (c >> 3) & 7;
@@ -1243,13 +1353,13 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
         }
         if constexpr (std::is_same<T, float>::value) {
             if (rhsc) {
-                rhs[r] = rhsc[(long)blockIdx.x * 512 + (RELAX ? (ipass - 1) * 256 + c : cc_index(ci[r], cj[r], ck[r]))];
+                rhs[r] = rhsc[(long)wa * 512 + (RELAX ? (ipass - 1) * 256 + c : cc_index(ci[r], cj[r], ck[r]))];
                 continue;
             }
         }
         rhs[r] = Arhs[(long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r])];
     }
-    if (halo) {
+    if (!CC && halo) {
         // halo-free form: the same-level faces' ghost cells taken from the neighbours' boundary
         // layers (what fmg_ghfix_samelev would copy there); only the face cells the 7-point
         // operator reads, not edges or corners.  The neighbour's cells next to this block's
@@ -1273,12 +1383,15 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
     const double hd = h_lev.v[R.l];
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
-        const long q = (long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r]);
+        const long q = CC ? (RELAX ? (long)wa * 512 + (ipass - 1) * 256 + threadIdx.x + r * NT
+                                   : (long)wa * 512 + threadIdx.x + r * NT)
+                          : (long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r]);
         if (RELAX) {
             T res = F.div(ci[r], cj[r], ck[r]) - rhs[r] * h;
             T resh2 = res * h;
             const T v = su[B::gi(ci[r], cj[r], ck[r])] + (T)(1.0 / 6.0) * resh2;
             Au[q] = v;
+            if constexpr (CC) push_faces(gs, ptab + wa * 6, ci[r], cj[r], ck[r], v);
             // the colour's cells are read by no stencil of this pass, so LDS can take the new value
             if (fbout) su[B::gi(ci[r], cj[r], ck[r])] = v;
             a = fmax(a, (double)fabs(resh2));
@@ -1330,6 +1443,39 @@ __global__ void __launch_bounds__(256) k_add_blk8_fb(T *A, const T *D, const int
     }
     __syncthreads();
     fb_write(fb, b, su, 256);
+}
+
+// fmg_addint at FMG level 0 in the compact form: U += the trilinear interpolation of the listed
+// blocks' FMG-level-1 correction (fmg_interp_linear, rounded to FP32 as the staged sequence stores
+// it before fmg_add), one workgroup per block (U by list position), which also writes the new
+// boundary cells into the same-level neighbours' ghost stores.  Only the interiors: the compact
+// form has no ghost cells to add to.
+__global__ void __launch_bounds__(256) k_addint_cc(float *U, const float *Ac, const int *list, float *gs,
+                                                   const int *ptab)
+{
+    const int a = (int)blockIdx.x, b = list[a];
+    const float *uc = Ac + (long)b * Blk<4>::LEN;
+    auto cget = [&](int x, int y, int z) -> float { return uc[Blk<4>::gi(x, y, z)]; };
+    for (int q = threadIdx.x; q < 512; q += 256) {
+        int i, j, k;
+        cc_ijk(q, i, j, k);
+        const float e = trilinear_at(i, j, k, cget);
+        const float v = U[(long)a * 512 + q] + e;
+        U[(long)a * 512 + q] = v;
+        push_faces(gs, ptab + a * 6, i, j, k, v);
+    }
+}
+
+// the FP64 update of the mixed solve from the compact FP32 correction: the listed blocks' interiors
+// (threads in the FP64 array's x-fastest order, so its rows are read and written whole)
+__global__ void k_add_cast_cc(double *A, const float *U, const int *list, int nb)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * 512) return;
+    const int b = list[t >> 9], c = (int)(t & 511);
+    const int i = c & 7, j = (c >> 3) & 7, k = c >> 6;
+    const long q = (long)b * Blk<8>::LEN + Blk<8>::gi(i, j, k);
+    A[q] = A[q] + (double)U[(t >> 9) * 512 + cc_index(i, j, k)];
 }
 
 // ---- single-workgroup helpers (blockDim.x threads, one workgroup) ----
