@@ -65,7 +65,7 @@ struct sg_amr {
     // (entries b*6+f), the child table, and per FMG level whether the hierarchy admits the fused
     // form (build_prep_tables); fused_on is set for the duration of a mixed solve that asks for it
     std::vector<int> covf;
-    int *d_covf = nullptr, *d_child = nullptr;
+    int *d_child = nullptr;
     // k_prep's source tables per fused FMG level (build_prep_tables): covered face-layer cells
     // {destination, restriction origin}, and per coarse-fine face the sources of the coarse
     // neighbour's face layer; d_cfrec holds {fine block, face + 8 octant, override slot, level}
@@ -91,8 +91,6 @@ struct sg_amr {
     // [0] relax FP32, [1] residual FP32, [2] relax FP64, [3] residual FP64; candidate timings in ms
     int blk8_nt[4] = {256, 256, 256, 256};
     float blk8_ms[4][3] = {};
-    // FMG-level-0 face store of U (FP32, fused mode): 6 x 64 values per block (amr_kernels.h)
-    float *fb0 = nullptr;
     // the inner solve's FMG-level-0 source in the fused mode, compact and colour-split
     // (amr_kernels.h cc_index), 512 values per active block
     float *rhs0c = nullptr;
@@ -308,15 +306,6 @@ template <typename T> int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, r
 // values in the outer residual, which refreshes without zeroing).
 inline bool fused(const sg_amr &s, int L) { return s.fused_on && L < kNfmg - 1 && s.fused_ok[L]; }
 
-// the level-0 face store of U applies in the inner (FP32) solve's fused mode
-template <typename T> inline T *face_store(sg_amr &s, int L)
-{
-    if constexpr (std::is_same<T, float>::value) {
-        if (L == 0 && fused(s, 0)) return s.fb0;
-    }
-    return nullptr;
-}
-
 // The inner solve's FMG-level-0 correction U is held in compact form (u0c with its ghost store gs0;
 // amr_kernels.h cc_index, gs_cell) where the fused form applies at FMG level 0 and the compact
 // tables could be built: every kernel that reads or writes it in fmg_lin then has a compact
@@ -334,14 +323,13 @@ template <int N, typename T> int prep_n(sg_amr &s, int L, int ju)
     const long work = (long)ncov + (long)ncf * H * H;
     LevelScalars hil{};
     for (int l = 0; l < kMaxLev; ++l) hil.v[l] = s.hil[L][l];
-    T *fb = ju == C_U ? face_store<T>(s, L) : nullptr;
     if constexpr (N == 8 && std::is_same<T, float>::value) {
         if (compact0<T>(s, L, ju))
             return go(s, k_prep<8, float, true>, work, s.u0c, s.f.ov[0], (const int2 *)s.d_cov_cc, ncov,
                       (const int4 *)s.d_cfrec, (const int *)s.d_src_cc, ncf, hil, s.gs0, (const int *)s.d_aidx);
     }
     return go(s, k_prep<N, T>, work, ar<T>(s).A[L][ju], ar<T>(s).ov[L], (const int2 *)s.d_prep_cov[L], ncov,
-              (const int4 *)s.d_cfrec, (const int *)s.d_prep_src[L], ncf, hil, fb, (const int *)s.d_covf);
+              (const int4 *)s.d_cfrec, (const int *)s.d_prep_src[L], ncf, hil, (T *)nullptr, (const int *)nullptr);
 }
 
 
@@ -380,25 +368,21 @@ int relax_pass(sg_amr &s, int L, int ju, int jrhs, const int *rec, int nrec, int
 constexpr int kBlk8Nt[3] = {64, 128, 256};
 
 // one launch of k_blk8 over the active blocks with nt threads per block
-// (fb: the face store of Au, read for the neighbours' faces and, by the relaxation, written back)
 // (cast, npart: the residual's FP32 copy and norm partials, k_blk8)
 // (gs: the compact level-0 form, FP32 only: Au and Aout compact, gs the ghost store, written
 // through the context's push table)
 template <bool RELAX, typename T>
 int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo,
-         T *fb = nullptr, float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr,
-         T *gs = nullptr)
+         float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr, T *gs = nullptr)
 {
     const dim3 g((unsigned)s.active.size());
     const int *rec = s.d_arec;
-    T *fbout = RELAX ? fb : nullptr;
     if constexpr (std::is_same<T, float>::value) {
         if (gs) {
             SG_REQUIRE(rhsc && s.d_ptab, "compact FMG-level-0 pass without its compact source or push table");
             auto launch = [&](auto kern, int n) {
                 hipLaunchKernelGGL(kern, g, dim3(n), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0], ov,
-                                   resmax, 1, (const T *)nullptr, (T *)nullptr, cast, npart, rhsc, gs,
-                                   (const int *)s.d_ptab);
+                                   resmax, 1, cast, npart, rhsc, gs, (const int *)s.d_ptab);
             };
             if (nt == 64) launch(k_blk8<RELAX, float, 64, true>, 64);
             else if (nt == 128) launch(k_blk8<RELAX, float, 128, true>, 128);
@@ -409,13 +393,13 @@ int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *o
     }
     if (nt == 64)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 64>), g, dim3(64), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0],
-                           ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
+                           ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
     else if (nt == 128)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 128>), g, dim3(128), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
+                           s.his[0], ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
     else
         hipLaunchKernelGGL((k_blk8<RELAX, T, 256>), g, dim3(256), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, (const T *)fb, fbout, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
+                           s.his[0], ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
@@ -440,10 +424,9 @@ template <typename T> int tune_blk8(sg_amr &s)
     if (s.active.empty()) return SG_OK;
     auto &R = ar<T>(s);
     const int halo = s.fused_ok[0];
-    T *fb = nullptr, *U = R.A[0][C_U], *RES = R.A[0][C_RES], *gs = nullptr;
+    T *U = R.A[0][C_U], *RES = R.A[0][C_RES], *gs = nullptr;
     const float *rc = nullptr;   // the solve's FP32 passes read the compact source
     if constexpr (std::is_same<T, float>::value) {
-        fb = halo ? s.fb0 : nullptr;
         rc = s.rhs0c;
         if (halo && s.cc_ok) {   // and, in the fused mode, keep U in compact form
             U = s.u0c;
@@ -459,10 +442,10 @@ template <typename T> int tune_blk8(sg_amr &s)
                 if (rep == 1) SG_HIP(hipEventRecord(s.ev0, s.stream));
                 if (which == 0)
                     SG_TRY((blk8<true, T>(s, nt, (T *)nullptr, U, R.A[0][C_RHS], 1 + (rep & 1), R.ov[0], nullptr, halo,
-                                          fb, nullptr, nullptr, rc, gs)));
+                                          nullptr, nullptr, rc, gs)));
                 else
-                    SG_TRY((blk8<false, T>(s, nt, RES, U, R.A[0][C_RHS], 0, R.ov[0], nullptr, halo,
-                                           fb, nullptr, nullptr, rc, gs)));
+                    SG_TRY((blk8<false, T>(s, nt, RES, U, R.A[0][C_RHS], 0, R.ov[0], nullptr, halo, nullptr, nullptr,
+                                           rc, gs)));
             }
             SG_HIP(hipEventRecord(s.ev1, s.stream));
             SG_HIP(hipEventSynchronize(s.ev1));
@@ -503,12 +486,10 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
         const bool hf = fused(s, L);
         if (L == 0 && compact0<T>(s, 0, ju)) {
             SG_TRY((blk8<true, T>(s, s.blk8_nt[kBlk8Slot<T>], (T *)nullptr, (T *)s.u0c, (const T *)nullptr, ipass,
-                                  ar<T>(s).ov[0], slots, 1, (T *)nullptr, nullptr, nullptr, src0c<T>(s, 0, jrhs),
-                                  (T *)s.gs0)));
+                                  ar<T>(s).ov[0], slots, 1, nullptr, nullptr, src0c<T>(s, 0, jrhs), (T *)s.gs0)));
         } else if (L == 0) {
             SG_TRY((blk8<true, T>(s, s.blk8_nt[kBlk8Slot<T>], (T *)nullptr, ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], ipass,
-                                  ar<T>(s).ov[0], slots, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr, nullptr,
-                                  nullptr, src0c<T>(s, 0, jrhs))));
+                                  ar<T>(s).ov[0], slots, hf ? 1 : 0, nullptr, nullptr, src0c<T>(s, 0, jrhs))));
         } else {
             SG_TRY(relax_pass<T>(s, L, ju, jrhs, s.d_arec, (int)s.active.size(), ipass, true, slots, hf));
         }
@@ -536,14 +517,12 @@ template <typename T> int poisson_resid(sg_amr &s, int L, int jres, int ju, int 
         // the residual in compact form (res0c), which the restriction to FMG level 1 reads
         SG_REQUIRE(jres == C_RES, "compact FMG-level-0 residual goes to RES");
         return blk8<false, T>(s, s.blk8_nt[kBlk8Slot<T> + 1], (T *)s.res0c, (T *)s.u0c, (const T *)nullptr, 0,
-                              ar<T>(s).ov[0], nullptr, 1, (T *)nullptr, nullptr, nullptr, src0c<T>(s, 0, jrhs),
-                              (T *)s.gs0);
+                              ar<T>(s).ov[0], nullptr, 1, nullptr, nullptr, src0c<T>(s, 0, jrhs), (T *)s.gs0);
     }
     if (L == 0) {
         const bool hf = fused(s, 0);
         return blk8<false, T>(s, s.blk8_nt[kBlk8Slot<T> + 1], ar<T>(s).A[0][jres], ar<T>(s).A[0][ju], ar<T>(s).A[0][jrhs], 0,
-                              ar<T>(s).ov[0], nullptr, hf ? 1 : 0, ju == C_U ? face_store<T>(s, 0) : nullptr, nullptr,
-                              nullptr, src0c<T>(s, 0, jrhs));
+                              ar<T>(s).ov[0], nullptr, hf ? 1 : 0, nullptr, nullptr, src0c<T>(s, 0, jrhs));
     }
     SG_NDISPATCH(L, resid_n, s, L, jres, ju, jrhs);
 }
@@ -571,7 +550,7 @@ template <typename T> int interp_blocks(sg_amr &s, int Lf, int iuf, int iuc, int
         // tricubic: one workgroup per block, the separable passes staged in LDS
         if (Lf == 0)
             hipLaunchKernelGGL((k_interp_cubic_blk<8, T>), dim3(na), dim3(256), 0, s.stream, R.A[0][iuf],
-                               (const T *)R.A[1][iuc], al, na, iuf == C_U ? face_store<T>(s, 0) : (T *)nullptr);
+                               (const T *)R.A[1][iuc], al, na);
         else
             hipLaunchKernelGGL((k_interp_cubic_blk<4, T>), dim3(na), dim3(256), 0, s.stream, R.A[1][iuf],
                                (const T *)R.A[2][iuc], al, na);
@@ -590,12 +569,12 @@ template <typename T> int fmg_interp(sg_amr &s, int Lf, int iuf, int iuc, int cu
     SG_TRY(ghostcell_fix<T>(s, Lc, iuc, cubic));
     SG_TRY(extrap_faces<T>(s, Lc, iuc, 0, s.nlev));
     if (cubic && compact0<T>(s, Lf, iuf)) {
-        // the compact level-0 U: interiors and face store; its physical faces are the ghost
-        // store's zeros, what fmg_boundary_fill0 would leave
+        // the compact level-0 U: interiors, and the boundary cells into the neighbours' ghost
+        // stores; its physical faces are the ghost store's zeros, what fmg_boundary_fill0 leaves
         if (!s.active.empty()) {
             hipLaunchKernelGGL((k_interp_cubic_blk<8, T>), dim3((unsigned)s.active.size()), dim3(256), 0, s.stream,
                                ar<T>(s).A[0][iuf], (const T *)ar<T>(s).A[1][iuc], (const int *)s.d_active,
-                               (int)s.active.size(), (T *)nullptr, (T *)s.u0c, (T *)s.gs0, (const int *)s.d_ptab);
+                               (int)s.active.size(), (T *)s.u0c, (T *)s.gs0, (const int *)s.d_ptab);
             SG_CHECK_LAUNCH();
         }
         return fill0_faces<T>(s, Lc, iuc, 0, s.nlev);
@@ -645,13 +624,6 @@ template <typename T> int fmg_addint(sg_amr &s, int Lf, int juf, int juc, int jr
         }
     }
     SG_TRY(fmg_interp<T>(s, Lf, jres, juc, 0));
-    T *fb = juf == C_U ? face_store<T>(s, Lf) : nullptr;
-    if (fb && !s.active.empty()) {
-        hipLaunchKernelGGL(k_add_blk8_fb<T>, dim3((unsigned)s.active.size()), dim3(256), 0, s.stream, ar<T>(s).A[0][juf],
-                           (const T *)ar<T>(s).A[0][jres], (const int *)s.d_active, fb);
-        SG_CHECK_LAUNCH();
-        return SG_OK;
-    }
     return fmg_add<T>(s, Lf, juf, jres);
 }
 
@@ -975,7 +947,7 @@ template <typename T> int fmg_lin(sg_amr &s)
 {
     const int Lmax = kNfmg - 1;
     const bool lean = kInner<T> && s.fused_on;
-    auto rhs = [lean](int v) { return lean && v == 0 ? C_RHO : C_RHS; };
+    auto rhs = [&](int v) { return lean && v == 0 ? C_RHO : C_RHS; };
     for (int L = 1; L <= Lmax; ++L) SG_TRY(fmg_rstrct<T>(s, L, C_RHO, C_RHO));
     SG_TRY(fmg_fill0<T>(s, Lmax, C_U));
     SG_TRY(vmg_fas<T>(s, C_RHO));
@@ -1207,8 +1179,6 @@ int ensure_mixed(sg_amr &s)
         SG_HIP(hipMalloc((void **)&s.bd_inv, sizeof(int) * inv.size()));
         SG_HIP(hipMemcpy(s.bd_inv, inv.data(), sizeof(int) * inv.size(), hipMemcpyHostToDevice));
     }
-    SG_HIP(hipMalloc((void **)&s.fb0, sizeof(float) * 6 * 64 * (size_t)s.nblk));
-    SG_HIP(hipMemset(s.fb0, 0, sizeof(float) * 6 * 64 * (size_t)s.nblk));
     SG_HIP(hipMalloc((void **)&s.rhs0c, sizeof(float) * 512 * std::max<size_t>(1, s.active.size())));
     SG_HIP(hipMemset(s.rhs0c, 0, sizeof(float) * 512 * std::max<size_t>(1, s.active.size())));
     const size_t nac = std::max<size_t>(1, s.active.size());
@@ -1223,9 +1193,9 @@ int ensure_mixed(sg_amr &s)
 }
 
 // The FP64 composite residual of the outer iteration (the reference operator: its refresh and
-// reflux) and its norms -> d_scal[2..4].  In the fused mode the residual kernel also writes the
-// inner solve's FP32 source (the leaf interiors, all the inner solve reads of it) and its blocks'
-// norm partials.
+// reflux) and its norms -> d_scal[2..4].  In the fused mode the residual kernel writes the inner
+// solve's FP32 source (the leaf interiors, all the inner solve reads of it) and its blocks' norm
+// partials in place of the FP64 residual array, which nothing then reads.
 int outer_resid(sg_amr &s)
 {
     if (!s.fused_on || s.active.empty()) {
@@ -1233,8 +1203,9 @@ int outer_resid(sg_amr &s)
         return norms(s, C_RHO);
     }
     SG_TRY(poisson_flux<double>(s, 0, C_PSI, 0));
-    SG_TRY((blk8<false, double>(s, s.blk8_nt[kBlk8Slot<double> + 1], s.d.A[0][C_RHO], s.d.A[0][C_PSI], s.d.A[0][C_SRC], 0,
-                                s.d.ov[0], nullptr, fused(s, 0) ? 1 : 0, nullptr, s.rhs0c, s.d_npart)));
+    // (only the FP32 copy and the norms are read: the FP64 residual array is not written)
+    SG_TRY((blk8<false, double>(s, s.blk8_nt[kBlk8Slot<double> + 1], (double *)nullptr, s.d.A[0][C_PSI],
+                                s.d.A[0][C_SRC], 0, s.d.ov[0], nullptr, fused(s, 0) ? 1 : 0, s.rhs0c, s.d_npart)));
     hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(1024), 0, s.stream, (const double *)s.d_npart, (int)s.active.size(),
                        s.d_scal + 2);
     SG_CHECK_LAUNCH();
@@ -1505,7 +1476,7 @@ bool build_compact_tables(const sg_amr &s, const std::vector<int> &cov, const st
                           std::vector<int> &aidx, std::vector<int> &ptab, std::vector<int> &cov_cc,
                           std::vector<int> &src_cc)
 {
-    const long LEN = 1000;
+    constexpr long LEN = 1000;
     const int na = (int)s.active.size();
     aidx.assign(s.nblk, -1);
     for (int a = 0; a < na; ++a) aidx[s.active[a]] = a;
@@ -1518,7 +1489,7 @@ bool build_compact_tables(const sg_amr &s, const std::vector<int> &cov, const st
         }
     }
     // a ghosted offset -> (block, i, j, k)
-    auto split = [LEN](long o, int &b, int &i, int &j, int &k) {
+    auto split = [](long o, int &b, int &i, int &j, int &k) {
         b = (int)(o / LEN);
         const int r = (int)(o - (long)b * LEN);
         i = r % 10 - 1;
@@ -1582,12 +1553,11 @@ void release(sg_amr *s)
     if (!s) return;
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
                  s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters,
-                 s->d_covf, s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap,
+                 s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap,
                  s->d_slot_u, s->d_slot_u1, s->d_slot_u2, s->d_aidx, s->d_ptab, s->d_cov_cc, s->d_src_cc};
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->d_crop) (void)hipFree(s->d_crop);
     for (int4 *q : {s->d_hx, s->d_fg, s->d_rf}) if (q) (void)hipFree(q);
-    if (s->fb0) (void)hipFree(s->fb0);
     if (s->rhs0c) (void)hipFree(s->rhs0c);
     for (float *q : {s->u0c, s->res0c, s->gs0}) if (q) (void)hipFree(q);
     for (int L = 0; L < kNfmg; ++L) {
@@ -1770,7 +1740,6 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
     SG_TRY(upload(*s, &s->d_cf_slot, s->cf_slot));
     SG_TRY(upload(*s, &s->d_ovslot, s->ovslot));
     SG_TRY(upload(*s, &s->d_cov, s->cov));
-    SG_TRY(upload(*s, &s->d_covf, s->covf));
     SG_TRY(upload(*s, &s->d_child, s->child));
     {
         std::vector<int> brec((size_t)kRec * nblk, 0), arec;
@@ -1805,6 +1774,8 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
             SG_TRY(upload(*s, &s->d_prep_cov[L], prep_cov[L]));
             SG_TRY(upload(*s, &s->d_prep_src[L], prep_src[L]));
         }
+        // the inner solve's fused form at FMG level 0 is the compact one: without its tables the
+        // level keeps the staged refresh (in both precisions)
         std::vector<int> aidx, ptab, cov_cc, src_cc;
         if (s->fused_ok[0] && build_compact_tables(*s, prep_cov[0], prep_src[0], aidx, ptab, cov_cc, src_cc)) {
             s->cc_ok = 1;
@@ -1812,6 +1783,8 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
             SG_TRY(upload(*s, &s->d_ptab, ptab));
             SG_TRY(upload(*s, &s->d_cov_cc, cov_cc));
             SG_TRY(upload(*s, &s->d_src_cc, src_cc));
+        } else {
+            s->fused_ok[0] = 0;
         }
     }
     for (int L = 0; L < kNfmg; ++L) {
@@ -2109,8 +2082,8 @@ int sg_amr_time_smoother(sg_amr *ctx, int prec, int reps, double *ms_per_sweep)
             if (prec == SG_PREC_MIXED) {   // (the compact form of U, as the fused mode keeps it)
                 const bool cc = halo && s.cc_ok;
                 SG_TRY((blk8<true, float>(s, s.blk8_nt[0], (float *)nullptr, cc ? s.u0c : s.f.A[0][C_U],
-                                          s.f.A[0][C_RHS], ipass, s.f.ov[0], nullptr, halo, halo ? s.fb0 : nullptr, nullptr,
-                                          nullptr, s.rhs0c, cc ? s.gs0 : nullptr)));
+                                          s.f.A[0][C_RHS], ipass, s.f.ov[0], nullptr, halo, nullptr, nullptr, s.rhs0c,
+                                          cc ? s.gs0 : nullptr)));
             } else {
                 SG_TRY((blk8<true, double>(s, s.blk8_nt[2], (double *)nullptr, s.d.A[0][C_U], s.d.A[0][C_RHS], ipass,
                                            s.d.ov[0], nullptr, 0)));

@@ -33,8 +33,6 @@ template <int N> struct Blk {
     }
 };
 
-// FMG-level-0 face store (mixed solve, fused mode): face f of block b, tangential (t1, t2) as Blk::rot
-__device__ __forceinline__ long fb_at(int b, int f, int t1, int t2) { return ((long)b * 6 + f) * 64 + t1 + 8 * t2; }
 __device__ __forceinline__ int ijkc(int i) { return i >= 0 ? i / 2 : -((1 - i) / 2); }   // floor(i/2)
 __device__ __forceinline__ int near_(int f, int c) { return c + 2 * (((f % 2) + 2) % 2) - 1; }
 
@@ -468,10 +466,10 @@ template <int N, typename T> __device__ __forceinline__ T src_val(const T *A, in
 // Reads only leaf interiors and physical ghost cells, which no item writes.
 // CC: the compact level-0 form of the inner solve (N = 8, T = float; A the compact interiors by
 // active-list position, the tables of build_compact_tables): covered items write their value to the
-// ghost store fbcov at cov[t].x, the fine ghosts go there too (covf: the blocks' active positions).
+// ghost store gs at cov[t].x, the fine ghosts go there too (aidx: the blocks' active positions).
 template <int N, typename T, bool CC = false>
 __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, const int *src, int ncf,
-                       LevelScalars hil, T *fbcov = nullptr, const int *covf = nullptr)
+                       LevelScalars hil, T *gs = nullptr, const int *aidx = nullptr)
 {
     using B = Blk<N>;
     constexpr int H = N / 2, S = H + 2;
@@ -480,14 +478,10 @@ __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, 
     if (t < ncov) {
         const int2 p = cov[t];
         if constexpr (CC) {
-            fbcov[p.x] = rstr_cc(A, p.y);
+            gs[p.x] = rstr_cc(A, p.y);
             return;
         }
-        const T v = rstr_at<N, T>(A, p.y);
-        // the items of one covered face are its N^2 layer cells in (t2, t1) order: with a face
-        // store, the value goes to the face's slab there instead of to the block's layer
-        if (fbcov) fbcov[(long)covf[t / (N * N)] * (N * N) + t % (N * N)] = v;
-        else A[p.x] = v;
+        A[p.x] = rstr_at<N, T>(A, p.y);
         return;
     }
     t -= ncov;
@@ -510,7 +504,7 @@ __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, 
     const int4 r = rec[e];
     const int b = r.x, f = r.y & 7, o = r.y >> 3, d = f >> 1, lr = f & 1;
     const int ag = lr ? N : -1, inner = lr ? -1 : 1;
-    const int ab = CC ? covf[b] : b;   // the block's position in A
+    const int ab = CC ? aidx[b] : b;   // the block's position in A
     T *u = A + (long)ab * (CC ? 512 : B::LEN);
     const T hi = (T)hil.v[r.w];
     T ffv[2][2];
@@ -530,7 +524,7 @@ __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, 
             T q = (T)10.0 * u0 + (T)8.0 * uH;
             q = q - (T)3.0 * uN;
             const T g = q / (T)15.0;
-            if constexpr (CC) fbcov[((long)ab * 6 + f) * 64 + gs_cell(f, s1, s2)] = g;
+            if constexpr (CC) gs[((long)ab * 6 + f) * 64 + gs_cell(f, s1, s2)] = g;
             else u[B::rot(d, ag, s1, s2)] = g;
             // face flux (u(af + 1) - u(af)) hi with af = -1 (left face) or N - 1 (right face)
             ffv[q1][q2] = lr ? (g - u0) * hi : (u0 - g) * hi;
@@ -825,8 +819,8 @@ template <int NF, bool CUBIC, typename T> __global__ void k_interp(T *Af, const 
 // (ucc: the fine blocks' interiors go there in compact form instead, list entry a at 512 a, and
 // their boundary cells into the same-level neighbours' ghost stores gs through ptab)
 template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cubic_blk(T *Af, const T *Ac, const int *list,
-                                                                                     int nb, T *fb = nullptr,
-                                                                                     T *ucc = nullptr, T *gs = nullptr,
+                                                                                     int nb, T *ucc = nullptr,
+                                                                                     T *gs = nullptr,
                                                                                      const int *ptab = nullptr)
 {
     constexpr int NC = NF / 2, W = NC + 2;   // coarse cells -1 .. NC per axis
@@ -873,15 +867,6 @@ template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cu
             push_faces(gs, ptab + blockIdx.x * 6, i, j, k, v);
         } else {
             uf[Blk<NF>::gi(i, j, k)] = v;
-        }
-        if (NF == 8 && fb) {
-            // the face store: each boundary cell on the one to three faces it lies on
-            if (i == 0) fb[fb_at(b, 0, j, k)] = v;
-            if (i == NF - 1) fb[fb_at(b, 1, j, k)] = v;
-            if (j == 0) fb[fb_at(b, 2, i, k)] = v;
-            if (j == NF - 1) fb[fb_at(b, 3, i, k)] = v;
-            if (k == 0) fb[fb_at(b, 4, i, j)] = v;
-            if (k == NF - 1) fb[fb_at(b, 5, i, j)] = v;
         }
     }
 }
@@ -1262,20 +1247,6 @@ __global__ void k_norms_final(const double *part, int G, double *out)
 //      the only load the data loads wait for; the right-hand side is fetched before the barrier.
 //      Fewer threads per block let a CU hold more blocks at once (the kernel waits on load latency
 //      more than on bandwidth); the AMR autotuner (amr.hip) picks NT per hierarchy. ----
-// ---- FMG-level-0 face store of the mixed solve's fused mode: per block and face, the block's
-//      boundary layer on that face, 64 contiguous values (t1 + 8 t2, tangential axes as Blk::rot),
-//      so that a neighbour's face is two cache lines instead of 64 scattered cells ----
-
-// the six faces of a block staged (ghosted) in LDS, into the store
-template <typename T> __device__ __forceinline__ void fb_write(T *fb, int b, const T *su, int nt)
-{
-    using B = Blk<8>;
-    for (int q = threadIdx.x; q < 6 * 64; q += nt) {
-        const int f = q >> 6, t1 = q & 7, t2 = (q >> 3) & 7;
-        fb[fb_at(b, f, t1, t2)] = su[B::rot(f >> 1, (f & 1) ? 7 : 0, t1, t2)];
-    }
-}
-
 __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
 {
     // select chain: a runtime index into the record's array would put it in scratch memory
@@ -1289,8 +1260,8 @@ __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
 }
 
 // The FP64 residual of the mixed solve can also write its FP32 copy (cast, in the compact
-// colour-split form of cc_index, which rhsc then feeds the FP32 passes from) and its block's
-// norm partials (npart[g], npart[G + g], npart[2G + g] for workgroup g of G: the per-cell
+// colour-split form of cc_index, which rhsc then feeds the FP32 passes from; Aout may then be null,
+// the FP64 residual not being read) and its block's norm partials (npart[g], npart[G + g], npart[2G + g] for workgroup g of G: the per-cell
 // terms of k_norms_part, so the two max norms are the same values; the sum adds in another order).
 //
 // CC: the compact level-0 form of the inner solve (T = float, halo-free, rhsc set), everything by
@@ -1301,9 +1272,8 @@ __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
 template <bool RELAX, typename T, int NT, bool CC = false>
 __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
-                                            int halo, const T *fbin = nullptr, T *fbout = nullptr,
-                                            float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr,
-                                            T *gs = nullptr, const int *ptab = nullptr)
+                                            int halo, float *cast = nullptr, double *npart = nullptr,
+                                            const float *rhsc = nullptr, T *gs = nullptr, const int *ptab = nullptr)
 {
     using B = Blk<8>;
     static_assert(!CC || std::is_same<T, float>::value, "the compact form is FP32");
@@ -1367,13 +1337,11 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
         // The barrier orders these LDS writes after the block copy above, which writes the same
         // ghost cells with their stored values.
         __syncthreads();
-        // (from the face store when there is one: the neighbour's face f^1 faces this block)
         for (int q = threadIdx.x; q < 6 * 64; q += NT) {
             const int f = q >> 6, t1 = q & 7, t2 = (q >> 3) & 7, d = f >> 1, lr = f & 1;
             const int m = rec_nb(R, f);
             if (m < 0) continue;
-            su[B::rot(d, lr ? 8 : -1, t1, t2)] =
-                fbin ? fbin[fb_at(m, f ^ 1, t1, t2)] : Au[(long)m * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
+            su[B::rot(d, lr ? 8 : -1, t1, t2)] = Au[(long)m * B::LEN + B::rot(d, lr ? 0 : 7, t1, t2)];
         }
     }
     __syncthreads();
@@ -1392,12 +1360,10 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             const T v = su[B::gi(ci[r], cj[r], ck[r])] + (T)(1.0 / 6.0) * resh2;
             Au[q] = v;
             if constexpr (CC) push_faces(gs, ptab + wa * 6, ci[r], cj[r], ck[r], v);
-            // the colour's cells are read by no stencil of this pass, so LDS can take the new value
-            if (fbout) su[B::gi(ci[r], cj[r], ck[r])] = v;
             a = fmax(a, (double)fabs(resh2));
         } else {
             const T v = rhs[r] - F.div(ci[r], cj[r], ck[r]) * hi;
-            Aout[q] = v;
+            if (Aout) Aout[q] = v;
             if constexpr (std::is_same<T, double>::value) {
                 if (cast) cast[(long)blockIdx.x * 512 + cc_index(ci[r], cj[r], ck[r])] = (float)v;
                 if (npart) {
@@ -1421,28 +1387,6 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
         }
     }
     if (RELAX && resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
-    if (RELAX && fbout) {
-        __syncthreads();
-        fb_write(fbout, R.b, su, NT);
-    }
-}
-
-// fmg_add (U += D over whole arrays) of the listed 8^3 blocks, one workgroup per block, which also
-// writes the blocks' faces into the face store
-template <typename T>
-__global__ void __launch_bounds__(256) k_add_blk8_fb(T *A, const T *D, const int *list, T *fb)
-{
-    using B = Blk<8>;
-    __shared__ T su[B::LEN];
-    const int b = list[blockIdx.x];
-    const long base = (long)b * B::LEN;
-    for (int q = threadIdx.x; q < B::LEN; q += 256) {
-        const T v = A[base + q] + D[base + q];
-        A[base + q] = v;
-        su[q] = v;
-    }
-    __syncthreads();
-    fb_write(fb, b, su, 256);
 }
 
 // fmg_addint at FMG level 0 in the compact form: U += the trilinear interpolation of the listed
