@@ -114,6 +114,8 @@ SIGNATURES = {
     "sg_amr_time_smoother": (_I, [_VP, _I, _I, _DP]),
     "sg_amr_set_base_solver": (_I, [_VP, _I]),
     "sg_amr_set_fp64_refresh": (_I, [_VP, _I]),
+    "sg_amr_adjoint_dev": (_I, [_VP, _VP, _VP, ctypes.POINTER(SgParams), ctypes.POINTER(SgStats)]),
+    "sg_amr_adjoint_apply": (_I, [_VP, _VP, _VP]),
     "sg_amr_set_blockmem": (_I, [_VP, ctypes.POINTER(SgBlockmem), ctypes.c_longlong, _IP, _IP, _IP]),
     "sg_amr_solve_blockmem_dev": (_I, [_VP, _VP, ctypes.POINTER(SgTimeslice), _D, ctypes.POINTER(SgParams),
                                        ctypes.POINTER(SgStats)]),

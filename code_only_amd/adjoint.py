@@ -9,7 +9,9 @@ through its hand-written adjoint (sg_uniform_adjoint_dev), both behind libsg's C
 The operator L (7-point, periodic) is symmetric, so the adjoint solve is the forward solver applied
 to the adjoint source; P (mean removal, the ZERO_AVERAGE_DENSITY projection of
 fmg_data_if.F90:53-78) is symmetric too.  SampledGradient's backward is the transpose kernel
-sg_uniform_gradient_sample_adjoint_dev.
+sg_uniform_gradient_sample_adjoint_dev.  AmrPoissonSolve does the same for the sphere-boundary AMR
+solve, whose composite operator is not symmetric: its backward solves the transposed problem
+(sg_amr_adjoint_dev).
 """
 import torch
 
@@ -48,3 +50,20 @@ def sampled_gradient_loss(rho, solver, cells, w, tol=1e-10, precision=SG_PREC_MI
     """L = sum_p w_p . grad_h phi(c_p) with phi the periodic potential of rho."""
     phi = PoissonSolve.apply(rho, solver, tol, precision)
     return (SampledGradient.apply(phi, solver, cells) * w).sum()
+
+
+class AmrPoissonSolve(torch.autograd.Function):
+    """psi = AmrPoissonSolve.apply(src, solver): the sphere-boundary AMR solve (AmrSolver.solve from a zero
+    guess, dense [nblk, 8, 8, 8] float64 GPU tensors); backward through sg_amr_adjoint_dev."""
+
+    @staticmethod
+    def forward(ctx, src, solver, tol=1e-10, precision=SG_PREC_MIXED):
+        psi = torch.zeros_like(src)
+        solver.solve(psi, src.contiguous(), tol=tol, maxit=60, stop_mode=SG_STOP_TRUE_RESIDUAL, precision=precision)
+        ctx.solver, ctx.tol, ctx.precision = solver, tol, precision
+        return psi
+
+    @staticmethod
+    def backward(ctx, gpsi):
+        grad, _ = ctx.solver.adjoint(gpsi.contiguous(), tol=ctx.tol, precision=ctx.precision)
+        return grad, None, None, None

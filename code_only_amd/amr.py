@@ -122,6 +122,35 @@ class AmrSolver:
                    allow=() if raise_on_fail else (SG_ERR_NOT_CONVERGED, SG_ERR_NONFINITE))
         return st.as_dict()
 
+    def adjoint(self, psi_bar, tol=1e-10, maxit=200, precision=SG_PREC_MIXED, use_graph=True, fused=True,
+                raise_on_fail=True):
+        """src_bar = (d psi / d src)^T psi_bar for the sphere-boundary solve (sg_amr_adjoint_dev), torch
+        float64 GPU tensors [nblk, 8, 8, 8]; precision is that of the forward solves preconditioning the
+        adjoint's FGMRES, tol its relative residual.  Returns (src_bar, stats)."""
+        import torch
+        if not (psi_bar.is_cuda and psi_bar.dtype == torch.float64 and psi_bar.is_contiguous()):
+            raise ValueError("expected a contiguous float64 GPU tensor")
+        if tuple(psi_bar.shape) != (self.h.nblk, 8, 8, 8):
+            raise ValueError("psi_bar must be [nblk, 8, 8, 8]")
+        out = torch.empty_like(psi_bar)
+        p = self._params(tol, maxit, SG_STOP_TRUE_RESIDUAL, precision, use_graph, fused)
+        st = _lib.SgStats()
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_adjoint_dev(self._ctx, ctypes.c_void_p(out.data_ptr()),
+                                                ctypes.c_void_p(psi_bar.data_ptr()), ctypes.byref(p), ctypes.byref(st)),
+                   allow=() if raise_on_fail else (SG_ERR_NOT_CONVERGED,))
+        return out, st.as_dict()
+
+    def adjoint_apply(self, x):
+        """y = A^T x for the homogeneous composite operator A at FMG level 0 (sg_amr_adjoint_apply)."""
+        import torch
+        if not (x.is_cuda and x.dtype == torch.float64 and x.is_contiguous()) or tuple(x.shape) != (self.h.nblk, 8, 8, 8):
+            raise ValueError("expected a contiguous float64 GPU tensor [nblk, 8, 8, 8]")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        _lib.check(self._lib.sg_amr_adjoint_apply(self._ctx, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr())))
+        return y
+
     def solve_host(self, src, psi=None, want_g=True, tol=1e-10, maxit=30, stop_mode=SG_STOP_TRUE_RESIDUAL,
                    precision=SG_PREC_FP64, use_graph=True, raise_on_fail=True, fused=True):
         src = np.ascontiguousarray(src, np.float64)

@@ -10,13 +10,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <type_traits>
 #include <vector>
 
 #include "amr_kernels.h"
+#include "amr_adjoint.h"
 #include "base_direct.h"
 #include "base_mg.h"
 #include "blockmem_kernels.h"
@@ -104,6 +107,19 @@ struct sg_amr {
     // its tables (build_compact_tables), present when cc_ok
     int cc_ok = 0;
     int *d_aidx = nullptr, *d_ptab = nullptr, *d_cov_cc = nullptr, *d_src_cc = nullptr;
+    // solves with zero physical ghosts instead of the multipole condition (the adjoint's
+    // preconditioner: the composite operator's homogeneous part)
+    int homog_bc = 0;
+    // the adjoint (build_adjoint): A^T of the homogeneous composite operator and B^T of its physical
+    // ghost coupling as CSR over the dense [block][8^3] index space and the ghost items, cell
+    // volumes (zero on covered blocks), work vectors
+    int adj_ready = 0;
+    long adj_ng = 0, adj_nnz = 0;
+    long *d_at_ptr = nullptr, *d_bt_ptr = nullptr;
+    int *d_at_col = nullptr, *d_bt_col = nullptr;
+    double *d_at_val = nullptr, *d_bt_val = nullptr, *d_vol = nullptr, *d_ivol = nullptr, *d_idiag = nullptr;
+    double *d_adj = nullptr, *d_adj_part = nullptr, *d_adj_g = nullptr;
+    int adj_m = 0;
     int fused_on = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
@@ -980,7 +996,8 @@ template <typename T> int fmg_lin(sg_amr &s)
 Geo geo(sg_amr &s) { return Geo{s.d_ijk, s.d_level, s.d_h[0], s.x0[0], s.x0[1], s.x0[2]}; }
 
 // fmg_boundary_physical_grid of the sphere module (fmg_boundary_phys_sphere.F90:80-354)
-int sphere_bc(sg_amr &s, int ju, int jrho)
+// squared radius of the sphere inside the domain whose outside the multipole condition masks
+double sphere_rad2(const sg_amr &s)
 {
     const int n = 8;
     const int NX = s.base_dims[0] * n, NY = s.base_dims[1] * n, NZ = s.base_dims[2] * n;
@@ -994,7 +1011,12 @@ int sphere_bc(sg_amr &s, int ju, int jrho)
     if (-xmin < m) m = -xmin;
     if (-ymin < m) m = -ymin;
     if (-zmin < m) m = -zmin;
-    const double rad2 = m * m;
+    return m * m;
+}
+
+int sphere_bc(sg_amr &s, int ju, int jrho)
+{
+    const double rad2 = sphere_rad2(s);
     Geo G = geo(s);
     SG_TRY(go(s, k_sphere_mask<8>, (long)s.nblk * lenof(0), s.d.A[0][jrho], s.nblk, G, rad2));
     const double dv = (s.h0 * s.h0) * s.h0;
@@ -1075,7 +1097,7 @@ int solve(sg_amr &s, double *psi, const double *src, double *g, const sg_params 
     SG_TRY(fmg_copy<double>(s, 0, C_SRC, C_RHO, 0));
     SG_TRY(c2p<double>(s, 0, C_SRC, 0));
     SG_TRY(c2p<double>(s, 0, C_PSI, 0));
-    if (s.bc == 1) SG_TRY(sphere_bc(s, C_PSI, C_SRC));
+    if (s.bc == 1 && !s.homog_bc) SG_TRY(sphere_bc(s, C_PSI, C_SRC));
     SG_TRY(norms(s, C_SRC));
     SG_TRY(read_scalars(s));
     const double resh2max = s.h_scal[2], srcmax = s.h_scal[3], srcsq = s.h_scal[4];
@@ -1247,7 +1269,7 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
     SG_TRY(fmg_copy<double>(s, 0, C_SRC, C_RHO, 0));
     SG_TRY(c2p<double>(s, 0, C_SRC, 0));
     SG_TRY(c2p<double>(s, 0, C_PSI, 0));
-    if (s.bc == 1) SG_TRY(sphere_bc(s, C_PSI, C_SRC));
+    if (s.bc == 1 && !s.homog_bc) SG_TRY(sphere_bc(s, C_PSI, C_SRC));
     SG_TRY(norms(s, C_SRC));
     SG_TRY(read_scalars(s));
     const double resh2max = s.h_scal[2], srcmax = s.h_scal[3], srcsq = s.h_scal[4];
@@ -1560,6 +1582,10 @@ void release(sg_amr *s)
     for (int4 *q : {s->d_hx, s->d_fg, s->d_rf}) if (q) (void)hipFree(q);
     if (s->rhs0c) (void)hipFree(s->rhs0c);
     for (float *q : {s->u0c, s->res0c, s->gs0}) if (q) (void)hipFree(q);
+    for (long *q : {s->d_at_ptr, s->d_bt_ptr}) if (q) (void)hipFree(q);
+    for (int *q : {s->d_at_col, s->d_bt_col}) if (q) (void)hipFree(q);
+    for (double *q : {s->d_at_val, s->d_bt_val, s->d_vol, s->d_ivol, s->d_idiag, s->d_adj, s->d_adj_part, s->d_adj_g})
+        if (q) (void)hipFree(q);
     for (int L = 0; L < kNfmg; ++L) {
         if (s->d_prep_cov[L]) (void)hipFree(s->d_prep_cov[L]);
         if (s->d_prep_src[L]) (void)hipFree(s->d_prep_src[L]);
@@ -1938,6 +1964,433 @@ int op(sg_amr &s, int opc, int L, int a, int b, int c, int flag, double *ret)
     return SG_OK;
 }
 
+// ---- the adjoint of the solve (sphere boundary) ----
+// The converged solve maps the source to the potential linearly:
+//   SRC = K C src (C: fmg_converge_c2p of the source, K: the multipole condition's mask),
+//   A psi_leaf + B g = SRC_leaf with g = Mp SRC (the multipole ghost values of fmg_boundary_phys_sphere),
+//   psi = C psi_leaf (the closing fmg_converge_c2p),
+// with A the homogeneous composite operator on the leaf cells (same-level faces, the covered
+// neighbours' restriction, the coarse-fine ghost of fmg_ghfix_parentlev and the reflux average of
+// fmg_reflux) and B its coupling to the physical ghosts.  Its transpose:
+//   src_bar = C^T K (lambda - Mp^T B^T lambda),  A^T lambda = (C^T psi_bar)_leaf.
+// build_adjoint assembles A^T and B^T on the host from the hierarchy tables (the ghost sources are
+// those of the fused refresh, build_prep_tables); A^T lambda = r is solved by FGMRES, right-
+// preconditioned with V A^-1 V^-1 (V: cell volumes; exact where V A is symmetric, i.e. away from
+// coarse-fine faces) applied through the forward solve with zero physical ghosts, with damped
+// Jacobi sweeps on A^T around it (adj_precond).  With one refinement level this converges in about
+// ten iterations; where refined regions nest (three or more AMR levels) the transposed interface
+// terms of one level feed the next and the iteration stalls well above tight tolerances, which the
+// status reports (SG_ERR_NOT_CONVERGED).
+
+int build_adjoint(sg_amr &s)
+{
+    if (s.adj_ready) return SG_OK;
+    std::vector<int> cov, tab;
+    SG_REQUIRE(build_prep_tables(s, 0, cov, tab), "sg_amr_adjoint: the hierarchy's coarse-fine ghosts have no single-pass sources");
+    const long n = (long)s.nblk * 512;
+    std::vector<int> cfidx((size_t)6 * s.nblk, -1), physidx((size_t)6 * s.nblk, -1);
+    for (size_t e = 0; e < s.cf.size(); ++e) cfidx[s.cf[e]] = (int)e;
+    for (size_t e = 0; e < s.phys.size(); ++e) physidx[s.phys[e]] = (int)e;
+    // reflux override slot and coarse face position -> {coarse-fine item, its coarse cell (c1, c2)}
+    std::vector<int> ovmap((size_t)std::max(1, s.nslots) * 64, -1);
+    for (size_t e = 0; e < s.cf.size(); ++e) {
+        const int fb = s.cf[e] / 6, d = (s.cf[e] % 6) / 2, o = s.octant[fb];
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        const int off1 = ((o >> ta) & 1) * 4, off2 = ((o >> tb) & 1) * 4;
+        for (int c2 = 0; c2 < 4; ++c2)
+            for (int c1 = 0; c1 < 4; ++c1) ovmap[(size_t)s.cf_slot[e] * 64 + (off1 + c1) + 8 * (off2 + c2)] = (int)e * 16 + c1 + 4 * c2;
+    }
+    std::vector<std::pair<int, double>> ent;    // one row's leaf entries
+    std::vector<std::pair<long, double>> gent;  // one row's ghost-item entries
+    bool ok = true;
+    // a cell of block b: the leaf cell, or a covered cell's restriction of its children
+    std::function<void(int, int, int, int, double)> addcell = [&](int b, int i, int j, int k, double c) {
+        if (!s.covered[b]) {
+            ent.push_back({b * 512 + i + 8 * j + 64 * k, c});
+            return;
+        }
+        const int o = (i >= 4 ? 1 : 0) + (j >= 4 ? 2 : 0) + (k >= 4 ? 4 : 0), ch = s.child[8 * b + o];
+        if (ch < 0) { ok = false; return; }
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) addcell(ch, 2 * (i & 3) + dx, 2 * (j & 3) + dy, 2 * (k & 3) + dz, c * 0.125);
+    };
+    // a stored physical ghost cell of block b -> its item in the physical-face list (k_sphere_ghost)
+    auto ghost_item = [&](int b, const int p[3]) -> long {
+        for (int D = 0; D < 3; ++D) {
+            if (p[D] != -1 && p[D] != 8) continue;
+            const int e = physidx[6 * b + 2 * D + (p[D] == 8 ? 1 : 0)];
+            if (e < 0) continue;
+            const int t1 = D == 0 ? p[1] : p[0], t2 = D == 2 ? p[1] : p[2];
+            return (long)e * 100 + (t1 + 1) + 10 * (t2 + 1);
+        }
+        return -1;
+    };
+    // a source of the fused refresh (build_prep_tables' convention) times c
+    auto addsrc = [&](int sv, double c) {
+        const long o = sv < 0 ? -(long)sv - 1 : sv;
+        const int b = (int)(o / 1000), r = (int)(o % 1000);
+        int p[3] = {r % 10 - 1, (r / 10) % 10 - 1, r / 100 - 1};
+        const bool in = p[0] >= 0 && p[0] < 8 && p[1] >= 0 && p[1] < 8 && p[2] >= 0 && p[2] < 8;
+        if (sv < 0) {
+            for (int dz = 0; dz < 2; ++dz)
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) addcell(b, p[0] + dx, p[1] + dy, p[2] + dz, c * 0.125);
+        } else if (in) {
+            addcell(b, p[0], p[1], p[2], c);
+        } else {
+            const long t = ghost_item(b, p);
+            if (t < 0) ok = false;
+            else gent.push_back({t, c});
+        }
+    };
+    // c times the fine face flux ffv of coarse-fine item e at fine tangential (2 c1 + q1, 2 c2 + q2):
+    // (+-)(g - u0) hi_f with g = (10 u0 + 8 uH - 3 uN) / 15 and uH the (9,3,3,1)/16 bilinear value
+    auto add_cf_flux = [&](int e, int c1, int c2, int q1, int q2, double c) {
+        const int fb = s.cf[e] / 6, f = s.cf[e] % 6, d = f / 2, lr = f & 1;
+        const double hif = 1.0 / std::ldexp(s.h0, -s.level[fb]);
+        const double sg = lr ? c * hif : -c * hif;
+        const int s1 = 2 * c1 + q1, s2 = 2 * c2 + q2;
+        auto at = [&](int a, double w) {
+            const int i = d == 0 ? a : s1, j = d == 1 ? a : (d == 0 ? s1 : s2), k = d == 2 ? a : s2;
+            addcell(fb, i, j, k, w);
+        };
+        at(lr ? 7 : 0, sg * (-5.0 / 15.0));
+        at(lr ? 6 : 1, sg * (-3.0 / 15.0));
+        const double cH = sg * (8.0 / 15.0) / 16.0;
+        const int *sr = tab.data() + (size_t)e * 36;
+        const int n1 = 2 * q1, n2 = 2 * q2;
+        addsrc(sr[(c2 + 1) * 6 + (c1 + 1)], 9.0 * cH);
+        addsrc(sr[(c2 + n2) * 6 + (c1 + n1)], cH);
+        addsrc(sr[(c2 + n2) * 6 + (c1 + 1)], 3.0 * cH);
+        addsrc(sr[(c2 + 1) * 6 + (c1 + n1)], 3.0 * cH);
+    };
+    // rows of A (leaf cells) as COO, merged per row; B as COO over ghost items
+    std::vector<int> ar, ac, br;
+    std::vector<long> bc;
+    std::vector<double> av, bv;
+    std::vector<double> vol(n, 0.0), ivol(n, 0.0), idiag(n, 0.0);
+    for (int b : s.active) {
+        const double h = std::ldexp(s.h0, -s.level[b]), hi = 1.0 / h, hi2 = hi * hi;
+        for (int k = 0; k < 8; ++k)
+            for (int j = 0; j < 8; ++j)
+                for (int i = 0; i < 8; ++i) {
+                    const int row = b * 512 + i + 8 * j + 64 * k;
+                    vol[row] = h * h * h;
+                    ivol[row] = 1.0 / vol[row];
+                    ent.clear();
+                    gent.clear();
+                    const int p[3] = {i, j, k};
+                    for (int d = 0; d < 3; ++d)
+                        for (int lr = 0; lr < 2; ++lr) {
+                            int q[3] = {i, j, k};
+                            q[d] += lr ? 1 : -1;
+                            if (q[d] >= 0 && q[d] < 8) {
+                                addcell(b, q[0], q[1], q[2], hi2);
+                                addcell(b, i, j, k, -hi2);
+                                continue;
+                            }
+                            const int f = 2 * d + lr, t1 = d == 0 ? j : i, t2 = d == 2 ? j : k;
+                            const double sgn = lr ? 1.0 : -1.0;
+                            if (s.ovslot[6 * b + f] >= 0) {   // fmg_reflux: the 2x2 average of the fine fluxes
+                                const int m = ovmap[(size_t)s.ovslot[6 * b + f] * 64 + t1 + 8 * t2];
+                                if (m < 0) { ok = false; continue; }
+                                for (int q2 = 0; q2 < 2; ++q2)
+                                    for (int q1 = 0; q1 < 2; ++q1) add_cf_flux(m / 16, m % 4, (m / 4) % 4, q1, q2, sgn * hi * 0.25);
+                                continue;
+                            }
+                            const int kind = s.nkind[6 * b + f];
+                            if (kind == NBR_SAME) {
+                                q[d] = lr ? 0 : 7;
+                                addcell(s.nbr[6 * b + f], q[0], q[1], q[2], hi2);
+                                addcell(b, i, j, k, -hi2);
+                            } else if (kind == NBR_PARENT) {
+                                const int e = cfidx[6 * b + f];
+                                if (e < 0) { ok = false; continue; }
+                                add_cf_flux(e, t1 >> 1, t2 >> 1, t1 & 1, t2 & 1, sgn * hi);
+                            } else {
+                                q[d] = lr ? 8 : -1;
+                                const long t = ghost_item(b, q);
+                                if (t < 0) { ok = false; continue; }
+                                gent.push_back({t, hi2});
+                                addcell(b, i, j, k, -hi2);
+                            }
+                        }
+                    (void)p;
+                    std::sort(ent.begin(), ent.end());
+                    for (size_t q = 0; q < ent.size();) {
+                        size_t r = q;
+                        double v = 0.0;
+                        while (r < ent.size() && ent[r].first == ent[q].first) v += ent[r++].second;
+                        ar.push_back(row);
+                        ac.push_back(ent[q].first);
+                        av.push_back(v);
+                        if (ent[q].first == row && v != 0.0) idiag[row] = 1.0 / v;
+                        q = r;
+                    }
+                    for (auto &g : gent) {
+                        br.push_back(row);
+                        bc.push_back(g.first);
+                        bv.push_back(g.second);
+                    }
+                }
+    }
+    SG_REQUIRE(ok, "sg_amr_adjoint: a cell of the composite operator has no source");
+    // A^T and B^T in CSR by counting sort on the column
+    const long ng = (long)s.phys.size() * 100;
+    std::vector<long> atp(n + 1, 0), btp(ng + 1, 0);
+    for (int c : ac) ++atp[c + 1];
+    for (long c : bc) ++btp[c + 1];
+    for (long q = 0; q < n; ++q) atp[q + 1] += atp[q];
+    for (long q = 0; q < ng; ++q) btp[q + 1] += btp[q];
+    std::vector<int> atc(ac.size()), btc(bc.size());
+    std::vector<double> atv(ac.size()), btv(bc.size());
+    {
+        std::vector<long> fill(atp.begin(), atp.end() - 1);
+        for (size_t q = 0; q < ac.size(); ++q) {
+            const long w = fill[ac[q]]++;
+            atc[w] = ar[q];
+            atv[w] = av[q];
+        }
+        std::vector<long> fillb(btp.begin(), btp.end() - 1);
+        for (size_t q = 0; q < bc.size(); ++q) {
+            const long w = fillb[bc[q]]++;
+            btc[w] = br[q];
+            btv[w] = bv[q];
+        }
+    }
+    SG_TRY(upload(s, &s.d_at_ptr, atp));
+    SG_TRY(upload(s, &s.d_at_col, atc));
+    SG_TRY(upload(s, &s.d_at_val, atv));
+    SG_TRY(upload(s, &s.d_bt_ptr, btp));
+    SG_TRY(upload(s, &s.d_bt_col, btc));
+    SG_TRY(upload(s, &s.d_bt_val, btv));
+    SG_TRY(upload(s, &s.d_vol, vol));
+    SG_TRY(upload(s, &s.d_ivol, ivol));
+    SG_TRY(upload(s, &s.d_idiag, idiag));
+    s.adj_ng = ng;
+    s.adj_nnz = (long)atc.size();
+    s.adj_ready = 1;
+    return SG_OK;
+}
+
+constexpr int kAdjM = 20;          // FGMRES restart length
+constexpr int kDotGroups = 1024;
+
+double *adj_vec(sg_amr &s, int q) { return s.d_adj + (size_t)q * s.nblk * 512; }
+
+int adj_alloc(sg_amr &s)
+{
+    if (s.d_adj) return SG_OK;
+    // basis V_0..V_m, preconditioned Z_0..Z_{m-1}, w, the forward solve's psi and src, the adjoint's
+    // right-hand side and solution, the preconditioner's A^T z and residual
+    const size_t nv = 2 * kAdjM + 1 + 7;
+    SG_HIP(hipMalloc((void **)&s.d_adj, sizeof(double) * nv * (size_t)s.nblk * 512));
+    SG_HIP(hipMalloc((void **)&s.d_adj_part, sizeof(double) * (kDotGroups + 1)));
+    // the ghost items' adjoint, the multipole partials (16 per workgroup of 256 items) and cbar
+    const long ng = std::max(1L, s.adj_ng);
+    SG_HIP(hipMalloc((void **)&s.d_adj_g, sizeof(double) * (ng + 16 * ((ng + 255) / 256) + 16)));
+    s.adj_m = kAdjM;
+    return SG_OK;
+}
+
+int adj_dot(sg_amr &s, const double *x, const double *y, double &out)
+{
+    const long n = (long)s.nblk * 512;
+    const int G = (int)std::min<long>(kDotGroups, (n + 255) / 256);
+    hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(256), 0, s.stream, x, y, n, s.d_adj_part);
+    SG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s.stream, (const double *)s.d_adj_part, G, s.d_adj_part + kDotGroups);
+    SG_CHECK_LAUNCH();
+    SG_HIP(hipMemcpyAsync(&out, s.d_adj_part + kDotGroups, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    SG_HIP(hipStreamSynchronize(s.stream));
+    return SG_OK;
+}
+
+int at_apply(sg_amr &s, double *y, const double *x)
+{
+    return go(s, k_csr_spmv, (long)s.nblk * 512, y, x, (const long *)s.d_at_ptr, (const int *)s.d_at_col,
+              (const double *)s.d_at_val, (long)s.nblk * 512);
+}
+
+// The preconditioner of A^T: z ~ A^-T v in two parts.  V A^-1 V^-1 (V: cell volumes) is A^-T
+// wherever V A is symmetric, i.e. away from coarse-fine faces; it is applied through the forward
+// solve with zero physical ghosts.  The transposed ghost and reflux terms make A^T act quite
+// differently from A on oscillating modes along coarse-fine faces, where that correction is nearly
+// blind, so damped Jacobi sweeps on A^T itself come before and after it (a two-level scheme: local
+// smoothing, global correction).  pp: the forward solve's precision, tolerance and iteration cap.
+constexpr int kAdjSweeps = 2;
+constexpr double kAdjOmega = 0.6;
+
+int adj_global(sg_amr &s, double *z, const double *v, const sg_params &pp)   // z = V A^-1 V^-1 v
+{
+    const long n = (long)s.nblk * 512;
+    double *psi = adj_vec(s, 2 * kAdjM + 2), *src = adj_vec(s, 2 * kAdjM + 3);
+    SG_TRY(go(s, k_scale_by, n, src, v, (const double *)s.d_ivol, 0, n));
+    SG_HIP(hipMemsetAsync(psi, 0, sizeof(double) * n, s.stream));
+    sg_stats st;
+    s.homog_bc = 1;
+    const int rc = pp.precision == SG_PREC_MIXED ? solve_mixed(s, psi, src, nullptr, pp, st) : solve(s, psi, src, nullptr, pp, st);
+    s.homog_bc = 0;
+    if (rc != SG_OK && rc != SG_ERR_NOT_CONVERGED) return rc;
+    return go(s, k_scale_by, n, z, (const double *)psi, (const double *)s.d_vol, 0, n);
+}
+
+int adj_precond(sg_amr &s, double *z, const double *v, const sg_params &pp)
+{
+    const long n = (long)s.nblk * 512;
+    double *atz = adj_vec(s, 2 * kAdjM + 6), *res = adj_vec(s, 2 * kAdjM + 7);
+    auto sweep = [&]() -> int {
+        SG_TRY(at_apply(s, atz, z));
+        return go(s, k_jacobi, n, z, v, (const double *)atz, (const double *)s.d_idiag, kAdjOmega, n);
+    };
+    SG_HIP(hipMemsetAsync(z, 0, sizeof(double) * n, s.stream));
+    for (int k = 0; k < kAdjSweeps; ++k) SG_TRY(sweep());
+    SG_TRY(at_apply(s, atz, z));
+    SG_TRY(go(s, k_axpby, n, atz, v, 1.0, -1.0, n));   // atz = v - A^T z
+    SG_TRY(adj_global(s, res, atz, pp));
+    SG_TRY(go(s, k_axpby, n, z, (const double *)res, 1.0, 1.0, n));
+    for (int k = 0; k < kAdjSweeps; ++k) SG_TRY(sweep());
+    return SG_OK;
+}
+
+// FGMRES(m) for A^T lam = r (lam starts from zero); tol on ||r - A^T lam|| / ||r||
+int adj_fgmres(sg_amr &s, double *lam, const double *r, const sg_params &pp, double tol, int maxit, int &iters,
+               double &relres)
+{
+    const long n = (long)s.nblk * 512;
+    const int m = s.adj_m;
+    double *w = adj_vec(s, 2 * m + 1);
+    auto V = [&](int q) { return adj_vec(s, q); };
+    auto Z = [&](int q) { return adj_vec(s, m + 1 + q); };
+    double rn = 0.0;
+    SG_TRY(adj_dot(s, r, r, rn));
+    rn = std::sqrt(rn);
+    SG_HIP(hipMemsetAsync(lam, 0, sizeof(double) * n, s.stream));
+    iters = 0;
+    relres = 0.0;
+    if (rn == 0.0) return SG_OK;
+    SG_HIP(hipMemcpyAsync(V(0), r, sizeof(double) * n, hipMemcpyDeviceToDevice, s.stream));   // residual of lam = 0
+    double beta = rn;
+    while (iters < maxit) {
+        SG_TRY(go(s, k_axpby, n, V(0), (const double *)V(0), 1.0 / beta, 0.0, n));
+        std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m && iters < maxit; ++j, ++iters) {
+            SG_TRY(adj_precond(s, Z(j), V(j), pp));
+            SG_TRY(at_apply(s, w, Z(j)));
+            for (int i = 0; i <= j; ++i) {   // modified Gram-Schmidt
+                double hij = 0.0;
+                SG_TRY(adj_dot(s, w, V(i), hij));
+                H[(size_t)i * m + j] = hij;
+                SG_TRY(go(s, k_axpby, n, w, (const double *)V(i), -hij, 1.0, n));
+            }
+            double hn = 0.0;
+            SG_TRY(adj_dot(s, w, w, hn));
+            hn = std::sqrt(hn);
+            H[(size_t)(j + 1) * m + j] = hn;
+            if (hn > 0.0) SG_TRY(go(s, k_axpby, n, V(j + 1), (const double *)w, 1.0 / hn, 0.0, n));
+            for (int i = 0; i < j; ++i) {   // earlier Givens rotations on column j
+                const double a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+                H[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
+                H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+            }
+            const double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+            const double den = std::sqrt(a * a + b * b);
+            cs[j] = den > 0.0 ? a / den : 1.0;
+            sn[j] = den > 0.0 ? b / den : 0.0;
+            H[(size_t)j * m + j] = den;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            if (std::fabs(g[j + 1]) <= tol * rn || hn == 0.0) {
+                ++j;
+                ++iters;
+                break;
+            }
+        }
+        // y = H^-1 g (upper triangular), lam += Z y
+        std::vector<double> y(j, 0.0);
+        for (int i = j - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int q = i + 1; q < j; ++q) t -= H[(size_t)i * m + q] * y[q];
+            y[i] = t / H[(size_t)i * m + i];
+        }
+        for (int i = 0; i < j; ++i) SG_TRY(go(s, k_axpby, n, lam, (const double *)Z(i), y[i], 1.0, n));
+        // true residual, the next cycle's start
+        SG_TRY(at_apply(s, w, lam));
+        SG_TRY(go(s, k_axpby, n, w, r, 1.0, -1.0, n));   // w = r - A^T lam
+        double res = 0.0;
+        SG_TRY(adj_dot(s, w, w, res));
+        beta = std::sqrt(res);
+        relres = beta / rn;
+        if (!std::isfinite(relres)) return SG_ERR_NONFINITE;
+        if (relres <= tol) return SG_OK;
+        SG_HIP(hipMemcpyAsync(V(0), w, sizeof(double) * n, hipMemcpyDeviceToDevice, s.stream));
+    }
+    return SG_ERR_NOT_CONVERGED;
+}
+
+// src_bar = (d psi / d src)^T psi_bar (dense [nblk][8^3] device arrays); pp: the adjoint's
+// tolerance and iteration cap, and the precision of the forward solves that precondition it
+int adjoint(sg_amr &s, double *src_bar, const double *psi_bar, const sg_params &pp, sg_stats &st)
+{
+    std::memset(&st, 0, sizeof(st));
+    SG_REQUIRE(s.bc == SG_BC_SPHERE, "sg_amr_adjoint: the adjoint covers the sphere boundary");
+    SG_REQUIRE(!s.zero_average, "sg_amr_adjoint: zero averaging is not part of the sphere solve");
+    SG_TRY(build_adjoint(s));
+    if (pp.precision == SG_PREC_MIXED) SG_TRY(ensure_mixed(s));
+    SG_TRY(adj_alloc(s));
+    const long n = (long)s.nblk * 512;
+    double *a = adj_vec(s, 2 * kAdjM + 4), *lam = adj_vec(s, 2 * kAdjM + 5);
+    const auto t0 = std::chrono::steady_clock::now();   // (the forward solves inside use the context's events)
+    // C^T psi_bar: the closing c2p's transpose, coarse levels first
+    SG_HIP(hipMemcpyAsync(a, psi_bar, sizeof(double) * n, hipMemcpyDeviceToDevice, s.stream));
+    for (int l = 0; l + 1 < s.nlev; ++l) {
+        const int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+        SG_TRY(go(s, k_c2p_adjoint, (long)nb * 512, a, b0, nb, (const int *)s.d_covered, (const int *)s.d_child));
+    }
+    // the inner solve's precision and stopping for the preconditioner
+    sg_params inner = pp;
+    inner.tol = 1e-6;
+    inner.maxit = 10;
+    inner.stop_mode = SG_STOP_TRUE_RESIDUAL;
+    int iters = 0;
+    double relres = 0.0;
+    const int rc = adj_fgmres(s, lam, a, inner, pp.tol, pp.maxit, iters, relres);
+    st.iterations = iters;
+    st.rel_l2 = relres;
+    st.status = rc;
+    if (rc != SG_OK && rc != SG_ERR_NOT_CONVERGED) return rc;
+    // src_bar = lam - Mp^T B^T lam on the leaves, the mask, then C^T of the source's c2p
+    double *gbar = s.d_adj_g, *part = s.d_adj_g + std::max(1L, s.adj_ng);
+    SG_HIP(hipMemcpyAsync(src_bar, lam, sizeof(double) * n, hipMemcpyDeviceToDevice, s.stream));
+    if (s.adj_ng > 0) {
+        SG_TRY(go(s, k_csr_spmv, s.adj_ng, gbar, (const double *)lam, (const long *)s.d_bt_ptr, (const int *)s.d_bt_col,
+                  (const double *)s.d_bt_val, s.adj_ng));
+        SG_TRY(go(s, k_axpby, s.adj_ng, gbar, (const double *)gbar, -1.0, 0.0, s.adj_ng));
+        const int G = (int)((s.adj_ng + 255) / 256);
+        Geo Gm = geo(s);
+        const int *faces = s.d_phys;
+        hipLaunchKernelGGL(k_sphere_adj_part, dim3(G), dim3(256), 0, s.stream, (const double *)gbar, s.adj_ng, faces, Gm, part);
+        SG_CHECK_LAUNCH();
+        double *cbar = part + (size_t)16 * G;
+        hipLaunchKernelGGL(k_sphere_adj_coef, dim3(1), dim3(256), 0, s.stream, (const double *)part, G, cbar);
+        SG_CHECK_LAUNCH();
+        const double dv = (s.h0 * s.h0) * s.h0;
+        SG_TRY(go(s, k_sphere_adj_src, (long)s.nbase * 512, src_bar, s.nbase, Gm, dv, (const double *)cbar));
+    }
+    SG_TRY(go(s, k_sphere_mask_dense, n, src_bar, s.nblk, geo(s), sphere_rad2(s)));
+    for (int l = 0; l + 1 < s.nlev; ++l) {
+        const int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
+        SG_TRY(go(s, k_c2p_adjoint, (long)nb * 512, src_bar, b0, nb, (const int *)s.d_covered, (const int *)s.d_child));
+    }
+    SG_HIP(hipStreamSynchronize(s.stream));
+    st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
 } // namespace
 
 extern "C" {
@@ -2064,6 +2517,26 @@ int sg_amr_set_fp64_refresh(sg_amr *ctx, int fused)
     SG_REQUIRE(ctx, "sg_amr_set_fp64_refresh: null context");
     SG_REQUIRE(fused == 0 || fused == 1, "sg_amr_set_fp64_refresh: mode must be 0 or 1, got %d", fused);
     ctx->fp64_fused = fused;
+    return SG_OK;
+}
+
+int sg_amr_adjoint_dev(sg_amr *ctx, double *src_bar, const double *psi_bar, const sg_params *p, sg_stats *st)
+{
+    SG_REQUIRE(ctx && src_bar && psi_bar && p && st, "sg_amr_adjoint_dev: null argument");
+    SG_REQUIRE(p->precision == SG_PREC_FP64 || p->precision == SG_PREC_MIXED, "sg_amr_adjoint_dev: unknown precision %d",
+               p->precision);
+    SG_REQUIRE(p->maxit > 0 && p->maxit <= 10000, "sg_amr_adjoint_dev: bad maxit %d", p->maxit);
+    SG_REQUIRE(p->tol > 0.0, "sg_amr_adjoint_dev: bad tol");
+    return adjoint(*ctx, src_bar, psi_bar, *p, *st);
+}
+
+int sg_amr_adjoint_apply(sg_amr *ctx, double *y, const double *x)
+{
+    SG_REQUIRE(ctx && y && x, "sg_amr_adjoint_apply: null argument");
+    SG_REQUIRE(ctx->bc == SG_BC_SPHERE, "sg_amr_adjoint_apply: the adjoint covers the sphere boundary");
+    SG_TRY(build_adjoint(*ctx));
+    SG_TRY(at_apply(*ctx, y, x));
+    SG_HIP(hipStreamSynchronize(ctx->stream));
     return SG_OK;
 }
 

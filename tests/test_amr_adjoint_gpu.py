@@ -1,0 +1,158 @@
+"""The adjoint of the sphere-boundary AMR solve (sg_amr_adjoint_dev, AmrPoissonSolve).
+
+* The assembled transpose A^T of the homogeneous composite operator against the forward operator
+  itself: <A x, y> = <x, A^T y> for random x, y, with A x formed by the library's fmg_poisson_resid
+  (zero source, zero physical ghosts), which the operator tests hold bit for bit equal to the oracle.
+  A wrong coefficient, sign or index of a transposed ghost, reflux or restriction term fails it.
+* The whole map: the converged solve is linear in the source (the multipole ghosts included), so
+  <S r, p> = <r, S^T p> to the solves' tolerance (forward 1e-11, adjoint 1e-12).
+* Central finite differences of a nonlinear loss through AmrPoissonSolve (step 1e-4 relative).
+* The adjoint preconditioned by mixed-precision forward solves equals the FP64-preconditioned one
+  (the FP64 reference adjoint solve) to its tolerance.
+The transpose itself is checked on three- and four-level hierarchies; the transposed solve reaches
+tight tolerances with one refinement level, and on nested refinement reports that it did not."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from code_only_amd import amr_hierarchy as ah
+from code_only_amd import problems
+from code_only_amd._lib import SG_PREC_FP64, SG_PREC_MIXED, SgError
+from code_only_amd.adjoint import AmrPoissonSolve
+from code_only_amd.amr import AmrSolver
+
+pytestmark = pytest.mark.gpu
+OPS = oracle
+
+HIERS = {"nested-sphere-4-3": lambda: ah.nested(base=4, levels=3, bc=ah.SPHERE),
+         "nested-sphere-8-4": lambda: ah.nested(base=8, levels=4, bc=ah.SPHERE)}
+# one refinement level: where the transposed solve converges
+SOLVE_HIERS = {"nested-sphere-4-2": lambda: ah.nested(base=4, levels=2, bc=ah.SPHERE),
+               "nested-sphere-8-2": lambda: ah.nested(base=8, levels=2, bc=ah.SPHERE)}
+
+
+def _leaf(h):
+    return np.asarray(h.leaf, bool)
+
+
+@pytest.mark.parametrize("which", sorted(HIERS))
+def test_transpose_against_forward_operator(which, gpu):
+    h = HIERS[which]()
+    g = AmrSolver(h)
+    try:
+        leaf = _leaf(h)
+        rng = np.random.default_rng(5)
+        for trial in range(3):
+            x = rng.standard_normal((h.nblk, 8, 8, 8))
+            y = rng.standard_normal((h.nblk, 8, 8, 8)) * leaf[:, None, None, None]
+            u = np.zeros((h.nblk, 10, 10, 10))
+            u[:, 1:-1, 1:-1, 1:-1] = x
+            g.set(0, OPS.U, u)
+            g.set(0, OPS.RHS, np.zeros_like(u))
+            g.op(OPS.OP_RESID, 0, OPS.RES, OPS.U, OPS.RHS, 1)
+            ax = -g.get(0, OPS.RES)[:, 1:-1, 1:-1, 1:-1] * leaf[:, None, None, None]
+            aty = g.adjoint_apply(torch.tensor(y, device=gpu)).cpu().numpy()
+            assert np.all(aty[~leaf] == 0.0)
+            lhs = float(np.sum(ax * y))
+            rhs = float(np.sum(x * leaf[:, None, None, None] * aty))
+            scale = np.sqrt(np.sum(ax * ax) * np.sum(y * y))
+            print(which, trial, lhs, rhs, abs(lhs - rhs) / scale, flush=True)
+            assert abs(lhs - rhs) <= 1e-13 * scale
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("which", sorted(SOLVE_HIERS))
+def test_solve_transpose_identity(which, gpu):
+    """<S r, p> = <r, S^T p>: S the converged FP64 solve (multipole ghosts from r included)."""
+    h = SOLVE_HIERS[which]()
+    g = AmrSolver(h)
+    try:
+        rng = np.random.default_rng(11)
+        r = torch.tensor(problems.amr_source(h, seed=3) + 0.1 * rng.standard_normal((h.nblk, 8, 8, 8)), device=gpu)
+        pbar = torch.tensor(rng.standard_normal((h.nblk, 8, 8, 8)), device=gpu)
+        psi = torch.zeros_like(r)
+        st = g.solve(psi, r, tol=1e-11, maxit=60)
+        sbar, sa = g.adjoint(pbar, tol=1e-12, maxit=400, precision=SG_PREC_FP64)
+        lhs = float(torch.sum(psi * pbar))
+        rhs = float(torch.sum(r * sbar))
+        print(which, st["iterations"], st["rel_linf"], sa, lhs, rhs, flush=True)
+        assert sa["status"] == 0 and sa["rel_l2"] <= 1e-12
+        assert abs(lhs - rhs) <= 1e-8 * abs(lhs)
+        # covered blocks' source is overwritten by the restriction of their children: no gradient
+        assert torch.all(sbar[torch.tensor(~_leaf(h), device=gpu)] == 0.0)
+    finally:
+        g.close()
+
+
+def test_finite_differences_through_autograd(gpu):
+    h = SOLVE_HIERS["nested-sphere-4-2"]()
+    g = AmrSolver(h)
+    try:
+        rng = np.random.default_rng(2)
+        rho0 = torch.tensor(problems.sis_source(h, seed=4), device=gpu)
+        w = torch.tensor(rng.standard_normal((h.nblk, 8, 8, 8)), device=gpu)
+        leafm = torch.tensor(_leaf(h)[:, None, None, None] * np.ones((1, 8, 8, 8)), device=gpu)
+
+        def loss(rho):
+            psi = AmrPoissonSolve.apply(rho, g, 1e-11, SG_PREC_FP64)
+            return torch.sum(w * psi * psi * leafm)
+
+        rho = rho0.clone().requires_grad_(True)
+        L = loss(rho)
+        L.backward()
+        grad = rho.grad.detach()
+        for trial in range(3):
+            d = torch.tensor(rng.standard_normal((h.nblk, 8, 8, 8)), device=gpu) * leafm
+            eps = 1e-4 * float(rho0.abs().max()) / float(d.abs().max())
+            with torch.no_grad():
+                fd = (loss(rho0 + eps * d) - loss(rho0 - eps * d)) / (2 * eps)
+            an = float(torch.sum(grad * d))
+            print(trial, float(fd), an, flush=True)
+            assert abs(float(fd) - an) <= 1e-5 * abs(an)
+    finally:
+        g.close()
+
+
+def test_mixed_preconditioning_equals_fp64(gpu):
+    h = SOLVE_HIERS["nested-sphere-8-2"]()
+    g = AmrSolver(h)
+    try:
+        rng = np.random.default_rng(8)
+        pbar = torch.tensor(rng.standard_normal((h.nblk, 8, 8, 8)), device=gpu)
+        s64, st64 = g.adjoint(pbar, tol=1e-12, precision=SG_PREC_FP64)
+        smx, stmx = g.adjoint(pbar, tol=1e-12, precision=SG_PREC_MIXED)
+        rel = float((smx - s64).norm() / s64.norm())
+        print(st64, stmx, rel, flush=True)
+        assert st64["status"] == 0 and stmx["status"] == 0
+        assert rel <= 1e-10
+    finally:
+        g.close()
+
+
+def test_rejects_periodic_hierarchy(gpu):
+    h = ah.nested(base=4, levels=3, bc=ah.PERIODIC)
+    g = AmrSolver(h)
+    try:
+        with pytest.raises(SgError):
+            g.adjoint(torch.zeros((h.nblk, 8, 8, 8), dtype=torch.float64, device=gpu))
+    finally:
+        g.close()
+
+
+def test_nested_refinement_reports_not_converged(gpu):
+    """Three AMR levels: the transposed solve stops at maxit above the tolerance and says so."""
+    from code_only_amd._lib import SG_ERR_NOT_CONVERGED
+    h = HIERS["nested-sphere-4-3"]()
+    g = AmrSolver(h)
+    try:
+        pbar = torch.tensor(np.random.default_rng(1).standard_normal((h.nblk, 8, 8, 8)), device=gpu)
+        sbar, st = g.adjoint(pbar, tol=1e-10, maxit=20, precision=SG_PREC_FP64, raise_on_fail=False)
+        print(st, flush=True)
+        assert st["status"] == SG_ERR_NOT_CONVERGED and st["iterations"] == 20
+        assert np.isfinite(st["rel_l2"]) and torch.isfinite(sbar).all()
+        with pytest.raises(SgError):
+            g.adjoint(pbar, tol=1e-10, maxit=20, precision=SG_PREC_FP64)
+    finally:
+        g.close()
