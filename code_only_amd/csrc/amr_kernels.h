@@ -137,6 +137,41 @@ __device__ __forceinline__ int gs_cell(int F, int t1, int t2)
     return ((((F & 1) ? 0 : 1) + t1 + t2) & 1) * 32 + (t1 >> 1) + 4 * t2;
 }
 
+// NC contiguous floats from / to p (16-, 8- or 4-byte accesses; p aligned to the access)
+template <int NC> __device__ __forceinline__ void ld_run(float (&d)[NC], const float *p)
+{
+    if constexpr (NC % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < NC / 4; ++q) {
+            const float4 v = ((const float4 *)p)[q];
+            d[4 * q] = v.x;
+            d[4 * q + 1] = v.y;
+            d[4 * q + 2] = v.z;
+            d[4 * q + 3] = v.w;
+        }
+    } else if constexpr (NC == 2) {
+        const float2 v = *(const float2 *)p;
+        d[0] = v.x;
+        d[1] = v.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) d[q] = p[q];
+    }
+}
+
+template <int NC> __device__ __forceinline__ void st_run(float *p, const float (&d)[NC])
+{
+    if constexpr (NC % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < NC / 4; ++q) ((float4 *)p)[q] = make_float4(d[4 * q], d[4 * q + 1], d[4 * q + 2], d[4 * q + 3]);
+    } else if constexpr (NC == 2) {
+        *(float2 *)p = make_float2(d[0], d[1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) p[q] = d[q];
+    }
+}
+
 // interior cell (i, j, k) with value v into the ghost stores of the same-level neighbours whose
 // ghost it is (pt: the block's six ptab entries)
 template <typename T> __device__ __forceinline__ void push_faces(T *gs, const int *pt, int i, int j, int k, T v)
@@ -1283,12 +1318,18 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
     BlockRec R;
     R.load(rec, wa);
     if constexpr (CC) {
-        // the interiors, then the ghost cells: for a colour pass only the parity its stencils read
-        const T *ug = Au + (long)wa * 512;
-        for (int q = threadIdx.x; q < 512; q += NT) {
-            int i, j, k;
-            cc_ijk(q, i, j, k);
-            su[B::gi(i, j, k)] = ug[q];
+        // the interiors (16-byte loads), then the ghost cells: for a colour pass only the parity its
+        // stencils read
+        const float4 *ug = (const float4 *)(Au + (long)wa * 512);
+        for (int q4 = threadIdx.x; q4 < 128; q4 += NT) {
+            const float4 v = ug[q4];
+            const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                int i, j, k;
+                cc_ijk(4 * q4 + u, i, j, k);
+                su[B::gi(i, j, k)] = e[u];
+            }
         }
         const T *gw = gs + (long)wa * 384;
         constexpr int NG = RELAX ? 192 : 384;
@@ -1303,13 +1344,15 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
         const T *ug = Au + (long)R.b * B::LEN;
         for (int q = threadIdx.x; q < B::LEN; q += NT) su[q] = ug[q];
     }
-    // this thread's cells: colour cells c = threadIdx.x + r NT (relax) or cells c (residual, in
-    // compact order in the compact form)
+    // this thread's cells: colour cells c = threadIdx.x + r NT (relax) or cells c (residual); in the
+    // compact form the NC contiguous cells c = NC threadIdx.x + r (one row of one colour for NC = 4),
+    // whose source and result move as one access
     int ci[NC], cj[NC], ck[NC];
     T rhs[NC];
+    if constexpr (CC) ld_run<NC>(rhs, rhsc + (long)wa * 512 + (RELAX ? (ipass - 1) * 256 : 0) + NC * threadIdx.x);
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
-        const int c = threadIdx.x + r * NT;
+        const int c = CC ? NC * (int)threadIdx.x + r : (int)threadIdx.x + r * NT;
         if (RELAX) {
             cj[r] = (c >> 2) & 7;
             ck[r] = c >> 5;
@@ -1321,6 +1364,7 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             cj[r] = (c >> 3) & 7;
             ck[r] = c >> 6;
         }
+        if constexpr (CC) continue;
         if constexpr (std::is_same<T, float>::value) {
             if (rhsc) {
                 rhs[r] = rhsc[(long)wa * 512 + (RELAX ? (ipass - 1) * 256 + c : cc_index(ci[r], cj[r], ck[r]))];
@@ -1349,21 +1393,25 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
     FluxR<8, T> F{su, nullptr, ov, hi, R};
     double a = 0.0, m2 = 0.0, sq = 0.0;
     const double hd = h_lev.v[R.l];
+    [[maybe_unused]] T vout[NC];   // the compact form's results, stored as one run after the loop
 #pragma unroll
     for (int r = 0; r < NC; ++r) {
-        const long q = CC ? (RELAX ? (long)wa * 512 + (ipass - 1) * 256 + threadIdx.x + r * NT
-                                   : (long)wa * 512 + threadIdx.x + r * NT)
-                          : (long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r]);
+        const long q = (long)R.b * B::LEN + B::gi(ci[r], cj[r], ck[r]);
         if (RELAX) {
             T res = F.div(ci[r], cj[r], ck[r]) - rhs[r] * h;
             T resh2 = res * h;
             const T v = su[B::gi(ci[r], cj[r], ck[r])] + (T)(1.0 / 6.0) * resh2;
-            Au[q] = v;
-            if constexpr (CC) push_faces(gs, ptab + wa * 6, ci[r], cj[r], ck[r], v);
+            if constexpr (CC) {
+                vout[r] = v;
+                push_faces(gs, ptab + wa * 6, ci[r], cj[r], ck[r], v);
+            } else {
+                Au[q] = v;
+            }
             a = fmax(a, (double)fabs(resh2));
         } else {
             const T v = rhs[r] - F.div(ci[r], cj[r], ck[r]) * hi;
-            if (Aout) Aout[q] = v;
+            if constexpr (CC) vout[r] = v;
+            else if (Aout) Aout[q] = v;
             if constexpr (std::is_same<T, double>::value) {
                 if (cast) cast[(long)blockIdx.x * 512 + cc_index(ci[r], cj[r], ck[r])] = (float)v;
                 if (npart) {
@@ -1375,6 +1423,7 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             }
         }
     }
+    if constexpr (CC) st_run<NC>((RELAX ? Au + (ipass - 1) * 256 : Aout) + (long)wa * 512 + NC * threadIdx.x, vout);
     if (!RELAX && npart) {
         const int G = gridDim.x;
         block_max_sum(m2, sq);
