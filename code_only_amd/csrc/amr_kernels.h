@@ -1331,14 +1331,18 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
                 su[B::gi(i, j, k)] = e[u];
             }
         }
-        const T *gw = gs + (long)wa * 384;
-        constexpr int NG = RELAX ? 192 : 384;
-        for (int q = threadIdx.x; q < NG; q += NT) {
-            const int f = RELAX ? q >> 5 : q >> 6;
-            const int par = RELAX ? (ipass + 1) & 1 : (q >> 5) & 1, x = q & 31;
-            const int d = f >> 1, lr = f & 1, t2 = x >> 2;
-            const int t1 = 2 * (x & 3) + ((par + (lr ? 0 : 1) + t2) & 1);
-            su[B::rot(d, lr ? 8 : -1, t1, t2)] = gw[f * 64 + par * 32 + x];
+        // (16-byte loads: four ghost cells along t1 of one t2 row per load)
+        const float4 *gw = (const float4 *)(gs + (long)wa * 384);
+        constexpr int NG4 = RELAX ? 48 : 96;
+        for (int q4 = threadIdx.x; q4 < NG4; q4 += NT) {
+            const int f = RELAX ? q4 >> 3 : q4 >> 4;
+            const int par = RELAX ? (ipass + 1) & 1 : (q4 >> 3) & 1, t2 = q4 & 7;
+            const int d = f >> 1, lr = f & 1, p1 = (par + (lr ? 0 : 1) + t2) & 1;
+            const float4 v = gw[f * 16 + par * 8 + t2];
+            su[B::rot(d, lr ? 8 : -1, p1, t2)] = v.x;
+            su[B::rot(d, lr ? 8 : -1, 2 + p1, t2)] = v.y;
+            su[B::rot(d, lr ? 8 : -1, 4 + p1, t2)] = v.z;
+            su[B::rot(d, lr ? 8 : -1, 6 + p1, t2)] = v.w;
         }
     } else {
         const T *ug = Au + (long)R.b * B::LEN;
@@ -1403,7 +1407,10 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             const T v = su[B::gi(ci[r], cj[r], ck[r])] + (T)(1.0 / 6.0) * resh2;
             if constexpr (CC) {
                 vout[r] = v;
-                push_faces(gs, ptab + wa * 6, ci[r], cj[r], ck[r], v);
+                // x faces here; the y and z faces take the thread's run of cells as one store below
+                const int i = ci[r], hp = ((i + cj[r] + ck[r]) & 1) * 32;
+                if (i == 0 && ptab[wa * 6] >= 0) gs[(long)ptab[wa * 6] * 64 + hp + (cj[r] >> 1) + 4 * ck[r]] = v;
+                if (i == 7 && ptab[wa * 6 + 1] >= 0) gs[(long)ptab[wa * 6 + 1] * 64 + hp + (cj[r] >> 1) + 4 * ck[r]] = v;
             } else {
                 Au[q] = v;
             }
@@ -1423,7 +1430,22 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             }
         }
     }
-    if constexpr (CC) st_run<NC>((RELAX ? Au + (ipass - 1) * 256 : Aout) + (long)wa * 512 + NC * threadIdx.x, vout);
+    if constexpr (CC) {
+        st_run<NC>((RELAX ? Au + (ipass - 1) * 256 : Aout) + (long)wa * 512 + NC * threadIdx.x, vout);
+        if constexpr (RELAX) {
+            // the run is NC cells of one row (fixed j, k; i = ci[0], ci[0] + 2, ...) of one parity:
+            // on a y or z face they are NC contiguous cells of the neighbour's ghost store half
+            const int j = cj[0], k = ck[0], hp = ((ci[0] + j + k) & 1) * 32, x0 = ci[0] >> 1;
+            auto run = [&](int f, int t2) {
+                const int tg = ptab[wa * 6 + f];
+                if (tg >= 0) st_run<NC>(gs + (long)tg * 64 + hp + x0 + 4 * t2, vout);
+            };
+            if (j == 0) run(2, k);
+            if (j == 7) run(3, k);
+            if (k == 0) run(4, j);
+            if (k == 7) run(5, j);
+        }
+    }
     if (!RELAX && npart) {
         const int G = gridDim.x;
         block_max_sum(m2, sq);
