@@ -1988,6 +1988,7 @@ int build_adjoint(sg_amr &s)
     std::vector<int> cov, tab;
     SG_REQUIRE(build_prep_tables(s, 0, cov, tab), "sg_amr_adjoint: the hierarchy's coarse-fine ghosts have no single-pass sources");
     const long n = (long)s.nblk * 512;
+    SG_REQUIRE(n < (long)INT32_MAX, "sg_amr_adjoint: %d blocks exceed the 32-bit cell indices of the transposed operator", s.nblk);
     std::vector<int> cfidx((size_t)6 * s.nblk, -1), physidx((size_t)6 * s.nblk, -1);
     for (size_t e = 0; e < s.cf.size(); ++e) cfidx[s.cf[e]] = (int)e;
     for (size_t e = 0; e < s.phys.size(); ++e) physidx[s.phys[e]] = (int)e;
@@ -2080,7 +2081,6 @@ int build_adjoint(sg_amr &s)
                     ivol[row] = 1.0 / vol[row];
                     ent.clear();
                     gent.clear();
-                    const int p[3] = {i, j, k};
                     for (int d = 0; d < 3; ++d)
                         for (int lr = 0; lr < 2; ++lr) {
                             int q[3] = {i, j, k};
@@ -2116,7 +2116,6 @@ int build_adjoint(sg_amr &s)
                                 addcell(b, i, j, k, -hi2);
                             }
                         }
-                    (void)p;
                     std::sort(ent.begin(), ent.end());
                     for (size_t q = 0; q < ent.size();) {
                         size_t r = q;
