@@ -26,10 +26,13 @@ pytestmark = pytest.mark.gpu
 OPS = oracle
 
 HIERS = {"nested-sphere-4-3": lambda: ah.nested(base=4, levels=3, bc=ah.SPHERE),
-         "nested-sphere-8-4": lambda: ah.nested(base=8, levels=4, bc=ah.SPHERE)}
+         "nested-sphere-8-4": lambda: ah.nested(base=8, levels=4, bc=ah.SPHERE),
+         # refinement reaching the domain boundary: physical ghosts of fine blocks in A and B
+         "random-sphere-4-3": lambda: ah.random_hierarchy(4, 3, 0.35, seed=3, bc=ah.SPHERE)}
 # one refinement level: where the transposed solve converges
 SOLVE_HIERS = {"nested-sphere-4-2": lambda: ah.nested(base=4, levels=2, bc=ah.SPHERE),
-               "nested-sphere-8-2": lambda: ah.nested(base=8, levels=2, bc=ah.SPHERE)}
+               "nested-sphere-8-2": lambda: ah.nested(base=8, levels=2, bc=ah.SPHERE),
+               "random-sphere-4-2": lambda: ah.random_hierarchy(4, 2, 0.35, seed=3, bc=ah.SPHERE)}
 
 
 def _leaf(h):

@@ -291,7 +291,7 @@ def test_zero_average_toggle_recaptures_the_graph(pair):
 
 # ---- fused refresh of the mixed solve ----
 
-@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic"])
+@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic", "random-sphere"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_fused_refresh_equals_staged(which, graph, gpu):
     """The mixed solve with one-launch refreshes (k_prep + halo-free colour passes) against the
@@ -300,7 +300,9 @@ def test_fused_refresh_equals_staged(which, graph, gpu):
     value the staged fill gives it.  The nested hierarchies must admit the fused form."""
     h = {"nested-sphere": lambda: ah.nested(base=4, levels=3, bc=ah.SPHERE),
          "nested-periodic": lambda: ah.nested(base=4, levels=3, bc=ah.PERIODIC),
-         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC)}[which]()
+         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC),
+         # refinement reaching the domain boundary: physical faces on fine blocks
+         "random-sphere": lambda: ah.random_hierarchy(4, 3, 0.35, seed=3, bc=ah.SPHERE)}[which]()
     g = AmrSolver(h)
     info = g.info()
     print(which, info, flush=True)
