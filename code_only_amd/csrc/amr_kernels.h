@@ -1313,7 +1313,7 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
     using B = Blk<8>;
     static_assert(!CC || std::is_same<T, float>::value, "the compact form is FP32");
     constexpr int NC = RELAX ? 256 / NT : 512 / NT;   // cells per thread
-    __shared__ T su[B::LEN];
+    __shared__ __align__(16) T su[B::LEN];
     const int wa = (int)blockIdx.x;
     BlockRec R;
     R.load(rec, wa);
@@ -1345,8 +1345,12 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             su[B::rot(d, lr ? 8 : -1, 6 + p1, t2)] = v.w;
         }
     } else {
-        const T *ug = Au + (long)R.b * B::LEN;
-        for (int q = threadIdx.x; q < B::LEN; q += NT) su[q] = ug[q];
+        // the ghosted block, 16 bytes per load (a block is 1000 values: 4000 or 8000 bytes)
+        constexpr int VW = 16 / sizeof(T), NV = B::LEN / VW;
+        static_assert(B::LEN % VW == 0, "the ghosted block splits into 16-byte loads");
+        using V = typename std::conditional<std::is_same<T, float>::value, float4, double2>::type;
+        const V *ug = (const V *)(Au + (long)R.b * B::LEN);
+        for (int q = threadIdx.x; q < NV; q += NT) *(V *)(su + VW * q) = ug[q];
     }
     // this thread's cells: colour cells c = threadIdx.x + r NT (relax) or cells c (residual); in the
     // compact form the NC contiguous cells c = NC threadIdx.x + r (one row of one colour for NC = 4),
