@@ -37,6 +37,9 @@ namespace {
 constexpr int kNfmg = 3;          // FMG levels with 8^3 blocks: 8^3, 4^3, 2^3 (fmg_data.F90:115)
 constexpr int kMaxLev = 16;       // AMR levels
 constexpr int kThreads = 256;
+// launches of go() over fewer items use half-size workgroups: the FMG-level-1 and -2 passes of a
+// 4096-block level then spread over twice the CUs (measured 3% off the mixed sphere solve)
+constexpr long kSmallWork = 1L << 20;
 
 inline int nof(int L) { return 8 >> L; }
 inline long lenof(int L) { long w = nof(L) + 2; return w * w * w; }
@@ -184,8 +187,9 @@ template <> inline Arrays<float> &ar<float>(sg_amr &s) { return s.f; }
 template <typename K, typename... Args> int go(sg_amr &s, K kern, long work, Args... args)
 {
     if (work <= 0) return SG_OK;
-    unsigned grid = (unsigned)((work + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, s.stream, args...);
+    const int nt = work < kSmallWork ? kThreads / 2 : kThreads;
+    unsigned grid = (unsigned)((work + nt - 1) / nt);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(nt), 0, s.stream, args...);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }
@@ -266,7 +270,8 @@ template <typename T> int ghost_level(sg_amr &s, int l, int L, int code, int cub
 
 // fmg_ghostcell_fix (fmg_ghostcell.F90:43-56): levels coarse to fine, one launch per (level,
 // direction).  `fused` fills each level in one launch instead (same values; measured slower on
-// MI355X than the three staged launches, so the solves do not use it).
+// MI355X than the three staged launches at every FMG level, 2^3 blocks included, since an edge or
+// corner cell re-evaluates the chain of passes it depends on, so the solves do not use it).
 template <typename T> int ghostcell_fix(sg_amr &s, int L, int code, int cubic, bool fused = false)
 {
     for (int l = 0; l < s.nlev; ++l) {
@@ -890,9 +895,15 @@ template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
                   (const int *)s.d_child, (const int *)s.d_covered, s.st));
         // vmg_poisson_tau + vmg_tau_crop (vmg_tau), then vmg_add_onov
         SG_TRY(vmg_ghostcell<T>(s, v, C_U));
-        SG_TRY(go(s, k_tau_fine_c2p<T>, (long)nlv(v) * LENV, RES, (const T *)U, s.lev_start[v], nlv(v), s.hil[LV][v],
-                  (const int *)s.d_parent, (const int *)s.d_octant));
-        SG_TRY(vmg_ghostcell<T>(s, v - 1, C_U));
+        if (s.vfill_ok[v - 1]) {   // with the coarser level's vmg_ghostcell gather in the same launch
+            const int e0 = s.vfill_off[v - 1], ne = s.vfill_off[v] - e0;
+            SG_TRY(go(s, k_tau_fine_c2p_fill<T>, (long)nlv(v) * LENV + ne, RES, U, s.lev_start[v], nlv(v), s.hil[LV][v],
+                      (const int *)s.d_parent, (const int *)s.d_octant, (const int2 *)s.d_vfill + e0, ne));
+        } else {
+            SG_TRY(go(s, k_tau_fine_c2p<T>, (long)nlv(v) * LENV, RES, (const T *)U, s.lev_start[v], nlv(v), s.hil[LV][v],
+                      (const int *)s.d_parent, (const int *)s.d_octant));
+            SG_TRY(vmg_ghostcell<T>(s, v - 1, C_U));
+        }
         const int c0 = s.cov_off[v - 1], nc = s.cov_off[v] - c0;
         SG_TRY(go(s, k_tau_finish<T>, (long)nc * LENV, RES, RHS, (const T *)U, (const int *)(s.d_cov + c0), nc,
                   s.hil[LV][v - 1], (const unsigned long long *)(s.d_crop + c0)));
@@ -906,10 +917,16 @@ template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
         SG_TRY(go(s, k_sub_fill<T>, (long)nlv(v - 1) * LENV, RES, (const T *)U, (const T *)PSI, base,
                   (int)((long)nlv(v - 1) * LENV), (const int *)(s.d_vmap + base)));
         SG_TRY(extrap_faces<T>(s, LV, C_RES, v - 1, v));
-        SG_TRY(go(s, k_p2c_add<T>, (long)nlv(v) * LENV, RES, U, s.lev_start[v], nlv(v), (const int *)s.d_parent,
-                  (const int *)s.d_octant));
+        // with vmg_gridboundary(v) in the same launch (it reads only level v's U and level v-1's U,
+        // which the fill0 of RES at v-1 after it does not write)
+        const int fill0 = s.phys_off[v + 1] > s.phys_off[v] ? 1 : 0;
+        if (nlv(v) > 0) {
+            hipLaunchKernelGGL(k_p2c_add_pfill<T>, dim3((unsigned)((nlv(v) + 7) / 8)), dim3(256), 0, s.stream, RES, U,
+                               s.lev_start[v], nlv(v), fill0, (const int *)s.d_nbr, (const int *)s.d_nkind,
+                               (const int *)s.d_parent, (const int *)s.d_octant);
+            SG_CHECK_LAUNCH();
+        }
         SG_TRY(fill0_faces<T>(s, LV, C_RES, v - 1, v));
-        SG_TRY(vmg_gridboundary<T>(s, v, C_U));
         for (int p = 0; p < 2; ++p) SG_TRY(vmg_relax<T>(s, v, C_U, C_RHS, p == 1));
     }
     return SG_OK;

@@ -433,12 +433,17 @@ template <int N> struct GhostCells {
 // with is the cell the copies lead back to) or, on a physical face, zero.  pairs[t] = {destination,
 // source or -1 for zero}, resolved by the host (amr.hip, same_fill_pairs), which also checks that no
 // source is a destination.
+template <typename T> __device__ __forceinline__ void fill_pair(T *A, const int2 *pairs, long t)
+{
+    const int2 p = pairs[t];
+    A[p.x] = p.y >= 0 ? A[p.y] : (T)0;
+}
+
 template <typename T> __global__ void k_fill_pairs(T *A, const int2 *pairs, int n)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int2 p = pairs[t];
-    A[p.x] = p.y >= 0 ? A[p.y] : (T)0;
+    fill_pair(A, pairs, t);
 }
 
 template <int N, typename T>
@@ -1002,8 +1007,8 @@ __device__ __forceinline__ bool interior(int i, int j, int k) { return i >= 0 &&
 // cells, never a same-level neighbour, so each block runs its passes on its own: 32 threads per
 // block (the two face slabs of the pass direction), a workgroup barrier between passes.
 template <typename T>
-__global__ void __launch_bounds__(256) k_parent_fill(T *A, int b0, int nb, int fill0, const int *nbr, const int *nkind,
-                                                     const int *octant)
+__device__ __forceinline__ void parent_fill_block(T *A, int b0, int nb, int fill0, const int *nbr, const int *nkind,
+                                                  const int *octant)
 {
     using B = Blk<2>;
     constexpr int N = 2, W = B::W, H = 1;
@@ -1041,6 +1046,13 @@ __global__ void __launch_bounds__(256) k_parent_fill(T *A, int b0, int nb, int f
         const int f = g / (W * W), r = g % (W * W);
         if (nkind[b * 6 + f] == NBR_NONE) ud[B::rot(f >> 1, (f & 1) ? N : -1, r % W - 1, r / W - 1)] = (T)0;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_parent_fill(T *A, int b0, int nb, int fill0, const int *nbr, const int *nkind,
+                                                     const int *octant)
+{
+    parent_fill_block(A, b0, nb, fill0, nbr, nkind, octant);
 }
 
 // fmg_converge_c2p_lev of two arrays at once (children [b0, b0+nb) into their parents)
@@ -1084,12 +1096,10 @@ __global__ void k_vmg_rstr(T *U, T *PSI, T *RHS, int b0, int nb, const int *chil
 // [b0, b0+nb)) + fmg_converge_c2p_lev of RES into the parents.  The thread of a block's first
 // interior cell also forms the restriction, from the eight values of its block (recomputed in the
 // same way, so equal to the stored ones).
-template <typename T> __global__ void k_tau_fine_c2p(T *RES, const T *U, int b0, int nb, double hi_d, const int *parent,
-                                                    const int *octant)
+template <typename T> __device__ __forceinline__ void tau_fine_c2p(T *RES, const T *U, int b0, double hi_d, const int *parent,
+                                                                  const int *octant, long t)
 {
     using B = Blk<2>;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nb * B::LEN) return;
     const int b = b0 + (int)(t / B::LEN), c = (int)(t % B::LEN);
     int i, j, k;
     vmgm::cell_of(c, i, j, k);
@@ -1118,6 +1128,26 @@ template <typename T> __global__ void k_tau_fine_c2p(T *RES, const T *U, int b0,
     sum = sum + f[1][1][1];
     const int o = octant[b];
     RES[(long)parent[b] * B::LEN + B::gi(o & 1, (o >> 1) & 1, (o >> 2) & 1)] = sum * (T)(1.0 / 8.0);
+}
+
+template <typename T> __global__ void k_tau_fine_c2p(T *RES, const T *U, int b0, int nb, double hi_d, const int *parent,
+                                                    const int *octant)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * Blk<2>::LEN) return;
+    tau_fine_c2p(RES, U, b0, hi_d, parent, octant, t);
+}
+
+// k_tau_fine_c2p on AMR level l (items [0, nb 64)) and the coarser level's vmg_ghostcell gather
+// (k_fill_pairs, items [nb 64, + np)) in one launch: the first reads level-l U and writes RES, the
+// second reads and writes level-(l-1) U only, so neither reads what the other writes.
+template <typename T> __global__ void k_tau_fine_c2p_fill(T *RES, T *U, int b0, int nb, double hi_d, const int *parent,
+                                                         const int *octant, const int2 *pairs, int np)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nt = (long)nb * Blk<2>::LEN;
+    if (t < nt) tau_fine_c2p(RES, (const T *)U, b0, hi_d, parent, octant, t);
+    else if (t < nt + np) fill_pair(U, pairs, t - nt);
 }
 
 // vmg_poisson_tau's coarse part (tau += (div F)/h inside) + vmg_tau_crop (the planes of the
@@ -1156,12 +1186,10 @@ template <typename T> __global__ void k_sub_fill(T *RES, const T *U, const T *PS
 
 // vmg_interp_p2c_lev (trilinear, child interiors from the parent's octant, its ghosts included) +
 // vmg_add (U += RES over whole arrays) for the blocks [b0, b0+nb)
-template <typename T> __global__ void k_p2c_add(T *RES, T *U, int b0, int nb, const int *parent, const int *octant)
+template <typename T> __device__ __forceinline__ void p2c_add_cell(T *RES, T *U, int b, int c, const int *parent,
+                                                                  const int *octant)
 {
     using B = Blk<2>;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nb * B::LEN) return;
-    const int b = b0 + (int)(t / B::LEN), c = (int)(t % B::LEN);
     int i, j, k;
     vmgm::cell_of(c, i, j, k);
     const long q = (long)b * B::LEN + c;
@@ -1176,6 +1204,30 @@ template <typename T> __global__ void k_p2c_add(T *RES, T *U, int b0, int nb, co
         return;
     }
     U[q] = U[q] + RES[q];
+}
+
+template <typename T> __global__ void k_p2c_add(T *RES, T *U, int b0, int nb, const int *parent, const int *octant)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nb * Blk<2>::LEN) return;
+    p2c_add_cell(RES, U, b0 + (int)(t / Blk<2>::LEN), (int)(t % Blk<2>::LEN), parent, octant);
+}
+
+// k_p2c_add, then vmg_gridboundary (k_parent_fill), for the blocks [b0, b0+nb) in one launch: 8
+// blocks per workgroup as in k_parent_fill.  A block's add writes only its own cells (from the
+// parent level's RES), its parent-level passes read only its own cells and the parent level's U,
+// so a workgroup barrier between the two is all the order they need.
+template <typename T>
+__global__ void __launch_bounds__(256) k_p2c_add_pfill(T *RES, T *U, int b0, int nb, int fill0, const int *nbr,
+                                                       const int *nkind, const int *parent, const int *octant)
+{
+    constexpr int LEN = Blk<2>::LEN;
+    const int bl = (int)threadIdx.x >> 5, idx = (int)threadIdx.x & 31;
+    const int b = b0 + (int)blockIdx.x * 8 + bl;
+    if (b < b0 + nb)
+        for (int c = idx; c < LEN; c += 32) p2c_add_cell(RES, U, b, c, parent, octant);
+    __syncthreads();
+    parent_fill_block(U, b0, nb, fill0, nbr, nkind, octant);
 }
 
 // ---- vmg_interp_p2c_lev (trilinear): child interiors of [b0, b0+nb) from the parent's octant ----
