@@ -277,17 +277,17 @@ __global__ void k_ghost(T *A, int b0, int nb, int d, int cubic, int kinds, const
     int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1, lr = (int)((t / (W * W)) & 1);
     int b = b0 + (int)(t / (2 * W * W));
     int f = 2 * d + lr;
-    int kind = nkind[b * 6 + f];
+    // the three table loads issued together, ahead of the test that may make two of them moot
+    const int kind = nkind[b * 6 + f], src = nbr[b * 6 + f], o = octant[b];
     if (kind == NBR_NONE || (kind == NBR_SAME && !(kinds & 1)) || (kind == NBR_PARENT && !(kinds & 2))) return;
     T *ud = A + (long)b * B::LEN;
-    const T *us = A + (long)nbr[b * 6 + f] * B::LEN;
+    const T *us = A + (long)src * B::LEN;
     const int ag = lr ? N : -1;
     if (kind == NBR_SAME) {
         ud[B::rot(d, ag, t1, t2)] = us[B::rot(d, lr ? 0 : N - 1, t1, t2)];
         return;
     }
     const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-    const int o = octant[b];
     const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
     const int an = lr ? 0 : N - 1;
     auto buf = [&](int c1, int c2) { return us[B::rot(d, an, c1 + e1, c2 + e2)]; };
@@ -1016,13 +1016,21 @@ __device__ __forceinline__ void parent_fill_block(T *A, int b0, int nb, int fill
     const int b = b0 + (int)blockIdx.x * 8 + bl;
     const bool mine = b < b0 + nb;
     const int lr = idx >> 4, t1 = (idx & 3) - 1, t2 = ((idx >> 2) & 3) - 1;
-    T *ud = A + (long)(mine ? b : b0) * B::LEN;
+    const int bb = mine ? b : b0;
+    T *ud = A + (long)bb * B::LEN;
+    // the block's tables for its three passes, loaded together ahead of them
+    int kd[3], src[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const int f = 2 * d + lr;
-        if (mine && nkind[b * 6 + f] == NBR_PARENT) {
-            const T *us = A + (long)nbr[b * 6 + f] * B::LEN;
+        kd[d] = nkind[bb * 6 + 2 * d + lr];
+        src[d] = nbr[bb * 6 + 2 * d + lr];
+    }
+    const int o = octant[bb];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (mine && kd[d] == NBR_PARENT) {
+            const T *us = A + (long)src[d] * B::LEN;
             const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-            const int o = octant[b];
             const int e1 = ((o >> ta) & 1) * H, e2 = ((o >> tb) & 1) * H;
             const int an = lr ? 0 : N - 1;
             auto buf = [&](int c1, int c2) { return us[B::rot(d, an, c1 + e1, c2 + e2)]; };
@@ -1080,9 +1088,10 @@ __global__ void k_vmg_rstr(T *U, T *PSI, T *RHS, int b0, int nb, const int *chil
     int i, j, k;
     vmgm::cell_of(c, i, j, k);
     const long q = (long)b * B::LEN + c;
-    if (vmgm::interior(i, j, k) && covered[b]) {
-        const int o = i + 2 * j + 4 * k;
-        const long cb = (long)child[8 * b + o] * B::LEN;
+    const bool in = vmgm::interior(i, j, k);
+    const int cov = covered[b], ch = in ? child[8 * b + i + 2 * j + 4 * k] : -1;   // loaded together
+    if (in && cov) {
+        const long cb = (long)ch * B::LEN;
         const T v = rstr_cub<2, T>(U + cb, 0, 0, 0, st);
         U[q] = v;
         PSI[q] = v;
