@@ -67,10 +67,12 @@ class AmrSolver:
     def info(self):
         """Context facts: FMG levels whose mixed-solve refreshes run fused (bit mask), kernel nodes of the
         captured mixed outer iteration, blocks, coarse-fine faces, AMR levels whose vmg_ghostcell
-        runs as one gather in the fused mode (bit mask)."""
-        v = (ctypes.c_int * 5)()
-        _lib.check(self._lib.sg_amr_info(self._ctx, v, 5))
-        return dict(fused_levels=v[0], graph_nodes=v[1], nblk=v[2], cf_faces=v[3], vmg_fill_levels=v[4])
+        runs as one gather in the fused mode (bit mask), FMG levels whose base-level
+        fmg_ghostcell_fix runs as one gather in the fused mode (bit mask)."""
+        v = (ctypes.c_int * 6)()
+        _lib.check(self._lib.sg_amr_info(self._ctx, v, 6))
+        return dict(fused_levels=v[0], graph_nodes=v[1], nblk=v[2], cf_faces=v[3], vmg_fill_levels=v[4],
+                    base_gather_levels=v[5])
 
     def tile_info(self):
         """The autotuner's threads per 8^3 block of the FMG-level-0 relaxation / residual kernels and

@@ -87,6 +87,11 @@ struct sg_amr {
     int *d_vfill = nullptr;
     int vfill_off[kMaxLev + 1] = {};
     int vfill_ok[kMaxLev] = {};
+    // fmg_ghostcell_fix of the base AMR level at FMG levels 1 and 2 as one gather each (k_fill_pairs;
+    // pairs [gfix0_off[L], gfix0_off[L+1]) of d_gfix0), where the hierarchy allows it
+    int *d_gfix0 = nullptr;
+    int gfix0_off[kNfmg + 1] = {};
+    int gfix0_ok[kNfmg] = {};
     // merged vmg_fas steps (vmg_fas_merged): per FMG-level-2 cell what vmg_ghostcell does to it
     // (-2 untouched, -1 zero, else the cell it copies), and per covered block the mask of the 64
     // cells vmg_tau_crop zeroes; usable when every level has its gather table
@@ -269,12 +274,20 @@ template <typename T> int ghost_level(sg_amr &s, int l, int L, int code, int cub
 }
 
 // fmg_ghostcell_fix (fmg_ghostcell.F90:43-56): levels coarse to fine, one launch per (level,
-// direction).  `fused` fills each level in one launch instead (same values; measured slower on
-// MI355X than the three staged launches at every FMG level, 2^3 blocks included, since an edge or
-// corner cell re-evaluates the chain of passes it depends on, so the solves do not use it).
+// direction); in fused mode the base level, whose faces are all same-level or physical, as one
+// gather of resolved copies (same values).  `fused` fills each level in one launch instead (same
+// values; measured slower on MI355X than the three staged launches at every FMG level, 2^3 blocks
+// included, since an edge or corner cell re-evaluates the chain of passes it depends on, so the
+// solves do not use it).
 template <typename T> int ghostcell_fix(sg_amr &s, int L, int code, int cubic, bool fused = false)
 {
     for (int l = 0; l < s.nlev; ++l) {
+        if (l == 0 && s.fused_on && s.gfix0_ok[L]) {
+            // the base level has same-level faces only: its three passes as one gather
+            const int e0 = s.gfix0_off[L], ne = s.gfix0_off[L + 1] - e0;
+            SG_TRY(go(s, k_fill_pairs<T>, (long)ne, ar<T>(s).A[L][code], (const int2 *)s.d_gfix0 + e0, ne));
+            continue;
+        }
         if (fused) SG_TRY(ghost_level<T>(s, l, L, code, cubic));
         else
             for (int d = 0; d < 3; ++d) SG_TRY(ghost_pass<T>(s, l, L, d, code, cubic, K_BOTH));
@@ -1411,7 +1424,7 @@ bool prep_source(const sg_amr &s, int N, int b, int i, int j, int k, int &src)
 // to x (the last pass writing it reads what the earlier ones left), and a cell on a physical face
 // ends as zero (source -1).  False where a single gather would not reproduce the passes: a source
 // that the fill itself writes (a pass would read it before a later pass changes it).
-bool same_fill_pairs(const sg_amr &s, int N, int l, std::vector<int> &pairs)
+bool same_fill_pairs(const sg_amr &s, int N, int l, std::vector<int> &pairs, bool zero_phys = true)
 {
     const int b0 = s.lev_start[l], nb = s.lev_start[l + 1] - b0;
     const long W = N + 2, LEN = W * W * W;
@@ -1432,10 +1445,10 @@ bool same_fill_pairs(const sg_amr &s, int N, int l, std::vector<int> &pairs)
                             phys = phys || kind == NBR_NONE;
                             same = same || kind == NBR_SAME;
                         }
-                    if (!ghost || !(phys || same)) continue;
+                    if (!ghost || !(same || (phys && zero_phys))) continue;
                     const long dst = addr(b, i, j, k);
                     target[(size_t)(dst - (long)b0 * LEN)] = 1;
-                    if (phys) {
+                    if (phys && zero_phys) {
                         pairs.insert(pairs.end(), {(int)dst, -1});
                         continue;
                     }
@@ -1592,7 +1605,7 @@ void release(sg_amr *s)
     if (!s) return;
     int *ip[] = {s->d_level, s->d_ijk, s->d_parent, s->d_octant, s->d_nbr, s->d_nkind, s->d_covered, s->d_skip,
                  s->d_active, s->d_phys, s->d_cf, s->d_cf_slot, s->d_ovslot, s->d_cov, s->d_iters,
-                 s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap,
+                 s->d_child, s->d_cfrec, s->d_brec, s->d_arec, s->d_vfill, s->d_vmap, s->d_gfix0,
                  s->d_slot_u, s->d_slot_u1, s->d_slot_u2, s->d_aidx, s->d_ptab, s->d_cov_cc, s->d_src_cc};
     for (int *q : ip) if (q) (void)hipFree(q);
     if (s->d_crop) (void)hipFree(s->d_crop);
@@ -1739,6 +1752,12 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
         s->vfill_ok[l] = same_fill_pairs(*s, 2, l, vfill) ? 1 : 0;
     }
     s->vfill_off[nlev] = (int)vfill.size() / 2;
+    std::vector<int> gfix0;
+    for (int L = 1; L < kNfmg; ++L) {
+        s->gfix0_off[L] = (int)gfix0.size() / 2;
+        s->gfix0_ok[L] = same_fill_pairs(*s, nof(L), 0, gfix0, false) ? 1 : 0;
+    }
+    s->gfix0_off[kNfmg] = (int)gfix0.size() / 2;
     std::vector<int> vmap;
     std::vector<unsigned long long> crop;
     s->vmg_merged_ok = 1;
@@ -1807,6 +1826,7 @@ int create(sg_amr *s, int nblk, int nlev, const int *level, const int *ijk, cons
         }
         SG_TRY(upload(*s, &s->d_cfrec, rec));
         SG_TRY(upload(*s, &s->d_vfill, vfill));
+        SG_TRY(upload(*s, &s->d_gfix0, gfix0));
         if (s->vmg_merged_ok) {
             SG_TRY(upload(*s, &s->d_vmap, vmap));
             SG_TRY(upload(*s, &s->d_crop, crop));
@@ -2514,9 +2534,12 @@ int sg_amr_info(sg_amr *ctx, int *info, int n)
     long vmg = 0;   // AMR levels whose vmg_ghostcell runs as one gather
     for (int l = 0; l < ctx->nlev; ++l)
         if (ctx->vfill_ok[l]) vmg |= 1L << l;
-    const long v[5] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
-                       (long)ctx->cf.size(), vmg};
-    for (int q = 0; q < n; ++q) info[q] = q < 5 ? (int)v[q] : 0;
+    long gfix = 0;   // FMG levels whose base-level fmg_ghostcell_fix runs as one gather
+    for (int L = 1; L < kNfmg; ++L)
+        if (ctx->gfix0_ok[L]) gfix |= 1L << L;
+    const long v[6] = {(ctx->fused_ok[0] ? 1 : 0) | (ctx->fused_ok[1] ? 2 : 0), (long)ctx->graph_nodes, ctx->nblk,
+                       (long)ctx->cf.size(), vmg, gfix};
+    for (int q = 0; q < n; ++q) info[q] = q < 6 ? (int)v[q] : 0;
     return SG_OK;
 }
 

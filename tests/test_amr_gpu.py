@@ -308,6 +308,8 @@ def test_fused_refresh_equals_staged(which, graph, gpu):
     print(which, info, flush=True)
     if which.startswith("nested"):
         assert info["fused_levels"] == 3
+    # the base level's fmg_ghostcell_fix at FMG levels 1 and 2 as one gather
+    assert info["base_gather_levels"] == 6
     src = problems.amr_source(h, seed=17)
     ps, gs, ss = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=graph, fused=False)
     pf, gf, sf = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=graph, fused=True)
