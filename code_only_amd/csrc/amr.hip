@@ -199,6 +199,12 @@ template <typename K, typename... Args> int go(sg_amr &s, K kern, long work, Arg
     return SG_OK;
 }
 
+// two independent steps (functors over items [0, n1) and [0, n2)) in one launch
+template <class F1, class F2> int go2(sg_amr &s, F1 f1, long n1, F2 f2, long n2)
+{
+    return go(s, k_pair<F1, F2>, n1 + n2, f1, n1, f2, n2);
+}
+
 template <typename K, typename... Args> int go1(sg_amr &s, K kern, Args... args)
 {
     hipLaunchKernelGGL(kern, dim3(1), dim3(1024), 0, s.stream, args...);
@@ -311,6 +317,9 @@ template <int N, typename T> int extrap_faces_n(sg_amr &s, int L, int code, int 
 {
     int e0 = s.phys_off[l0], ne = s.phys_off[l1] - e0;
     long W = N + 2;
+    if (s.fused_on)   // the three passes in one launch (same values)
+        return go(s, k_extrap_all<N, T>, (long)ne * W * W, ar<T>(s).A[L][code], (const int *)s.d_phys + e0, ne,
+                  (const int *)s.d_nkind);
     for (int d = 0; d < 3; ++d)
         SG_TRY(go(s, k_extrap_faces<N, T>, (long)ne * W * W, ar<T>(s).A[L][code], s.d_phys + e0, ne, d));
     return SG_OK;
@@ -625,6 +634,29 @@ template <typename T> int fmg_fill0(sg_amr &s, int L, int code)
               (int)lenof(L));
 }
 
+// fmg_rstrct(Lc, iuc <- iuf), then fmg_fill0(Lc, jz); in fused mode one launch (the zero writes
+// neither the restriction's source nor its destination)
+template <typename T> int fmg_rstrct_fill0(sg_amr &s, int Lc, int iuc, int iuf, int jz)
+{
+    if (!s.fused_on || iuc == jz) {
+        SG_TRY(fmg_rstrct<T>(s, Lc, iuc, iuf));
+        return fmg_fill0<T>(s, Lc, jz);
+    }
+    const int na = (int)s.active.size();
+    auto &R = ar<T>(s);
+    const Fill0BlocksF<T> z{R.A[Lc][jz], (const int *)s.d_active, 0, (int)lenof(Lc)};
+    const long nz = (long)na * lenof(Lc);
+    if constexpr (std::is_same<T, float>::value) {
+        if (Lc == 1 && src0c<T>(s, 0, iuf))
+            return go2(s, RstrctCCF{s.f.A[1][iuc], (const float *)s.rhs0c, (const int *)s.d_active}, (long)na * 64, z, nz);
+        if (Lc == 1 && iuf == C_RES && compact0<T>(s, 0, C_U))
+            return go2(s, RstrctCCF{s.f.A[1][iuc], (const float *)s.res0c, (const int *)s.d_active}, (long)na * 64, z, nz);
+    }
+    if (Lc == 1)
+        return go2(s, RstrctF<4, T>{R.A[1][iuc], (const T *)R.A[0][iuf], (const int *)s.d_active}, (long)na * 64, z, nz);
+    return go2(s, RstrctF<2, T>{R.A[2][iuc], (const T *)R.A[1][iuf], (const int *)s.d_active}, (long)na * 8, z, nz);
+}
+
 template <typename T> int fmg_copy(sg_amr &s, int L, int out, int in, int noskip)
 {
     int na = noskip ? s.nblk : (int)s.active.size();
@@ -894,8 +926,10 @@ template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
     for (int lc = s.nlev - 2; lc >= 0; --lc)   // fmg_converge_c2p of U and of RHO, finest first
         SG_TRY(go(s, k_c2p_two<T>, (long)nlv(lc + 1), U, RHO, s.lev_start[lc + 1], nlv(lc + 1), (const int *)s.d_parent,
                   (const int *)s.d_octant));
-    SG_TRY(fill0_faces<T>(s, LV, C_U, 0, s.nlev));
-    SG_TRY(go(s, k_blocks_copy<T>, nall, RHS, (const T *)RHO, (const int *)nullptr, 0, s.nblk, (int)LENV));
+    // fmg_boundary_fill0 of U on every level with the copy RHS <- RHO in one launch
+    const int p0 = s.phys_off[0], np = s.phys_off[s.nlev] - p0;
+    SG_TRY(go2(s, Fill0FacesF<2, T>{U, (const int *)s.d_phys + p0}, (long)np * 16, CopyBlocksF<T>{RHS, (const T *)RHO,
+               (const int *)nullptr, 0, (int)LENV}, nall));
     for (int v = s.nlev - 1; v >= 1; --v) {
         SG_TRY(vmg_gridboundary<T>(s, v, C_U));
         // the inner solve's relaxation keeps no refresh here: the parent- and same-level passes
@@ -994,8 +1028,8 @@ template <typename T> int fmg_lin(sg_amr &s)
     const int Lmax = kNfmg - 1;
     const bool lean = kInner<T> && s.fused_on;
     auto rhs = [&](int v) { return lean && v == 0 ? C_RHO : C_RHS; };
-    for (int L = 1; L <= Lmax; ++L) SG_TRY(fmg_rstrct<T>(s, L, C_RHO, C_RHO));
-    SG_TRY(fmg_fill0<T>(s, Lmax, C_U));
+    for (int L = 1; L < Lmax; ++L) SG_TRY(fmg_rstrct<T>(s, L, C_RHO, C_RHO));
+    SG_TRY(fmg_rstrct_fill0<T>(s, Lmax, C_RHO, C_RHO, C_U));
     SG_TRY(vmg_fas<T>(s, C_RHO));
     for (int Lf = Lmax - 1; Lf >= 0; --Lf) {
         SG_TRY(fmg_interp<T>(s, Lf, C_U, C_U, 1));
@@ -1004,8 +1038,7 @@ template <typename T> int fmg_lin(sg_amr &s)
             for (int v = Lf; v <= Lmax - 1; ++v) {
                 for (int p = 0; p < 2; ++p) SG_TRY(poisson_relax<T>(s, v, C_U, rhs(v), 1, false));
                 SG_TRY(poisson_resid<T>(s, v, C_RES, C_U, rhs(v), 1));
-                SG_TRY(fmg_rstrct<T>(s, v + 1, C_RHS, C_RES));
-                SG_TRY(fmg_fill0<T>(s, v + 1, C_U));
+                SG_TRY(fmg_rstrct_fill0<T>(s, v + 1, C_RHS, C_RES, C_U));
                 s.fill0_level = v + 1;
             }
             SG_TRY(vmg_fas<T>(s, C_RHS));

@@ -578,16 +578,26 @@ __global__ void k_prep(T *A, T *ov, const int2 *cov, int ncov, const int4 *rec, 
 }
 
 // ---- fmg_boundary_fill0 on a list of physical faces (entries b*6+f) ----
+template <int N, typename T> struct Fill0FacesF {   // item t of nf (N+2)^2
+    T *A;
+    const int *faces;
+    __device__ void operator()(long t) const
+    {
+        using B = Blk<N>;
+        constexpr int W = B::W;
+        int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1;
+        int e = faces[t / (W * W)];
+        int b = e / 6, f = e % 6, d = f / 2;
+        A[(long)b * B::LEN + B::rot(d, (f & 1) ? N : -1, t1, t2)] = (T)0;
+    }
+};
+
 template <int N, typename T> __global__ void k_fill0_faces(T *A, const int *faces, int nf)
 {
-    using B = Blk<N>;
-    constexpr int W = B::W;
+    constexpr int W = Blk<N>::W;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nf * W * W) return;
-    int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1;
-    int e = faces[t / (W * W)];
-    int b = e / 6, f = e % 6, d = f / 2;
-    A[(long)b * B::LEN + B::rot(d, (f & 1) ? N : -1, t1, t2)] = (T)0;
+    Fill0FacesF<N, T>{A, faces}(t);
 }
 
 // ---- fmg_boundary_extrap, the faces of direction d only ----
@@ -603,6 +613,38 @@ template <int N, typename T> __global__ void k_extrap_faces(T *A, const int *fac
     if (f / 2 != d) return;
     T *u = A + (long)b * B::LEN;
     u[B::rot(d, (f & 1) ? N : -1, t1, t2)] = -u[B::rot(d, (f & 1) ? N - 1 : 0, t1, t2)];
+}
+
+// fmg_boundary_extrap's three directions in one launch, the values the passes x, y, z leave: a pass
+// sets a physical-face ghost cell to minus the cell next to it inside, so the final value of a cell
+// is plus or minus the cell reached by moving each of its ghost coordinates that lies on a physical
+// face to the adjacent layer, last pass first; that cell is on no physical face, so no item writes
+// it.  Items: the (N+2)^2 cells of each physical face (nkind names a block's other physical faces);
+// a cell on two physical faces is written by both with the same value.
+template <int N, typename T> __global__ void k_extrap_all(T *A, const int *faces, int nf, const int *nkind)
+{
+    using B = Blk<N>;
+    constexpr int W = B::W;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nf * W * W) return;
+    const int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1;
+    const int e = faces[t / (W * W)];
+    const int b = e / 6, f = e % 6, d = f / 2;
+    int c[3];
+    c[d] = (f & 1) ? N : -1;
+    c[d == 0 ? 1 : 0] = t1;
+    c[d == 2 ? 1 : 2] = t2;
+    const int dst = B::gi(c[0], c[1], c[2]);
+    bool neg = false;
+#pragma unroll
+    for (int D = 2; D >= 0; --D)
+        if ((c[D] == -1 || c[D] == N) && nkind[b * 6 + 2 * D + (c[D] == N ? 1 : 0)] == NBR_NONE) {
+            c[D] = c[D] == N ? N - 1 : 0;
+            neg = !neg;
+        }
+    T *u = A + (long)b * B::LEN;
+    const T v = u[B::gi(c[0], c[1], c[2])];
+    u[dst] = neg ? -v : v;
 }
 
 // ---- fmg_reflux_do: fine face fluxes averaged 2x2 into the coarse neighbour's override ----
@@ -816,24 +858,45 @@ __global__ void k_resid(T *Ares, const T *Au, const T *Arhs, const int *rec, int
 }
 
 // ---- fmg_rstrct (linear): coarse level with NC cells from the fine level with 2 NC ----
+template <int NC, typename T> struct RstrctF {   // item t of nb NC^3
+    T *Ac;
+    const T *Af;
+    const int *list;
+    __device__ void operator()(long t) const
+    {
+        constexpr int PER = NC * NC * NC;
+        int i = (int)(t % NC), j = (int)((t / NC) % NC), k = (int)((t / (NC * NC)) % NC);
+        int b = list[t / PER];
+        Ac[(long)b * Blk<NC>::LEN + Blk<NC>::gi(i, j, k)] =
+            rstr_lin<2 * NC, T>(Af + (long)b * Blk<2 * NC>::LEN, 2 * i, 2 * j, 2 * k);
+    }
+};
+
 template <int NC, typename T> __global__ void k_rstrct(T *Ac, const T *Af, const int *list, int nb)
 {
-    constexpr int PER = NC * NC * NC;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nb * PER) return;
-    int i = (int)(t % NC), j = (int)((t / NC) % NC), k = (int)((t / (NC * NC)) % NC);
-    int b = list[t / PER];
-    Ac[(long)b * Blk<NC>::LEN + Blk<NC>::gi(i, j, k)] = rstr_lin<2 * NC, T>(Af + (long)b * Blk<2 * NC>::LEN, 2 * i, 2 * j, 2 * k);
+    if (t >= (long)nb * NC * NC * NC) return;
+    RstrctF<NC, T>{Ac, Af, list}(t);
 }
 
 // fmg_rstrct from FMG level 0 to 1 with the level-0 source in compact form (src + a 512 for list
 // entry a)
+struct RstrctCCF {   // item t of nb 64
+    float *Ac;
+    const float *src;
+    const int *list;
+    __device__ void operator()(long t) const
+    {
+        const int i = (int)(t & 3), j = (int)((t >> 2) & 3), k = (int)((t >> 4) & 3), a = (int)(t >> 6);
+        Ac[(long)list[a] * Blk<4>::LEN + Blk<4>::gi(i, j, k)] = rstr_lin_cc(src + (long)a * 512, 2 * i, 2 * j, 2 * k);
+    }
+};
+
 __global__ void k_rstrct_cc(float *Ac, const float *src, const int *list, int nb)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * 64) return;
-    const int i = (int)(t & 3), j = (int)((t >> 2) & 3), k = (int)((t >> 4) & 3), a = (int)(t >> 6);
-    Ac[(long)list[a] * Blk<4>::LEN + Blk<4>::gi(i, j, k)] = rstr_lin_cc(src + (long)a * 512, 2 * i, 2 * j, 2 * k);
+    RstrctCCF{Ac, src, list}(t);
 }
 
 // ---- fmg_interp_cubic / fmg_interp_linear: fine interior of each listed block from its own
@@ -912,19 +975,46 @@ template <int NF, typename T> __global__ void __launch_bounds__(256) k_interp_cu
 }
 
 // ---- whole-array block operations (ghosts included) ----
+template <typename T> struct Fill0BlocksF {   // item t of nb len
+    T *A;
+    const int *list;
+    int b0, len;
+    __device__ void operator()(long t) const { A[(long)pick_block(list, b0, (int)(t / len)) * len + t % len] = (T)0; }
+};
+
+template <typename T> struct CopyBlocksF {   // item t of nb len
+    T *O;
+    const T *I;
+    const int *list;
+    int b0, len;
+    __device__ void operator()(long t) const
+    {
+        long q = (long)pick_block(list, b0, (int)(t / len)) * len + t % len;
+        O[q] = I[q];
+    }
+};
+
+// two independent steps in one launch: items [0, n1) of f1, then [n1, n1 + n2) of f2 (neither may
+// read what the other writes)
+template <class F1, class F2> __global__ void k_pair(F1 f1, long n1, F2 f2, long n2)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n1) f1(t);
+    else if (t < n1 + n2) f2(t - n1);
+}
+
 template <typename T> __global__ void k_blocks_fill0(T *A, const int *list, int b0, int nb, int len)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * len) return;
-    A[(long)pick_block(list, b0, (int)(t / len)) * len + t % len] = (T)0;
+    Fill0BlocksF<T>{A, list, b0, len}(t);
 }
 
 template <typename T> __global__ void k_blocks_copy(T *O, const T *I, const int *list, int b0, int nb, int len)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nb * len) return;
-    long q = (long)pick_block(list, b0, (int)(t / len)) * len + t % len;
-    O[q] = I[q];
+    CopyBlocksF<T>{O, I, list, b0, len}(t);
 }
 
 template <typename T> __global__ void k_blocks_add(T *A, const T *D, const int *list, int b0, int nb, int len)
