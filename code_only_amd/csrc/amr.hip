@@ -129,6 +129,9 @@ struct sg_amr {
     double *d_adj = nullptr, *d_adj_part = nullptr, *d_adj_g = nullptr;
     int adj_m = 0;
     int fused_on = 0;
+    // set around a refresh that must be the staged one within a fused-mode solve (the fused refresh
+    // skipped; the other fused-mode launch forms, which give the staged values, kept)
+    int staged_refresh = 0;
     // device state: FP64 arrays (plain solve and the mixed solve's outer loop), FP32 arrays (the
     // mixed solve's inner fmg_lin), allocated on first mixed use
     Arrays<double> d;
@@ -313,6 +316,16 @@ template <typename T> int fill0_faces(sg_amr &s, int L, int code, int l0, int l1
     SG_NDISPATCH(L, fill0_faces_n, s, L, code, l0, l1);
 }
 
+// fmg_boundary_fill0 of FMG level Lc = Lf + 1 (array iuc) and of Lf (array iuf) in one launch
+template <int NF, typename T> int fill0_two(sg_amr &s, int Lf, int iuf, int iuc)
+{
+    constexpr int NC = NF / 2;
+    const int e0 = s.phys_off[0], ne = s.phys_off[s.nlev] - e0;
+    const int *ph = (const int *)s.d_phys + e0;
+    return go2(s, Fill0FacesF<NC, T>{ar<T>(s).A[Lf + 1][iuc], ph}, (long)ne * (NC + 2) * (NC + 2),
+               Fill0FacesF<NF, T>{ar<T>(s).A[Lf][iuf], ph}, (long)ne * (NF + 2) * (NF + 2));
+}
+
 template <int N, typename T> int extrap_faces_n(sg_amr &s, int L, int code, int l0, int l1)
 {
     int e0 = s.phys_off[l0], ne = s.phys_off[l1] - e0;
@@ -347,7 +360,10 @@ template <typename T> int reflux(sg_amr &s, int L, int code) { SG_NDISPATCH(L, r
 // fused refresh computes the same zeros the skipped refresh would leave.  Physical faces keep
 // their stored values (zero in the inner solve, whose every refresh zeroes them; the multipole
 // values in the outer residual, which refreshes without zeroing).
-inline bool fused(const sg_amr &s, int L) { return s.fused_on && L < kNfmg - 1 && s.fused_ok[L]; }
+inline bool fused(const sg_amr &s, int L)
+{
+    return s.fused_on && !s.staged_refresh && L < kNfmg - 1 && s.fused_ok[L];
+}
 
 // The inner solve's FMG-level-0 correction U is held in compact form (u0c with its ghost store gs0;
 // amr_kernels.h cc_index, gs_cell) where the fused form applies at FMG level 0 and the compact
@@ -519,9 +535,9 @@ template <typename T> int poisson_relax(sg_amr &s, int L, int ju, int jrhs, int 
     }
     for (int ipass = 1; ipass <= 2; ++ipass) {
         if (staged_last && ipass == 2 && s.fused_on) {
-            s.fused_on = 0;
+            s.staged_refresh = 1;
             const int rc = poisson_flux<T>(s, L, ju, fill0flag);
-            s.fused_on = 1;
+            s.staged_refresh = 0;
             SG_TRY(rc);
         } else {
             SG_TRY(poisson_flux<T>(s, L, ju, fill0flag));
@@ -623,6 +639,7 @@ template <typename T> int fmg_interp(sg_amr &s, int Lf, int iuf, int iuc, int cu
         return fill0_faces<T>(s, Lc, iuc, 0, s.nlev);
     }
     SG_TRY(interp_blocks<T>(s, Lf, iuf, iuc, cubic));
+    if (s.fused_on) return Lf == 0 ? fill0_two<8, T>(s, Lf, iuf, iuc) : fill0_two<4, T>(s, Lf, iuf, iuc);
     SG_TRY(fill0_faces<T>(s, Lc, iuc, 0, s.nlev));
     return fill0_faces<T>(s, Lf, iuf, 0, s.nlev);
 }
