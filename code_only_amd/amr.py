@@ -87,8 +87,8 @@ class AmrSolver:
         return out
 
     def set_base_solver(self, chip):
-        """Base-grid mg_cycle of the plain FP64 solve spread over the chip (True, default) or in one
-        workgroup (False); the values are the same."""
+        """Base-grid mg_cycle of the plain FP64 solve, and the mixed solve's exact base solve, spread
+        over the chip (True, default) or in one workgroup (False); the values are the same."""
         _lib.check(self._lib.sg_amr_set_base_solver(self._ctx, 1 if chip else 0))
 
     def set_fp64_refresh(self, fused):

@@ -146,6 +146,7 @@ struct sg_amr {
     hipGraphExec_t graph = nullptr;   // mixed solve: one outer iteration, captured once
     int graph_zero_average = -1;      // the zero_average setting the graph was captured with
     int graph_fused = -1;             // and the fused-refresh setting
+    int graph_base_chip = -1;         // and the base-solve form
     size_t graph_nodes = 0;           // kernel nodes of the captured outer iteration
     double hl[kNfmg][kMaxLev] = {}, hil[kNfmg][kMaxLev] = {};
     double *d_h[kNfmg] = {}, *d_hi[kNfmg] = {};
@@ -889,10 +890,11 @@ int base_mg_chip(sg_amr &s)
 }
 
 // vmg_slvsml (vmg.F90:387-432).  FP64: the reference's mg_cycle on the gathered base grid.  Inner
-// (FP32) solve: the same base problem solved exactly by separable transforms in one workgroup.
+// (FP32) solve: the same base problem solved exactly by separable transforms, spread over the chip
+// (base_chip, the default) or in one workgroup (the same values).
 template <typename T> int vmg_slvsml(sg_amr &s, int ju, int jrhs)
 {
-    if (kInner<T> && s.bd_ok && s.bd_grid) {
+    if (kInner<T> && s.bd_ok && s.bd_grid && s.base_chip) {
         // gather, three forward passes (the third divided by the eigenvalue sums), three back
         // (the last scattered into the base blocks): the values of the one-workgroup form
         const int n = s.bd.n, n3 = n * n * n;
@@ -1377,7 +1379,8 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
         if (p.stop_mode == SG_STOP_TRUE_RESIDUAL && st.rel_linf < p.tol) { status = SG_OK; break; }
         if (it >= nmax) break;
         if (p.use_graph) {
-            if (!s.graph || s.graph_zero_average != s.zero_average || s.graph_fused != s.fused_on) {
+            if (!s.graph || s.graph_zero_average != s.zero_average || s.graph_fused != s.fused_on ||
+                s.graph_base_chip != s.base_chip) {
                 if (s.graph) (void)hipGraphExecDestroy(s.graph);
                 s.graph = nullptr;
                 hipGraph_t gr = nullptr;
@@ -1391,6 +1394,7 @@ int solve_mixed(sg_amr &s, double *psi, const double *src, double *g, const sg_p
                 SG_HIP(e);
                 s.graph_zero_average = s.zero_average;
                 s.graph_fused = s.fused_on;
+                s.graph_base_chip = s.base_chip;
             }
             SG_HIP(hipGraphLaunch(s.graph, s.stream));
         } else {

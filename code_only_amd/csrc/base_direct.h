@@ -111,8 +111,18 @@ __global__ void k_bd_pass(float *out, const float *in, BaseDirect D, float *Au, 
     const int stride = AXIS == 0 ? 1 : (AXIS == 1 ? n : n * n);
     const int base = t - m * stride;
     const float *q = D.Q + m * n;
+    // the line and the basis row loaded whole first (n <= kDirectMaxN), then summed in order
+    float x[kDirectMaxN], qq[kDirectMaxN];
+#pragma unroll
+    for (int kk = 0; kk < kDirectMaxN; ++kk)
+        if (kk < n) {
+            x[kk] = in[base + kk * stride];
+            qq[kk] = q[kk];
+        }
     float acc = 0.f;
-    for (int kk = 0; kk < n; ++kk) acc += q[kk] * in[base + kk * stride];
+#pragma unroll
+    for (int kk = 0; kk < kDirectMaxN; ++kk)
+        if (kk < n) acc += qq[kk] * x[kk];
     if (MODE == 1) {
         const float den = D.lam[i] + D.lam[j] + D.lam[k];
         acc = den != 0.f ? acc / den : 0.f;

@@ -319,6 +319,27 @@ def test_fused_refresh_equals_staged(which, graph, gpu):
     g.close()
 
 
+@pytest.mark.parametrize("which", ["sphere-4", "periodic-4", "sphere-16"])
+def test_mixed_base_solve_chip_equals_one_workgroup(which, gpu):
+    """The inner solve's exact base solve as six chip-wide transform passes equals the one-workgroup
+    form bit for bit (each output summed over the same terms in the same order): the whole mixed
+    solve, graphed, is the same either way (8^3 and 32^3 base grids; DST and Hartley bases)."""
+    base, bc = {"sphere-4": (4, ah.SPHERE), "periodic-4": (4, ah.PERIODIC), "sphere-16": (16, ah.SPHERE)}[which]
+    h = ah.nested(base=base, levels=3, bc=bc)
+    g = AmrSolver(h)
+    try:
+        src = problems.amr_source(h, seed=23)
+        pc, gc, sc = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=True)
+        g.set_base_solver(False)
+        pw, gw, sw = g.solve_host(src, tol=1e-10, maxit=30, precision=SG_PREC_MIXED, use_graph=True)
+        g.set_base_solver(True)
+        print(which, sc["iterations"], sw["iterations"], flush=True)
+        assert sc["iterations"] == sw["iterations"] and sc["hist"] == sw["hist"]
+        assert np.array_equal(pc, pw) and np.array_equal(gc, gw)
+    finally:
+        g.close()
+
+
 def test_fused_refresh_full_sphere_hierarchy_launches(gpu):
     """16^3 base blocks, five levels: the fused form applies at FMG levels 0 and 1, reproduces
     the staged mixed solve bit for bit, and needs fewer kernel nodes per outer iteration."""
