@@ -949,8 +949,10 @@ template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
     const int p0 = s.phys_off[0], np = s.phys_off[s.nlev] - p0;
     SG_TRY(go2(s, Fill0FacesF<2, T>{U, (const int *)s.d_phys + p0}, (long)np * 16, CopyBlocksF<T>{RHS, (const T *)RHO,
                (const int *)nullptr, 0, (int)LENV}, nall));
+    bool bounded = false;   // level v's vmg_gridboundary ran with the previous step's tau finish
     for (int v = s.nlev - 1; v >= 1; --v) {
-        SG_TRY(vmg_gridboundary<T>(s, v, C_U));
+        if (!bounded) SG_TRY(vmg_gridboundary<T>(s, v, C_U));
+        bounded = false;
         // the inner solve's relaxation keeps no refresh here: the parent- and same-level passes
         // below rewrite every ghost cell it would write (from the relaxed interiors, the parent
         // level and physical ghosts, none of which it writes)
@@ -971,8 +973,20 @@ template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
             SG_TRY(vmg_ghostcell<T>(s, v - 1, C_U));
         }
         const int c0 = s.cov_off[v - 1], nc = s.cov_off[v] - c0;
-        SG_TRY(go(s, k_tau_finish<T>, (long)nc * LENV, RES, RHS, (const T *)U, (const int *)(s.d_cov + c0), nc,
-                  s.hil[LV][v - 1], (const unsigned long long *)(s.d_crop + c0)));
+        if (v - 1 >= 1 && nlv(v - 1) > 0) {
+            // with the next step's vmg_gridboundary(v - 1) in the same launch
+            const int l = v - 1, fill0 = s.phys_off[l + 1] > s.phys_off[l] ? 1 : 0;
+            const long g = (nlv(l) + 7) / 8 + ((long)nc * LENV + 255) / 256;
+            hipLaunchKernelGGL(k_tau_finish_pfill<T>, dim3((unsigned)g), dim3(256), 0, s.stream, RES, RHS, U,
+                               (const int *)(s.d_cov + c0), nc, s.hil[LV][l], (const unsigned long long *)(s.d_crop + c0),
+                               s.lev_start[l], nlv(l), fill0, (const int *)s.d_nbr, (const int *)s.d_nkind,
+                               (const int *)s.d_octant);
+            SG_CHECK_LAUNCH();
+            bounded = true;
+        } else {
+            SG_TRY(go(s, k_tau_finish<T>, (long)nc * LENV, RES, RHS, (const T *)U, (const int *)(s.d_cov + c0), nc,
+                      s.hil[LV][v - 1], (const unsigned long long *)(s.d_crop + c0)));
+        }
     }
     if (s.zero_average) SG_TRY(go1(s, k_zero_average<2, T>, R.A[LV][C_RHS], s.nbase, s.P_base, s.d_part));
     SG_TRY(vmg_slvsml<T>(s, C_U, C_RHS));

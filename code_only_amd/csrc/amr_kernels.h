@@ -1252,12 +1252,10 @@ template <typename T> __global__ void k_tau_fine_c2p_fill(T *RES, T *U, int b0, 
 // vmg_poisson_tau's coarse part (tau += (div F)/h inside) + vmg_tau_crop (the planes of the
 // parent next to a child face that meets the parent's level set to zero; crop[e] is that mask of
 // the 64 cells of covered block list[e]) + vmg_add_onov (RHS += tau) on the covered blocks of one level.
-template <typename T> __global__ void k_tau_finish(T *RES, T *RHS, const T *U, const int *list, int nc, double hi_d,
-                                                  const unsigned long long *crop)
+template <typename T> __device__ __forceinline__ void tau_finish_cell(T *RES, T *RHS, const T *U, const int *list,
+                                                                     double hi_d, const unsigned long long *crop, long t)
 {
     using B = Blk<2>;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nc * B::LEN) return;
     const int e = (int)(t / B::LEN), c = (int)(t % B::LEN), b = list[e];
     int i, j, k;
     vmgm::cell_of(c, i, j, k);
@@ -1270,6 +1268,33 @@ template <typename T> __global__ void k_tau_finish(T *RES, T *RHS, const T *U, c
     if ((crop[e] >> c) & 1ULL) tau = (T)0;
     RES[q] = tau;
     RHS[q] = RHS[q] + tau;
+}
+
+template <typename T> __global__ void k_tau_finish(T *RES, T *RHS, const T *U, const int *list, int nc, double hi_d,
+                                                  const unsigned long long *crop)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nc * Blk<2>::LEN) return;
+    tau_finish_cell(RES, RHS, U, list, hi_d, crop, t);
+}
+
+// k_tau_finish on the covered blocks of AMR level l (workgroups from (nb + 7) / 8 on) and the next
+// VMG step's vmg_gridboundary of level l (k_parent_fill, the first (nb + 7) / 8 workgroups) in one
+// launch.  A covered block never meets a coarser level, so the parent-level passes write no cell of
+// a covered block but its physical-face ghosts, which hold zero already (the vmg_ghostcell before
+// the finish zeroed them) and get zero again; the finish writes only RES and RHS.
+template <typename T>
+__global__ void __launch_bounds__(256) k_tau_finish_pfill(T *RES, T *RHS, T *U, const int *list, int nc, double hi_d,
+                                                          const unsigned long long *crop, int b0, int nb, int fill0,
+                                                          const int *nbr, const int *nkind, const int *octant)
+{
+    const int g1 = (nb + 7) / 8;
+    if ((int)blockIdx.x < g1) {   // uniform per workgroup: the parent fill's barriers see whole workgroups
+        parent_fill_block(U, b0, nb, fill0, nbr, nkind, octant);
+        return;
+    }
+    const long t = (long)(blockIdx.x - g1) * blockDim.x + threadIdx.x;
+    if (t < (long)nc * Blk<2>::LEN) tau_finish_cell(RES, RHS, (const T *)U, list, hi_d, crop, t);
 }
 
 // vmg_sub (RES = U - PSI over whole arrays of one level) + vmg_ghostcell of RES there: map[t] is -2
