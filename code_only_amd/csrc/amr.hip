@@ -962,12 +962,16 @@ template <typename T> int vmg_fas_merged(sg_amr &s, int jrho)
         SG_TRY(go(s, k_vmg_rstr<T>, (long)nlv(v - 1) * LENV, U, PSI, RHS, s.lev_start[v - 1], nlv(v - 1),
                   (const int *)s.d_child, (const int *)s.d_covered, s.st));
         // vmg_poisson_tau + vmg_tau_crop (vmg_tau), then vmg_add_onov
-        SG_TRY(vmg_ghostcell<T>(s, v, C_U));
-        if (s.vfill_ok[v - 1]) {   // with the coarser level's vmg_ghostcell gather in the same launch
-            const int e0 = s.vfill_off[v - 1], ne = s.vfill_off[v] - e0;
+        if (s.vfill_ok[v - 1] && s.vfill_ok[v]) {
+            // vmg_ghostcell(v) and the coarser level's vmg_ghostcell (adjacent in the pair table) in
+            // the same launch: tau reads level v's interiors and face ghosts, on which the gather
+            // writes the values the refresh above left (same-level faces: the same neighbour cells;
+            // parent-level faces: untouched; physical faces: zero)
+            const int e0 = s.vfill_off[v - 1], ne = s.vfill_off[v + 1] - e0;
             SG_TRY(go(s, k_tau_fine_c2p_fill<T>, (long)nlv(v) * LENV + ne, RES, U, s.lev_start[v], nlv(v), s.hil[LV][v],
                       (const int *)s.d_parent, (const int *)s.d_octant, (const int2 *)s.d_vfill + e0, ne));
         } else {
+            SG_TRY(vmg_ghostcell<T>(s, v, C_U));
             SG_TRY(go(s, k_tau_fine_c2p<T>, (long)nlv(v) * LENV, RES, (const T *)U, s.lev_start[v], nlv(v), s.hil[LV][v],
                       (const int *)s.d_parent, (const int *)s.d_octant));
             SG_TRY(vmg_ghostcell<T>(s, v - 1, C_U));

@@ -1237,9 +1237,9 @@ template <typename T> __global__ void k_tau_fine_c2p(T *RES, const T *U, int b0,
     tau_fine_c2p(RES, U, b0, hi_d, parent, octant, t);
 }
 
-// k_tau_fine_c2p on AMR level l (items [0, nb 64)) and the coarser level's vmg_ghostcell gather
-// (k_fill_pairs, items [nb 64, + np)) in one launch: the first reads level-l U and writes RES, the
-// second reads and writes level-(l-1) U only, so neither reads what the other writes.
+// k_tau_fine_c2p on AMR level l (items [0, nb 64)) and vmg_ghostcell gathers (k_fill_pairs, items
+// [nb 64, + np)) in one launch: of the coarser level, whose U the first does not read, and of level
+// l itself where the gather leaves every cell the first reads (interiors, face ghosts) as it is.
 template <typename T> __global__ void k_tau_fine_c2p_fill(T *RES, T *U, int b0, int nb, double hi_d, const int *parent,
                                                          const int *octant, const int2 *pairs, int np)
 {
