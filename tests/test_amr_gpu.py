@@ -548,14 +548,16 @@ def test_base_solver_chip_equals_one_workgroup(which, gpu):
         g.close()
 
 
-@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic"])
+@pytest.mark.parametrize("which", ["nested-sphere", "nested-periodic", "random-periodic", "random-sphere"])
 def test_fp64_fused_refresh_equals_staged(which, gpu):
     """The plain FP64 solve with the mixed solve's launch structure (fused refresh, halo-free colour
     passes, merged VMG steps) against the reference's staged sequence: the same iterates bit for
     bit."""
     h = {"nested-sphere": lambda: ah.nested(base=4, levels=3, bc=ah.SPHERE),
          "nested-periodic": lambda: ah.nested(base=4, levels=3, bc=ah.PERIODIC),
-         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC)}[which]()
+         "random-periodic": lambda: ah.random_hierarchy(4, 3, 0.35, seed=7, bc=ah.PERIODIC),
+         # refinement reaching the domain boundary: physical faces on fine blocks
+         "random-sphere": lambda: ah.random_hierarchy(4, 3, 0.35, seed=3, bc=ah.SPHERE)}[which]()
     g = AmrSolver(h)
     src = problems.amr_source(h, seed=29)
     try:
