@@ -433,7 +433,7 @@ constexpr int kBlk8Nt[3] = {64, 128, 256};
 // through the context's push table)
 template <bool RELAX, typename T>
 int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *ov, double *resmax, int halo,
-         float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr, T *gs = nullptr)
+         float *cast = nullptr, double *npart = nullptr, const float *rhsc = nullptr, T *gs = nullptr, int xcd = 0)
 {
     const dim3 g((unsigned)s.active.size());
     const int *rec = s.d_arec;
@@ -442,7 +442,7 @@ int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *o
             SG_REQUIRE(rhsc && s.d_ptab, "compact FMG-level-0 pass without its compact source or push table");
             auto launch = [&](auto kern, int n) {
                 hipLaunchKernelGGL(kern, g, dim3(n), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0], ov,
-                                   resmax, 1, cast, npart, rhsc, gs, (const int *)s.d_ptab);
+                                   resmax, 1, cast, npart, rhsc, gs, (const int *)s.d_ptab, xcd);
             };
             if (nt == 64) launch(k_blk8<RELAX, float, 64, true>, 64);
             else if (nt == 128) launch(k_blk8<RELAX, float, 128, true>, 128);
@@ -451,15 +451,18 @@ int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *o
             return SG_OK;
         }
     }
+    // a halo-free pass reads its same-level neighbours' boundary cells: blocks in contiguous ranges
+    // per XCD, so that those reads mostly hit the reading XCD's L2
+    xcd = xcd || halo;
     if (nt == 64)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 64>), g, dim3(64), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0],
-                           ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
+                           ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr, xcd);
     else if (nt == 128)
         hipLaunchKernelGGL((k_blk8<RELAX, T, 128>), g, dim3(128), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
+                           s.his[0], ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr, xcd);
     else
         hipLaunchKernelGGL((k_blk8<RELAX, T, 256>), g, dim3(256), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0],
-                           s.his[0], ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr);
+                           s.his[0], ov, resmax, halo, cast, npart, rhsc, (T *)nullptr, (const int *)nullptr, xcd);
     SG_CHECK_LAUNCH();
     return SG_OK;
 }

@@ -750,6 +750,15 @@ __global__ void k_max_slots(const double *slots, double *out)
 // (-1 where there is none), two unused.  One 64-byte load gives a workgroup or thread everything
 // it needs before its data loads; the cell widths come in by value (LevelScalars).
 constexpr int kRec = 16;
+// Workgroup w of G runs on XCD w mod 8 (the dispatcher deals workgroups out round-robin): the
+// block it takes under the map that gives each XCD a contiguous range of the G blocks, so that
+// blocks next to each other in the list share that XCD's L2.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_block(int w, int G)
+{
+    const int x = w & 7, i = w >> 3, q = G >> 3, r = G & 7;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 struct BlockRec {
     int b, l, nb[6], slot[6];
     __device__ void load(const int *rec, long a)
@@ -1483,14 +1492,14 @@ __device__ __forceinline__ int rec_nb(const BlockRec &R, int f)
 template <bool RELAX, typename T, int NT, bool CC = false>
 __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, const int *rec, int ipass,
                                             LevelScalars h_lev, LevelScalars hi_lev, const T *ov, double *resmax,
-                                            int halo, float *cast = nullptr, double *npart = nullptr,
-                                            const float *rhsc = nullptr, T *gs = nullptr, const int *ptab = nullptr)
+                                            int halo, float *cast, double *npart, const float *rhsc, T *gs,
+                                            const int *ptab, int xcd)
 {
     using B = Blk<8>;
     static_assert(!CC || std::is_same<T, float>::value, "the compact form is FP32");
     constexpr int NC = RELAX ? 256 / NT : 512 / NT;   // cells per thread
     __shared__ __align__(16) T su[B::LEN];
-    const int wa = (int)blockIdx.x;
+    const int wa = xcd ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
     BlockRec R;
     R.load(rec, wa);
     if constexpr (CC) {
@@ -1600,7 +1609,7 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
             if constexpr (CC) vout[r] = v;
             else if (Aout) Aout[q] = v;
             if constexpr (std::is_same<T, double>::value) {
-                if (cast) cast[(long)blockIdx.x * 512 + cc_index(ci[r], cj[r], ck[r])] = (float)v;
+                if (cast) cast[(long)wa * 512 + cc_index(ci[r], cj[r], ck[r])] = (float)v;
                 if (npart) {
                     const double av = fabs(v);
                     a = fmax(a, av);
@@ -1632,9 +1641,9 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
         double dummy = 0.0;
         block_max_sum(a, dummy);
         if (threadIdx.x == 0) {
-            npart[blockIdx.x] = m2;
-            npart[G + blockIdx.x] = a;
-            npart[2 * G + blockIdx.x] = sq;
+            npart[wa] = m2;
+            npart[G + wa] = a;
+            npart[2 * G + wa] = sq;
         }
     }
     if (RELAX && resmax) block_atomic_max(resmax + (blockIdx.x & (kMaxSlots - 1)), a);
