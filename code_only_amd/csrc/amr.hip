@@ -440,9 +440,11 @@ int blk8(sg_amr &s, int nt, T *Aout, T *Au, const T *Arhs, int ipass, const T *o
     if constexpr (std::is_same<T, float>::value) {
         if (gs) {
             SG_REQUIRE(rhsc && s.d_ptab, "compact FMG-level-0 pass without its compact source or push table");
+            // blocks in contiguous ranges per XCD: the pushes into neighbours' ghost stores mostly
+            // stay in the writing XCD's L2
             auto launch = [&](auto kern, int n) {
                 hipLaunchKernelGGL(kern, g, dim3(n), 0, s.stream, Aout, Au, Arhs, rec, ipass, s.hs[0], s.his[0], ov,
-                                   resmax, 1, cast, npart, rhsc, gs, (const int *)s.d_ptab, xcd);
+                                   resmax, 1, cast, npart, rhsc, gs, (const int *)s.d_ptab, 1);
             };
             if (nt == 64) launch(k_blk8<RELAX, float, 64, true>, 64);
             else if (nt == 128) launch(k_blk8<RELAX, float, 128, true>, 128);
