@@ -828,7 +828,10 @@ __global__ void k_relax(T *Au, const T *Arhs, const int *rec, int nrec, int ipas
 {
     using B = Blk<N>;
     constexpr int HN = (N + 1) / 2, PER = N * N * HN;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // halo-free 4^3 passes: blocks in contiguous ranges per XCD (xcd_block), the neighbour reads
+    // then mostly hit the reading XCD's L2 (the few workgroups of a 2^3 pass gain nothing)
+    const long w = (halo && N >= 4) ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (long)blockIdx.x;
+    long t = w * blockDim.x + threadIdx.x;
     double a = 0.0;
     if (t < (long)nrec * PER) {
         int ih = (int)(t % HN), j = (int)((t / HN) % N), k = (int)((t / (HN * N)) % N);
@@ -855,7 +858,9 @@ __global__ void k_resid(T *Ares, const T *Au, const T *Arhs, const int *rec, int
 {
     using B = Blk<N>;
     constexpr int PER = N * N * N;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // halo-free 4^3 passes: blocks in contiguous ranges per XCD, as in k_relax
+    const long w = (halo && N >= 4) ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (long)blockIdx.x;
+    long t = w * blockDim.x + threadIdx.x;
     if (t >= (long)nrec * PER) return;
     int i = (int)(t % N), j = (int)((t / N) % N), k = (int)((t / (N * N)) % N);
     BlockRec R;
