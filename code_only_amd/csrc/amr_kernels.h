@@ -33,6 +33,15 @@ template <int N> struct Blk {
     }
 };
 
+// Workgroup w of G runs on XCD w mod 8 (the dispatcher deals workgroups out round-robin): the
+// block it takes under the map that gives each XCD a contiguous range of the G blocks, so that
+// blocks next to each other in the list share that XCD's L2.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_block(int w, int G)
+{
+    const int x = w & 7, i = w >> 3, q = G >> 3, r = G & 7;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 __device__ __forceinline__ int ijkc(int i) { return i >= 0 ? i / 2 : -((1 - i) / 2); }   // floor(i/2)
 __device__ __forceinline__ int near_(int f, int c) { return c + 2 * (((f % 2) + 2) % 2) - 1; }
 
@@ -272,7 +281,10 @@ __global__ void k_ghost(T *A, int b0, int nb, int d, int cubic, int kinds, const
 {
     using B = Blk<N>;
     constexpr int W = B::W, H = N / 2;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // 4^3 and larger: workgroups over the XCDs in contiguous ranges (xcd_block), neighbour reads
+    // then mostly hit the reading XCD's L2
+    const long w = N >= 4 ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (long)blockIdx.x;
+    long t = w * blockDim.x + threadIdx.x;
     if (t >= (long)nb * 2 * W * W) return;
     int t1 = (int)(t % W) - 1, t2 = (int)((t / W) % W) - 1, lr = (int)((t / (W * W)) & 1);
     int b = b0 + (int)(t / (2 * W * W));
@@ -750,15 +762,6 @@ __global__ void k_max_slots(const double *slots, double *out)
 // (-1 where there is none), two unused.  One 64-byte load gives a workgroup or thread everything
 // it needs before its data loads; the cell widths come in by value (LevelScalars).
 constexpr int kRec = 16;
-// Workgroup w of G runs on XCD w mod 8 (the dispatcher deals workgroups out round-robin): the
-// block it takes under the map that gives each XCD a contiguous range of the G blocks, so that
-// blocks next to each other in the list share that XCD's L2.  A bijection of [0, G).
-__device__ __forceinline__ int xcd_block(int w, int G)
-{
-    const int x = w & 7, i = w >> 3, q = G >> 3, r = G & 7;
-    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-}
-
 struct BlockRec {
     int b, l, nb[6], slot[6];
     __device__ void load(const int *rec, long a)
@@ -1662,7 +1665,7 @@ __global__ void __launch_bounds__(NT) k_blk8(T *Aout, T *Au, const T *Arhs, cons
 __global__ void __launch_bounds__(256) k_addint_cc(float *U, const float *Ac, const int *list, float *gs,
                                                    const int *ptab)
 {
-    const int a = (int)blockIdx.x, b = list[a];
+    const int a = xcd_block((int)blockIdx.x, (int)gridDim.x), b = list[a];   // per XCD a contiguous range
     const float *uc = Ac + (long)b * Blk<4>::LEN;
     auto cget = [&](int x, int y, int z) -> float { return uc[Blk<4>::gi(x, y, z)]; };
     for (int q = threadIdx.x; q < 512; q += 256) {
